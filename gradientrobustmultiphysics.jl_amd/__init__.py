@@ -1,0 +1,20 @@
+"""MI355X (gfx950) assembly-and-solve engine for GradientRobustMultiPhysics.jl's
+hot path, driven from Python.
+
+The package keeps the reference's vocabulary (ExtendableGrid, FESpace,
+FEVector, FEMatrix, PDEDescription, LaplaceOperator, LinearForm, assemble,
+solve) and routes all work through libgrmp_hip.so (include/grmp.h).  The
+directory name contains a dot, so it is loaded by path, see ``load()`` in
+__graft_entry__.py; inside the package, imports are relative.
+"""
+from ._abi import (Context, DeviceVector, GrmpError, default_context, lib)  # noqa: F401
+from . import _abi  # noqa: F401
+from .fe import (ExtendableGrid, FEMatrix, FEType, FESpace, FEVector, H1P1, H1P2,  # noqa: F401
+                 grid_unitcube, grid_unitsquare)
+from .pde import (APT_BilinearForm, APT_SymmetricBilinearForm, BilinearForm, CellCoefficient,  # noqa: F401
+                  DataFunction, Gradient, HomogeneousDirichletBoundary, HookStiffnessOperator2D, Identity,
+                  LaplaceOperator, LinearElasticityProblem, LinearForm, PDEDescription, PoissonProblem,
+                  ReactionOperator, SolverConfig, SymmetricGradient, assemble, assemble_linearform, solve,
+                  solve_system)
+
+__version__ = "0.1.0"

@@ -1,0 +1,1055 @@
+// Element assembly of libgrmp_hip.so: sparsity pattern, BilinearForm,
+// LinearForm and ItemIntegrator kernels for gfx950.
+//
+// Compiled with -ffp-contract=off (and the pragma below): every product and
+// sum of the element math is rounded separately and in the order the
+// reference's Julia loops use, so local values are bit-identical to the
+// reference's on the same geometry.
+//
+// Design.  The reference loops over cells and scatters each local matrix
+// into an ExtendableSparseMatrix (src/assemblypatterns/bilinearform.jl:226-377,
+// _addnz src/fematrix.jl:54-65).  Here one thread owns one matrix ROW: it
+// walks the cells that contain its dof (the space's incidence list, sorted by
+// cell), recomputes that cell's local row, and adds each entry into an LDS
+// copy of its row at a precomputed position (the 8-bit slot map).  Every
+// entry is therefore summed in cell order -- the reference's order -- with no
+// atomics, and each row is written to HBM once, coalesced across the block.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+
+#include "grmp_internal.h"
+
+#pragma clang fp contract(off)
+
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+#define MAXFORMS 4
+// forms of one block row, kept in device memory (uniform loads in kernels)
+struct FormPack {
+    const DevForm *f;
+    int n;
+};
+
+// ------------------------------------------------------- reference tables
+
+// Triangle2D quadrature (src/quadrature.jl:173-195): NQ = 1 midpoint,
+// NQ = 3 face midpoints [(1/2,1/2),(0,1/2),(1/2,0)], weights summing to 1.
+template <int NQ>
+__device__ __forceinline__ void qtri(int q, double &x0, double &x1, double &w)
+{
+    if (NQ == 1) {
+        x0 = 1.0 / 3.0; x1 = 1.0 / 3.0; w = 1.0;
+    } else {
+        x0 = (q == 1) ? 0.0 : 0.5;
+        x1 = (q == 2) ? 0.0 : 0.5;
+        w = 1.0 / 3.0;
+    }
+}
+
+// P1 triangle values (src/fedefs/h1_p1.jl:64-75)
+__device__ __forceinline__ double p1tri_val(int a, double x0, double x1)
+{
+    double l = 1.0;
+    l -= x0;
+    l -= x1;
+    return a == 0 ? l : (a == 1 ? x0 : x1);
+}
+__device__ __forceinline__ double p1tri_der(int a, int j)
+{
+    if (a == 0) return -1.0;
+    return (a == j + 1) ? 1.0 : 0.0;
+}
+// P2 triangle values / exact derivatives (src/fedefs/h1_p2.jl:134-145)
+__device__ __forceinline__ double p2tri_val(int a, double x0, double x1)
+{
+    double l = 1.0 - x0 - x1;
+    switch (a) {
+    case 0: return 2.0 * l * (l - 0.5);
+    case 1: return 2.0 * x0 * (x0 - 0.5);
+    case 2: return 2.0 * x1 * (x1 - 0.5);
+    case 3: return 4.0 * l * x0;
+    case 4: return 4.0 * x0 * x1;
+    default: return 4.0 * x1 * l;
+    }
+}
+__device__ __forceinline__ double p2tri_der(int a, int j, double x0, double x1)
+{
+    double l = 1.0 - x0 - x1;
+    switch (a) {
+    case 0: return -(4.0 * l - 1.0);
+    case 1: return j == 0 ? 4.0 * x0 - 1.0 : 0.0;
+    case 2: return j == 1 ? 4.0 * x1 - 1.0 : 0.0;
+    case 3: return j == 0 ? 4.0 * (l - x0) : -4.0 * x0;
+    case 4: return j == 0 ? 4.0 * x1 : 4.0 * x0;
+    default: return j == 0 ? -4.0 * x1 : 4.0 * (l - x1);
+    }
+}
+
+// ------------------------------------------------------------ affine maps
+// Triangle: A = [x2-x1, x3-x1], det = A11 A22 - A12 A21,
+// M = A^{-T} = [A22 -A21; -A12 A11]/det, vol = |det|/2
+// (ExtendableGrids update_trafo!/mapderiv!, called at src/feevaluator.jl:371-380).
+__device__ __forceinline__ void trafo_tri(const DevForm &F, int64_t e, double M[2][2], double &vol)
+{
+    const int64_t nc = F.ncells, nn = F.nnodes;
+    int32_t v0 = F.cells[e], v1 = F.cells[nc + e], v2 = F.cells[2 * nc + e];
+    double x0 = F.coords[v0], y0 = F.coords[nn + v0];
+    double x1 = F.coords[v1], y1 = F.coords[nn + v1];
+    double x2 = F.coords[v2], y2 = F.coords[nn + v2];
+    double A11 = x1 - x0, A12 = x2 - x0, A21 = y1 - y0, A22 = y2 - y0;
+    double det = A11 * A22 - A12 * A21;
+    M[0][0] = A22 / det;
+    M[0][1] = -A21 / det;
+    M[1][0] = -A12 / det;
+    M[1][1] = A11 / det;
+    vol = fabs(det) * 0.5;
+}
+
+__device__ __forceinline__ void trafo_tet(const DevForm &F, int64_t e, double M[3][3], double &vol)
+{
+    const int64_t nc = F.ncells, nn = F.nnodes;
+    double X[4][3];
+    for (int v = 0; v < 4; ++v) {
+        int32_t n = F.cells[v * nc + e];
+        X[v][0] = F.coords[n];
+        X[v][1] = F.coords[nn + n];
+        X[v][2] = F.coords[2 * nn + n];
+    }
+    double A[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k) A[r][k] = X[k + 1][r] - X[0][r];
+    double c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+    double c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+    double c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+    double c10 = A[0][2] * A[2][1] - A[0][1] * A[2][2];
+    double c11 = A[0][0] * A[2][2] - A[0][2] * A[2][0];
+    double c12 = A[0][1] * A[2][0] - A[0][0] * A[2][1];
+    double c20 = A[0][1] * A[1][2] - A[0][2] * A[1][1];
+    double c21 = A[0][2] * A[1][0] - A[0][0] * A[1][2];
+    double c22 = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+    double det = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
+    M[0][0] = c00 / det; M[0][1] = c01 / det; M[0][2] = c02 / det;
+    M[1][0] = c10 / det; M[1][1] = c11 / det; M[1][2] = c12 / det;
+    M[2][0] = c20 / det; M[2][1] = c21 / det; M[2][2] = c22 / det;
+    vol = fabs(det) / 6.0;
+}
+
+// ----------------------------------------------------------- local rows
+// Each row function writes v[b] = localmatrix(a,b) * itemfactor for the
+// local row a of cell e: the value the reference passes to _addnz for the
+// entry (dof_a, dof_b) of this cell.  For the symmetric pattern type the
+// value of (a,b) is the upper-triangle entry (min, max) with the action
+// applied to the min dof (bilinearform.jl:196-210, 329-346).  Loops over b
+// are unrolled at compile time; the runtime index a only selects values.
+
+// gradient of local scalar dof d (runtime) at a quadrature point:
+// s_k = (0 + M[k][0] D[d][0]) + M[k][1] D[d][1]   (feevaluator_h1.jl:68-72)
+template <int FE>
+__device__ __forceinline__ void grad_tri(const double M[2][2], int d, double x0, double x1, double g[2])
+{
+    double d0 = (FE == 1) ? p1tri_der(d, 0) : p2tri_der(d, 0, x0, x1);
+    double d1 = (FE == 1) ? p1tri_der(d, 1) : p2tri_der(d, 1, x0, x1);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        double s = 0.0;
+        s += M[k][0] * d0;
+        s += M[k][1] * d1;
+        g[k] = s;
+    }
+}
+
+// (kappa grad u, grad v) with P1/P2 on triangles; NQ quadrature points.
+template <int FE, int NQ>
+__device__ __forceinline__ void row_laplace_tri(const DevForm &F, int64_t e, int a, double *v)
+{
+    constexpr int ND = (FE == 1) ? 3 : 6;
+    double M[2][2], vol;
+    trafo_tri(F, e, M, vol);
+    const double kap = F.coef_cell ? F.kappa[e] : 1.0;
+    const double itemfactor = vol * F.factor * 1.0;
+    double ga[NQ][2];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double x0, x1, w;
+        qtri<NQ>(q, x0, x1, w);
+        grad_tri<FE>(M, a, x0, x1, ga[q]);
+    }
+#pragma unroll
+    for (int b = 0; b < ND; ++b) {
+        double L = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double x0, x1, w;
+            qtri<NQ>(q, x0, x1, w);
+            double gb[2];
+            grad_tri<FE>(M, b, x0, x1, gb);
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                double lo_k = (a <= b) ? ga[q][k] : gb[k];
+                double hi_k = (a <= b) ? gb[k] : ga[q][k];
+                double act = F.coef_cell ? kap * lo_k : lo_k;
+                t += act * hi_k;
+            }
+            L += w * t;
+        }
+        v[b] = L * itemfactor;
+    }
+}
+
+// (kappa grad u, grad v) with P1 on tetrahedra, midpoint rule.
+__device__ __forceinline__ void grad_tet(const double M[3][3], int d, double g[3])
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            double dj = (d == 0) ? -1.0 : ((d == j + 1) ? 1.0 : 0.0);
+            s += M[k][j] * dj;
+        }
+        g[k] = s;
+    }
+}
+
+__device__ __forceinline__ void row_laplace_tet(const DevForm &F, int64_t e, int a, double *v)
+{
+    double M[3][3], vol;
+    trafo_tet(F, e, M, vol);
+    const double kap = F.coef_cell ? F.kappa[e] : 1.0;
+    const double itemfactor = vol * F.factor * 1.0;
+    double ga[3];
+    grad_tet(M, a, ga);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        double gb[3];
+        grad_tet(M, b, gb);
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double lo_k = (a <= b) ? ga[k] : gb[k];
+            double hi_k = (a <= b) ? gb[k] : ga[k];
+            double act = F.coef_cell ? kap * lo_k : lo_k;
+            t += act * hi_k;
+        }
+        double L = 0.0;
+        L += 1.0 * t;
+        v[b] = L * itemfactor;
+    }
+}
+
+// (C eps(u), eps(v)) on P1{2} triangles (HookStiffnessOperator2D,
+// src/pdeoperators.jl:256-276), pattern type APT_BilinearForm: the action
+// acts on the row dof.  The per-cell coefficient scales both Lame
+// parameters.  SymmetricGradient{1} Voigt values (feevaluator_h1.jl:97-116,
+// compression [1,3,3,2]): a dof of component 0 gives [gx, 0, gy], of
+// component 1 gives [0, gy, gx].
+template <int NQ>
+__device__ __forceinline__ void row_hooke_tri(const DevForm &F, int64_t e, int a, double *v)
+{
+    double M[2][2], vol;
+    trafo_tri(F, e, M, vol);
+    double kap = F.coef_cell ? F.kappa[e] : 1.0;
+    double lam = kap * F.lam, mu = kap * F.mu;
+    const int ca = a / 3, sa = a - 3 * ca;
+    double ga[2];
+    grad_tri<1>(M, sa, 0.0, 0.0, ga);
+    double in0 = ca == 0 ? ga[0] : 0.0, in1 = ca == 0 ? 0.0 : ga[1], in2 = ca == 0 ? ga[1] : ga[0];
+    double r0 = (lam + 2.0 * mu) * in0 + lam * in1;
+    double r1 = (lam + 2.0 * mu) * in1 + lam * in0;
+    double r2 = mu * in2;
+    const double itemfactor = vol * F.factor * 1.0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        const int cb = b / 3, sb = b % 3;
+        double gb[2];
+        grad_tri<1>(M, sb, 0.0, 0.0, gb);
+        double c0 = cb == 0 ? gb[0] : 0.0, c1 = cb == 0 ? 0.0 : gb[1], c2 = cb == 0 ? gb[1] : gb[0];
+        double t = 0.0;
+        t += r0 * c0;
+        t += r1 * c1;
+        t += r2 * c2;
+        double L = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double x0, x1, w;
+            qtri<NQ>(q, x0, x1, w);
+            L += w * t;
+        }
+        v[b] = L * itemfactor;
+    }
+}
+
+// (kappa u, v) with P1/P2 on triangles (ReactionOperator, symmetric).
+template <int FE, int NQ>
+__device__ __forceinline__ void row_mass_tri(const DevForm &F, int64_t e, int a, double *v)
+{
+    constexpr int ND = (FE == 1) ? 3 : 6;
+    double M[2][2], vol;
+    trafo_tri(F, e, M, vol);
+    const double kap = F.coef_cell ? F.kappa[e] : 1.0;
+    const double itemfactor = vol * F.factor * 1.0;
+#pragma unroll
+    for (int b = 0; b < ND; ++b) {
+        double L = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double x0, x1, w;
+            qtri<NQ>(q, x0, x1, w);
+            double pa = (FE == 1) ? p1tri_val(a, x0, x1) : p2tri_val(a, x0, x1);
+            double pb = (FE == 1) ? p1tri_val(b, x0, x1) : p2tri_val(b, x0, x1);
+            double plo = (a <= b) ? pa : pb, phi = (a <= b) ? pb : pa;
+            double act = F.coef_cell ? kap * plo : plo;
+            double t = 0.0;
+            t += act * phi;
+            L += w * t;
+        }
+        v[b] = L * itemfactor;
+    }
+}
+
+// form-kind traits: number of column dofs and the row function
+template <int CN_, void (*ROW)(const DevForm &, int64_t, int, double *)>
+struct Kind {
+    static constexpr int CN = CN_;
+    __device__ static __forceinline__ void row(const DevForm &F, int64_t e, int a, double *v) { ROW(F, e, a, v); }
+};
+using K_LAP_P1T1 = Kind<3, row_laplace_tri<1, 1>>;
+using K_LAP_P1T3 = Kind<3, row_laplace_tri<1, 3>>;
+using K_LAP_P2T3 = Kind<6, row_laplace_tri<2, 3>>;
+using K_LAP_P1TET = Kind<4, row_laplace_tet>;
+using K_HOOKE1 = Kind<6, row_hooke_tri<1>>;
+using K_HOOKE3 = Kind<6, row_hooke_tri<3>>;
+using K_MASS_P1T1 = Kind<3, row_mass_tri<1, 1>>;
+using K_MASS_P1T3 = Kind<3, row_mass_tri<1, 3>>;
+
+// runs the statement(s) with KT bound to the kind type of form F
+#define GRMP_KCASE(CODE, TYPE, ...) \
+    case CODE: {                    \
+        using KT = TYPE;            \
+        __VA_ARGS__;                \
+    } break;
+#define GRMP_DISPATCH(F, ...)                                                     \
+    switch ((F).kind * 4 + (F).nq / 2) {                                          \
+        GRMP_KCASE(FK_LAPLACE_P1TRI * 4 + 0, K_LAP_P1T1, __VA_ARGS__)             \
+        GRMP_KCASE(FK_LAPLACE_P1TRI * 4 + 1, K_LAP_P1T3, __VA_ARGS__)             \
+        GRMP_KCASE(FK_LAPLACE_P2TRI * 4 + 1, K_LAP_P2T3, __VA_ARGS__)             \
+        GRMP_KCASE(FK_LAPLACE_P1TET * 4 + 0, K_LAP_P1TET, __VA_ARGS__)            \
+        GRMP_KCASE(FK_HOOKE_P1VTRI * 4 + 0, K_HOOKE1, __VA_ARGS__)                \
+        GRMP_KCASE(FK_HOOKE_P1VTRI * 4 + 1, K_HOOKE3, __VA_ARGS__)                \
+        GRMP_KCASE(FK_MASS_P1TRI * 4 + 0, K_MASS_P1T1, __VA_ARGS__)               \
+        GRMP_KCASE(FK_MASS_P1TRI * 4 + 1, K_MASS_P1T3, __VA_ARGS__)               \
+    default: break;                                                               \
+    }
+
+// ---------------------------------------------------------------- pattern
+// Per-thread sorted column set in LDS (layout set[k*B + tid]).
+template <int B>
+__device__ __forceinline__ bool set_insert(int32_t *set, int &n, int rmax, int32_t c)
+{
+    const int t = threadIdx.x;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        int32_t v = set[mid * B + t];
+        if (v < c) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo < n && set[lo * B + t] == c) return true;
+    if (n >= rmax) return false;
+    for (int k = n; k > lo; --k) set[k * B + t] = set[(k - 1) * B + t];
+    set[lo * B + t] = c;
+    ++n;
+    return true;
+}
+
+template <int B>
+__device__ __forceinline__ int set_find(const int32_t *set, int n, int32_t c)
+{
+    const int t = threadIdx.x;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (set[mid * B + t] < c) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// collects the columns with a nonzero local value of row i of form F
+template <class K, int B>
+__device__ __forceinline__ void pattern_collect(const DevForm &F, int64_t i, int32_t *set, int &n, int rmax, bool &ok)
+{
+    const int32_t b0 = F.inc_ptr[i], b1 = F.inc_ptr[i + 1];
+    for (int32_t t = b0; t < b1; ++t) {
+        const int32_t pk = F.inc[t];
+        const int64_t e = pk / F.rnloc;
+        const int a = pk - (int)e * F.rnloc;
+        double v[K::CN];
+        K::row(F, e, a, v);
+#pragma unroll
+        for (int b = 0; b < K::CN; ++b)
+            if (v[b] != 0.0) ok &= set_insert<B>(set, n, rmax, (int32_t)(F.co + F.cdofs[(int64_t)b * F.ncells + e]));
+    }
+}
+
+// writes the slot map of row i of form F: byte b of incidence t is the
+// position of column cdofs[b] in the row, 255 for a zero local value
+template <class K, int B>
+__device__ __forceinline__ void pattern_slots(const DevForm &F, int64_t i, const int32_t *set, int n)
+{
+    const int32_t b0 = F.inc_ptr[i], b1 = F.inc_ptr[i + 1];
+    constexpr int W = (K::CN + 3) / 4;
+    for (int32_t t = b0; t < b1; ++t) {
+        const int32_t pk = F.inc[t];
+        const int64_t e = pk / F.rnloc;
+        const int a = pk - (int)e * F.rnloc;
+        double v[K::CN];
+        K::row(F, e, a, v);
+        uint32_t w[W];
+#pragma unroll
+        for (int q = 0; q < W; ++q) w[q] = 0;
+#pragma unroll
+        for (int b = 0; b < K::CN; ++b) {
+            uint32_t p = 255u;
+            if (v[b] != 0.0) p = (uint32_t)set_find<B>(set, n, (int32_t)(F.co + F.cdofs[(int64_t)b * F.ncells + e]));
+            w[b >> 2] |= p << (8 * (b & 3));
+        }
+#pragma unroll
+        for (int q = 0; q < W; ++q) F.pos[(int64_t)t * W + q] = w[q];
+    }
+}
+
+// FILL = false: count the row; FILL = true: write columns, diagonal
+// position and the slot map of every (form, incidence, local column).
+template <int B, int RMAX, bool FILL>
+__global__ __launch_bounds__(B) void k_pattern(FormPack P, int64_t g0, int64_t g1, int64_t rblock,
+                                               const uint8_t *__restrict__ fixed, int32_t *__restrict__ count,
+                                               const int32_t *__restrict__ rowptr, int32_t *__restrict__ cols,
+                                               uint8_t *__restrict__ diagpos, int *__restrict__ flag)
+{
+    __shared__ int32_t set[RMAX * B];
+    const int64_t g = g0 + (int64_t)blockIdx.x * B + threadIdx.x;
+    if (g >= g1) return;
+    const int64_t i = g - rblock;
+    int n = 0;
+    bool ok = true;
+    if (fixed[g]) ok &= set_insert<B>(set, n, RMAX, (int32_t)g);
+    for (int f = 0; f < P.n; ++f) {
+        const DevForm F = P.f[f];
+        GRMP_DISPATCH(F, pattern_collect<KT, B>(F, i, set, n, RMAX, ok));
+    }
+    if (!ok) {
+        atomicOr(flag, 1);
+        return;
+    }
+    if (!FILL) {
+        count[g] = n;
+        atomicMax(flag + 1, n);
+        return;
+    }
+    const int32_t r0 = rowptr[g];
+    for (int k = 0; k < n; ++k) cols[r0 + k] = set[k * B + threadIdx.x];
+    {
+        int p = set_find<B>(set, n, (int32_t)g);
+        diagpos[g] = (p < n && set[p * B + threadIdx.x] == (int32_t)g) ? (uint8_t)p : (uint8_t)255;
+    }
+    for (int f = 0; f < P.n; ++f) {
+        const DevForm F = P.f[f];
+        GRMP_DISPATCH(F, pattern_slots<KT, B>(F, i, set, n));
+    }
+}
+
+// --------------------------------------------------------------- assembly
+template <class K>
+__device__ __forceinline__ void assemble_row(const DevForm &F, int64_t i, double *row)
+{
+    const int32_t b0 = F.inc_ptr[i], b1 = F.inc_ptr[i + 1];
+    constexpr int W = (K::CN + 3) / 4;
+    for (int32_t t = b0; t < b1; ++t) {
+        const int32_t pk = F.inc[t];
+        const int64_t e = pk / F.rnloc;
+        const int a = pk - (int)e * F.rnloc;
+        double v[K::CN];
+        K::row(F, e, a, v);
+        uint32_t w[W];
+#pragma unroll
+        for (int q = 0; q < W; ++q) w[q] = F.pos[(int64_t)t * W + q];
+#pragma unroll
+        for (int b = 0; b < K::CN; ++b) {
+            uint32_t p = (w[b >> 2] >> (8 * (b & 3))) & 255u;
+            if (p != 255u) row[p] += v[b];
+        }
+    }
+}
+
+// One thread per row; the block's rows are staged contiguously in LDS
+// (dynamic shared memory of rowptr[end]-rowptr[start] doubles).
+__global__ void k_assemble(FormPack P, int64_t g0, int64_t g1, int64_t rblock, const int32_t *__restrict__ rowptr,
+                           const uint8_t *__restrict__ diagpos, const uint8_t *__restrict__ fixed, double penalty,
+                           double *__restrict__ val64, float *__restrict__ val32, double *__restrict__ dinv64,
+                           float *__restrict__ dinv32)
+{
+    extern __shared__ double buf[];
+    const int B = blockDim.x;
+    const int64_t rs = g0 + (int64_t)blockIdx.x * B;
+    const int64_t re = min(rs + B, g1);
+    const int32_t base = rowptr[rs];
+    const int32_t len = rowptr[re] - base;
+    for (int32_t k = threadIdx.x; k < len; k += B) buf[k] = 0.0;
+    __syncthreads();
+    const int64_t g = rs + threadIdx.x;
+    const bool mine = g < re;
+    int32_t off = 0, rl = 0;
+    bool fx = false;
+    if (mine) {
+        off = rowptr[g] - base;
+        rl = rowptr[g + 1] - rowptr[g];
+        fx = fixed[g] != 0;
+        const int64_t i = g - rblock;
+        for (int f = 0; f < P.n; ++f) {
+            const DevForm F = P.f[f];
+            GRMP_DISPATCH(F, assemble_row<KT>(F, i, buf + off));
+        }
+        const uint8_t dp = diagpos[g];
+        if (fx && dp != 255) buf[off + dp] = penalty;
+        double d = (dp != 255) ? buf[off + dp] : 0.0;
+        double di = (fx || d == 0.0) ? 0.0 : 1.0 / d;
+        if (dinv64) dinv64[g] = di;
+        if (dinv32) dinv32[g] = (float)di;
+    }
+    __syncthreads();
+    if (val64)
+        for (int32_t k = threadIdx.x; k < len; k += B) val64[base + k] = buf[k];
+    if (val32) {
+        __syncthreads();
+        if (mine && fx)
+            for (int32_t k = 0; k < rl; ++k) buf[off + k] = 0.0;
+        __syncthreads();
+        for (int32_t k = threadIdx.x; k < len; k += B) val32[base + k] = (float)buf[k];
+    }
+}
+
+// ------------------------------------------------------------ matrix API
+
+static int quad_nq_tri(int order) { return order <= 1 ? 1 : (order == 2 ? 3 : 0); }
+
+int form_resolve(grmp_matrix *A, FormRec &r)
+{
+    grmp_ctx *ctx = A->ctx;
+    const grmp_form &f = r.f;
+    const grmp_space *rs = r.rs, *cs = r.cs;
+    GRMP_CHECK(ctx, rs->mesh == cs->mesh, GRMP_ERR_ARG, "form: row and column spaces live on different meshes");
+    const int dim = rs->mesh->dim;
+    r.kind = 0;
+    r.nq = 0;
+    if (f.kind == GRMP_BLF_LAPLACE && rs == cs && rs->ncomp == 1) {
+        if (dim == 2 && rs->fe == GRMP_FE_H1P1) { r.kind = FK_LAPLACE_P1TRI; r.nq = quad_nq_tri(f.quadorder); }
+        if (dim == 2 && rs->fe == GRMP_FE_H1P2 && f.quadorder == 2) { r.kind = FK_LAPLACE_P2TRI; r.nq = 3; }
+        if (dim == 3 && rs->fe == GRMP_FE_H1P1 && f.quadorder <= 1) { r.kind = FK_LAPLACE_P1TET; r.nq = 1; }
+    } else if (f.kind == GRMP_BLF_HOOKE2D && rs == cs && rs->ncomp == 2 && dim == 2 && rs->fe == GRMP_FE_H1P1) {
+        r.kind = FK_HOOKE_P1VTRI;
+        r.nq = quad_nq_tri(f.quadorder);
+    } else if (f.kind == GRMP_BLF_MASS && rs == cs && rs->ncomp == 1 && dim == 2 && rs->fe == GRMP_FE_H1P1) {
+        r.kind = FK_MASS_P1TRI;
+        r.nq = quad_nq_tri(f.quadorder);
+    }
+    GRMP_CHECK(ctx, r.kind != 0 && r.nq != 0, GRMP_ERR_UNSUPPORTED,
+               "form kind %d (quadorder %d) on FE %d/%d with %d components in %dD is not built", f.kind,
+               f.quadorder, rs->fe, cs->fe, rs->ncomp, dim);
+    GRMP_CHECK(ctx, f.coef_mode == GRMP_COEF_FACTOR || (f.coef_mode == GRMP_COEF_CELL && f.cell_coef &&
+                                                         f.cell_coef->n == rs->mesh->ncells),
+               GRMP_ERR_ARG, "form: GRMP_COEF_CELL needs a cell_coef vector of length ncells");
+    r.words = (cs->nloc + 3) / 4;
+    return GRMP_OK;
+}
+
+void fill_devform(const grmp_matrix *A, const FormRec &r, DevForm &d)
+{
+    memset(&d, 0, sizeof(d));
+    const grmp_mesh *m = r.rs->mesh;
+    d.kind = r.kind;
+    d.nq = r.nq;
+    d.coef_cell = r.f.coef_mode == GRMP_COEF_CELL;
+    d.rnloc = r.rs->nloc;
+    d.cnloc = r.cs->nloc;
+    d.words = r.words;
+    d.factor = r.f.factor;
+    d.lam = r.f.params[0];
+    d.mu = r.f.params[1];
+    d.kappa = d.coef_cell ? r.f.cell_coef->d : nullptr;
+    d.coords = m->coords;
+    d.cells = m->cells;
+    d.nnodes = m->nnodes;
+    d.ncells = m->ncells;
+    d.rdofs = r.rs->celldofs;
+    d.inc_ptr = r.rs->inc_ptr;
+    d.inc = r.rs->inc;
+    d.cdofs = r.cs->celldofs;
+    d.ro = A->offs[r.f.row_block];
+    d.co = A->offs[r.f.col_block];
+    d.pos = r.pos;
+}
+
+extern "C" int grmp_matrix_create(grmp_ctx *ctx, int nblocks, const grmp_space *const *spaces, grmp_matrix **out)
+{
+    GRMP_CHECK(ctx, ctx && out && spaces && nblocks >= 1 && nblocks <= 8, GRMP_ERR_ARG, "grmp_matrix_create: bad arguments");
+    grmp_matrix *A = new grmp_matrix();
+    A->ctx = ctx;
+    A->nblocks = nblocks;
+    A->offs[0] = 0;
+    for (int b = 0; b < nblocks; ++b) {
+        if (!spaces[b]) { delete A; return grmp_fail(ctx, GRMP_ERR_ARG, "grmp_matrix_create: null space"); }
+        A->spaces[b] = spaces[b];
+        A->offs[b + 1] = A->offs[b] + spaces[b]->ndofs;
+    }
+    A->n = A->offs[nblocks];
+    if (A->n >= (1ll << 31)) { delete A; return grmp_fail(ctx, GRMP_ERR_ARG, "grmp_matrix_create: too many dofs"); }
+    *out = A;
+    return GRMP_OK;
+}
+
+static void matrix_free_pattern(grmp_matrix *A)
+{
+    grmp_ctx *ctx = A->ctx;
+    dfree(ctx, A->rowptr);
+    dfree(ctx, A->cols);
+    dfree(ctx, A->diagpos);
+    dfree(ctx, A->val64);
+    dfree(ctx, A->val32);
+    dfree(ctx, A->dinv64);
+    dfree(ctx, A->dinv32);
+    for (auto &r : A->forms) dfree(ctx, r.pos);
+    A->have_pattern = false;
+    A->values = 0;
+}
+
+extern "C" int grmp_matrix_destroy(grmp_matrix *A)
+{
+    if (!A) return GRMP_OK;
+    std::lock_guard<std::mutex> g(A->ctx->mu);
+    matrix_free_pattern(A);
+    dfree(A->ctx, A->fixed);
+    delete A;
+    return GRMP_OK;
+}
+
+extern "C" int grmp_matrix_add_form(grmp_matrix *A, const grmp_form *form)
+{
+    if (!A || !form) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = A->ctx;
+    GRMP_CHECK(ctx, form->row_block >= 0 && form->row_block < A->nblocks && form->col_block >= 0 &&
+                        form->col_block < A->nblocks,
+               GRMP_ERR_ARG, "grmp_matrix_add_form: block index out of range");
+    int same = 0;
+    for (auto &r : A->forms) same += r.f.row_block == form->row_block;
+    GRMP_CHECK(ctx, same < MAXFORMS, GRMP_ERR_UNSUPPORTED, "grmp_matrix_add_form: at most %d forms per block row", MAXFORMS);
+    FormRec r;
+    r.f = *form;
+    r.rs = A->spaces[form->row_block];
+    r.cs = A->spaces[form->col_block];
+    GRMP_TRY(form_resolve(A, r));
+    std::lock_guard<std::mutex> g(ctx->mu);
+    matrix_free_pattern(A);
+    A->forms.push_back(r);
+    return GRMP_OK;
+}
+
+extern "C" int grmp_matrix_set_fixed(grmp_matrix *A, int64_t nfixed, const int64_t *fixed, double penalty)
+{
+    if (!A || nfixed < 0 || (nfixed && !fixed)) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = A->ctx;
+    std::vector<uint8_t> mask(A->n, 0);
+    for (int64_t k = 0; k < nfixed; ++k) {
+        GRMP_CHECK(ctx, fixed[k] >= 1 && fixed[k] <= A->n, GRMP_ERR_ARG, "grmp_matrix_set_fixed: dof %lld out of range",
+                   (long long)fixed[k]);
+        mask[fixed[k] - 1] = 1;
+    }
+    std::lock_guard<std::mutex> g(ctx->mu);
+    matrix_free_pattern(A);
+    A->fixed_host.assign(fixed, fixed + nfixed);
+    A->penalty = penalty;
+    if (!A->fixed) GRMP_TRY(dalloc(ctx, &A->fixed, A->n));
+    GRMP_HIP(ctx, hipMemcpyAsync(A->fixed, mask.data(), A->n, hipMemcpyHostToDevice, ctx->stream));
+    GRMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GRMP_OK;
+}
+
+// uploads the DevForms of every block row (in registration order) and
+// returns the device pack of block row rb
+static int upload_forms(grmp_matrix *A, const double *kappa_override = nullptr)
+{
+    grmp_ctx *ctx = A->ctx;
+    std::vector<DevForm> h;
+    A->form_first.assign(A->nblocks + 1, 0);
+    for (int rb = 0; rb < A->nblocks; ++rb) {
+        A->form_first[rb] = (int)h.size();
+        for (auto &r : A->forms)
+            if (r.f.row_block == rb) {
+                DevForm d;
+                fill_devform(A, r, d);
+                if (kappa_override && d.coef_cell) d.kappa = kappa_override;
+                h.push_back(d);
+            }
+    }
+    A->form_first[A->nblocks] = (int)h.size();
+    if (!A->dforms) GRMP_TRY(dalloc(ctx, &A->dforms, A->nblocks * MAXFORMS));
+    GRMP_HIP(ctx, hipMemcpyAsync(A->dforms, h.data(), sizeof(DevForm) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+    // the copy source is a host vector: finish it before returning
+    GRMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GRMP_OK;
+}
+
+static FormPack row_block_pack(const grmp_matrix *A, int rb)
+{
+    FormPack P;
+    P.f = A->dforms + A->form_first[rb];
+    P.n = A->form_first[rb + 1] - A->form_first[rb];
+    return P;
+}
+
+template <int B, int RMAX>
+static int pattern_pass(grmp_matrix *A, bool fill, int32_t *count)
+{
+    grmp_ctx *ctx = A->ctx;
+    for (int rb = 0; rb < A->nblocks; ++rb) {
+        FormPack P = row_block_pack(A, rb);
+        int64_t g0 = A->offs[rb], g1 = A->offs[rb + 1];
+        if (g1 <= g0) continue;
+        if (fill)
+            hipLaunchKernelGGL((k_pattern<B, RMAX, true>), dim3(nblk(g1 - g0, B)), dim3(B), 0, ctx->stream, P, g0, g1,
+                               g0, A->fixed, count, A->rowptr, A->cols, A->diagpos, ctx->dflag);
+        else
+            hipLaunchKernelGGL((k_pattern<B, RMAX, false>), dim3(nblk(g1 - g0, B)), dim3(B), 0, ctx->stream, P, g0, g1,
+                               g0, A->fixed, count, A->rowptr, A->cols, A->diagpos, ctx->dflag);
+        GRMP_HIP(ctx, hipGetLastError());
+    }
+    return GRMP_OK;
+}
+
+extern "C" int grmp_matrix_build_pattern(grmp_matrix *A)
+{
+    if (!A) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = A->ctx;
+    GRMP_CHECK(ctx, !A->forms.empty(), GRMP_ERR_ARG, "grmp_matrix_build_pattern: no forms registered");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    matrix_free_pattern(A);
+    if (!A->fixed) {
+        GRMP_TRY(dalloc(ctx, &A->fixed, A->n));
+        GRMP_HIP(ctx, hipMemsetAsync(A->fixed, 0, A->n, ctx->stream));
+    }
+    for (auto &r : A->forms) GRMP_TRY(dalloc(ctx, &r.pos, (size_t)r.rs->mesh->ncells * r.rs->nloc * r.words));
+    GRMP_TRY(upload_forms(A));
+    int32_t *count = nullptr;
+    GRMP_TRY(dalloc(ctx, &count, A->n));
+    GRMP_TRY(dalloc(ctx, &A->rowptr, A->n + 1));
+    GRMP_TRY(dalloc(ctx, &A->diagpos, A->n));
+    int flags[2] = {0, 0};
+    int big = 0;
+    for (;; ++big) {
+        GRMP_HIP(ctx, hipMemsetAsync(ctx->dflag, 0, 2 * sizeof(int), ctx->stream));
+        if (big == 0) GRMP_TRY((pattern_pass<64, 64>(A, false, count)));
+        else GRMP_TRY((pattern_pass<32, 255>(A, false, count)));
+        GRMP_HIP(ctx, hipMemcpyAsync(flags, ctx->dflag, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        GRMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (!flags[0]) break;
+        if (big) {
+            dfree(ctx, count);
+            return grmp_fail(ctx, GRMP_ERR_UNSUPPORTED, "grmp_matrix_build_pattern: a row has more than 255 entries");
+        }
+    }
+    A->max_row = flags[1];
+    GRMP_TRY(exclusive_scan_i32(ctx, count, A->rowptr, A->n));
+    int32_t nnz32 = 0;
+    GRMP_HIP(ctx, hipMemcpyAsync(&nnz32, A->rowptr + A->n, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    GRMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    A->nnz = nnz32;
+    GRMP_TRY(dalloc(ctx, &A->cols, A->nnz));
+    if (big == 0) GRMP_TRY((pattern_pass<64, 64>(A, true, count)));
+    else GRMP_TRY((pattern_pass<32, 255>(A, true, count)));
+    GRMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    dfree(ctx, count);
+    A->have_pattern = true;
+    return GRMP_OK;
+}
+
+static int asm_block_size(int max_row)
+{
+    int B = 256;
+    while (B > 32 && (size_t)B * max_row * sizeof(double) > 48 * 1024) B >>= 1;
+    return B;
+}
+
+extern "C" int grmp_matrix_assemble(grmp_matrix *A, int values)
+{
+    if (!A) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = A->ctx;
+    if (!A->have_pattern) GRMP_TRY(grmp_matrix_build_pattern(A));
+    GRMP_CHECK(ctx, values & (GRMP_VALUES_FP64 | GRMP_VALUES_FP32), GRMP_ERR_ARG, "grmp_matrix_assemble: no value precision requested");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if ((values & GRMP_VALUES_FP64) && !A->val64) GRMP_TRY(dalloc(ctx, &A->val64, A->nnz));
+    if ((values & GRMP_VALUES_FP32) && !A->val32) GRMP_TRY(dalloc(ctx, &A->val32, A->nnz));
+    if (!A->dinv64) GRMP_TRY(dalloc(ctx, &A->dinv64, A->n));
+    if (!A->dinv32) GRMP_TRY(dalloc(ctx, &A->dinv32, A->n));
+    const int B = asm_block_size(A->max_row);
+    hipEvent_t e0, e1;
+    GRMP_HIP(ctx, hipEventCreate(&e0));
+    GRMP_HIP(ctx, hipEventCreate(&e1));
+    GRMP_HIP(ctx, hipEventRecord(e0, ctx->stream));
+    for (int rb = 0; rb < A->nblocks; ++rb) {
+        FormPack P = row_block_pack(A, rb);
+        int64_t g0 = A->offs[rb], g1 = A->offs[rb + 1];
+        if (g1 <= g0) continue;
+        size_t shm = (size_t)B * (A->max_row > 0 ? A->max_row : 1) * sizeof(double);
+        hipLaunchKernelGGL(k_assemble, dim3(nblk(g1 - g0, B)), dim3(B), shm, ctx->stream, P, g0, g1, g0, A->rowptr,
+                           A->diagpos, A->fixed, A->penalty, (values & GRMP_VALUES_FP64) ? A->val64 : nullptr,
+                           (values & GRMP_VALUES_FP32) ? A->val32 : nullptr, A->dinv64, A->dinv32);
+        GRMP_HIP(ctx, hipGetLastError());
+    }
+    GRMP_HIP(ctx, hipEventRecord(e1, ctx->stream));
+    GRMP_HIP(ctx, hipEventSynchronize(e1));
+    GRMP_HIP(ctx, hipEventElapsedTime(&A->last_asm_ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    A->values |= values;
+    return GRMP_OK;
+}
+
+extern "C" double grmp_matrix_last_assembly_seconds(const grmp_matrix *A) { return A ? A->last_asm_ms * 1e-3 : -1.0; }
+
+extern "C" int grmp_matrix_info(const grmp_matrix *A, int64_t *n, int64_t *nnz, int *max_row)
+{
+    if (!A) return GRMP_ERR_ARG;
+    if (n) *n = A->n;
+    if (nnz) *nnz = A->have_pattern ? A->nnz : -1;
+    if (max_row) *max_row = A->max_row;
+    return GRMP_OK;
+}
+
+// CSR of A -> CSC of A (1-based), i.e. CSR of A^T, built on the host.
+extern "C" int grmp_matrix_export_csc(const grmp_matrix *A, int64_t *colptr, int64_t *rowval, double *nzval)
+{
+    if (!A || !colptr || !rowval) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = A->ctx;
+    GRMP_CHECK(ctx, A->have_pattern, GRMP_ERR_ARG, "grmp_matrix_export_csc: no pattern built");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    const int64_t n = A->n, nnz = A->nnz;
+    std::vector<int32_t> rp(n + 1), cl(nnz);
+    std::vector<double> vl(nzval ? nnz : 0);
+    GRMP_HIP(ctx, hipMemcpyAsync(rp.data(), A->rowptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost, ctx->stream));
+    GRMP_HIP(ctx, hipMemcpyAsync(cl.data(), A->cols, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost, ctx->stream));
+    if (nzval) {
+        GRMP_CHECK(ctx, A->val64, GRMP_ERR_ARG, "grmp_matrix_export_csc: FP64 values not assembled");
+        GRMP_HIP(ctx, hipMemcpyAsync(vl.data(), A->val64, sizeof(double) * nnz, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    GRMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<int64_t> cnt(n + 1, 0);
+    for (int64_t k = 0; k < nnz; ++k) cnt[cl[k] + 1]++;
+    for (int64_t j = 0; j < n; ++j) cnt[j + 1] += cnt[j];
+    for (int64_t j = 0; j <= n; ++j) colptr[j] = cnt[j] + 1;
+    std::vector<int64_t> cur(cnt.begin(), cnt.end() - 1);
+    for (int64_t i = 0; i < n; ++i)
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+            int64_t p = cur[cl[k]]++;
+            rowval[p] = i + 1;
+            if (nzval) nzval[p] = vl[k];
+        }
+    return GRMP_OK;
+}
+
+// ------------------------------------------------------------ LinearForm
+// b[off + i] += sum over cells of dof i (cell order) of localb * itemfactor,
+// localb[a] = sum_q (sum_k f_k phi_k(a)) * w_q (src/assemblypatterns/linearform.jl:181-220).
+struct LfDev {
+    int fe, ncomp, nq, nloc, dim;
+    double factor;
+    double f[4];
+    const double *coords;
+    const int32_t *cells;
+    int64_t nnodes, ncells;
+    const int32_t *inc_ptr, *inc;
+};
+
+template <int FE, int NQ>
+__device__ __forceinline__ double lf_local(const LfDev &L, int64_t e, int a)
+{
+    // cell volume only (the basis values are geometry independent)
+    const int64_t nc = L.ncells, nn = L.nnodes;
+    int32_t v0 = L.cells[e], v1 = L.cells[nc + e], v2 = L.cells[2 * nc + e];
+    double A11 = L.coords[v1] - L.coords[v0], A12 = L.coords[v2] - L.coords[v0];
+    double A21 = L.coords[nn + v1] - L.coords[nn + v0], A22 = L.coords[nn + v2] - L.coords[nn + v0];
+    double det = A11 * A22 - A12 * A21;
+    double vol = fabs(det) * 0.5;
+    constexpr int NS = (FE == 1) ? 3 : 6;
+    const int c = a / NS, s = a % NS;
+    double lb = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double x0, x1, w;
+        qtri<NQ>(q, x0, x1, w);
+        double phi = (FE == 1) ? p1tri_val(s, x0, x1) : p2tri_val(s, x0, x1);
+        double t = 0.0;
+        for (int k = 0; k < L.ncomp; ++k) t += L.f[k] * (k == c ? phi : 0.0);
+        lb += t * w;
+    }
+    double itemfactor = L.factor * vol * 1.0;
+    return lb * itemfactor;
+}
+
+template <int FE, int NQ>
+__global__ void k_linearform(LfDev L, double *__restrict__ b, int64_t ndofs)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndofs) return;
+    double acc = b[i];
+    const int32_t t0 = L.inc_ptr[i], t1 = L.inc_ptr[i + 1];
+    for (int32_t t = t0; t < t1; ++t) {
+        const int32_t pk = L.inc[t];
+        const int64_t e = pk / L.nloc;
+        const int a = pk - (int)e * L.nloc;
+        acc += lf_local<FE, NQ>(L, e, a);
+    }
+    b[i] = acc;
+}
+
+extern "C" int grmp_assemble_linearform(grmp_ctx *ctx, const grmp_space *s, const grmp_lform *form, grmp_vector *b,
+                                        int64_t offset)
+{
+    GRMP_CHECK(ctx, ctx && s && form && b, GRMP_ERR_ARG, "grmp_assemble_linearform: null argument");
+    GRMP_CHECK(ctx, offset >= 0 && offset + s->ndofs <= b->n, GRMP_ERR_ARG, "grmp_assemble_linearform: vector too short");
+    const grmp_mesh *m = s->mesh;
+    int nq = quad_nq_tri(form->quadorder);
+    GRMP_CHECK(ctx, form->kind == GRMP_LF_CONST && m->dim == 2 && (s->fe == GRMP_FE_H1P1 || s->fe == GRMP_FE_H1P2) && nq,
+               GRMP_ERR_UNSUPPORTED, "grmp_assemble_linearform: LF kind %d on FE %d (dim %d, quadorder %d) not built",
+               form->kind, s->fe, m->dim, form->quadorder);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    LfDev L;
+    memset(&L, 0, sizeof(L));
+    L.fe = s->fe;
+    L.ncomp = s->ncomp;
+    L.nq = nq;
+    L.nloc = s->nloc;
+    L.dim = m->dim;
+    L.factor = form->factor;
+    for (int k = 0; k < 4; ++k) L.f[k] = form->f[k];
+    L.coords = m->coords;
+    L.cells = m->cells;
+    L.nnodes = m->nnodes;
+    L.ncells = m->ncells;
+    L.inc_ptr = s->inc_ptr;
+    L.inc = s->inc;
+    dim3 grid(nblk(s->ndofs, 256)), blk(256);
+    double *bp = b->d + offset;
+    if (s->fe == GRMP_FE_H1P1 && nq == 1) hipLaunchKernelGGL((k_linearform<1, 1>), grid, blk, 0, ctx->stream, L, bp, s->ndofs);
+    else if (s->fe == GRMP_FE_H1P1) hipLaunchKernelGGL((k_linearform<1, 3>), grid, blk, 0, ctx->stream, L, bp, s->ndofs);
+    else if (nq == 1) hipLaunchKernelGGL((k_linearform<2, 1>), grid, blk, 0, ctx->stream, L, bp, s->ndofs);
+    else hipLaunchKernelGGL((k_linearform<2, 3>), grid, blk, 0, ctx->stream, L, bp, s->ndofs);
+    GRMP_HIP(ctx, hipGetLastError());
+    return GRMP_OK;
+}
+
+// --------------------------------------------------------- ItemIntegrator
+// per cell: out[k, c] = sum_q (sum_dofs u_dof * phi_k(dof)) * w_q * vol
+// (NoAction branch of src/assemblypatterns/itemintegrator.jl:276-283)
+template <int FE, int NQ>
+__global__ void k_integrate_identity(LfDev L, const int32_t *__restrict__ dofs, const double *__restrict__ u,
+                                     double *__restrict__ out)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= L.ncells) return;
+    const int64_t nc = L.ncells, nn = L.nnodes;
+    int32_t v0 = L.cells[e], v1 = L.cells[nc + e], v2 = L.cells[2 * nc + e];
+    double A11 = L.coords[v1] - L.coords[v0], A12 = L.coords[v2] - L.coords[v0];
+    double A21 = L.coords[nn + v1] - L.coords[nn + v0], A22 = L.coords[nn + v2] - L.coords[nn + v0];
+    double vol = fabs(A11 * A22 - A12 * A21) * 0.5;
+    constexpr int NS = (FE == 1) ? 3 : 6;
+    for (int k = 0; k < L.ncomp; ++k) {
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double x0, x1, w;
+            qtri<NQ>(q, x0, x1, w);
+            double in = 0.0;
+            for (int c = 0; c < L.ncomp; ++c)
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    double phi = (FE == 1) ? p1tri_val(s, x0, x1) : p2tri_val(s, x0, x1);
+                    double uv = u[dofs[(int64_t)(c * NS + s) * nc + e]];
+                    in += uv * (c == k ? phi : 0.0);
+                }
+            acc += in * w * vol;
+        }
+        out[e * L.ncomp + k] = acc;
+    }
+}
+
+extern "C" int grmp_integrate_identity(grmp_ctx *ctx, const grmp_space *s, int quadorder, const grmp_vector *u,
+                                       int64_t offset, double *out)
+{
+    GRMP_CHECK(ctx, ctx && s && u && out, GRMP_ERR_ARG, "grmp_integrate_identity: null argument");
+    GRMP_CHECK(ctx, offset >= 0 && offset + s->ndofs <= u->n, GRMP_ERR_ARG, "grmp_integrate_identity: vector too short");
+    const grmp_mesh *m = s->mesh;
+    int nq = quad_nq_tri(quadorder);
+    GRMP_CHECK(ctx, m->dim == 2 && (s->fe == GRMP_FE_H1P1 || s->fe == GRMP_FE_H1P2) && nq, GRMP_ERR_UNSUPPORTED,
+               "grmp_integrate_identity: FE %d in %dD with quadorder %d not built", s->fe, m->dim, quadorder);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    LfDev L;
+    memset(&L, 0, sizeof(L));
+    L.ncomp = s->ncomp;
+    L.coords = m->coords;
+    L.cells = m->cells;
+    L.nnodes = m->nnodes;
+    L.ncells = m->ncells;
+    double *d = nullptr;
+    GRMP_TRY(dalloc(ctx, &d, m->ncells * s->ncomp));
+    dim3 grid(nblk(m->ncells, 256)), blk(256);
+    const double *up = u->d + offset;
+    if (s->fe == GRMP_FE_H1P1 && nq == 1) hipLaunchKernelGGL((k_integrate_identity<1, 1>), grid, blk, 0, ctx->stream, L, s->celldofs, up, d);
+    else if (s->fe == GRMP_FE_H1P1) hipLaunchKernelGGL((k_integrate_identity<1, 3>), grid, blk, 0, ctx->stream, L, s->celldofs, up, d);
+    else if (nq == 1) hipLaunchKernelGGL((k_integrate_identity<2, 1>), grid, blk, 0, ctx->stream, L, s->celldofs, up, d);
+    else hipLaunchKernelGGL((k_integrate_identity<2, 3>), grid, blk, 0, ctx->stream, L, s->celldofs, up, d);
+    GRMP_HIP(ctx, hipGetLastError());
+    GRMP_HIP(ctx, hipMemcpyAsync(out, d, sizeof(double) * m->ncells * s->ncomp, hipMemcpyDeviceToHost, ctx->stream));
+    GRMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    dfree(ctx, d);
+    return GRMP_OK;
+}
+
+// ------------------------------------------------------------ CellVolumes
+__global__ void k_cellvolumes(const double *coords, const int32_t *cells, int64_t nn, int64_t nc, int dim, double *out)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nc) return;
+    DevForm F;
+    F.coords = coords;
+    F.cells = cells;
+    F.nnodes = nn;
+    F.ncells = nc;
+    double vol;
+    if (dim == 2) {
+        double M[2][2];
+        trafo_tri(F, e, M, vol);
+    } else {
+        double M[3][3];
+        trafo_tet(F, e, M, vol);
+    }
+    out[e] = vol;
+}
+
+extern "C" int grmp_mesh_cellvolumes(const grmp_mesh *m, double *out)
+{
+    if (!m || !out) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = m->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    double *d = nullptr;
+    GRMP_TRY(dalloc(ctx, &d, m->ncells));
+    hipLaunchKernelGGL(k_cellvolumes, dim3(nblk(m->ncells, 256)), dim3(256), 0, ctx->stream, m->coords, m->cells,
+                       m->nnodes, m->ncells, m->dim, d);
+    GRMP_HIP(ctx, hipGetLastError());
+    GRMP_HIP(ctx, hipMemcpyAsync(out, d, sizeof(double) * m->ncells, hipMemcpyDeviceToHost, ctx->stream));
+    GRMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    dfree(ctx, d);
+    return GRMP_OK;
+}

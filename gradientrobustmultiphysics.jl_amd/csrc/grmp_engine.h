@@ -1,0 +1,97 @@
+// Batched Krylov engine of libgrmp_hip.so.
+//
+// Many independent sparse systems ("problems") are solved together by one
+// sequence of kernel launches: every kernel walks a table of row tiles, each
+// tile belonging to one problem, and a per-problem phase machine decides what
+// a tile does (FP64 residual, FP32 inner CG step, FP64 update).  Scalars of
+// a problem are reduced by the last block that finishes one of its tiles, in
+// a fixed order, so a problem's iterates do not depend on which other
+// problems share the launch.  The launch sequence of one step never changes,
+// so it is captured once as a hipGraph and replayed; problems join and leave
+// between replays by rewriting the tile table.
+#pragma once
+#include "grmp_internal.h"
+
+enum KPhase { PH_IDLE = 0, PH_RESID = 1, PH_INIT = 2, PH_ITER = 3, PH_UPDATE = 4, PH_DONE = 5, PH_FAIL = 6 };
+
+struct KProb {
+    // matrix (CSR, 0-based), fixed-dof mask, Jacobi inverse diagonal
+    const int32_t *rowptr;
+    const int32_t *cols;
+    const double *val64;
+    const float *val32;
+    const float *dinv32;
+    const uint8_t *fixed;
+    // FP64 outer vectors
+    const double *b;
+    double *x;
+    double *r64;
+    // FP32 inner vectors
+    float *r, *p, *q, *dx;
+    int32_t n, lanes;
+    int32_t phase, status;
+    int32_t outer, max_outer;
+    int32_t inner_it, inner_maxit;
+    int64_t iters, maxit;
+    double bb, rtol, inner_rtol;
+    double rnorm;       // ||r64|| of the current sweep (correction scale)
+    double rz, alpha, beta, relres;
+    unsigned int cnt;   // tiles finished in the current kernel
+    int32_t pad_;
+};
+
+struct KTile {
+    int32_t prob, row0, row1, pad;
+};
+
+struct KRange {
+    int32_t t0, nt;
+};
+
+struct EngineSlot {
+    bool used = false;
+    int32_t ntiles = 0;
+};
+
+class KEngine {
+  public:
+    KEngine(grmp_ctx *ctx, hipStream_t st, int cap_probs, int cap_tiles, int tile_rows);
+    ~KEngine();
+    int init();
+    // registers a problem (device pointers valid until remove); returns slot
+    int add(const KProb &p, int *slot);
+    void remove(int slot);
+    int nactive() const { return nact_; }
+    // uploads the tile table if it changed (stream ordered)
+    int sync_tables();
+    // runs `steps` engine steps (graph replay when use_graph)
+    int run(int steps, bool use_graph);
+    // reads phase/status/iterations of all slots (blocking)
+    int poll(std::vector<KProb> &out);
+    int64_t total_tiles() const { return ttot_; }
+
+  private:
+    int capture(int grid);
+    int launch_step(int grid);
+    grmp_ctx *ctx_;
+    hipStream_t st_;
+    int cap_p_, cap_t_, tile_rows_;
+    KProb *dP_ = nullptr;
+    KTile *dT_ = nullptr;
+    KRange *dR_ = nullptr;
+    double *dpart_ = nullptr;
+    int *dnt_ = nullptr;
+    KProb *hP_ = nullptr;       // pinned poll buffer
+    KTile *hT_ = nullptr;       // pinned table staging
+    KRange *hR_ = nullptr;
+    std::vector<EngineSlot> slots_;
+    std::vector<KProb> init_;
+    int nact_ = 0;
+    int64_t ttot_ = 0;
+    bool dirty_ = true;
+    int graph_grid_ = 0;
+    hipGraphExec_t exec_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+};
+
+int engine_pick_lanes(double nnz_per_row);

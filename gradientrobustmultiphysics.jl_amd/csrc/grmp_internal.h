@@ -1,0 +1,175 @@
+// Internal declarations of libgrmp_hip.so (gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/grmp.h"
+
+// ------------------------------------------------------------------ errors
+struct grmp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::mutex mu;
+    // reusable device scratch for library primitives (scan temp storage, flags)
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    int *dflag = nullptr;  // device error/overflow flag
+};
+
+int grmp_fail(grmp_ctx *ctx, int code, const char *fmt, ...);
+
+#define GRMP_HIP(ctx, call)                                                              \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return grmp_fail((ctx), e_ == hipErrorOutOfMemory ? GRMP_ERR_OOM : GRMP_ERR_HIP, \
+                             "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),      \
+                             __FILE__, __LINE__);                                        \
+    } while (0)
+
+#define GRMP_CHECK(ctx, cond, code, ...)                 \
+    do {                                                 \
+        if (!(cond)) return grmp_fail((ctx), (code), __VA_ARGS__); \
+    } while (0)
+
+#define GRMP_TRY(expr)            \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_ != GRMP_OK) return rc_; \
+    } while (0)
+
+// device allocation helpers (stream-ordered)
+template <class T>
+int dalloc(grmp_ctx *ctx, T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    GRMP_HIP(ctx, hipMallocAsync((void **)p, count * sizeof(T), ctx->stream));
+    return GRMP_OK;
+}
+template <class T>
+void dfree(grmp_ctx *ctx, T *&p)
+{
+    if (p) (void)hipFreeAsync((void *)p, ctx->stream);
+    p = nullptr;
+}
+int ensure_scratch(grmp_ctx *ctx, size_t bytes);
+
+// ------------------------------------------------------------------ objects
+struct grmp_vector {
+    grmp_ctx *ctx;
+    int64_t n;
+    double *d;
+};
+
+enum MeshKind { MESH_GENERIC = 0, MESH_UNITSQUARE = 1, MESH_UNITCUBE = 2 };
+
+struct grmp_mesh {
+    grmp_ctx *ctx;
+    int dim;
+    int64_t nnodes, ncells;
+    int nloc;            // vertices per cell
+    double *coords;      // SoA [dim][nnodes]
+    int32_t *cells;      // SoA [nloc][ncells], 0-based
+    int kind;            // MeshKind
+    int N;               // structured resolution
+};
+
+struct grmp_space {
+    grmp_ctx *ctx;
+    const grmp_mesh *mesh;
+    int fe, ncomp;
+    int64_t ndofs;
+    int nloc;            // dofs per cell (all components)
+    int32_t *celldofs;   // SoA [nloc][ncells], 0-based
+    int32_t *inc_ptr;    // [ndofs+1] cell incidences per dof
+    int32_t *inc;        // [ncells*nloc] packed cell*nloc + local, sorted per dof
+    int max_inc;
+    int64_t nfaces;      // P2 on unit square: number of faces
+};
+
+// element/form kinds resolved on the device
+enum FormKind {
+    FK_LAPLACE_P1TRI = 1,
+    FK_LAPLACE_P2TRI = 2,
+    FK_LAPLACE_P1TET = 3,
+    FK_HOOKE_P1VTRI = 4,
+    FK_MASS_P1TRI = 5,
+    FK_MASS_P2TRI = 6,
+};
+
+struct DevForm {
+    int kind;
+    int nq;              // quadrature points
+    int coef_cell;       // kappa through the action
+    int rnloc, cnloc;    // local dofs of row / column space
+    int words;           // slot words per incidence
+    int pad_;
+    double factor;
+    double lam, mu;
+    const double *kappa;
+    const double *coords;
+    const int32_t *cells;
+    int64_t nnodes, ncells;
+    const int32_t *rdofs, *inc_ptr, *inc;
+    const int32_t *cdofs;
+    int64_t ro, co;      // global offsets of row / column block
+    uint32_t *pos;       // slot map (uint8 positions relative to the row start)
+};
+
+struct FormRec {
+    grmp_form f;
+    const grmp_space *rs, *cs;
+    int kind;
+    int nq;
+    uint32_t *pos = nullptr;
+    int words = 0;
+};
+
+struct grmp_matrix {
+    grmp_ctx *ctx;
+    int nblocks;
+    const grmp_space *spaces[8];
+    int64_t offs[9];
+    int64_t n;
+    std::vector<FormRec> forms;
+    // pattern
+    bool have_pattern = false;
+    int64_t nnz = 0;
+    int max_row = 0;
+    int32_t *rowptr = nullptr;  // [n+1]
+    int32_t *cols = nullptr;    // [nnz]
+    uint8_t *diagpos = nullptr; // [n]
+    uint8_t *fixed = nullptr;   // [n] mask
+    std::vector<int64_t> fixed_host;
+    double penalty = 1e60;
+    // values
+    int values = 0;
+    double *val64 = nullptr;
+    float *val32 = nullptr;
+    double *dinv64 = nullptr;
+    float *dinv32 = nullptr;
+    float last_asm_ms = 0.f;
+    DevForm *dforms = nullptr;      // device copy of the forms, grouped by block row
+    std::vector<int> form_first;    // first form of each block row in dforms
+};
+
+// ------------------------------------------------------------- internals
+int build_incidence(grmp_ctx *ctx, grmp_space *s);
+int exclusive_scan_i32(grmp_ctx *ctx, const int32_t *in, int32_t *out, int64_t n);  // out[n+1]
+int form_resolve(grmp_matrix *A, FormRec &r);
+void fill_devform(const grmp_matrix *A, const FormRec &r, DevForm &d);
+
+// solver internals shared with the server
+struct ProblemDesc;
+int spmv_launch(grmp_ctx *ctx, const grmp_matrix *A, int values, const double *x, double *y,
+                hipStream_t st);
+
+// timing
+double wall_seconds();

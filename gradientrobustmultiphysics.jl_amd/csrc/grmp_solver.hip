@@ -1,0 +1,818 @@
+// Sparse solves of libgrmp_hip.so: CSR SpMV, the plain one-problem FP64
+// Jacobi-PCG and the batched FP32-inner / FP64-refinement Krylov engine.
+//
+// These replace the direct factorization behind solve_direct!
+// (src/solvers.jl:622-686: LinearSolve + UMFPACK at :55,656-660).  Fixed
+// (penalized) dofs are eliminated instead of carried with the 1e60 penalty:
+// x_fixed = b_fixed / A_dd, their rows are dropped from the Krylov space and
+// the residual is measured on the free rows only, the convention of the
+// reference's residual check (src/solvers.jl:663-668).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "grmp_engine.h"
+
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+#define KB 256
+
+// ------------------------------------------------------------ reductions
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// block sums of (a, b) in a fixed order; valid in thread 0
+__device__ __forceinline__ void block_sum2(double &a, double &b)
+{
+    __shared__ double sa[KB / 64], sb[KB / 64];
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) {
+        sa[w] = a;
+        sb[w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double x = 0.0, y = 0.0;
+#pragma unroll
+        for (int k = 0; k < KB / 64; ++k) {
+            x += sa[k];
+            y += sb[k];
+        }
+        a = x;
+        b = y;
+    }
+    __syncthreads();
+}
+
+// row i of a CSR matrix times x with L lanes per row; full sum in lane 0
+template <int L, class V, class X>
+__device__ __forceinline__ double csr_row(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                          const V *__restrict__ v, const X *__restrict__ x, int i, int lane)
+{
+    double s = 0.0;
+    const int32_t k1 = rp[i + 1];
+    for (int32_t k = rp[i] + lane; k < k1; k += L) s += (double)v[k] * (double)x[ci[k]];
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) s += __shfl_down(s, o, L);
+    return s;
+}
+
+// FP32 variant accumulating in FP32 (the inner solve's arithmetic)
+template <int L>
+__device__ __forceinline__ float csr_row32(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                           const float *__restrict__ v, const float *__restrict__ x, int i, int lane)
+{
+    float s = 0.0f;
+    const int32_t k1 = rp[i + 1];
+    for (int32_t k = rp[i] + lane; k < k1; k += L) s += v[k] * x[ci[k]];
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) s += __shfl_down(s, o, L);
+    return s;
+}
+
+int engine_pick_lanes(double nnz_per_row)
+{
+    if (nnz_per_row <= 3.0) return 2;
+    if (nnz_per_row <= 12.0) return 4;
+    if (nnz_per_row <= 24.0) return 8;
+    return 16;
+}
+
+// --------------------------------------------------------------- SpMV API
+template <int L, class V>
+__global__ void k_spmv(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci, const V *__restrict__ v,
+                       const double *__restrict__ x, double *__restrict__ y, int64_t n)
+{
+    const int lane = threadIdx.x % L;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / L;
+    if (i >= n) return;
+    double s = csr_row<L>(rp, ci, v, x, (int)i, lane);
+    if (lane == 0) y[i] = s;
+}
+
+int spmv_launch(grmp_ctx *ctx, const grmp_matrix *A, int values, const double *x, double *y, hipStream_t st)
+{
+    const int64_t n = A->n;
+    const int L = engine_pick_lanes((double)A->nnz / (double)(n ? n : 1));
+    dim3 grid(nblk(n * L, 256)), blk(256);
+#define SPMV_CASE(LL)                                                                                         \
+    case LL:                                                                                                  \
+        if (values == GRMP_VALUES_FP64)                                                                       \
+            hipLaunchKernelGGL((k_spmv<LL, double>), grid, blk, 0, st, A->rowptr, A->cols, A->val64, x, y, n); \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_spmv<LL, float>), grid, blk, 0, st, A->rowptr, A->cols, A->val32, x, y, n);  \
+        break;
+    switch (L) {
+        SPMV_CASE(2)
+        SPMV_CASE(4)
+        SPMV_CASE(8)
+        SPMV_CASE(16)
+    default: break;
+    }
+#undef SPMV_CASE
+    GRMP_HIP(ctx, hipGetLastError());
+    return GRMP_OK;
+}
+
+extern "C" int grmp_matrix_spmv(grmp_matrix *A, int values, const grmp_vector *x, grmp_vector *y)
+{
+    if (!A || !x || !y) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = A->ctx;
+    GRMP_CHECK(ctx, x->n == A->n && y->n == A->n, GRMP_ERR_ARG, "grmp_matrix_spmv: vector length != %lld", (long long)A->n);
+    GRMP_CHECK(ctx, (values == GRMP_VALUES_FP64 && A->val64) || (values == GRMP_VALUES_FP32 && A->val32), GRMP_ERR_ARG,
+               "grmp_matrix_spmv: requested values not assembled");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    return spmv_launch(ctx, A, values, x->d, y->d, ctx->stream);
+}
+
+// ------------------------------------------------------- plain FP64 PCG
+// One problem, host-driven: every iteration launches its kernels and reads
+// the dot products back to the host, which computes alpha/beta and checks
+// convergence.  This is the conventional GPU CG and the baseline the
+// batched engine is measured against.
+#define PB 256
+#define PGRID 1024
+
+__global__ void k_plain_init(const int32_t *rp, const uint8_t *dpos, const double *val, const uint8_t *fixed,
+                             const double *b, double *x, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (fixed[i]) {
+        double d = dpos[i] != 255 ? val[rp[i] + dpos[i]] : 1.0;
+        x[i] = b[i] / d;
+    } else {
+        x[i] = 0.0;
+    }
+}
+
+// r = b - A x on free rows (0 on fixed rows); z = dinv r; p = z;
+// partials of (r.z, r.r)
+template <int L>
+__global__ void k_plain_resid(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci, const double *__restrict__ v,
+                              const uint8_t *__restrict__ fixed, const double *__restrict__ dinv, const double *__restrict__ b,
+                              const double *__restrict__ x, double *__restrict__ r, double *__restrict__ z,
+                              double *__restrict__ p, int64_t n, double *__restrict__ part)
+{
+    const int lane = threadIdx.x % L;
+    double a0 = 0.0, a1 = 0.0;
+    for (int64_t i = ((int64_t)blockIdx.x * PB + threadIdx.x) / L; i < n; i += (int64_t)gridDim.x * PB / L) {
+        double s = csr_row<L>(rp, ci, v, x, (int)i, lane);
+        if (lane == 0) {
+            double ri = fixed[i] ? 0.0 : b[i] - s;
+            double zi = dinv[i] * ri;
+            r[i] = ri;
+            z[i] = zi;
+            p[i] = zi;
+            a0 += ri * zi;
+            a1 += ri * ri;
+        }
+    }
+    block_sum2(a0, a1);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = a0;
+        part[2 * blockIdx.x + 1] = a1;
+    }
+}
+
+// q = A p on free rows; partial p.q
+template <int L>
+__global__ void k_plain_spmv(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci, const double *__restrict__ v,
+                             const uint8_t *__restrict__ fixed, const double *__restrict__ p, double *__restrict__ q,
+                             int64_t n, double *__restrict__ part)
+{
+    const int lane = threadIdx.x % L;
+    double a0 = 0.0, a1 = 0.0;
+    for (int64_t i = ((int64_t)blockIdx.x * PB + threadIdx.x) / L; i < n; i += (int64_t)gridDim.x * PB / L) {
+        double s = csr_row<L>(rp, ci, v, p, (int)i, lane);
+        if (lane == 0) {
+            double qi = fixed[i] ? 0.0 : s;
+            q[i] = qi;
+            a0 += p[i] * qi;
+        }
+    }
+    block_sum2(a0, a1);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = a0;
+        part[2 * blockIdx.x + 1] = a1;
+    }
+}
+
+// x += alpha p; r -= alpha q; z = dinv r; partials (r.z, r.r)
+__global__ void k_plain_axpy(double alpha, const double *__restrict__ p, const double *__restrict__ q,
+                             const double *__restrict__ dinv, double *__restrict__ x, double *__restrict__ r,
+                             double *__restrict__ z, int64_t n, double *__restrict__ part)
+{
+    double a0 = 0.0, a1 = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x; i < n; i += (int64_t)gridDim.x * PB) {
+        x[i] += alpha * p[i];
+        double ri = r[i] - alpha * q[i];
+        double zi = dinv[i] * ri;
+        r[i] = ri;
+        z[i] = zi;
+        a0 += ri * zi;
+        a1 += ri * ri;
+    }
+    block_sum2(a0, a1);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = a0;
+        part[2 * blockIdx.x + 1] = a1;
+    }
+}
+
+__global__ void k_plain_pupd(double beta, const double *__restrict__ z, double *__restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x; i < n; i += (int64_t)gridDim.x * PB) p[i] = z[i] + beta * p[i];
+}
+
+static int plain_sum(grmp_ctx *ctx, const double *dpart, double *hpart, int grid, double &a, double &b)
+{
+    GRMP_HIP(ctx, hipMemcpyAsync(hpart, dpart, sizeof(double) * 2 * grid, hipMemcpyDeviceToHost, ctx->stream));
+    GRMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    a = 0.0;
+    b = 0.0;
+    for (int k = 0; k < grid; ++k) {
+        a += hpart[2 * k];
+        b += hpart[2 * k + 1];
+    }
+    return GRMP_OK;
+}
+
+static int plain_cg(grmp_matrix *A, const grmp_vector *bv, grmp_vector *xv, const grmp_solve_opts &o, grmp_solve_info &info)
+{
+    grmp_ctx *ctx = A->ctx;
+    const int64_t n = A->n;
+    double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *part = nullptr;
+    GRMP_TRY(dalloc(ctx, &r, n));
+    GRMP_TRY(dalloc(ctx, &z, n));
+    GRMP_TRY(dalloc(ctx, &p, n));
+    GRMP_TRY(dalloc(ctx, &q, n));
+    GRMP_TRY(dalloc(ctx, &part, 2 * PGRID));
+    double *hpart = nullptr;
+    GRMP_HIP(ctx, hipHostMalloc((void **)&hpart, sizeof(double) * 2 * PGRID, hipHostMallocDefault));
+    const int L = engine_pick_lanes((double)A->nnz / (double)n);
+    const int grid = (int)std::min<int64_t>(PGRID, std::max<int64_t>(1, (n * L + PB - 1) / PB));
+    const int gride = (int)std::min<int64_t>(PGRID, std::max<int64_t>(1, (n + PB - 1) / PB));
+    hipLaunchKernelGGL(k_plain_init, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, A->rowptr, A->diagpos, A->val64, A->fixed,
+                       bv->d, xv->d, n);
+#define PL_CASE(LL, KERNEL, ...) \
+    case LL: hipLaunchKernelGGL((KERNEL<LL>), dim3(grid), dim3(PB), 0, ctx->stream, __VA_ARGS__); break;
+#define PL_SWITCH(KERNEL, ...)                 \
+    switch (L) {                               \
+        PL_CASE(2, KERNEL, __VA_ARGS__)        \
+        PL_CASE(4, KERNEL, __VA_ARGS__)        \
+        PL_CASE(8, KERNEL, __VA_ARGS__)        \
+        PL_CASE(16, KERNEL, __VA_ARGS__)       \
+    default: break;                            \
+    }
+    PL_SWITCH(k_plain_resid, A->rowptr, A->cols, A->val64, A->fixed, A->dinv64, bv->d, xv->d, r, z, p, n, part);
+    double rz, rr, pq, dummy;
+    int rc = plain_sum(ctx, part, hpart, grid, rz, rr);
+    double bb = rr;
+    int64_t it = 0;
+    int status = GRMP_OK;
+    const int every = std::max(1, o.check_every);
+    while (rc == GRMP_OK) {
+        if (!(rr == rr) || std::isinf(rr)) { status = GRMP_ERR_NONFINITE; break; }
+        if (rr <= o.rtol * o.rtol * bb || rr == 0.0) break;
+        if (it >= o.maxit) { status = GRMP_ERR_NOCONV; break; }
+        PL_SWITCH(k_plain_spmv, A->rowptr, A->cols, A->val64, A->fixed, p, q, n, part);
+        if ((rc = plain_sum(ctx, part, hpart, grid, pq, dummy))) break;
+        double alpha = rz / pq;
+        hipLaunchKernelGGL(k_plain_axpy, dim3(gride), dim3(PB), 0, ctx->stream, alpha, p, q, A->dinv64, xv->d, r, z, n, part);
+        double rz2;
+        if ((rc = plain_sum(ctx, part, hpart, gride, rz2, rr))) break;
+        double beta = rz2 / rz;
+        rz = rz2;
+        hipLaunchKernelGGL(k_plain_pupd, dim3(gride), dim3(PB), 0, ctx->stream, beta, z, p, n);
+        ++it;
+        (void)every;
+    }
+#undef PL_SWITCH
+#undef PL_CASE
+    GRMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    info.iterations = it;
+    info.outer = 1;
+    info.relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+    info.status = rc ? rc : status;
+    (void)hipHostFree(hpart);
+    dfree(ctx, r);
+    dfree(ctx, z);
+    dfree(ctx, p);
+    dfree(ctx, q);
+    dfree(ctx, part);
+    return rc;
+}
+
+// ============================================================ the engine
+
+// all threads: true if this block finished the last tile of problem pi
+// (thread 0 holds the block sums a, b on entry)
+__device__ __forceinline__ bool finish_tile(KProb *P, const KRange *R, double *part, int t, int pi, double a, double b)
+{
+    __shared__ int s_last;
+    if (threadIdx.x == 0) {
+        part[2 * t] = a;
+        part[2 * t + 1] = b;
+        __threadfence();
+        unsigned prev = atomicAdd(&P[pi].cnt, 1u);
+        s_last = (prev + 1 == (unsigned)R[pi].nt) ? 1 : 0;
+    }
+    __syncthreads();
+    const bool last = s_last != 0;
+    __syncthreads();
+    return last;
+}
+
+// sums the tile partials of problem pi in a fixed order (valid in thread 0)
+__device__ __forceinline__ void sum_parts(const KRange *R, const double *part, int pi, double &a, double &b)
+{
+    __threadfence();
+    a = 0.0;
+    b = 0.0;
+    const int t0 = R[pi].t0, nt = R[pi].nt;
+    for (int j = threadIdx.x; j < nt; j += KB) {
+        a += __builtin_nontemporal_load(&part[2 * (t0 + j)]);
+        b += __builtin_nontemporal_load(&part[2 * (t0 + j) + 1]);
+    }
+    block_sum2(a, b);
+}
+
+#define ENGINE_ARGS KProb *__restrict__ P, const KTile *__restrict__ T, const KRange *__restrict__ R, \
+                    double *__restrict__ part, const int *__restrict__ ntiles
+
+// SpMV step of the inner CG (phase ITER): q = A32 p, alpha = rz / p.q
+template <int L>
+__device__ __forceinline__ double spmv_tile(const KProb &pr, int row0, int row1)
+{
+    const int lane = threadIdx.x % L;
+    double acc = 0.0;
+    for (int i = row0 + (int)threadIdx.x / L; i < row1; i += KB / L) {
+        float s = csr_row32<L>(pr.rowptr, pr.cols, pr.val32, pr.p, i, lane);
+        if (lane == 0) {
+            pr.q[i] = s;
+            acc += (double)pr.p[i] * (double)s;
+        }
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(KB) void k_eng_spmv(ENGINE_ARGS)
+{
+    const int nt = *ntiles;
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        const KTile tl = T[t];
+        const int pi = tl.prob;
+        if (P[pi].phase != PH_ITER) continue;
+        const KProb pr = P[pi];
+        double acc = 0.0, dummy = 0.0;
+        switch (pr.lanes) {
+        case 2: acc = spmv_tile<2>(pr, tl.row0, tl.row1); break;
+        case 4: acc = spmv_tile<4>(pr, tl.row0, tl.row1); break;
+        case 8: acc = spmv_tile<8>(pr, tl.row0, tl.row1); break;
+        default: acc = spmv_tile<16>(pr, tl.row0, tl.row1); break;
+        }
+        block_sum2(acc, dummy);
+        if (finish_tile(P, R, part, t, pi, acc, dummy)) {
+            double pq, d2;
+            sum_parts(R, part, pi, pq, d2);
+            if (threadIdx.x == 0) {
+                KProb &w = P[pi];
+                double alpha = w.rz / pq;
+                w.alpha = alpha;
+                if (!(pq == pq) || __builtin_isinf(pq)) {
+                    w.status = GRMP_ERR_NONFINITE;
+                    w.phase = PH_FAIL;
+                } else if (!(pq > 0.0) || __builtin_isinf(alpha)) {
+                    w.phase = PH_UPDATE;  // breakdown: keep the correction so far
+                    w.alpha = 0.0;
+                }
+                w.cnt = 0;
+            }
+        }
+    }
+}
+
+// dx += alpha p; r -= alpha q; z = dinv r; (r.z, r.r) -> beta / convergence
+__global__ __launch_bounds__(KB) void k_eng_axpy(ENGINE_ARGS)
+{
+    const int nt = *ntiles;
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        const KTile tl = T[t];
+        const int pi = tl.prob;
+        if (P[pi].phase != PH_ITER) continue;
+        const KProb pr = P[pi];
+        const float a = (float)pr.alpha;
+        double rz = 0.0, rr = 0.0;
+        for (int i = tl.row0 + threadIdx.x; i < tl.row1; i += KB) {
+            float pi_ = pr.p[i];
+            pr.dx[i] += a * pi_;
+            float ri = pr.r[i] - a * pr.q[i];
+            pr.r[i] = ri;
+            float zi = pr.dinv32[i] * ri;
+            rz += (double)ri * (double)zi;
+            rr += (double)ri * (double)ri;
+        }
+        block_sum2(rz, rr);
+        if (finish_tile(P, R, part, t, pi, rz, rr)) {
+            double srz, srr;
+            sum_parts(R, part, pi, srz, srr);
+            if (threadIdx.x == 0) {
+                KProb &w = P[pi];
+                w.inner_it += 1;
+                w.iters += 1;
+                if (!(srr == srr) || __builtin_isinf(srr)) {
+                    w.status = GRMP_ERR_NONFINITE;
+                    w.phase = PH_FAIL;
+                } else if (std::sqrt(srr) <= w.inner_rtol || w.inner_it >= w.inner_maxit || w.iters >= w.maxit) {
+                    w.phase = PH_UPDATE;
+                } else {
+                    w.beta = srz / w.rz;
+                    w.rz = srz;
+                }
+                w.cnt = 0;
+            }
+        }
+    }
+}
+
+// elementwise phases: ITER p = dinv r + beta p; UPDATE x += ||r64|| dx then
+// RESID; INIT r = r64/||r64||, dx = 0, p = dinv r then ITER
+__global__ __launch_bounds__(KB) void k_eng_elem(ENGINE_ARGS)
+{
+    const int nt = *ntiles;
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        const KTile tl = T[t];
+        const int pi = tl.prob;
+        const int ph = P[pi].phase;
+        if (ph != PH_ITER && ph != PH_UPDATE && ph != PH_INIT) continue;
+        const KProb pr = P[pi];
+        if (ph == PH_ITER) {
+            const float beta = (float)pr.beta;
+            for (int i = tl.row0 + threadIdx.x; i < tl.row1; i += KB) pr.p[i] = pr.dinv32[i] * pr.r[i] + beta * pr.p[i];
+            continue;
+        }
+        double acc = 0.0, dummy = 0.0;
+        if (ph == PH_UPDATE) {
+            for (int i = tl.row0 + threadIdx.x; i < tl.row1; i += KB) pr.x[i] += pr.rnorm * (double)pr.dx[i];
+        } else {
+            const double inv = 1.0 / pr.rnorm;
+            for (int i = tl.row0 + threadIdx.x; i < tl.row1; i += KB) {
+                float ri = (float)(pr.r64[i] * inv);
+                float zi = pr.dinv32[i] * ri;
+                pr.r[i] = ri;
+                pr.dx[i] = 0.0f;
+                pr.p[i] = zi;
+                acc += (double)ri * (double)zi;
+            }
+        }
+        block_sum2(acc, dummy);
+        if (finish_tile(P, R, part, t, pi, acc, dummy)) {
+            double s, d2;
+            sum_parts(R, part, pi, s, d2);
+            if (threadIdx.x == 0) {
+                KProb &w = P[pi];
+                if (ph == PH_UPDATE) {
+                    w.phase = PH_RESID;
+                } else {
+                    w.rz = s;
+                    w.inner_it = 0;
+                    w.phase = PH_ITER;
+                }
+                w.cnt = 0;
+            }
+        }
+    }
+}
+
+// FP64 residual r64 = b - A64 x (phase RESID) -> converged / next sweep
+template <int L>
+__device__ __forceinline__ double resid_tile(const KProb &pr, int row0, int row1)
+{
+    const int lane = threadIdx.x % L;
+    double acc = 0.0;
+    for (int i = row0 + (int)threadIdx.x / L; i < row1; i += KB / L) {
+        double s = csr_row<L>(pr.rowptr, pr.cols, pr.val64, pr.x, i, lane);
+        if (lane == 0) {
+            double ri = pr.fixed[i] ? 0.0 : pr.b[i] - s;
+            pr.r64[i] = ri;
+            acc += ri * ri;
+        }
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(KB) void k_eng_resid(ENGINE_ARGS)
+{
+    const int nt = *ntiles;
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        const KTile tl = T[t];
+        const int pi = tl.prob;
+        if (P[pi].phase != PH_RESID) continue;
+        const KProb pr = P[pi];
+        double acc = 0.0, dummy = 0.0;
+        switch (pr.lanes) {
+        case 2: acc = resid_tile<2>(pr, tl.row0, tl.row1); break;
+        case 4: acc = resid_tile<4>(pr, tl.row0, tl.row1); break;
+        case 8: acc = resid_tile<8>(pr, tl.row0, tl.row1); break;
+        default: acc = resid_tile<16>(pr, tl.row0, tl.row1); break;
+        }
+        block_sum2(acc, dummy);
+        if (finish_tile(P, R, part, t, pi, acc, dummy)) {
+            double rr, d2;
+            sum_parts(R, part, pi, rr, d2);
+            if (threadIdx.x == 0) {
+                KProb &w = P[pi];
+                if (w.outer == 0) w.bb = rr;
+                w.relres = w.bb > 0.0 ? std::sqrt(rr / w.bb) : 0.0;
+                if (!(rr == rr) || __builtin_isinf(rr)) {
+                    w.status = GRMP_ERR_NONFINITE;
+                    w.phase = PH_FAIL;
+                } else if (rr == 0.0 || w.relres <= w.rtol) {
+                    w.phase = PH_DONE;
+                } else if (w.outer >= w.max_outer || w.iters >= w.maxit) {
+                    w.status = GRMP_ERR_NOCONV;
+                    w.phase = PH_FAIL;
+                } else {
+                    w.rnorm = std::sqrt(rr);
+                    w.outer += 1;
+                    w.phase = PH_INIT;
+                }
+                w.cnt = 0;
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------- engine host
+KEngine::KEngine(grmp_ctx *ctx, hipStream_t st, int cap_probs, int cap_tiles, int tile_rows)
+    : ctx_(ctx), st_(st), cap_p_(cap_probs), cap_t_(cap_tiles), tile_rows_(tile_rows)
+{
+}
+
+KEngine::~KEngine()
+{
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (dP_) (void)hipFree(dP_);
+    if (dT_) (void)hipFree(dT_);
+    if (dR_) (void)hipFree(dR_);
+    if (dpart_) (void)hipFree(dpart_);
+    if (dnt_) (void)hipFree(dnt_);
+    if (hP_) (void)hipHostFree(hP_);
+    if (hT_) (void)hipHostFree(hT_);
+    if (hR_) (void)hipHostFree(hR_);
+}
+
+int KEngine::init()
+{
+    grmp_ctx *ctx = ctx_;
+    GRMP_HIP(ctx, hipMalloc(&dP_, sizeof(KProb) * cap_p_));
+    GRMP_HIP(ctx, hipMalloc(&dT_, sizeof(KTile) * cap_t_));
+    GRMP_HIP(ctx, hipMalloc(&dR_, sizeof(KRange) * cap_p_));
+    GRMP_HIP(ctx, hipMalloc(&dpart_, sizeof(double) * 2 * cap_t_));
+    GRMP_HIP(ctx, hipMalloc(&dnt_, sizeof(int)));
+    GRMP_HIP(ctx, hipHostMalloc((void **)&hP_, sizeof(KProb) * cap_p_, hipHostMallocDefault));
+    GRMP_HIP(ctx, hipHostMalloc((void **)&hT_, sizeof(KTile) * cap_t_, hipHostMallocDefault));
+    GRMP_HIP(ctx, hipHostMalloc((void **)&hR_, sizeof(KRange) * cap_p_, hipHostMallocDefault));
+    GRMP_HIP(ctx, hipMemsetAsync(dP_, 0, sizeof(KProb) * cap_p_, st_));
+    GRMP_HIP(ctx, hipMemsetAsync(dnt_, 0, sizeof(int), st_));
+    slots_.assign(cap_p_, EngineSlot());
+    init_.assign(cap_p_, KProb());
+    GRMP_HIP(ctx, hipStreamSynchronize(st_));
+    return GRMP_OK;
+}
+
+int KEngine::add(const KProb &p, int *slot)
+{
+    grmp_ctx *ctx = ctx_;
+    int s = -1;
+    for (int k = 0; k < cap_p_; ++k)
+        if (!slots_[k].used) { s = k; break; }
+    GRMP_CHECK(ctx, s >= 0, GRMP_ERR_ADMISSION, "engine: all %d problem slots in use", cap_p_);
+    int nt = (int)((p.n + tile_rows_ - 1) / tile_rows_);
+    GRMP_CHECK(ctx, ttot_ + nt <= cap_t_, GRMP_ERR_ADMISSION, "engine: tile table full");
+    KProb q = p;
+    q.phase = PH_RESID;
+    q.status = GRMP_OK;
+    q.outer = 0;
+    q.inner_it = 0;
+    q.iters = 0;
+    q.cnt = 0;
+    init_[s] = q;
+    // the host copy lives in init_ until the stream has consumed it
+    GRMP_HIP(ctx, hipMemcpyAsync(dP_ + s, &init_[s], sizeof(KProb), hipMemcpyHostToDevice, st_));
+    slots_[s].used = true;
+    slots_[s].ntiles = nt;
+    ttot_ += nt;
+    ++nact_;
+    dirty_ = true;
+    *slot = s;
+    return GRMP_OK;
+}
+
+void KEngine::remove(int slot)
+{
+    if (slot < 0 || slot >= cap_p_ || !slots_[slot].used) return;
+    slots_[slot].used = false;
+    ttot_ -= slots_[slot].ntiles;
+    slots_[slot].ntiles = 0;
+    --nact_;
+    dirty_ = true;
+}
+
+int KEngine::sync_tables()
+{
+    if (!dirty_) return GRMP_OK;
+    grmp_ctx *ctx = ctx_;
+    // the pinned staging buffers may still feed an earlier copy
+    GRMP_HIP(ctx, hipStreamSynchronize(st_));
+    int64_t t = 0;
+    for (int s = 0; s < cap_p_; ++s) {
+        if (!slots_[s].used) {
+            hR_[s].t0 = 0;
+            hR_[s].nt = 0;
+            continue;
+        }
+        const int n = init_[s].n;
+        hR_[s].t0 = (int32_t)t;
+        hR_[s].nt = slots_[s].ntiles;
+        for (int k = 0; k < slots_[s].ntiles; ++k) {
+            hT_[t].prob = s;
+            hT_[t].row0 = k * tile_rows_;
+            hT_[t].row1 = std::min(n, (k + 1) * tile_rows_);
+            hT_[t].pad = 0;
+            ++t;
+        }
+    }
+    int ntot = (int)t;
+    GRMP_HIP(ctx, hipMemcpyAsync(dT_, hT_, sizeof(KTile) * (t ? t : 1), hipMemcpyHostToDevice, st_));
+    GRMP_HIP(ctx, hipMemcpyAsync(dR_, hR_, sizeof(KRange) * cap_p_, hipMemcpyHostToDevice, st_));
+    GRMP_HIP(ctx, hipMemcpyAsync(dnt_, &ntot, sizeof(int), hipMemcpyHostToDevice, st_));
+    // ntot and the staging buffers must outlive the copies
+    GRMP_HIP(ctx, hipStreamSynchronize(st_));
+    dirty_ = false;
+    return GRMP_OK;
+}
+
+int KEngine::launch_step(int grid)
+{
+    dim3 g(grid), b(KB);
+    hipLaunchKernelGGL(k_eng_spmv, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_);
+    hipLaunchKernelGGL(k_eng_axpy, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_);
+    hipLaunchKernelGGL(k_eng_elem, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_);
+    hipLaunchKernelGGL(k_eng_resid, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_);
+    GRMP_HIP(ctx_, hipGetLastError());
+    return GRMP_OK;
+}
+
+#define ENGINE_GRAPH_STEPS 8
+
+int KEngine::capture(int grid)
+{
+    grmp_ctx *ctx = ctx_;
+    if (exec_) { (void)hipGraphExecDestroy(exec_); exec_ = nullptr; }
+    if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+    GRMP_HIP(ctx, hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < ENGINE_GRAPH_STEPS; ++s) {
+        dim3 g(grid), b(KB);
+        hipLaunchKernelGGL(k_eng_spmv, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_);
+        hipLaunchKernelGGL(k_eng_axpy, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_);
+        hipLaunchKernelGGL(k_eng_elem, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_);
+        hipLaunchKernelGGL(k_eng_resid, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_);
+    }
+    GRMP_HIP(ctx, hipStreamEndCapture(st_, &graph_));
+    GRMP_HIP(ctx, hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+    graph_grid_ = grid;
+    return GRMP_OK;
+}
+
+int KEngine::run(int steps, bool use_graph)
+{
+    GRMP_TRY(sync_tables());
+    if (ttot_ == 0) return GRMP_OK;
+    int grid = 64;
+    while (grid < ttot_ && grid < 4096) grid <<= 1;
+    if (use_graph) {
+        if (!exec_ || graph_grid_ != grid) GRMP_TRY(capture(grid));
+        for (int s = 0; s < steps; s += ENGINE_GRAPH_STEPS) GRMP_HIP(ctx_, hipGraphLaunch(exec_, st_));
+    } else {
+        for (int s = 0; s < steps; ++s) GRMP_TRY(launch_step(grid));
+    }
+    return GRMP_OK;
+}
+
+int KEngine::poll(std::vector<KProb> &out)
+{
+    grmp_ctx *ctx = ctx_;
+    GRMP_HIP(ctx, hipMemcpyAsync(hP_, dP_, sizeof(KProb) * cap_p_, hipMemcpyDeviceToHost, st_));
+    GRMP_HIP(ctx, hipStreamSynchronize(st_));
+    out.assign(hP_, hP_ + cap_p_);
+    return GRMP_OK;
+}
+
+// ------------------------------------------------------------ solve API
+extern "C" void grmp_solve_opts_default(grmp_solve_opts *o)
+{
+    if (!o) return;
+    o->method = GRMP_SOLVE_CG;
+    o->precond = GRMP_PREC_JACOBI;
+    o->precision = GRMP_PRECISION_FP64;
+    o->rtol = 1e-10;
+    o->maxit = 1000000;
+    o->inner_rtol = 1e-4;
+    o->inner_maxit = 100000;
+    o->check_every = 1;
+}
+
+static int solve_engine(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, const grmp_solve_opts &o, grmp_solve_info &info)
+{
+    grmp_ctx *ctx = A->ctx;
+    const int64_t n = A->n;
+    GRMP_CHECK(ctx, A->val32 && A->val64, GRMP_ERR_ARG, "FP32_IR solve needs FP64 and FP32 values assembled");
+    double *r64 = nullptr;
+    float *buf = nullptr;
+    GRMP_TRY(dalloc(ctx, &r64, n));
+    GRMP_TRY(dalloc(ctx, &buf, 4 * n));
+    hipLaunchKernelGGL(k_plain_init, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, A->rowptr, A->diagpos, A->val64, A->fixed,
+                       b->d, x->d, n);
+    KEngine eng(ctx, ctx->stream, 1, (int)((n + 2047) / 2048) + 1, 2048);
+    GRMP_TRY(eng.init());
+    KProb p;
+    memset(&p, 0, sizeof(p));
+    p.rowptr = A->rowptr;
+    p.cols = A->cols;
+    p.val64 = A->val64;
+    p.val32 = A->val32;
+    p.dinv32 = A->dinv32;
+    p.fixed = A->fixed;
+    p.b = b->d;
+    p.x = x->d;
+    p.r64 = r64;
+    p.r = buf;
+    p.p = buf + n;
+    p.q = buf + 2 * n;
+    p.dx = buf + 3 * n;
+    p.n = (int32_t)n;
+    p.lanes = engine_pick_lanes((double)A->nnz / (double)n);
+    p.max_outer = 60;
+    p.inner_maxit = (int32_t)std::min<int64_t>(o.inner_maxit, 1 << 30);
+    p.maxit = o.maxit;
+    p.rtol = o.rtol;
+    p.inner_rtol = o.inner_rtol;
+    int slot;
+    GRMP_TRY(eng.add(p, &slot));
+    std::vector<KProb> st;
+    int status = GRMP_OK;
+    for (;;) {
+        GRMP_TRY(eng.run(64, true));
+        GRMP_TRY(eng.poll(st));
+        const KProb &q = st[slot];
+        if (q.phase == PH_DONE || q.phase == PH_FAIL) {
+            status = q.phase == PH_DONE ? GRMP_OK : q.status;
+            info.iterations = q.iters;
+            info.outer = q.outer;
+            info.relres = q.relres;
+            break;
+        }
+    }
+    info.status = status;
+    dfree(ctx, r64);
+    dfree(ctx, buf);
+    GRMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GRMP_OK;
+}
+
+extern "C" int grmp_solve(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, const grmp_solve_opts *opts,
+                          grmp_solve_info *info)
+{
+    if (!A || !b || !x) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = A->ctx;
+    grmp_solve_opts o;
+    grmp_solve_opts_default(&o);
+    if (opts) o = *opts;
+    grmp_solve_info inf;
+    memset(&inf, 0, sizeof(inf));
+    GRMP_CHECK(ctx, A->have_pattern && A->val64, GRMP_ERR_ARG, "grmp_solve: matrix not assembled in FP64");
+    GRMP_CHECK(ctx, b->n == A->n && x->n == A->n, GRMP_ERR_ARG, "grmp_solve: vector length != %lld", (long long)A->n);
+    GRMP_CHECK(ctx, o.method == GRMP_SOLVE_CG, GRMP_ERR_UNSUPPORTED, "grmp_solve: only CG is built");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    double t0 = wall_seconds();
+    int rc;
+    if (o.precision == GRMP_PRECISION_FP32_IR) rc = solve_engine(A, b, x, o, inf);
+    else rc = plain_cg(A, b, x, o, inf);
+    inf.seconds = wall_seconds() - t0;
+    if (info) *info = inf;
+    if (rc) return rc;
+    if (inf.status == GRMP_ERR_NOCONV)
+        return grmp_fail(ctx, GRMP_ERR_NOCONV, "grmp_solve: no convergence, relres %.3e after %lld iterations",
+                         inf.relres, (long long)inf.iterations);
+    if (inf.status == GRMP_ERR_NONFINITE) return grmp_fail(ctx, GRMP_ERR_NONFINITE, "grmp_solve: non-finite values in the Krylov loop");
+    return inf.status;
+}

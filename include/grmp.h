@@ -1,0 +1,271 @@
+/*
+ * grmp.h -- C ABI of libgrmp_hip.so, the MI355X (gfx950) assembly-and-solve
+ * engine behind GradientRobustMultiPhysics.jl's hot path.
+ *
+ * The entry points replace, one for one, the reference functions named in
+ * each comment (paths relative to the reference root).  They take plain
+ * pointers and sizes only.  Index conventions follow Julia, the reference's
+ * host language: index arrays are 1-based and matrices are column-major, so
+ * a Julia caller passes xgrid[Coordinates] ([dim, nnodes]), xgrid[CellNodes]
+ * ([nloc, ncells]) and FES[CellDofs] ([ndofs_loc, ncells]) unchanged.
+ *
+ * Ownership: host pointers are borrowed for the duration of a call; the
+ * library copies what it keeps.  Device buffers belong to the opaque handles
+ * and are released by the matching *_destroy.  Calls on one context are
+ * serialized by the caller; different contexts are independent.
+ *
+ * Every function returns GRMP_OK (0) or an error code; grmp_last_error()
+ * gives the message of the last failure on that context.
+ */
+#ifndef GRMP_H
+#define GRMP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define GRMP_OK 0
+#define GRMP_ERR_ARG 1          /* malformed argument (cf. @assert, bilinearform.jl:39-40) */
+#define GRMP_ERR_OOM 2          /* device allocation failed */
+#define GRMP_ERR_HIP 3          /* HIP runtime error */
+#define GRMP_ERR_NOCONV 4       /* max iterations reached (cf. solvers.jl:1398-1400) */
+#define GRMP_ERR_NONFINITE 5    /* NaN/Inf met in a Krylov loop */
+#define GRMP_ERR_ADMISSION 6    /* request larger than the server's memory budget */
+#define GRMP_ERR_UNSUPPORTED 7  /* element / operator combination not built */
+#define GRMP_ERR_BUSY 8         /* result not ready yet (server poll) */
+
+/* --------------------------------------------------------- enumerations */
+/* finite element types (files under src/fedefs/) */
+#define GRMP_FE_H1P1 1          /* H1P1{ncomp}      src/fedefs/h1_p1.jl */
+#define GRMP_FE_H1P2 2          /* H1P2{ncomp,2}    src/fedefs/h1_p2.jl */
+#define GRMP_FE_H1BR 3          /* H1BR{2}          src/fedefs/h1v_br.jl */
+#define GRMP_FE_L2P0 4          /* L2P0{1}          src/fedefs/l2_p0.jl */
+
+/* bilinear forms (src/pdeoperators.jl) */
+#define GRMP_BLF_LAPLACE 1      /* (kappa grad u, grad v): LaplaceOperator :154-169, or
+                                   BilinearForm([Gradient,Gradient], kappa-action;
+                                   APT_SymmetricBilinearForm) :355-409 */
+#define GRMP_BLF_HOOKE2D 2      /* (C eps(u), eps(v)): HookStiffnessOperator2D :256-276 */
+#define GRMP_BLF_MASS 3         /* (kappa u, v): ReactionOperator :176-203 */
+#define GRMP_BLF_DIVP0 4        /* LagrangeMultiplier(Divergence) :214-226 (with
+                                   transposed copy) */
+
+/* how the coefficient enters (src/assemblypatterns/bilinearform.jl:306-313) */
+#define GRMP_COEF_FACTOR 0      /* NoAction: constant coefficient multiplies itemfactor */
+#define GRMP_COEF_CELL 1        /* item-dependent Action: result = kappa[cell] .* input */
+
+/* linear forms */
+#define GRMP_LF_CONST 1         /* LinearForm(Identity, DataFunction(f)) :694-705 */
+
+/* value precision flags of an assembled matrix */
+#define GRMP_VALUES_FP64 1
+#define GRMP_VALUES_FP32 2
+
+/* solver choices (replaces the LinearSolve/UMFPACK call, src/solvers.jl:656-660) */
+#define GRMP_SOLVE_CG 1
+#define GRMP_SOLVE_MINRES 2
+#define GRMP_PREC_JACOBI 1
+#define GRMP_PRECISION_FP64 1   /* plain FP64 Krylov */
+#define GRMP_PRECISION_FP32_IR 2/* FP32 matrix/SpMV/preconditioner inside FP64 refinement */
+
+/* ---------------------------------------------------------------- types */
+typedef struct grmp_ctx grmp_ctx;
+typedef struct grmp_mesh grmp_mesh;
+typedef struct grmp_space grmp_space;
+typedef struct grmp_vector grmp_vector;
+typedef struct grmp_matrix grmp_matrix;
+typedef struct grmp_server grmp_server;
+
+typedef struct grmp_form {
+    int kind;                 /* GRMP_BLF_* */
+    int coef_mode;            /* GRMP_COEF_* */
+    int quadorder;            /* as prepare_assembly! chooses it (assemblypatterns.jl:559-565) */
+    int row_block, col_block; /* block of FEMatrix (0-based) */
+    double factor;            /* PDEOperator factor (NoAction coefficient) */
+    double params[4];         /* Hooke: lambda, mu */
+    const grmp_vector *cell_coef; /* per-cell kappa for GRMP_COEF_CELL, else NULL */
+} grmp_form;
+
+typedef struct grmp_lform {
+    int kind;                 /* GRMP_LF_CONST */
+    int quadorder;
+    double factor;
+    double f[4];              /* constant data, ncomp values */
+} grmp_lform;
+
+typedef struct grmp_solve_opts {
+    int method;               /* GRMP_SOLVE_CG / GRMP_SOLVE_MINRES */
+    int precond;              /* GRMP_PREC_JACOBI */
+    int precision;            /* GRMP_PRECISION_FP64 / GRMP_PRECISION_FP32_IR */
+    double rtol;              /* ||b - Ax|| <= rtol ||b|| over free dofs */
+    int64_t maxit;            /* total Krylov iterations */
+    double inner_rtol;        /* FP32 inner solve reduction per refinement sweep */
+    int64_t inner_maxit;
+    int check_every;          /* plain mode: host residual check period (1 = every iteration) */
+} grmp_solve_opts;
+
+typedef struct grmp_solve_info {
+    int status;
+    int outer;                /* refinement sweeps */
+    int64_t iterations;       /* Krylov iterations (inner ones in FP32_IR) */
+    double relres;            /* final FP64 relative residual */
+    double seconds;
+} grmp_solve_info;
+
+/* ------------------------------------------------------------- context */
+int grmp_ctx_create(int device, grmp_ctx **out);
+int grmp_ctx_destroy(grmp_ctx *ctx);
+const char *grmp_last_error(const grmp_ctx *ctx);
+int grmp_ctx_synchronize(grmp_ctx *ctx);
+/* library build/ABI version: major*10000 + minor*100 + patch */
+int grmp_version(void);
+
+/* ---------------------------------------------------- device vectors */
+int grmp_vector_create(grmp_ctx *ctx, int64_t n, grmp_vector **out);
+int grmp_vector_destroy(grmp_vector *v);
+int64_t grmp_vector_length(const grmp_vector *v);
+int grmp_vector_upload(grmp_vector *v, const double *host);
+int grmp_vector_download(const grmp_vector *v, double *host);
+int grmp_vector_fill(grmp_vector *v, double value);
+/* v[idx[k]] = values[k] (idx 1-based); the b[fixed] = penalty*target write of
+ * solve_direct! (src/solvers.jl:649-650) */
+int grmp_vector_scatter(grmp_vector *v, int64_t count, const int64_t *idx, const double *values);
+/* seeded log-normal field exp(mu + sigma Z), Z ~ N(0,1) from Philox-4x32-10
+ * keyed by seed and counted by entry index (Box-Muller) */
+int grmp_vector_lognormal(grmp_vector *v, uint64_t seed, double mu, double sigma);
+
+/* --------------------------------------------------------------- meshes */
+/* ExtendableGrid upload: Coordinates [dim,nnodes], CellNodes [nloc,ncells]
+ * (1-based).  dim 2 (Triangle2D, nloc 3) or 3 (Tetrahedron3D, nloc 4). */
+int grmp_mesh_upload(grmp_ctx *ctx, int dim, int64_t nnodes, const double *coords,
+                     int64_t ncells, int nloc, const int32_t *cellnodes, grmp_mesh **out);
+/* Unit square with N x N squares, each split along its (0,0)-(1,1) diagonal,
+ * generated on the device (node id j(N+1)+i, coordinates i/N). */
+int grmp_mesh_unitsquare(grmp_ctx *ctx, int N, grmp_mesh **out);
+/* Unit cube with n^3 cubes in 6 tetrahedra each (Kuhn split), interior
+ * nodes jittered by +-jitter*h per coordinate, boundary nodes tangentially. */
+int grmp_mesh_unitcube(grmp_ctx *ctx, int n, double jitter, uint64_t seed, grmp_mesh **out);
+int grmp_mesh_destroy(grmp_mesh *m);
+int grmp_mesh_info(const grmp_mesh *m, int *dim, int64_t *nnodes, int64_t *ncells);
+int grmp_mesh_download(const grmp_mesh *m, double *coords, int32_t *cellnodes);
+/* per-cell volumes (ExtendableGrids CellVolumes) */
+int grmp_mesh_cellvolumes(const grmp_mesh *m, double *out);
+
+/* ------------------------------------------------------------- FESpace */
+/* FESpace from the caller's dofmap: FES[CellDofs] ([ndofs_loc, ncells],
+ * 1-based; src/dofmaps.jl:201-363).  Replaces FESpace{FEType}(xgrid)
+ * (src/finiteelements.jl:72-112) on the device side. */
+int grmp_space_upload(grmp_ctx *ctx, const grmp_mesh *mesh, int fe, int ncomp, int64_t ndofs,
+                      int ndofs_loc, const int32_t *celldofs, grmp_space **out);
+/* FESpace with the dofmap generated on the device: P1 ("N1") on any mesh;
+ * P2 ("N1F1") on a grmp_mesh_unitsquare mesh, faces numbered by first
+ * appearance in cell order with local faces [1 2; 2 3; 3 1]. */
+int grmp_space_create(grmp_ctx *ctx, const grmp_mesh *mesh, int fe, int ncomp, grmp_space **out);
+int grmp_space_destroy(grmp_space *s);
+int grmp_space_info(const grmp_space *s, int64_t *ndofs, int *ndofs_loc);
+int grmp_space_download_celldofs(const grmp_space *s, int32_t *celldofs);
+/* BFaceDofs of all boundary faces of a unit-square/unit-cube mesh (the dofs
+ * HomogeneousDirichletBoundary fixes, src/boundarydata.jl:198-257); 1-based,
+ * sorted.  Call with out == NULL to get the count. */
+int grmp_space_boundary_dofs(const grmp_space *s, int64_t *count, int64_t *out);
+
+/* ------------------------------------------------------- system matrix */
+/* FEMatrix over nblocks FESpaces (src/fematrix.jl:186-213). */
+int grmp_matrix_create(grmp_ctx *ctx, int nblocks, const grmp_space *const *spaces,
+                       grmp_matrix **out);
+int grmp_matrix_destroy(grmp_matrix *A);
+/* register a BilinearForm for block (row_block, col_block); forms are
+ * assembled in registration order, as operators of one block are. */
+int grmp_matrix_add_form(grmp_matrix *A, const grmp_form *form);
+/* fixed dofs (1-based, global) and penalty: A[d,d] = penalty after assembly
+ * (apply_penalties!, src/fematrix.jl:349-355; src/solvers.jl:647-652) */
+int grmp_matrix_set_fixed(grmp_matrix *A, int64_t nfixed, const int64_t *fixed, double penalty);
+/* Builds the sparsity pattern the reference's first assembly creates: the
+ * set of (i,j) that received a nonzero local contribution (_addnz,
+ * src/fematrix.jl:54-65) plus penalized diagonals, with the element-to-slot
+ * map.  Uses the current coefficient values. */
+int grmp_matrix_build_pattern(grmp_matrix *A);
+/* assemble!(A::FEMatrixBlock, AP) for all registered forms
+ * (src/assemblypatterns/bilinearform.jl:92-400).  values: GRMP_VALUES_* mask. */
+int grmp_matrix_assemble(grmp_matrix *A, int values);
+int grmp_matrix_info(const grmp_matrix *A, int64_t *n, int64_t *nnz, int *max_row);
+/* A.entries.cscmatrix: colptr[n+1], rowval[nnz] (1-based), nzval[nnz] */
+int grmp_matrix_export_csc(const grmp_matrix *A, int64_t *colptr, int64_t *rowval, double *nzval);
+/* y = A x with the FP64 or FP32 values (x, y FP64 device vectors) */
+int grmp_matrix_spmv(grmp_matrix *A, int values, const grmp_vector *x, grmp_vector *y);
+/* seconds of the last assemble call's kernels (HIP events) */
+double grmp_matrix_last_assembly_seconds(const grmp_matrix *A);
+
+/* ---------------------------------------------------------- LinearForm */
+/* assemble!(b::FEVectorBlock, AP) for a LinearForm, added into b at offset
+ * (src/assemblypatterns/linearform.jl:47-251) */
+int grmp_assemble_linearform(grmp_ctx *ctx, const grmp_space *space, const grmp_lform *form,
+                             grmp_vector *b, int64_t offset);
+
+/* ------------------------------------------------------ ItemIntegrator */
+/* evaluate(ItemIntegrator([Identity]), u): per-cell integrals of the FE
+ * function u (coefficients at offset), out [ncomp, ncells] column-major
+ * (src/assemblypatterns/itemintegrator.jl:169-360) */
+int grmp_integrate_identity(grmp_ctx *ctx, const grmp_space *space, int quadorder,
+                            const grmp_vector *u, int64_t offset, double *out);
+
+/* ---------------------------------------------------------------- solve */
+void grmp_solve_opts_default(grmp_solve_opts *o);
+/* Solves the penalized system A x = b (b[fixed] = penalty*value) as
+ * solve_direct! does (src/solvers.jl:622-686), with fixed dofs eliminated. */
+int grmp_solve(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, const grmp_solve_opts *opts,
+               grmp_solve_info *info);
+
+/* ---------------------------------------------------------------- serving */
+#define GRMP_REQ_POISSON 1      /* PoissonProblem, per-cell log-normal kappa, f = 1 */
+#define GRMP_REQ_ELASTICITY 2   /* LinearElasticityProblem P1{2}, Hooke scaled per cell */
+
+typedef struct grmp_request {
+    int kind;                 /* GRMP_REQ_* */
+    int fe;                   /* GRMP_FE_H1P1 / GRMP_FE_H1P2 */
+    int N;                    /* N x N unit square */
+    uint64_t seed;            /* coefficient field seed */
+    double sigma;             /* log-normal sigma */
+    double E, nu;             /* elasticity */
+    double rtol;
+    double *x_out;            /* host buffer of ndofs doubles, filled on completion (may be NULL) */
+} grmp_request;
+
+typedef struct grmp_result {
+    int status;
+    int64_t ndofs;
+    int64_t iterations;
+    int outer;
+    double relres;
+    double t_submit, t_admit, t_done; /* seconds on the server clock */
+} grmp_result;
+
+typedef struct grmp_server_config {
+    int mode;                 /* 1 = batched + FP32_IR (served), 2 = plain one-at-a-time FP64 */
+    int64_t memory_budget;    /* bytes of HBM for resident problems */
+    int max_active;           /* problems in flight */
+    int graph;                /* capture the Krylov step as a hipGraph */
+} grmp_server_config;
+
+/* Request loop (continuous batching): requests are admitted by HBM footprint
+ * as memory and slots free up, assembled by batched kernels and solved by a
+ * block-diagonal Krylov step shared by every problem in flight.  Runs its own
+ * host thread and HIP stream. */
+int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *cfg, grmp_server **out);
+int grmp_server_submit(grmp_server *s, const grmp_request *req, int64_t *ticket);
+/* GRMP_OK when done (result filled), GRMP_ERR_BUSY while pending */
+int grmp_server_poll(grmp_server *s, int64_t ticket, grmp_result *res);
+int grmp_server_wait(grmp_server *s, int64_t ticket, grmp_result *res);
+double grmp_server_clock(const grmp_server *s);
+int grmp_server_destroy(grmp_server *s);
+/* footprint in bytes the server would reserve for a request */
+int64_t grmp_request_footprint(const grmp_request *req);
+int64_t grmp_request_ndofs(const grmp_request *req);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GRMP_H */

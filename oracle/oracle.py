@@ -1,0 +1,220 @@
+"""Python front end of the CPU oracle (test infrastructure only).
+
+Only tests/, the smoke check and the benchmark's CPU-baseline leg import this
+module; it is the checker, never the product.  It wraps libgrmp_oracle.so, the
+C restatement of GradientRobustMultiPhysics.jl's element loops (see the header
+of grmp_oracle.c for the reference file:line each function follows), and adds
+the host-side bookkeeping the reference does in Julia:
+
+* ``boundary_dofs`` -- BFaceDofs of all boundary faces, the fixed-dof list of
+  HomogeneousDirichletBoundary (src/boundarydata.jl:198-257,
+  src/dofmaps.jl:201-363).
+* ``assemble_system`` -- the system assemble + penalty of solve_direct!
+  (src/solvers.jl:622-660; penalty src/solvers.jl:647-652 and
+  src/fematrix.jl:349-355).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+FE_P1, FE_P2, FE_BR, FE_P0 = 1, 2, 3, 4
+BLF_LAPLACE, BLF_HOOKE2D, BLF_MASS = 1, 2, 3
+COEF_FACTOR, COEF_CELL = 0, 1
+PENALTY = 1e60
+
+_d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_I64 = ctypes.c_int64
+_VP = ctypes.c_void_p
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libgrmp_oracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.or_unitsquare.argtypes = [ctypes.c_int, _d, _i32]
+        L.or_tri_faces.argtypes = [_I64, _I64, _i32, _i32, _i32, _i32]
+        L.or_tri_faces.restype = _I64
+        L.or_celldofs.argtypes = [ctypes.c_int, ctypes.c_int, _I64, _I64, _I64, _i32, _i32, _i32]
+        L.or_matrix_new.argtypes = [_I64]
+        L.or_matrix_new.restype = _VP
+        L.or_matrix_free.argtypes = [_VP]
+        L.or_matrix_flush.argtypes = [_VP]
+        L.or_matrix_flush.restype = _I64
+        L.or_matrix_export.argtypes = [_VP, _i64, _i32, _d]
+        L.or_matrix_penalize.argtypes = [_VP, _I64, _i32, ctypes.c_double]
+        L.or_assemble_blf.argtypes = [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_double, _d, _d, _I64,
+                                      _d, _i32, _i32, ctypes.c_int32, ctypes.c_int32]
+        L.or_assemble_lf_const.argtypes = [_d, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           _d, ctypes.c_double, _I64, _d, _i32, _i32, ctypes.c_int32]
+        L.or_integrate_identity.argtypes = [_d, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            _d, _I64, _d, _i32, _i32]
+        L.or_pcg.argtypes = [_I64, _i64, _i32, _d, _d, _d, _u8, ctypes.c_double, _I64,
+                             ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.or_pcg.restype = _I64
+        L.or_num_threads.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+# ------------------------------------------------------------------ meshes
+
+def unitsquare(N):
+    """Structured N x N unit square, two triangles per square (0-based)."""
+    nn, nc = (N + 1) ** 2, 2 * N * N
+    coords = np.zeros((nn, 2))
+    cells = np.zeros((nc, 3), dtype=np.int32)
+    lib().or_unitsquare(N, coords, cells)
+    return coords, cells
+
+
+def tri_faces(cells, nnodes):
+    nc = cells.shape[0]
+    cellfaces = np.zeros((nc, 3), dtype=np.int32)
+    facenodes = np.zeros((3 * nc, 2), dtype=np.int32)
+    facecells = np.zeros((3 * nc, 2), dtype=np.int32)
+    nf = lib().or_tri_faces(nc, nnodes, np.ascontiguousarray(cells, dtype=np.int32),
+                            cellfaces, facenodes, facecells)
+    return cellfaces, facenodes[:nf].copy(), facecells[:nf].copy()
+
+
+def celldofs(fe, ncomp, cells, nnodes, cellfaces=None, nfaces=0):
+    nloc = {FE_P1: 3, FE_P2: 6}[fe]
+    nc = cells.shape[0]
+    out = np.zeros((nc, nloc * ncomp), dtype=np.int32)
+    cf = cellfaces if cellfaces is not None else np.zeros((nc, 3), dtype=np.int32)
+    lib().or_celldofs(fe, ncomp, nc, nnodes, nfaces, np.ascontiguousarray(cells, dtype=np.int32),
+                      np.ascontiguousarray(cf, dtype=np.int32), out)
+    ndofs = (nnodes + (nfaces if fe == FE_P2 else 0)) * ncomp
+    return out, ndofs
+
+
+def boundary_dofs(fe, ncomp, nnodes, facenodes, facecells, nfaces):
+    """Fixed dofs of a homogeneous Dirichlet condition on every boundary face."""
+    bf = np.nonzero(facecells[:, 1] < 0)[0]
+    nodes = np.unique(facenodes[bf].ravel())
+    per = nnodes + (nfaces if fe == FE_P2 else 0)
+    out = []
+    for k in range(ncomp):
+        out.append(nodes + k * per)
+        if fe == FE_P2:
+            out.append(nnodes + bf + k * per)
+    return np.unique(np.concatenate(out)).astype(np.int32)
+
+
+class Space:
+    """Host FE space for the oracle: structured unit square + P1/P2 dofmaps."""
+
+    def __init__(self, N, fe, ncomp=1):
+        self.N, self.fe, self.ncomp = N, fe, ncomp
+        self.coords, self.cells = unitsquare(N)
+        self.nnodes = self.coords.shape[0]
+        self.cellfaces, self.facenodes, self.facecells = tri_faces(self.cells, self.nnodes)
+        self.nfaces = self.facenodes.shape[0]
+        self.celldofs, self.ndofs = celldofs(fe, ncomp, self.cells, self.nnodes, self.cellfaces,
+                                             self.nfaces)
+        self.fixed = boundary_dofs(fe, ncomp, self.nnodes, self.facenodes, self.facecells,
+                                   self.nfaces)
+
+
+# ---------------------------------------------------------------- assembly
+
+class CSC:
+    def __init__(self, n, colptr, rowval, nzval):
+        self.n, self.colptr, self.rowval, self.nzval = n, colptr, rowval, nzval
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csc_matrix((self.nzval, self.rowval, self.colptr), shape=(self.n, self.n))
+
+
+def assemble_matrix(n, forms, fixed=None, penalty=PENALTY):
+    """Assembles the BilinearForms in ``forms`` (sequentially, as the reference
+    assembles the operators of one block) into a fresh matrix, applies the
+    penalty on ``fixed`` and returns the flushed CSC (0-based)."""
+    L = lib()
+    h = L.or_matrix_new(n)
+    try:
+        for f in forms:
+            geom = f.get("geom", 2)
+            kappa = np.ascontiguousarray(f.get("kappa", np.zeros(1)), dtype=np.float64)
+            params = np.ascontiguousarray(f.get("params", np.zeros(2)), dtype=np.float64)
+            L.or_assemble_blf(h, f["kind"], geom, f["fe"], f.get("ncomp", 1), f["quadorder"],
+                              f.get("coef", COEF_FACTOR), float(f.get("factor", 1.0)), kappa,
+                              params, f["cells"].shape[0],
+                              np.ascontiguousarray(f["coords"], dtype=np.float64),
+                              np.ascontiguousarray(f["cells"], dtype=np.int32),
+                              np.ascontiguousarray(f["celldofs"], dtype=np.int32),
+                              f.get("ro", 0), f.get("co", 0))
+        if fixed is not None and len(fixed):
+            fx = np.ascontiguousarray(fixed, dtype=np.int32)
+            L.or_matrix_penalize(h, fx.size, fx, penalty)
+        nnz = L.or_matrix_flush(h)
+        colptr = np.zeros(n + 1, dtype=np.int64)
+        rowval = np.zeros(max(nnz, 1), dtype=np.int32)
+        nzval = np.zeros(max(nnz, 1))
+        L.or_matrix_export(h, colptr, rowval, nzval)
+        return CSC(n, colptr, rowval[:nnz], nzval[:nnz])
+    finally:
+        L.or_matrix_free(h)
+
+
+def assemble_lf_const(b, fe, ncomp, quadorder, f, coords, cells, celldofs, factor=1.0, geom=2,
+                      offset=0):
+    fv = np.ascontiguousarray(np.atleast_1d(np.asarray(f, dtype=np.float64)))
+    lib().or_assemble_lf_const(b, geom, fe, ncomp, quadorder, fv, factor, cells.shape[0],
+                               np.ascontiguousarray(coords, dtype=np.float64),
+                               np.ascontiguousarray(cells, dtype=np.int32),
+                               np.ascontiguousarray(celldofs, dtype=np.int32), offset)
+    return b
+
+
+def integrate_identity(u, fe, ncomp, quadorder, coords, cells, celldofs, geom=2):
+    nc = cells.shape[0]
+    out = np.zeros(nc * ncomp)
+    lib().or_integrate_identity(out, geom, fe, ncomp, quadorder,
+                                np.ascontiguousarray(u, dtype=np.float64), nc,
+                                np.ascontiguousarray(coords, dtype=np.float64),
+                                np.ascontiguousarray(cells, dtype=np.int32),
+                                np.ascontiguousarray(celldofs, dtype=np.int32))
+    return out.reshape(nc, ncomp)
+
+
+def penalized_rhs(b, fixed, penalty=PENALTY, values=None):
+    b = b.copy()
+    b[fixed] = penalty * (0.0 if values is None else values)
+    return b
+
+
+def pcg(A, b, fixed, rtol=1e-10, maxit=100000, threads=0):
+    """FP64 Jacobi-PCG with fixed dofs eliminated; returns (x, iters, relres)."""
+    n = A.n
+    mask = np.zeros(n, dtype=np.uint8)
+    mask[fixed] = 1
+    x = np.zeros(n)
+    rr = ctypes.c_double(0.0)
+    it = lib().or_pcg(n, A.colptr, A.rowval, A.nzval, np.ascontiguousarray(b, dtype=np.float64), x,
+                      mask, rtol, maxit, threads, ctypes.byref(rr))
+    return x, int(it), rr.value
+
+
+def direct_solve(A, b):
+    """FP64 sparse LU (SuperLU) of the penalized CSC: the direct-solve stand-in
+    for UMFPACK (src/solvers.jl:55,656-660)."""
+    import scipy.sparse.linalg as spla
+    return spla.spsolve(A.to_scipy().tocsc(), b)

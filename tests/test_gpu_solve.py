@@ -1,0 +1,76 @@
+"""Device solves against the oracle's FP64 sparse LU (the direct-solve
+stand-in for UMFPACK) on the same penalized systems.
+
+Bar: relative error of the solution vector <= 1e-8 (stated tolerance of
+the path; the solves stop at relative residual 1e-12)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _poisson(grmp, O, N, fe, seed):
+    grid = grmp.grid_unitsquare(N)
+    ft = grmp.H1P1(1) if fe == 1 else grmp.H1P2(1)
+    fes = grmp.FESpace(ft, grid)
+    kap = grmp.DeviceVector(grid.ctx, grid.ncells)
+    kap.lognormal(seed, 0.0, 1.0)
+    pde = grmp.PoissonProblem(cell_coefficient=grmp.CellCoefficient(kap))
+    pde.add_rhsdata(0, grmp.LinearForm(grmp.Identity, grmp.DataFunction([1.0])))
+    pde.add_boundarydata(0, grmp.HomogeneousDirichletBoundary)
+    S = O.Space(N, fe)
+    ref = O.assemble_matrix(S.ndofs, [dict(kind=O.BLF_LAPLACE, fe=fe, quadorder=2 * fe - 2, factor=1.0,
+                                          coef=O.COEF_CELL, kappa=kap.download(), coords=S.coords, cells=S.cells,
+                                          celldofs=S.celldofs)], fixed=S.fixed)
+    b = np.zeros(S.ndofs)
+    O.assemble_lf_const(b, fe, 1, fe, [1.0], S.coords, S.cells, S.celldofs)
+    b = O.penalized_rhs(b, S.fixed)
+    xref = O.direct_solve(ref, b)
+    return grid, fes, pde, xref
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+@pytest.mark.parametrize("N,fe", [(24, 1), (24, 2), (2, 2)])
+def test_poisson_solve(grmp, O, precision, N, fe):
+    print(f"[case] poisson solve N={N} fe={fe} {precision}", flush=True)
+    grid, fes, pde, xref = _poisson(grmp, O, N, fe, 17)
+    sol = grmp.FEVector([fes])
+    info, A, b = grmp.solve(sol, pde, precision=precision, rtol=1e-12)
+    x = sol.entries()
+    err = np.linalg.norm(x - xref) / np.linalg.norm(xref)
+    assert info.status == 0
+    assert info.relres <= 1e-12
+    assert err <= 1e-8, err
+
+
+def test_elasticity_solve(grmp, O):
+    N = 20
+    grid = grmp.grid_unitsquare(N)
+    fes = grmp.FESpace(grmp.H1P1(2), grid)
+    kap = grmp.DeviceVector(grid.ctx, grid.ncells)
+    kap.lognormal(5, 0.0, 1.0)
+    pde = grmp.LinearElasticityProblem(1.0, 0.3, grmp.CellCoefficient(kap))
+    pde.add_rhsdata(0, grmp.LinearForm(grmp.Identity, grmp.DataFunction([0.0, -1.0])))
+    pde.add_boundarydata(0, grmp.HomogeneousDirichletBoundary)
+    sol = grmp.FEVector([fes])
+    info, A, b = grmp.solve(sol, pde, precision="fp32_ir", rtol=1e-12)
+    S = O.Space(N, O.FE_P1, ncomp=2)
+    mu, lam = 1.0 / 2.6, 0.3 / (1.3 * 0.4)
+    ref = O.assemble_matrix(S.ndofs, [dict(kind=O.BLF_HOOKE2D, fe=O.FE_P1, ncomp=2, quadorder=0, factor=1.0,
+                                          coef=O.COEF_CELL, kappa=kap.download(), params=np.array([lam, mu]),
+                                          coords=S.coords, cells=S.cells, celldofs=S.celldofs)], fixed=S.fixed)
+    bb = np.zeros(S.ndofs)
+    O.assemble_lf_const(bb, O.FE_P1, 2, 1, [0.0, -1.0], S.coords, S.cells, S.celldofs)
+    xref = O.direct_solve(ref, O.penalized_rhs(bb, S.fixed))
+    err = np.linalg.norm(sol.entries() - xref) / np.linalg.norm(xref)
+    assert info.status == 0 and err <= 1e-8, err
+
+
+def test_nonconvergence_reported(grmp, O):
+    """An unreachable tolerance with few iterations returns GRMP_ERR_NOCONV
+    (the reference warns, src/solvers.jl:1398-1400)."""
+    grid, fes, pde, xref = _poisson(grmp, O, 16, 2, 1)
+    sol = grmp.FEVector([fes])
+    with pytest.raises(grmp.GrmpError) as ei:
+        grmp.solve(sol, pde, precision="fp64", rtol=1e-30, maxiterations=5)
+    assert ei.value.code == grmp._abi.ERR_NOCONV
