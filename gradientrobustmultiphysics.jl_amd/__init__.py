@@ -17,4 +17,7 @@ from .pde import (APT_BilinearForm, APT_SymmetricBilinearForm, BilinearForm, Cel
                   ReactionOperator, SolverConfig, SymmetricGradient, assemble, assemble_linearform, solve,
                   solve_system)
 
+from . import server  # noqa: F401
+from .server import Server  # noqa: F401
+
 __version__ = "0.1.0"

@@ -75,7 +75,12 @@ class Result(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("memory_budget", I64), ("max_active", ctypes.c_int),
-                ("graph", ctypes.c_int)]
+                ("graph", ctypes.c_int), ("profile", ctypes.c_int)]
+
+
+class KernelStats(ctypes.Structure):
+    _fields_ = [("spmv_seconds", ctypes.c_double), ("spmv_launches", I64), ("spmv_bytes", ctypes.c_double),
+                ("steps", I64), ("templates", I64), ("template_seconds", ctypes.c_double)]
 
 
 # name -> (restype, argtypes); every symbol include/grmp.h declares
@@ -121,12 +126,14 @@ SIGNATURES = {
     "grmp_integrate_identity": (ctypes.c_int, [VP, VP, ctypes.c_int, VP, I64, DP]),
     "grmp_solve_opts_default": (None, [ctypes.POINTER(SolveOpts)]),
     "grmp_solve": (ctypes.c_int, [VP, VP, VP, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
+    "grmp_solve_csc": (ctypes.c_int, [VP, I64, I64P, I64P, DP, DP, DP, I64, I64P, ctypes.c_int, ctypes.c_double]),
     "grmp_server_create": (ctypes.c_int, [VP, ctypes.POINTER(ServerConfig), ctypes.POINTER(VP)]),
     "grmp_server_submit": (ctypes.c_int, [VP, ctypes.POINTER(Request), I64P]),
     "grmp_server_poll": (ctypes.c_int, [VP, I64, ctypes.POINTER(Result)]),
     "grmp_server_wait": (ctypes.c_int, [VP, I64, ctypes.POINTER(Result)]),
     "grmp_server_clock": (ctypes.c_double, [VP]),
     "grmp_server_destroy": (ctypes.c_int, [VP]),
+    "grmp_server_stats": (ctypes.c_int, [VP, ctypes.POINTER(KernelStats)]),
     "grmp_request_footprint": (I64, [ctypes.POINTER(Request)]),
     "grmp_request_ndofs": (I64, [ctypes.POINTER(Request)]),
 }

@@ -817,6 +817,42 @@ extern "C" int grmp_matrix_assemble(grmp_matrix *A, int values)
     return GRMP_OK;
 }
 
+// Assembles the matrix values of A's pattern into the given buffers on
+// stream st, with kappa replacing the cell coefficient of every
+// GRMP_COEF_CELL form.  dforms: device array (>= forms.size()) owned by the
+// caller; hforms: host staging that must stay valid until st passes it.
+int assemble_values(const grmp_matrix *A, const double *kappa, DevForm *dforms, DevForm *hforms, double *val64,
+                    float *val32, double *dinv64, float *dinv32, hipStream_t st)
+{
+    grmp_ctx *ctx = A->ctx;
+    int first[9];
+    int k = 0;
+    for (int rb = 0; rb < A->nblocks; ++rb) {
+        first[rb] = k;
+        for (auto &r : A->forms)
+            if (r.f.row_block == rb) {
+                fill_devform(A, r, hforms[k]);
+                if (kappa && hforms[k].coef_cell) hforms[k].kappa = kappa;
+                ++k;
+            }
+    }
+    first[A->nblocks] = k;
+    GRMP_HIP(ctx, hipMemcpyAsync(dforms, hforms, sizeof(DevForm) * k, hipMemcpyHostToDevice, st));
+    const int B = asm_block_size(A->max_row);
+    for (int rb = 0; rb < A->nblocks; ++rb) {
+        FormPack P;
+        P.f = dforms + first[rb];
+        P.n = first[rb + 1] - first[rb];
+        int64_t g0 = A->offs[rb], g1 = A->offs[rb + 1];
+        if (g1 <= g0) continue;
+        size_t shm = (size_t)B * (A->max_row > 0 ? A->max_row : 1) * sizeof(double);
+        hipLaunchKernelGGL(k_assemble, dim3(nblk(g1 - g0, B)), dim3(B), shm, st, P, g0, g1, g0, A->rowptr,
+                           A->diagpos, A->fixed, A->penalty, val64, val32, dinv64, dinv32);
+        GRMP_HIP(ctx, hipGetLastError());
+    }
+    return GRMP_OK;
+}
+
 extern "C" double grmp_matrix_last_assembly_seconds(const grmp_matrix *A) { return A ? A->last_asm_ms * 1e-3 : -1.0; }
 
 extern "C" int grmp_matrix_info(const grmp_matrix *A, int64_t *n, int64_t *nnz, int *max_row)
@@ -914,17 +950,13 @@ __global__ void k_linearform(LfDev L, double *__restrict__ b, int64_t ndofs)
     b[i] = acc;
 }
 
-extern "C" int grmp_assemble_linearform(grmp_ctx *ctx, const grmp_space *s, const grmp_lform *form, grmp_vector *b,
-                                        int64_t offset)
+int linearform_launch(grmp_ctx *ctx, const grmp_space *s, const grmp_lform *form, double *bp, hipStream_t st)
 {
-    GRMP_CHECK(ctx, ctx && s && form && b, GRMP_ERR_ARG, "grmp_assemble_linearform: null argument");
-    GRMP_CHECK(ctx, offset >= 0 && offset + s->ndofs <= b->n, GRMP_ERR_ARG, "grmp_assemble_linearform: vector too short");
     const grmp_mesh *m = s->mesh;
     int nq = quad_nq_tri(form->quadorder);
     GRMP_CHECK(ctx, form->kind == GRMP_LF_CONST && m->dim == 2 && (s->fe == GRMP_FE_H1P1 || s->fe == GRMP_FE_H1P2) && nq,
-               GRMP_ERR_UNSUPPORTED, "grmp_assemble_linearform: LF kind %d on FE %d (dim %d, quadorder %d) not built",
-               form->kind, s->fe, m->dim, form->quadorder);
-    std::lock_guard<std::mutex> g(ctx->mu);
+               GRMP_ERR_UNSUPPORTED, "linear form kind %d on FE %d (dim %d, quadorder %d) not built", form->kind, s->fe,
+               m->dim, form->quadorder);
     LfDev L;
     memset(&L, 0, sizeof(L));
     L.fe = s->fe;
@@ -941,13 +973,21 @@ extern "C" int grmp_assemble_linearform(grmp_ctx *ctx, const grmp_space *s, cons
     L.inc_ptr = s->inc_ptr;
     L.inc = s->inc;
     dim3 grid(nblk(s->ndofs, 256)), blk(256);
-    double *bp = b->d + offset;
-    if (s->fe == GRMP_FE_H1P1 && nq == 1) hipLaunchKernelGGL((k_linearform<1, 1>), grid, blk, 0, ctx->stream, L, bp, s->ndofs);
-    else if (s->fe == GRMP_FE_H1P1) hipLaunchKernelGGL((k_linearform<1, 3>), grid, blk, 0, ctx->stream, L, bp, s->ndofs);
-    else if (nq == 1) hipLaunchKernelGGL((k_linearform<2, 1>), grid, blk, 0, ctx->stream, L, bp, s->ndofs);
-    else hipLaunchKernelGGL((k_linearform<2, 3>), grid, blk, 0, ctx->stream, L, bp, s->ndofs);
+    if (s->fe == GRMP_FE_H1P1 && nq == 1) hipLaunchKernelGGL((k_linearform<1, 1>), grid, blk, 0, st, L, bp, s->ndofs);
+    else if (s->fe == GRMP_FE_H1P1) hipLaunchKernelGGL((k_linearform<1, 3>), grid, blk, 0, st, L, bp, s->ndofs);
+    else if (nq == 1) hipLaunchKernelGGL((k_linearform<2, 1>), grid, blk, 0, st, L, bp, s->ndofs);
+    else hipLaunchKernelGGL((k_linearform<2, 3>), grid, blk, 0, st, L, bp, s->ndofs);
     GRMP_HIP(ctx, hipGetLastError());
     return GRMP_OK;
+}
+
+extern "C" int grmp_assemble_linearform(grmp_ctx *ctx, const grmp_space *s, const grmp_lform *form, grmp_vector *b,
+                                        int64_t offset)
+{
+    GRMP_CHECK(ctx, ctx && s && form && b, GRMP_ERR_ARG, "grmp_assemble_linearform: null argument");
+    GRMP_CHECK(ctx, offset >= 0 && offset + s->ndofs <= b->n, GRMP_ERR_ARG, "grmp_assemble_linearform: vector too short");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    return linearform_launch(ctx, s, form, b->d + offset, ctx->stream);
 }
 
 // --------------------------------------------------------- ItemIntegrator

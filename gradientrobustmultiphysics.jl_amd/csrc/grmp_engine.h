@@ -69,6 +69,10 @@ class KEngine {
     // reads phase/status/iterations of all slots (blocking)
     int poll(std::vector<KProb> &out);
     int64_t total_tiles() const { return ttot_; }
+    // SpMV kernel accounting: events around every engine SpMV launch when
+    // profiling, and a device counter of algorithmic bytes (always on)
+    void set_profile(bool on) { profile_ = on; }
+    int stats(double *seconds, int64_t *launches, double *bytes, int64_t *steps);
 
   private:
     int capture(int grid);
@@ -92,6 +96,13 @@ class KEngine {
     int graph_grid_ = 0;
     hipGraphExec_t exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
+    bool profile_ = false;
+    bool graph_profile_ = false;
+    hipEvent_t ev_[2 * 8] = {};
+    unsigned long long *dbytes_ = nullptr;
+    double spmv_s_ = 0.0;
+    int64_t spmv_n_ = 0, steps_ = 0;
+    int read_events(int pairs);
 };
 
 int engine_pick_lanes(double nnz_per_row);

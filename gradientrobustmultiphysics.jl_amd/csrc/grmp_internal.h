@@ -166,10 +166,20 @@ int exclusive_scan_i32(grmp_ctx *ctx, const int32_t *in, int32_t *out, int64_t n
 int form_resolve(grmp_matrix *A, FormRec &r);
 void fill_devform(const grmp_matrix *A, const FormRec &r, DevForm &d);
 
+// assembly internals shared with the server
+int assemble_values(const grmp_matrix *A, const double *kappa, DevForm *dforms, DevForm *hforms, double *val64,
+                    float *val32, double *dinv64, float *dinv32, hipStream_t st);
+int linearform_launch(grmp_ctx *ctx, const grmp_space *s, const grmp_lform *form, double *bp, hipStream_t st);
+
 // solver internals shared with the server
 struct ProblemDesc;
 int spmv_launch(grmp_ctx *ctx, const grmp_matrix *A, int values, const double *x, double *y,
                 hipStream_t st);
+// x = b/A_dd on fixed rows, 0 elsewhere
+int init_x_launch(const grmp_matrix *A, const double *b, double *x, hipStream_t st);
+// b[i] = 0 on fixed rows (homogeneous Dirichlet, penalty * 0)
+int zero_fixed_launch(const grmp_matrix *A, double *b, hipStream_t st);
+int plain_cg_solve(grmp_matrix *A, const double *b, double *x, const grmp_solve_opts &o, grmp_solve_info &info);
 
 // timing
 double wall_seconds();

@@ -1,19 +1,592 @@
-// Request server (continuous batching) -- filled in by the serving layer.
+// Request server of libgrmp_hip.so: continuous batching of many small and
+// large PDE solves on one GPU.
+//
+// A request names a problem (Poisson or P1 elasticity on an N x N unit
+// square, P1 or P2, seeded log-normal cell coefficient).  Served mode:
+//   * admission: a request is admitted when a problem slot is free and its
+//     HBM footprint fits the memory budget; otherwise it waits in FIFO order;
+//   * resident state: mesh, dofmap and sparsity pattern (with the element-
+//     to-slot map) are built once per problem shape and kept in HBM (LRU);
+//     per request only the coefficient, matrix values, right-hand side and
+//     Krylov vectors are allocated;
+//   * overlap: assembly of newly admitted requests runs on its own stream
+//     while the Krylov engine steps the problems already in flight on
+//     another; an assembled problem joins the engine at the next step;
+//   * the engine runs FP32 CG sweeps inside FP64 iterative refinement for all
+//     problems in flight at once (block-diagonal batch), replayed as a
+//     hipGraph.
+// Plain mode (the baseline): requests are served one at a time, each built
+// from scratch and solved by the host-driven FP64 PCG.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <list>
+#include <map>
+#include <thread>
+#include <tuple>
+#include <unordered_map>
+
 #include "grmp_engine.h"
 
-struct grmp_server {
-    grmp_ctx *ctx;
+static const int64_t kTileRows = 2048;
+
+static inline int64_t req_nnodes(const grmp_request &r) { return (int64_t)(r.N + 1) * (r.N + 1); }
+static inline int64_t req_ncells(const grmp_request &r) { return 2ll * r.N * r.N; }
+
+extern "C" int64_t grmp_request_ndofs(const grmp_request *r)
+{
+    if (!r || r->N < 1) return -1;
+    const int64_t N = r->N, nn = req_nnodes(*r);
+    if (r->kind == GRMP_REQ_ELASTICITY) return 2 * nn;
+    if (r->fe == GRMP_FE_H1P2) return nn + 3 * N * N + 2 * N;
+    return nn;
+}
+
+static int64_t req_nnz_bound(const grmp_request &r)
+{
+    const int64_t n = grmp_request_ndofs(&r);
+    if (r.kind == GRMP_REQ_ELASTICITY) return 14 * n;
+    if (r.fe == GRMP_FE_H1P2) return 12 * n;
+    return 7 * n;
+}
+
+// bytes of the per-request device buffer (see Job::layout)
+static inline size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
+extern "C" int64_t grmp_request_footprint(const grmp_request *r)
+{
+    if (!r || r->N < 1) return -1;
+    const int64_t n = grmp_request_ndofs(r), nnz = req_nnz_bound(*r), nc = req_ncells(*r);
+    size_t b = al(nc * 8) + al(nnz * 8) + al(nnz * 4) + al(n * 8) + al(n * 4) + 3 * al(n * 8) + al(4 * n * 4) +
+               al(sizeof(DevForm) * 4);
+    return (int64_t)b;
+}
+
+struct Template {
+    int kind = 0, fe = 0, N = 0;
+    grmp_mesh *mesh = nullptr;
+    grmp_space *space = nullptr;
+    grmp_matrix *A = nullptr;
+    grmp_vector *kappa0 = nullptr;
+    int lanes = 4;
+    int refs = 0;
+    int64_t bytes = 0;
+    uint64_t last_use = 0;
+    grmp_lform lf;
 };
 
-extern "C" int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *, grmp_server **out)
+struct Job {
+    int64_t ticket = 0;
+    grmp_request req;
+    grmp_result res;
+    int state = 0;  // 0 queued, 1 assembling, 2 solving, 3 done
+    Template *tpl = nullptr;
+    char *dbuf = nullptr;
+    size_t dbytes = 0;
+    double *kappa = nullptr, *val64 = nullptr, *dinv64 = nullptr, *b = nullptr, *x = nullptr, *r64 = nullptr;
+    float *val32 = nullptr, *dinv32 = nullptr, *f4 = nullptr;
+    DevForm *dforms = nullptr;
+    DevForm hforms[4];
+    hipEvent_t ev = nullptr;
+    int slot = -1;
+    int64_t footprint = 0;
+    int64_t n = 0;
+};
+
+struct grmp_server {
+    grmp_ctx *user = nullptr;
+    grmp_ctx *sctx = nullptr;  // server context; its stream assembles
+    hipStream_t st_solve = nullptr;
+    grmp_server_config cfg;
+    KEngine *eng = nullptr;
+    std::thread worker;
+    std::mutex mu;
+    std::condition_variable cv, cv_done;
+    std::deque<Job *> queue;
+    std::unordered_map<int64_t, Job *> jobs;
+    std::list<Job *> assembling, solving;
+    bool stop = false;
+    int64_t next_ticket = 1;
+    double t0 = 0.0;
+    int64_t mem_used = 0;
+    int64_t tpl_bytes = 0;
+    uint64_t use_clock = 0;
+    std::map<std::tuple<int, int, int>, Template *> templates;
+    int fatal = GRMP_OK;
+    std::string fatal_msg;
+    int64_t ntemplates = 0;
+    double template_s = 0.0;
+};
+
+static double now(const grmp_server *s) { return wall_seconds() - s->t0; }
+
+// ------------------------------------------------------------ templates
+static void template_free(Template *t)
 {
-    if (out) *out = nullptr;
-    return grmp_fail(ctx, GRMP_ERR_UNSUPPORTED, "server not built");
+    grmp_matrix_destroy(t->A);
+    grmp_vector_destroy(t->kappa0);
+    grmp_space_destroy(t->space);
+    grmp_mesh_destroy(t->mesh);
+    delete t;
 }
-extern "C" int grmp_server_submit(grmp_server *, const grmp_request *, int64_t *) { return GRMP_ERR_UNSUPPORTED; }
-extern "C" int grmp_server_poll(grmp_server *, int64_t, grmp_result *) { return GRMP_ERR_UNSUPPORTED; }
-extern "C" int grmp_server_wait(grmp_server *, int64_t, grmp_result *) { return GRMP_ERR_UNSUPPORTED; }
-extern "C" double grmp_server_clock(const grmp_server *) { return 0.0; }
-extern "C" int grmp_server_destroy(grmp_server *) { return GRMP_OK; }
-extern "C" int64_t grmp_request_footprint(const grmp_request *) { return -1; }
-extern "C" int64_t grmp_request_ndofs(const grmp_request *) { return -1; }
+
+static int template_make(grmp_server *S, const grmp_request &r, Template **out)
+{
+    grmp_ctx *c = S->sctx;
+    Template *t = new Template();
+    t->kind = r.kind;
+    t->fe = r.fe;
+    t->N = r.N;
+    const int ncomp = r.kind == GRMP_REQ_ELASTICITY ? 2 : 1;
+    const int fe = r.kind == GRMP_REQ_ELASTICITY ? GRMP_FE_H1P1 : r.fe;
+    int rc;
+    if ((rc = grmp_mesh_unitsquare(c, r.N, &t->mesh)) || (rc = grmp_space_create(c, t->mesh, fe, ncomp, &t->space)) ||
+        (rc = grmp_vector_create(c, t->mesh->ncells, &t->kappa0)) ||
+        (rc = grmp_vector_lognormal(t->kappa0, r.seed, 0.0, r.sigma))) {
+        template_free(t);
+        return rc;
+    }
+    const grmp_space *sp[1] = {t->space};
+    if ((rc = grmp_matrix_create(c, 1, sp, &t->A))) { template_free(t); return rc; }
+    grmp_form f;
+    memset(&f, 0, sizeof(f));
+    f.coef_mode = GRMP_COEF_CELL;
+    f.cell_coef = t->kappa0;
+    f.factor = 1.0;
+    memset(&t->lf, 0, sizeof(t->lf));
+    t->lf.kind = GRMP_LF_CONST;
+    t->lf.factor = 1.0;
+    if (r.kind == GRMP_REQ_ELASTICITY) {
+        const double E = r.E > 0 ? r.E : 1.0, nu = r.nu > 0 ? r.nu : 0.3;
+        f.kind = GRMP_BLF_HOOKE2D;
+        f.quadorder = 0;
+        f.params[0] = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+        f.params[1] = E / (2.0 * (1.0 + nu));
+        t->lf.quadorder = 1;
+        t->lf.f[0] = 0.0;
+        t->lf.f[1] = -1.0;
+    } else {
+        f.kind = GRMP_BLF_LAPLACE;
+        f.quadorder = fe == GRMP_FE_H1P2 ? 2 : 0;
+        t->lf.quadorder = fe == GRMP_FE_H1P2 ? 2 : 1;
+        t->lf.f[0] = 1.0;
+    }
+    int64_t nb = 0;
+    if ((rc = grmp_matrix_add_form(t->A, &f)) || (rc = grmp_space_boundary_dofs(t->space, &nb, nullptr))) {
+        template_free(t);
+        return rc;
+    }
+    std::vector<int64_t> bd(nb);
+    if ((rc = grmp_space_boundary_dofs(t->space, &nb, bd.data())) || (rc = grmp_matrix_set_fixed(t->A, nb, bd.data(), 1e60)) ||
+        (rc = grmp_matrix_build_pattern(t->A))) {
+        template_free(t);
+        return rc;
+    }
+    t->lanes = engine_pick_lanes((double)t->A->nnz / (double)t->A->n);
+    const int64_t nc = t->mesh->ncells, nn = t->mesh->nnodes, n = t->A->n;
+    t->bytes = nn * 16 + nc * 12 + nc * t->space->nloc * 8 + n * 4 + t->A->nnz * 4 + n * 6 + nc * 8 +
+               nc * t->space->nloc * 4 * ((t->space->nloc + 3) / 4);
+    *out = t;
+    return GRMP_OK;
+}
+
+static void templates_trim(grmp_server *S, int64_t limit)
+{
+    while (S->tpl_bytes > limit) {
+        Template *victim = nullptr;
+        for (auto &kv : S->templates)
+            if (kv.second->refs == 0 && (!victim || kv.second->last_use < victim->last_use)) victim = kv.second;
+        if (!victim) return;
+        S->templates.erase(std::make_tuple(victim->kind, victim->fe, victim->N));
+        S->tpl_bytes -= victim->bytes;
+        template_free(victim);
+    }
+}
+
+static int template_get(grmp_server *S, const grmp_request &r, Template **out)
+{
+    auto key = std::make_tuple(r.kind, r.kind == GRMP_REQ_ELASTICITY ? GRMP_FE_H1P1 : r.fe, r.N);
+    auto it = S->templates.find(key);
+    if (it == S->templates.end()) {
+        Template *t = nullptr;
+        double t0 = wall_seconds();
+        GRMP_TRY(template_make(S, r, &t));
+        S->template_s += wall_seconds() - t0;
+        S->ntemplates++;
+        it = S->templates.emplace(key, t).first;
+        S->tpl_bytes += t->bytes;
+    }
+    it->second->last_use = ++S->use_clock;
+    *out = it->second;
+    return GRMP_OK;
+}
+
+// ----------------------------------------------------------------- jobs
+static int job_start(grmp_server *S, Job *j)
+{
+    grmp_ctx *c = S->sctx;
+    hipStream_t st = c->stream;
+    j->res.t_admit = now(S);
+    GRMP_TRY(template_get(S, j->req, &j->tpl));
+    Template *t = j->tpl;
+    t->refs++;
+    const grmp_matrix *A = t->A;
+    const int64_t n = A->n, nnz = A->nnz, nc = t->mesh->ncells;
+    j->n = n;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += al(bytes); return o; };
+    size_t o_k = take(nc * 8), o_v64 = take(nnz * 8), o_v32 = take(nnz * 4), o_d64 = take(n * 8), o_d32 = take(n * 4);
+    size_t o_b = take(n * 8), o_x = take(n * 8), o_r = take(n * 8), o_f = take(4 * n * 4), o_df = take(sizeof(DevForm) * 4);
+    j->dbytes = off;
+    GRMP_HIP(c, hipMallocAsync((void **)&j->dbuf, off, st));
+    j->kappa = (double *)(j->dbuf + o_k);
+    j->val64 = (double *)(j->dbuf + o_v64);
+    j->val32 = (float *)(j->dbuf + o_v32);
+    j->dinv64 = (double *)(j->dbuf + o_d64);
+    j->dinv32 = (float *)(j->dbuf + o_d32);
+    j->b = (double *)(j->dbuf + o_b);
+    j->x = (double *)(j->dbuf + o_x);
+    j->r64 = (double *)(j->dbuf + o_r);
+    j->f4 = (float *)(j->dbuf + o_f);
+    j->dforms = (DevForm *)(j->dbuf + o_df);
+    // coefficient field, matrix values (FP64 + FP32 + Jacobi), rhs, x0
+    grmp_vector kv{c, nc, j->kappa};
+    GRMP_TRY(grmp_vector_lognormal(&kv, j->req.seed, 0.0, j->req.sigma));
+    GRMP_TRY(assemble_values(A, j->kappa, j->dforms, j->hforms, j->val64, j->val32, j->dinv64, j->dinv32, st));
+    GRMP_HIP(c, hipMemsetAsync(j->b, 0, n * 8, st));
+    GRMP_TRY(linearform_launch(c, t->space, &t->lf, j->b, st));
+    GRMP_TRY(zero_fixed_launch(A, j->b, st));
+    GRMP_TRY(init_x_launch(A, j->b, j->x, st));
+    GRMP_HIP(c, hipEventCreateWithFlags(&j->ev, hipEventDisableTiming));
+    GRMP_HIP(c, hipEventRecord(j->ev, st));
+    j->state = 1;
+    return GRMP_OK;
+}
+
+static int job_activate(grmp_server *S, Job *j)
+{
+    const grmp_matrix *A = j->tpl->A;
+    KProb p;
+    memset(&p, 0, sizeof(p));
+    p.rowptr = A->rowptr;
+    p.cols = A->cols;
+    p.val64 = j->val64;
+    p.val32 = j->val32;
+    p.dinv32 = j->dinv32;
+    p.fixed = A->fixed;
+    p.b = j->b;
+    p.x = j->x;
+    p.r64 = j->r64;
+    p.r = j->f4;
+    p.p = j->f4 + j->n;
+    p.q = j->f4 + 2 * j->n;
+    p.dx = j->f4 + 3 * j->n;
+    p.n = (int32_t)j->n;
+    p.lanes = j->tpl->lanes;
+    p.max_outer = 60;
+    p.inner_maxit = 1 << 30;
+    p.maxit = 4000000;
+    p.rtol = j->req.rtol > 0 ? j->req.rtol : 1e-10;
+    p.inner_rtol = 1e-4;
+    GRMP_TRY(S->eng->add(p, &j->slot));
+    j->state = 2;
+    return GRMP_OK;
+}
+
+static void job_release(grmp_server *S, Job *j)
+{
+    if (j->dbuf) (void)hipFreeAsync(j->dbuf, S->st_solve);
+    j->dbuf = nullptr;
+    if (j->ev) (void)hipEventDestroy(j->ev);
+    j->ev = nullptr;
+    if (j->tpl) j->tpl->refs--;
+    S->mem_used -= j->footprint;
+}
+
+static void job_finish(grmp_server *S, Job *j, const KProb &q)
+{
+    j->res.status = q.phase == PH_DONE ? GRMP_OK : (q.status ? q.status : GRMP_ERR_NOCONV);
+    j->res.iterations = q.iters;
+    j->res.outer = q.outer;
+    j->res.relres = q.relres;
+    j->res.ndofs = j->n;
+    if (j->req.x_out) {
+        (void)hipMemcpyAsync(j->req.x_out, j->x, j->n * 8, hipMemcpyDeviceToHost, S->st_solve);
+        (void)hipStreamSynchronize(S->st_solve);
+    }
+    S->eng->remove(j->slot);
+    job_release(S, j);
+    j->res.t_done = now(S);
+}
+
+static void job_fail(grmp_server *S, Job *j, int rc)
+{
+    j->res.status = rc;
+    j->res.t_done = now(S);
+    job_release(S, j);
+    j->state = 3;
+}
+
+// ------------------------------------------------------------ served loop
+static void served_loop(grmp_server *S)
+{
+    std::vector<KProb> st;
+    const int steps = 16;
+    for (;;) {
+        std::vector<Job *> admit;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            for (;;) {
+                if (S->queue.empty()) break;
+                Job *j = S->queue.front();
+                const int inflight = (int)(S->assembling.size() + S->solving.size() + admit.size());
+                if (inflight >= S->cfg.max_active) break;
+                if (inflight > 0 && S->mem_used + j->footprint > S->cfg.memory_budget) break;
+                S->queue.pop_front();
+                S->mem_used += j->footprint;
+                admit.push_back(j);
+            }
+            if (admit.empty() && S->assembling.empty() && S->solving.empty()) {
+                if (S->stop) return;
+                S->cv.wait(lk, [&] { return S->stop || !S->queue.empty(); });
+                continue;
+            }
+        }
+        for (Job *j : admit) {
+            int rc = job_start(S, j);
+            std::lock_guard<std::mutex> g(S->mu);
+            if (rc) {
+                job_fail(S, j, rc);
+                S->cv_done.notify_all();
+            } else {
+                S->assembling.push_back(j);
+            }
+        }
+        // assembled problems join the engine
+        for (auto it = S->assembling.begin(); it != S->assembling.end();) {
+            Job *j = *it;
+            if (hipEventQuery(j->ev) != hipSuccess) { ++it; continue; }
+            int rc = job_activate(S, j);
+            std::lock_guard<std::mutex> g(S->mu);
+            it = S->assembling.erase(it);
+            if (rc) {
+                job_fail(S, j, rc);
+                S->cv_done.notify_all();
+            } else {
+                S->solving.push_back(j);
+            }
+        }
+        if (S->eng->nactive() == 0) {
+            if (!S->assembling.empty()) (void)hipEventSynchronize(S->assembling.front()->ev);
+            continue;
+        }
+        int rc = S->eng->run(steps, S->cfg.graph != 0);
+        if (!rc) rc = S->eng->poll(st);
+        if (rc) {
+            std::lock_guard<std::mutex> g(S->mu);
+            S->fatal = rc;
+            S->fatal_msg = S->sctx->err;
+            for (Job *j : S->solving) job_fail(S, j, rc);
+            S->solving.clear();
+            S->cv_done.notify_all();
+            continue;
+        }
+        for (auto it = S->solving.begin(); it != S->solving.end();) {
+            Job *j = *it;
+            const KProb &q = st[j->slot];
+            if (q.phase != PH_DONE && q.phase != PH_FAIL) { ++it; continue; }
+            job_finish(S, j, q);
+            std::lock_guard<std::mutex> g(S->mu);
+            j->state = 3;
+            it = S->solving.erase(it);
+            S->cv_done.notify_all();
+        }
+        {
+            std::lock_guard<std::mutex> g(S->mu);
+            templates_trim(S, S->cfg.memory_budget / 2);
+        }
+    }
+}
+
+// ------------------------------------------------------------- plain loop
+static int plain_one(grmp_server *S, Job *j)
+{
+    grmp_ctx *c = S->sctx;
+    j->res.t_admit = now(S);
+    Template *t = nullptr;
+    GRMP_TRY(template_make(S, j->req, &t));
+    // the pattern above used this request's own coefficient field (same seed)
+    grmp_matrix *A = t->A;
+    int rc = grmp_matrix_assemble(A, GRMP_VALUES_FP64);
+    grmp_vector *b = nullptr, *x = nullptr;
+    if (!rc) rc = grmp_vector_create(c, A->n, &b);
+    if (!rc) rc = grmp_vector_create(c, A->n, &x);
+    if (!rc) rc = linearform_launch(c, t->space, &t->lf, b->d, c->stream);
+    if (!rc) rc = zero_fixed_launch(A, b->d, c->stream);
+    grmp_solve_opts o;
+    grmp_solve_opts_default(&o);
+    o.rtol = j->req.rtol > 0 ? j->req.rtol : 1e-10;
+    grmp_solve_info info;
+    memset(&info, 0, sizeof(info));
+    if (!rc) rc = plain_cg_solve(A, b->d, x->d, o, info);
+    if (!rc && j->req.x_out) rc = grmp_vector_download(x, j->req.x_out);
+    j->res.status = rc ? rc : info.status;
+    j->res.iterations = info.iterations;
+    j->res.outer = 1;
+    j->res.relres = info.relres;
+    j->res.ndofs = A->n;
+    grmp_vector_destroy(b);
+    grmp_vector_destroy(x);
+    template_free(t);
+    (void)hipStreamSynchronize(c->stream);
+    j->res.t_done = now(S);
+    return GRMP_OK;
+}
+
+static void plain_loop(grmp_server *S)
+{
+    for (;;) {
+        Job *j = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv.wait(lk, [&] { return S->stop || !S->queue.empty(); });
+            if (S->queue.empty()) return;
+            j = S->queue.front();
+            S->queue.pop_front();
+        }
+        int rc = plain_one(S, j);
+        std::lock_guard<std::mutex> g(S->mu);
+        if (rc) j->res.status = rc;
+        j->state = 3;
+        S->cv_done.notify_all();
+    }
+}
+
+// ------------------------------------------------------------------ API
+extern "C" int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *cfg, grmp_server **out)
+{
+    GRMP_CHECK(ctx, ctx && out, GRMP_ERR_ARG, "grmp_server_create: null argument");
+    grmp_server *S = new grmp_server();
+    S->user = ctx;
+    S->cfg.mode = 1;
+    S->cfg.memory_budget = 64ll << 30;
+    S->cfg.max_active = 256;
+    S->cfg.graph = 1;
+    S->cfg.profile = 0;
+    if (cfg) {
+        if (cfg->mode) S->cfg.mode = cfg->mode;
+        if (cfg->memory_budget > 0) S->cfg.memory_budget = cfg->memory_budget;
+        if (cfg->max_active > 0) S->cfg.max_active = std::min(cfg->max_active, 4096);
+        S->cfg.graph = cfg->graph;
+        S->cfg.profile = cfg->profile;
+    }
+    int rc = grmp_ctx_create(ctx->device, &S->sctx);
+    if (rc) { delete S; return grmp_fail(ctx, rc, "grmp_server_create: no server context"); }
+    if (hipStreamCreateWithFlags(&S->st_solve, hipStreamNonBlocking) != hipSuccess) {
+        grmp_ctx_destroy(S->sctx);
+        delete S;
+        return grmp_fail(ctx, GRMP_ERR_HIP, "grmp_server_create: stream");
+    }
+    if (S->cfg.mode == 1) {
+        S->eng = new KEngine(S->sctx, S->st_solve, S->cfg.max_active, 1 << 18, (int)kTileRows);
+        S->eng->set_profile(S->cfg.profile != 0);
+        if ((rc = S->eng->init())) {
+            delete S->eng;
+            (void)hipStreamDestroy(S->st_solve);
+            grmp_ctx_destroy(S->sctx);
+            delete S;
+            return grmp_fail(ctx, rc, "grmp_server_create: engine init failed");
+        }
+    }
+    S->t0 = wall_seconds();
+    S->worker = std::thread([S] {
+        (void)hipSetDevice(S->sctx->device);
+        if (S->cfg.mode == 1) served_loop(S);
+        else plain_loop(S);
+    });
+    *out = S;
+    return GRMP_OK;
+}
+
+extern "C" int grmp_server_submit(grmp_server *S, const grmp_request *req, int64_t *ticket)
+{
+    if (!S || !req || !ticket) return GRMP_ERR_ARG;
+    GRMP_CHECK(S->user, req->N >= 1 && req->N <= 4096 && (req->kind == GRMP_REQ_POISSON || req->kind == GRMP_REQ_ELASTICITY) &&
+                            (req->fe == GRMP_FE_H1P1 || req->fe == GRMP_FE_H1P2),
+               GRMP_ERR_ARG, "grmp_server_submit: malformed request");
+    Job *j = new Job();
+    j->req = *req;
+    memset(&j->res, 0, sizeof(j->res));
+    j->footprint = grmp_request_footprint(req);
+    if (S->cfg.mode == 1 && j->footprint > S->cfg.memory_budget) {
+        delete j;
+        return grmp_fail(S->user, GRMP_ERR_ADMISSION, "grmp_server_submit: footprint %lld B exceeds the budget %lld B",
+                         (long long)grmp_request_footprint(req), (long long)S->cfg.memory_budget);
+    }
+    std::lock_guard<std::mutex> g(S->mu);
+    j->ticket = S->next_ticket++;
+    j->res.t_submit = now(S);
+    S->jobs[j->ticket] = j;
+    S->queue.push_back(j);
+    *ticket = j->ticket;
+    S->cv.notify_all();
+    return GRMP_OK;
+}
+
+extern "C" int grmp_server_poll(grmp_server *S, int64_t ticket, grmp_result *res)
+{
+    if (!S) return GRMP_ERR_ARG;
+    std::lock_guard<std::mutex> g(S->mu);
+    auto it = S->jobs.find(ticket);
+    if (it == S->jobs.end()) return GRMP_ERR_ARG;
+    if (it->second->state != 3) return GRMP_ERR_BUSY;
+    if (res) *res = it->second->res;
+    return GRMP_OK;
+}
+
+extern "C" int grmp_server_wait(grmp_server *S, int64_t ticket, grmp_result *res)
+{
+    if (!S) return GRMP_ERR_ARG;
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->jobs.find(ticket);
+    if (it == S->jobs.end()) return GRMP_ERR_ARG;
+    Job *j = it->second;
+    S->cv_done.wait(lk, [&] { return j->state == 3; });
+    if (res) *res = j->res;
+    return GRMP_OK;
+}
+
+extern "C" double grmp_server_clock(const grmp_server *S) { return S ? now(S) : 0.0; }
+
+extern "C" int grmp_server_stats(grmp_server *S, grmp_kernel_stats *out)
+{
+    if (!S || !out) return GRMP_ERR_ARG;
+    memset(out, 0, sizeof(*out));
+    std::lock_guard<std::mutex> g(S->mu);
+    out->templates = S->ntemplates;
+    out->template_seconds = S->template_s;
+    if (!S->eng) return GRMP_OK;
+    return S->eng->stats(&out->spmv_seconds, &out->spmv_launches, &out->spmv_bytes, &out->steps);
+}
+
+extern "C" int grmp_server_destroy(grmp_server *S)
+{
+    if (!S) return GRMP_OK;
+    {
+        std::lock_guard<std::mutex> g(S->mu);
+        S->stop = true;
+        S->cv.notify_all();
+    }
+    if (S->worker.joinable()) S->worker.join();
+    (void)hipStreamSynchronize(S->st_solve);
+    delete S->eng;
+    for (auto &kv : S->templates) template_free(kv.second);
+    for (auto &kv : S->jobs) delete kv.second;
+    (void)hipStreamDestroy(S->st_solve);
+    grmp_ctx_destroy(S->sctx);
+    delete S;
+    return GRMP_OK;
+}

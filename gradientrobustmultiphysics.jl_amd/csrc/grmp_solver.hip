@@ -244,7 +244,7 @@ static int plain_sum(grmp_ctx *ctx, const double *dpart, double *hpart, int grid
     return GRMP_OK;
 }
 
-static int plain_cg(grmp_matrix *A, const grmp_vector *bv, grmp_vector *xv, const grmp_solve_opts &o, grmp_solve_info &info)
+int plain_cg_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solve_opts &o, grmp_solve_info &info)
 {
     grmp_ctx *ctx = A->ctx;
     const int64_t n = A->n;
@@ -260,7 +260,7 @@ static int plain_cg(grmp_matrix *A, const grmp_vector *bv, grmp_vector *xv, cons
     const int grid = (int)std::min<int64_t>(PGRID, std::max<int64_t>(1, (n * L + PB - 1) / PB));
     const int gride = (int)std::min<int64_t>(PGRID, std::max<int64_t>(1, (n + PB - 1) / PB));
     hipLaunchKernelGGL(k_plain_init, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, A->rowptr, A->diagpos, A->val64, A->fixed,
-                       bv->d, xv->d, n);
+                       bd, xd, n);
 #define PL_CASE(LL, KERNEL, ...) \
     case LL: hipLaunchKernelGGL((KERNEL<LL>), dim3(grid), dim3(PB), 0, ctx->stream, __VA_ARGS__); break;
 #define PL_SWITCH(KERNEL, ...)                 \
@@ -271,7 +271,7 @@ static int plain_cg(grmp_matrix *A, const grmp_vector *bv, grmp_vector *xv, cons
         PL_CASE(16, KERNEL, __VA_ARGS__)       \
     default: break;                            \
     }
-    PL_SWITCH(k_plain_resid, A->rowptr, A->cols, A->val64, A->fixed, A->dinv64, bv->d, xv->d, r, z, p, n, part);
+    PL_SWITCH(k_plain_resid, A->rowptr, A->cols, A->val64, A->fixed, A->dinv64, bd, xd, r, z, p, n, part);
     double rz, rr, pq, dummy;
     int rc = plain_sum(ctx, part, hpart, grid, rz, rr);
     double bb = rr;
@@ -285,7 +285,7 @@ static int plain_cg(grmp_matrix *A, const grmp_vector *bv, grmp_vector *xv, cons
         PL_SWITCH(k_plain_spmv, A->rowptr, A->cols, A->val64, A->fixed, p, q, n, part);
         if ((rc = plain_sum(ctx, part, hpart, grid, pq, dummy))) break;
         double alpha = rz / pq;
-        hipLaunchKernelGGL(k_plain_axpy, dim3(gride), dim3(PB), 0, ctx->stream, alpha, p, q, A->dinv64, xv->d, r, z, n, part);
+        hipLaunchKernelGGL(k_plain_axpy, dim3(gride), dim3(PB), 0, ctx->stream, alpha, p, q, A->dinv64, xd, r, z, n, part);
         double rz2;
         if ((rc = plain_sum(ctx, part, hpart, gride, rz2, rr))) break;
         double beta = rz2 / rz;
@@ -308,6 +308,27 @@ static int plain_cg(grmp_matrix *A, const grmp_vector *bv, grmp_vector *xv, cons
     dfree(ctx, q);
     dfree(ctx, part);
     return rc;
+}
+
+__global__ void k_zero_fixed(const uint8_t *fixed, double *b, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && fixed[i]) b[i] = 0.0;
+}
+
+int init_x_launch(const grmp_matrix *A, const double *b, double *x, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_plain_init, dim3(nblk(A->n, 256)), dim3(256), 0, st, A->rowptr, A->diagpos, A->val64, A->fixed, b,
+                       x, A->n);
+    GRMP_HIP(A->ctx, hipGetLastError());
+    return GRMP_OK;
+}
+
+int zero_fixed_launch(const grmp_matrix *A, double *b, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_zero_fixed, dim3(nblk(A->n, 256)), dim3(256), 0, st, A->fixed, b, A->n);
+    GRMP_HIP(A->ctx, hipGetLastError());
+    return GRMP_OK;
 }
 
 // ============================================================ the engine
@@ -345,7 +366,7 @@ __device__ __forceinline__ void sum_parts(const KRange *R, const double *part, i
 }
 
 #define ENGINE_ARGS KProb *__restrict__ P, const KTile *__restrict__ T, const KRange *__restrict__ R, \
-                    double *__restrict__ part, const int *__restrict__ ntiles
+                    double *__restrict__ part, const int *__restrict__ ntiles, unsigned long long *__restrict__ nbytes
 
 // SpMV step of the inner CG (phase ITER): q = A32 p, alpha = rz / p.q
 template <int L>
@@ -379,6 +400,12 @@ __global__ __launch_bounds__(KB) void k_eng_spmv(ENGINE_ARGS)
         default: acc = spmv_tile<16>(pr, tl.row0, tl.row1); break;
         }
         block_sum2(acc, dummy);
+        if (threadIdx.x == 0) {
+            // algorithmic bytes of this tile: values + columns, row pointers, x gather, y write
+            const unsigned long long rows = (unsigned long long)(tl.row1 - tl.row0);
+            const unsigned long long nz = (unsigned long long)(pr.rowptr[tl.row1] - pr.rowptr[tl.row0]);
+            atomicAdd(nbytes, nz * 8ull + 4ull * (rows + 1) + 8ull * rows);
+        }
         if (finish_tile(P, R, part, t, pi, acc, dummy)) {
             double pq, d2;
             sum_parts(R, part, pi, pq, d2);
@@ -565,6 +592,9 @@ KEngine::~KEngine()
     if (dR_) (void)hipFree(dR_);
     if (dpart_) (void)hipFree(dpart_);
     if (dnt_) (void)hipFree(dnt_);
+    if (dbytes_) (void)hipFree(dbytes_);
+    for (auto &e : ev_)
+        if (e) (void)hipEventDestroy(e);
     if (hP_) (void)hipHostFree(hP_);
     if (hT_) (void)hipHostFree(hT_);
     if (hR_) (void)hipHostFree(hR_);
@@ -578,6 +608,9 @@ int KEngine::init()
     GRMP_HIP(ctx, hipMalloc(&dR_, sizeof(KRange) * cap_p_));
     GRMP_HIP(ctx, hipMalloc(&dpart_, sizeof(double) * 2 * cap_t_));
     GRMP_HIP(ctx, hipMalloc(&dnt_, sizeof(int)));
+    GRMP_HIP(ctx, hipMalloc(&dbytes_, sizeof(unsigned long long)));
+    GRMP_HIP(ctx, hipMemsetAsync(dbytes_, 0, sizeof(unsigned long long), st_));
+    for (auto &e : ev_) GRMP_HIP(ctx, hipEventCreate(&e));
     GRMP_HIP(ctx, hipHostMalloc((void **)&hP_, sizeof(KProb) * cap_p_, hipHostMallocDefault));
     GRMP_HIP(ctx, hipHostMalloc((void **)&hT_, sizeof(KTile) * cap_t_, hipHostMallocDefault));
     GRMP_HIP(ctx, hipHostMalloc((void **)&hR_, sizeof(KRange) * cap_p_, hipHostMallocDefault));
@@ -664,11 +697,39 @@ int KEngine::sync_tables()
 int KEngine::launch_step(int grid)
 {
     dim3 g(grid), b(KB);
-    hipLaunchKernelGGL(k_eng_spmv, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_);
-    hipLaunchKernelGGL(k_eng_axpy, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_);
-    hipLaunchKernelGGL(k_eng_elem, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_);
-    hipLaunchKernelGGL(k_eng_resid, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_);
+    if (profile_) GRMP_HIP(ctx_, hipEventRecord(ev_[0], st_));
+    hipLaunchKernelGGL(k_eng_spmv, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
+    if (profile_) GRMP_HIP(ctx_, hipEventRecord(ev_[1], st_));
+    hipLaunchKernelGGL(k_eng_axpy, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
+    hipLaunchKernelGGL(k_eng_elem, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
+    hipLaunchKernelGGL(k_eng_resid, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
     GRMP_HIP(ctx_, hipGetLastError());
+    ++steps_;
+    if (profile_) GRMP_TRY(read_events(1));
+    return GRMP_OK;
+}
+
+int KEngine::read_events(int pairs)
+{
+    GRMP_HIP(ctx_, hipEventSynchronize(ev_[2 * pairs - 1]));
+    for (int k = 0; k < pairs; ++k) {
+        float ms = 0.f;
+        GRMP_HIP(ctx_, hipEventElapsedTime(&ms, ev_[2 * k], ev_[2 * k + 1]));
+        spmv_s_ += ms * 1e-3;
+        ++spmv_n_;
+    }
+    return GRMP_OK;
+}
+
+int KEngine::stats(double *seconds, int64_t *launches, double *bytes, int64_t *steps)
+{
+    unsigned long long b = 0;
+    GRMP_HIP(ctx_, hipMemcpyAsync(&b, dbytes_, sizeof(b), hipMemcpyDeviceToHost, st_));
+    GRMP_HIP(ctx_, hipStreamSynchronize(st_));
+    if (seconds) *seconds = spmv_s_;
+    if (launches) *launches = spmv_n_;
+    if (bytes) *bytes = (double)b;
+    if (steps) *steps = steps_;
     return GRMP_OK;
 }
 
@@ -682,14 +743,17 @@ int KEngine::capture(int grid)
     GRMP_HIP(ctx, hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
     for (int s = 0; s < ENGINE_GRAPH_STEPS; ++s) {
         dim3 g(grid), b(KB);
-        hipLaunchKernelGGL(k_eng_spmv, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_);
-        hipLaunchKernelGGL(k_eng_axpy, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_);
-        hipLaunchKernelGGL(k_eng_elem, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_);
-        hipLaunchKernelGGL(k_eng_resid, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_);
+        if (profile_) GRMP_HIP(ctx, hipEventRecord(ev_[2 * s], st_));
+        hipLaunchKernelGGL(k_eng_spmv, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
+        if (profile_) GRMP_HIP(ctx, hipEventRecord(ev_[2 * s + 1], st_));
+        hipLaunchKernelGGL(k_eng_axpy, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
+        hipLaunchKernelGGL(k_eng_elem, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
+        hipLaunchKernelGGL(k_eng_resid, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
     }
     GRMP_HIP(ctx, hipStreamEndCapture(st_, &graph_));
     GRMP_HIP(ctx, hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
     graph_grid_ = grid;
+    graph_profile_ = profile_;
     return GRMP_OK;
 }
 
@@ -700,8 +764,12 @@ int KEngine::run(int steps, bool use_graph)
     int grid = 64;
     while (grid < ttot_ && grid < 4096) grid <<= 1;
     if (use_graph) {
-        if (!exec_ || graph_grid_ != grid) GRMP_TRY(capture(grid));
-        for (int s = 0; s < steps; s += ENGINE_GRAPH_STEPS) GRMP_HIP(ctx_, hipGraphLaunch(exec_, st_));
+        if (!exec_ || graph_grid_ != grid || graph_profile_ != profile_) GRMP_TRY(capture(grid));
+        for (int s = 0; s < steps; s += ENGINE_GRAPH_STEPS) {
+            GRMP_HIP(ctx_, hipGraphLaunch(exec_, st_));
+            steps_ += ENGINE_GRAPH_STEPS;
+            if (profile_) GRMP_TRY(read_events(ENGINE_GRAPH_STEPS));
+        }
     } else {
         for (int s = 0; s < steps; ++s) GRMP_TRY(launch_step(grid));
     }
@@ -806,7 +874,7 @@ extern "C" int grmp_solve(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, 
     double t0 = wall_seconds();
     int rc;
     if (o.precision == GRMP_PRECISION_FP32_IR) rc = solve_engine(A, b, x, o, inf);
-    else rc = plain_cg(A, b, x, o, inf);
+    else rc = plain_cg_solve(A, b->d, x->d, o, inf);
     inf.seconds = wall_seconds() - t0;
     if (info) *info = inf;
     if (rc) return rc;
@@ -815,4 +883,104 @@ extern "C" int grmp_solve(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, 
                          inf.relres, (long long)inf.iterations);
     if (inf.status == GRMP_ERR_NONFINITE) return grmp_fail(ctx, GRMP_ERR_NONFINITE, "grmp_solve: non-finite values in the Krylov loop");
     return inf.status;
+}
+
+// ------------------------------------------------------- CSC plug-in solve
+extern "C" int grmp_solve_csc(grmp_ctx *ctx, int64_t n, const int64_t *colptr, const int64_t *rowval, const double *nzval,
+                              const double *b, double *x, int64_t nfixed, const int64_t *fixed, int precision, double rtol)
+{
+    GRMP_CHECK(ctx, ctx && n > 0 && n < (1ll << 31) && colptr && rowval && nzval && b && x, GRMP_ERR_ARG,
+               "grmp_solve_csc: bad arguments");
+    const int64_t nnz = colptr[n] - 1;
+    GRMP_CHECK(ctx, nnz >= 0 && nnz < (1ll << 31), GRMP_ERR_ARG, "grmp_solve_csc: nnz out of range");
+    // CSC (columns) -> CSR (rows), 0-based, rows sorted by column
+    std::vector<int32_t> rp(n + 1, 0), cl(nnz);
+    std::vector<double> v64(nnz);
+    for (int64_t k = 0; k < nnz; ++k) {
+        GRMP_CHECK(ctx, rowval[k] >= 1 && rowval[k] <= n, GRMP_ERR_ARG, "grmp_solve_csc: row index out of range");
+        rp[rowval[k]]++;
+    }
+    for (int64_t i = 0; i < n; ++i) rp[i + 1] += rp[i];
+    std::vector<int32_t> cur(rp.begin(), rp.end() - 1);
+    for (int64_t j = 0; j < n; ++j)
+        for (int64_t k = colptr[j] - 1; k < colptr[j + 1] - 1; ++k) {
+            int32_t p = cur[rowval[k] - 1]++;
+            cl[p] = (int32_t)j;
+            v64[p] = nzval[k];
+        }
+    std::vector<uint8_t> fx(n, 0), dpos(n, 255);
+    for (int64_t k = 0; k < nfixed; ++k) {
+        GRMP_CHECK(ctx, fixed[k] >= 1 && fixed[k] <= n, GRMP_ERR_ARG, "grmp_solve_csc: fixed dof out of range");
+        fx[fixed[k] - 1] = 1;
+    }
+    std::vector<double> di64(n, 0.0);
+    std::vector<float> di32(n, 0.f), v32(nnz);
+    for (int64_t i = 0; i < n; ++i) {
+        double d = 0.0;
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+            if (cl[k] == i) {
+                d = v64[k];
+                if (k - rp[i] < 255) dpos[i] = (uint8_t)(k - rp[i]);
+            }
+            v32[k] = fx[i] ? 0.f : (float)v64[k];
+        }
+        GRMP_CHECK(ctx, fx[i] == 0 || dpos[i] != 255, GRMP_ERR_ARG, "grmp_solve_csc: fixed dof without a diagonal entry");
+        di64[i] = (fx[i] || d == 0.0) ? 0.0 : 1.0 / d;
+        di32[i] = (float)di64[i];
+    }
+    grmp_matrix A;
+    A.ctx = ctx;
+    A.nblocks = 0;
+    A.n = n;
+    A.nnz = nnz;
+    A.have_pattern = true;
+    int rc = GRMP_OK;
+    grmp_vector bv{ctx, n, nullptr}, xv{ctx, n, nullptr};
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        GRMP_TRY(dalloc(ctx, &A.rowptr, n + 1));
+        GRMP_TRY(dalloc(ctx, &A.cols, nnz));
+        GRMP_TRY(dalloc(ctx, &A.val64, nnz));
+        GRMP_TRY(dalloc(ctx, &A.val32, nnz));
+        GRMP_TRY(dalloc(ctx, &A.dinv64, n));
+        GRMP_TRY(dalloc(ctx, &A.dinv32, n));
+        GRMP_TRY(dalloc(ctx, &A.diagpos, n));
+        GRMP_TRY(dalloc(ctx, &A.fixed, n));
+        GRMP_TRY(dalloc(ctx, &bv.d, n));
+        GRMP_TRY(dalloc(ctx, &xv.d, n));
+        hipStream_t st = ctx->stream;
+        GRMP_HIP(ctx, hipMemcpyAsync(A.rowptr, rp.data(), 4 * (n + 1), hipMemcpyHostToDevice, st));
+        GRMP_HIP(ctx, hipMemcpyAsync(A.cols, cl.data(), 4 * nnz, hipMemcpyHostToDevice, st));
+        GRMP_HIP(ctx, hipMemcpyAsync(A.val64, v64.data(), 8 * nnz, hipMemcpyHostToDevice, st));
+        GRMP_HIP(ctx, hipMemcpyAsync(A.val32, v32.data(), 4 * nnz, hipMemcpyHostToDevice, st));
+        GRMP_HIP(ctx, hipMemcpyAsync(A.dinv64, di64.data(), 8 * n, hipMemcpyHostToDevice, st));
+        GRMP_HIP(ctx, hipMemcpyAsync(A.dinv32, di32.data(), 4 * n, hipMemcpyHostToDevice, st));
+        GRMP_HIP(ctx, hipMemcpyAsync(A.diagpos, dpos.data(), n, hipMemcpyHostToDevice, st));
+        GRMP_HIP(ctx, hipMemcpyAsync(A.fixed, fx.data(), n, hipMemcpyHostToDevice, st));
+        GRMP_HIP(ctx, hipMemcpyAsync(bv.d, b, 8 * n, hipMemcpyHostToDevice, st));
+        GRMP_HIP(ctx, hipStreamSynchronize(st));
+    }
+    grmp_solve_opts o;
+    grmp_solve_opts_default(&o);
+    o.precision = precision == GRMP_PRECISION_FP32_IR ? GRMP_PRECISION_FP32_IR : GRMP_PRECISION_FP64;
+    o.rtol = rtol > 0 ? rtol : 1e-10;
+    grmp_solve_info info;
+    rc = grmp_solve(&A, &bv, &xv, &o, &info);
+    if (rc == GRMP_OK || rc == GRMP_ERR_NOCONV) {
+        int rc2 = grmp_vector_download(&xv, x);
+        if (rc == GRMP_OK) rc = rc2;
+    }
+    std::lock_guard<std::mutex> g(ctx->mu);
+    dfree(ctx, A.rowptr);
+    dfree(ctx, A.cols);
+    dfree(ctx, A.val64);
+    dfree(ctx, A.val32);
+    dfree(ctx, A.dinv64);
+    dfree(ctx, A.dinv32);
+    dfree(ctx, A.diagpos);
+    dfree(ctx, A.fixed);
+    dfree(ctx, bv.d);
+    dfree(ctx, xv.d);
+    (void)hipStreamSynchronize(ctx->stream);
+    return rc;
 }

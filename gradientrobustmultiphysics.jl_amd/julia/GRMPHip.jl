@@ -1,0 +1,194 @@
+#=
+GRMPHip.jl -- Julia binding of libgrmp_hip.so for GradientRobustMultiPhysics.jl.
+
+Loaded next to GradientRobustMultiPhysics, it
+  * adds `GRMPKrylov`, a LinearSolve algorithm that solve! accepts through its
+    `linsolver` keyword (src/solvers.jl:55, 89-101, 656-660), and
+  * adds `hip_assemble!` methods for the pattern-level assemble! of the
+    BilinearForm / LinearForm kinds the library builds
+    (src/assemblypatterns/bilinearform.jl:384-400, linearform.jl:239-251).
+
+Every call passes plain pointers: xgrid[Coordinates] ([dim, nnodes]),
+xgrid[CellNodes] ([nloc, ncells]) and FES[CellDofs] ([ndofs_loc, ncells]) in
+Julia's own 1-based column-major layout, which is what include/grmp.h expects.
+This file has not been run: the build container has no Julia toolchain.
+=#
+module GRMPHip
+
+using GradientRobustMultiPhysics
+using ExtendableGrids
+using ExtendableSparse
+using SparseArrays
+using LinearSolve
+
+const libgrmp = joinpath(@__DIR__, "..", "libgrmp_hip.so")
+
+# ---------------------------------------------------------------- ABI
+const GRMP_OK = Cint(0)
+const GRMP_FE = Dict(H1P1 => Cint(1), H1P2 => Cint(2), H1BR => Cint(3), L2P0 => Cint(4))
+const GRMP_BLF_LAPLACE, GRMP_BLF_HOOKE2D, GRMP_BLF_MASS = Cint(1), Cint(2), Cint(3)
+const GRMP_COEF_FACTOR, GRMP_COEF_CELL = Cint(0), Cint(1)
+const GRMP_VALUES_FP64, GRMP_VALUES_FP32 = Cint(1), Cint(2)
+
+struct GrmpForm
+    kind::Cint
+    coef_mode::Cint
+    quadorder::Cint
+    row_block::Cint
+    col_block::Cint
+    factor::Cdouble
+    params::NTuple{4,Cdouble}
+    cell_coef::Ptr{Cvoid}
+end
+
+struct GrmpLForm
+    kind::Cint
+    quadorder::Cint
+    factor::Cdouble
+    f::NTuple{4,Cdouble}
+end
+
+struct GrmpSolveOpts
+    method::Cint
+    precond::Cint
+    precision::Cint
+    rtol::Cdouble
+    maxit::Int64
+    inner_rtol::Cdouble
+    inner_maxit::Int64
+    check_every::Cint
+end
+
+mutable struct GrmpSolveInfo
+    status::Cint
+    outer::Cint
+    iterations::Int64
+    relres::Cdouble
+    seconds::Cdouble
+end
+GrmpSolveInfo() = GrmpSolveInfo(0, 0, 0, 0.0, 0.0)
+
+function check(ctx::Ptr{Cvoid}, rc::Cint)
+    rc == GRMP_OK && return nothing
+    msg = unsafe_string(ccall((:grmp_last_error, libgrmp), Cstring, (Ptr{Cvoid},), ctx))
+    error("libgrmp_hip error $rc: $msg")
+end
+
+# one context per process (device 0)
+const CTX = Ref{Ptr{Cvoid}}(C_NULL)
+function context()
+    if CTX[] == C_NULL
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        rc = ccall((:grmp_ctx_create, libgrmp), Cint, (Cint, Ref{Ptr{Cvoid}}), 0, h)
+        rc == GRMP_OK || error("grmp_ctx_create failed ($rc)")
+        CTX[] = h[]
+    end
+    return CTX[]
+end
+
+# ------------------------------------------------- grids and spaces
+# device images are cached per xgrid / FESpace object
+const MESHES = IdDict{Any,Ptr{Cvoid}}()
+const SPACES = IdDict{Any,Ptr{Cvoid}}()
+
+function device_mesh(xgrid::ExtendableGrid)
+    get!(MESHES, xgrid) do
+        coords = xgrid[Coordinates]                 # [dim, nnodes]
+        cells = Int32.(xgrid[CellNodes])            # [nloc, ncells], 1-based
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        GC.@preserve coords cells begin
+            check(context(), ccall((:grmp_mesh_upload, libgrmp), Cint,
+                (Ptr{Cvoid}, Cint, Int64, Ptr{Cdouble}, Int64, Cint, Ptr{Int32}, Ref{Ptr{Cvoid}}),
+                context(), size(coords, 1), size(coords, 2), coords, size(cells, 2), size(cells, 1), cells, h))
+        end
+        h[]
+    end
+end
+
+function device_space(FES::FESpace{Tv,Ti,FEType}) where {Tv,Ti,FEType}
+    get!(SPACES, FES) do
+        dofs = Int32.(Matrix(FES[CellDofs]))        # [ndofs_loc, ncells], 1-based
+        fe = GRMP_FE[FEType.name.wrapper]
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        GC.@preserve dofs begin
+            check(context(), ccall((:grmp_space_upload, libgrmp), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Int64, Cint, Ptr{Int32}, Ref{Ptr{Cvoid}}),
+                context(), device_mesh(FES.xgrid), fe, get_ncomponents(FEType), FES.ndofs, size(dofs, 1), dofs, h))
+        end
+        h[]
+    end
+end
+
+# ---------------------------------------------- pattern-level assemble!
+"""
+    hip_assemble!(A::FEMatrixBlock, O::PDEOperator; kappa = nothing)
+
+Device replacement of assemble!(A, AssemblyPattern) for LaplaceOperator,
+ReactionOperator, HookStiffnessOperator2D and the per-cell-coefficient
+BilinearForm([Gradient,Gradient], action; APT_SymmetricBilinearForm): the
+block is assembled on the GPU into the reference's CSC pattern (_addnz zero
+skip) and added into A.entries.
+"""
+function hip_assemble!(A::FEMatrixBlock, kind::Cint, quadorder::Int; factor = 1.0, kappa = nothing,
+                       params = (0.0, 0.0, 0.0, 0.0))
+    ctx = context()
+    sx = device_space(A.FESX)
+    m = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ctx, ccall((:grmp_matrix_create, libgrmp), Cint, (Ptr{Cvoid}, Cint, Ref{Ptr{Cvoid}}, Ref{Ptr{Cvoid}}),
+                     ctx, 1, Ref(sx), m))
+    kv = Ref{Ptr{Cvoid}}(C_NULL)
+    if kappa !== nothing
+        check(ctx, ccall((:grmp_vector_create, libgrmp), Cint, (Ptr{Cvoid}, Int64, Ref{Ptr{Cvoid}}), ctx, length(kappa), kv))
+        check(ctx, ccall((:grmp_vector_upload, libgrmp), Cint, (Ptr{Cvoid}, Ptr{Cdouble}), kv[], Vector{Float64}(kappa)))
+    end
+    form = GrmpForm(kind, kappa === nothing ? GRMP_COEF_FACTOR : GRMP_COEF_CELL, quadorder, 0, 0, factor, params, kv[])
+    check(ctx, ccall((:grmp_matrix_add_form, libgrmp), Cint, (Ptr{Cvoid}, Ref{GrmpForm}), m[], form))
+    check(ctx, ccall((:grmp_matrix_assemble, libgrmp), Cint, (Ptr{Cvoid}, Cint), m[], GRMP_VALUES_FP64))
+    n, nnz, mr = Ref{Int64}(0), Ref{Int64}(0), Ref{Cint}(0)
+    check(ctx, ccall((:grmp_matrix_info, libgrmp), Cint, (Ptr{Cvoid}, Ref{Int64}, Ref{Int64}, Ref{Cint}), m[], n, nnz, mr))
+    colptr, rowval, nzval = Vector{Int64}(undef, n[] + 1), Vector{Int64}(undef, nnz[]), Vector{Float64}(undef, nnz[])
+    check(ctx, ccall((:grmp_matrix_export_csc, libgrmp), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Cdouble}),
+                     m[], colptr, rowval, nzval))
+    ccall((:grmp_matrix_destroy, libgrmp), Cint, (Ptr{Cvoid},), m[])
+    kv[] != C_NULL && ccall((:grmp_vector_destroy, libgrmp), Cint, (Ptr{Cvoid},), kv[])
+    # add into the block (same entries the reference's _addnz would create)
+    B = SparseMatrixCSC(n[], n[], colptr, rowval, nzval)
+    rows, cols, vals = findnz(B)
+    for k in eachindex(vals)
+        rawupdateindex!(A.entries, +, vals[k], rows[k] + A.offsetX, cols[k] + A.offsetY)
+    end
+    flush!(A.entries)
+    return nothing
+end
+
+# ---------------------------------------------------- LinearSolve plug-in
+"""
+    GRMPKrylov(; precision = :fp32_ir, rtol = 1e-10)
+
+LinearSolve algorithm for solve!(Solution, PDE; linsolver = GRMPKrylov()):
+Jacobi-PCG on the GPU, FP32 inner sweeps inside FP64 refinement
+(`precision = :fp32_ir`) or plain FP64 (`:fp64`).  Penalized rows
+(diagonal >= 1e59, src/solvers.jl:649-652) are eliminated.
+"""
+struct GRMPKrylov <: LinearSolve.SciMLLinearSolveAlgorithm
+    precision::Symbol
+    rtol::Float64
+end
+GRMPKrylov(; precision = :fp32_ir, rtol = 1e-10) = GRMPKrylov(precision, rtol)
+
+function LinearSolve.solve(cache::LinearSolve.LinearCache, alg::GRMPKrylov; kwargs...)
+    A::SparseMatrixCSC{Float64,Int64} = cache.A
+    b::Vector{Float64} = cache.b
+    x = zeros(length(b))
+    n = size(A, 1)
+    fixed = [j for j in 1:n if A[j, j] >= 1e59]
+    # CSC of A is CSR of A^T; the systems solved here are symmetric
+    rc = ccall((:grmp_solve_csc, libgrmp), Cint,
+               (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Int64, Ptr{Int64},
+                Cint, Cdouble), context(), n, A.colptr, A.rowval, A.nzval, b, x, length(fixed), fixed,
+               alg.precision == :fp64 ? Cint(1) : Cint(2), alg.rtol)
+    check(context(), rc)
+    return LinearSolve.SciMLBase.build_linear_solution(alg, x, nothing, cache)
+end
+
+end # module

@@ -1,0 +1,97 @@
+"""Request server: continuous batching of PDE solves on one GPU.
+
+Thin wrapper of grmp_server_* (include/grmp.h).  ``Server(mode="served")``
+admits requests by HBM footprint, keeps meshes and patterns resident,
+assembles new requests while others iterate, and solves all problems in
+flight with one batched FP32-in-FP64 Krylov engine.  ``mode="plain"`` serves
+one request at a time with the host-driven FP64 PCG, built from scratch.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import VP, default_context, lib
+
+POISSON, ELASTICITY = _abi.REQ_POISSON, _abi.REQ_ELASTICITY
+
+
+def request(kind, fe, N, seed, sigma=1.0, rtol=1e-10, E=1.0, nu=0.3, x_out=None):
+    r = _abi.Request()
+    r.kind, r.fe, r.N, r.seed = int(kind), int(fe), int(N), int(seed)
+    r.sigma, r.E, r.nu, r.rtol = float(sigma), float(E), float(nu), float(rtol)
+    r.x_out = x_out.ctypes.data_as(_abi.DP) if x_out is not None else None
+    return r
+
+
+def ndofs(kind, fe, N):
+    return int(lib().grmp_request_ndofs(ctypes.byref(request(kind, fe, N, 0))))
+
+
+def footprint(kind, fe, N):
+    return int(lib().grmp_request_footprint(ctypes.byref(request(kind, fe, N, 0))))
+
+
+class Server:
+    def __init__(self, mode="served", memory_budget=64 << 30, max_active=256, graph=True, profile=False, ctx=None):
+        self.ctx = ctx or default_context()
+        cfg = _abi.ServerConfig()
+        cfg.mode = 1 if mode == "served" else 2
+        cfg.memory_budget = int(memory_budget)
+        cfg.max_active = int(max_active)
+        cfg.graph = 1 if graph else 0
+        cfg.profile = 1 if profile else 0
+        h = VP()
+        self.ctx.check(lib().grmp_server_create(self.ctx.h, ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self._outs = {}
+
+    def submit(self, kind, fe, N, seed, sigma=1.0, rtol=1e-10, keep_solution=True, **kw):
+        """Returns a ticket; the solution lands in a host array on completion."""
+        x = np.zeros(ndofs(kind, fe, N)) if keep_solution else None
+        r = request(kind, fe, N, seed, sigma, rtol, x_out=x, **kw)
+        t = ctypes.c_int64()
+        self.ctx.check(lib().grmp_server_submit(self.h, ctypes.byref(r), ctypes.byref(t)))
+        self._outs[t.value] = x
+        return t.value
+
+    def poll(self, ticket):
+        res = _abi.Result()
+        rc = lib().grmp_server_poll(self.h, int(ticket), ctypes.byref(res))
+        if rc == _abi.ERR_BUSY:
+            return None
+        self.ctx.check(rc)
+        return res
+
+    def wait(self, ticket):
+        res = _abi.Result()
+        self.ctx.check(lib().grmp_server_wait(self.h, int(ticket), ctypes.byref(res)))
+        return res
+
+    def solution(self, ticket):
+        return self._outs.get(ticket)
+
+    def stats(self):
+        st = _abi.KernelStats()
+        self.ctx.check(lib().grmp_server_stats(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def clock(self):
+        return lib().grmp_server_clock(self.h)
+
+    def close(self):
+        if self.h:
+            lib().grmp_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -219,6 +219,14 @@ void grmp_solve_opts_default(grmp_solve_opts *o);
 int grmp_solve(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, const grmp_solve_opts *opts,
                grmp_solve_info *info);
 
+/* Solver plug-in: solves LinearProblem(A.entries.cscmatrix, b.entries) as the
+ * linsolver of solve! (src/solvers.jl:55, 89-101, 656-660).  colptr[n+1],
+ * rowval[nnz] 1-based (Int64, as SparseMatrixCSC{Float64,Int64}); fixed[nfixed]
+ * 1-based penalized dofs, eliminated (x_d = b_d / A_dd).  A must be symmetric
+ * positive definite on the free dofs.  precision: GRMP_PRECISION_*. */
+int grmp_solve_csc(grmp_ctx *ctx, int64_t n, const int64_t *colptr, const int64_t *rowval, const double *nzval,
+                   const double *b, double *x, int64_t nfixed, const int64_t *fixed, int precision, double rtol);
+
 /* ---------------------------------------------------------------- serving */
 #define GRMP_REQ_POISSON 1      /* PoissonProblem, per-cell log-normal kappa, f = 1 */
 #define GRMP_REQ_ELASTICITY 2   /* LinearElasticityProblem P1{2}, Hooke scaled per cell */
@@ -248,7 +256,17 @@ typedef struct grmp_server_config {
     int64_t memory_budget;    /* bytes of HBM for resident problems */
     int max_active;           /* problems in flight */
     int graph;                /* capture the Krylov step as a hipGraph */
+    int profile;              /* time the engine SpMV kernel with HIP events */
 } grmp_server_config;
+
+typedef struct grmp_kernel_stats {
+    double spmv_seconds;      /* summed durations of the engine SpMV launches (HIP events) */
+    int64_t spmv_launches;
+    double spmv_bytes;        /* algorithmic bytes: nnz*(4+4) + 4(n+1) + 4n (x) + 4n (y) per row tile */
+    int64_t steps;            /* engine steps run */
+    int64_t templates;        /* problem shapes built (mesh + dofmap + pattern) */
+    double template_seconds;  /* host-observed time spent building them */
+} grmp_kernel_stats;
 
 /* Request loop (continuous batching): requests are admitted by HBM footprint
  * as memory and slots free up, assembled by batched kernels and solved by a
@@ -260,6 +278,8 @@ int grmp_server_submit(grmp_server *s, const grmp_request *req, int64_t *ticket)
 int grmp_server_poll(grmp_server *s, int64_t ticket, grmp_result *res);
 int grmp_server_wait(grmp_server *s, int64_t ticket, grmp_result *res);
 double grmp_server_clock(const grmp_server *s);
+/* engine counters since creation (profile = 1 for the SpMV timings) */
+int grmp_server_stats(grmp_server *s, grmp_kernel_stats *out);
 int grmp_server_destroy(grmp_server *s);
 /* footprint in bytes the server would reserve for a request */
 int64_t grmp_request_footprint(const grmp_request *req);

@@ -74,3 +74,30 @@ def test_nonconvergence_reported(grmp, O):
     with pytest.raises(grmp.GrmpError) as ei:
         grmp.solve(sol, pde, precision="fp64", rtol=1e-30, maxiterations=5)
     assert ei.value.code == grmp._abi.ERR_NOCONV
+
+
+@pytest.mark.parametrize("precision", [1, 2])
+def test_solve_csc_plugin(grmp, O, precision):
+    """The LinearSolve plug-in entry on a CSC assembled by the oracle."""
+    import ctypes
+    N = 20
+    S = O.Space(N, O.FE_P2)
+    rng = np.random.default_rng(4)
+    kap = np.exp(rng.standard_normal(S.cells.shape[0]))
+    A = O.assemble_matrix(S.ndofs, [dict(kind=O.BLF_LAPLACE, fe=O.FE_P2, quadorder=2, factor=1.0, coef=O.COEF_CELL,
+                                        kappa=kap, coords=S.coords, cells=S.cells, celldofs=S.celldofs)],
+                          fixed=S.fixed)
+    b = np.zeros(S.ndofs)
+    O.assemble_lf_const(b, O.FE_P2, 1, 2, [1.0], S.coords, S.cells, S.celldofs)
+    b = O.penalized_rhs(b, S.fixed)
+    xref = O.direct_solve(A, b)
+    ab = grmp._abi
+    colptr = np.ascontiguousarray(A.colptr + 1, dtype=np.int64)
+    rowval = np.ascontiguousarray(A.rowval + 1, dtype=np.int64)
+    fixed = np.ascontiguousarray(S.fixed + 1, dtype=np.int64)
+    x = np.zeros(S.ndofs)
+    ctx = grmp.default_context()
+    ctx.check(grmp.lib().grmp_solve_csc(ctx.h, S.ndofs, ab.i64ptr(colptr), ab.i64ptr(rowval), ab.dptr(A.nzval),
+                                        ab.dptr(b), ab.dptr(x), fixed.size, ab.i64ptr(fixed), precision, 1e-12))
+    err = np.linalg.norm(x - xref) / np.linalg.norm(xref)
+    assert err <= 1e-8, err
