@@ -825,6 +825,12 @@ int assemble_values(const grmp_matrix *A, const double *kappa, DevForm *dforms, 
                     float *val32, double *dinv64, float *dinv32, hipStream_t st)
 {
     grmp_ctx *ctx = A->ctx;
+    GRMP_CHECK(ctx, A->have_pattern && A->rowptr && A->diagpos && A->fixed && dforms && hforms && dinv64 && dinv32 &&
+                        (val64 || val32),
+               GRMP_ERR_ARG, "assemble_values: pattern or output buffers missing");
+    for (auto &r : A->forms)
+        GRMP_CHECK(ctx, r.pos && (r.f.coef_mode != GRMP_COEF_CELL || kappa || r.f.cell_coef), GRMP_ERR_ARG,
+                   "assemble_values: slot map or coefficient missing");
     int first[9];
     int k = 0;
     for (int rb = 0; rb < A->nblocks; ++rb) {

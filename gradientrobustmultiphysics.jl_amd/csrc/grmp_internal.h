@@ -176,7 +176,7 @@ struct ProblemDesc;
 int spmv_launch(grmp_ctx *ctx, const grmp_matrix *A, int values, const double *x, double *y,
                 hipStream_t st);
 // x = b/A_dd on fixed rows, 0 elsewhere
-int init_x_launch(const grmp_matrix *A, const double *b, double *x, hipStream_t st);
+int init_x_launch(const grmp_matrix *A, const double *val64, const double *b, double *x, hipStream_t st);
 // b[i] = 0 on fixed rows (homogeneous Dirichlet, penalty * 0)
 int zero_fixed_launch(const grmp_matrix *A, double *b, hipStream_t st);
 int plain_cg_solve(grmp_matrix *A, const double *b, double *x, const grmp_solve_opts &o, grmp_solve_info &info);

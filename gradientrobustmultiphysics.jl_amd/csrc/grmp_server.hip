@@ -259,7 +259,7 @@ static int job_start(grmp_server *S, Job *j)
     GRMP_HIP(c, hipMemsetAsync(j->b, 0, n * 8, st));
     GRMP_TRY(linearform_launch(c, t->space, &t->lf, j->b, st));
     GRMP_TRY(zero_fixed_launch(A, j->b, st));
-    GRMP_TRY(init_x_launch(A, j->b, j->x, st));
+    GRMP_TRY(init_x_launch(A, j->val64, j->b, j->x, st));
     GRMP_HIP(c, hipEventCreateWithFlags(&j->ev, hipEventDisableTiming));
     GRMP_HIP(c, hipEventRecord(j->ev, st));
     j->state = 1;

@@ -248,6 +248,8 @@ int plain_cg_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solv
 {
     grmp_ctx *ctx = A->ctx;
     const int64_t n = A->n;
+    GRMP_CHECK(ctx, A->rowptr && A->cols && A->val64 && A->dinv64 && A->fixed && A->diagpos && bd && xd, GRMP_ERR_ARG,
+               "plain CG: matrix not assembled in FP64");
     double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *part = nullptr;
     GRMP_TRY(dalloc(ctx, &r, n));
     GRMP_TRY(dalloc(ctx, &z, n));
@@ -316,9 +318,11 @@ __global__ void k_zero_fixed(const uint8_t *fixed, double *b, int64_t n)
     if (i < n && fixed[i]) b[i] = 0.0;
 }
 
-int init_x_launch(const grmp_matrix *A, const double *b, double *x, hipStream_t st)
+int init_x_launch(const grmp_matrix *A, const double *val64, const double *b, double *x, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_plain_init, dim3(nblk(A->n, 256)), dim3(256), 0, st, A->rowptr, A->diagpos, A->val64, A->fixed, b,
+    GRMP_CHECK(A->ctx, A->rowptr && A->diagpos && A->fixed && val64 && b && x, GRMP_ERR_ARG,
+               "init_x_launch: pattern or FP64 values missing");
+    hipLaunchKernelGGL(k_plain_init, dim3(nblk(A->n, 256)), dim3(256), 0, st, A->rowptr, A->diagpos, val64, A->fixed, b,
                        x, A->n);
     GRMP_HIP(A->ctx, hipGetLastError());
     return GRMP_OK;
@@ -326,6 +330,7 @@ int init_x_launch(const grmp_matrix *A, const double *b, double *x, hipStream_t 
 
 int zero_fixed_launch(const grmp_matrix *A, double *b, hipStream_t st)
 {
+    GRMP_CHECK(A->ctx, A->fixed && b, GRMP_ERR_ARG, "zero_fixed_launch: fixed mask or vector missing");
     hipLaunchKernelGGL(k_zero_fixed, dim3(nblk(A->n, 256)), dim3(256), 0, st, A->fixed, b, A->n);
     GRMP_HIP(A->ctx, hipGetLastError());
     return GRMP_OK;
@@ -803,7 +808,8 @@ static int solve_engine(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, co
 {
     grmp_ctx *ctx = A->ctx;
     const int64_t n = A->n;
-    GRMP_CHECK(ctx, A->val32 && A->val64, GRMP_ERR_ARG, "FP32_IR solve needs FP64 and FP32 values assembled");
+    GRMP_CHECK(ctx, A->val32 && A->val64 && A->dinv32 && A->fixed && A->diagpos, GRMP_ERR_ARG,
+               "FP32_IR solve needs FP64 and FP32 values assembled");
     double *r64 = nullptr;
     float *buf = nullptr;
     GRMP_TRY(dalloc(ctx, &r64, n));
