@@ -97,8 +97,9 @@ class KEngine {
     hipGraphExec_t exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
     bool profile_ = false;
-    bool graph_profile_ = false;
-    hipEvent_t ev_[2 * 8] = {};
+    static constexpr int EV_PAIRS = 32;
+    hipEvent_t ev_[2 * EV_PAIRS] = {};
+    int npending_ = 0;
     unsigned long long *dbytes_ = nullptr;
     double spmv_s_ = 0.0;
     int64_t spmv_n_ = 0, steps_ = 0;

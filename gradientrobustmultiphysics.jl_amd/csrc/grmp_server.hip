@@ -296,9 +296,13 @@ static int job_activate(grmp_server *S, Job *j)
     return GRMP_OK;
 }
 
+// Frees the job's device buffer on the stream that used it last: the solve
+// stream once the job has been in the engine (after its slot went idle),
+// the assembly stream before that.
 static void job_release(grmp_server *S, Job *j)
 {
-    if (j->dbuf) (void)hipFreeAsync(j->dbuf, S->st_solve);
+    hipStream_t st = j->slot >= 0 ? S->st_solve : S->sctx->stream;
+    if (j->dbuf) (void)hipFreeAsync(j->dbuf, st);
     j->dbuf = nullptr;
     if (j->ev) (void)hipEventDestroy(j->ev);
     j->ev = nullptr;
@@ -328,6 +332,33 @@ static void job_fail(grmp_server *S, Job *j, int rc)
     j->res.t_done = now(S);
     job_release(S, j);
     j->state = 3;
+}
+
+// A device error ends the server: every job still pending fails with the
+// error, engine slots are released first (their records go idle before any
+// buffer is freed), and later submissions are refused.
+static void server_fatal(grmp_server *S, int rc)
+{
+    (void)hipStreamSynchronize(S->st_solve);
+    (void)hipStreamSynchronize(S->sctx->stream);
+    std::lock_guard<std::mutex> g(S->mu);
+    S->fatal = rc;
+    S->fatal_msg = S->sctx->err;
+    for (Job *j : S->solving) {
+        S->eng->remove(j->slot);
+        job_fail(S, j, rc);
+    }
+    S->solving.clear();
+    for (Job *j : S->assembling) job_fail(S, j, rc);
+    S->assembling.clear();
+    for (Job *j : S->queue) {
+        j->res.status = rc;
+        j->res.t_done = now(S);
+        j->state = 3;
+    }
+    S->queue.clear();
+    (void)hipStreamSynchronize(S->st_solve);
+    S->cv_done.notify_all();
 }
 
 // ------------------------------------------------------------ served loop
@@ -386,13 +417,8 @@ static void served_loop(grmp_server *S)
         int rc = S->eng->run(steps, S->cfg.graph != 0);
         if (!rc) rc = S->eng->poll(st);
         if (rc) {
-            std::lock_guard<std::mutex> g(S->mu);
-            S->fatal = rc;
-            S->fatal_msg = S->sctx->err;
-            for (Job *j : S->solving) job_fail(S, j, rc);
-            S->solving.clear();
-            S->cv_done.notify_all();
-            continue;
+            server_fatal(S, rc);
+            return;
         }
         for (auto it = S->solving.begin(); it != S->solving.end();) {
             Job *j = *it;
@@ -527,6 +553,11 @@ extern "C" int grmp_server_submit(grmp_server *S, const grmp_request *req, int64
                          (long long)grmp_request_footprint(req), (long long)S->cfg.memory_budget);
     }
     std::lock_guard<std::mutex> g(S->mu);
+    if (S->fatal) {
+        delete j;
+        return grmp_fail(S->user, S->fatal, "grmp_server_submit: server stopped after a device error: %s",
+                         S->fatal_msg.c_str());
+    }
     j->ticket = S->next_ticket++;
     j->res.t_submit = now(S);
     S->jobs[j->ticket] = j;

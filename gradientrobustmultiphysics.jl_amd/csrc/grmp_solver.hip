@@ -8,6 +8,7 @@
 // the residual is measured on the free rows only, the convention of the
 // reference's residual check (src/solvers.jl:663-668).
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 
@@ -658,6 +659,10 @@ int KEngine::add(const KProb &p, int *slot)
 void KEngine::remove(int slot)
 {
     if (slot < 0 || slot >= cap_p_ || !slots_[slot].used) return;
+    // the slot's record goes idle on the device before its buffers can be
+    // released: no kernel touches a problem whose phase is idle
+    static const int32_t idle = PH_IDLE;
+    (void)hipMemcpyAsync((char *)(dP_ + slot) + offsetof(KProb, phase), &idle, sizeof(int32_t), hipMemcpyHostToDevice, st_);
     slots_[slot].used = false;
     ttot_ -= slots_[slot].ntiles;
     slots_[slot].ntiles = 0;
@@ -702,20 +707,26 @@ int KEngine::sync_tables()
 int KEngine::launch_step(int grid)
 {
     dim3 g(grid), b(KB);
-    if (profile_) GRMP_HIP(ctx_, hipEventRecord(ev_[0], st_));
+    const int pr = npending_;
+    if (profile_) GRMP_HIP(ctx_, hipEventRecord(ev_[2 * pr], st_));
     hipLaunchKernelGGL(k_eng_spmv, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
-    if (profile_) GRMP_HIP(ctx_, hipEventRecord(ev_[1], st_));
+    if (profile_) {
+        GRMP_HIP(ctx_, hipEventRecord(ev_[2 * pr + 1], st_));
+        ++npending_;
+    }
     hipLaunchKernelGGL(k_eng_axpy, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
     hipLaunchKernelGGL(k_eng_elem, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
     hipLaunchKernelGGL(k_eng_resid, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
     GRMP_HIP(ctx_, hipGetLastError());
     ++steps_;
-    if (profile_) GRMP_TRY(read_events(1));
+    if (profile_ && npending_ == EV_PAIRS) GRMP_TRY(read_events(npending_));
     return GRMP_OK;
 }
 
 int KEngine::read_events(int pairs)
 {
+    npending_ = 0;
+    if (pairs == 0) return GRMP_OK;
     GRMP_HIP(ctx_, hipEventSynchronize(ev_[2 * pairs - 1]));
     for (int k = 0; k < pairs; ++k) {
         float ms = 0.f;
@@ -748,9 +759,7 @@ int KEngine::capture(int grid)
     GRMP_HIP(ctx, hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
     for (int s = 0; s < ENGINE_GRAPH_STEPS; ++s) {
         dim3 g(grid), b(KB);
-        if (profile_) GRMP_HIP(ctx, hipEventRecord(ev_[2 * s], st_));
         hipLaunchKernelGGL(k_eng_spmv, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
-        if (profile_) GRMP_HIP(ctx, hipEventRecord(ev_[2 * s + 1], st_));
         hipLaunchKernelGGL(k_eng_axpy, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
         hipLaunchKernelGGL(k_eng_elem, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
         hipLaunchKernelGGL(k_eng_resid, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
@@ -758,7 +767,6 @@ int KEngine::capture(int grid)
     GRMP_HIP(ctx, hipStreamEndCapture(st_, &graph_));
     GRMP_HIP(ctx, hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
     graph_grid_ = grid;
-    graph_profile_ = profile_;
     return GRMP_OK;
 }
 
@@ -768,15 +776,17 @@ int KEngine::run(int steps, bool use_graph)
     if (ttot_ == 0) return GRMP_OK;
     int grid = 64;
     while (grid < ttot_ && grid < 4096) grid <<= 1;
-    if (use_graph) {
-        if (!exec_ || graph_grid_ != grid || graph_profile_ != profile_) GRMP_TRY(capture(grid));
+    // profiling times every SpMV launch with an event pair, which a graph
+    // replay cannot carry, so profiled runs launch the steps one by one
+    if (use_graph && !profile_) {
+        if (!exec_ || graph_grid_ != grid) GRMP_TRY(capture(grid));
         for (int s = 0; s < steps; s += ENGINE_GRAPH_STEPS) {
             GRMP_HIP(ctx_, hipGraphLaunch(exec_, st_));
             steps_ += ENGINE_GRAPH_STEPS;
-            if (profile_) GRMP_TRY(read_events(ENGINE_GRAPH_STEPS));
         }
     } else {
         for (int s = 0; s < steps; ++s) GRMP_TRY(launch_step(grid));
+        if (profile_) GRMP_TRY(read_events(npending_));
     }
     return GRMP_OK;
 }
