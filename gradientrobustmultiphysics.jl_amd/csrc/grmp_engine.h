@@ -12,7 +12,7 @@
 #pragma once
 #include "grmp_internal.h"
 
-enum KPhase { PH_IDLE = 0, PH_RESID = 1, PH_INIT = 2, PH_ITER = 3, PH_UPDATE = 4, PH_DONE = 5, PH_FAIL = 6 };
+enum KPhase { PH_IDLE = 0, PH_RESID = 1, PH_INIT = 2, PH_ITER = 3, PH_UPDATE = 4, PH_DONE = 5, PH_FAIL = 6, PH_REINIT = 7 };
 
 struct KProb {
     // matrix (CSR, 0-based), fixed-dof mask, Jacobi inverse diagonal
@@ -30,14 +30,15 @@ struct KProb {
     float *r, *p, *q, *dx;
     int32_t n, lanes;
     int32_t phase, status;
-    int32_t outer, max_outer;
-    int32_t inner_it, inner_maxit;
+    int32_t outer;      // FP64 residual refreshes so far
+    int32_t max_outer;
     int64_t iters, maxit;
-    double bb, rtol, inner_rtol;
-    double rnorm;       // ||r64|| of the current sweep (correction scale)
-    double rz, alpha, beta, relres;
-    unsigned int cnt;   // tiles finished in the current kernel
-    int32_t pad_;
+    double bb, bnorm;   // ||b - A x0||^2 and its root (free rows)
+    double rtol;        // target relative FP64 residual
+    double delta;       // reliable-update threshold on the FP32 residual
+    double scale;       // FP32 quantities are FP64 ones times scale
+    double pscale;      // search-direction rescale at a refresh
+    double rz, alpha, beta, relres, rmax;
 };
 
 struct KTile {
@@ -77,6 +78,7 @@ class KEngine {
   private:
     int capture(int grid);
     int launch_step(int grid);
+    void launch_kernels(int grid, bool timed);
     grmp_ctx *ctx_;
     hipStream_t st_;
     int cap_p_, cap_t_, tile_rows_;

@@ -286,11 +286,10 @@ static int job_activate(grmp_server *S, Job *j)
     p.dx = j->f4 + 3 * j->n;
     p.n = (int32_t)j->n;
     p.lanes = j->tpl->lanes;
-    p.max_outer = 60;
-    p.inner_maxit = 1 << 30;
+    p.max_outer = 200;
     p.maxit = 4000000;
     p.rtol = j->req.rtol > 0 ? j->req.rtol : 1e-10;
-    p.inner_rtol = 1e-4;
+    p.delta = 0.1;
     GRMP_TRY(S->eng->add(p, &j->slot));
     j->state = 2;
     return GRMP_OK;

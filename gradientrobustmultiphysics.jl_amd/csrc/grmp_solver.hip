@@ -338,43 +338,30 @@ int zero_fixed_launch(const grmp_matrix *A, double *b, hipStream_t st)
 }
 
 // ============================================================ the engine
-
-// all threads: true if this block finished the last tile of problem pi
-// (thread 0 holds the block sums a, b on entry)
-__device__ __forceinline__ bool finish_tile(KProb *P, const KRange *R, double *part, int t, int pi, double a, double b)
-{
-    __shared__ int s_last;
-    if (threadIdx.x == 0) {
-        part[2 * t] = a;
-        part[2 * t + 1] = b;
-        __threadfence();
-        unsigned prev = atomicAdd(&P[pi].cnt, 1u);
-        s_last = (prev + 1 == (unsigned)R[pi].nt) ? 1 : 0;
-    }
-    __syncthreads();
-    const bool last = s_last != 0;
-    __syncthreads();
-    return last;
-}
-
-// sums the tile partials of problem pi in a fixed order (valid in thread 0)
-__device__ __forceinline__ void sum_parts(const KRange *R, const double *part, int pi, double &a, double &b)
-{
-    __threadfence();
-    a = 0.0;
-    b = 0.0;
-    const int t0 = R[pi].t0, nt = R[pi].nt;
-    for (int j = threadIdx.x; j < nt; j += KB) {
-        a += __builtin_nontemporal_load(&part[2 * (t0 + j)]);
-        b += __builtin_nontemporal_load(&part[2 * (t0 + j) + 1]);
-    }
-    block_sum2(a, b);
-}
+//
+// Step = 7 launches over the tile table:
+//   spmv  (ITER)            q = A32 p, tile partials of p.q
+//   red A                   alpha = rz / p.q
+//   axpy  (ITER)            dx += alpha p, r -= alpha q, tile partials (r.z, r.r)
+//   red B                   convergence estimate, reliable-update trigger, beta
+//   elem  (ITER)            p = D r + beta p
+//         (UPDATE)          x += dx / s                       (FP64)
+//         (INIT/REINIT)     r = s r64, dx = 0, p = D r | p *= s/s_old
+//   resid (RESID)           r64 = b - A64 x, tile partials r64.r64   (FP64)
+//   red C                   phase changes of elem / resid
+// Tile kernels write one pair of partials per tile; the reduce kernels sum a
+// problem's partials in a fixed order (one block per problem).  Kernel
+// boundaries order the memory, so no fence or atomic is needed on the data
+// path, and a problem's arithmetic does not depend on its neighbours.
+//
+// Mixed precision: CG with reliable updates (FP32 recurrences; the FP64
+// residual b - A x replaces the FP32 one whenever the latter has dropped by
+// delta since the last refresh, keeping the search direction), so the
+// Krylov space is not discarded as in restarted refinement.
 
 #define ENGINE_ARGS KProb *__restrict__ P, const KTile *__restrict__ T, const KRange *__restrict__ R, \
                     double *__restrict__ part, const int *__restrict__ ntiles, unsigned long long *__restrict__ nbytes
 
-// SpMV step of the inner CG (phase ITER): q = A32 p, alpha = rz / p.q
 template <int L>
 __device__ __forceinline__ double spmv_tile(const KProb &pr, int row0, int row1)
 {
@@ -407,32 +394,15 @@ __global__ __launch_bounds__(KB) void k_eng_spmv(ENGINE_ARGS)
         }
         block_sum2(acc, dummy);
         if (threadIdx.x == 0) {
+            part[2 * t] = acc;
             // algorithmic bytes of this tile: values + columns, row pointers, x gather, y write
             const unsigned long long rows = (unsigned long long)(tl.row1 - tl.row0);
             const unsigned long long nz = (unsigned long long)(pr.rowptr[tl.row1] - pr.rowptr[tl.row0]);
             atomicAdd(nbytes, nz * 8ull + 4ull * (rows + 1) + 8ull * rows);
         }
-        if (finish_tile(P, R, part, t, pi, acc, dummy)) {
-            double pq, d2;
-            sum_parts(R, part, pi, pq, d2);
-            if (threadIdx.x == 0) {
-                KProb &w = P[pi];
-                double alpha = w.rz / pq;
-                w.alpha = alpha;
-                if (!(pq == pq) || __builtin_isinf(pq)) {
-                    w.status = GRMP_ERR_NONFINITE;
-                    w.phase = PH_FAIL;
-                } else if (!(pq > 0.0) || __builtin_isinf(alpha)) {
-                    w.phase = PH_UPDATE;  // breakdown: keep the correction so far
-                    w.alpha = 0.0;
-                }
-                w.cnt = 0;
-            }
-        }
     }
 }
 
-// dx += alpha p; r -= alpha q; z = dinv r; (r.z, r.r) -> beta / convergence
 __global__ __launch_bounds__(KB) void k_eng_axpy(ENGINE_ARGS)
 {
     const int nt = *ntiles;
@@ -444,8 +414,7 @@ __global__ __launch_bounds__(KB) void k_eng_axpy(ENGINE_ARGS)
         const float a = (float)pr.alpha;
         double rz = 0.0, rr = 0.0;
         for (int i = tl.row0 + threadIdx.x; i < tl.row1; i += KB) {
-            float pi_ = pr.p[i];
-            pr.dx[i] += a * pi_;
+            pr.dx[i] += a * pr.p[i];
             float ri = pr.r[i] - a * pr.q[i];
             pr.r[i] = ri;
             float zi = pr.dinv32[i] * ri;
@@ -453,30 +422,13 @@ __global__ __launch_bounds__(KB) void k_eng_axpy(ENGINE_ARGS)
             rr += (double)ri * (double)ri;
         }
         block_sum2(rz, rr);
-        if (finish_tile(P, R, part, t, pi, rz, rr)) {
-            double srz, srr;
-            sum_parts(R, part, pi, srz, srr);
-            if (threadIdx.x == 0) {
-                KProb &w = P[pi];
-                w.inner_it += 1;
-                w.iters += 1;
-                if (!(srr == srr) || __builtin_isinf(srr)) {
-                    w.status = GRMP_ERR_NONFINITE;
-                    w.phase = PH_FAIL;
-                } else if (std::sqrt(srr) <= w.inner_rtol || w.inner_it >= w.inner_maxit || w.iters >= w.maxit) {
-                    w.phase = PH_UPDATE;
-                } else {
-                    w.beta = srz / w.rz;
-                    w.rz = srz;
-                }
-                w.cnt = 0;
-            }
+        if (threadIdx.x == 0) {
+            part[2 * t] = rz;
+            part[2 * t + 1] = rr;
         }
     }
 }
 
-// elementwise phases: ITER p = dinv r + beta p; UPDATE x += ||r64|| dx then
-// RESID; INIT r = r64/||r64||, dx = 0, p = dinv r then ITER
 __global__ __launch_bounds__(KB) void k_eng_elem(ENGINE_ARGS)
 {
     const int nt = *ntiles;
@@ -484,47 +436,37 @@ __global__ __launch_bounds__(KB) void k_eng_elem(ENGINE_ARGS)
         const KTile tl = T[t];
         const int pi = tl.prob;
         const int ph = P[pi].phase;
-        if (ph != PH_ITER && ph != PH_UPDATE && ph != PH_INIT) continue;
+        if (ph != PH_ITER && ph != PH_UPDATE && ph != PH_INIT && ph != PH_REINIT) continue;
         const KProb pr = P[pi];
         if (ph == PH_ITER) {
             const float beta = (float)pr.beta;
             for (int i = tl.row0 + threadIdx.x; i < tl.row1; i += KB) pr.p[i] = pr.dinv32[i] * pr.r[i] + beta * pr.p[i];
             continue;
         }
-        double acc = 0.0, dummy = 0.0;
         if (ph == PH_UPDATE) {
-            for (int i = tl.row0 + threadIdx.x; i < tl.row1; i += KB) pr.x[i] += pr.rnorm * (double)pr.dx[i];
-        } else {
-            const double inv = 1.0 / pr.rnorm;
-            for (int i = tl.row0 + threadIdx.x; i < tl.row1; i += KB) {
-                float ri = (float)(pr.r64[i] * inv);
-                float zi = pr.dinv32[i] * ri;
-                pr.r[i] = ri;
-                pr.dx[i] = 0.0f;
-                pr.p[i] = zi;
-                acc += (double)ri * (double)zi;
-            }
+            const double inv = 1.0 / pr.scale;
+            for (int i = tl.row0 + threadIdx.x; i < tl.row1; i += KB) pr.x[i] += inv * (double)pr.dx[i];
+            continue;
         }
-        block_sum2(acc, dummy);
-        if (finish_tile(P, R, part, t, pi, acc, dummy)) {
-            double s, d2;
-            sum_parts(R, part, pi, s, d2);
-            if (threadIdx.x == 0) {
-                KProb &w = P[pi];
-                if (ph == PH_UPDATE) {
-                    w.phase = PH_RESID;
-                } else {
-                    w.rz = s;
-                    w.inner_it = 0;
-                    w.phase = PH_ITER;
-                }
-                w.cnt = 0;
-            }
+        double rz = 0.0, rr = 0.0;
+        const float ps = (float)pr.pscale;
+        for (int i = tl.row0 + threadIdx.x; i < tl.row1; i += KB) {
+            float ri = (float)(pr.r64[i] * pr.scale);
+            float zi = pr.dinv32[i] * ri;
+            pr.r[i] = ri;
+            pr.dx[i] = 0.0f;
+            pr.p[i] = (ph == PH_INIT) ? zi : ps * pr.p[i];
+            rz += (double)ri * (double)zi;
+            rr += (double)ri * (double)ri;
+        }
+        block_sum2(rz, rr);
+        if (threadIdx.x == 0) {
+            part[2 * t] = rz;
+            part[2 * t + 1] = rr;
         }
     }
 }
 
-// FP64 residual r64 = b - A64 x (phase RESID) -> converged / next sweep
 template <int L>
 __device__ __forceinline__ double resid_tile(const KProb &pr, int row0, int row1)
 {
@@ -557,27 +499,96 @@ __global__ __launch_bounds__(KB) void k_eng_resid(ENGINE_ARGS)
         default: acc = resid_tile<16>(pr, tl.row0, tl.row1); break;
         }
         block_sum2(acc, dummy);
-        if (finish_tile(P, R, part, t, pi, acc, dummy)) {
-            double rr, d2;
-            sum_parts(R, part, pi, rr, d2);
-            if (threadIdx.x == 0) {
-                KProb &w = P[pi];
-                if (w.outer == 0) w.bb = rr;
-                w.relres = w.bb > 0.0 ? std::sqrt(rr / w.bb) : 0.0;
-                if (!(rr == rr) || __builtin_isinf(rr)) {
-                    w.status = GRMP_ERR_NONFINITE;
-                    w.phase = PH_FAIL;
-                } else if (rr == 0.0 || w.relres <= w.rtol) {
-                    w.phase = PH_DONE;
-                } else if (w.outer >= w.max_outer || w.iters >= w.maxit) {
-                    w.status = GRMP_ERR_NOCONV;
-                    w.phase = PH_FAIL;
-                } else {
-                    w.rnorm = std::sqrt(rr);
-                    w.outer += 1;
-                    w.phase = PH_INIT;
-                }
-                w.cnt = 0;
+        if (threadIdx.x == 0) part[2 * t] = acc;
+    }
+}
+
+// sums the tile partials of problem s in a fixed order (valid in thread 0)
+__device__ __forceinline__ void sum_parts(const KRange &rg, const double *part, double &a, double &b)
+{
+    a = 0.0;
+    b = 0.0;
+    for (int j = threadIdx.x; j < rg.nt; j += KB) {
+        a += part[2 * (rg.t0 + j)];
+        b += part[2 * (rg.t0 + j) + 1];
+    }
+    block_sum2(a, b);
+}
+
+__device__ __forceinline__ bool finite(double v) { return v == v && !__builtin_isinf(v); }
+
+enum { RED_A = 0, RED_B = 1, RED_C = 2 };
+
+template <int WHICH>
+__global__ __launch_bounds__(KB) void k_eng_reduce(KProb *__restrict__ P, const KRange *__restrict__ R, const double *__restrict__ part,
+                                                  int nslots)
+{
+    for (int s = blockIdx.x; s < nslots; s += gridDim.x) {
+        const KRange rg = R[s];
+        if (rg.nt == 0) continue;
+        const int ph = P[s].phase;
+        if (WHICH == RED_A || WHICH == RED_B) {
+            if (ph != PH_ITER) continue;
+        } else if (ph != PH_UPDATE && ph != PH_INIT && ph != PH_REINIT && ph != PH_RESID) {
+            continue;
+        }
+        double a = 0.0, b = 0.0;
+        if (!(WHICH == RED_C && ph == PH_UPDATE)) sum_parts(rg, part, a, b);
+        if (threadIdx.x != 0) continue;
+        KProb &w = P[s];
+        if (WHICH == RED_A) {
+            // a = p.q
+            const double alpha = w.rz / a;
+            if (!finite(a)) {
+                w.status = GRMP_ERR_NONFINITE;
+                w.phase = PH_FAIL;
+            } else if (!(a > 0.0) || !finite(alpha)) {
+                w.alpha = 0.0;
+                w.phase = PH_UPDATE;  // breakdown: refresh from the FP64 residual
+            } else {
+                w.alpha = alpha;
+            }
+        } else if (WHICH == RED_B) {
+            // a = r.z, b = r.r of the scaled FP32 residual
+            w.iters += 1;
+            const double rn = sqrt(b);
+            if (!finite(b)) {
+                w.status = GRMP_ERR_NONFINITE;
+                w.phase = PH_FAIL;
+            } else if (rn <= w.rtol * w.bnorm * w.scale || rn < w.delta * w.rmax || w.iters >= w.maxit) {
+                w.phase = PH_UPDATE;  // converged estimate or reliable update
+            } else {
+                w.beta = a / w.rz;
+                w.rz = a;
+                w.rmax = rn > w.rmax ? rn : w.rmax;
+            }
+        } else if (ph == PH_UPDATE) {
+            w.phase = PH_RESID;
+        } else if (ph == PH_INIT || ph == PH_REINIT) {
+            w.rz = a;
+            w.rmax = sqrt(b);
+            w.phase = PH_ITER;
+        } else {
+            // ph == PH_RESID, a = ||b - A x||^2 over free rows
+            if (w.outer == 0) {
+                w.bb = a;
+                w.bnorm = sqrt(a);
+            }
+            w.relres = w.bb > 0.0 ? sqrt(a / w.bb) : 0.0;
+            if (!finite(a)) {
+                w.status = GRMP_ERR_NONFINITE;
+                w.phase = PH_FAIL;
+            } else if (a == 0.0 || w.relres <= w.rtol) {
+                w.phase = PH_DONE;
+            } else if (w.outer >= w.max_outer || w.iters >= w.maxit) {
+                w.status = GRMP_ERR_NOCONV;
+                w.phase = PH_FAIL;
+            } else {
+                const double snew = 1.0 / sqrt(a);
+                w.pscale = (w.outer == 0) ? 1.0 : snew / w.scale;
+                w.scale = snew;
+                w.phase = (w.outer == 0) ? PH_INIT : PH_REINIT;
+                w.outer += 1;
             }
         }
     }
@@ -641,9 +652,10 @@ int KEngine::add(const KProb &p, int *slot)
     q.phase = PH_RESID;
     q.status = GRMP_OK;
     q.outer = 0;
-    q.inner_it = 0;
     q.iters = 0;
-    q.cnt = 0;
+    q.scale = 1.0;
+    q.pscale = 1.0;
+    q.rmax = 0.0;
     init_[s] = q;
     // the host copy lives in init_ until the stream has consumed it
     GRMP_HIP(ctx, hipMemcpyAsync(dP_ + s, &init_[s], sizeof(KProb), hipMemcpyHostToDevice, st_));
@@ -704,19 +716,27 @@ int KEngine::sync_tables()
     return GRMP_OK;
 }
 
-int KEngine::launch_step(int grid)
+void KEngine::launch_kernels(int grid, bool timed)
 {
-    dim3 g(grid), b(KB);
+    dim3 g(grid), b(KB), gr(std::min(cap_p_, 1024));
     const int pr = npending_;
-    if (profile_) GRMP_HIP(ctx_, hipEventRecord(ev_[2 * pr], st_));
+    if (timed) (void)hipEventRecord(ev_[2 * pr], st_);
     hipLaunchKernelGGL(k_eng_spmv, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
-    if (profile_) {
-        GRMP_HIP(ctx_, hipEventRecord(ev_[2 * pr + 1], st_));
+    if (timed) {
+        (void)hipEventRecord(ev_[2 * pr + 1], st_);
         ++npending_;
     }
+    hipLaunchKernelGGL(k_eng_reduce<RED_A>, gr, b, 0, st_, dP_, dR_, dpart_, cap_p_);
     hipLaunchKernelGGL(k_eng_axpy, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
+    hipLaunchKernelGGL(k_eng_reduce<RED_B>, gr, b, 0, st_, dP_, dR_, dpart_, cap_p_);
     hipLaunchKernelGGL(k_eng_elem, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
     hipLaunchKernelGGL(k_eng_resid, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
+    hipLaunchKernelGGL(k_eng_reduce<RED_C>, gr, b, 0, st_, dP_, dR_, dpart_, cap_p_);
+}
+
+int KEngine::launch_step(int grid)
+{
+    launch_kernels(grid, profile_);
     GRMP_HIP(ctx_, hipGetLastError());
     ++steps_;
     if (profile_ && npending_ == EV_PAIRS) GRMP_TRY(read_events(npending_));
@@ -757,13 +777,7 @@ int KEngine::capture(int grid)
     if (exec_) { (void)hipGraphExecDestroy(exec_); exec_ = nullptr; }
     if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
     GRMP_HIP(ctx, hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
-    for (int s = 0; s < ENGINE_GRAPH_STEPS; ++s) {
-        dim3 g(grid), b(KB);
-        hipLaunchKernelGGL(k_eng_spmv, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
-        hipLaunchKernelGGL(k_eng_axpy, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
-        hipLaunchKernelGGL(k_eng_elem, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
-        hipLaunchKernelGGL(k_eng_resid, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
-    }
+    for (int s = 0; s < ENGINE_GRAPH_STEPS; ++s) launch_kernels(grid, false);
     GRMP_HIP(ctx, hipStreamEndCapture(st_, &graph_));
     GRMP_HIP(ctx, hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
     graph_grid_ = grid;
@@ -809,7 +823,7 @@ extern "C" void grmp_solve_opts_default(grmp_solve_opts *o)
     o->precision = GRMP_PRECISION_FP64;
     o->rtol = 1e-10;
     o->maxit = 1000000;
-    o->inner_rtol = 1e-4;
+    o->inner_rtol = 0.1;
     o->inner_maxit = 100000;
     o->check_every = 1;
 }
@@ -845,11 +859,10 @@ static int solve_engine(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, co
     p.dx = buf + 3 * n;
     p.n = (int32_t)n;
     p.lanes = engine_pick_lanes((double)A->nnz / (double)n);
-    p.max_outer = 60;
-    p.inner_maxit = (int32_t)std::min<int64_t>(o.inner_maxit, 1 << 30);
+    p.max_outer = 200;
     p.maxit = o.maxit;
     p.rtol = o.rtol;
-    p.inner_rtol = o.inner_rtol;
+    p.delta = o.inner_rtol;
     int slot;
     GRMP_TRY(eng.add(p, &slot));
     std::vector<KProb> st;

@@ -102,14 +102,15 @@ typedef struct grmp_solve_opts {
     int precision;            /* GRMP_PRECISION_FP64 / GRMP_PRECISION_FP32_IR */
     double rtol;              /* ||b - Ax|| <= rtol ||b|| over free dofs */
     int64_t maxit;            /* total Krylov iterations */
-    double inner_rtol;        /* FP32 inner solve reduction per refinement sweep */
-    int64_t inner_maxit;
+    double inner_rtol;        /* FP32_IR: reliable-update threshold, the FP64 residual is
+                                 recomputed when the FP32 one has dropped by this factor */
+    int64_t inner_maxit;      /* unused, kept for layout */
     int check_every;          /* plain mode: host residual check period (1 = every iteration) */
 } grmp_solve_opts;
 
 typedef struct grmp_solve_info {
     int status;
-    int outer;                /* refinement sweeps */
+    int outer;                /* FP64 residual evaluations (refinement refreshes) */
     int64_t iterations;       /* Krylov iterations (inner ones in FP32_IR) */
     double relres;            /* final FP64 relative residual */
     double seconds;

@@ -118,3 +118,18 @@ def test_memory_budget_serializes(grmp):
         for spec, t in zip(specs, ts):
             assert S.wait(t).status == 0
             assert np.array_equal(S.solution(t), alone[spec][2])
+
+
+def test_profiled_engine_counts_spmv(grmp):
+    """Profiled serving times every engine SpMV launch with HIP events and
+    counts its algorithmic bytes; results equal the unprofiled ones."""
+    specs = MIX[:3]
+    alone = _alone(grmp, specs)
+    with grmp.Server("served", profile=True) as S:
+        ts = [S.submit(*spec) for spec in specs]
+        for spec, t in zip(specs, ts):
+            assert S.wait(t).status == 0
+            assert np.array_equal(S.solution(t), alone[spec][2])
+        st = S.stats()
+    assert st["spmv_launches"] > 0 and st["spmv_seconds"] > 0 and st["spmv_bytes"] > 0
+    assert st["steps"] >= st["spmv_launches"]
