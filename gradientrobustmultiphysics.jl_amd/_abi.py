@@ -4,10 +4,20 @@ This module is the only place the host code touches the native library.
 There is no fallback: if the library is missing or a call fails, a
 ``GrmpError`` is raised.
 """
+import atexit
 import ctypes
 import os
 
 import numpy as np
+
+# Handles are not released once the interpreter shuts down: the HIP runtime
+# may already be tearing down, and the process releases the device anyway.
+_finalizing = [False]
+atexit.register(lambda: _finalizing.__setitem__(0, True))
+
+
+def alive():
+    return not _finalizing[0]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libgrmp_hip.so")
@@ -188,9 +198,9 @@ class Context:
         self.check(lib().grmp_ctx_synchronize(self.h))
 
     def close(self):
-        if self.h:
+        if self.h and alive():
             lib().grmp_ctx_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -241,8 +251,8 @@ class DeviceVector:
 
     def __del__(self):
         try:
-            if self.h:
+            if self.h and alive():
                 lib().grmp_vector_destroy(self.h)
-                self.h = None
+            self.h = None
         except Exception:
             pass

@@ -32,6 +32,8 @@ struct KProb {
     int32_t phase, status;
     int32_t outer;      // FP64 residual refreshes so far
     int32_t max_outer;
+    int32_t restart;    // restart the search direction at the next refresh
+    int32_t pad_;
     int64_t iters, maxit;
     double bb, bnorm;   // ||b - A x0||^2 and its root (free rows)
     double rtol;        // target relative FP64 residual

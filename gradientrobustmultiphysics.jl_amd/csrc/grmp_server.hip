@@ -289,7 +289,7 @@ static int job_activate(grmp_server *S, Job *j)
     p.max_outer = 200;
     p.maxit = 4000000;
     p.rtol = j->req.rtol > 0 ? j->req.rtol : 1e-10;
-    p.delta = 0.1;
+    p.delta = 0.1;  // reliable update when the FP32 residual dropped 10x
     GRMP_TRY(S->eng->add(p, &j->slot));
     j->state = 2;
     return GRMP_OK;

@@ -345,7 +345,7 @@ int zero_fixed_launch(const grmp_matrix *A, double *b, hipStream_t st)
 //   axpy  (ITER)            dx += alpha p, r -= alpha q, tile partials (r.z, r.r)
 //   red B                   convergence estimate, reliable-update trigger, beta
 //   elem  (ITER)            p = D r + beta p
-//         (UPDATE)          x += dx / s                       (FP64)
+//         (UPDATE)          x += dx / s (FP64), and p = D r + beta p
 //         (INIT/REINIT)     r = s r64, dx = 0, p = D r | p *= s/s_old
 //   resid (RESID)           r64 = b - A64 x, tile partials r64.r64   (FP64)
 //   red C                   phase changes of elem / resid
@@ -445,7 +445,12 @@ __global__ __launch_bounds__(KB) void k_eng_elem(ENGINE_ARGS)
         }
         if (ph == PH_UPDATE) {
             const double inv = 1.0 / pr.scale;
-            for (int i = tl.row0 + threadIdx.x; i < tl.row1; i += KB) pr.x[i] += inv * (double)pr.dx[i];
+            const float beta = (float)pr.beta;
+            const bool pupd = pr.restart == 0;
+            for (int i = tl.row0 + threadIdx.x; i < tl.row1; i += KB) {
+                pr.x[i] += inv * (double)pr.dx[i];
+                if (pupd) pr.p[i] = pr.dinv32[i] * pr.r[i] + beta * pr.p[i];
+            }
             continue;
         }
         double rz = 0.0, rr = 0.0;
@@ -515,7 +520,7 @@ __device__ __forceinline__ void sum_parts(const KRange &rg, const double *part, 
     block_sum2(a, b);
 }
 
-__device__ __forceinline__ bool finite(double v) { return v == v && !__builtin_isinf(v); }
+__device__ __forceinline__ bool isfin(double v) { return v == v && !__builtin_isinf(v); }
 
 enum { RED_A = 0, RED_B = 1, RED_C = 2 };
 
@@ -539,28 +544,31 @@ __global__ __launch_bounds__(KB) void k_eng_reduce(KProb *__restrict__ P, const 
         if (WHICH == RED_A) {
             // a = p.q
             const double alpha = w.rz / a;
-            if (!finite(a)) {
+            if (!isfin(a)) {
                 w.status = GRMP_ERR_NONFINITE;
                 w.phase = PH_FAIL;
-            } else if (!(a > 0.0) || !finite(alpha)) {
+            } else if (!(a > 0.0) || !isfin(alpha)) {
                 w.alpha = 0.0;
-                w.phase = PH_UPDATE;  // breakdown: refresh from the FP64 residual
+                w.restart = 1;        // breakdown: refresh and restart the direction
+                w.phase = PH_UPDATE;
             } else {
                 w.alpha = alpha;
             }
         } else if (WHICH == RED_B) {
-            // a = r.z, b = r.r of the scaled FP32 residual
+            // a = r.z, b = r.r of the scaled FP32 residual.  The direction
+            // update p = D r + beta p happens also on a refresh step: the
+            // refreshed iteration continues along the new direction.
             w.iters += 1;
             const double rn = sqrt(b);
-            if (!finite(b)) {
+            if (!isfin(a) || !isfin(b)) {
                 w.status = GRMP_ERR_NONFINITE;
                 w.phase = PH_FAIL;
-            } else if (rn <= w.rtol * w.bnorm * w.scale || rn < w.delta * w.rmax || w.iters >= w.maxit) {
-                w.phase = PH_UPDATE;  // converged estimate or reliable update
             } else {
                 w.beta = a / w.rz;
                 w.rz = a;
                 w.rmax = rn > w.rmax ? rn : w.rmax;
+                if (rn <= w.rtol * w.bnorm * w.scale || rn < w.delta * w.rmax || w.iters >= w.maxit)
+                    w.phase = PH_UPDATE;  // converged estimate or reliable update
             }
         } else if (ph == PH_UPDATE) {
             w.phase = PH_RESID;
@@ -575,7 +583,7 @@ __global__ __launch_bounds__(KB) void k_eng_reduce(KProb *__restrict__ P, const 
                 w.bnorm = sqrt(a);
             }
             w.relres = w.bb > 0.0 ? sqrt(a / w.bb) : 0.0;
-            if (!finite(a)) {
+            if (!isfin(a)) {
                 w.status = GRMP_ERR_NONFINITE;
                 w.phase = PH_FAIL;
             } else if (a == 0.0 || w.relres <= w.rtol) {
@@ -585,9 +593,11 @@ __global__ __launch_bounds__(KB) void k_eng_reduce(KProb *__restrict__ P, const 
                 w.phase = PH_FAIL;
             } else {
                 const double snew = 1.0 / sqrt(a);
-                w.pscale = (w.outer == 0) ? 1.0 : snew / w.scale;
+                const bool fresh = w.outer == 0 || w.restart;
+                w.pscale = fresh ? 1.0 : snew / w.scale;
                 w.scale = snew;
-                w.phase = (w.outer == 0) ? PH_INIT : PH_REINIT;
+                w.phase = fresh ? PH_INIT : PH_REINIT;
+                w.restart = 0;
                 w.outer += 1;
             }
         }
@@ -656,6 +666,7 @@ int KEngine::add(const KProb &p, int *slot)
     q.scale = 1.0;
     q.pscale = 1.0;
     q.rmax = 0.0;
+    q.restart = 0;
     init_[s] = q;
     // the host copy lives in init_ until the stream has consumed it
     GRMP_HIP(ctx, hipMemcpyAsync(dP_ + s, &init_[s], sizeof(KProb), hipMemcpyHostToDevice, st_));

@@ -72,9 +72,9 @@ class ExtendableGrid:
 
     def __del__(self):
         try:
-            if self.h:
+            if self.h and _abi.alive():
                 lib().grmp_mesh_destroy(self.h)
-                self.h = None
+            self.h = None
         except Exception:
             pass
 
@@ -134,9 +134,9 @@ class FESpace:
 
     def __del__(self):
         try:
-            if self.h:
+            if self.h and _abi.alive():
                 lib().grmp_space_destroy(self.h)
-                self.h = None
+            self.h = None
         except Exception:
             pass
 
@@ -239,9 +239,9 @@ class FEMatrix:
 
     def __del__(self):
         try:
-            if self.h:
+            if self.h and _abi.alive():
                 lib().grmp_matrix_destroy(self.h)
-                self.h = None
+            self.h = None
         except Exception:
             pass
 

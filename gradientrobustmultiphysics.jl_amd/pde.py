@@ -195,7 +195,7 @@ def LinearElasticityProblem(E=1.0, nu=0.3, cell_coefficient=None):
 class SolverConfig:
     """Keyword configuration of solve (src/solvers.jl:38-59)."""
 
-    def __init__(self, precision="fp64", rtol=1e-10, maxiterations=1000000, inner_rtol=1e-4,
+    def __init__(self, precision="fp64", rtol=1e-10, maxiterations=1000000, inner_rtol=0.1,
                  fixed_penalty=1e60):
         self.precision, self.rtol, self.maxiterations = precision, rtol, maxiterations
         self.inner_rtol, self.fixed_penalty = inner_rtol, fixed_penalty

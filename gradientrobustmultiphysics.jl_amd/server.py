@@ -80,9 +80,9 @@ class Server:
         return lib().grmp_server_clock(self.h)
 
     def close(self):
-        if self.h:
+        if self.h and _abi.alive():
             lib().grmp_server_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __enter__(self):
         return self
