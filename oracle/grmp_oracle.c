@@ -686,3 +686,359 @@ int or_num_threads(void)
     return 1;
 #endif
 }
+
+/* ============================================================== Stokes ===
+ * Bernardi-Raugel H1BR{2} x L2P0 with the BR -> RT0 reconstructed right-hand
+ * side, following:
+ *   H1BR{2} basis / dofmap "N1f1" / face-normal coefficients ... src/fedefs/h1v_br.jl:15-35,117-130,150-162
+ *   Gradient with coefficients .................................. src/feevaluator_h1.jl:77-94
+ *   Divergence with coefficients ................................ src/feevaluator_h1.jl:129-145
+ *   LagrangeMultiplier(Divergence) + transposed copy ............ src/pdeoperators.jl:214-226,
+ *                                                                  src/assemblypatterns/bilinearform.jl:348-364
+ *   HDIVRT0{2} basis, CellFaceSigns coefficients ................ src/fedefs/hdiv_rt0.jl:67-73,106-116
+ *   Piola map of ReconstructionIdentity ......................... src/feevaluator_h1.jl:342-381
+ *   BR -> RT0 reconstruction coefficients ....................... src/reconstructions.jl:381-403
+ *   Stroud conical product rule (Golub-Welsch) .................. src/quadrature.jl:528-562
+ *   FixedIntegralMean ........................................... src/globalconstraints.jl:259-283,337-351
+ * FaceNormals/FaceVolumes/CellFaceSigns come from ExtendableGrids (not in
+ * the reference tree): restated as n = (t_y, -t_x)/|t| for the face tangent
+ * t = x_b - x_a in its global orientation (a, b) (first cell that met it), and
+ * sign = +1 where the cell's local face orientation equals the global one. */
+
+void or_face_geometry(int64_t nc, int64_t nf, const double *coords, const int32_t *cells, const int32_t *cellfaces,
+                      const int32_t *facenodes, double *fnormal /*[nf][2]*/, double *fvol, int8_t *csign /*[nc][3]*/)
+{
+    for (int64_t f = 0; f < nf; ++f) {
+        int32_t a = facenodes[2 * f], b = facenodes[2 * f + 1];
+        double tx = coords[2 * b] - coords[2 * a], ty = coords[2 * b + 1] - coords[2 * a + 1];
+        double len = sqrt(tx * tx + ty * ty);
+        fvol[f] = len;
+        fnormal[2 * f] = ty / len;
+        fnormal[2 * f + 1] = -tx / len;
+    }
+    for (int64_t c = 0; c < nc; ++c)
+        for (int k = 0; k < 3; ++k) {
+            int32_t f = cellfaces[3 * c + k];
+            int32_t a = cells[3 * c + k], b = cells[3 * c + (k + 1) % 3];
+            csign[3 * c + k] = (facenodes[2 * f] == a && facenodes[2 * f + 1] == b) ? 1 : -1;
+        }
+}
+
+/* "N1f1": component-blocked node dofs (offset nnodes), then one dof per face */
+void or_br_celldofs(int64_t nc, int64_t nnodes, const int32_t *cells, const int32_t *cellfaces, int32_t *celldofs)
+{
+    for (int64_t c = 0; c < nc; ++c) {
+        for (int a = 0; a < 3; ++a) {
+            celldofs[9 * c + a] = cells[3 * c + a];
+            celldofs[9 * c + 3 + a] = (int32_t)(cells[3 * c + a] + nnodes);
+            celldofs[9 * c + 6 + a] = (int32_t)(2 * nnodes + cellfaces[3 * c + a]);
+        }
+    }
+}
+
+/* reference gradients of the BR scalar shape functions: P1 nodes 0..2, face
+ * bubbles 3..5 (6 x1 l1, 6 x1 x2, 6 l1 x2 with l1 = 1 - x1 - x2) */
+static void br_ref_ders(const double *x, double d[6][2])
+{
+    double l = 1.0;
+    l -= x[0];
+    l -= x[1];
+    p1_tri_ders(d);
+    d[3][0] = 6.0 * (l - x[0]); d[3][1] = -6.0 * x[0];
+    d[4][0] = 6.0 * x[1];       d[4][1] = 6.0 * x[0];
+    d[5][0] = -6.0 * x[1];      d[5][1] = 6.0 * (l - x[1]);
+}
+
+/* cvals of Gradient (resultdim 4) and Divergence (resultdim 1) of the 9 BR
+ * dofs at one point, in the reference's accumulation order */
+static void br_cvals(const double M[2][2], const double *x, const double n[3][2], double grad[9][4], double div[9])
+{
+    double d[6][2];
+    br_ref_ders(x, d);
+    for (int dof = 0; dof < 9; ++dof) {
+        /* scalar function and coefficient per component */
+        int sf = dof < 6 ? dof % 3 : 3 + (dof - 6);
+        for (int c = 0; c < 2; ++c) {
+            int active = dof < 6 ? (dof / 3 == c) : 1;
+            double coef = dof < 6 ? 1.0 : n[dof - 6][c];
+            for (int k = 0; k < 2; ++k) {
+                double s = 0.0;
+                for (int j = 0; j < 2; ++j) s += M[k][j] * (active ? d[sf][j] : 0.0);
+                s *= coef;
+                grad[dof][k + 2 * c] = s;
+            }
+        }
+        double dv = 0.0;
+        for (int k = 0; k < 2; ++k) {
+            int active = dof < 6 ? (dof / 3 == k) : 1;
+            double coef = dof < 6 ? 1.0 : n[dof - 6][k];
+            for (int j = 0; j < 2; ++j) dv += M[k][j] * (active ? d[sf][j] : 0.0) * coef;
+        }
+        div[dof] = dv;
+    }
+}
+
+static void cell_face_normals(int64_t c, const int32_t *cellfaces, const double *fnormal, double n[3][2])
+{
+    for (int k = 0; k < 3; ++k) {
+        int32_t f = cellfaces[3 * c + k];
+        n[k][0] = fnormal[2 * f];
+        n[k][1] = fnormal[2 * f + 1];
+    }
+}
+
+/* Stokes system blocks into matrix h: [1,1] = LaplaceOperator(nu) on BR
+ * (symmetric, quadorder 2), [1,2] = LagrangeMultiplier(Divergence) with
+ * factor -1 (quadorder 1) and its transposed copy into [2,1]. */
+void or_assemble_stokes(void *h, double nu, int64_t nc, int64_t nnodes, int64_t nf, const double *coords,
+                        const int32_t *cells, const int32_t *cellfaces, const double *fnormal)
+{
+    or_matrix *A = (or_matrix *)h;
+    const int64_t nU = 2 * nnodes + nf;
+    double X[6], M[2][2], vol;
+    /* [1,1] */
+    double xr[6], w[3];
+    int nq = or_quad_tri(2, xr, w);
+    for (int64_t c = 0; c < nc; ++c) {
+        cell_coords(2, coords, cells, c, X);
+        trafo_tri(X, M, &vol);
+        double n[3][2];
+        cell_face_normals(c, cellfaces, fnormal, n);
+        double G[3][9][4], D[3][9], L[9][9];
+        for (int q = 0; q < nq; ++q) br_cvals(M, xr + 2 * q, n, G[q], D[q]);
+        for (int a = 0; a < 9; ++a)
+            for (int b = 0; b < 9; ++b) L[a][b] = 0.0;
+        for (int q = 0; q < nq; ++q)
+            for (int a = 0; a < 9; ++a)
+                for (int b = a; b < 9; ++b) {
+                    double t = 0.0;
+                    for (int k = 0; k < 4; ++k) t += G[q][a][k] * G[q][b][k];
+                    L[a][b] += w[q] * t;
+                }
+        double itemfactor = vol * nu * 1.0;
+        int32_t dofs[9];
+        for (int a = 0; a < 3; ++a) {
+            dofs[a] = cells[3 * c + a];
+            dofs[3 + a] = (int32_t)(cells[3 * c + a] + nnodes);
+            dofs[6 + a] = (int32_t)(2 * nnodes + cellfaces[3 * c + a]);
+        }
+        for (int a = 0; a < 9; ++a)
+            for (int b = a + 1; b < 9; ++b) {
+                double v = L[a][b] * itemfactor;
+                if (v != 0.0) {
+                    lnk_add(A->L, dofs[a], dofs[b], v * 1.0);
+                    lnk_add(A->L, dofs[b], dofs[a], v * 1.0);
+                }
+            }
+        for (int a = 0; a < 9; ++a) {
+            double v = L[a][a] * itemfactor;
+            if (v != 0.0) lnk_add(A->L, dofs[a], dofs[a], v * 1.0);
+        }
+    }
+    /* [1,2] and its transposed copy */
+    nq = or_quad_tri(1, xr, w);
+    const double factor = -1.0, factor_t = -1.0;
+    for (int64_t c = 0; c < nc; ++c) {
+        cell_coords(2, coords, cells, c, X);
+        trafo_tri(X, M, &vol);
+        double n[3][2];
+        cell_face_normals(c, cellfaces, fnormal, n);
+        double G[9][4], D[9], L[9];
+        br_cvals(M, xr, n, G, D);
+        for (int a = 0; a < 9; ++a) {
+            double t = 0.0;
+            t += D[a] * 1.0;
+            L[a] = 0.0;
+            L[a] += w[0] * t;
+        }
+        double itemfactor = vol * factor * 1.0;
+        for (int a = 0; a < 3; ++a) (void)a;
+        int32_t dofs[9];
+        for (int a = 0; a < 3; ++a) {
+            dofs[a] = cells[3 * c + a];
+            dofs[3 + a] = (int32_t)(cells[3 * c + a] + nnodes);
+            dofs[6 + a] = (int32_t)(2 * nnodes + cellfaces[3 * c + a]);
+        }
+        int32_t prow = (int32_t)(nU + c);
+        for (int a = 0; a < 9; ++a) {
+            double v = L[a] * itemfactor;
+            if (v != 0.0) lnk_add(A->L, dofs[a], prow, v * 1.0);
+            double vt = L[a] * itemfactor / factor * factor_t;
+            if (vt != 0.0) lnk_add(A->L, prow, dofs[a], vt * -1.0);
+        }
+    }
+}
+
+/* ---- Stroud rule: Gauss-Legendre x Gauss-Jacobi(1,0) by Golub-Welsch,
+ * eigenpairs of the symmetric tridiagonal Jacobi matrices by cyclic Jacobi
+ * rotations, then the conical product (src/quadrature.jl:528-562). */
+static void sym_eigen(int n, double *Amat /*n x n, destroyed*/, double *evals, double *evecs /*n x n, column k = vector k*/)
+{
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) evecs[i * n + j] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = i + 1; j < n; ++j) off += Amat[i * n + j] * Amat[i * n + j];
+        if (off < 1e-60) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                double apq = Amat[p * n + q];
+                if (fabs(apq) < 1e-300) continue;
+                double theta = (Amat[q * n + q] - Amat[p * n + p]) / (2.0 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
+                for (int k = 0; k < n; ++k) {
+                    double akp = Amat[k * n + p], akq = Amat[k * n + q];
+                    Amat[k * n + p] = cs * akp - sn * akq;
+                    Amat[k * n + q] = sn * akp + cs * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double apk = Amat[p * n + k], aqk = Amat[q * n + k];
+                    Amat[p * n + k] = cs * apk - sn * aqk;
+                    Amat[q * n + k] = sn * apk + cs * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double vkp = evecs[k * n + p], vkq = evecs[k * n + q];
+                    evecs[k * n + p] = cs * vkp - sn * vkq;
+                    evecs[k * n + q] = sn * vkp + cs * vkq;
+                }
+            }
+    }
+    for (int i = 0; i < n; ++i) evals[i] = Amat[i * n + i];
+    /* ascending order */
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j)
+            if (evals[j] < evals[i]) {
+                double t = evals[i]; evals[i] = evals[j]; evals[j] = t;
+                for (int k = 0; k < n; ++k) {
+                    double v = evecs[k * n + i]; evecs[k * n + i] = evecs[k * n + j]; evecs[k * n + j] = v;
+                }
+            }
+}
+
+int or_quad_stroud(int order, double *xref /*[nq][2]*/, double *w)
+{
+    int n = order / 2 + 1;
+    double J[64], ev[8], V[64], r[8], a[8], s[8], b[8];
+    for (int i = 0; i < n * n; ++i) J[i] = 0.0;
+    for (int k = 1; k < n; ++k) {
+        double g = (double)k / sqrt(4.0 * k * k - 1.0);
+        J[(k - 1) * n + k] = J[k * n + k - 1] = g;
+    }
+    sym_eigen(n, J, ev, V);
+    for (int i = 0; i < n; ++i) { r[i] = 0.5 * ev[i] + 0.5; a[i] = 0.5 * (2.0 * V[i] * V[i]); }
+    for (int i = 0; i < n * n; ++i) J[i] = 0.0;
+    for (int k = 1; k <= n; ++k) J[(k - 1) * n + k - 1] = -1.0 / (4.0 * k * k - 1.0);
+    for (int k = 1; k < n; ++k) {
+        double g = sqrt((double)(k + 1) * k) / (2.0 * (k + 1) - 1.0);
+        J[(k - 1) * n + k] = J[k * n + k - 1] = g;
+    }
+    sym_eigen(n, J, ev, V);
+    for (int i = 0; i < n; ++i) { s[i] = 0.5 * ev[i] + 0.5; b[i] = 0.5 * (2.0 * V[i] * V[i]); }
+    /* xref[j*n + i] = (s_j, r_i (1 - s_j)), w = a_i b_j */
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            xref[2 * (j * n + i)] = s[j];
+            xref[2 * (j * n + i) + 1] = r[i] * (1.0 - s[j]);
+            w[j * n + i] = a[i] * b[j];
+        }
+    return n * n;
+}
+
+/* body force f = grad(phi) + curl(psi) with phi = sum_m A_m sin(2 pi (p_m x + q_m y) + t_m)
+ * + c0 x^3 + c1 x y^2 + c2 y^2, psi = sum_m B_m sin(2 pi (p'_m x + q'_m y) + t'_m);
+ * modes: [K][4] = (amplitude, p, q, phase) for phi, then for psi. */
+static void stokes_force(int K, const double *modes, const double *poly, const double *x, double *f)
+{
+    const double tp = 6.283185307179586;
+    double fx = 0.0, fy = 0.0;
+    for (int m = 0; m < K; ++m) {
+        const double *md = modes + 4 * m;
+        double arg = tp * (md[1] * x[0] + md[2] * x[1]) + md[3];
+        double cs = md[0] * tp * cos(arg);
+        fx += cs * md[1];
+        fy += cs * md[2];
+    }
+    for (int m = 0; m < K; ++m) {
+        const double *md = modes + 4 * (K + m);
+        double arg = tp * (md[1] * x[0] + md[2] * x[1]) + md[3];
+        double cs = md[0] * tp * cos(arg);
+        fx += cs * md[2];   /* d psi / dy */
+        fy -= cs * md[1];   /* -d psi / dx */
+    }
+    fx += 3.0 * poly[0] * x[0] * x[0] + poly[1] * x[1] * x[1];
+    fy += 2.0 * poly[1] * x[0] * x[1] + 2.0 * poly[2] * x[1];
+    f[0] = fx;
+    f[1] = fy;
+}
+
+/* b[0:nU] += LinearForm(ReconstructionIdentity{HDIVRT0{2}}, f), quadrature
+ * order `order` (Stroud for order > 2) */
+void or_assemble_stokes_rhs(double *b, int order, int K, const double *modes, const double *poly, int64_t nc,
+                            int64_t nnodes, const double *coords, const int32_t *cells, const int32_t *cellfaces,
+                            const double *fnormal, const double *fvol, const int8_t *csign)
+{
+    double xr[2 * 64], w[64];
+    int nq = order <= 2 ? or_quad_tri(order, xr, w) : or_quad_stroud(order, xr, w);
+    double X[6];
+    for (int64_t c = 0; c < nc; ++c) {
+        cell_coords(2, coords, cells, c, X);
+        double A11 = X[2] - X[0], A12 = X[4] - X[0], A21 = X[3] - X[1], A22 = X[5] - X[1];
+        double det = A11 * A22 - A12 * A21;
+        double vol = fabs(det) * 0.5;
+        double rc[9][3];
+        for (int i = 0; i < 9; ++i) for (int j = 0; j < 3; ++j) rc[i][j] = 0.0;
+        for (int f = 0; f < 3; ++f) {
+            int32_t face = cellfaces[3 * c + f];
+            for (int nn = 0; nn < 2; ++nn) {
+                int node = (f + nn) % 3;
+                for (int k = 0; k < 2; ++k) rc[3 * k + node][f] = 0.5 * fvol[face] * fnormal[2 * face + k];
+            }
+            rc[6 + f][f] = fvol[face];
+        }
+        double lb[9];
+        for (int i = 0; i < 9; ++i) lb[i] = 0.0;
+        for (int q = 0; q < nq; ++q) {
+            const double *x = xr + 2 * q;
+            /* RT0 reference basis and Piola map with CellFaceSigns */
+            double phi[3][2] = {{x[0], x[1] - 1.0}, {x[0], x[1]}, {x[0] - 1.0, x[1]}};
+            double T[3][2];
+            for (int j = 0; j < 3; ++j) {
+                double s = (double)csign[3 * c + j];
+                double t0 = 0.0, t1 = 0.0;
+                t0 += A11 * phi[j][0];
+                t0 += A12 * phi[j][1];
+                t1 += A21 * phi[j][0];
+                t1 += A22 * phi[j][1];
+                t0 *= s / det;
+                t1 *= s / det;
+                T[j][0] = t0;
+                T[j][1] = t1;
+            }
+            double cv[9][2];
+            for (int i = 0; i < 9; ++i) { cv[i][0] = 0.0; cv[i][1] = 0.0; }
+            for (int i = 0; i < 9; ++i)
+                for (int j = 0; j < 3; ++j)
+                    if (rc[i][j] != 0.0)
+                        for (int k = 0; k < 2; ++k) cv[i][k] += rc[i][j] * T[j][k];
+            /* physical point x = x1 + A xref */
+            double xp[2];
+            xp[0] = X[0] + A11 * x[0] + A12 * x[1];
+            xp[1] = X[1] + A21 * x[0] + A22 * x[1];
+            double f[2];
+            stokes_force(K, modes, poly, xp, f);
+            for (int i = 0; i < 9; ++i) {
+                double t = 0.0;
+                for (int k = 0; k < 2; ++k) t += f[k] * cv[i][k];
+                lb[i] += t * w[q];
+            }
+        }
+        double itemfactor = 1.0 * vol * 1.0;
+        for (int a = 0; a < 3; ++a) {
+            b[cells[3 * c + a]] += lb[a] * itemfactor;
+            b[cells[3 * c + a] + nnodes] += lb[3 + a] * itemfactor;
+            b[2 * nnodes + cellfaces[3 * c + a]] += lb[6 + a] * itemfactor;
+        }
+    }
+}

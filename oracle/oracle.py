@@ -218,3 +218,92 @@ def direct_solve(A, b):
     for UMFPACK (src/solvers.jl:55,656-660)."""
     import scipy.sparse.linalg as spla
     return spla.spsolve(A.to_scipy().tocsc(), b)
+
+
+# ------------------------------------------------------------------ Stokes
+
+_i8 = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
+
+
+def _stokes_lib():
+    L = lib()
+    if not hasattr(L, "_stokes_ready"):
+        L.or_face_geometry.argtypes = [_I64, _I64, _d, _i32, _i32, _i32, _d, _d, _i8]
+        L.or_br_celldofs.argtypes = [_I64, _I64, _i32, _i32, _i32]
+        L.or_assemble_stokes.argtypes = [_VP, ctypes.c_double, _I64, _I64, _I64, _d, _i32, _i32, _d]
+        L.or_quad_stroud.argtypes = [ctypes.c_int, _d, _d]
+        L.or_quad_stroud.restype = ctypes.c_int
+        L.or_assemble_stokes_rhs.argtypes = [_d, ctypes.c_int, ctypes.c_int, _d, _d, _I64, _I64, _d, _i32, _i32, _d,
+                                             _d, _i8]
+        L._stokes_ready = True
+    return L
+
+
+def quad_stroud(order):
+    xr = np.zeros(2 * 64)
+    w = np.zeros(64)
+    nq = _stokes_lib().or_quad_stroud(order, xr, w)
+    return xr[:2 * nq].reshape(nq, 2), w[:nq]
+
+
+class StokesSpace:
+    """H1BR{2} x L2P0 on the structured unit square: dofs, face geometry,
+    no-slip fixed dofs and the pinned first pressure dof (FixedIntegralMean)."""
+
+    def __init__(self, N):
+        L = _stokes_lib()
+        self.N = N
+        self.coords, self.cells = unitsquare(N)
+        self.nnodes = self.coords.shape[0]
+        self.nc = self.cells.shape[0]
+        self.cellfaces, self.facenodes, self.facecells = tri_faces(self.cells, self.nnodes)
+        self.nf = self.facenodes.shape[0]
+        self.fnormal = np.zeros(2 * self.nf)
+        self.fvol = np.zeros(self.nf)
+        self.csign = np.zeros(3 * self.nc, dtype=np.int8)
+        L.or_face_geometry(self.nc, self.nf, self.coords, self.cells, np.ascontiguousarray(self.cellfaces),
+                           np.ascontiguousarray(self.facenodes), self.fnormal, self.fvol, self.csign)
+        self.celldofs = np.zeros((self.nc, 9), dtype=np.int32)
+        L.or_br_celldofs(self.nc, self.nnodes, self.cells, np.ascontiguousarray(self.cellfaces), self.celldofs)
+        self.nU = 2 * self.nnodes + self.nf
+        self.nP = self.nc
+        self.n = self.nU + self.nP
+        bf = np.nonzero(self.facecells[:, 1] < 0)[0]
+        bn = np.unique(self.facenodes[bf].ravel())
+        self.fixed = np.unique(np.concatenate([bn, bn + self.nnodes, 2 * self.nnodes + bf, [self.nU]])).astype(np.int32)
+
+    def matrix(self, nu):
+        L = _stokes_lib()
+        h = L.or_matrix_new(self.n)
+        try:
+            L.or_assemble_stokes(h, nu, self.nc, self.nnodes, self.nf, self.coords, self.cells,
+                                 np.ascontiguousarray(self.cellfaces), self.fnormal)
+            L.or_matrix_penalize(h, self.fixed.size, self.fixed, PENALTY)
+            nnz = L.or_matrix_flush(h)
+            colptr = np.zeros(self.n + 1, dtype=np.int64)
+            rowval = np.zeros(max(nnz, 1), dtype=np.int32)
+            nzval = np.zeros(max(nnz, 1))
+            L.or_matrix_export(h, colptr, rowval, nzval)
+            return CSC(self.n, colptr, rowval[:nnz], nzval[:nnz])
+        finally:
+            L.or_matrix_free(h)
+
+    def rhs(self, order, modes, poly):
+        K = len(modes) // 2
+        b = np.zeros(self.n)
+        _stokes_lib().or_assemble_stokes_rhs(b, order, K, np.ascontiguousarray(np.asarray(modes, dtype=np.float64)
+                                                                                  .reshape(-1)),
+                                             np.ascontiguousarray(np.asarray(poly, dtype=np.float64)), self.nc,
+                                             self.nnodes, self.coords, self.cells,
+                                             np.ascontiguousarray(self.cellfaces), self.fnormal, self.fvol,
+                                             self.csign)
+        return b
+
+    def pressure_mean_shift(self, x):
+        """realize_constraint!(FixedIntegralMean(2, 0)) on P0: p -= mean(p)."""
+        vol = np.full(self.nc, 0.5 / self.N / self.N)
+        p = x[self.nU:]
+        mean = float(np.sum(p * vol) / np.sum(vol))
+        y = x.copy()
+        y[self.nU:] -= mean
+        return y
