@@ -9,13 +9,14 @@ __graft_entry__.py; inside the package, imports are relative.
 """
 from ._abi import (Context, DeviceVector, GrmpError, default_context, lib)  # noqa: F401
 from . import _abi  # noqa: F401
-from .fe import (ExtendableGrid, FEMatrix, FEType, FESpace, FEVector, H1P1, H1P2,  # noqa: F401
+from .fe import (ExtendableGrid, FEMatrix, FEType, FESpace, FEVector, H1BR, H1P1, H1P2, L2P0,  # noqa: F401
                  grid_unitcube, grid_unitsquare)
 from .pde import (APT_BilinearForm, APT_SymmetricBilinearForm, BilinearForm, CellCoefficient,  # noqa: F401
-                  DataFunction, Gradient, HomogeneousDirichletBoundary, HookStiffnessOperator2D, Identity,
-                  LaplaceOperator, LinearElasticityProblem, LinearForm, PDEDescription, PoissonProblem,
-                  ReactionOperator, SolverConfig, SymmetricGradient, assemble, assemble_linearform, solve,
-                  solve_system)
+                  DataFunction, Divergence, FixedIntegralMean, Gradient, GradientCurlData, HDIVRT0,
+                  HomogeneousDirichletBoundary, HookStiffnessOperator2D, Identity, IncompressibleNavierStokesProblem,
+                  LagrangeMultiplier, LaplaceOperator, LinearElasticityProblem, LinearForm, PDEDescription,
+                  PoissonProblem, ReactionOperator, ReconstructionIdentity, SolverConfig, SymmetricGradient, assemble,
+                  assemble_linearform, realize_constraints, solve, solve_system)
 
 from . import server  # noqa: F401
 from .server import Server  # noqa: F401

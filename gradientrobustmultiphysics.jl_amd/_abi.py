@@ -29,7 +29,7 @@ ERR_NAMES = {0: "ok", 1: "bad argument", 2: "out of device memory", 3: "HIP erro
 FE_H1P1, FE_H1P2, FE_H1BR, FE_L2P0 = 1, 2, 3, 4
 BLF_LAPLACE, BLF_HOOKE2D, BLF_MASS, BLF_DIVP0 = 1, 2, 3, 4
 COEF_FACTOR, COEF_CELL = 0, 1
-LF_CONST = 1
+LF_CONST, LF_RECO_RT0 = 1, 2
 VALUES_FP64, VALUES_FP32 = 1, 2
 SOLVE_CG, SOLVE_MINRES = 1, 2
 PREC_JACOBI = 1
@@ -57,7 +57,8 @@ class Form(ctypes.Structure):
 
 class LForm(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("quadorder", ctypes.c_int), ("factor", ctypes.c_double),
-                ("f", ctypes.c_double * 4)]
+                ("f", ctypes.c_double * 4), ("nmodes", ctypes.c_int), ("pad_", ctypes.c_int), ("modes", DP),
+                ("poly", ctypes.c_double * 4)]
 
 
 class SolveOpts(ctypes.Structure):

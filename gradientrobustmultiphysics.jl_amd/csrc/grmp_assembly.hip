@@ -17,6 +17,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "grmp_internal.h"
@@ -310,6 +311,158 @@ __device__ __forceinline__ void row_mass_tri(const DevForm &F, int64_t e, int a,
     }
 }
 
+// ------------------------------------------------------- Bernardi-Raugel
+// H1BR{2} (src/fedefs/h1v_br.jl:117-130,150-162): P1 node functions of both
+// components, then face bubbles 6 l_a l_b times the global face normal.
+// FaceNormals are restated from the face's global orientation (the first
+// cell that met it, CellFaceSigns = +1): n = (t_y, -t_x)/|t|.
+__device__ __forceinline__ void br_normals(const DevForm &F, int64_t e, double n[3][2])
+{
+    const int64_t nc = F.ncells, nn = F.nnodes;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int32_t a = F.cells[k * nc + e], b = F.cells[((k + 1) % 3) * nc + e];
+        const bool own = F.fsign[k * nc + e] > 0;
+        const int32_t ag = own ? a : b, bg = own ? b : a;
+        double tx = F.coords[bg] - F.coords[ag], ty = F.coords[nn + bg] - F.coords[nn + ag];
+        double len = sqrt(tx * tx + ty * ty);
+        n[k][0] = ty / len;
+        n[k][1] = -tx / len;
+    }
+}
+
+// reference gradient of BR scalar function sf (0..2 nodes, 3..5 bubbles
+// 6 x1 l1, 6 x1 x2, 6 l1 x2 with l1 = 1 - x1 - x2)
+__device__ __forceinline__ void br_ref_der(int sf, double x0, double x1, double &d0, double &d1)
+{
+    double l = 1.0;
+    l -= x0;
+    l -= x1;
+    if (sf < 3) {
+        d0 = p1tri_der(sf, 0);
+        d1 = p1tri_der(sf, 1);
+    } else if (sf == 3) {
+        d0 = 6.0 * (l - x0);
+        d1 = -6.0 * x0;
+    } else if (sf == 4) {
+        d0 = 6.0 * x1;
+        d1 = 6.0 * x0;
+    } else {
+        d0 = -6.0 * x1;
+        d1 = 6.0 * (l - x1);
+    }
+}
+
+// Gradient with coefficients (feevaluator_h1.jl:77-94) of BR dof `dof`:
+// g[k + 2c] = (sum_j M[k][j] D_c[j]) * coefficient[c]
+__device__ __forceinline__ void br_grad(const double M[2][2], const double n[3][2], int dof, double x0, double x1,
+                                        double g[4])
+{
+    const int sf = dof < 6 ? dof % 3 : dof - 3;
+    double d0, d1;
+    br_ref_der(sf, x0, x1, d0, d1);
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const bool active = dof < 6 ? (dof / 3 == c) : true;
+        const double coef = dof < 6 ? 1.0 : (dof == 6 ? n[0][c] : (dof == 7 ? n[1][c] : n[2][c]));
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            double s = 0.0;
+            s += M[k][0] * (active ? d0 : 0.0);
+            s += M[k][1] * (active ? d1 : 0.0);
+            s *= coef;
+            g[k + 2 * c] = s;
+        }
+    }
+}
+
+// Divergence with coefficients (feevaluator_h1.jl:129-145)
+__device__ __forceinline__ double br_div(const double M[2][2], const double n[3][2], int dof, double x0, double x1)
+{
+    const int sf = dof < 6 ? dof % 3 : dof - 3;
+    double d0, d1;
+    br_ref_der(sf, x0, x1, d0, d1);
+    double dv = 0.0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const bool active = dof < 6 ? (dof / 3 == k) : true;
+        const double coef = dof < 6 ? 1.0 : (dof == 6 ? n[0][k] : (dof == 7 ? n[1][k] : n[2][k]));
+        dv += M[k][0] * (active ? d0 : 0.0) * coef;
+        dv += M[k][1] * (active ? d1 : 0.0) * coef;
+    }
+    return dv;
+}
+
+// LaplaceOperator(nu) on H1BR{2}: symmetric, NoAction, face-midpoint rule
+template <int NQ>
+__device__ __forceinline__ void row_laplace_br(const DevForm &F, int64_t e, int a, double *v)
+{
+    double M[2][2], vol, n[3][2];
+    trafo_tri(F, e, M, vol);
+    br_normals(F, e, n);
+    const double itemfactor = vol * F.factor * 1.0;
+    double ga[NQ][4];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double x0, x1, w;
+        qtri<NQ>(q, x0, x1, w);
+        br_grad(M, n, a, x0, x1, ga[q]);
+    }
+#pragma unroll
+    for (int b = 0; b < 9; ++b) {
+        double L = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double x0, x1, w;
+            qtri<NQ>(q, x0, x1, w);
+            double gb[4];
+            br_grad(M, n, b, x0, x1, gb);
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) t += (a <= b ? ga[q][k] * gb[k] : gb[k] * ga[q][k]);
+            L += w * t;
+        }
+        v[b] = L * itemfactor;
+    }
+}
+
+// LagrangeMultiplier(Divergence) block [BR rows, P0 column]: midpoint rule,
+// P0 basis 1, itemfactor = vol * factor (factor -1)
+__device__ __forceinline__ void row_div_brp0(const DevForm &F, int64_t e, int a, double *v)
+{
+    double M[2][2], vol, n[3][2];
+    trafo_tri(F, e, M, vol);
+    br_normals(F, e, n);
+    double x0, x1, w;
+    qtri<1>(0, x0, x1, w);
+    double t = 0.0;
+    t += br_div(M, n, a, x0, x1) * 1.0;
+    double L = 0.0;
+    L += w * t;
+    v[0] = L * (vol * F.factor * 1.0);
+}
+
+// its transposed copy [P0 row, BR columns]:
+// (L * itemfactor / factor * factor_t) * -1 (bilinearform.jl:358-363)
+__device__ __forceinline__ void row_divt_p0br(const DevForm &F, int64_t e, int a, double *v)
+{
+    (void)a;
+    double M[2][2], vol, n[3][2];
+    trafo_tri(F, e, M, vol);
+    br_normals(F, e, n);
+    double x0, x1, w;
+    qtri<1>(0, x0, x1, w);
+    const double itemfactor = vol * F.factor * 1.0;
+#pragma unroll
+    for (int b = 0; b < 9; ++b) {
+        double t = 0.0;
+        t += br_div(M, n, b, x0, x1) * 1.0;
+        double L = 0.0;
+        L += w * t;
+        v[b] = (L * itemfactor / F.factor * F.factor_t) * -1.0;
+    }
+}
+
 // form-kind traits: number of column dofs and the row function
 template <int CN_, void (*ROW)(const DevForm &, int64_t, int, double *)>
 struct Kind {
@@ -324,6 +477,9 @@ using K_HOOKE1 = Kind<6, row_hooke_tri<1>>;
 using K_HOOKE3 = Kind<6, row_hooke_tri<3>>;
 using K_MASS_P1T1 = Kind<3, row_mass_tri<1, 1>>;
 using K_MASS_P1T3 = Kind<3, row_mass_tri<1, 3>>;
+using K_LAP_BR = Kind<9, row_laplace_br<3>>;
+using K_DIV_BRP0 = Kind<1, row_div_brp0>;
+using K_DIVT_P0BR = Kind<9, row_divt_p0br>;
 
 // runs the statement(s) with KT bound to the kind type of form F
 #define GRMP_KCASE(CODE, TYPE, ...) \
@@ -341,6 +497,9 @@ using K_MASS_P1T3 = Kind<3, row_mass_tri<1, 3>>;
         GRMP_KCASE(FK_HOOKE_P1VTRI * 4 + 1, K_HOOKE3, __VA_ARGS__)                \
         GRMP_KCASE(FK_MASS_P1TRI * 4 + 0, K_MASS_P1T1, __VA_ARGS__)               \
         GRMP_KCASE(FK_MASS_P1TRI * 4 + 1, K_MASS_P1T3, __VA_ARGS__)               \
+        GRMP_KCASE(FK_LAPLACE_BR * 4 + 1, K_LAP_BR, __VA_ARGS__)                  \
+        GRMP_KCASE(FK_DIV_BRP0 * 4 + 0, K_DIV_BRP0, __VA_ARGS__)                  \
+        GRMP_KCASE(FK_DIVT_P0BR * 4 + 0, K_DIVT_P0BR, __VA_ARGS__)                \
     default: break;                                                               \
     }
 
@@ -555,7 +714,17 @@ int form_resolve(grmp_matrix *A, FormRec &r)
     } else if (f.kind == GRMP_BLF_MASS && rs == cs && rs->ncomp == 1 && dim == 2 && rs->fe == GRMP_FE_H1P1) {
         r.kind = FK_MASS_P1TRI;
         r.nq = quad_nq_tri(f.quadorder);
+    } else if (f.kind == GRMP_BLF_LAPLACE && rs == cs && rs->fe == GRMP_FE_H1BR && dim == 2 && f.quadorder == 2 &&
+               f.coef_mode == GRMP_COEF_FACTOR) {
+        r.kind = FK_LAPLACE_BR;
+        r.nq = 3;
+    } else if (f.kind == GRMP_BLF_DIVP0 && rs->fe == GRMP_FE_H1BR && cs->fe == GRMP_FE_L2P0 && dim == 2 &&
+               f.quadorder <= 1 && f.coef_mode == GRMP_COEF_FACTOR) {
+        r.kind = FK_DIV_BRP0;
+        r.nq = 1;
     }
+    GRMP_CHECK(ctx, (r.kind != FK_LAPLACE_BR && r.kind != FK_DIV_BRP0) || rs->mesh->fsign, GRMP_ERR_UNSUPPORTED,
+               "H1BR forms need the mesh's CellFaceSigns (generated meshes carry them)");
     GRMP_CHECK(ctx, r.kind != 0 && r.nq != 0, GRMP_ERR_UNSUPPORTED,
                "form kind %d (quadorder %d) on FE %d/%d with %d components in %dD is not built", f.kind,
                f.quadorder, rs->fe, cs->fe, rs->ncomp, dim);
@@ -577,6 +746,8 @@ void fill_devform(const grmp_matrix *A, const FormRec &r, DevForm &d)
     d.cnloc = r.cs->nloc;
     d.words = r.words;
     d.factor = r.f.factor;
+    d.factor_t = r.f.factor;
+    d.fsign = m->fsign;
     d.lam = r.f.params[0];
     d.mu = r.f.params[1];
     d.kappa = d.coef_cell ? r.f.cell_coef->d : nullptr;
@@ -654,6 +825,20 @@ extern "C" int grmp_matrix_add_form(grmp_matrix *A, const grmp_form *form)
     std::lock_guard<std::mutex> g(ctx->mu);
     matrix_free_pattern(A);
     A->forms.push_back(r);
+    if (r.kind == FK_DIV_BRP0) {
+        // LagrangeMultiplier: transposed copy into the mirrored block
+        // (src/pdeoperators.jl:214-226, assemble! with At)
+        FormRec t = r;
+        t.f.row_block = r.f.col_block;
+        t.f.col_block = r.f.row_block;
+        t.rs = r.cs;
+        t.cs = r.rs;
+        t.kind = FK_DIVT_P0BR;
+        t.nq = 1;
+        t.pos = nullptr;
+        t.words = (t.cs->nloc + 3) / 4;
+        A->forms.push_back(t);
+    }
     return GRMP_OK;
 }
 
@@ -987,12 +1172,247 @@ int linearform_launch(grmp_ctx *ctx, const grmp_space *s, const grmp_lform *form
     return GRMP_OK;
 }
 
+// ------------------------------------------ reconstructed Stokes right-hand side
+// LinearForm(ReconstructionIdentity{HDIVRT0{2}}, f) on H1BR{2}
+// (src/feevaluator_h1.jl:342-381, src/reconstructions.jl:381-403): the test
+// function is replaced by its RT0 reconstruction sum_f rc[a][f] Piola(RT0_f),
+// rc = |F|/2 n_k(F) for the node dofs of face F, |F| for its bubble.
+
+#define RECO_MAXQ 25
+#define RECO_MAXMODES 16
+struct RecoDev {
+    int nq, K;
+    double xq[RECO_MAXQ][2], wq[RECO_MAXQ];
+    double modes[2 * RECO_MAXMODES][4];
+    double poly[4];
+    double factor;
+    const double *coords;
+    const int32_t *cells;
+    const int8_t *fsign;
+    int64_t nnodes, ncells;
+    const int32_t *inc_ptr, *inc;
+};
+
+// symmetric eigenpairs by cyclic Jacobi rotations (n <= 8), ascending
+static void jacobi_eigen(int n, double *A, double *ev, double *V)
+{
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) V[i * n + j] = i == j ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = i + 1; j < n; ++j) off += A[i * n + j] * A[i * n + j];
+        if (off < 1e-60) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = A[p * n + q];
+                if (std::fabs(apq) < 1e-300) continue;
+                const double th = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < n; ++k) {
+                    const double x = A[k * n + p], y = A[k * n + q];
+                    A[k * n + p] = c * x - sn * y;
+                    A[k * n + q] = sn * x + c * y;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double x = A[p * n + k], y = A[q * n + k];
+                    A[p * n + k] = c * x - sn * y;
+                    A[q * n + k] = sn * x + c * y;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double x = V[k * n + p], y = V[k * n + q];
+                    V[k * n + p] = c * x - sn * y;
+                    V[k * n + q] = sn * x + c * y;
+                }
+            }
+    }
+    for (int i = 0; i < n; ++i) ev[i] = A[i * n + i];
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j)
+            if (ev[j] < ev[i]) {
+                std::swap(ev[i], ev[j]);
+                for (int k = 0; k < n; ++k) std::swap(V[k * n + i], V[k * n + j]);
+            }
+}
+
+// Triangle rule of the given order: orders <= 2 as qtri, otherwise the
+// Stroud conical product of Gauss-Legendre and Gauss-Jacobi(1,0) points
+// (src/quadrature.jl:528-562), both from their Jacobi matrices (Golub-Welsch).
+static int tri_rule(int order, double (*xq)[2], double *wq)
+{
+    if (order <= 1) {
+        xq[0][0] = xq[0][1] = 1.0 / 3.0;
+        wq[0] = 1.0;
+        return 1;
+    }
+    if (order == 2) {
+        const double pts[3][2] = {{0.5, 0.5}, {0.0, 0.5}, {0.5, 0.0}};
+        for (int q = 0; q < 3; ++q) {
+            xq[q][0] = pts[q][0];
+            xq[q][1] = pts[q][1];
+            wq[q] = 1.0 / 3.0;
+        }
+        return 3;
+    }
+    const int n = order / 2 + 1;
+    if (n * n > RECO_MAXQ) return 0;
+    double J[64], ev[8], V[64], r[8], a[8], sj[8], bj[8];
+    for (int i = 0; i < n * n; ++i) J[i] = 0.0;
+    for (int k = 1; k < n; ++k) J[(k - 1) * n + k] = J[k * n + k - 1] = (double)k / std::sqrt(4.0 * k * k - 1.0);
+    jacobi_eigen(n, J, ev, V);
+    for (int i = 0; i < n; ++i) {
+        r[i] = 0.5 * ev[i] + 0.5;
+        a[i] = 0.5 * (2.0 * V[i] * V[i]);
+    }
+    for (int i = 0; i < n * n; ++i) J[i] = 0.0;
+    for (int k = 1; k <= n; ++k) J[(k - 1) * n + k - 1] = -1.0 / (4.0 * k * k - 1.0);
+    for (int k = 1; k < n; ++k)
+        J[(k - 1) * n + k] = J[k * n + k - 1] = std::sqrt((double)(k + 1) * k) / (2.0 * (k + 1) - 1.0);
+    jacobi_eigen(n, J, ev, V);
+    for (int i = 0; i < n; ++i) {
+        sj[i] = 0.5 * ev[i] + 0.5;
+        bj[i] = 0.5 * (2.0 * V[i] * V[i]);
+    }
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            xq[j * n + i][0] = sj[j];
+            xq[j * n + i][1] = r[i] * (1.0 - sj[j]);
+            wq[j * n + i] = a[i] * bj[j];
+        }
+    return n * n;
+}
+
+__device__ __forceinline__ void stokes_force(const RecoDev &R, double x, double y, double &fx, double &fy)
+{
+    const double tp = 6.283185307179586;
+    fx = 0.0;
+    fy = 0.0;
+    for (int m = 0; m < R.K; ++m) {
+        const double *md = R.modes[m];
+        const double cs = md[0] * tp * cos(tp * (md[1] * x + md[2] * y) + md[3]);
+        fx += cs * md[1];
+        fy += cs * md[2];
+    }
+    for (int m = 0; m < R.K; ++m) {
+        const double *md = R.modes[R.K + m];
+        const double cs = md[0] * tp * cos(tp * (md[1] * x + md[2] * y) + md[3]);
+        fx += cs * md[2];
+        fy -= cs * md[1];
+    }
+    fx += 3.0 * R.poly[0] * x * x + R.poly[1] * y * y;
+    fy += 2.0 * R.poly[1] * x * y + 2.0 * R.poly[2] * y;
+}
+
+__device__ double reco_local(const RecoDev &R, int64_t e, int a)
+{
+    const int64_t nc = R.ncells, nn = R.nnodes;
+    int32_t v[3];
+    double X[3], Y[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        v[k] = R.cells[k * nc + e];
+        X[k] = R.coords[v[k]];
+        Y[k] = R.coords[nn + v[k]];
+    }
+    const double A11 = X[1] - X[0], A12 = X[2] - X[0], A21 = Y[1] - Y[0], A22 = Y[2] - Y[0];
+    const double det = A11 * A22 - A12 * A21;
+    const double vol = fabs(det) * 0.5;
+    double sgn[3], rc[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        const bool own = R.fsign[f * nc + e] > 0;
+        sgn[f] = own ? 1.0 : -1.0;
+        const int ia = own ? f : (f + 1) % 3, ib = own ? (f + 1) % 3 : f;
+        const double tx = X[ib] - X[ia], ty = Y[ib] - Y[ia];
+        const double len = sqrt(tx * tx + ty * ty);
+        const double nk[2] = {ty / len, -tx / len};
+        if (a < 6) {
+            const int node = a % 3, k = a / 3;
+            const bool on = node == f || node == (f + 1) % 3;
+            rc[f] = on ? 0.5 * len * nk[k] : 0.0;
+        } else {
+            rc[f] = (a - 6 == f) ? len : 0.0;
+        }
+    }
+    double lb = 0.0;
+    for (int q = 0; q < R.nq; ++q) {
+        const double x0 = R.xq[q][0], x1 = R.xq[q][1];
+        const double phi[3][2] = {{x0, x1 - 1.0}, {x0, x1}, {x0 - 1.0, x1}};
+        double cv0 = 0.0, cv1 = 0.0;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            if (rc[f] == 0.0) continue;
+            double t0 = 0.0, t1 = 0.0;
+            t0 += A11 * phi[f][0];
+            t0 += A12 * phi[f][1];
+            t1 += A21 * phi[f][0];
+            t1 += A22 * phi[f][1];
+            const double sd = sgn[f] / det;
+            t0 *= sd;
+            t1 *= sd;
+            cv0 += rc[f] * t0;
+            cv1 += rc[f] * t1;
+        }
+        const double xp = X[0] + A11 * x0 + A12 * x1, yp = Y[0] + A21 * x0 + A22 * x1;
+        double fx, fy;
+        stokes_force(R, xp, yp, fx, fy);
+        double t = 0.0;
+        t += fx * cv0;
+        t += fy * cv1;
+        lb += t * R.wq[q];
+    }
+    return lb * (R.factor * vol * 1.0);
+}
+
+__global__ void k_linearform_reco(RecoDev R, double *__restrict__ b, int64_t ndofs)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndofs) return;
+    double acc = b[i];
+    for (int32_t t = R.inc_ptr[i]; t < R.inc_ptr[i + 1]; ++t) {
+        const int32_t pk = R.inc[t];
+        const int64_t e = pk / 9;
+        acc += reco_local(R, e, pk - (int)e * 9);
+    }
+    b[i] = acc;
+}
+
+static int linearform_reco_launch(grmp_ctx *ctx, const grmp_space *s, const grmp_lform *form, double *bp, hipStream_t st)
+{
+    const grmp_mesh *m = s->mesh;
+    GRMP_CHECK(ctx, s->fe == GRMP_FE_H1BR && m->fsign, GRMP_ERR_UNSUPPORTED,
+               "ReconstructionIdentity{HDIVRT0{2}} needs H1BR{2} on a mesh with CellFaceSigns");
+    GRMP_CHECK(ctx, form->nmodes >= 0 && form->nmodes <= RECO_MAXMODES && (form->nmodes == 0 || form->modes),
+               GRMP_ERR_ARG, "RECO_RT0 form: at most %d modes", RECO_MAXMODES);
+    RecoDev R;
+    memset(&R, 0, sizeof(R));
+    R.nq = tri_rule(form->quadorder, R.xq, R.wq);
+    GRMP_CHECK(ctx, R.nq > 0, GRMP_ERR_UNSUPPORTED, "RECO_RT0 form: quadrature order %d too high", form->quadorder);
+    R.K = form->nmodes;
+    for (int m2 = 0; m2 < 2 * R.K; ++m2)
+        for (int k = 0; k < 4; ++k) R.modes[m2][k] = form->modes[4 * m2 + k];
+    for (int k = 0; k < 4; ++k) R.poly[k] = form->poly[k];
+    R.factor = form->factor;
+    R.coords = m->coords;
+    R.cells = m->cells;
+    R.fsign = m->fsign;
+    R.nnodes = m->nnodes;
+    R.ncells = m->ncells;
+    R.inc_ptr = s->inc_ptr;
+    R.inc = s->inc;
+    hipLaunchKernelGGL(k_linearform_reco, dim3(nblk(s->ndofs, 128)), dim3(128), 0, st, R, bp, s->ndofs);
+    GRMP_HIP(ctx, hipGetLastError());
+    return GRMP_OK;
+}
+
 extern "C" int grmp_assemble_linearform(grmp_ctx *ctx, const grmp_space *s, const grmp_lform *form, grmp_vector *b,
                                         int64_t offset)
 {
     GRMP_CHECK(ctx, ctx && s && form && b, GRMP_ERR_ARG, "grmp_assemble_linearform: null argument");
     GRMP_CHECK(ctx, offset >= 0 && offset + s->ndofs <= b->n, GRMP_ERR_ARG, "grmp_assemble_linearform: vector too short");
     std::lock_guard<std::mutex> g(ctx->mu);
+    if (form->kind == GRMP_LF_RECO_RT0) return linearform_reco_launch(ctx, s, form, b->d + offset, ctx->stream);
     return linearform_launch(ctx, s, form, b->d + offset, ctx->stream);
 }
 
@@ -1031,6 +1451,23 @@ __global__ void k_integrate_identity(LfDev L, const int32_t *__restrict__ dofs, 
     }
 }
 
+// P0 (NoAction, one quadrature point of weight 1): out[c] = (0 + u_c*1) * 1 * vol
+__global__ void k_integrate_p0(LfDev L, const double *__restrict__ u, double *__restrict__ out)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= L.ncells) return;
+    const int64_t nc = L.ncells, nn = L.nnodes;
+    int32_t v0 = L.cells[e], v1 = L.cells[nc + e], v2 = L.cells[2 * nc + e];
+    double A11 = L.coords[v1] - L.coords[v0], A12 = L.coords[v2] - L.coords[v0];
+    double A21 = L.coords[nn + v1] - L.coords[nn + v0], A22 = L.coords[nn + v2] - L.coords[nn + v0];
+    double vol = fabs(A11 * A22 - A12 * A21) * 0.5;
+    double in = 0.0;
+    in += u[e] * 1.0;
+    double acc = 0.0;
+    acc += in * 1.0 * vol;
+    out[e] = acc;
+}
+
 extern "C" int grmp_integrate_identity(grmp_ctx *ctx, const grmp_space *s, int quadorder, const grmp_vector *u,
                                        int64_t offset, double *out)
 {
@@ -1038,8 +1475,9 @@ extern "C" int grmp_integrate_identity(grmp_ctx *ctx, const grmp_space *s, int q
     GRMP_CHECK(ctx, offset >= 0 && offset + s->ndofs <= u->n, GRMP_ERR_ARG, "grmp_integrate_identity: vector too short");
     const grmp_mesh *m = s->mesh;
     int nq = quad_nq_tri(quadorder);
-    GRMP_CHECK(ctx, m->dim == 2 && (s->fe == GRMP_FE_H1P1 || s->fe == GRMP_FE_H1P2) && nq, GRMP_ERR_UNSUPPORTED,
-               "grmp_integrate_identity: FE %d in %dD with quadorder %d not built", s->fe, m->dim, quadorder);
+    GRMP_CHECK(ctx, m->dim == 2 && (s->fe == GRMP_FE_H1P1 || s->fe == GRMP_FE_H1P2 || s->fe == GRMP_FE_L2P0) && nq,
+               GRMP_ERR_UNSUPPORTED, "grmp_integrate_identity: FE %d in %dD with quadorder %d not built", s->fe, m->dim,
+               quadorder);
     std::lock_guard<std::mutex> g(ctx->mu);
     LfDev L;
     memset(&L, 0, sizeof(L));
@@ -1052,7 +1490,8 @@ extern "C" int grmp_integrate_identity(grmp_ctx *ctx, const grmp_space *s, int q
     GRMP_TRY(dalloc(ctx, &d, m->ncells * s->ncomp));
     dim3 grid(nblk(m->ncells, 256)), blk(256);
     const double *up = u->d + offset;
-    if (s->fe == GRMP_FE_H1P1 && nq == 1) hipLaunchKernelGGL((k_integrate_identity<1, 1>), grid, blk, 0, ctx->stream, L, s->celldofs, up, d);
+    if (s->fe == GRMP_FE_L2P0) hipLaunchKernelGGL(k_integrate_p0, grid, blk, 0, ctx->stream, L, up, d);
+    else if (s->fe == GRMP_FE_H1P1 && nq == 1) hipLaunchKernelGGL((k_integrate_identity<1, 1>), grid, blk, 0, ctx->stream, L, s->celldofs, up, d);
     else if (s->fe == GRMP_FE_H1P1) hipLaunchKernelGGL((k_integrate_identity<1, 3>), grid, blk, 0, ctx->stream, L, s->celldofs, up, d);
     else if (nq == 1) hipLaunchKernelGGL((k_integrate_identity<2, 1>), grid, blk, 0, ctx->stream, L, s->celldofs, up, d);
     else hipLaunchKernelGGL((k_integrate_identity<2, 3>), grid, blk, 0, ctx->stream, L, s->celldofs, up, d);

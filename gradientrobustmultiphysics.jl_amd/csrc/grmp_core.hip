@@ -247,7 +247,9 @@ __global__ void k_aos_to_soa_i32_base1(const int32_t *in, int32_t *out, int64_t 
     }
 }
 
-__global__ void k_unitsquare(int N, double *coords, int32_t *cells)
+// CellFaceSigns of the unit-square triangulation: +1 where the cell is the
+// first to meet the face (its local orientation is the face's), else -1.
+__global__ void k_unitsquare(int N, double *coords, int32_t *cells, int8_t *fsign)
 {
     int64_t n1 = N + 1, nn = n1 * n1, nc = 2ll * N * N;
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -263,6 +265,15 @@ __global__ void k_unitsquare(int N, double *coords, int32_t *cells)
         cells[t] = v00;
         cells[nc + t] = upper ? v11 : v10;
         cells[2 * nc + t] = upper ? v01 : v11;
+        if (!upper) {
+            fsign[t] = j == 0 ? 1 : -1;
+            fsign[nc + t] = 1;
+            fsign[2 * nc + t] = 1;
+        } else {
+            fsign[t] = -1;
+            fsign[nc + t] = 1;
+            fsign[2 * nc + t] = i == 0 ? 1 : -1;
+        }
     }
 }
 
@@ -307,6 +318,7 @@ static grmp_mesh *mesh_new(grmp_ctx *ctx, int dim, int64_t nnodes, int64_t ncell
     m->N = 0;
     m->coords = nullptr;
     m->cells = nullptr;
+    m->fsign = nullptr;
     return m;
 }
 
@@ -315,6 +327,7 @@ extern "C" int grmp_mesh_destroy(grmp_mesh *m)
     if (!m) return GRMP_OK;
     dfree(m->ctx, m->coords);
     dfree(m->ctx, m->cells);
+    dfree(m->ctx, m->fsign);
     delete m;
     return GRMP_OK;
 }
@@ -369,14 +382,15 @@ extern "C" int grmp_mesh_unitsquare(grmp_ctx *ctx, int N, grmp_mesh **out)
     int64_t nn = (int64_t)(N + 1) * (N + 1), nc = 2ll * N * N;
     grmp_mesh *m = mesh_new(ctx, 2, nn, nc, 3);
     int rc;
-    if ((rc = dalloc(ctx, &m->coords, 2 * nn)) || (rc = dalloc(ctx, &m->cells, 3 * nc))) {
+    if ((rc = dalloc(ctx, &m->coords, 2 * nn)) || (rc = dalloc(ctx, &m->cells, 3 * nc)) ||
+        (rc = dalloc(ctx, &m->fsign, 3 * nc))) {
         grmp_mesh_destroy(m);
         return rc;
     }
     m->kind = MESH_UNITSQUARE;
     m->N = N;
     int64_t tot = nn > nc ? nn : nc;
-    hipLaunchKernelGGL(k_unitsquare, dim3(nblk(tot, 256)), dim3(256), 0, ctx->stream, N, m->coords, m->cells);
+    hipLaunchKernelGGL(k_unitsquare, dim3(nblk(tot, 256)), dim3(256), 0, ctx->stream, N, m->coords, m->cells, m->fsign);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) { grmp_mesh_destroy(m); return grmp_fail(ctx, GRMP_ERR_HIP, "k_unitsquare: %s", hipGetErrorString(e)); }
     *out = m;
@@ -510,6 +524,38 @@ __global__ void k_celldofs_p2_square(const int32_t *cells, int32_t *dofs, int64_
     }
 }
 
+// H1BR{2} "N1f1": node dofs of both components (offset nnodes), then one
+// dof per face (offset 2 nnodes), faces numbered by first appearance
+__global__ void k_celldofs_br_square(const int32_t *cells, int32_t *dofs, int64_t nc, int N, int64_t nnodes)
+{
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    int64_t sq = c >> 1, j = sq / N, i = sq % N;
+    bool upper = c & 1;
+    int64_t f[3];
+    if (!upper) {
+        f[0] = sq_H(N, i, j);
+        f[1] = sq_V(N, i + 1, j);
+        f[2] = sq_D(N, i, j);
+    } else {
+        f[0] = sq_D(N, i, j);
+        f[1] = sq_H(N, i, j + 1);
+        f[2] = sq_V(N, i, j);
+    }
+    for (int a = 0; a < 3; ++a) {
+        dofs[(int64_t)a * nc + c] = cells[(int64_t)a * nc + c];
+        dofs[(int64_t)(3 + a) * nc + c] = (int32_t)(cells[(int64_t)a * nc + c] + nnodes);
+        dofs[(int64_t)(6 + a) * nc + c] = (int32_t)(2 * nnodes + f[a]);
+    }
+}
+
+// L2P0 "I1": the cell's own dof
+__global__ void k_celldofs_p0(int32_t *dofs, int64_t nc)
+{
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nc) dofs[c] = (int32_t)c;
+}
+
 __global__ void k_inc_count(const int32_t *dofs, int64_t nc, int nloc, int32_t *cnt)
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -635,9 +681,13 @@ extern "C" int grmp_space_upload(grmp_ctx *ctx, const grmp_mesh *mesh, int fe, i
 extern "C" int grmp_space_create(grmp_ctx *ctx, const grmp_mesh *mesh, int fe, int ncomp, grmp_space **out)
 {
     GRMP_CHECK(ctx, ctx && mesh && out && ncomp >= 1 && ncomp <= 3, GRMP_ERR_ARG, "grmp_space_create: bad arguments");
-    GRMP_CHECK(ctx, fe == GRMP_FE_H1P1 || (fe == GRMP_FE_H1P2 && mesh->kind == MESH_UNITSQUARE), GRMP_ERR_UNSUPPORTED,
-               "grmp_space_create: device dofmaps exist for H1P1 on any mesh and H1P2 on unit-square meshes; "
-               "upload FES[CellDofs] with grmp_space_upload otherwise");
+    GRMP_CHECK(ctx, fe == GRMP_FE_H1P1 || fe == GRMP_FE_L2P0 ||
+                        ((fe == GRMP_FE_H1P2 || fe == GRMP_FE_H1BR) && mesh->kind == MESH_UNITSQUARE),
+               GRMP_ERR_UNSUPPORTED,
+               "grmp_space_create: device dofmaps exist for H1P1/L2P0 on any mesh and H1P2/H1BR on unit-square "
+               "meshes; upload FES[CellDofs] with grmp_space_upload otherwise");
+    GRMP_CHECK(ctx, fe != GRMP_FE_H1BR || ncomp == 2, GRMP_ERR_ARG, "grmp_space_create: H1BR{2} has 2 components");
+    GRMP_CHECK(ctx, fe != GRMP_FE_L2P0 || ncomp == 1, GRMP_ERR_UNSUPPORTED, "grmp_space_create: L2P0 with 1 component");
     std::lock_guard<std::mutex> g(ctx->mu);
     grmp_space *s = new grmp_space();
     s->ctx = ctx;
@@ -650,6 +700,15 @@ extern "C" int grmp_space_create(grmp_ctx *ctx, const grmp_mesh *mesh, int fe, i
         s->nloc = nv * ncomp;
         s->ndofs = mesh->nnodes * ncomp;
         s->nfaces = 0;
+    } else if (fe == GRMP_FE_L2P0) {
+        s->nloc = 1;
+        s->ndofs = nc;
+        s->nfaces = 0;
+    } else if (fe == GRMP_FE_H1BR) {
+        int64_t N = mesh->N;
+        s->nfaces = 3 * N * N + 2 * N;
+        s->nloc = 9;
+        s->ndofs = 2 * mesh->nnodes + s->nfaces;
     } else {
         int64_t N = mesh->N;
         s->nfaces = 3 * N * N + 2 * N;
@@ -661,6 +720,11 @@ extern "C" int grmp_space_create(grmp_ctx *ctx, const grmp_mesh *mesh, int fe, i
     if (fe == GRMP_FE_H1P1)
         hipLaunchKernelGGL(k_celldofs_p1, dim3(nblk(nc, 256)), dim3(256), 0, ctx->stream, mesh->cells, s->celldofs, nc,
                            nv, ncomp, mesh->nnodes);
+    else if (fe == GRMP_FE_L2P0)
+        hipLaunchKernelGGL(k_celldofs_p0, dim3(nblk(nc, 256)), dim3(256), 0, ctx->stream, s->celldofs, nc);
+    else if (fe == GRMP_FE_H1BR)
+        hipLaunchKernelGGL(k_celldofs_br_square, dim3(nblk(nc, 256)), dim3(256), 0, ctx->stream, mesh->cells,
+                           s->celldofs, nc, mesh->N, mesh->nnodes);
     else
         hipLaunchKernelGGL(k_celldofs_p2_square, dim3(nblk(nc, 256)), dim3(256), 0, ctx->stream, mesh->cells,
                            s->celldofs, nc, mesh->N, ncomp, mesh->nnodes, s->nfaces);
@@ -704,6 +768,34 @@ extern "C" int grmp_space_boundary_dofs(const grmp_space *s, int64_t *count, int
                "grmp_space_boundary_dofs: boundary enumeration exists for generated meshes only");
     std::vector<int64_t> b;
     int64_t N = m->N, n1 = N + 1;
+    if (s->fe == GRMP_FE_L2P0) {
+        *count = 0;  // no dofs on faces
+        return GRMP_OK;
+    }
+    if (m->kind == MESH_UNITSQUARE && s->fe == GRMP_FE_H1BR) {
+        // BFaceDofs "N1i1": both components of boundary nodes, and the bubble of each boundary face
+        std::vector<int64_t> nodes;
+        for (int64_t j = 0; j <= N; ++j)
+            for (int64_t i = 0; i <= N; ++i)
+                if (i == 0 || j == 0 || i == N || j == N) nodes.push_back(j * n1 + i);
+        for (int64_t v : nodes) {
+            b.push_back(v);
+            b.push_back(v + m->nnodes);
+        }
+        for (int64_t i = 0; i < N; ++i) {
+            b.push_back(2 * m->nnodes + sq_H(N, i, 0));
+            b.push_back(2 * m->nnodes + sq_H(N, i, N));
+        }
+        for (int64_t j = 0; j < N; ++j) {
+            b.push_back(2 * m->nnodes + sq_V(N, 0, j));
+            b.push_back(2 * m->nnodes + sq_V(N, N, j));
+        }
+        std::sort(b.begin(), b.end());
+        *count = (int64_t)b.size();
+        if (out)
+            for (size_t q = 0; q < b.size(); ++q) out[q] = b[q] + 1;
+        return GRMP_OK;
+    }
     if (m->kind == MESH_UNITSQUARE) {
         for (int64_t j = 0; j <= N; ++j)
             for (int64_t i = 0; i <= N; ++i)

@@ -79,6 +79,7 @@ struct grmp_mesh {
     int32_t *cells;      // SoA [nloc][ncells], 0-based
     int kind;            // MeshKind
     int N;               // structured resolution
+    int8_t *fsign;       // SoA [3][ncells] CellFaceSigns (triangles; nullptr if unknown)
 };
 
 struct grmp_space {
@@ -102,6 +103,9 @@ enum FormKind {
     FK_HOOKE_P1VTRI = 4,
     FK_MASS_P1TRI = 5,
     FK_MASS_P2TRI = 6,
+    FK_LAPLACE_BR = 7,   // (nu grad u, grad v) on H1BR{2}
+    FK_DIV_BRP0 = 8,     // LagrangeMultiplier(Divergence): rows BR, columns P0
+    FK_DIVT_P0BR = 9,    // its transposed copy: rows P0, columns BR
 };
 
 struct DevForm {
@@ -112,8 +116,10 @@ struct DevForm {
     int words;           // slot words per incidence
     int pad_;
     double factor;
+    double factor_t;     // transposed-copy factor (FK_DIVT_P0BR)
     double lam, mu;
     const double *kappa;
+    const int8_t *fsign; // CellFaceSigns of the mesh (H1BR)
     const double *coords;
     const int32_t *cells;
     int64_t nnodes, ncells;
@@ -180,6 +186,7 @@ int init_x_launch(const grmp_matrix *A, const double *val64, const double *b, do
 // b[i] = 0 on fixed rows (homogeneous Dirichlet, penalty * 0)
 int zero_fixed_launch(const grmp_matrix *A, double *b, hipStream_t st);
 int plain_cg_solve(grmp_matrix *A, const double *b, double *x, const grmp_solve_opts &o, grmp_solve_info &info);
+int minres_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solve_opts &o, grmp_solve_info &info);
 
 // timing
 double wall_seconds();

@@ -16,66 +16,7 @@
 
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-#define KB 256
-
-// ------------------------------------------------------------ reductions
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    return v;
-}
-
-// block sums of (a, b) in a fixed order; valid in thread 0
-__device__ __forceinline__ void block_sum2(double &a, double &b)
-{
-    __shared__ double sa[KB / 64], sb[KB / 64];
-    a = wave_sum(a);
-    b = wave_sum(b);
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    if (l == 0) {
-        sa[w] = a;
-        sb[w] = b;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double x = 0.0, y = 0.0;
-#pragma unroll
-        for (int k = 0; k < KB / 64; ++k) {
-            x += sa[k];
-            y += sb[k];
-        }
-        a = x;
-        b = y;
-    }
-    __syncthreads();
-}
-
-// row i of a CSR matrix times x with L lanes per row; full sum in lane 0
-template <int L, class V, class X>
-__device__ __forceinline__ double csr_row(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                                          const V *__restrict__ v, const X *__restrict__ x, int i, int lane)
-{
-    double s = 0.0;
-    const int32_t k1 = rp[i + 1];
-    for (int32_t k = rp[i] + lane; k < k1; k += L) s += (double)v[k] * (double)x[ci[k]];
-#pragma unroll
-    for (int o = L / 2; o > 0; o >>= 1) s += __shfl_down(s, o, L);
-    return s;
-}
-
-// FP32 variant accumulating in FP32 (the inner solve's arithmetic)
-template <int L>
-__device__ __forceinline__ float csr_row32(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                                           const float *__restrict__ v, const float *__restrict__ x, int i, int lane)
-{
-    float s = 0.0f;
-    const int32_t k1 = rp[i + 1];
-    for (int32_t k = rp[i] + lane; k < k1; k += L) s += v[k] * x[ci[k]];
-#pragma unroll
-    for (int o = L / 2; o > 0; o >>= 1) s += __shfl_down(s, o, L);
-    return s;
-}
+#include "grmp_sparse.h"
 
 int engine_pick_lanes(double nnz_per_row)
 {
@@ -909,11 +850,13 @@ extern "C" int grmp_solve(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, 
     memset(&inf, 0, sizeof(inf));
     GRMP_CHECK(ctx, A->have_pattern && A->val64, GRMP_ERR_ARG, "grmp_solve: matrix not assembled in FP64");
     GRMP_CHECK(ctx, b->n == A->n && x->n == A->n, GRMP_ERR_ARG, "grmp_solve: vector length != %lld", (long long)A->n);
-    GRMP_CHECK(ctx, o.method == GRMP_SOLVE_CG, GRMP_ERR_UNSUPPORTED, "grmp_solve: only CG is built");
+    GRMP_CHECK(ctx, o.method == GRMP_SOLVE_CG || o.method == GRMP_SOLVE_MINRES, GRMP_ERR_UNSUPPORTED,
+               "grmp_solve: method %d is not built", o.method);
     std::lock_guard<std::mutex> g(ctx->mu);
     double t0 = wall_seconds();
     int rc;
-    if (o.precision == GRMP_PRECISION_FP32_IR) rc = solve_engine(A, b, x, o, inf);
+    if (o.method == GRMP_SOLVE_MINRES) rc = minres_solve(A, b->d, x->d, o, inf);
+    else if (o.precision == GRMP_PRECISION_FP32_IR) rc = solve_engine(A, b, x, o, inf);
     else rc = plain_cg_solve(A, b->d, x->d, o, inf);
     inf.seconds = wall_seconds() - t0;
     if (info) *info = inf;
@@ -1002,6 +945,11 @@ extern "C" int grmp_solve_csc(grmp_ctx *ctx, int64_t n, const int64_t *colptr, c
     }
     grmp_solve_opts o;
     grmp_solve_opts_default(&o);
+    for (int64_t i = 0; i < n; ++i)
+        if (!fx[i] && di64[i] == 0.0) {
+            o.method = GRMP_SOLVE_MINRES;  // constraint rows: the system is a saddle point
+            break;
+        }
     o.precision = precision == GRMP_PRECISION_FP32_IR ? GRMP_PRECISION_FP32_IR : GRMP_PRECISION_FP64;
     o.rtol = rtol > 0 ? rtol : 1e-10;
     grmp_solve_info info;

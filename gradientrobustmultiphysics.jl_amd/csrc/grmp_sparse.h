@@ -1,0 +1,67 @@
+// Device helpers shared by the Krylov solvers: fixed-order block
+// reductions and the lane-split CSR row product.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#define KB 256
+
+// ------------------------------------------------------------ reductions
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// block sums of (a, b) in a fixed order; valid in thread 0
+__device__ __forceinline__ void block_sum2(double &a, double &b)
+{
+    __shared__ double sa[KB / 64], sb[KB / 64];
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) {
+        sa[w] = a;
+        sb[w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double x = 0.0, y = 0.0;
+#pragma unroll
+        for (int k = 0; k < KB / 64; ++k) {
+            x += sa[k];
+            y += sb[k];
+        }
+        a = x;
+        b = y;
+    }
+    __syncthreads();
+}
+
+// row i of a CSR matrix times x with L lanes per row; full sum in lane 0
+template <int L, class V, class X>
+__device__ __forceinline__ double csr_row(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                          const V *__restrict__ v, const X *__restrict__ x, int i, int lane)
+{
+    double s = 0.0;
+    const int32_t k1 = rp[i + 1];
+    for (int32_t k = rp[i] + lane; k < k1; k += L) s += (double)v[k] * (double)x[ci[k]];
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) s += __shfl_down(s, o, L);
+    return s;
+}
+
+// FP32 variant accumulating in FP32 (the inner solve's arithmetic)
+template <int L>
+__device__ __forceinline__ float csr_row32(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                           const float *__restrict__ v, const float *__restrict__ x, int i, int lane)
+{
+    float s = 0.0f;
+    const int32_t k1 = rp[i + 1];
+    for (int32_t k = rp[i] + lane; k < k1; k += L) s += v[k] * x[ci[k]];
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) s += __shfl_down(s, o, L);
+    return s;
+}
+

@@ -34,6 +34,21 @@ def H1P2(ncomp=1, edim=2):
     return FEType(_abi.FE_H1P2, ncomp, 2, f"H1P2{{{ncomp},{edim}}}")
 
 
+def H1BR(ncomp=2):
+    """Bernardi-Raugel H1BR{2}: P1 vector field plus one normal face bubble per
+    face (src/fedefs/h1v_br.jl); polynomial order 2 on triangles."""
+    if ncomp != 2:
+        raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "H1BR is built for ncomp = 2")
+    return FEType(_abi.FE_H1BR, 2, 2, "H1BR{2}")
+
+
+def L2P0(ncomp=1):
+    """Piecewise constants L2P0{1} (src/fedefs/l2v_p0.jl)."""
+    if ncomp != 1:
+        raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "L2P0 is built for ncomp = 1")
+    return FEType(_abi.FE_L2P0, 1, 0, "L2P0{1}")
+
+
 # -------------------------------------------------------------------- grids
 
 class ExtendableGrid:

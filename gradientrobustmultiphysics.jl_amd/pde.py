@@ -18,9 +18,21 @@ from . import _abi
 from .fe import FEMatrix, FEVector
 from ._abi import lib
 
-Identity, Gradient, SymmetricGradient = "Identity", "Gradient", "SymmetricGradient"
+Identity, Gradient, SymmetricGradient, Divergence = "Identity", "Gradient", "SymmetricGradient", "Divergence"
 APT_SymmetricBilinearForm, APT_BilinearForm = "symmetric", "full"
-_DERIV = {Identity: 0, Gradient: 1, SymmetricGradient: 1}
+_DERIV = {Identity: 0, Gradient: 1, SymmetricGradient: 1, Divergence: 1}
+HDIVRT0 = "HDIVRT0{2}"
+
+
+class ReconstructionIdentity:
+    """ReconstructionIdentity{FER}: the test function's Identity replaced by
+    its reconstruction in FER (src/functionoperators.jl, evaluated by
+    src/feevaluator_h1.jl:342-381); HDIVRT0 on H1BR{2} is built."""
+
+    def __init__(self, target=HDIVRT0):
+        if target != HDIVRT0:
+            raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, f"ReconstructionIdentity{{{target}}} is not built")
+        self.target = target
 
 
 class DataFunction:
@@ -57,16 +69,19 @@ class _BLF(PDEOperator):
         self.kind, self.operators, self.factor, self.action = kind, operators, factor, action
         self.APT, self.params, self.name = apt, params or [0.0, 0.0], name
 
-    def quadorder(self, fes):
-        q = sum(fes.FEType.order - _DERIV[o] for o in self.operators)
+    def quadorder(self, fes, fes_col=None):
+        """Sum of the polynomial orders minus one per derivative plus the
+        action's bonus order (src/assemblypatterns.jl:559-565)."""
+        fes_col = fes if fes_col is None else fes_col
+        q = (fes.FEType.order - _DERIV[self.operators[0]]) + (fes_col.FEType.order - _DERIV[self.operators[1]])
         if self.action is not None:
             q += self.action.bonus_quadorder
         return max(q, 0)
 
-    def form(self, fes, rb, cb, ctx):
+    def form(self, fes, rb, cb, ctx, fes_col=None):
         f = _abi.Form()
         f.kind = self.kind
-        f.quadorder = self.quadorder(fes)
+        f.quadorder = self.quadorder(fes, fes_col)
         f.row_block, f.col_block = rb, cb
         f.factor = float(self.factor)
         for k, p in enumerate(self.params[:4]):
@@ -101,6 +116,16 @@ def HookStiffnessOperator2D(mu, lam, cell_coefficient=None, name="auto"):
                 apt=APT_BilinearForm, params=[lam, mu], name=name)
 
 
+def LagrangeMultiplier(operator=Divergence, factor=-1.0, name="auto"):
+    """LagrangeMultiplier(Divergence) (src/pdeoperators.jl:214-226): the
+    block (velocity rows, pressure columns) gets factor * (div u, q) and the
+    transposed block gets the copy with the opposite sign
+    (src/assemblypatterns/bilinearform.jl:358-363)."""
+    if operator != Divergence:
+        raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, f"LagrangeMultiplier({operator}) is not built")
+    return _BLF(_abi.BLF_DIVP0, [Divergence, Identity], factor=factor, apt=APT_BilinearForm, name=name)
+
+
 def BilinearForm(operators, action=None, APT=APT_SymmetricBilinearForm, factor=1.0, name="auto"):
     """BilinearForm([Gradient, Gradient], CellCoefficient(kappa); APT)
     (src/pdeoperators.jl:355-409) for the operator pairs built on the device."""
@@ -111,21 +136,52 @@ def BilinearForm(operators, action=None, APT=APT_SymmetricBilinearForm, factor=1
     raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, f"BilinearForm{tuple(operators)} is not built")
 
 
+class GradientCurlData:
+    """Right-hand side f = grad(phi) + curl(psi) in closed form, the device's
+    stand-in for a DataFunction closure (src/userdata.jl:3-63):
+    phi = sum_m A_m sin(2 pi (p_m x + q_m y) + t_m) + c0 x^3 + c1 x y^2 + c2 y^2,
+    psi = the same sum over the second set of modes.  ``modes_phi`` and
+    ``modes_psi`` are (K, 4) arrays (A, p, q, t) with the same K."""
+
+    def __init__(self, modes_phi, modes_psi, poly=(0.0, 0.0, 0.0), bonus_quadorder=2):
+        a = np.asarray(modes_phi, dtype=np.float64).reshape(-1, 4)
+        b = np.asarray(modes_psi, dtype=np.float64).reshape(-1, 4)
+        if a.shape != b.shape:
+            raise _abi.GrmpError(_abi.ERR_ARG, "GradientCurlData: phi and psi need the same number of modes")
+        self.modes = np.ascontiguousarray(np.concatenate([a, b]))
+        self.K = a.shape[0]
+        self.poly = [float(v) for v in poly] + [0.0] * (4 - len(poly))
+        self.bonus_quadorder = bonus_quadorder
+
+
 class LinearForm(PDEOperator):
-    """LinearForm(Identity, DataFunction(f)) (src/pdeoperators.jl:694-705)."""
+    """LinearForm(Identity, DataFunction(f)) (src/pdeoperators.jl:694-705),
+    and LinearForm(ReconstructionIdentity(HDIVRT0), GradientCurlData) on
+    H1BR{2}, the pressure-robust Stokes right-hand side."""
 
     def __init__(self, operator, data, factor=1.0, name="auto"):
-        if operator != Identity or not isinstance(data, DataFunction):
-            raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "only LinearForm(Identity, constant DataFunction) is built")
+        reco = isinstance(operator, ReconstructionIdentity)
+        if not ((operator == Identity and isinstance(data, DataFunction)) or
+                (reco and isinstance(data, GradientCurlData))):
+            raise _abi.GrmpError(_abi.ERR_UNSUPPORTED,
+                                 "LinearForm(Identity, constant DataFunction) and "
+                                 "LinearForm(ReconstructionIdentity(HDIVRT0), GradientCurlData) are built")
         self.operator, self.data, self.factor, self.name = operator, data, factor, name
 
     def lform(self, fes):
         lf = _abi.LForm()
-        lf.kind = _abi.LF_CONST
         lf.quadorder = fes.FEType.order + self.data.bonus_quadorder
         lf.factor = float(self.factor)
-        for k, v in enumerate(self.data.c[:4]):
-            lf.f[k] = float(v)
+        if isinstance(self.operator, ReconstructionIdentity):
+            lf.kind = _abi.LF_RECO_RT0
+            lf.nmodes = self.data.K
+            lf.modes = self.data.modes.ctypes.data_as(_abi.DP)
+            for k, v in enumerate(self.data.poly[:4]):
+                lf.poly[k] = v
+        else:
+            lf.kind = _abi.LF_CONST
+            for k, v in enumerate(self.data.c[:4]):
+                lf.f[k] = float(v)
         return lf
 
 
@@ -141,9 +197,18 @@ class HomogeneousDirichletBoundary:
     (src/boundarydata.jl:198-257)."""
 
 
+class FixedIntegralMean:
+    """FixedIntegralMean(unknown, value) (src/globalconstraints.jl:23-44):
+    the unknown's first dof is fixed during the solve (apply, :259-283) and
+    the integral mean is shifted to ``value`` afterwards (realize, :337-351)."""
+
+    def __init__(self, unknown, value=0.0):
+        self.unknown, self.value = unknown, float(value)
+
+
 class PDEDescription:
-    """Unknowns, LHS operators per block, RHS operators per block and
-    boundary data (src/pdedescription.jl:40-284)."""
+    """Unknowns, LHS operators per block, RHS operators per block, boundary
+    data and global constraints (src/pdedescription.jl:40-284)."""
 
     def __init__(self, name="PDE"):
         self.name = name
@@ -151,6 +216,7 @@ class PDEDescription:
         self.LHS = {}
         self.RHS = {}
         self.BoundaryOperators = {}
+        self.GlobalConstraints = []
 
     def add_unknown(self, name):
         self.unknowns.append(name)
@@ -167,6 +233,9 @@ class PDEDescription:
 
     def add_boundarydata(self, j, data=HomogeneousDirichletBoundary):
         self.BoundaryOperators[j] = data
+
+    def add_constraint(self, c):
+        self.GlobalConstraints.append(c)
 
 
 def PoissonProblem(diffusion=1.0, cell_coefficient=None):
@@ -192,21 +261,66 @@ def LinearElasticityProblem(E=1.0, nu=0.3, cell_coefficient=None):
     return pde
 
 
+def IncompressibleNavierStokesProblem(dim=2, viscosity=1.0, nonlinear=False):
+    """The linear Stokes prototype (src/pdeprototypes.jl:17-55): unknowns
+    velocity and pressure, LaplaceOperator(viscosity) on the velocity,
+    LagrangeMultiplier(Divergence) coupling, no-slip boundary data and
+    FixedIntegralMean(pressure, 0)."""
+    if dim != 2 or nonlinear:
+        raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "only the linear 2D Stokes problem is built")
+    pde = PDEDescription("incompressible Stokes problem")
+    pde.add_unknown("velocity")
+    pde.add_unknown("pressure")
+    pde.add_operator((0, 0), LaplaceOperator(viscosity))
+    pde.add_operator((0, 1), LagrangeMultiplier(Divergence))
+    pde.add_boundarydata(0, HomogeneousDirichletBoundary)
+    pde.add_constraint(FixedIntegralMean(1, 0.0))
+    return pde
+
+
 class SolverConfig:
     """Keyword configuration of solve (src/solvers.jl:38-59)."""
 
     def __init__(self, precision="fp64", rtol=1e-10, maxiterations=1000000, inner_rtol=0.1,
-                 fixed_penalty=1e60):
+                 fixed_penalty=1e60, minres_inner_rtol=1e-4):
         self.precision, self.rtol, self.maxiterations = precision, rtol, maxiterations
         self.inner_rtol, self.fixed_penalty = inner_rtol, fixed_penalty
+        self.minres_inner_rtol = minres_inner_rtol
 
 
 def fixed_dofs(pde, Solution):
+    """Boundary dofs of every unknown with boundary data, plus the first dof
+    of every unknown under a FixedIntegralMean."""
     out = []
     for j in sorted(pde.BoundaryOperators):
         fes = Solution.FES[j]
         out.append(fes.boundary_dofs() + int(Solution.offsets[j]))
+    for c in pde.GlobalConstraints:
+        out.append(np.array([int(Solution.offsets[c.unknown])], dtype=np.int64))
     return np.unique(np.concatenate(out)) if out else np.zeros(0, dtype=np.int64)
+
+
+def realize_constraints(pde, Solution):
+    """realize_constraint!(FixedIntegralMean) (src/globalconstraints.jl:337-351):
+    shift the unknown by (value - integral mean); the cell integrals come from
+    the device ItemIntegrator."""
+    for c in pde.GlobalConstraints:
+        fes = Solution.FES[c.unknown]
+        off = int(Solution.offsets[c.unknown])
+        ints = np.empty(fes.xgrid.ncells * fes.FEType.ncomp)
+        Solution.ctx.check(lib().grmp_integrate_identity(Solution.ctx.h, fes.h, 2 * fes.FEType.order, Solution.vec.h,
+                                                         off, _abi.dptr(ints)))
+        area = float(np.sum(fes.xgrid.cellvolumes()))
+        shift = c.value - float(np.sum(ints)) / area
+        x = Solution.entries()
+        x[off:off + fes.ndofs] += shift
+        Solution.set_entries(x)
+
+
+def _saddle_point(pde, Solution):
+    """True when an unknown has no operator on its diagonal block (a Lagrange
+    multiplier), which makes the system indefinite."""
+    return any(not pde.LHS.get((j, j)) for j in range(len(Solution.FES)))
 
 
 def assemble(pde, Solution, values=_abi.VALUES_FP64, penalty=1e60):
@@ -215,7 +329,7 @@ def assemble(pde, Solution, values=_abi.VALUES_FP64, penalty=1e60):
     A = FEMatrix(Solution.FES)
     for (i, j), ops in sorted(pde.LHS.items()):
         for op in ops:
-            form, keep = op.form(Solution.FES[i], i, j, A.ctx)
+            form, keep = op.form(Solution.FES[i], i, j, A.ctx, Solution.FES[j])
             A.add_form(form, keep)
     fixed = fixed_dofs(pde, Solution)
     A.set_fixed(fixed, penalty)
@@ -238,16 +352,21 @@ def solve(Solution, pde, config=None, **kwargs):
     fp32 = cfg.precision == "fp32_ir"
     values = _abi.VALUES_FP64 | (_abi.VALUES_FP32 if fp32 else 0)
     A, b, fixed = assemble(pde, Solution, values, cfg.fixed_penalty)
-    return solve_system(A, b, Solution, cfg), A, b
+    method = _abi.SOLVE_MINRES if _saddle_point(pde, Solution) else _abi.SOLVE_CG
+    info = solve_system(A, b, Solution, cfg, method)
+    realize_constraints(pde, Solution)
+    return info, A, b
 
 
-def solve_system(A, b, Solution, cfg):
+def solve_system(A, b, Solution, cfg, method=_abi.SOLVE_CG):
     o = _abi.SolveOpts()
     lib().grmp_solve_opts_default(ctypes.byref(o))
+    o.method = method
     o.precision = _abi.PRECISION_FP32_IR if cfg.precision == "fp32_ir" else _abi.PRECISION_FP64
     o.rtol = cfg.rtol
     o.maxit = int(cfg.maxiterations)
-    o.inner_rtol = cfg.inner_rtol
+    o.inner_rtol = cfg.inner_rtol if method == _abi.SOLVE_CG else cfg.minres_inner_rtol
+    o.check_every = 16 if method == _abi.SOLVE_MINRES else 1
     info = _abi.SolveInfo()
     rc = lib().grmp_solve(A.h, b.vec.h, Solution.vec.h, ctypes.byref(o), ctypes.byref(info))
     if rc not in (_abi.OK,):

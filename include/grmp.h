@@ -59,6 +59,8 @@ extern "C" {
 
 /* linear forms */
 #define GRMP_LF_CONST 1         /* LinearForm(Identity, DataFunction(f)) :694-705 */
+#define GRMP_LF_RECO_RT0 2      /* LinearForm(ReconstructionIdentity{HDIVRT0{2}}, f) on H1BR{2},
+                                   f = grad(phi) + curl(psi) (x-dependent, see grmp_lform) */
 
 /* value precision flags of an assembled matrix */
 #define GRMP_VALUES_FP64 1
@@ -90,22 +92,33 @@ typedef struct grmp_form {
 } grmp_form;
 
 typedef struct grmp_lform {
-    int kind;                 /* GRMP_LF_CONST */
-    int quadorder;
+    int kind;                 /* GRMP_LF_CONST / GRMP_LF_RECO_RT0 */
+    int quadorder;            /* RECO_RT0: 2 + bonus order of f (Stroud rule above 2) */
     double factor;
     double f[4];              /* constant data, ncomp values */
+    int nmodes;               /* RECO_RT0: phi and psi are sums of nmodes sine modes each */
+    int pad_;
+    const double *modes;      /* [2*nmodes][4] = (amplitude, p, q, phase): A sin(2 pi (p x + q y) + t),
+                                 phi's modes first; host pointer, copied */
+    double poly[4];           /* phi += poly0 x^3 + poly1 x y^2 + poly2 y^2 */
 } grmp_lform;
 
 typedef struct grmp_solve_opts {
-    int method;               /* GRMP_SOLVE_CG / GRMP_SOLVE_MINRES */
-    int precond;              /* GRMP_PREC_JACOBI */
+    int method;               /* GRMP_SOLVE_CG (symmetric positive definite systems) /
+                                 GRMP_SOLVE_MINRES (saddle points: free rows without a diagonal
+                                 are constraint rows, negated to make the system symmetric) */
+    int precond;              /* GRMP_PREC_JACOBI (MINRES: 1/|A_ii|, and on constraint rows
+                                 1 / sum_j A_ij^2 / A_jj) */
     int precision;            /* GRMP_PRECISION_FP64 / GRMP_PRECISION_FP32_IR */
-    double rtol;              /* ||b - Ax|| <= rtol ||b|| over free dofs */
+    double rtol;              /* CG: ||b - Ax|| <= rtol ||b - A x0|| over free dofs;
+                                 MINRES: the same in the preconditioner's norm */
     int64_t maxit;            /* total Krylov iterations */
-    double inner_rtol;        /* FP32_IR: reliable-update threshold, the FP64 residual is
-                                 recomputed when the FP32 one has dropped by this factor */
-    int64_t inner_maxit;      /* unused, kept for layout */
-    int check_every;          /* plain mode: host residual check period (1 = every iteration) */
+    double inner_rtol;        /* FP32_IR CG: reliable-update threshold, the FP64 residual is
+                                 recomputed when the FP32 one has dropped by this factor;
+                                 FP32_IR MINRES: relative tolerance of each FP32 inner solve */
+    int64_t inner_maxit;      /* FP32_IR MINRES: iteration cap of each inner solve */
+    int check_every;          /* MINRES: status poll period in iterations (results do not
+                                 depend on it) */
 } grmp_solve_opts;
 
 typedef struct grmp_solve_info {
@@ -209,7 +222,7 @@ int grmp_assemble_linearform(grmp_ctx *ctx, const grmp_space *space, const grmp_
 /* ------------------------------------------------------ ItemIntegrator */
 /* evaluate(ItemIntegrator([Identity]), u): per-cell integrals of the FE
  * function u (coefficients at offset), out [ncomp, ncells] column-major
- * (src/assemblypatterns/itemintegrator.jl:169-360) */
+ * (src/assemblypatterns/itemintegrator.jl:169-360).  H1P1, H1P2, L2P0. */
 int grmp_integrate_identity(grmp_ctx *ctx, const grmp_space *space, int quadorder,
                             const grmp_vector *u, int64_t offset, double *out);
 

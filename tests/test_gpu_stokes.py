@@ -1,0 +1,133 @@
+"""Stokes on the device (H1BR{2} x L2P0, LaplaceOperator(nu) +
+LagrangeMultiplier(Divergence), reconstructed BR -> RT0 right-hand side,
+MINRES) against the oracle.
+
+Stated tolerances: sparsity pattern identical; matrix values within 1e-12
+relative to the largest entry of the row (the summation order follows the
+reference's, so most are bitwise equal); right-hand side within 1e-12
+relative to max|b| (the device's sin/cos differ from the host libm in the
+last bits); solutions within 1e-8 relative of the FP64 sparse-LU solution
+after the FixedIntegralMean shift; a gradient force gives |u_h| below
+1e-10 max|p_h| (pressure robustness, examples/Example222_PressureRobustness2D.jl)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NU = 1e-2
+MODES = np.array([[0.3, 1, 0, 0.1], [-0.2, 1, 2, 1.3], [0.25, 0, 1, 0.7],   # phi
+                  [0.5, 1, 1, 0.4], [0.1, 2, -1, 2.0], [-0.3, 1, 3, 0.9]])  # psi
+POLY = [0.4, -0.3, 0.2]
+
+
+def _stokes(grmp, N, modes=MODES, poly=POLY, values=None, precision="fp64"):
+    grid = grmp.grid_unitsquare(N)
+    V = grmp.FESpace(grmp.H1BR(2), grid)
+    Q = grmp.FESpace(grmp.L2P0(1), grid)
+    pde = grmp.IncompressibleNavierStokesProblem(2, viscosity=NU)
+    K = len(modes) // 2
+    f = grmp.GradientCurlData(modes[:K], modes[K:], poly, bonus_quadorder=2)
+    pde.add_rhsdata(0, grmp.LinearForm(grmp.ReconstructionIdentity(grmp.HDIVRT0), f))
+    sol = grmp.FEVector([V, Q])
+    return grid, V, Q, pde, sol
+
+
+def test_stokes_dofmaps(grmp, O):
+    print("[case] stokes dofmaps N=6", flush=True)
+    grid, V, Q, pde, sol = _stokes(grmp, 6)
+    S = O.StokesSpace(6)
+    assert V.ndofs == S.nU and Q.ndofs == S.nP
+    assert np.array_equal(V.celldofs(), S.celldofs)
+    assert np.array_equal(Q.celldofs()[:, 0], np.arange(S.nc))
+    fixed = np.concatenate([V.boundary_dofs(), [S.nU]])
+    assert np.array_equal(np.unique(fixed), S.fixed)
+
+
+@pytest.mark.parametrize("N", [4, 13, 32])
+def test_stokes_matrix_parity(grmp, O, N):
+    print(f"[case] stokes matrix N={N}", flush=True)
+    grid, V, Q, pde, sol = _stokes(grmp, N)
+    A, b, fixed = grmp.assemble(pde, sol)
+    S = O.StokesSpace(N)
+    R = S.matrix(NU)
+    colptr, rowval, nzval = A.csc()
+    assert np.array_equal(colptr, R.colptr)
+    assert np.array_equal(rowval, R.rowval)
+    Ar = R.to_scipy().tocsr()
+    scale = np.maximum(np.abs(Ar).max(axis=1).toarray().ravel(), 1e-300)
+    diff = np.abs((A.to_scipy().tocsr() - Ar)).tocoo()
+    rel = diff.data / scale[diff.row] if diff.nnz else np.zeros(1)
+    print(f"  bitwise equal: {np.count_nonzero(nzval == R.nzval)} of {nzval.size}, max rel {rel.max():.2e}")
+    assert rel.max() <= 1e-12
+
+
+@pytest.mark.parametrize("N", [4, 16])
+def test_stokes_rhs_parity(grmp, O, N):
+    print(f"[case] stokes rhs N={N}", flush=True)
+    grid, V, Q, pde, sol = _stokes(grmp, N)
+    b = grmp.FEVector([V, Q])
+    grmp.assemble_linearform(b, 0, pde.RHS[0][0])
+    got = b.entries()
+    S = O.StokesSpace(N)
+    ref = S.rhs(2 + 2, MODES, POLY)
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    print(f"  max rel {err:.2e}")
+    assert err <= 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_stokes_minres_vs_lu(grmp, O, precision):
+    N = 24
+    print(f"[case] stokes minres {precision} N={N}", flush=True)
+    grid, V, Q, pde, sol = _stokes(grmp, N)
+    info, A, b = grmp.solve(sol, pde, precision=precision, rtol=1e-11)
+    assert info.status == 0, info.status
+    x = sol.entries()
+    S = O.StokesSpace(N)
+    bref = O.penalized_rhs(S.rhs(4, MODES, POLY), S.fixed)
+    xref = S.pressure_mean_shift(O.direct_solve(S.matrix(NU), bref))
+    err = np.linalg.norm(x - xref) / np.linalg.norm(xref)
+    print(f"  iterations {info.iterations} outer {info.outer} rel err {err:.2e}")
+    assert err <= 1e-8
+    # FixedIntegralMean: the pressure has zero mean
+    p = x[S.nU:]
+    assert abs(np.sum(p) / p.size) <= 1e-12 * max(1.0, np.max(np.abs(p)))
+
+
+def test_stokes_pressure_robust(grmp, O):
+    """A gradient force is balanced by the pressure alone: u_h = 0."""
+    N = 16
+    print(f"[case] stokes pressure robustness N={N}", flush=True)
+    grid, V, Q, pde, sol = _stokes(grmp, N, modes=np.array([[0.4, 1, 2, 0.3], [0.0, 0, 0, 0.0]]),
+                                   poly=[1.0, 0.5, -2.0])
+    info, A, b = grmp.solve(sol, pde, rtol=1e-13)
+    assert info.status == 0
+    x = sol.entries()
+    u, p = x[:V.ndofs], x[V.ndofs:]
+    print(f"  max|u| {np.max(np.abs(u)):.2e} max|p| {np.max(np.abs(p)):.2e}")
+    assert np.max(np.abs(p)) > 1e-3
+    assert np.max(np.abs(u)) <= 1e-10 * np.max(np.abs(p))
+
+
+def test_stokes_csc_plugin_picks_minres(grmp, O):
+    """The LinearSolve plug-in entry point recognises the saddle point."""
+    import ctypes
+    N = 8
+    print(f"[case] stokes csc plug-in N={N}", flush=True)
+    S = O.StokesSpace(N)
+    R = S.matrix(NU)
+    bref = O.penalized_rhs(S.rhs(4, MODES, POLY), S.fixed)
+    xref = O.direct_solve(R, bref)
+    ctx = grmp.default_context()
+    x = np.zeros(S.n)
+    cp = np.ascontiguousarray(R.colptr + 1, dtype=np.int64)
+    rv = np.ascontiguousarray(R.rowval + 1, dtype=np.int64)
+    nz = np.ascontiguousarray(R.nzval)
+    fx = np.ascontiguousarray(S.fixed + 1, dtype=np.int64)
+    lib = grmp.lib()
+    rc = lib.grmp_solve_csc(ctx.h, S.n, grmp._abi.i64ptr(cp), grmp._abi.i64ptr(rv), grmp._abi.dptr(nz),
+                            grmp._abi.dptr(bref), grmp._abi.dptr(x), fx.size, grmp._abi.i64ptr(fx),
+                            grmp._abi.PRECISION_FP64, ctypes.c_double(1e-11))
+    ctx.check(rc)
+    err = np.linalg.norm(x - xref) / np.linalg.norm(xref)
+    assert err <= 1e-8, err
