@@ -1381,8 +1381,9 @@ __global__ void k_linearform_reco(RecoDev R, double *__restrict__ b, int64_t ndo
 static int linearform_reco_launch(grmp_ctx *ctx, const grmp_space *s, const grmp_lform *form, double *bp, hipStream_t st)
 {
     const grmp_mesh *m = s->mesh;
-    GRMP_CHECK(ctx, s->fe == GRMP_FE_H1BR && m->fsign, GRMP_ERR_UNSUPPORTED,
+    GRMP_CHECK(ctx, s->fe == GRMP_FE_H1BR && s->nloc == 9 && m->dim == 2 && m->fsign, GRMP_ERR_UNSUPPORTED,
                "ReconstructionIdentity{HDIVRT0{2}} needs H1BR{2} on a mesh with CellFaceSigns");
+    GRMP_CHECK(ctx, s->inc_ptr && s->inc, GRMP_ERR_ARG, "RECO_RT0 form: the space has no dof-cell incidence");
     GRMP_CHECK(ctx, form->nmodes >= 0 && form->nmodes <= RECO_MAXMODES && (form->nmodes == 0 || form->modes),
                GRMP_ERR_ARG, "RECO_RT0 form: at most %d modes", RECO_MAXMODES);
     RecoDev R;

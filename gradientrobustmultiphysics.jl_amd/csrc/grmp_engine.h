@@ -33,6 +33,8 @@ struct KProb {
     int32_t outer;      // FP64 residual refreshes so far
     int32_t max_outer;
     int32_t restart;    // restart the search direction at the next refresh
+    int32_t discard;    // the FP32 sweep overflowed: drop its correction
+    int32_t stall;      // consecutive refreshes that did not halve the FP64 residual
     int32_t pad_;
     int64_t iters, maxit;
     double bb, bnorm;   // ||b - A x0||^2 and its root (free rows)
@@ -41,6 +43,7 @@ struct KProb {
     double scale;       // FP32 quantities are FP64 ones times scale
     double pscale;      // search-direction rescale at a refresh
     double rz, alpha, beta, relres, rmax;
+    double best;        // smallest FP64 relative residual so far
 };
 
 struct KTile {

@@ -306,20 +306,22 @@ __global__ void k_mr_init_x(const int32_t *rp, const uint8_t *dpos, const double
 // ------------------------------------------------------------ host side
 namespace {
 
+// z_k and v_k live in slot k % 2 (z_{j+1} and v_{j+1} are written element by
+// element over z_{j-1} and v_{j-1}); w_k lives in slot k % 3.
 template <class T>
 struct MrBufs {
-    T *Z[2], *V[3], *W[3], *q;
+    T *Z[2], *V[2], *W[3], *q;
 };
 
 template <class T>
 int mr_alloc(grmp_ctx *ctx, MrBufs<T> &B, T **block, int64_t n)
 {
-    GRMP_TRY(dalloc(ctx, block, 9 * n));
+    GRMP_TRY(dalloc(ctx, block, 8 * n));
     T *p = *block;
     for (int k = 0; k < 2; ++k) B.Z[k] = p + k * n;
-    for (int k = 0; k < 3; ++k) B.V[k] = p + (2 + k) * n;
-    for (int k = 0; k < 3; ++k) B.W[k] = p + (5 + k) * n;
-    B.q = p + 8 * n;
+    for (int k = 0; k < 2; ++k) B.V[k] = p + (2 + k) * n;
+    for (int k = 0; k < 3; ++k) B.W[k] = p + (4 + k) * n;
+    B.q = p + 7 * n;
     return GRMP_OK;
 }
 
@@ -362,7 +364,7 @@ int mr_run(MrHost &H, const grmp_matrix *A, const V *vals, const T *dsg, const R
     for (int64_t j = 1;; ++j) {
         T *z = B.Z[j & 1], *zprev = B.Z[(j + 1) & 1];
         T *wnew = B.W[j % 3], *wm2 = B.W[(j + 1) % 3], *wm1 = B.W[(j + 2) % 3];
-        T *vj = B.V[j % 3], *vm1 = B.V[(j + 2) % 3];
+        T *vj = B.V[j & 1], *vm1 = B.V[(j + 1) & 1];
         T *znew = zprev;  // z_{j+1} replaces z_{j-1}
 #define MR_APPLY(LL) k_mr_apply<LL, T, V, X>
         switch (H.L) {

@@ -388,8 +388,9 @@ __global__ __launch_bounds__(KB) void k_eng_elem(ENGINE_ARGS)
             const double inv = 1.0 / pr.scale;
             const float beta = (float)pr.beta;
             const bool pupd = pr.restart == 0;
+            const bool keep = pr.discard == 0;
             for (int i = tl.row0 + threadIdx.x; i < tl.row1; i += KB) {
-                pr.x[i] += inv * (double)pr.dx[i];
+                if (keep) pr.x[i] += inv * (double)pr.dx[i];
                 if (pupd) pr.p[i] = pr.dinv32[i] * pr.r[i] + beta * pr.p[i];
             }
             continue;
@@ -484,13 +485,14 @@ __global__ __launch_bounds__(KB) void k_eng_reduce(KProb *__restrict__ P, const 
         KProb &w = P[s];
         if (WHICH == RED_A) {
             // a = p.q
+            // Non-finite FP32 values with finite FP64 data are an overflow of
+            // the inner recurrence (it can diverge at the FP64 noise floor):
+            // the sweep's correction is dropped and the refresh restarts.
             const double alpha = w.rz / a;
-            if (!isfin(a)) {
-                w.status = GRMP_ERR_NONFINITE;
-                w.phase = PH_FAIL;
-            } else if (!(a > 0.0) || !isfin(alpha)) {
+            if (!(a > 0.0) || !isfin(a) || !isfin(alpha)) {
                 w.alpha = 0.0;
                 w.restart = 1;        // breakdown: refresh and restart the direction
+                w.discard = !isfin(a) || !isfin(w.rz);
                 w.phase = PH_UPDATE;
             } else {
                 w.alpha = alpha;
@@ -502,8 +504,9 @@ __global__ __launch_bounds__(KB) void k_eng_reduce(KProb *__restrict__ P, const 
             w.iters += 1;
             const double rn = sqrt(b);
             if (!isfin(a) || !isfin(b)) {
-                w.status = GRMP_ERR_NONFINITE;
-                w.phase = PH_FAIL;
+                w.restart = 1;
+                w.discard = 1;
+                w.phase = PH_UPDATE;
             } else {
                 w.beta = a / w.rz;
                 w.rz = a;
@@ -533,13 +536,25 @@ __global__ __launch_bounds__(KB) void k_eng_reduce(KProb *__restrict__ P, const 
                 w.status = GRMP_ERR_NOCONV;
                 w.phase = PH_FAIL;
             } else {
-                const double snew = 1.0 / sqrt(a);
-                const bool fresh = w.outer == 0 || w.restart;
-                w.pscale = fresh ? 1.0 : snew / w.scale;
-                w.scale = snew;
-                w.phase = fresh ? PH_INIT : PH_REINIT;
-                w.restart = 0;
-                w.outer += 1;
+                // a refresh that does not halve the residual restarts the
+                // direction; three in a row mean the tolerance lies below
+                // what the FP64 residual resolves
+                const bool stalled = w.outer > 0 && !(w.relres <= 0.5 * w.best);
+                w.stall = stalled ? w.stall + 1 : 0;
+                if (w.outer == 0 || w.relres < w.best) w.best = w.relres;
+                if (w.stall >= 3) {
+                    w.status = GRMP_ERR_NOCONV;
+                    w.phase = PH_FAIL;
+                } else {
+                    const double snew = 1.0 / sqrt(a);
+                    const bool fresh = w.outer == 0 || w.restart || stalled;
+                    w.pscale = fresh ? 1.0 : snew / w.scale;
+                    w.scale = snew;
+                    w.phase = fresh ? PH_INIT : PH_REINIT;
+                    w.restart = 0;
+                    w.discard = 0;
+                    w.outer += 1;
+                }
             }
         }
     }

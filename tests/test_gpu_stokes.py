@@ -8,7 +8,9 @@ reference's, so most are bitwise equal); right-hand side within 1e-12
 relative to max|b| (the device's sin/cos differ from the host libm in the
 last bits); solutions within 1e-8 relative of the FP64 sparse-LU solution
 after the FixedIntegralMean shift; a gradient force gives |u_h| below
-1e-10 max|p_h| (pressure robustness, examples/Example222_PressureRobustness2D.jl)."""
+1e-12 max|p_h| when the device-assembled system is solved exactly (pressure
+robustness, examples/Example222_PressureRobustness2D.jl), and below 1e-6
+max|p_h| after MINRES to 1e-13."""
 import numpy as np
 import pytest
 
@@ -95,18 +97,26 @@ def test_stokes_minres_vs_lu(grmp, O, precision):
 
 
 def test_stokes_pressure_robust(grmp, O):
-    """A gradient force is balanced by the pressure alone: u_h = 0."""
+    """A gradient force is balanced by the pressure alone: the device-assembled
+    system solved exactly gives u_h = 0 to rounding, and MINRES leaves only
+    its iteration error in u_h."""
+    import scipy.sparse.linalg as spla
     N = 16
     print(f"[case] stokes pressure robustness N={N}", flush=True)
     grid, V, Q, pde, sol = _stokes(grmp, N, modes=np.array([[0.4, 1, 2, 0.3], [0.0, 0, 0, 0.0]]),
                                    poly=[1.0, 0.5, -2.0])
+    A, b, fixed = grmp.assemble(pde, sol)
+    x = spla.spsolve(A.to_scipy().tocsc(), b.entries())
+    u, p = x[:V.ndofs], x[V.ndofs:]
+    print(f"  LU on the device system: max|u| {np.max(np.abs(u)):.2e} max|p| {np.max(np.abs(p)):.2e}")
+    assert np.max(np.abs(p)) > 1e-3
+    assert np.max(np.abs(u)) <= 1e-12 * np.max(np.abs(p))
     info, A, b = grmp.solve(sol, pde, rtol=1e-13)
     assert info.status == 0
     x = sol.entries()
-    u, p = x[:V.ndofs], x[V.ndofs:]
-    print(f"  max|u| {np.max(np.abs(u)):.2e} max|p| {np.max(np.abs(p)):.2e}")
-    assert np.max(np.abs(p)) > 1e-3
-    assert np.max(np.abs(u)) <= 1e-10 * np.max(np.abs(p))
+    u, p2 = x[:V.ndofs], x[V.ndofs:]
+    print(f"  MINRES: max|u| {np.max(np.abs(u)):.2e}, iterations {info.iterations}")
+    assert np.max(np.abs(u)) <= 1e-6 * np.max(np.abs(p2))
 
 
 def test_stokes_csc_plugin_picks_minres(grmp, O):
