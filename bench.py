@@ -69,6 +69,7 @@ def serve(grmp, mode, stream, rtol, keep=(), profile=False, respect_arrivals=Tru
     ok = np.array([r.status == 0 for r in res])
     dofs = np.array([r.ndofs for r in res])
     return dict(wall=wall, latency=lat, ok=ok, dofs=dofs, iters=np.array([r.iterations for r in res]),
+                status=np.array([r.status for r in res]),
                 outer=np.array([r.outer for r in res]), sols=sols, stats=stats,
                 service=np.array([r.t_done - r.t_admit for r in res]))
 
@@ -134,6 +135,141 @@ def cpu_baseline(stream, budget_s=15.0, rtol=1e-10):
                        "requests too large for the time budget are skipped, which favours the CPU")
 
 
+# ------------------------------------------------- single-instance workloads
+def _spmv_rate(grmp, A, values, reps=20):
+    """Wall time per SpMV over ``reps`` back-to-back launches on the context's
+    stream (launch gaps are hidden behind kernels of this size) and the
+    algorithmic bytes nnz (s_v + 4) + 4 (n + 1) + 16 n (FP64 x and y)."""
+    n, nnz, _ = A.info()
+    ctx = A.ctx
+    x = grmp.DeviceVector(ctx, n)
+    y = grmp.DeviceVector(ctx, n)
+    x.fill(1.0)
+    A.spmv(x, y, values)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        A.spmv(x, y, values)
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / reps
+    sv = 8 if values == grmp._abi.VALUES_FP64 else 4
+    nbytes = nnz * (sv + 4) + 4 * (n + 1) + 16 * n
+    gbs = nbytes / dt / 1e9
+    return dict(us=round(dt * 1e6, 1), bytes=int(nbytes), gbs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4))
+
+
+def _assembly_rate(grmp, A, values, ncells, nloc, nnodes, dim, reps=3):
+    """Element loop time (HIP events inside the library, pattern reused; best
+    of ``reps``) and the design-independent minimum traffic
+    C (4 n_loc + 8) + 8 d V + nnz s_v + 8 n (connectivity, coefficient,
+    coordinates once, values and inverse diagonal written once)."""
+    ts = []
+    for _ in range(reps):
+        A.assemble(values)
+        ts.append(A.last_assembly_seconds())
+    t = min(ts)
+    n, nnz, _ = A.info()
+    sv = (8 if values & grmp._abi.VALUES_FP64 else 0) + (4 if values & grmp._abi.VALUES_FP32 else 0)
+    nbytes = ncells * (4 * nloc + 8) + 8 * dim * nnodes + nnz * sv + 8 * n
+    gbs = nbytes / t / 1e9
+    return dict(ms=round(t * 1e3, 3), elements_per_s=round(ncells / t, 1), min_bytes=int(nbytes), gbs=round(gbs, 1),
+                frac=round(gbs / HBM_PEAK_GBS, 4))
+
+
+def run_workload(grmp, which, args, log):
+    ctx = grmp.default_context()
+    V64, V32 = grmp._abi.VALUES_FP64, grmp._abi.VALUES_FP32
+    out = {"workload": which, "n_gpus": 1, "data": "synthetic"}
+    if which in ("c2-p1", "c2-p2", "c3"):
+        if which == "c3":
+            n3 = args.size or 160
+            grid = grmp.grid_unitcube(n3, jitter=0.1, seed=3)
+            fes = grmp.FESpace(grmp.H1P1(1), grid)
+            sigma, nloc, dim = 0.5, 4, 3
+            out["config"] = {"workload": f"3D P1 Poisson, {n3}^3 x 6 Kuhn tets (interior nodes jittered +-0.1h), "
+                                         "log-normal per-cell kappa sigma=0.5", "cells": grid.ncells}
+        else:
+            fe = 1 if which == "c2-p1" else 2
+            N = args.size or 2048
+            grid = grmp.grid_unitsquare(N)
+            fes = grmp.FESpace(grmp.H1P1(1) if fe == 1 else grmp.H1P2(1), grid)
+            sigma, nloc, dim = 1.0, 3 * fe, 2
+            out["config"] = {"workload": f"2D P{fe} Poisson, {N}x{N} triangulated square, log-normal per-cell "
+                                         "kappa sigma=1, f=1", "cells": grid.ncells}
+        kap = grmp.DeviceVector(ctx, grid.ncells)
+        kap.lognormal(2024, 0.0, sigma)
+        pde = grmp.PoissonProblem(cell_coefficient=grmp.CellCoefficient(kap))
+        pde.add_rhsdata(0, grmp.LinearForm(grmp.Identity, grmp.DataFunction([1.0])))
+        pde.add_boundarydata(0)
+        sol = grmp.FEVector(fes)
+        log(f"[bench] {which}: pattern + assembly, {fes.ndofs} dofs")
+        t0 = time.perf_counter()
+        A, b, fixed = grmp.assemble(pde, sol, values=V64 | V32)
+        ctx.sync()
+        t_first = time.perf_counter() - t0
+        n, nnz, maxrow = A.info()
+        out["config"].update(ndofs=int(n), nnz=int(nnz))
+        out["first_assemble_s"] = round(t_first, 3)
+        out["assembly_fp64"] = _assembly_rate(grmp, A, V64, grid.ncells, nloc, grid.nnodes, dim)
+        out["assembly_fp32"] = _assembly_rate(grmp, A, V32, grid.ncells, nloc, grid.nnodes, dim)
+        out["assembly_fp64_fp32"] = _assembly_rate(grmp, A, V64 | V32, grid.ncells, nloc, grid.nnodes, dim)
+        log(f"[bench] {which}: spmv")
+        out["spmv_fp64"] = _spmv_rate(grmp, A, V64)
+        out["spmv_fp32"] = _spmv_rate(grmp, A, V32)
+        if which != "c3" or args.solve:
+            log(f"[bench] {which}: FP64 CG")
+            x64 = grmp.FEVector(fes)
+            i64 = grmp.solve_system(A, b, x64, grmp.SolverConfig(precision="fp64", rtol=args.rtol))
+            log(f"[bench] {which}: FP32-inner / FP64-refined CG")
+            x32 = grmp.FEVector(fes)
+            i32 = grmp.solve_system(A, b, x32, grmp.SolverConfig(precision="fp32_ir", rtol=args.rtol))
+            a, c = x64.entries(), x32.entries()
+            out["solve"] = {"rtol": args.rtol,
+                            "fp64": {"status": i64.status, "iterations": int(i64.iterations), "s": round(i64.seconds, 3),
+                                     "dof_solves_per_s": round(n / i64.seconds, 1)},
+                            "fp32_ir": {"status": i32.status, "iterations": int(i32.iterations), "outer": int(i32.outer),
+                                        "s": round(i32.seconds, 3), "dof_solves_per_s": round(n / i32.seconds, 1)},
+                            "rel_diff_fp32_ir_vs_fp64": float(np.linalg.norm(c - a) / np.linalg.norm(a))}
+    elif which == "c4":
+        N = args.size or 512
+        grid = grmp.grid_unitsquare(N)
+        Vs = grmp.FESpace(grmp.H1BR(2), grid)
+        Qs = grmp.FESpace(grmp.L2P0(1), grid)
+        pde = grmp.IncompressibleNavierStokesProblem(2, viscosity=1e-2)
+        rng = np.random.default_rng(4)
+        K = 8
+        modes = np.column_stack([rng.normal(0, 0.3, 2 * K), rng.integers(-3, 4, 2 * K), rng.integers(-3, 4, 2 * K),
+                                 rng.uniform(0, 2 * np.pi, 2 * K)])
+        f = grmp.GradientCurlData(modes[:K], modes[K:], (0.5, -0.25, 0.3), bonus_quadorder=2)
+        pde.add_rhsdata(0, grmp.LinearForm(grmp.ReconstructionIdentity(grmp.HDIVRT0), f))
+        out["config"] = {"workload": f"Stokes H1BR{{2}} x L2P0, {N}x{N} triangulated square, nu=1e-2, "
+                                     "reconstructed BR->RT0 RHS of grad phi + curl psi (K=8 seeded modes)",
+                         "cells": grid.ncells}
+        res = {}
+        xs = {}
+        for prec in ("fp64", "fp32_ir"):
+            sol = grmp.FEVector([Vs, Qs])
+            log(f"[bench] c4: {prec}")
+            t0 = time.perf_counter()
+            info, A, b = grmp.solve(sol, pde, precision=prec, rtol=args.rtol)
+            ctx.sync()
+            wall = time.perf_counter() - t0
+            n, nnz, _ = A.info()
+            res[prec] = {"status": info.status, "iterations": int(info.iterations), "outer": int(info.outer),
+                         "solve_s": round(info.seconds, 3), "wall_s": round(wall, 3),
+                         "dof_solves_per_s": round(n / info.seconds, 1)}
+            xs[prec] = sol.entries()
+            if prec == "fp64":
+                out["config"].update(ndofs=int(n), nnz=int(nnz))
+                out["assembly_fp64"] = _assembly_rate(grmp, A, V64, grid.ncells, 9, grid.nnodes, 2)
+                out["spmv_fp64"] = _spmv_rate(grmp, A, V64)
+        res["rel_diff_fp32_ir_vs_fp64"] = float(np.linalg.norm(xs["fp32_ir"] - xs["fp64"]) / np.linalg.norm(xs["fp64"]))
+        out["solve"] = res
+    else:
+        raise SystemExit(f"unknown workload {which}")
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,14 +282,21 @@ def main():
     ap.add_argument("--verify", type=int, default=12, help="requests re-solved in FP64 for the error check")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="served",
+                    help="served (the headline stream) | c2-p1 | c2-p2 | c3 | c4 (single large instances)")
+    ap.add_argument("--size", type=int, default=0, help="single-instance workloads: N (default 2048 / 160 / 512)")
+    ap.add_argument("--solve", action="store_true", help="c3: also solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the served stream shards by request "
               "and is not measured across GPUs here", file=sys.stderr)
         sys.exit(2)
     grmp = GE.load()
-    stream = make_stream(args.requests, args.rate)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    if args.workload != "served":
+        run_workload(grmp, args.workload, args, log)
+        return
+    stream = make_stream(args.requests, args.rate)
 
     # warmup: the first 256 requests of the stream, untimed
     for w in range(args.warmup):
@@ -172,7 +315,12 @@ def main():
         GE.load().default_context().sync()
     run = runs[-1]
     walls = [r["wall"] for r in runs]
-    assert run["ok"].all(), f"{int((~run['ok']).sum())} requests failed"
+    if not run["ok"].all():
+        for k in np.nonzero(~run["ok"])[0]:
+            r = stream[k]
+            log(f"[bench] request {k} failed: kind {r['kind']} fe {r['fe']} N {r['N']} seed {r['seed']} "
+                f"status {run['status'][k]} iterations {run['iters'][k]} outer {run['outer'][k]}")
+        raise SystemExit(f"{int((~run['ok']).sum())} requests failed")
     value = float(np.mean([r["dofs"].sum() / r["wall"] for r in runs]))
 
     log("[bench] verification against FP64 solves")

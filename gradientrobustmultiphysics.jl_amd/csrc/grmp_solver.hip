@@ -511,8 +511,15 @@ __global__ __launch_bounds__(KB) void k_eng_reduce(KProb *__restrict__ P, const 
                 w.beta = a / w.rz;
                 w.rz = a;
                 w.rmax = rn > w.rmax ? rn : w.rmax;
-                if (rn <= w.rtol * w.bnorm * w.scale || rn < w.delta * w.rmax || w.iters >= w.maxit)
+                if (rn > 1e6) {
+                    // the scaled residual started near 1: a sweep that grew it
+                    // a millionfold has diverged and is dropped
+                    w.restart = 1;
+                    w.discard = 1;
+                    w.phase = PH_UPDATE;
+                } else if (rn <= w.rtol * w.bnorm * w.scale || rn < w.delta * w.rmax || w.iters >= w.maxit) {
                     w.phase = PH_UPDATE;  // converged estimate or reliable update
+                }
             }
         } else if (ph == PH_UPDATE) {
             w.phase = PH_RESID;
@@ -536,18 +543,26 @@ __global__ __launch_bounds__(KB) void k_eng_reduce(KProb *__restrict__ P, const 
                 w.status = GRMP_ERR_NOCONV;
                 w.phase = PH_FAIL;
             } else {
-                // a refresh that does not halve the residual restarts the
-                // direction; three in a row mean the tolerance lies below
-                // what the FP64 residual resolves
-                const bool stalled = w.outer > 0 && !(w.relres <= 0.5 * w.best);
-                w.stall = stalled ? w.stall + 1 : 0;
-                if (w.outer == 0 || w.relres < w.best) w.best = w.relres;
-                if (w.stall >= 3) {
+                // six refreshes without halving the best FP64 residual mean
+                // the tolerance lies below what the FP64 residual resolves
+                if (w.outer == 0) {
+                    w.best = w.mark = w.relres;
+                    w.stall = 0;
+                } else {
+                    if (w.relres < w.best) w.best = w.relres;
+                    if (w.best <= 0.5 * w.mark) {
+                        w.mark = w.best;
+                        w.stall = 0;
+                    } else {
+                        w.stall += 1;
+                    }
+                }
+                if (w.stall >= 6) {
                     w.status = GRMP_ERR_NOCONV;
                     w.phase = PH_FAIL;
                 } else {
                     const double snew = 1.0 / sqrt(a);
-                    const bool fresh = w.outer == 0 || w.restart || stalled;
+                    const bool fresh = w.outer == 0 || w.restart;
                     w.pscale = fresh ? 1.0 : snew / w.scale;
                     w.scale = snew;
                     w.phase = fresh ? PH_INIT : PH_REINIT;
