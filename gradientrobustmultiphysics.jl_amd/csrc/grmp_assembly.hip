@@ -1125,6 +1125,51 @@ __device__ __forceinline__ double lf_local(const LfDev &L, int64_t e, int a)
     return lb * itemfactor;
 }
 
+// P1 on tetrahedra, order <= 1 (midpoint, weight 1, all four basis values 1/4)
+__device__ __forceinline__ double lf_local_tet(const LfDev &L, int64_t e, int a)
+{
+    const int64_t nc = L.ncells, nn = L.nnodes;
+    double X[4][3];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int32_t n = L.cells[v * nc + e];
+        X[v][0] = L.coords[n];
+        X[v][1] = L.coords[nn + n];
+        X[v][2] = L.coords[2 * nn + n];
+    }
+    double A[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) A[r][k] = X[k + 1][r] - X[0][r];
+    const double c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+    const double c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+    const double c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+    const double det = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
+    const double vol = fabs(det) / 6.0;
+    const int c = a / 4;
+    const double phi = 0.25;
+    double t = 0.0;
+    for (int k = 0; k < L.ncomp; ++k) t += L.f[k] * (k == c ? phi : 0.0);
+    double lb = 0.0;
+    lb += t * 1.0;
+    const double itemfactor = L.factor * vol * 1.0;
+    return lb * itemfactor;
+}
+
+__global__ void k_linearform_tet(LfDev L, double *__restrict__ b, int64_t ndofs)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndofs) return;
+    double acc = b[i];
+    for (int32_t t = L.inc_ptr[i]; t < L.inc_ptr[i + 1]; ++t) {
+        const int32_t pk = L.inc[t];
+        const int64_t e = pk / L.nloc;
+        acc += lf_local_tet(L, e, pk - (int)e * L.nloc);
+    }
+    b[i] = acc;
+}
+
 template <int FE, int NQ>
 __global__ void k_linearform(LfDev L, double *__restrict__ b, int64_t ndofs)
 {
@@ -1145,9 +1190,13 @@ int linearform_launch(grmp_ctx *ctx, const grmp_space *s, const grmp_lform *form
 {
     const grmp_mesh *m = s->mesh;
     int nq = quad_nq_tri(form->quadorder);
-    GRMP_CHECK(ctx, form->kind == GRMP_LF_CONST && m->dim == 2 && (s->fe == GRMP_FE_H1P1 || s->fe == GRMP_FE_H1P2) && nq,
+    const bool tet = m->dim == 3 && s->fe == GRMP_FE_H1P1 && form->quadorder <= 1;
+    GRMP_CHECK(ctx, form->kind == GRMP_LF_CONST &&
+                        (tet || (m->dim == 2 && (s->fe == GRMP_FE_H1P1 || s->fe == GRMP_FE_H1P2) && nq)),
                GRMP_ERR_UNSUPPORTED, "linear form kind %d on FE %d (dim %d, quadorder %d) not built", form->kind, s->fe,
                m->dim, form->quadorder);
+    GRMP_CHECK(ctx, s->inc_ptr && s->inc && s->ncomp >= 1 && s->ncomp <= 4, GRMP_ERR_ARG,
+               "linear form: space without dof-cell incidence");
     LfDev L;
     memset(&L, 0, sizeof(L));
     L.fe = s->fe;
@@ -1164,7 +1213,8 @@ int linearform_launch(grmp_ctx *ctx, const grmp_space *s, const grmp_lform *form
     L.inc_ptr = s->inc_ptr;
     L.inc = s->inc;
     dim3 grid(nblk(s->ndofs, 256)), blk(256);
-    if (s->fe == GRMP_FE_H1P1 && nq == 1) hipLaunchKernelGGL((k_linearform<1, 1>), grid, blk, 0, st, L, bp, s->ndofs);
+    if (tet) hipLaunchKernelGGL(k_linearform_tet, grid, blk, 0, st, L, bp, s->ndofs);
+    else if (s->fe == GRMP_FE_H1P1 && nq == 1) hipLaunchKernelGGL((k_linearform<1, 1>), grid, blk, 0, st, L, bp, s->ndofs);
     else if (s->fe == GRMP_FE_H1P1) hipLaunchKernelGGL((k_linearform<1, 3>), grid, blk, 0, st, L, bp, s->ndofs);
     else if (nq == 1) hipLaunchKernelGGL((k_linearform<2, 1>), grid, blk, 0, st, L, bp, s->ndofs);
     else hipLaunchKernelGGL((k_linearform<2, 3>), grid, blk, 0, st, L, bp, s->ndofs);

@@ -34,7 +34,7 @@ struct KProb {
     int32_t max_outer;
     int32_t restart;    // restart the search direction at the next refresh
     int32_t discard;    // the FP32 sweep overflowed: drop its correction
-    int32_t stall;      // refreshes since the best FP64 residual last halved
+    int32_t stall;      // consecutive refreshes that did not halve the best residual
     int32_t pad_;
     int64_t iters, maxit;
     double bb, bnorm;   // ||b - A x0||^2 and its root (free rows)
@@ -44,7 +44,6 @@ struct KProb {
     double pscale;      // search-direction rescale at a refresh
     double rz, alpha, beta, relres, rmax;
     double best;        // smallest FP64 relative residual so far
-    double mark;        // best at the last halving
 };
 
 struct KTile {

@@ -543,33 +543,20 @@ __global__ __launch_bounds__(KB) void k_eng_reduce(KProb *__restrict__ P, const 
                 w.status = GRMP_ERR_NOCONV;
                 w.phase = PH_FAIL;
             } else {
-                // six refreshes without halving the best FP64 residual mean
-                // the tolerance lies below what the FP64 residual resolves
-                if (w.outer == 0) {
-                    w.best = w.mark = w.relres;
-                    w.stall = 0;
-                } else {
-                    if (w.relres < w.best) w.best = w.relres;
-                    if (w.best <= 0.5 * w.mark) {
-                        w.mark = w.best;
-                        w.stall = 0;
-                    } else {
-                        w.stall += 1;
-                    }
-                }
-                if (w.stall >= 6) {
-                    w.status = GRMP_ERR_NOCONV;
-                    w.phase = PH_FAIL;
-                } else {
-                    const double snew = 1.0 / sqrt(a);
-                    const bool fresh = w.outer == 0 || w.restart;
-                    w.pscale = fresh ? 1.0 : snew / w.scale;
-                    w.scale = snew;
-                    w.phase = fresh ? PH_INIT : PH_REINIT;
-                    w.restart = 0;
-                    w.discard = 0;
-                    w.outer += 1;
-                }
+                // a refresh that did not halve the best FP64 residual restarts
+                // the direction: the carried one no longer fits the refreshed
+                // residual (FP32 rounding of A relative to its conditioning)
+                const bool stalled = w.outer > 0 && !(w.relres <= 0.5 * w.best);
+                if (w.outer == 0 || w.relres < w.best) w.best = w.relres;
+                w.stall = stalled ? w.stall + 1 : 0;
+                const double snew = 1.0 / sqrt(a);
+                const bool fresh = w.outer == 0 || w.restart || stalled;
+                w.pscale = fresh ? 1.0 : snew / w.scale;
+                w.scale = snew;
+                w.phase = fresh ? PH_INIT : PH_REINIT;
+                w.restart = 0;
+                w.discard = 0;
+                w.outer += 1;
             }
         }
     }

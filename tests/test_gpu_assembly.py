@@ -153,6 +153,13 @@ def test_tet_parity_and_mesh(grmp, O):
                                             coef=O.COEF_CELL, kappa=kap.download(), coords=c, cells=cn,
                                             celldofs=cn)], fixed=bd.astype(np.int32))
     assert _check_csc(A, ref) == 1.0
+    # LinearForm(Identity, f = 1) on the tetrahedra, bitwise
+    b = grmp.FEVector([fes])
+    grmp.assemble_linearform(b, 0, grmp.LinearForm(grmp.Identity, grmp.DataFunction([1.0])))
+    bref = np.zeros(fes.ndofs)
+    O.assemble_lf_const(bref, O.FE_P1, 1, 1, [1.0], c, cn, cn, geom=3)
+    assert np.array_equal(b.entries(), bref)
+    assert abs(bref.sum() - 1.0) < 1e-13
 
 
 @pytest.mark.parametrize("N,fe,ncomp,f", [(32, 1, 1, [1.0]), (32, 2, 1, [1.0]), (20, 1, 2, [0.0, -1.0])])
