@@ -60,7 +60,12 @@ def serve(grmp, mode, stream, rtol, keep=(), profile=False, respect_arrivals=Tru
             if dt > 0:
                 time.sleep(dt)
         tickets.append(S.submit(r["kind"], r["fe"], r["N"], r["seed"], rtol=rtol, keep_solution=k in keep))
-    res = [S.wait(t) for t in tickets]
+    res = []
+    for k, t in enumerate(tickets):
+        res.append(S.wait(t))
+        if (k + 1) % 256 == 0 or k + 1 == len(tickets):
+            print(f"[bench] {mode}: {k + 1}/{len(tickets)} results, {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                  flush=True)
     wall = time.perf_counter() - t0
     sols = {k: S.solution(tickets[k]).copy() for k in keep}
     stats = S.stats()
@@ -301,7 +306,7 @@ def main():
     # warmup: the first 256 requests of the stream, untimed
     for w in range(args.warmup):
         log(f"[bench] warmup {w}")
-        serve(grmp, "served", stream[:256], args.rtol)
+        serve(grmp, "served", stream[:256], args.rtol, profile=True)
 
     # verification sample: spread over the size range, largest included
     order = np.argsort([r["N"] * (2 if r["fe"] == 2 or r["kind"] == 2 else 1) for r in stream])

@@ -645,11 +645,14 @@ __device__ __forceinline__ void assemble_row(const DevForm &F, int64_t i, double
 }
 
 // One thread per row; the block's rows are staged contiguously in LDS
-// (dynamic shared memory of rowptr[end]-rowptr[start] doubles).
-__global__ void k_assemble(FormPack P, int64_t g0, int64_t g1, int64_t rblock, const int32_t *__restrict__ rowptr,
-                           const uint8_t *__restrict__ diagpos, const uint8_t *__restrict__ fixed, double penalty,
-                           double *__restrict__ val64, float *__restrict__ val32, double *__restrict__ dinv64,
-                           float *__restrict__ dinv32)
+// (dynamic shared memory of rowptr[end]-rowptr[start] doubles).  ``add``
+// accumulates the forms of the row block into a row (FP64, reference order).
+template <class AddRow>
+__device__ __forceinline__ void assemble_block(AddRow add, int64_t g0, int64_t g1, int64_t rblock,
+                                               const int32_t *__restrict__ rowptr, const uint8_t *__restrict__ diagpos,
+                                               const uint8_t *__restrict__ fixed, double penalty,
+                                               double *__restrict__ val64, float *__restrict__ val32,
+                                               double *__restrict__ dinv64, float *__restrict__ dinv32)
 {
     extern __shared__ double buf[];
     const int B = blockDim.x;
@@ -667,11 +670,7 @@ __global__ void k_assemble(FormPack P, int64_t g0, int64_t g1, int64_t rblock, c
         off = rowptr[g] - base;
         rl = rowptr[g + 1] - rowptr[g];
         fx = fixed[g] != 0;
-        const int64_t i = g - rblock;
-        for (int f = 0; f < P.n; ++f) {
-            const DevForm F = P.f[f];
-            GRMP_DISPATCH(F, assemble_row<KT>(F, i, buf + off));
-        }
+        add(g - rblock, buf + off);
         const uint8_t dp = diagpos[g];
         if (fx && dp != 255) buf[off + dp] = penalty;
         double d = (dp != 255) ? buf[off + dp] : 0.0;
@@ -689,6 +688,49 @@ __global__ void k_assemble(FormPack P, int64_t g0, int64_t g1, int64_t rblock, c
         __syncthreads();
         for (int32_t k = threadIdx.x; k < len; k += B) val32[base + k] = (float)buf[k];
     }
+}
+
+// any number of forms per row block, kinds dispatched per form
+__global__ void k_assemble(FormPack P, int64_t g0, int64_t g1, int64_t rblock, const int32_t *__restrict__ rowptr,
+                           const uint8_t *__restrict__ diagpos, const uint8_t *__restrict__ fixed, double penalty,
+                           double *__restrict__ val64, float *__restrict__ val32, double *__restrict__ dinv64,
+                           float *__restrict__ dinv32)
+{
+    assemble_block(
+        [&](int64_t i, double *row) {
+            for (int f = 0; f < P.n; ++f) {
+                const DevForm F = P.f[f];
+                GRMP_DISPATCH(F, assemble_row<KT>(F, i, row));
+            }
+        },
+        g0, g1, rblock, rowptr, diagpos, fixed, penalty, val64, val32, dinv64, dinv32);
+}
+
+// one form of a known kind, passed by value (kernel arguments stay scalar)
+template <class K>
+__global__ void k_assemble1(DevForm F, int64_t g0, int64_t g1, int64_t rblock, const int32_t *__restrict__ rowptr,
+                            const uint8_t *__restrict__ diagpos, const uint8_t *__restrict__ fixed, double penalty,
+                            double *__restrict__ val64, float *__restrict__ val32, double *__restrict__ dinv64,
+                            float *__restrict__ dinv32)
+{
+    assemble_block([&](int64_t i, double *row) { assemble_row<K>(F, i, row); }, g0, g1, rblock, rowptr, diagpos, fixed,
+                   penalty, val64, val32, dinv64, dinv32);
+}
+
+// Launches the value assembly of row block [g0, g1): the per-kind kernel when
+// the block has one form (hf = its host copy), the generic one otherwise.
+static void launch_assemble(const grmp_matrix *A, const DevForm *hf, FormPack P, int64_t g0, int64_t g1, int B,
+                            size_t shm, double *val64, float *val32, double *dinv64, float *dinv32, hipStream_t st)
+{
+    const dim3 grid(nblk(g1 - g0, B)), blk(B);
+    if (P.n == 1) {
+        const DevForm F = *hf;
+        GRMP_DISPATCH(F, hipLaunchKernelGGL((k_assemble1<KT>), grid, blk, shm, st, F, g0, g1, g0, A->rowptr,
+                                            A->diagpos, A->fixed, A->penalty, val64, val32, dinv64, dinv32));
+        return;
+    }
+    hipLaunchKernelGGL(k_assemble, grid, blk, shm, st, P, g0, g1, g0, A->rowptr, A->diagpos, A->fixed, A->penalty, val64,
+                       val32, dinv64, dinv32);
 }
 
 // ------------------------------------------------------------ matrix API
@@ -979,6 +1021,13 @@ extern "C" int grmp_matrix_assemble(grmp_matrix *A, int values)
     if (!A->dinv64) GRMP_TRY(dalloc(ctx, &A->dinv64, A->n));
     if (!A->dinv32) GRMP_TRY(dalloc(ctx, &A->dinv32, A->n));
     const int B = asm_block_size(A->max_row);
+    std::vector<DevForm> hf(A->forms.size() + 1);
+    {
+        int k = 0;
+        for (int rb = 0; rb < A->nblocks; ++rb)
+            for (auto &r : A->forms)
+                if (r.f.row_block == rb) fill_devform(A, r, hf[k++]);
+    }
     hipEvent_t e0, e1;
     GRMP_HIP(ctx, hipEventCreate(&e0));
     GRMP_HIP(ctx, hipEventCreate(&e1));
@@ -988,9 +1037,8 @@ extern "C" int grmp_matrix_assemble(grmp_matrix *A, int values)
         int64_t g0 = A->offs[rb], g1 = A->offs[rb + 1];
         if (g1 <= g0) continue;
         size_t shm = (size_t)B * (A->max_row > 0 ? A->max_row : 1) * sizeof(double);
-        hipLaunchKernelGGL(k_assemble, dim3(nblk(g1 - g0, B)), dim3(B), shm, ctx->stream, P, g0, g1, g0, A->rowptr,
-                           A->diagpos, A->fixed, A->penalty, (values & GRMP_VALUES_FP64) ? A->val64 : nullptr,
-                           (values & GRMP_VALUES_FP32) ? A->val32 : nullptr, A->dinv64, A->dinv32);
+        launch_assemble(A, &hf[A->form_first[rb]], P, g0, g1, B, shm, (values & GRMP_VALUES_FP64) ? A->val64 : nullptr,
+                        (values & GRMP_VALUES_FP32) ? A->val32 : nullptr, A->dinv64, A->dinv32, ctx->stream);
         GRMP_HIP(ctx, hipGetLastError());
     }
     GRMP_HIP(ctx, hipEventRecord(e1, ctx->stream));
@@ -1037,8 +1085,7 @@ int assemble_values(const grmp_matrix *A, const double *kappa, DevForm *dforms, 
         int64_t g0 = A->offs[rb], g1 = A->offs[rb + 1];
         if (g1 <= g0) continue;
         size_t shm = (size_t)B * (A->max_row > 0 ? A->max_row : 1) * sizeof(double);
-        hipLaunchKernelGGL(k_assemble, dim3(nblk(g1 - g0, B)), dim3(B), shm, st, P, g0, g1, g0, A->rowptr,
-                           A->diagpos, A->fixed, A->penalty, val64, val32, dinv64, dinv32);
+        launch_assemble(A, &hforms[first[rb]], P, g0, g1, B, shm, val64, val32, dinv64, dinv32, st);
         GRMP_HIP(ctx, hipGetLastError());
     }
     return GRMP_OK;

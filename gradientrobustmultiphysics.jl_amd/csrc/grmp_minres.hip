@@ -34,8 +34,6 @@ static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / 
 
 #define MR_GRID 1024
 
-int engine_pick_lanes(double nnz_per_row);
-
 struct MrScal {
     double g, gp;          // gamma_j, gamma_{j-1}
     double c, cp, s, sp;   // Givens c_j, c_{j-1}, s_j, s_{j-1}
@@ -86,7 +84,7 @@ __global__ void k_mr_diag(const int32_t *__restrict__ rp, const int32_t *__restr
 // ------------------------------------------------------------ init
 // v1 = S (rhs - A x) on free rows, z1 = |d| v1, partial z1.v1; with
 // A == nullptr the residual is rhs itself (inner solves start from 0).
-template <int L, class T, class V, class X, class R>
+template <class T, class V, class X, class R>
 __global__ __launch_bounds__(KB) void k_mr_init(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                                                 const V *__restrict__ a, const uint8_t *__restrict__ fixed,
                                                 const T *__restrict__ dsg, const R *__restrict__ rhs,
@@ -94,11 +92,13 @@ __global__ __launch_bounds__(KB) void k_mr_init(const int32_t *__restrict__ rp, 
                                                 T *__restrict__ v0, T *__restrict__ w0, T *__restrict__ w1, int64_t n,
                                                 double *__restrict__ part)
 {
-    const int lane = threadIdx.x % L;
+    __shared__ double lds[SPMV_CAP];
     double acc = 0.0, dummy = 0.0;
-    for (int64_t i = ((int64_t)blockIdx.x * KB + threadIdx.x) / L; i < n; i += (int64_t)gridDim.x * KB / L) {
-        double s = a ? csr_row<L>(rp, ci, a, x, (int)i, lane) : 0.0;
-        if (lane == 0) {
+    for (int64_t r0 = (int64_t)blockIdx.x * KB; r0 < n; r0 += (int64_t)gridDim.x * KB) {
+        const int64_t r1 = r0 + KB < n ? r0 + KB : n;
+        const double s = a ? csr_rows_lds<double>(rp, ci, a, x, r0, r1, lds) : 0.0;
+        const int64_t i = r0 + threadIdx.x;
+        if (i < r1) {
             const double d = (double)dsg[i];
             double r = fixed[i] ? 0.0 : ((double)rhs[i] - s);
             if (d < 0.0) r = -r;
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(KB) void k_mr_start(MrScal *__restrict__ S, const d
 // ------------------------------------------------------------ iteration
 // K1: deferred w/x update of the previous iteration, then
 //     q = S A z_j / gamma_j (0 on fixed rows), partial q.z_j / gamma_j
-template <int L, class T, class V, class X>
+template <class T, class V, class X>
 __global__ __launch_bounds__(KB) void k_mr_apply(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                                                  const V *__restrict__ a, const uint8_t *__restrict__ fixed,
                                                  const T *__restrict__ dsg, const T *__restrict__ z,
@@ -150,24 +150,26 @@ __global__ __launch_bounds__(KB) void k_mr_apply(const int32_t *__restrict__ rp,
                                                  X *__restrict__ x, T *__restrict__ q, int64_t n,
                                                  const MrScal *__restrict__ S, double *__restrict__ part)
 {
+    __shared__ T lds[SPMV_CAP];
     const int status = S->status;
     if (status == 2) return;
-    const int lane = threadIdx.x % L;
     const bool pend = S->pending != 0;
     const double uig = S->u_ig, ua1 = S->u_a1, ua2 = S->u_a2, ua3 = S->u_a3, uce = S->u_ceta;
     const double ig = 1.0 / S->g;
     double acc = 0.0, dummy = 0.0;
-    for (int64_t i = ((int64_t)blockIdx.x * KB + threadIdx.x) / L; i < n; i += (int64_t)gridDim.x * KB / L) {
-        if (pend && lane == 0) {
+    for (int64_t r0 = (int64_t)blockIdx.x * KB; r0 < n; r0 += (int64_t)gridDim.x * KB) {
+        const int64_t r1 = r0 + KB < n ? r0 + KB : n;
+        const int64_t i = r0 + threadIdx.x;
+        if (pend && i < r1) {
             const double wn = ((double)zprev[i] * uig - ua3 * (double)wm2[i] - ua2 * (double)wm1[i]) / ua1;
             wnew[i] = (T)wn;
             x[i] = (X)((double)x[i] + uce * wn);
         }
         if (status == 0) {
-            double s = csr_row<L>(rp, ci, a, z, (int)i, lane);
-            if (lane == 0) {
+            const T s = csr_rows_lds<T>(rp, ci, a, z, r0, r1, lds);
+            if (i < r1) {
                 const double d = (double)dsg[i];
-                double qi = fixed[i] ? 0.0 : s * ig;
+                double qi = fixed[i] ? 0.0 : (double)s * ig;
                 if (d < 0.0) qi = -qi;
                 const T qt = (T)qi;
                 q[i] = qt;
@@ -260,18 +262,19 @@ __global__ __launch_bounds__(KB) void k_mr_givens(MrScal *__restrict__ S, const 
 
 // ------------------------------------------------------- FP64 refinement
 // r = S (b - A x) on free rows (FP64), partial r.|d| r
-template <int L>
 __global__ __launch_bounds__(KB) void k_mr_resid64(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                                                    const double *__restrict__ a, const uint8_t *__restrict__ fixed,
                                                    const double *__restrict__ dsg, const double *__restrict__ b,
                                                    const double *__restrict__ x, double *__restrict__ r, int64_t n,
                                                    double *__restrict__ part)
 {
-    const int lane = threadIdx.x % L;
+    __shared__ double lds[SPMV_CAP];
     double acc = 0.0, dummy = 0.0;
-    for (int64_t i = ((int64_t)blockIdx.x * KB + threadIdx.x) / L; i < n; i += (int64_t)gridDim.x * KB / L) {
-        double s = csr_row<L>(rp, ci, a, x, (int)i, lane);
-        if (lane == 0) {
+    for (int64_t r0 = (int64_t)blockIdx.x * KB; r0 < n; r0 += (int64_t)gridDim.x * KB) {
+        const int64_t r1 = r0 + KB < n ? r0 + KB : n;
+        const double s = csr_rows_lds<double>(rp, ci, a, x, r0, r1, lds);
+        const int64_t i = r0 + threadIdx.x;
+        if (i < r1) {
             const double d = dsg[i];
             double ri = fixed[i] ? 0.0 : b[i] - s;
             r[i] = ri;
@@ -331,11 +334,9 @@ struct MrHost {
     MrScal *dS;
     MrScal *hS;
     double *part;
-    int grid, gride, L;
+    int grid, gride;
 };
 
-#define MR_LCASE(LL, KERNEL, GRID, ...) \
-    case LL: hipLaunchKernelGGL((KERNEL), dim3(GRID), dim3(KB), 0, H.st, __VA_ARGS__); break;
 
 // one MINRES run on S A y = rhs from y = y0 (the residual of y0 against rhs
 // is formed when Ainit != nullptr, else y0 must be 0); returns via hS
@@ -345,19 +346,8 @@ int mr_run(MrHost &H, const grmp_matrix *A, const V *vals, const T *dsg, const R
 {
     const int64_t n = A->n;
     const V *ainit = y_nonzero ? vals : nullptr;
-#define MR_INIT(LL) k_mr_init<LL, T, V, X, R>
-    switch (H.L) {
-        MR_LCASE(2, MR_INIT(2), H.grid, A->rowptr, A->cols, ainit, A->fixed, dsg, rhs, y, B.V[1], B.Z[1], B.V[0],
-                 B.W[0], B.W[1], n, H.part)
-        MR_LCASE(4, MR_INIT(4), H.grid, A->rowptr, A->cols, ainit, A->fixed, dsg, rhs, y, B.V[1], B.Z[1], B.V[0],
-                 B.W[0], B.W[1], n, H.part)
-        MR_LCASE(8, MR_INIT(8), H.grid, A->rowptr, A->cols, ainit, A->fixed, dsg, rhs, y, B.V[1], B.Z[1], B.V[0],
-                 B.W[0], B.W[1], n, H.part)
-        MR_LCASE(16, MR_INIT(16), H.grid, A->rowptr, A->cols, ainit, A->fixed, dsg, rhs, y, B.V[1], B.Z[1], B.V[0],
-                 B.W[0], B.W[1], n, H.part)
-    default: break;
-    }
-#undef MR_INIT
+    hipLaunchKernelGGL((k_mr_init<T, V, X, R>), dim3(H.grid), dim3(KB), 0, H.st, A->rowptr, A->cols, ainit, A->fixed,
+                       dsg, rhs, y, B.V[1], B.Z[1], B.V[0], B.W[0], B.W[1], n, H.part);
     hipLaunchKernelGGL(k_mr_start, dim3(1), dim3(KB), 0, H.st, H.dS, H.part, H.grid, rtol, maxit);
     GRMP_HIP(H.ctx, hipGetLastError());
     const int every = std::max(1, check_every);
@@ -366,19 +356,8 @@ int mr_run(MrHost &H, const grmp_matrix *A, const V *vals, const T *dsg, const R
         T *wnew = B.W[j % 3], *wm2 = B.W[(j + 1) % 3], *wm1 = B.W[(j + 2) % 3];
         T *vj = B.V[j & 1], *vm1 = B.V[(j + 1) & 1];
         T *znew = zprev;  // z_{j+1} replaces z_{j-1}
-#define MR_APPLY(LL) k_mr_apply<LL, T, V, X>
-        switch (H.L) {
-            MR_LCASE(2, MR_APPLY(2), H.grid, A->rowptr, A->cols, vals, A->fixed, dsg, z, zprev, wnew, wm2, wm1, y, B.q,
-                     n, H.dS, H.part)
-            MR_LCASE(4, MR_APPLY(4), H.grid, A->rowptr, A->cols, vals, A->fixed, dsg, z, zprev, wnew, wm2, wm1, y, B.q,
-                     n, H.dS, H.part)
-            MR_LCASE(8, MR_APPLY(8), H.grid, A->rowptr, A->cols, vals, A->fixed, dsg, z, zprev, wnew, wm2, wm1, y, B.q,
-                     n, H.dS, H.part)
-            MR_LCASE(16, MR_APPLY(16), H.grid, A->rowptr, A->cols, vals, A->fixed, dsg, z, zprev, wnew, wm2, wm1, y,
-                     B.q, n, H.dS, H.part)
-        default: break;
-        }
-#undef MR_APPLY
+        hipLaunchKernelGGL((k_mr_apply<T, V, X>), dim3(H.grid), dim3(KB), 0, H.st, A->rowptr, A->cols, vals, A->fixed,
+                           dsg, z, zprev, wnew, wm2, wm1, y, B.q, n, H.dS, H.part);
         hipLaunchKernelGGL(k_mr_delta, dim3(1), dim3(KB), 0, H.st, H.dS, H.part, H.grid);
         hipLaunchKernelGGL(k_mr_lanczos<T>, dim3(H.gride), dim3(KB), 0, H.st, B.q, vj, vm1, dsg, znew, n, H.dS, H.part);
         hipLaunchKernelGGL(k_mr_givens, dim3(1), dim3(KB), 0, H.st, H.dS, H.part, H.gride);
@@ -392,7 +371,6 @@ int mr_run(MrHost &H, const grmp_matrix *A, const V *vals, const T *dsg, const R
     *iters_out = H.hS->it;
     return GRMP_OK;
 }
-#undef MR_LCASE
 
 }  // namespace
 
@@ -407,8 +385,7 @@ int minres_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solve_
     MrHost H;
     H.ctx = ctx;
     H.st = ctx->stream;
-    H.L = engine_pick_lanes((double)A->nnz / (double)std::max<int64_t>(n, 1));
-    H.grid = (int)std::min<int64_t>(MR_GRID, std::max<int64_t>(1, (n * H.L + KB - 1) / KB));
+    H.grid = (int)std::min<int64_t>(MR_GRID, std::max<int64_t>(1, (n + KB - 1) / KB));
     H.gride = (int)std::min<int64_t>(MR_GRID, std::max<int64_t>(1, (n + KB - 1) / KB));
     H.dS = nullptr;
     H.hS = nullptr;
@@ -459,19 +436,8 @@ int minres_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solve_
             double g0 = -1.0;
             for (;;) {
                 // FP64 residual in the preconditioner's norm
-                switch (H.L) {
-#define MR_R(LL)                                                                                                        \
-    case LL:                                                                                                            \
-        hipLaunchKernelGGL(k_mr_resid64<LL>, dim3(H.grid), dim3(KB), 0, H.st, A->rowptr, A->cols, A->val64, A->fixed,   \
-                           dsg, bd, xd, r64, n, H.part);                                                                \
-        break;
-                    MR_R(2)
-                    MR_R(4)
-                    MR_R(8)
-                    MR_R(16)
-#undef MR_R
-                default: break;
-                }
+                hipLaunchKernelGGL(k_mr_resid64, dim3(H.grid), dim3(KB), 0, H.st, A->rowptr, A->cols, A->val64, A->fixed,
+                                   dsg, bd, xd, r64, n, H.part);
                 GRMP_HIP(ctx, hipMemcpyAsync(hres, H.part, sizeof(double) * H.grid, hipMemcpyDeviceToHost, H.st));
                 GRMP_HIP(ctx, hipStreamSynchronize(H.st));
                 double rr = 0.0;

@@ -27,37 +27,39 @@ int engine_pick_lanes(double nnz_per_row)
 }
 
 // --------------------------------------------------------------- SpMV API
-template <int L, class V>
-__global__ void k_spmv(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci, const V *__restrict__ v,
-                       const double *__restrict__ x, double *__restrict__ y, int64_t n)
+__global__ __launch_bounds__(KB) void k_spmv(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                            const double *__restrict__ v, const double *__restrict__ x,
+                                            double *__restrict__ y, int64_t n)
 {
-    const int lane = threadIdx.x % L;
-    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / L;
-    if (i >= n) return;
-    double s = csr_row<L>(rp, ci, v, x, (int)i, lane);
-    if (lane == 0) y[i] = s;
+    __shared__ double lds[SPMV_CAP];
+    for (int64_t r0 = (int64_t)blockIdx.x * KB; r0 < n; r0 += (int64_t)gridDim.x * KB) {
+        const int64_t r1 = r0 + KB < n ? r0 + KB : n;
+        const double s = csr_rows_lds<double>(rp, ci, v, x, r0, r1, lds);
+        if (r0 + threadIdx.x < r1) y[r0 + threadIdx.x] = s;
+    }
+}
+
+// FP32 values, FP64 input, sums and output
+__global__ __launch_bounds__(KB) void k_spmv32(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                              const float *__restrict__ v, const double *__restrict__ x,
+                                              double *__restrict__ y, int64_t n)
+{
+    __shared__ double lds[SPMV_CAP];
+    for (int64_t r0 = (int64_t)blockIdx.x * KB; r0 < n; r0 += (int64_t)gridDim.x * KB) {
+        const int64_t r1 = r0 + KB < n ? r0 + KB : n;
+        const double s = csr_rows_lds<double>(rp, ci, v, x, r0, r1, lds);
+        if (r0 + threadIdx.x < r1) y[r0 + threadIdx.x] = s;
+    }
 }
 
 int spmv_launch(grmp_ctx *ctx, const grmp_matrix *A, int values, const double *x, double *y, hipStream_t st)
 {
     const int64_t n = A->n;
-    const int L = engine_pick_lanes((double)A->nnz / (double)(n ? n : 1));
-    dim3 grid(nblk(n * L, 256)), blk(256);
-#define SPMV_CASE(LL)                                                                                         \
-    case LL:                                                                                                  \
-        if (values == GRMP_VALUES_FP64)                                                                       \
-            hipLaunchKernelGGL((k_spmv<LL, double>), grid, blk, 0, st, A->rowptr, A->cols, A->val64, x, y, n); \
-        else                                                                                                  \
-            hipLaunchKernelGGL((k_spmv<LL, float>), grid, blk, 0, st, A->rowptr, A->cols, A->val32, x, y, n);  \
-        break;
-    switch (L) {
-        SPMV_CASE(2)
-        SPMV_CASE(4)
-        SPMV_CASE(8)
-        SPMV_CASE(16)
-    default: break;
-    }
-#undef SPMV_CASE
+    dim3 grid((unsigned)std::min<int64_t>(nblk(n, KB), 1 << 20)), blk(KB);
+    if (values == GRMP_VALUES_FP64)
+        hipLaunchKernelGGL(k_spmv, grid, blk, 0, st, A->rowptr, A->cols, A->val64, x, y, n);
+    else
+        hipLaunchKernelGGL(k_spmv32, grid, blk, 0, st, A->rowptr, A->cols, A->val32, x, y, n);
     GRMP_HIP(ctx, hipGetLastError());
     return GRMP_OK;
 }
@@ -96,17 +98,20 @@ __global__ void k_plain_init(const int32_t *rp, const uint8_t *dpos, const doubl
 
 // r = b - A x on free rows (0 on fixed rows); z = dinv r; p = z;
 // partials of (r.z, r.r)
-template <int L>
-__global__ void k_plain_resid(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci, const double *__restrict__ v,
-                              const uint8_t *__restrict__ fixed, const double *__restrict__ dinv, const double *__restrict__ b,
-                              const double *__restrict__ x, double *__restrict__ r, double *__restrict__ z,
-                              double *__restrict__ p, int64_t n, double *__restrict__ part)
+__global__ __launch_bounds__(PB) void k_plain_resid(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                   const double *__restrict__ v, const uint8_t *__restrict__ fixed,
+                                                   const double *__restrict__ dinv, const double *__restrict__ b,
+                                                   const double *__restrict__ x, double *__restrict__ r,
+                                                   double *__restrict__ z, double *__restrict__ p, int64_t n,
+                                                   double *__restrict__ part)
 {
-    const int lane = threadIdx.x % L;
+    __shared__ double lds[SPMV_CAP];
     double a0 = 0.0, a1 = 0.0;
-    for (int64_t i = ((int64_t)blockIdx.x * PB + threadIdx.x) / L; i < n; i += (int64_t)gridDim.x * PB / L) {
-        double s = csr_row<L>(rp, ci, v, x, (int)i, lane);
-        if (lane == 0) {
+    for (int64_t r0 = (int64_t)blockIdx.x * PB; r0 < n; r0 += (int64_t)gridDim.x * PB) {
+        const int64_t r1 = r0 + PB < n ? r0 + PB : n;
+        const double s = csr_rows_lds<double>(rp, ci, v, x, r0, r1, lds);
+        const int64_t i = r0 + threadIdx.x;
+        if (i < r1) {
             double ri = fixed[i] ? 0.0 : b[i] - s;
             double zi = dinv[i] * ri;
             r[i] = ri;
@@ -124,16 +129,18 @@ __global__ void k_plain_resid(const int32_t *__restrict__ rp, const int32_t *__r
 }
 
 // q = A p on free rows; partial p.q
-template <int L>
-__global__ void k_plain_spmv(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci, const double *__restrict__ v,
-                             const uint8_t *__restrict__ fixed, const double *__restrict__ p, double *__restrict__ q,
-                             int64_t n, double *__restrict__ part)
+__global__ __launch_bounds__(PB) void k_plain_spmv(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                  const double *__restrict__ v, const uint8_t *__restrict__ fixed,
+                                                  const double *__restrict__ p, double *__restrict__ q, int64_t n,
+                                                  double *__restrict__ part)
 {
-    const int lane = threadIdx.x % L;
+    __shared__ double lds[SPMV_CAP];
     double a0 = 0.0, a1 = 0.0;
-    for (int64_t i = ((int64_t)blockIdx.x * PB + threadIdx.x) / L; i < n; i += (int64_t)gridDim.x * PB / L) {
-        double s = csr_row<L>(rp, ci, v, p, (int)i, lane);
-        if (lane == 0) {
+    for (int64_t r0 = (int64_t)blockIdx.x * PB; r0 < n; r0 += (int64_t)gridDim.x * PB) {
+        const int64_t r1 = r0 + PB < n ? r0 + PB : n;
+        const double s = csr_rows_lds<double>(rp, ci, v, p, r0, r1, lds);
+        const int64_t i = r0 + threadIdx.x;
+        if (i < r1) {
             double qi = fixed[i] ? 0.0 : s;
             q[i] = qi;
             a0 += p[i] * qi;
@@ -200,21 +207,11 @@ int plain_cg_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solv
     GRMP_TRY(dalloc(ctx, &part, 2 * PGRID));
     double *hpart = nullptr;
     GRMP_HIP(ctx, hipHostMalloc((void **)&hpart, sizeof(double) * 2 * PGRID, hipHostMallocDefault));
-    const int L = engine_pick_lanes((double)A->nnz / (double)n);
-    const int grid = (int)std::min<int64_t>(PGRID, std::max<int64_t>(1, (n * L + PB - 1) / PB));
-    const int gride = (int)std::min<int64_t>(PGRID, std::max<int64_t>(1, (n + PB - 1) / PB));
+    const int grid = (int)std::min<int64_t>(PGRID, std::max<int64_t>(1, (n + PB - 1) / PB));
+    const int gride = grid;
     hipLaunchKernelGGL(k_plain_init, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, A->rowptr, A->diagpos, A->val64, A->fixed,
                        bd, xd, n);
-#define PL_CASE(LL, KERNEL, ...) \
-    case LL: hipLaunchKernelGGL((KERNEL<LL>), dim3(grid), dim3(PB), 0, ctx->stream, __VA_ARGS__); break;
-#define PL_SWITCH(KERNEL, ...)                 \
-    switch (L) {                               \
-        PL_CASE(2, KERNEL, __VA_ARGS__)        \
-        PL_CASE(4, KERNEL, __VA_ARGS__)        \
-        PL_CASE(8, KERNEL, __VA_ARGS__)        \
-        PL_CASE(16, KERNEL, __VA_ARGS__)       \
-    default: break;                            \
-    }
+#define PL_SWITCH(KERNEL, ...) hipLaunchKernelGGL(KERNEL, dim3(grid), dim3(PB), 0, ctx->stream, __VA_ARGS__)
     PL_SWITCH(k_plain_resid, A->rowptr, A->cols, A->val64, A->fixed, A->dinv64, bd, xd, r, z, p, n, part);
     double rz, rr, pq, dummy;
     int rc = plain_sum(ctx, part, hpart, grid, rz, rr);
@@ -239,7 +236,6 @@ int plain_cg_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solv
         (void)every;
     }
 #undef PL_SWITCH
-#undef PL_CASE
     GRMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     info.iterations = it;
     info.outer = 1;
@@ -303,14 +299,16 @@ int zero_fixed_launch(const grmp_matrix *A, double *b, hipStream_t st)
 #define ENGINE_ARGS KProb *__restrict__ P, const KTile *__restrict__ T, const KRange *__restrict__ R, \
                     double *__restrict__ part, const int *__restrict__ ntiles, unsigned long long *__restrict__ nbytes
 
-template <int L>
-__device__ __forceinline__ double spmv_tile(const KProb &pr, int row0, int row1)
+// q = A32 p over rows row0 .. row1 of one problem (FP32 products and sums),
+// staged through LDS one KB-row block at a time; returns the partial p.q
+__device__ __forceinline__ double spmv_tile(const KProb &pr, int row0, int row1, float *lds)
 {
-    const int lane = threadIdx.x % L;
     double acc = 0.0;
-    for (int i = row0 + (int)threadIdx.x / L; i < row1; i += KB / L) {
-        float s = csr_row32<L>(pr.rowptr, pr.cols, pr.val32, pr.p, i, lane);
-        if (lane == 0) {
+    for (int r0 = row0; r0 < row1; r0 += KB) {
+        const int r1 = r0 + KB < row1 ? r0 + KB : row1;
+        const float s = csr_rows_lds<float>(pr.rowptr, pr.cols, pr.val32, pr.p, r0, r1, lds);
+        const int i = r0 + (int)threadIdx.x;
+        if (i < r1) {
             pr.q[i] = s;
             acc += (double)pr.p[i] * (double)s;
         }
@@ -320,19 +318,14 @@ __device__ __forceinline__ double spmv_tile(const KProb &pr, int row0, int row1)
 
 __global__ __launch_bounds__(KB) void k_eng_spmv(ENGINE_ARGS)
 {
+    __shared__ float lds32[SPMV_CAP];
     const int nt = *ntiles;
     for (int t = blockIdx.x; t < nt; t += gridDim.x) {
         const KTile tl = T[t];
         const int pi = tl.prob;
         if (P[pi].phase != PH_ITER) continue;
         const KProb pr = P[pi];
-        double acc = 0.0, dummy = 0.0;
-        switch (pr.lanes) {
-        case 2: acc = spmv_tile<2>(pr, tl.row0, tl.row1); break;
-        case 4: acc = spmv_tile<4>(pr, tl.row0, tl.row1); break;
-        case 8: acc = spmv_tile<8>(pr, tl.row0, tl.row1); break;
-        default: acc = spmv_tile<16>(pr, tl.row0, tl.row1); break;
-        }
+        double acc = spmv_tile(pr, tl.row0, tl.row1, lds32), dummy = 0.0;
         block_sum2(acc, dummy);
         if (threadIdx.x == 0) {
             part[2 * t] = acc;
@@ -414,14 +407,15 @@ __global__ __launch_bounds__(KB) void k_eng_elem(ENGINE_ARGS)
     }
 }
 
-template <int L>
-__device__ __forceinline__ double resid_tile(const KProb &pr, int row0, int row1)
+// r64 = b - A64 x on free rows of row0 .. row1 (FP64); returns the partial r.r
+__device__ __forceinline__ double resid_tile(const KProb &pr, int row0, int row1, double *lds)
 {
-    const int lane = threadIdx.x % L;
     double acc = 0.0;
-    for (int i = row0 + (int)threadIdx.x / L; i < row1; i += KB / L) {
-        double s = csr_row<L>(pr.rowptr, pr.cols, pr.val64, pr.x, i, lane);
-        if (lane == 0) {
+    for (int r0 = row0; r0 < row1; r0 += KB) {
+        const int r1 = r0 + KB < row1 ? r0 + KB : row1;
+        const double s = csr_rows_lds<double>(pr.rowptr, pr.cols, pr.val64, pr.x, r0, r1, lds);
+        const int i = r0 + (int)threadIdx.x;
+        if (i < r1) {
             double ri = pr.fixed[i] ? 0.0 : pr.b[i] - s;
             pr.r64[i] = ri;
             acc += ri * ri;
@@ -432,19 +426,14 @@ __device__ __forceinline__ double resid_tile(const KProb &pr, int row0, int row1
 
 __global__ __launch_bounds__(KB) void k_eng_resid(ENGINE_ARGS)
 {
+    __shared__ double lds[SPMV_CAP];
     const int nt = *ntiles;
     for (int t = blockIdx.x; t < nt; t += gridDim.x) {
         const KTile tl = T[t];
         const int pi = tl.prob;
         if (P[pi].phase != PH_RESID) continue;
         const KProb pr = P[pi];
-        double acc = 0.0, dummy = 0.0;
-        switch (pr.lanes) {
-        case 2: acc = resid_tile<2>(pr, tl.row0, tl.row1); break;
-        case 4: acc = resid_tile<4>(pr, tl.row0, tl.row1); break;
-        case 8: acc = resid_tile<8>(pr, tl.row0, tl.row1); break;
-        default: acc = resid_tile<16>(pr, tl.row0, tl.row1); break;
-        }
+        double acc = resid_tile(pr, tl.row0, tl.row1, lds), dummy = 0.0;
         block_sum2(acc, dummy);
         if (threadIdx.x == 0) part[2 * t] = acc;
     }

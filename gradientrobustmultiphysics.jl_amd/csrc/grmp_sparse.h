@@ -65,3 +65,36 @@ __device__ __forceinline__ float csr_row32(const int32_t *__restrict__ rp, const
     return s;
 }
 
+
+// Nonzeros staged per block by csr_rows_lds (32 KB of LDS in FP64).
+#define SPMV_CAP 4096
+
+// Rows r0 .. r1-1 (r1 - r0 <= blockDim.x) of y = A x.  The block streams its
+// contiguous nonzero range rowptr[r0] .. rowptr[r1] through LDS as products
+// v[k] * x[col[k]] (coalesced loads of values and columns), then thread t sums
+// row r0 + t in column order.  Every thread of the block must call it with the
+// same r0, r1; the result is valid for threads with r0 + t < r1.
+template <class T, class V, class X>
+__device__ __forceinline__ T csr_rows_lds(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                          const V *__restrict__ v, const X *__restrict__ x, int64_t r0, int64_t r1,
+                                          T *__restrict__ lds)
+{
+    const int t = threadIdx.x;
+    const int64_t r = r0 + t;
+    const int32_t base = rp[r0], end = rp[r1];
+    int32_t my0 = 0, my1 = 0;
+    if (r < r1) {
+        my0 = rp[r];
+        my1 = rp[r + 1];
+    }
+    T s = (T)0;
+    for (int32_t c0 = base; c0 < end; c0 += SPMV_CAP) {
+        const int32_t c1 = c0 + SPMV_CAP < end ? c0 + SPMV_CAP : end;
+        __syncthreads();
+        for (int32_t k = c0 + t; k < c1; k += blockDim.x) lds[k - c0] = (T)v[k] * (T)x[ci[k]];
+        __syncthreads();
+        const int32_t a = my0 > c0 ? my0 : c0, b = my1 < c1 ? my1 : c1;
+        for (int32_t k = a; k < b; ++k) s += lds[k - c0];
+    }
+    return s;
+}
