@@ -48,9 +48,9 @@ def make_stream(nreq, rate, seed=2024):
             for i in range(nreq)]
 
 
-def serve(grmp, mode, stream, rtol, keep=(), profile=False, respect_arrivals=True):
+def serve(grmp, mode, stream, rtol, keep=(), profile=False, respect_arrivals=True, tile_rows=0):
     """Submits the stream at its arrival times and waits for every result."""
-    S = grmp.Server(mode, profile=profile, memory_budget=96 << 30, max_active=512)
+    S = grmp.Server(mode, profile=profile, memory_budget=96 << 30, max_active=512, tile_rows=tile_rows)
     keep = set(keep)
     tickets = []
     t0 = time.perf_counter()
@@ -74,7 +74,7 @@ def serve(grmp, mode, stream, rtol, keep=(), profile=False, respect_arrivals=Tru
     ok = np.array([r.status == 0 for r in res])
     dofs = np.array([r.ndofs for r in res])
     return dict(wall=wall, latency=lat, ok=ok, dofs=dofs, iters=np.array([r.iterations for r in res]),
-                status=np.array([r.status for r in res]),
+                status=np.array([r.status for r in res]), fallback=np.array([r.fp64_fallback for r in res]),
                 outer=np.array([r.outer for r in res]), sols=sols, stats=stats,
                 service=np.array([r.t_done - r.t_admit for r in res]))
 
@@ -291,6 +291,7 @@ def main():
                     help="served (the headline stream) | c2-p1 | c2-p2 | c3 | c4 (single large instances)")
     ap.add_argument("--size", type=int, default=0, help="single-instance workloads: N (default 2048 / 160 / 512)")
     ap.add_argument("--solve", action="store_true", help="c3: also solve")
+    ap.add_argument("--tile-rows", type=int, default=0, help="engine rows per work tile (0 = library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the served stream shards by request "
@@ -316,7 +317,7 @@ def main():
     for s in range(args.steps):
         log(f"[bench] step {s}: {len(stream)} requests at {args.rate:g}/s")
         GE.load().default_context().sync()
-        runs.append(serve(grmp, "served", stream, args.rtol, keep=vidx, profile=True))
+        runs.append(serve(grmp, "served", stream, args.rtol, keep=vidx, profile=True, tile_rows=args.tile_rows))
         GE.load().default_context().sync()
     run = runs[-1]
     walls = [r["wall"] for r in runs]
@@ -386,7 +387,14 @@ def main():
         "engine": {"steps": int(st["steps"]), "templates": int(st["templates"]),
                    "template_seconds": round(st["template_seconds"], 3),
                    "mean_outer_sweeps": round(float(run["outer"].mean()), 2),
-                   "mean_iterations": round(float(run["iters"].mean()), 1)},
+                   "mean_iterations": round(float(run["iters"].mean()), 1),
+                   "p99_iterations": int(np.percentile(run["iters"], 99)),
+                   "max_iterations": int(run["iters"].max()),
+                   "fp64_fallbacks": int(run["fallback"].sum()),
+                   "slowest": {k: v for k, v in stream[int(np.argmax(run["iters"]))].items() if k != "t"} | {
+                       "iterations": int(run["iters"].max()), "outer": int(run["outer"][int(np.argmax(run["iters"]))]),
+                       "fp64_fallback": int(run["fallback"][int(np.argmax(run["iters"]))])},
+                   "tile_rows": args.tile_rows or 2048},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -75,18 +75,18 @@ class SolveInfo(ctypes.Structure):
 class Request(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("fe", ctypes.c_int), ("N", ctypes.c_int), ("seed", ctypes.c_uint64),
                 ("sigma", ctypes.c_double), ("E", ctypes.c_double), ("nu", ctypes.c_double),
-                ("rtol", ctypes.c_double), ("x_out", DP)]
+                ("rtol", ctypes.c_double), ("x_out", DP), ("precision", ctypes.c_int), ("pad_", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("ndofs", I64), ("iterations", I64), ("outer", ctypes.c_int),
                 ("relres", ctypes.c_double), ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
-                ("t_done", ctypes.c_double)]
+                ("t_done", ctypes.c_double), ("fp64_fallback", ctypes.c_int), ("pad_", ctypes.c_int)]
 
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("memory_budget", I64), ("max_active", ctypes.c_int),
-                ("graph", ctypes.c_int), ("profile", ctypes.c_int)]
+                ("graph", ctypes.c_int), ("profile", ctypes.c_int), ("tile_rows", ctypes.c_int)]
 
 
 class KernelStats(ctypes.Structure):

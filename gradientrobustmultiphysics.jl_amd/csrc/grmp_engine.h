@@ -12,7 +12,21 @@
 #pragma once
 #include "grmp_internal.h"
 
-enum KPhase { PH_IDLE = 0, PH_RESID = 1, PH_INIT = 2, PH_ITER = 3, PH_UPDATE = 4, PH_DONE = 5, PH_FAIL = 6, PH_REINIT = 7 };
+enum KPhase {
+    PH_IDLE = 0,
+    PH_RESID = 1,
+    PH_INIT = 2,
+    PH_ITER = 3,
+    PH_UPDATE = 4,
+    PH_DONE = 5,
+    PH_FAIL = 6,
+    PH_REINIT = 7,
+    PH_INIT64 = 8  // start (or restart) FP64 CG from the FP64 residual
+};
+
+// Rows per engine tile (the unit of work of one block) and the largest grid.
+#define ENGINE_TILE_ROWS 2048
+#define ENGINE_MAX_GRID 4096
 
 struct KProb {
     // matrix (CSR, 0-based), fixed-dof mask, Jacobi inverse diagonal
@@ -21,6 +35,7 @@ struct KProb {
     const double *val64;
     const float *val32;
     const float *dinv32;
+    const double *dinv64;
     const uint8_t *fixed;
     // FP64 outer vectors
     const double *b;
@@ -35,7 +50,10 @@ struct KProb {
     int32_t restart;    // restart the search direction at the next refresh
     int32_t discard;    // the FP32 sweep overflowed: drop its correction
     int32_t stall;      // consecutive refreshes that did not halve the best residual
-    int32_t pad_;
+    int32_t fp64;       // FP64 fallback: the FP32 sweeps stalled, plain FP64 CG finishes
+    int32_t slow;       // the current FP32 sweep ran past its iteration cap
+    int64_t sweep0;     // iterations done when the current FP32 sweep started
+    double *p64, *q64;  // FP64 direction and product (alias the FP32 vector storage)
     int64_t iters, maxit;
     double bb, bnorm;   // ||b - A x0||^2 and its root (free rows)
     double rtol;        // target relative FP64 residual
@@ -108,6 +126,7 @@ class KEngine {
     hipEvent_t ev_[2 * EV_PAIRS] = {};
     int npending_ = 0;
     unsigned long long *dbytes_ = nullptr;
+    unsigned long long *dbytes_blk_ = nullptr;  // per-block SpMV bytes of the last step
     double spmv_s_ = 0.0;
     int64_t spmv_n_ = 0, steps_ = 0;
     int read_events(int pairs);

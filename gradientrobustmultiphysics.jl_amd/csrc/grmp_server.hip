@@ -32,7 +32,7 @@
 
 #include "grmp_engine.h"
 
-static const int64_t kTileRows = 2048;
+static const int64_t kTileRows = ENGINE_TILE_ROWS;
 
 static inline int64_t req_nnodes(const grmp_request &r) { return (int64_t)(r.N + 1) * (r.N + 1); }
 static inline int64_t req_ncells(const grmp_request &r) { return 2ll * r.N * r.N; }
@@ -276,6 +276,7 @@ static int job_activate(grmp_server *S, Job *j)
     p.val64 = j->val64;
     p.val32 = j->val32;
     p.dinv32 = j->dinv32;
+    p.dinv64 = j->dinv64;
     p.fixed = A->fixed;
     p.b = j->b;
     p.x = j->x;
@@ -284,12 +285,15 @@ static int job_activate(grmp_server *S, Job *j)
     p.p = j->f4 + j->n;
     p.q = j->f4 + 2 * j->n;
     p.dx = j->f4 + 3 * j->n;
+    p.p64 = (double *)j->f4;
+    p.q64 = (double *)(j->f4 + 2 * j->n);
     p.n = (int32_t)j->n;
     p.lanes = j->tpl->lanes;
     p.max_outer = 200;
     p.maxit = 4000000;
     p.rtol = j->req.rtol > 0 ? j->req.rtol : 1e-10;
     p.delta = 0.1;  // reliable update when the FP32 residual dropped 10x
+    p.fp64 = j->req.precision == GRMP_PRECISION_FP64;
     GRMP_TRY(S->eng->add(p, &j->slot));
     j->state = 2;
     return GRMP_OK;
@@ -315,6 +319,7 @@ static void job_finish(grmp_server *S, Job *j, const KProb &q)
     j->res.iterations = q.iters;
     j->res.outer = q.outer;
     j->res.relres = q.relres;
+    j->res.fp64_fallback = q.fp64;
     j->res.ndofs = j->n;
     if (j->req.x_out) {
         (void)hipMemcpyAsync(j->req.x_out, j->x, j->n * 8, hipMemcpyDeviceToHost, S->st_solve);
@@ -501,12 +506,14 @@ extern "C" int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *cfg, 
     S->cfg.max_active = 256;
     S->cfg.graph = 1;
     S->cfg.profile = 0;
+    S->cfg.tile_rows = (int)kTileRows;
     if (cfg) {
         if (cfg->mode) S->cfg.mode = cfg->mode;
         if (cfg->memory_budget > 0) S->cfg.memory_budget = cfg->memory_budget;
         if (cfg->max_active > 0) S->cfg.max_active = std::min(cfg->max_active, 4096);
         S->cfg.graph = cfg->graph;
         S->cfg.profile = cfg->profile;
+        if (cfg->tile_rows > 0) S->cfg.tile_rows = std::max(64, std::min(cfg->tile_rows, 8192));
     }
     int rc = grmp_ctx_create(ctx->device, &S->sctx);
     if (rc) { delete S; return grmp_fail(ctx, rc, "grmp_server_create: no server context"); }
@@ -516,7 +523,7 @@ extern "C" int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *cfg, 
         return grmp_fail(ctx, GRMP_ERR_HIP, "grmp_server_create: stream");
     }
     if (S->cfg.mode == 1) {
-        S->eng = new KEngine(S->sctx, S->st_solve, S->cfg.max_active, 1 << 18, (int)kTileRows);
+        S->eng = new KEngine(S->sctx, S->st_solve, S->cfg.max_active, 1 << 20, S->cfg.tile_rows);
         S->eng->set_profile(S->cfg.profile != 0);
         if ((rc = S->eng->init())) {
             delete S->eng;

@@ -316,24 +316,65 @@ __device__ __forceinline__ double spmv_tile(const KProb &pr, int row0, int row1,
     return acc;
 }
 
+// FP64 fallback: q64 = A64 p64 on free rows; returns the partial p.q.  lds
+// holds SPMV_CAP / 2 doubles (the FP32 tiles' buffer).
+__device__ __forceinline__ double spmv_tile64(const KProb &pr, int row0, int row1, double *lds)
+{
+    double acc = 0.0;
+    for (int r0 = row0; r0 < row1; r0 += KB) {
+        const int r1 = r0 + KB < row1 ? r0 + KB : row1;
+        const double s = csr_rows_lds<double, double, double, SPMV_CAP / 2>(pr.rowptr, pr.cols, pr.val64, pr.p64, r0,
+                                                                             r1, lds);
+        const int i = r0 + (int)threadIdx.x;
+        if (i < r1) {
+            const double qi = pr.fixed[i] ? 0.0 : s;
+            pr.q64[i] = qi;
+            acc += pr.p64[i] * qi;
+        }
+    }
+    return acc;
+}
+
 __global__ __launch_bounds__(KB) void k_eng_spmv(ENGINE_ARGS)
 {
-    __shared__ float lds32[SPMV_CAP];
+    // SPMV_CAP floats; FP64 fallback tiles stage SPMV_CAP / 2 doubles in it
+    __shared__ __align__(16) float lds32[SPMV_CAP];
+    unsigned long long myb = 0;
     const int nt = *ntiles;
     for (int t = blockIdx.x; t < nt; t += gridDim.x) {
         const KTile tl = T[t];
         const int pi = tl.prob;
         if (P[pi].phase != PH_ITER) continue;
         const KProb pr = P[pi];
-        double acc = spmv_tile(pr, tl.row0, tl.row1, lds32), dummy = 0.0;
+        double acc = pr.fp64 ? spmv_tile64(pr, tl.row0, tl.row1, (double *)lds32) : spmv_tile(pr, tl.row0, tl.row1, lds32);
+        double dummy = 0.0;
         block_sum2(acc, dummy);
         if (threadIdx.x == 0) {
             part[2 * t] = acc;
             // algorithmic bytes of this tile: values + columns, row pointers, x gather, y write
             const unsigned long long rows = (unsigned long long)(tl.row1 - tl.row0);
             const unsigned long long nz = (unsigned long long)(pr.rowptr[tl.row1] - pr.rowptr[tl.row0]);
-            atomicAdd(nbytes, nz * 8ull + 4ull * (rows + 1) + 8ull * rows);
+            const unsigned long long sv = pr.fp64 ? 8ull : 4ull;
+            myb += nz * (sv + 4ull) + 4ull * (rows + 1) + 2ull * sv * rows;
         }
+    }
+    // per-block byte count, summed by k_eng_bytes (no atomics in the hot loop)
+    if (threadIdx.x == 0) nbytes[blockIdx.x] = myb;
+}
+
+// adds the per-block SpMV byte counts of one step to the running total
+__global__ __launch_bounds__(KB) void k_eng_bytes(const unsigned long long *__restrict__ blk, int nblk,
+                                                  unsigned long long *__restrict__ total)
+{
+    __shared__ unsigned long long sh[KB];
+    unsigned long long a = 0;
+    for (int k = threadIdx.x; k < nblk; k += KB) a += blk[k];
+    sh[threadIdx.x] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int k = 0; k < KB; ++k) t += sh[k];
+        *total += t;
     }
 }
 
@@ -345,8 +386,25 @@ __global__ __launch_bounds__(KB) void k_eng_axpy(ENGINE_ARGS)
         const int pi = tl.prob;
         if (P[pi].phase != PH_ITER) continue;
         const KProb pr = P[pi];
-        const float a = (float)pr.alpha;
         double rz = 0.0, rr = 0.0;
+        if (pr.fp64) {
+            // FP64 fallback: x += alpha p; r -= alpha q; z = D r
+            const double a = pr.alpha;
+            for (int i = tl.row0 + threadIdx.x; i < tl.row1; i += KB) {
+                pr.x[i] += a * pr.p64[i];
+                const double ri = pr.r64[i] - a * pr.q64[i];
+                pr.r64[i] = ri;
+                rz += ri * (pr.dinv64[i] * ri);
+                rr += ri * ri;
+            }
+            block_sum2(rz, rr);
+            if (threadIdx.x == 0) {
+                part[2 * t] = rz;
+                part[2 * t + 1] = rr;
+            }
+            continue;
+        }
+        const float a = (float)pr.alpha;
         for (int i = tl.row0 + threadIdx.x; i < tl.row1; i += KB) {
             pr.dx[i] += a * pr.p[i];
             float ri = pr.r[i] - a * pr.q[i];
@@ -370,8 +428,29 @@ __global__ __launch_bounds__(KB) void k_eng_elem(ENGINE_ARGS)
         const KTile tl = T[t];
         const int pi = tl.prob;
         const int ph = P[pi].phase;
-        if (ph != PH_ITER && ph != PH_UPDATE && ph != PH_INIT && ph != PH_REINIT) continue;
+        if (ph != PH_ITER && ph != PH_UPDATE && ph != PH_INIT && ph != PH_REINIT && ph != PH_INIT64) continue;
         const KProb pr = P[pi];
+        if (ph == PH_ITER && pr.fp64) {
+            const double beta = pr.beta;
+            for (int i = tl.row0 + threadIdx.x; i < tl.row1; i += KB) pr.p64[i] = pr.dinv64[i] * pr.r64[i] + beta * pr.p64[i];
+            continue;
+        }
+        if (ph == PH_INIT64) {
+            double rz = 0.0, rr = 0.0;
+            for (int i = tl.row0 + threadIdx.x; i < tl.row1; i += KB) {
+                const double ri = pr.r64[i];
+                const double zi = pr.dinv64[i] * ri;
+                pr.p64[i] = zi;
+                rz += ri * zi;
+                rr += ri * ri;
+            }
+            block_sum2(rz, rr);
+            if (threadIdx.x == 0) {
+                part[2 * t] = rz;
+                part[2 * t + 1] = rr;
+            }
+            continue;
+        }
         if (ph == PH_ITER) {
             const float beta = (float)pr.beta;
             for (int i = tl.row0 + threadIdx.x; i < tl.row1; i += KB) pr.p[i] = pr.dinv32[i] * pr.r[i] + beta * pr.p[i];
@@ -465,7 +544,7 @@ __global__ __launch_bounds__(KB) void k_eng_reduce(KProb *__restrict__ P, const 
         const int ph = P[s].phase;
         if (WHICH == RED_A || WHICH == RED_B) {
             if (ph != PH_ITER) continue;
-        } else if (ph != PH_UPDATE && ph != PH_INIT && ph != PH_REINIT && ph != PH_RESID) {
+        } else if (ph != PH_UPDATE && ph != PH_INIT && ph != PH_REINIT && ph != PH_RESID && ph != PH_INIT64) {
             continue;
         }
         double a = 0.0, b = 0.0;
@@ -482,7 +561,7 @@ __global__ __launch_bounds__(KB) void k_eng_reduce(KProb *__restrict__ P, const 
                 w.alpha = 0.0;
                 w.restart = 1;        // breakdown: refresh and restart the direction
                 w.discard = !isfin(a) || !isfin(w.rz);
-                w.phase = PH_UPDATE;
+                w.phase = w.fp64 ? PH_RESID : PH_UPDATE;
             } else {
                 w.alpha = alpha;
             }
@@ -492,6 +571,19 @@ __global__ __launch_bounds__(KB) void k_eng_reduce(KProb *__restrict__ P, const 
             // refreshed iteration continues along the new direction.
             w.iters += 1;
             const double rn = sqrt(b);
+            if (w.fp64) {
+                // FP64 fallback: the recurrence residual decides when the true
+                // residual is checked
+                if (!isfin(a) || !isfin(b)) {
+                    w.status = GRMP_ERR_NONFINITE;
+                    w.phase = PH_FAIL;
+                } else {
+                    w.beta = a / w.rz;
+                    w.rz = a;
+                    if (rn <= w.rtol * w.bnorm || w.iters >= w.maxit) w.phase = PH_RESID;
+                }
+                continue;
+            }
             if (!isfin(a) || !isfin(b)) {
                 w.restart = 1;
                 w.discard = 1;
@@ -500,6 +592,13 @@ __global__ __launch_bounds__(KB) void k_eng_reduce(KProb *__restrict__ P, const 
                 w.beta = a / w.rz;
                 w.rz = a;
                 w.rmax = rn > w.rmax ? rn : w.rmax;
+                // once two sweeps are complete, a sweep running past 4x their
+                // mean length has lost its convergence (FP32 recurrences near
+                // the FP64 floor of an ill-conditioned problem): it ends and
+                // the problem finishes in FP64
+                const int64_t done = w.outer - 1;
+                const int64_t cur = w.iters - w.sweep0;
+                const int64_t cap = done >= 2 ? max((int64_t)256, 4 * (w.sweep0 / done)) : (int64_t)1 << 62;
                 if (rn > 1e6) {
                     // the scaled residual started near 1: a sweep that grew it
                     // a millionfold has diverged and is dropped
@@ -508,11 +607,14 @@ __global__ __launch_bounds__(KB) void k_eng_reduce(KProb *__restrict__ P, const 
                     w.phase = PH_UPDATE;
                 } else if (rn <= w.rtol * w.bnorm * w.scale || rn < w.delta * w.rmax || w.iters >= w.maxit) {
                     w.phase = PH_UPDATE;  // converged estimate or reliable update
+                } else if (cur > cap) {
+                    w.slow = 1;
+                    w.phase = PH_UPDATE;
                 }
             }
         } else if (ph == PH_UPDATE) {
             w.phase = PH_RESID;
-        } else if (ph == PH_INIT || ph == PH_REINIT) {
+        } else if (ph == PH_INIT || ph == PH_REINIT || ph == PH_INIT64) {
             w.rz = a;
             w.rmax = sqrt(b);
             w.phase = PH_ITER;
@@ -538,6 +640,28 @@ __global__ __launch_bounds__(KB) void k_eng_reduce(KProb *__restrict__ P, const 
                 const bool stalled = w.outer > 0 && !(w.relres <= 0.5 * w.best);
                 if (w.outer == 0 || w.relres < w.best) w.best = w.relres;
                 w.stall = stalled ? w.stall + 1 : 0;
+                if (w.fp64 && w.stall >= 2) {
+                    // FP64 CG restarts no longer halve the true residual either:
+                    // x is at the FP64 floor of b - A x.  Within 100 rtol that
+                    // is the attainable answer; below, the tolerance cannot be met
+                    if (w.relres <= 100.0 * w.rtol) {
+                        w.phase = PH_DONE;
+                    } else {
+                        w.status = GRMP_ERR_NOCONV;
+                        w.phase = PH_FAIL;
+                    }
+                    continue;
+                }
+                if (w.fp64 || w.stall >= 2 || w.slow) {
+                    // two refreshes in a row without halving the residual, or a
+                    // sweep past its iteration cap: the FP32 recurrences are too
+                    // coarse for this problem's condition number, and FP64 CG
+                    // finishes it from the current x
+                    w.fp64 = 1;
+                    w.phase = PH_INIT64;
+                    w.outer += 1;
+                    continue;
+                }
                 const double snew = 1.0 / sqrt(a);
                 const bool fresh = w.outer == 0 || w.restart || stalled;
                 w.pscale = fresh ? 1.0 : snew / w.scale;
@@ -545,6 +669,7 @@ __global__ __launch_bounds__(KB) void k_eng_reduce(KProb *__restrict__ P, const 
                 w.phase = fresh ? PH_INIT : PH_REINIT;
                 w.restart = 0;
                 w.discard = 0;
+                w.sweep0 = w.iters;
                 w.outer += 1;
             }
         }
@@ -567,6 +692,7 @@ KEngine::~KEngine()
     if (dpart_) (void)hipFree(dpart_);
     if (dnt_) (void)hipFree(dnt_);
     if (dbytes_) (void)hipFree(dbytes_);
+    if (dbytes_blk_) (void)hipFree(dbytes_blk_);
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
     if (hP_) (void)hipHostFree(hP_);
@@ -583,6 +709,7 @@ int KEngine::init()
     GRMP_HIP(ctx, hipMalloc(&dpart_, sizeof(double) * 2 * cap_t_));
     GRMP_HIP(ctx, hipMalloc(&dnt_, sizeof(int)));
     GRMP_HIP(ctx, hipMalloc(&dbytes_, sizeof(unsigned long long)));
+    GRMP_HIP(ctx, hipMalloc(&dbytes_blk_, sizeof(unsigned long long) * ENGINE_MAX_GRID));
     GRMP_HIP(ctx, hipMemsetAsync(dbytes_, 0, sizeof(unsigned long long), st_));
     for (auto &e : ev_) GRMP_HIP(ctx, hipEventCreate(&e));
     GRMP_HIP(ctx, hipHostMalloc((void **)&hP_, sizeof(KProb) * cap_p_, hipHostMallocDefault));
@@ -679,10 +806,11 @@ void KEngine::launch_kernels(int grid, bool timed)
     dim3 g(grid), b(KB), gr(std::min(cap_p_, 1024));
     const int pr = npending_;
     if (timed) (void)hipEventRecord(ev_[2 * pr], st_);
-    hipLaunchKernelGGL(k_eng_spmv, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
+    hipLaunchKernelGGL(k_eng_spmv, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_blk_);
     if (timed) {
         (void)hipEventRecord(ev_[2 * pr + 1], st_);
         ++npending_;
+        hipLaunchKernelGGL(k_eng_bytes, dim3(1), b, 0, st_, dbytes_blk_, grid, dbytes_);
     }
     hipLaunchKernelGGL(k_eng_reduce<RED_A>, gr, b, 0, st_, dP_, dR_, dpart_, cap_p_);
     hipLaunchKernelGGL(k_eng_axpy, g, b, 0, st_, dP_, dT_, dR_, dpart_, dnt_, dbytes_);
@@ -747,7 +875,7 @@ int KEngine::run(int steps, bool use_graph)
     GRMP_TRY(sync_tables());
     if (ttot_ == 0) return GRMP_OK;
     int grid = 64;
-    while (grid < ttot_ && grid < 4096) grid <<= 1;
+    while (grid < ttot_ && grid < ENGINE_MAX_GRID) grid <<= 1;
     // profiling times every SpMV launch with an event pair, which a graph
     // replay cannot carry, so profiled runs launch the steps one by one
     if (use_graph && !profile_) {
@@ -798,7 +926,7 @@ static int solve_engine(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, co
     GRMP_TRY(dalloc(ctx, &buf, 4 * n));
     hipLaunchKernelGGL(k_plain_init, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, A->rowptr, A->diagpos, A->val64, A->fixed,
                        b->d, x->d, n);
-    KEngine eng(ctx, ctx->stream, 1, (int)((n + 2047) / 2048) + 1, 2048);
+    KEngine eng(ctx, ctx->stream, 1, (int)((n + ENGINE_TILE_ROWS - 1) / ENGINE_TILE_ROWS) + 1, ENGINE_TILE_ROWS);
     GRMP_TRY(eng.init());
     KProb p;
     memset(&p, 0, sizeof(p));
@@ -807,6 +935,7 @@ static int solve_engine(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, co
     p.val64 = A->val64;
     p.val32 = A->val32;
     p.dinv32 = A->dinv32;
+    p.dinv64 = A->dinv64;
     p.fixed = A->fixed;
     p.b = b->d;
     p.x = x->d;
@@ -815,6 +944,8 @@ static int solve_engine(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, co
     p.p = buf + n;
     p.q = buf + 2 * n;
     p.dx = buf + 3 * n;
+    p.p64 = (double *)buf;
+    p.q64 = (double *)(buf + 2 * n);
     p.n = (int32_t)n;
     p.lanes = engine_pick_lanes((double)A->nnz / (double)n);
     p.max_outer = 200;

@@ -74,7 +74,8 @@ __device__ __forceinline__ float csr_row32(const int32_t *__restrict__ rp, const
 // v[k] * x[col[k]] (coalesced loads of values and columns), then thread t sums
 // row r0 + t in column order.  Every thread of the block must call it with the
 // same r0, r1; the result is valid for threads with r0 + t < r1.
-template <class T, class V, class X>
+// CAP is the number of entries lds holds.
+template <class T, class V, class X, int CAP = SPMV_CAP>
 __device__ __forceinline__ T csr_rows_lds(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                                           const V *__restrict__ v, const X *__restrict__ x, int64_t r0, int64_t r1,
                                           T *__restrict__ lds)
@@ -88,8 +89,8 @@ __device__ __forceinline__ T csr_rows_lds(const int32_t *__restrict__ rp, const 
         my1 = rp[r + 1];
     }
     T s = (T)0;
-    for (int32_t c0 = base; c0 < end; c0 += SPMV_CAP) {
-        const int32_t c1 = c0 + SPMV_CAP < end ? c0 + SPMV_CAP : end;
+    for (int32_t c0 = base; c0 < end; c0 += CAP) {
+        const int32_t c1 = c0 + CAP < end ? c0 + CAP : end;
         __syncthreads();
         for (int32_t k = c0 + t; k < c1; k += blockDim.x) lds[k - c0] = (T)v[k] * (T)x[ci[k]];
         __syncthreads();

@@ -16,10 +16,11 @@ from ._abi import VP, default_context, lib
 POISSON, ELASTICITY = _abi.REQ_POISSON, _abi.REQ_ELASTICITY
 
 
-def request(kind, fe, N, seed, sigma=1.0, rtol=1e-10, E=1.0, nu=0.3, x_out=None):
+def request(kind, fe, N, seed, sigma=1.0, rtol=1e-10, E=1.0, nu=0.3, x_out=None, precision="fp32_ir"):
     r = _abi.Request()
     r.kind, r.fe, r.N, r.seed = int(kind), int(fe), int(N), int(seed)
     r.sigma, r.E, r.nu, r.rtol = float(sigma), float(E), float(nu), float(rtol)
+    r.precision = _abi.PRECISION_FP64 if precision == "fp64" else _abi.PRECISION_FP32_IR
     r.x_out = x_out.ctypes.data_as(_abi.DP) if x_out is not None else None
     return r
 
@@ -33,7 +34,8 @@ def footprint(kind, fe, N):
 
 
 class Server:
-    def __init__(self, mode="served", memory_budget=64 << 30, max_active=256, graph=True, profile=False, ctx=None):
+    def __init__(self, mode="served", memory_budget=64 << 30, max_active=256, graph=True, profile=False, ctx=None,
+                 tile_rows=0):
         self.ctx = ctx or default_context()
         cfg = _abi.ServerConfig()
         cfg.mode = 1 if mode == "served" else 2
@@ -41,6 +43,7 @@ class Server:
         cfg.max_active = int(max_active)
         cfg.graph = 1 if graph else 0
         cfg.profile = 1 if profile else 0
+        cfg.tile_rows = int(tile_rows)
         h = VP()
         self.ctx.check(lib().grmp_server_create(self.ctx.h, ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h

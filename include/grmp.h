@@ -254,6 +254,9 @@ typedef struct grmp_request {
     double E, nu;             /* elasticity */
     double rtol;
     double *x_out;            /* host buffer of ndofs doubles, filled on completion (may be NULL) */
+    int precision;            /* 0 or GRMP_PRECISION_FP32_IR: FP32 sweeps with FP64 refinement and
+                                 an FP64 fallback; GRMP_PRECISION_FP64: FP64 CG in the engine */
+    int pad_;
 } grmp_request;
 
 typedef struct grmp_result {
@@ -263,6 +266,8 @@ typedef struct grmp_result {
     int outer;
     double relres;
     double t_submit, t_admit, t_done; /* seconds on the server clock */
+    int fp64_fallback;        /* 1 when the FP32 sweeps stalled and FP64 CG finished the solve */
+    int pad_;
 } grmp_result;
 
 typedef struct grmp_server_config {
@@ -271,6 +276,7 @@ typedef struct grmp_server_config {
     int max_active;           /* problems in flight */
     int graph;                /* capture the Krylov step as a hipGraph */
     int profile;              /* time the engine SpMV kernel with HIP events */
+    int tile_rows;            /* engine rows per work tile (0 = 2048; 64 .. 8192) */
 } grmp_server_config;
 
 typedef struct grmp_kernel_stats {

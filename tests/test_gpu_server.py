@@ -92,7 +92,9 @@ def test_failures_stay_local(grmp):
         ts = [S.submit(*spec) for spec in good]
         tbad = S.submit(1, 1, 12, 10, rtol=1e-30)
         assert S.wait(tnan).status == grmp._abi.ERR_NONFINITE
-        assert S.wait(tbad).status == grmp._abi.ERR_NOCONV
+        rb = S.wait(tbad)
+        # the FP32 sweeps stall, FP64 CG takes over and stalls at its floor
+        assert rb.status == grmp._abi.ERR_NOCONV and rb.fp64_fallback == 1
         for spec, t in zip(good, ts):
             assert S.wait(t).status == 0
             assert np.array_equal(S.solution(t), alone[spec][2])
@@ -133,3 +135,33 @@ def test_profiled_engine_counts_spmv(grmp):
         st = S.stats()
     assert st["spmv_launches"] > 0 and st["spmv_seconds"] > 0 and st["spmv_bytes"] > 0
     assert st["steps"] >= st["spmv_launches"]
+
+
+def test_fp64_engine_requests(grmp, O):
+    """FP64 requests run plain FP64 CG inside the batched engine (the path
+    stalled FP32 sweeps fall back to): they match sparse LU and leave their
+    neighbours in the batch unchanged."""
+    print("[case] server FP64 engine requests", flush=True)
+    good = MIX[:2]
+    alone = _alone(grmp, good)
+    kind, fe, N, seed = 1, 2, 40, 21
+    with grmp.Server("served") as S:
+        ts = [S.submit(*spec) for spec in good]
+        t = S.submit(kind, fe, N, seed, rtol=1e-11, precision="fp64")
+        r = S.wait(t)
+        assert r.status == 0 and r.fp64_fallback == 1
+        x = S.solution(t)
+        for spec, tt in zip(good, ts):
+            assert S.wait(tt).status == 0
+            assert np.array_equal(S.solution(tt), alone[spec][2])
+    grid = grmp.grid_unitsquare(N)
+    kv = grmp.DeviceVector(grid.ctx, grid.ncells)
+    kv.lognormal(seed, 0.0, 1.0)
+    Sp = O.Space(N, fe)
+    A = O.assemble_matrix(Sp.ndofs, [dict(kind=O.BLF_LAPLACE, fe=fe, quadorder=2, factor=1.0, coef=O.COEF_CELL,
+                                          kappa=kv.download(), coords=Sp.coords, cells=Sp.cells,
+                                          celldofs=Sp.celldofs)], fixed=Sp.fixed)
+    b = np.zeros(Sp.ndofs)
+    O.assemble_lf_const(b, fe, 1, fe, [1.0], Sp.coords, Sp.cells, Sp.celldofs)
+    xref = O.direct_solve(A, O.penalized_rhs(b, Sp.fixed))
+    assert np.linalg.norm(x - xref) / np.linalg.norm(xref) <= 1e-9
