@@ -279,7 +279,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0, help="untimed runs of the first 256 requests")
     ap.add_argument("--requests", type=int, default=4096)
     ap.add_argument("--rate", type=float, default=5000.0)
     ap.add_argument("--rtol", type=float, default=1e-10)
