@@ -706,12 +706,102 @@ __global__ void k_assemble(FormPack P, int64_t g0, int64_t g1, int64_t rblock, c
         g0, g1, rblock, rowptr, diagpos, fixed, penalty, val64, val32, dinv64, dinv32);
 }
 
-// one form of a known kind, passed by value (kernel arguments stay scalar)
+// One form of a known kind, passed by value (kernel arguments stay scalar),
+// with the cell work spread over the block.  The block owns rb <= ASM_NT rows
+// and walks their contiguous incidence range in chunks:
+//   1. every thread computes local rows of the chunk's incidences into LDS
+//      (the gathers of all threads are in flight at once);
+//   2. each row's thread adds its incidences of the chunk, in incidence
+//      order, through the slot map into its LDS row.
+// Per entry the summation order is the serial walk's (cell order), so the
+// values are the same bits.
+#define ASM_NT 256
+#define ASM_STAGE 2048  // FP64 staging entries (16 KB)
 template <class K>
-__global__ void k_assemble1(DevForm F, int64_t g0, int64_t g1, int64_t rblock, const int32_t *__restrict__ rowptr,
-                            const uint8_t *__restrict__ diagpos, const uint8_t *__restrict__ fixed, double penalty,
-                            double *__restrict__ val64, float *__restrict__ val32, double *__restrict__ dinv64,
-                            float *__restrict__ dinv32)
+__global__ __launch_bounds__(ASM_NT) void k_assemble1(DevForm F, int rb, int64_t g0, int64_t g1, int64_t rblock,
+                                                      const int32_t *__restrict__ rowptr,
+                                                      const uint8_t *__restrict__ diagpos,
+                                                      const uint8_t *__restrict__ fixed, double penalty,
+                                                      double *__restrict__ val64, float *__restrict__ val32,
+                                                      double *__restrict__ dinv64, float *__restrict__ dinv32)
+{
+    constexpr int CN = K::CN, W = (CN + 3) / 4, SCAP = ASM_STAGE / CN;
+    extern __shared__ double shm[];
+    double *stage = shm;             // SCAP * CN local values
+    double *buf = shm + ASM_STAGE;   // the block's rows
+    const int t = threadIdx.x;
+    const int64_t rs = g0 + (int64_t)blockIdx.x * rb;
+    const int64_t re = min(rs + rb, g1);
+    const int32_t base = rowptr[rs];
+    const int32_t len = rowptr[re] - base;
+    for (int32_t k = t; k < len; k += ASM_NT) buf[k] = 0.0;
+    const int64_t g = rs + t;
+    const bool mine = t < rb && g < re;
+    int32_t off = 0, rl = 0, my0 = 0, my1 = 0;
+    if (mine) {
+        off = rowptr[g] - base;
+        rl = rowptr[g + 1] - rowptr[g];
+        my0 = F.inc_ptr[g - rblock];
+        my1 = F.inc_ptr[g - rblock + 1];
+    }
+    const int32_t i0 = F.inc_ptr[rs - rblock], i1 = F.inc_ptr[re - rblock];
+    for (int32_t c0 = i0; c0 < i1; c0 += SCAP) {
+        const int32_t c1 = min(c0 + SCAP, i1);
+        __syncthreads();
+        for (int32_t j = c0 + t; j < c1; j += ASM_NT) {
+            const int32_t pk = F.inc[j];
+            const int64_t e = pk / F.rnloc;
+            const int a = pk - (int)e * F.rnloc;
+            double v[CN];
+            K::row(F, e, a, v);
+#pragma unroll
+            for (int b = 0; b < CN; ++b) stage[(j - c0) * CN + b] = v[b];
+        }
+        __syncthreads();
+        if (mine) {
+            const int32_t ja = max(my0, c0), jb = min(my1, c1);
+            for (int32_t j = ja; j < jb; ++j) {
+                uint32_t w[W];
+#pragma unroll
+                for (int q = 0; q < W; ++q) w[q] = F.pos[(int64_t)j * W + q];
+#pragma unroll
+                for (int b = 0; b < CN; ++b) {
+                    const uint32_t p = (w[b >> 2] >> (8 * (b & 3))) & 255u;
+                    if (p != 255u) buf[off + p] += stage[(j - c0) * CN + b];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    bool fx = false;
+    if (mine) {
+        fx = fixed[g] != 0;
+        const uint8_t dp = diagpos[g];
+        if (fx && dp != 255) buf[off + dp] = penalty;
+        const double d = (dp != 255) ? buf[off + dp] : 0.0;
+        const double di = (fx || d == 0.0) ? 0.0 : 1.0 / d;
+        if (dinv64) dinv64[g] = di;
+        if (dinv32) dinv32[g] = (float)di;
+    }
+    __syncthreads();
+    if (val64)
+        for (int32_t k = t; k < len; k += ASM_NT) val64[base + k] = buf[k];
+    if (val32) {
+        __syncthreads();
+        if (mine && fx)
+            for (int32_t k = 0; k < rl; ++k) buf[off + k] = 0.0;
+        __syncthreads();
+        for (int32_t k = t; k < len; k += ASM_NT) val32[base + k] = (float)buf[k];
+    }
+}
+
+// One form of a known kind, one thread per row walking its cells serially:
+// the better choice when rows have few incidences (triangles)
+template <class K>
+__global__ void k_assemble1s(DevForm F, int64_t g0, int64_t g1, int64_t rblock, const int32_t *__restrict__ rowptr,
+                             const uint8_t *__restrict__ diagpos, const uint8_t *__restrict__ fixed, double penalty,
+                             double *__restrict__ val64, float *__restrict__ val32, double *__restrict__ dinv64,
+                             float *__restrict__ dinv32)
 {
     assemble_block([&](int64_t i, double *row) { assemble_row<K>(F, i, row); }, g0, g1, rblock, rowptr, diagpos, fixed,
                    penalty, val64, val32, dinv64, dinv32);
@@ -719,15 +809,31 @@ __global__ void k_assemble1(DevForm F, int64_t g0, int64_t g1, int64_t rblock, c
 
 // Launches the value assembly of row block [g0, g1): the per-kind kernel when
 // the block has one form (hf = its host copy), the generic one otherwise.
-static void launch_assemble(const grmp_matrix *A, const DevForm *hf, FormPack P, int64_t g0, int64_t g1, int B,
-                            size_t shm, double *val64, float *val32, double *dinv64, float *dinv32, hipStream_t st)
+static void launch_assemble(const grmp_matrix *A, const grmp_space *sp, const DevForm *hf, FormPack P, int64_t g0,
+                            int64_t g1, int B, size_t shm, double *val64, float *val32, double *dinv64, float *dinv32,
+                            hipStream_t st)
 {
     const dim3 grid(nblk(g1 - g0, B)), blk(B);
-    if (P.n == 1) {
+    const double inc_per_row = (double)sp->mesh->ncells * sp->nloc / (double)std::max<int64_t>(sp->ndofs, 1);
+    if (P.n == 1 && inc_per_row <= 12.0) {
         const DevForm F = *hf;
-        GRMP_DISPATCH(F, hipLaunchKernelGGL((k_assemble1<KT>), grid, blk, shm, st, F, g0, g1, g0, A->rowptr,
-                                            A->diagpos, A->fixed, A->penalty, val64, val32, dinv64, dinv32));
+        GRMP_DISPATCH(F, hipLaunchKernelGGL((k_assemble1s<KT>), grid, blk, shm, st, F, g0, g1, g0, A->rowptr, A->diagpos,
+                                            A->fixed, A->penalty, val64, val32, dinv64, dinv32));
         return;
+    }
+    if (P.n == 1) {
+        // rows per block: about 6 incidences per thread in a chunk, and the
+        // block's rows (rb * max_row doubles) within 48 KB next to the 16 KB stage
+        int rb = ASM_NT;
+        while (rb > 32 && ((double)rb * inc_per_row > 6.0 * ASM_NT || (size_t)rb * A->max_row * 8 > 48 * 1024)) rb >>= 1;
+        if ((size_t)rb * A->max_row * 8 <= 48 * 1024) {
+            const DevForm F = *hf;
+            const dim3 grid1(nblk(g1 - g0, rb)), blk1(ASM_NT);
+            const size_t shm1 = (ASM_STAGE + (size_t)rb * (A->max_row > 0 ? A->max_row : 1)) * sizeof(double);
+            GRMP_DISPATCH(F, hipLaunchKernelGGL((k_assemble1<KT>), grid1, blk1, shm1, st, F, rb, g0, g1, g0, A->rowptr,
+                                                A->diagpos, A->fixed, A->penalty, val64, val32, dinv64, dinv32));
+            return;
+        }
     }
     hipLaunchKernelGGL(k_assemble, grid, blk, shm, st, P, g0, g1, g0, A->rowptr, A->diagpos, A->fixed, A->penalty, val64,
                        val32, dinv64, dinv32);
@@ -1037,7 +1143,8 @@ extern "C" int grmp_matrix_assemble(grmp_matrix *A, int values)
         int64_t g0 = A->offs[rb], g1 = A->offs[rb + 1];
         if (g1 <= g0) continue;
         size_t shm = (size_t)B * (A->max_row > 0 ? A->max_row : 1) * sizeof(double);
-        launch_assemble(A, &hf[A->form_first[rb]], P, g0, g1, B, shm, (values & GRMP_VALUES_FP64) ? A->val64 : nullptr,
+        launch_assemble(A, A->spaces[rb], &hf[A->form_first[rb]], P, g0, g1, B, shm,
+                        (values & GRMP_VALUES_FP64) ? A->val64 : nullptr,
                         (values & GRMP_VALUES_FP32) ? A->val32 : nullptr, A->dinv64, A->dinv32, ctx->stream);
         GRMP_HIP(ctx, hipGetLastError());
     }
@@ -1085,7 +1192,7 @@ int assemble_values(const grmp_matrix *A, const double *kappa, DevForm *dforms, 
         int64_t g0 = A->offs[rb], g1 = A->offs[rb + 1];
         if (g1 <= g0) continue;
         size_t shm = (size_t)B * (A->max_row > 0 ? A->max_row : 1) * sizeof(double);
-        launch_assemble(A, &hforms[first[rb]], P, g0, g1, B, shm, val64, val32, dinv64, dinv32, st);
+        launch_assemble(A, A->spaces[rb], &hforms[first[rb]], P, g0, g1, B, shm, val64, val32, dinv64, dinv32, st);
         GRMP_HIP(ctx, hipGetLastError());
     }
     return GRMP_OK;
