@@ -48,9 +48,11 @@ def make_stream(nreq, rate, seed=2024):
             for i in range(nreq)]
 
 
-def serve(grmp, mode, stream, rtol, keep=(), profile=False, respect_arrivals=True, tile_rows=0):
+def serve(grmp, mode, stream, rtol, keep=(), profile=False, respect_arrivals=True, tile_rows=0, precision="fp32_ir",
+          fp64_first_sweep=0):
     """Submits the stream at its arrival times and waits for every result."""
-    S = grmp.Server(mode, profile=profile, memory_budget=96 << 30, max_active=512, tile_rows=tile_rows)
+    S = grmp.Server(mode, profile=profile, memory_budget=96 << 30, max_active=512, tile_rows=tile_rows,
+                    fp64_first_sweep=fp64_first_sweep)
     keep = set(keep)
     tickets = []
     t0 = time.perf_counter()
@@ -59,7 +61,8 @@ def serve(grmp, mode, stream, rtol, keep=(), profile=False, respect_arrivals=Tru
             dt = r["t"] - (time.perf_counter() - t0)
             if dt > 0:
                 time.sleep(dt)
-        tickets.append(S.submit(r["kind"], r["fe"], r["N"], r["seed"], rtol=rtol, keep_solution=k in keep))
+        tickets.append(S.submit(r["kind"], r["fe"], r["N"], r["seed"], rtol=rtol, keep_solution=k in keep,
+                                precision=precision))
     res = []
     for k, t in enumerate(tickets):
         res.append(S.wait(t))
@@ -292,6 +295,10 @@ def main():
     ap.add_argument("--size", type=int, default=0, help="single-instance workloads: N (default 2048 / 160 / 512)")
     ap.add_argument("--solve", action="store_true", help="c3: also solve")
     ap.add_argument("--tile-rows", type=int, default=0, help="engine rows per work tile (0 = library default)")
+    ap.add_argument("--fp64-first-sweep", type=int, default=0,
+                    help="served requests whose first FP32 sweep takes more iterations continue in FP64 (0: off)")
+    ap.add_argument("--precision", default="fp32_ir", choices=["fp32_ir", "fp64"],
+                    help="served requests: FP32 sweeps with FP64 refinement and fallback, or FP64 CG in the engine")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the served stream shards by request "
@@ -317,7 +324,8 @@ def main():
     for s in range(args.steps):
         log(f"[bench] step {s}: {len(stream)} requests at {args.rate:g}/s")
         GE.load().default_context().sync()
-        runs.append(serve(grmp, "served", stream, args.rtol, keep=vidx, profile=True, tile_rows=args.tile_rows))
+        runs.append(serve(grmp, "served", stream, args.rtol, keep=vidx, profile=True, tile_rows=args.tile_rows,
+                          precision=args.precision, fp64_first_sweep=args.fp64_first_sweep))
         GE.load().default_context().sync()
     run = runs[-1]
     walls = [r["wall"] for r in runs]

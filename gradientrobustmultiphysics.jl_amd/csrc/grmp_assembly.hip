@@ -1713,6 +1713,7 @@ __global__ void k_cellvolumes(const double *coords, const int32_t *cells, int64_
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nc) return;
     DevForm F;
+    memset(&F, 0, sizeof(F));
     F.coords = coords;
     F.cells = cells;
     F.nnodes = nn;

@@ -52,6 +52,8 @@ struct KProb {
     int32_t stall;      // consecutive refreshes that did not halve the best residual
     int32_t fp64;       // FP64 fallback: the FP32 sweeps stalled, plain FP64 CG finishes
     int32_t slow;       // the current FP32 sweep ran past its iteration cap
+    int32_t fp64_after; // > 0: switch to FP64 when the first FP32 sweep took more iterations
+    int32_t pad2_;
     int64_t sweep0;     // iterations done when the current FP32 sweep started
     double *p64, *q64;  // FP64 direction and product (alias the FP32 vector storage)
     int64_t iters, maxit;

@@ -294,6 +294,7 @@ static int job_activate(grmp_server *S, Job *j)
     p.rtol = j->req.rtol > 0 ? j->req.rtol : 1e-10;
     p.delta = 0.1;  // reliable update when the FP32 residual dropped 10x
     p.fp64 = j->req.precision == GRMP_PRECISION_FP64;
+    p.fp64_after = S->cfg.fp64_first_sweep;
     GRMP_TRY(S->eng->add(p, &j->slot));
     j->state = 2;
     return GRMP_OK;
@@ -507,6 +508,7 @@ extern "C" int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *cfg, 
     S->cfg.graph = 1;
     S->cfg.profile = 0;
     S->cfg.tile_rows = (int)kTileRows;
+    S->cfg.fp64_first_sweep = 0;
     if (cfg) {
         if (cfg->mode) S->cfg.mode = cfg->mode;
         if (cfg->memory_budget > 0) S->cfg.memory_budget = cfg->memory_budget;
@@ -514,6 +516,7 @@ extern "C" int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *cfg, 
         S->cfg.graph = cfg->graph;
         S->cfg.profile = cfg->profile;
         if (cfg->tile_rows > 0) S->cfg.tile_rows = std::max(64, std::min(cfg->tile_rows, 8192));
+        if (cfg->fp64_first_sweep >= 0) S->cfg.fp64_first_sweep = cfg->fp64_first_sweep;
     }
     int rc = grmp_ctx_create(ctx->device, &S->sctx);
     if (rc) { delete S; return grmp_fail(ctx, rc, "grmp_server_create: no server context"); }

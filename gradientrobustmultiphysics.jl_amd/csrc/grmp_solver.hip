@@ -652,7 +652,10 @@ __global__ __launch_bounds__(KB) void k_eng_reduce(KProb *__restrict__ P, const 
                     }
                     continue;
                 }
-                if (w.fp64 || w.stall >= 2 || w.slow) {
+                // a first sweep this long marks an ill-conditioned problem, whose
+                // FP32 sweeps would stall near the FP64 floor: FP64 from here
+                const bool hard = w.fp64_after > 0 && w.outer == 1 && w.iters > w.fp64_after;
+                if (w.fp64 || w.stall >= 2 || w.slow || hard) {
                     // two refreshes in a row without halving the residual, or a
                     // sweep past its iteration cap: the FP32 recurrences are too
                     // coarse for this problem's condition number, and FP64 CG

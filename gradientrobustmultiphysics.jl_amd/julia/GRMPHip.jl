@@ -42,11 +42,16 @@ struct GrmpForm
 end
 
 struct GrmpLForm
-    kind::Cint
+    kind::Cint                # GRMP_LF_CONST = 1, GRMP_LF_RECO_RT0 = 2
     quadorder::Cint
     factor::Cdouble
     f::NTuple{4,Cdouble}
+    nmodes::Cint              # RECO_RT0: gradient + curl force modes
+    pad_::Cint
+    modes::Ptr{Cdouble}       # [2*nmodes][4] = (amplitude, p, q, phase)
+    poly::NTuple{4,Cdouble}
 end
+GrmpLForm(kind, quadorder, factor, f) = GrmpLForm(kind, quadorder, factor, f, 0, 0, C_NULL, (0.0, 0.0, 0.0, 0.0))
 
 struct GrmpSolveOpts
     method::Cint

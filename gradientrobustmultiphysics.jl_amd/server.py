@@ -35,7 +35,7 @@ def footprint(kind, fe, N):
 
 class Server:
     def __init__(self, mode="served", memory_budget=64 << 30, max_active=256, graph=True, profile=False, ctx=None,
-                 tile_rows=0):
+                 tile_rows=0, fp64_first_sweep=0):
         self.ctx = ctx or default_context()
         cfg = _abi.ServerConfig()
         cfg.mode = 1 if mode == "served" else 2
@@ -44,6 +44,7 @@ class Server:
         cfg.graph = 1 if graph else 0
         cfg.profile = 1 if profile else 0
         cfg.tile_rows = int(tile_rows)
+        cfg.fp64_first_sweep = int(fp64_first_sweep)
         h = VP()
         self.ctx.check(lib().grmp_server_create(self.ctx.h, ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h

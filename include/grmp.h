@@ -277,6 +277,8 @@ typedef struct grmp_server_config {
     int graph;                /* capture the Krylov step as a hipGraph */
     int profile;              /* time the engine SpMV kernel with HIP events */
     int tile_rows;            /* engine rows per work tile (0 = 2048; 64 .. 8192) */
+    int fp64_first_sweep;     /* > 0: a request whose first FP32 sweep takes more iterations
+                                 continues in FP64 (0 = only on stalls) */
 } grmp_server_config;
 
 typedef struct grmp_kernel_stats {
