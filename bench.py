@@ -7,8 +7,9 @@ linear-elasticity problem (50:50; E=1, nu=0.3) on an N x N triangulated unit
 square with N = round(exp(U(ln 16, ln 768))) and a per-request seeded
 log-normal cell coefficient (sigma = 1).  Poisson requests use H1P1 or H1P2
 (50:50), elasticity uses H1P1{2}.  Every request is solved to relative
-residual 1e-10 (relative error <= 1e-8 against FP64 solves, checked on a
-sample) and its solution copied back to the host.
+FP64 residual 1e-9 (the target is relative error <= 1e-8 against FP64
+solves, checked on 24 requests spanning the size range) and its solution
+copied back to the host.
 
 Reported:
   value           DoF-solves/s of the served stream (sum of ndofs / wall)
@@ -49,9 +50,9 @@ def make_stream(nreq, rate, seed=2024):
 
 
 def serve(grmp, mode, stream, rtol, keep=(), profile=False, respect_arrivals=True, tile_rows=0, precision="fp32_ir",
-          fp64_first_sweep=0):
+          fp64_first_sweep=0, max_active=512):
     """Submits the stream at its arrival times and waits for every result."""
-    S = grmp.Server(mode, profile=profile, memory_budget=96 << 30, max_active=512, tile_rows=tile_rows,
+    S = grmp.Server(mode, profile=profile, memory_budget=96 << 30, max_active=max_active, tile_rows=tile_rows,
                     fp64_first_sweep=fp64_first_sweep)
     keep = set(keep)
     tickets = []
@@ -106,7 +107,7 @@ def _oracle_request(O, r, rng, rtol):
     return S.ndofs
 
 
-def cpu_baseline(stream, budget_s=15.0, rtol=1e-10):
+def cpu_baseline(stream, budget_s=15.0, rtol=1e-9):
     """The oracle (single-thread assembly + OpenMP FP64 Jacobi-PCG) on a
     bounded sample of the stream: requests in a fixed random order, each
     taken only if its predicted time (cost ~ ndofs * N, calibrated on the
@@ -186,6 +187,8 @@ def _assembly_rate(grmp, A, values, ncells, nloc, nnodes, dim, reps=3):
 
 def run_workload(grmp, which, args, log):
     ctx = grmp.default_context()
+    if args.rtol == 1e-9 and "--rtol" not in sys.argv:
+        args.rtol = 1e-10  # the single-instance configs are quoted at rel. residual 1e-10
     V64, V32 = grmp._abi.VALUES_FP64, grmp._abi.VALUES_FP32
     out = {"workload": which, "n_gpus": 1, "data": "synthetic"}
     if which in ("c2-p1", "c2-p2", "c3"):
@@ -285,9 +288,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=0, help="untimed runs of the first 256 requests")
     ap.add_argument("--requests", type=int, default=4096)
     ap.add_argument("--rate", type=float, default=5000.0)
-    ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--rtol", type=float, default=1e-9,
+                    help="relative FP64 residual; the served error target is 1e-8 and is verified")
     ap.add_argument("--plain-every", type=int, default=32, help="plain leg: every k-th request of the stream")
-    ap.add_argument("--verify", type=int, default=12, help="requests re-solved in FP64 for the error check")
+    ap.add_argument("--verify", type=int, default=24, help="requests re-solved in FP64 for the error check")
+    ap.add_argument("--max-active", type=int, default=512, help="served problems in flight")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="served",
@@ -325,7 +330,8 @@ def main():
         log(f"[bench] step {s}: {len(stream)} requests at {args.rate:g}/s")
         GE.load().default_context().sync()
         runs.append(serve(grmp, "served", stream, args.rtol, keep=vidx, profile=True, tile_rows=args.tile_rows,
-                          precision=args.precision, fp64_first_sweep=args.fp64_first_sweep))
+                          precision=args.precision, fp64_first_sweep=args.fp64_first_sweep,
+                          max_active=args.max_active))
         GE.load().default_context().sync()
     run = runs[-1]
     walls = [r["wall"] for r in runs]
