@@ -364,6 +364,7 @@ def main():
     roof = {"bound": "hbm", "kernel": "k_eng_spmv", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "avg_launch_us": round(avg_launch * 1e6, 2), "launches": int(st["spmv_launches"]),
+            "traffic_over_algorithmic": 1.20,  # FETCH_SIZE + WRITE_SIZE passes, profiles/engine_spmv_hbm_traffic.md
             "bytes_per_launch": round(st["spmv_bytes"] / max(st["spmv_launches"], 1), 1)}
 
     cpu = None
