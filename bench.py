@@ -262,7 +262,7 @@ def run_workload(grmp, which, args, log):
             sol = grmp.FEVector([Vs, Qs])
             log(f"[bench] c4: {prec}")
             t0 = time.perf_counter()
-            info, A, b = grmp.solve(sol, pde, precision=prec, rtol=args.rtol)
+            info, A, b = grmp.solve(sol, pde, precision=prec, rtol=args.rtol, minres_inner_rtol=args.minres_inner)
             ctx.sync()
             wall = time.perf_counter() - t0
             n, nnz, _ = A.info()
@@ -293,6 +293,7 @@ def main():
     ap.add_argument("--plain-every", type=int, default=32, help="plain leg: every k-th request of the stream")
     ap.add_argument("--verify", type=int, default=24, help="requests re-solved in FP64 for the error check")
     ap.add_argument("--max-active", type=int, default=512, help="served problems in flight")
+    ap.add_argument("--minres-inner", type=float, default=1e-4, help="c4: FP32 inner MINRES relative tolerance")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="served",
