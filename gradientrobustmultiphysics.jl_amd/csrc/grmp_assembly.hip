@@ -1134,10 +1134,18 @@ extern "C" int grmp_matrix_assemble(grmp_matrix *A, int values)
             for (auto &r : A->forms)
                 if (r.f.row_block == rb) fill_devform(A, r, hf[k++]);
     }
-    hipEvent_t e0, e1;
-    GRMP_HIP(ctx, hipEventCreate(&e0));
-    GRMP_HIP(ctx, hipEventCreate(&e1));
-    GRMP_HIP(ctx, hipEventRecord(e0, ctx->stream));
+    // the event pair lives until the function returns, whatever the path
+    struct EventPair {
+        hipEvent_t a = nullptr, b = nullptr;
+        ~EventPair()
+        {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } ev;
+    GRMP_HIP(ctx, hipEventCreate(&ev.a));
+    GRMP_HIP(ctx, hipEventCreate(&ev.b));
+    GRMP_HIP(ctx, hipEventRecord(ev.a, ctx->stream));
     for (int rb = 0; rb < A->nblocks; ++rb) {
         FormPack P = row_block_pack(A, rb);
         int64_t g0 = A->offs[rb], g1 = A->offs[rb + 1];
@@ -1148,11 +1156,9 @@ extern "C" int grmp_matrix_assemble(grmp_matrix *A, int values)
                         (values & GRMP_VALUES_FP32) ? A->val32 : nullptr, A->dinv64, A->dinv32, ctx->stream);
         GRMP_HIP(ctx, hipGetLastError());
     }
-    GRMP_HIP(ctx, hipEventRecord(e1, ctx->stream));
-    GRMP_HIP(ctx, hipEventSynchronize(e1));
-    GRMP_HIP(ctx, hipEventElapsedTime(&A->last_asm_ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    GRMP_HIP(ctx, hipEventRecord(ev.b, ctx->stream));
+    GRMP_HIP(ctx, hipEventSynchronize(ev.b));
+    GRMP_HIP(ctx, hipEventElapsedTime(&A->last_asm_ms, ev.a, ev.b));
     A->values |= values;
     return GRMP_OK;
 }

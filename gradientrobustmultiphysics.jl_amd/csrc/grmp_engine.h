@@ -43,7 +43,7 @@ struct KProb {
     double *r64;
     // FP32 inner vectors
     float *r, *p, *q, *dx;
-    int32_t n, lanes;
+    int32_t n, pad0_;
     int32_t phase, status;
     int32_t outer;      // FP64 residual refreshes so far
     int32_t max_outer;
@@ -134,4 +134,3 @@ class KEngine {
     int read_events(int pairs);
 };
 
-int engine_pick_lanes(double nnz_per_row);

@@ -71,7 +71,6 @@ struct Template {
     grmp_space *space = nullptr;
     grmp_matrix *A = nullptr;
     grmp_vector *kappa0 = nullptr;
-    int lanes = 4;
     int refs = 0;
     int64_t bytes = 0;
     uint64_t last_use = 0;
@@ -185,7 +184,6 @@ static int template_make(grmp_server *S, const grmp_request &r, Template **out)
         template_free(t);
         return rc;
     }
-    t->lanes = engine_pick_lanes((double)t->A->nnz / (double)t->A->n);
     const int64_t nc = t->mesh->ncells, nn = t->mesh->nnodes, n = t->A->n;
     t->bytes = nn * 16 + nc * 12 + nc * t->space->nloc * 8 + n * 4 + t->A->nnz * 4 + n * 6 + nc * 8 +
                nc * t->space->nloc * 4 * ((t->space->nloc + 3) / 4);
@@ -288,7 +286,6 @@ static int job_activate(grmp_server *S, Job *j)
     p.p64 = (double *)j->f4;
     p.q64 = (double *)(j->f4 + 2 * j->n);
     p.n = (int32_t)j->n;
-    p.lanes = j->tpl->lanes;
     p.max_outer = 200;
     p.maxit = 4000000;
     p.rtol = j->req.rtol > 0 ? j->req.rtol : 1e-10;

@@ -18,14 +18,6 @@ static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / 
 
 #include "grmp_sparse.h"
 
-int engine_pick_lanes(double nnz_per_row)
-{
-    if (nnz_per_row <= 3.0) return 2;
-    if (nnz_per_row <= 12.0) return 4;
-    if (nnz_per_row <= 24.0) return 8;
-    return 16;
-}
-
 // --------------------------------------------------------------- SpMV API
 __global__ __launch_bounds__(KB) void k_spmv(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                                             const double *__restrict__ v, const double *__restrict__ x,
@@ -950,7 +942,6 @@ static int solve_engine(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, co
     p.p64 = (double *)buf;
     p.q64 = (double *)(buf + 2 * n);
     p.n = (int32_t)n;
-    p.lanes = engine_pick_lanes((double)A->nnz / (double)n);
     p.max_outer = 200;
     p.maxit = o.maxit;
     p.rtol = o.rtol;

@@ -39,33 +39,6 @@ __device__ __forceinline__ void block_sum2(double &a, double &b)
     __syncthreads();
 }
 
-// row i of a CSR matrix times x with L lanes per row; full sum in lane 0
-template <int L, class V, class X>
-__device__ __forceinline__ double csr_row(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                                          const V *__restrict__ v, const X *__restrict__ x, int i, int lane)
-{
-    double s = 0.0;
-    const int32_t k1 = rp[i + 1];
-    for (int32_t k = rp[i] + lane; k < k1; k += L) s += (double)v[k] * (double)x[ci[k]];
-#pragma unroll
-    for (int o = L / 2; o > 0; o >>= 1) s += __shfl_down(s, o, L);
-    return s;
-}
-
-// FP32 variant accumulating in FP32 (the inner solve's arithmetic)
-template <int L>
-__device__ __forceinline__ float csr_row32(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                                           const float *__restrict__ v, const float *__restrict__ x, int i, int lane)
-{
-    float s = 0.0f;
-    const int32_t k1 = rp[i + 1];
-    for (int32_t k = rp[i] + lane; k < k1; k += L) s += v[k] * x[ci[k]];
-#pragma unroll
-    for (int o = L / 2; o > 0; o >>= 1) s += __shfl_down(s, o, L);
-    return s;
-}
-
-
 // Nonzeros staged per block by csr_rows_lds (32 KB of LDS in FP64).
 #define SPMV_CAP 4096
 
