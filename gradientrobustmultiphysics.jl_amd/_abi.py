@@ -117,6 +117,7 @@ SIGNATURES = {
     "grmp_mesh_info": (ctypes.c_int, [VP, ctypes.POINTER(ctypes.c_int), I64P, I64P]),
     "grmp_mesh_download": (ctypes.c_int, [VP, DP, I32P]),
     "grmp_mesh_cellvolumes": (ctypes.c_int, [VP, DP]),
+    "grmp_mesh_faces": (ctypes.c_int, [VP, ctypes.POINTER(I64), I32P, ctypes.c_void_p, ctypes.c_void_p]),
     "grmp_space_upload": (ctypes.c_int, [VP, VP, ctypes.c_int, ctypes.c_int, I64, ctypes.c_int, I32P,
                                          ctypes.POINTER(VP)]),
     "grmp_space_create": (ctypes.c_int, [VP, VP, ctypes.c_int, ctypes.c_int, ctypes.POINTER(VP)]),

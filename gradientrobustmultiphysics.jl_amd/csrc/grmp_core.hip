@@ -306,6 +306,125 @@ __global__ void k_unitcube(int n, double jitter, uint32_t k0, uint32_t k1, doubl
     }
 }
 
+// ------------------------------------------------- faces of triangle meshes
+// Local face f of a cell joins its local nodes f and (f+1) % 3 (the order of
+// the reference's Triangle2D face nodes).  Faces are numbered by first
+// appearance, cell by cell, local face by local face; the face's global
+// orientation is the one of its first appearance, and CellFaceSigns is +1
+// where a cell traverses the face that way, -1 otherwise.
+__global__ void k_face_keys(const int32_t *__restrict__ cells, int64_t nc, int64_t nn, uint64_t *__restrict__ key,
+                            int32_t *__restrict__ app)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * nc) return;
+    const int64_t c = t / 3;
+    const int f = (int)(t - 3 * c);
+    const int64_t a = cells[f * nc + c], b = cells[((f + 1) % 3) * nc + c];
+    key[t] = (uint64_t)(a < b ? a : b) * (uint64_t)nn + (uint64_t)(a < b ? b : a);
+    app[t] = (int32_t)t;
+}
+
+// heads of equal-key segments (the sort is stable, so a head carries the
+// segment's first appearance): mark[first appearance] = 1
+__global__ void k_face_heads(const uint64_t *__restrict__ key, const int32_t *__restrict__ app, int64_t m,
+                             int32_t *__restrict__ mark)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    if (i == 0 || key[i] != key[i - 1]) mark[app[i]] = 1;
+}
+
+// every appearance gets its segment's face id (rank of the segment's first
+// appearance), its CellFaceSign and, for one-cell segments, the boundary flag
+__global__ void k_face_assign(const uint64_t *__restrict__ key, const int32_t *__restrict__ app,
+                              const int32_t *__restrict__ rank, const int32_t *__restrict__ cells, int64_t nc, int64_t m,
+                              int32_t *__restrict__ cellfaces, int8_t *__restrict__ fsign, uint8_t *__restrict__ bface,
+                              int *__restrict__ bad)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    int64_t h = i;
+    while (h > 0 && key[h - 1] == key[i]) --h;  // segments hold two entries on a conforming mesh
+    if (i - h > 1) atomicOr(bad, 1);           // a face shared by three or more cells
+    const int32_t first = app[h];
+    const int32_t id = rank[first];
+    const int32_t me = app[i];
+    const int64_t c = me / 3, c0 = first / 3;
+    const int f = me - 3 * (int)c, f0 = first - 3 * (int)c0;
+    cellfaces[f * nc + c] = id;
+    const int32_t a = cells[f * nc + c], a0 = cells[f0 * nc + c0];
+    fsign[f * nc + c] = (int8_t)(a == a0 ? 1 : -1);
+    const bool single = (i == h) && (i + 1 >= m || key[i + 1] != key[i]);
+    if (single) bface[id] = 1;
+}
+
+int mesh_build_faces(grmp_ctx *ctx, grmp_mesh *m)
+{
+    const int64_t nc = m->ncells, M = 3 * nc;
+    GRMP_CHECK(ctx, m->dim == 2 && m->nloc == 3 && M < (1ll << 31), GRMP_ERR_ARG, "mesh faces: triangle meshes only");
+    uint64_t *k0 = nullptr, *k1 = nullptr;
+    int32_t *a0 = nullptr, *a1 = nullptr, *mark = nullptr, *rank = nullptr;
+    int rc = GRMP_OK;
+    do {
+        if ((rc = dalloc(ctx, &k0, M)) || (rc = dalloc(ctx, &k1, M)) || (rc = dalloc(ctx, &a0, M)) ||
+            (rc = dalloc(ctx, &a1, M)) || (rc = dalloc(ctx, &mark, M)) || (rc = dalloc(ctx, &rank, M + 1)))
+            break;
+        hipLaunchKernelGGL(k_face_keys, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, m->cells, nc, m->nnodes, k0, a0);
+        // keys are below nnodes^2: sort only the bits they use
+        int bits = 1;
+        while (bits < 64 && ((uint64_t)1 << bits) < (uint64_t)m->nnodes * (uint64_t)m->nnodes) ++bits;
+        size_t tmp = 0;
+        if (hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, k0, k1, a0, a1, (int)M, 0, bits, ctx->stream) !=
+            hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "mesh faces: sort setup");
+            break;
+        }
+        if ((rc = ensure_scratch(ctx, tmp))) break;
+        if (hipcub::DeviceRadixSort::SortPairs(ctx->scratch, tmp, k0, k1, a0, a1, (int)M, 0, bits, ctx->stream) !=
+            hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "mesh faces: sort");
+            break;
+        }
+        if (hipMemsetAsync(mark, 0, sizeof(int32_t) * M, ctx->stream) != hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "mesh faces: memset");
+            break;
+        }
+        hipLaunchKernelGGL(k_face_heads, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, k1, a1, M, mark);
+        if ((rc = exclusive_scan_i32(ctx, mark, rank, M))) break;
+        int32_t nf = 0;
+        if (hipMemcpyAsync(&nf, rank + M, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "mesh faces: count");
+            break;
+        }
+        m->nfaces = nf;
+        if ((rc = dalloc(ctx, &m->cellfaces, M)) || (rc = dalloc(ctx, &m->bface, nf)) ||
+            (!m->fsign && (rc = dalloc(ctx, &m->fsign, M))))
+            break;
+        if (hipMemsetAsync(m->bface, 0, nf, ctx->stream) != hipSuccess ||
+            hipMemsetAsync(ctx->dflag, 0, sizeof(int), ctx->stream) != hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "mesh faces: memset");
+            break;
+        }
+        hipLaunchKernelGGL(k_face_assign, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, k1, a1, rank, m->cells, nc, M,
+                           m->cellfaces, m->fsign, m->bface, ctx->dflag);
+        int bad = 0;
+        if (hipMemcpyAsync(&bad, ctx->dflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "mesh faces: assign");
+            break;
+        }
+        if (bad) rc = grmp_fail(ctx, GRMP_ERR_ARG, "mesh faces: a face is shared by more than two cells");
+    } while (0);
+    dfree(ctx, k0);
+    dfree(ctx, k1);
+    dfree(ctx, a0);
+    dfree(ctx, a1);
+    dfree(ctx, mark);
+    dfree(ctx, rank);
+    return rc;
+}
+
 static grmp_mesh *mesh_new(grmp_ctx *ctx, int dim, int64_t nnodes, int64_t ncells, int nloc)
 {
     grmp_mesh *m = new grmp_mesh();
@@ -319,6 +438,9 @@ static grmp_mesh *mesh_new(grmp_ctx *ctx, int dim, int64_t nnodes, int64_t ncell
     m->coords = nullptr;
     m->cells = nullptr;
     m->fsign = nullptr;
+    m->cellfaces = nullptr;
+    m->bface = nullptr;
+    m->nfaces = 0;
     return m;
 }
 
@@ -328,6 +450,8 @@ extern "C" int grmp_mesh_destroy(grmp_mesh *m)
     dfree(m->ctx, m->coords);
     dfree(m->ctx, m->cells);
     dfree(m->ctx, m->fsign);
+    dfree(m->ctx, m->cellfaces);
+    dfree(m->ctx, m->bface);
     delete m;
     return GRMP_OK;
 }
@@ -370,6 +494,10 @@ extern "C" int grmp_mesh_upload(grmp_ctx *ctx, int dim, int64_t nnodes, const do
     if (bad) {
         grmp_mesh_destroy(m);
         return grmp_fail(ctx, GRMP_ERR_ARG, "grmp_mesh_upload: cell node index outside 1..nnodes");
+    }
+    if (dim == 2 && (rc = mesh_build_faces(ctx, m))) {
+        grmp_mesh_destroy(m);
+        return rc;
     }
     *out = m;
     return GRMP_OK;
@@ -490,6 +618,22 @@ __device__ __host__ inline int64_t sq_D(int64_t N, int64_t i, int64_t j)
     return i == 0 ? sq_base(N, j) + 1 : sq_base(N, j) + 4 + 3 * (i - 1) + 1;
 }
 
+// CellFaces of cell c of the generated unit square (lower triangle of square
+// (i,j) first, then the upper one)
+__device__ __forceinline__ void sq_cellfaces(int64_t N, int64_t c, int64_t f[3])
+{
+    const int64_t sq = c >> 1, j = sq / N, i = sq % N;
+    if (!(c & 1)) {
+        f[0] = sq_H(N, i, j);
+        f[1] = sq_V(N, i + 1, j);
+        f[2] = sq_D(N, i, j);
+    } else {
+        f[0] = sq_D(N, i, j);
+        f[1] = sq_H(N, i, j + 1);
+        f[2] = sq_V(N, i, j);
+    }
+}
+
 __global__ void k_celldofs_p1(const int32_t *cells, int32_t *dofs, int64_t nc, int nv, int ncomp, int64_t nnodes)
 {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -505,18 +649,8 @@ __global__ void k_celldofs_p2_square(const int32_t *cells, int32_t *dofs, int64_
 {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
-    int64_t sq = c >> 1, j = sq / N, i = sq % N;
-    bool upper = c & 1;
     int64_t f[3];
-    if (!upper) {
-        f[0] = sq_H(N, i, j);
-        f[1] = sq_V(N, i + 1, j);
-        f[2] = sq_D(N, i, j);
-    } else {
-        f[0] = sq_D(N, i, j);
-        f[1] = sq_H(N, i, j + 1);
-        f[2] = sq_V(N, i, j);
-    }
+    sq_cellfaces(N, c, f);
     int64_t coff = nnodes + nfaces;
     for (int k = 0; k < ncomp; ++k) {
         for (int a = 0; a < 3; ++a) dofs[(int64_t)(k * 6 + a) * nc + c] = (int32_t)(cells[(int64_t)a * nc + c] + k * coff);
@@ -530,22 +664,38 @@ __global__ void k_celldofs_br_square(const int32_t *cells, int32_t *dofs, int64_
 {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
-    int64_t sq = c >> 1, j = sq / N, i = sq % N;
-    bool upper = c & 1;
     int64_t f[3];
-    if (!upper) {
-        f[0] = sq_H(N, i, j);
-        f[1] = sq_V(N, i + 1, j);
-        f[2] = sq_D(N, i, j);
-    } else {
-        f[0] = sq_D(N, i, j);
-        f[1] = sq_H(N, i, j + 1);
-        f[2] = sq_V(N, i, j);
-    }
+    sq_cellfaces(N, c, f);
     for (int a = 0; a < 3; ++a) {
         dofs[(int64_t)a * nc + c] = cells[(int64_t)a * nc + c];
         dofs[(int64_t)(3 + a) * nc + c] = (int32_t)(cells[(int64_t)a * nc + c] + nnodes);
         dofs[(int64_t)(6 + a) * nc + c] = (int32_t)(2 * nnodes + f[a]);
+    }
+}
+
+// H1P2 / H1BR{2} dofmaps from enumerated faces (uploaded triangle meshes)
+__global__ void k_celldofs_p2_faces(const int32_t *cells, const int32_t *cellfaces, int32_t *dofs, int64_t nc, int ncomp,
+                                    int64_t nnodes, int64_t nfaces)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int64_t coff = nnodes + nfaces;
+    for (int k = 0; k < ncomp; ++k) {
+        for (int a = 0; a < 3; ++a) dofs[(int64_t)(k * 6 + a) * nc + c] = (int32_t)(cells[(int64_t)a * nc + c] + k * coff);
+        for (int a = 0; a < 3; ++a)
+            dofs[(int64_t)(k * 6 + 3 + a) * nc + c] = (int32_t)(nnodes + cellfaces[(int64_t)a * nc + c] + k * coff);
+    }
+}
+
+__global__ void k_celldofs_br_faces(const int32_t *cells, const int32_t *cellfaces, int32_t *dofs, int64_t nc,
+                                    int64_t nnodes)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    for (int a = 0; a < 3; ++a) {
+        dofs[(int64_t)a * nc + c] = cells[(int64_t)a * nc + c];
+        dofs[(int64_t)(3 + a) * nc + c] = (int32_t)(cells[(int64_t)a * nc + c] + nnodes);
+        dofs[(int64_t)(6 + a) * nc + c] = (int32_t)(2 * nnodes + cellfaces[(int64_t)a * nc + c]);
     }
 }
 
@@ -681,10 +831,10 @@ extern "C" int grmp_space_upload(grmp_ctx *ctx, const grmp_mesh *mesh, int fe, i
 extern "C" int grmp_space_create(grmp_ctx *ctx, const grmp_mesh *mesh, int fe, int ncomp, grmp_space **out)
 {
     GRMP_CHECK(ctx, ctx && mesh && out && ncomp >= 1 && ncomp <= 3, GRMP_ERR_ARG, "grmp_space_create: bad arguments");
-    GRMP_CHECK(ctx, fe == GRMP_FE_H1P1 || fe == GRMP_FE_L2P0 ||
-                        ((fe == GRMP_FE_H1P2 || fe == GRMP_FE_H1BR) && mesh->kind == MESH_UNITSQUARE),
+    const bool faces = mesh->kind == MESH_UNITSQUARE || mesh->cellfaces;
+    GRMP_CHECK(ctx, fe == GRMP_FE_H1P1 || fe == GRMP_FE_L2P0 || ((fe == GRMP_FE_H1P2 || fe == GRMP_FE_H1BR) && faces),
                GRMP_ERR_UNSUPPORTED,
-               "grmp_space_create: device dofmaps exist for H1P1/L2P0 on any mesh and H1P2/H1BR on unit-square "
+               "grmp_space_create: device dofmaps exist for H1P1/L2P0 on any mesh and H1P2/H1BR on triangle "
                "meshes; upload FES[CellDofs] with grmp_space_upload otherwise");
     GRMP_CHECK(ctx, fe != GRMP_FE_H1BR || ncomp == 2, GRMP_ERR_ARG, "grmp_space_create: H1BR{2} has 2 components");
     GRMP_CHECK(ctx, fe != GRMP_FE_L2P0 || ncomp == 1, GRMP_ERR_UNSUPPORTED, "grmp_space_create: L2P0 with 1 component");
@@ -706,12 +856,12 @@ extern "C" int grmp_space_create(grmp_ctx *ctx, const grmp_mesh *mesh, int fe, i
         s->nfaces = 0;
     } else if (fe == GRMP_FE_H1BR) {
         int64_t N = mesh->N;
-        s->nfaces = 3 * N * N + 2 * N;
+        s->nfaces = mesh->cellfaces ? mesh->nfaces : 3 * N * N + 2 * N;
         s->nloc = 9;
         s->ndofs = 2 * mesh->nnodes + s->nfaces;
     } else {
         int64_t N = mesh->N;
-        s->nfaces = 3 * N * N + 2 * N;
+        s->nfaces = mesh->cellfaces ? mesh->nfaces : 3 * N * N + 2 * N;
         s->nloc = 6 * ncomp;
         s->ndofs = (mesh->nnodes + s->nfaces) * ncomp;
     }
@@ -722,9 +872,15 @@ extern "C" int grmp_space_create(grmp_ctx *ctx, const grmp_mesh *mesh, int fe, i
                            nv, ncomp, mesh->nnodes);
     else if (fe == GRMP_FE_L2P0)
         hipLaunchKernelGGL(k_celldofs_p0, dim3(nblk(nc, 256)), dim3(256), 0, ctx->stream, s->celldofs, nc);
+    else if (fe == GRMP_FE_H1BR && mesh->cellfaces)
+        hipLaunchKernelGGL(k_celldofs_br_faces, dim3(nblk(nc, 256)), dim3(256), 0, ctx->stream, mesh->cells,
+                           mesh->cellfaces, s->celldofs, nc, mesh->nnodes);
     else if (fe == GRMP_FE_H1BR)
         hipLaunchKernelGGL(k_celldofs_br_square, dim3(nblk(nc, 256)), dim3(256), 0, ctx->stream, mesh->cells,
                            s->celldofs, nc, mesh->N, mesh->nnodes);
+    else if (mesh->cellfaces)
+        hipLaunchKernelGGL(k_celldofs_p2_faces, dim3(nblk(nc, 256)), dim3(256), 0, ctx->stream, mesh->cells,
+                           mesh->cellfaces, s->celldofs, nc, ncomp, mesh->nnodes, s->nfaces);
     else
         hipLaunchKernelGGL(k_celldofs_p2_square, dim3(nblk(nc, 256)), dim3(256), 0, ctx->stream, mesh->cells,
                            s->celldofs, nc, mesh->N, ncomp, mesh->nnodes, s->nfaces);
@@ -759,19 +915,176 @@ extern "C" int grmp_space_download_celldofs(const grmp_space *s, int32_t *celldo
 }
 
 // boundary dofs of structured meshes, enumerated on the host (O(boundary))
+// CellFaces / CellFaceSigns in the caller's layout ([ncells][3], faces 1-based)
+__global__ void k_faces_out(const int32_t *__restrict__ cellfaces, const int8_t *__restrict__ fsign, int64_t nc, int N,
+                            int32_t *__restrict__ cf, int8_t *__restrict__ cs)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    int64_t f[3];
+    if (cellfaces)
+        for (int k = 0; k < 3; ++k) f[k] = cellfaces[k * nc + c];
+    else
+        sq_cellfaces(N, c, f);
+    for (int k = 0; k < 3; ++k) {
+        cf[3 * c + k] = (int32_t)(f[k] + 1);
+        cs[3 * c + k] = fsign[k * nc + c];
+    }
+}
+
+// boundary faces of the generated unit square: bottom, top, left, right
+__global__ void k_bface_square(int N, uint8_t *__restrict__ bface)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 4 * (int64_t)N) return;
+    const int64_t side = t / N, i = t % N;
+    const int64_t f = side == 0 ? sq_H(N, i, 0) : side == 1 ? sq_H(N, i, N) : side == 2 ? sq_V(N, 0, i) : sq_V(N, N, i);
+    bface[f] = 1;
+}
+
+extern "C" int grmp_mesh_faces(const grmp_mesh *m, int64_t *nfaces, int32_t *cellfaces, int8_t *cellfacesigns,
+                               uint8_t *bface)
+{
+    if (!m || !nfaces) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = m->ctx;
+    const bool square = m->kind == MESH_UNITSQUARE;
+    GRMP_CHECK(ctx, square || m->cellfaces, GRMP_ERR_UNSUPPORTED,
+               "grmp_mesh_faces: face adjacency exists for triangle meshes");
+    const int64_t nc = m->ncells, N = m->N;
+    const int64_t nf = square ? 3 * N * N + 2 * N : m->nfaces;
+    *nfaces = nf;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    int32_t *cf = nullptr;
+    int8_t *cs = nullptr;
+    uint8_t *bf = nullptr;
+    int rc = GRMP_OK;
+    do {
+        if (cellfaces || cellfacesigns) {
+            if ((rc = dalloc(ctx, &cf, 3 * nc)) || (rc = dalloc(ctx, &cs, 3 * nc))) break;
+            hipLaunchKernelGGL(k_faces_out, dim3(nblk(nc, 256)), dim3(256), 0, ctx->stream,
+                               square ? nullptr : m->cellfaces, m->fsign, nc, (int)N, cf, cs);
+            if ((cellfaces && hipMemcpyAsync(cellfaces, cf, sizeof(int32_t) * 3 * nc, hipMemcpyDeviceToHost,
+                                             ctx->stream) != hipSuccess) ||
+                (cellfacesigns &&
+                 hipMemcpyAsync(cellfacesigns, cs, 3 * nc, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)) {
+                rc = grmp_fail(ctx, GRMP_ERR_HIP, "grmp_mesh_faces: download");
+                break;
+            }
+        }
+        if (bface) {
+            const uint8_t *src = m->bface;
+            if (square) {
+                if ((rc = dalloc(ctx, &bf, nf))) break;
+                if (hipMemsetAsync(bf, 0, nf, ctx->stream) != hipSuccess) {
+                    rc = grmp_fail(ctx, GRMP_ERR_HIP, "grmp_mesh_faces: memset");
+                    break;
+                }
+                hipLaunchKernelGGL(k_bface_square, dim3(nblk(4 * N, 256)), dim3(256), 0, ctx->stream, (int)N, bf);
+                src = bf;
+            }
+            if (hipMemcpyAsync(bface, src, nf, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) {
+                rc = grmp_fail(ctx, GRMP_ERR_HIP, "grmp_mesh_faces: download");
+                break;
+            }
+        }
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = grmp_fail(ctx, GRMP_ERR_HIP, "grmp_mesh_faces: sync");
+    } while (0);
+    dfree(ctx, cf);
+    dfree(ctx, cs);
+    dfree(ctx, bf);
+    return rc;
+}
+
+// BFaceDofs of enumerated boundary faces: every (cell, local face) on the
+// boundary marks the face's two nodes in each component and, for P2 / BR, the
+// face's own dof (src/dofmaps.jl BFaceDofs patterns "N1F1" / "N1i1")
+__global__ void k_mark_bdofs(const int32_t *__restrict__ cells, const int32_t *__restrict__ cellfaces,
+                             const uint8_t *__restrict__ bface, int64_t nc, int64_t nnodes, int fe, int ncomp,
+                             int64_t coff, uint8_t *__restrict__ mark)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * nc) return;
+    const int64_t c = t / 3;
+    const int f = (int)(t - 3 * c);
+    const int32_t face = cellfaces[f * nc + c];
+    if (!bface[face]) return;
+    const int64_t a = cells[f * nc + c], b = cells[((f + 1) % 3) * nc + c];
+    if (fe == GRMP_FE_H1BR) {
+        mark[a] = mark[b] = mark[a + nnodes] = mark[b + nnodes] = 1;
+        mark[2 * nnodes + face] = 1;
+        return;
+    }
+    for (int k = 0; k < ncomp; ++k) {
+        mark[a + k * coff] = mark[b + k * coff] = 1;
+        if (fe == GRMP_FE_H1P2) mark[nnodes + face + k * coff] = 1;
+    }
+}
+
+static int boundary_dofs_faces(grmp_ctx *ctx, const grmp_space *s, int64_t *count, int64_t *out)
+{
+    const grmp_mesh *m = s->mesh;
+    const int64_t nc = m->ncells, nd = s->ndofs;
+    uint8_t *mark = nullptr;
+    int64_t *sel = nullptr, *nsel = nullptr;
+    int rc = GRMP_OK;
+    do {
+        if ((rc = dalloc(ctx, &mark, nd)) || (rc = dalloc(ctx, &sel, nd)) || (rc = dalloc(ctx, &nsel, 1))) break;
+        if (hipMemsetAsync(mark, 0, nd, ctx->stream) != hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "boundary dofs: memset");
+            break;
+        }
+        const int64_t coff = s->ndofs / s->ncomp;
+        hipLaunchKernelGGL(k_mark_bdofs, dim3(nblk(3 * nc, 256)), dim3(256), 0, ctx->stream, m->cells, m->cellfaces,
+                           m->bface, nc, m->nnodes, s->fe, s->ncomp, coff, mark);
+        // stream compaction keeps index order, so the list comes out sorted
+        hipcub::CountingInputIterator<int64_t> idx(1);  // 1-based dof numbers
+        size_t tmp = 0;
+        if (hipcub::DeviceSelect::Flagged(nullptr, tmp, idx, mark, sel, nsel, (int)nd, ctx->stream) != hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "boundary dofs: select setup");
+            break;
+        }
+        if ((rc = ensure_scratch(ctx, tmp))) break;
+        if (hipcub::DeviceSelect::Flagged(ctx->scratch, tmp, idx, mark, sel, nsel, (int)nd, ctx->stream) !=
+            hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "boundary dofs: select");
+            break;
+        }
+        int64_t cnt = 0;
+        if (hipMemcpyAsync(&cnt, nsel, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "boundary dofs: count");
+            break;
+        }
+        *count = cnt;
+        if (out && cnt &&
+            (hipMemcpyAsync(out, sel, sizeof(int64_t) * cnt, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+             hipStreamSynchronize(ctx->stream) != hipSuccess))
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "boundary dofs: download");
+    } while (0);
+    dfree(ctx, mark);
+    dfree(ctx, sel);
+    dfree(ctx, nsel);
+    return rc;
+}
+
 extern "C" int grmp_space_boundary_dofs(const grmp_space *s, int64_t *count, int64_t *out)
 {
     if (!s || !count) return GRMP_ERR_ARG;
     const grmp_mesh *m = s->mesh;
     grmp_ctx *ctx = s->ctx;
-    GRMP_CHECK(ctx, m->kind == MESH_UNITSQUARE || m->kind == MESH_UNITCUBE, GRMP_ERR_UNSUPPORTED,
-               "grmp_space_boundary_dofs: boundary enumeration exists for generated meshes only");
-    std::vector<int64_t> b;
-    int64_t N = m->N, n1 = N + 1;
     if (s->fe == GRMP_FE_L2P0) {
         *count = 0;  // no dofs on faces
         return GRMP_OK;
     }
+    if (m->kind != MESH_UNITSQUARE && m->cellfaces) {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        return boundary_dofs_faces(ctx, s, count, out);
+    }
+    GRMP_CHECK(ctx, m->kind == MESH_UNITSQUARE || m->kind == MESH_UNITCUBE, GRMP_ERR_UNSUPPORTED,
+               "grmp_space_boundary_dofs: boundary enumeration exists for generated meshes and uploaded "
+               "triangle meshes");
+    std::vector<int64_t> b;
+    int64_t N = m->N, n1 = N + 1;
     if (m->kind == MESH_UNITSQUARE && s->fe == GRMP_FE_H1BR) {
         // BFaceDofs "N1i1": both components of boundary nodes, and the bubble of each boundary face
         std::vector<int64_t> nodes;

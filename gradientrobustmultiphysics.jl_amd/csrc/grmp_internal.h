@@ -80,6 +80,11 @@ struct grmp_mesh {
     int kind;            // MeshKind
     int N;               // structured resolution
     int8_t *fsign;       // SoA [3][ncells] CellFaceSigns (triangles; nullptr if unknown)
+    // faces of uploaded triangle meshes, numbered by first appearance in cell
+    // order (generated unit squares number theirs in closed form instead)
+    int32_t *cellfaces;  // SoA [3][ncells], or nullptr
+    uint8_t *bface;      // [nfaces] 1 for a boundary face (one cell)
+    int64_t nfaces;
 };
 
 struct grmp_space {

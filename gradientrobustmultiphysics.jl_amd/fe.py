@@ -85,6 +85,17 @@ class ExtendableGrid:
         self.ctx.check(lib().grmp_mesh_cellvolumes(self.h, dptr(out)))
         return out
 
+    def faces(self):
+        """Triangle meshes: (CellFaces (ncells, 3) 0-based, CellFaceSigns
+        (ncells, 3), boundary flag per face), numbered by first appearance."""
+        nf = ctypes.c_int64()
+        self.ctx.check(lib().grmp_mesh_faces(self.h, ctypes.byref(nf), None, None, None))
+        cf = np.empty((self.ncells, 3), dtype=np.int32)
+        cs = np.empty((self.ncells, 3), dtype=np.int8)
+        bf = np.empty(nf.value, dtype=np.uint8)
+        self.ctx.check(lib().grmp_mesh_faces(self.h, ctypes.byref(nf), i32ptr(cf), cs.ctypes.data, bf.ctypes.data))
+        return cf - 1, cs, bf.astype(bool)
+
     def __del__(self):
         try:
             if self.h and _abi.alive():

@@ -167,6 +167,17 @@ int grmp_mesh_info(const grmp_mesh *m, int *dim, int64_t *nnodes, int64_t *ncell
 int grmp_mesh_download(const grmp_mesh *m, double *coords, int32_t *cellnodes);
 /* per-cell volumes (ExtendableGrids CellVolumes) */
 int grmp_mesh_cellvolumes(const grmp_mesh *m, double *out);
+/* Face adjacency of a triangle mesh, built on the device when the mesh is
+ * uploaded: ExtendableGrids' CellFaces [3,ncells] (1-based, column-major, so
+ * cellfaces[3c+f]), CellFaceSigns [3,ncells] and a 0/1 boundary flag per face
+ * (the faces BFaceFaces lists).  Local face f joins local nodes f and f%3+1;
+ * faces are numbered by first appearance in cell order and oriented by that
+ * appearance.  Replaces instantiate(xgrid, CellFaces) / CellFaceSigns /
+ * BFaceFaces (ExtendableGrids; call sites src/dofmaps.jl:236,401-460,
+ * src/fedefs/h1v_br.jl:151-161).  Any output may be NULL; *nfaces is always
+ * written. */
+int grmp_mesh_faces(const grmp_mesh *m, int64_t *nfaces, int32_t *cellfaces, int8_t *cellfacesigns,
+                    uint8_t *bface);
 
 /* ------------------------------------------------------------- FESpace */
 /* FESpace from the caller's dofmap: FES[CellDofs] ([ndofs_loc, ncells],

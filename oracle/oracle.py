@@ -247,13 +247,18 @@ def quad_stroud(order):
 
 
 class StokesSpace:
-    """H1BR{2} x L2P0 on the structured unit square: dofs, face geometry,
-    no-slip fixed dofs and the pinned first pressure dof (FixedIntegralMean)."""
+    """H1BR{2} x L2P0 on the structured unit square (or on the triangle mesh
+    ``coords``, ``cells`` when given): dofs, face geometry, no-slip fixed
+    dofs and the pinned first pressure dof (FixedIntegralMean)."""
 
-    def __init__(self, N):
+    def __init__(self, N, coords=None, cells=None):
         L = _stokes_lib()
         self.N = N
-        self.coords, self.cells = unitsquare(N)
+        if coords is None:
+            self.coords, self.cells = unitsquare(N)
+        else:
+            self.coords = np.ascontiguousarray(coords, dtype=np.float64)
+            self.cells = np.ascontiguousarray(cells, dtype=np.int32)
         self.nnodes = self.coords.shape[0]
         self.nc = self.cells.shape[0]
         self.cellfaces, self.facenodes, self.facecells = tri_faces(self.cells, self.nnodes)
@@ -301,7 +306,9 @@ class StokesSpace:
 
     def pressure_mean_shift(self, x):
         """realize_constraint!(FixedIntegralMean(2, 0)) on P0: p -= mean(p)."""
-        vol = np.full(self.nc, 0.5 / self.N / self.N)
+        P = self.coords[self.cells]
+        vol = 0.5 * np.abs((P[:, 1, 0] - P[:, 0, 0]) * (P[:, 2, 1] - P[:, 0, 1])
+                           - (P[:, 2, 0] - P[:, 0, 0]) * (P[:, 1, 1] - P[:, 0, 1]))
         p = x[self.nU:]
         mean = float(np.sum(p * vol) / np.sum(vol))
         y = x.copy()
