@@ -193,5 +193,29 @@ int zero_fixed_launch(const grmp_matrix *A, double *b, hipStream_t st);
 int plain_cg_solve(grmp_matrix *A, const double *b, double *x, const grmp_solve_opts &o, grmp_solve_info &info);
 int minres_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solve_opts &o, grmp_solve_info &info);
 
+// multigrid-preconditioned CG on unit-square hierarchies (grmp_mg.hip)
+struct MgLevelIn {
+    const grmp_matrix *A;    // pattern of the level (values are the job's own)
+    const grmp_space *sp;
+    int N, fe, ncomp;
+};
+struct MgStatus {            // host copy of the job's scalars
+    double relres;
+    int it, done, status, pad;
+};
+struct MgJob;
+int64_t mg_job_bytes(const MgLevelIn *in, int nlev);
+int mg_job_create(grmp_ctx *ctx, hipStream_t st, const MgLevelIn *in, int nlev, bool profile, MgJob **out);
+double *mg_job_kappa0(MgJob *j);     // level-0 cell coefficient, filled by the caller
+double *mg_job_rhs(MgJob *j);        // level-0 right-hand side, filled by the caller
+const double *mg_job_solution(const MgJob *j);
+int mg_job_setup(MgJob *j, double rtol, int maxit);
+int mg_job_iterate(MgJob *j, int iters);
+int mg_job_verify(MgJob *j);         // after done == 2: check the true residual
+int mg_job_status_async(MgJob *j, MgStatus *host);
+bool mg_job_status_ready(MgJob *j);
+int mg_job_wait(MgJob *j);
+void mg_job_profile(MgJob *j, double *seconds, double *bytes, int64_t *launches);
+void mg_job_destroy(MgJob *j);
 // timing
 double wall_seconds();

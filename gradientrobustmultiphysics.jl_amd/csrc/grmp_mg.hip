@@ -1,0 +1,848 @@
+// Multigrid-preconditioned CG for the served unit-square families (Poisson
+// P1 / P2, elasticity P1 with a per-cell coefficient, homogeneous Dirichlet
+// data on the whole boundary, penalized fixed rows as in the reference's
+// solve_direct!, src/solvers.jl:647-652).
+//
+// The reference solves these systems with a sparse factorization or an
+// unpreconditioned Krylov method (src/solvers.jl:38-59, 622-660); the
+// solution it defines is the FP64 solution of the penalized system, which this
+// solver reaches to the requested relative residual.  The preconditioner is
+// this library's own: one V-cycle of a geometric hierarchy
+//   level 0      the request's own matrix (P1, P2 or P1 x 2 on N x N),
+//   [level 1     P1 on the same mesh, when level 0 is P2],
+//   then         P1 on ceil(N_l / 2) x ceil(N_l / 2), down to N_l <= 4,
+// each coarse matrix assembled by the same row-owner kernels from a
+// coefficient coarsened by the geometric mean of the fine cells whose
+// centroid lies in the coarse cell.  Transfers are linear interpolation on the
+// coarse triangulation (nested or not) and its exact transpose, both written
+// as gathers so that every sum has a fixed order; fixed dofs receive no
+// correction and carry no residual.  Smoothing is Chebyshev of degree 2 on
+// D^-1 A over [lmax/8, lmax] with lmax the Gershgorin bound of D^-1 A; the
+// coarsest level is solved exactly with its dense inverse.  Pre- and
+// post-smoothing are the same polynomial, so the V-cycle is symmetric and CG
+// applies.  All scalars live on the device; the host only polls a status
+// copy.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "grmp_internal.h"
+#include "grmp_sparse.h"
+#include "grmp_square.h"
+
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+#define MG_MAX_LEVELS 16
+#define MG_DENSE_MAX 64  // dofs of the coarsest level (P1 x 2 on 4 x 4: 50)
+
+// Chebyshev degree 2 on [lmax/8, lmax]: theta = 9/16 lmax, delta = 7/16 lmax,
+// rho0 = 7/9, rho1 = 63/113.
+//   x1 = x0 + d0,  d0 = D^-1 r0 / theta
+//   x2 = x1 + C1 d0 + C2 D^-1 (b - A x1),  C1 = rho1 rho0, C2 = 2 rho1 / delta
+#define CHEB_ITHETA(l) (16.0 / (9.0 * (l)))
+#define CHEB_C1 (49.0 / 113.0)
+#define CHEB_C2(l) (288.0 / (113.0 * (l)))
+
+// done: 0 iterating, 2 the updated residual met rtol (the true residual is
+// checked next), 1 finished (status says how)
+struct MgScal {
+    double rz, pq, rr, rr0, alpha, beta, relres, rtol;
+    int it, done, status, maxit;
+    double best;      // smallest true relative residual seen at a check
+    int restarts, pad;
+};
+
+// ------------------------------------------------------------ row kernels
+enum MgOp { OP_RESID = 0, OP_CHEB_PRE = 1, OP_CHEB_POST1 = 2, OP_CHEB_POST2 = 3, OP_PCG_Q = 4 };
+
+struct MgRows {
+    const int32_t *rp, *ci;
+    const double *val;
+    const uint8_t *fixed;
+    int64_t n;
+    const double *v;     // the vector A multiplies
+    const double *b, *dinv, *d;
+    double *o1, *o2;
+    const double *lam;   // Gershgorin bound of D^-1 A (device)
+    const MgScal *sc;
+    double *part;        // block partials (OP_PCG_Q: p.q; OP_CHEB_POST2 at level 0: r.z)
+};
+
+// 256 rows per block through the LDS-staged row product, then the epilogue:
+//   RESID   o1 = b - A v
+//   PRE     v = d0:  o1 = (1 + C1) d0 + C2 D^-1 (b - A d0)
+//   POST1   v = x:   d = D^-1 (b - A x) / theta -> o1,  x + d -> o2
+//   POST2   v = x1:  o1 = x1 + C1 d + C2 D^-1 (b - A x1)   [+ partials of b . o1]
+//   PCG_Q   o1 = A v                                        [+ partials of v . o1]
+// Fixed rows get 0 (no residual, no correction).
+template <int OP>
+__global__ __launch_bounds__(KB) void k_mg_rows(MgRows a)
+{
+    if (a.sc->done) return;
+    __shared__ double lds[SPMV_CAP];
+    const int64_t r0 = (int64_t)blockIdx.x * KB;
+    const int64_t r1 = r0 + KB < a.n ? r0 + KB : a.n;
+    const double s = csr_rows_lds<double, double, double>(a.rp, a.ci, a.val, a.v, r0, r1, lds);
+    const int64_t i = r0 + threadIdx.x;
+    double pa = 0.0, pb = 0.0;
+    if (i < r1) {
+        const bool fx = a.fixed[i] != 0;
+        if (OP == OP_RESID) {
+            a.o1[i] = fx ? 0.0 : a.b[i] - s;
+        } else if (OP == OP_CHEB_PRE) {
+            const double l = *a.lam;
+            a.o1[i] = fx ? 0.0 : (1.0 + CHEB_C1) * a.v[i] + CHEB_C2(l) * a.dinv[i] * (a.b[i] - s);
+        } else if (OP == OP_CHEB_POST1) {
+            const double l = *a.lam;
+            const double d = fx ? 0.0 : a.dinv[i] * (a.b[i] - s) * CHEB_ITHETA(l);
+            a.o1[i] = d;
+            a.o2[i] = a.v[i] + d;
+        } else if (OP == OP_CHEB_POST2) {
+            const double l = *a.lam;
+            const double z = fx ? 0.0 : a.v[i] + CHEB_C1 * a.d[i] + CHEB_C2(l) * a.dinv[i] * (a.b[i] - s);
+            a.o1[i] = z;
+            pa = a.b[i] * z;
+        } else {
+            const double q = fx ? 0.0 : s;
+            a.o1[i] = q;
+            pa = a.v[i] * q;
+        }
+    }
+    if ((OP == OP_CHEB_POST2 || OP == OP_PCG_Q) && a.part) {
+        block_sum2(pa, pb);
+        if (threadIdx.x == 0) a.part[blockIdx.x] = pa;
+    }
+}
+
+// first pre-smoothing step from x0 = 0: d0 = D^-1 b / theta
+__global__ void k_mg_cheb_first(const double *__restrict__ b, const double *__restrict__ dinv,
+                                const uint8_t *__restrict__ fixed, const double *__restrict__ lam, int64_t n,
+                                double *__restrict__ d, const MgScal *sc)
+{
+    if (sc->done) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    d[i] = fixed[i] ? 0.0 : dinv[i] * b[i] * CHEB_ITHETA(*lam);
+}
+
+// Gershgorin bound max_i sum_j |A_ij| / A_ii over free rows (a max, so the
+// result does not depend on the order of the atomics)
+__global__ void k_mg_gersh(const int32_t *__restrict__ rp, const double *__restrict__ val,
+                           const double *__restrict__ dinv, const uint8_t *__restrict__ fixed, int64_t n,
+                           unsigned long long *lam)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (i < n && !fixed[i]) {
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) s += fabs(val[k]);
+        s *= fabs(dinv[i]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s = fmax(s, __shfl_down(s, o, 64));
+    if ((threadIdx.x & 63) == 0 && s > 0.0) atomicMax(lam, (unsigned long long)__double_as_longlong(s));
+}
+
+// ------------------------------------------------------------- transfers
+// Fine node (i, j) of the Nf-square in the Nc-triangulation: coarse square
+// (ic, jc), offsets u = un / Nf, v = vn / Nf in [0, 1], and the interpolation
+// weights of the coarse triangle holding it (lower when u >= v).
+struct P1Loc {
+    int64_t a, b, c;  // coarse nodes
+    double wa, wb, wc;
+};
+
+__device__ __forceinline__ P1Loc p1_locate(int64_t Nf, int64_t Nc, int64_t i, int64_t j)
+{
+    int64_t ic = i * Nc / Nf, jc = j * Nc / Nf;
+    if (ic > Nc - 1) ic = Nc - 1;
+    if (jc > Nc - 1) jc = Nc - 1;
+    const int64_t un = i * Nc - ic * Nf, vn = j * Nc - jc * Nf;
+    const double inv = 1.0 / (double)Nf;
+    const int64_t n00 = jc * (Nc + 1) + ic;
+    P1Loc L;
+    if (un >= vn) {  // lower triangle (ic,jc), (ic+1,jc), (ic+1,jc+1)
+        L.a = n00;
+        L.b = n00 + 1;
+        L.c = n00 + Nc + 2;
+        L.wa = (double)(Nf - un) * inv;
+        L.wb = (double)(un - vn) * inv;
+        L.wc = (double)vn * inv;
+    } else {         // upper triangle (ic,jc), (ic+1,jc+1), (ic,jc+1)
+        L.a = n00;
+        L.b = n00 + Nc + 2;
+        L.c = n00 + Nc + 1;
+        L.wa = (double)(Nf - vn) * inv;
+        L.wb = (double)un * inv;
+        L.wc = (double)(vn - un) * inv;
+    }
+    return L;
+}
+
+// xf += P xc on free fine dofs (components blocked by node count)
+__global__ void k_mg_prolong_p1(int64_t Nf, int64_t Nc, int ncomp, const double *__restrict__ xc,
+                                const uint8_t *__restrict__ fixed_f, double *__restrict__ xf, const MgScal *sc)
+{
+    if (sc->done) return;
+    const int64_t nnf = (Nf + 1) * (Nf + 1), nnc = (Nc + 1) * (Nc + 1);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nnf * ncomp) return;
+    const int k = (int)(t / nnf);
+    const int64_t node = t - k * nnf;
+    if (fixed_f[t]) return;
+    const P1Loc L = p1_locate(Nf, Nc, node % (Nf + 1), node / (Nf + 1));
+    const double *x = xc + k * nnc;
+    xf[t] += L.wa * x[L.a] + L.wb * x[L.b] + L.wc * x[L.c];
+}
+
+// bc = P^T rf on free coarse dofs, 0 on fixed ones.  Coarse node (I, J)
+// gathers the fine nodes whose coarse triangle has it as a vertex; they lie in
+// the fine index box below, visited row by row.
+__global__ void k_mg_restrict_p1(int64_t Nf, int64_t Nc, int ncomp, const double *__restrict__ rf,
+                                 const uint8_t *__restrict__ fixed_c, double *__restrict__ bc, const MgScal *sc)
+{
+    if (sc->done) return;
+    const int64_t nnf = (Nf + 1) * (Nf + 1), nnc = (Nc + 1) * (Nc + 1);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nnc * ncomp) return;
+    if (fixed_c[t]) {
+        bc[t] = 0.0;
+        return;
+    }
+    const int k = (int)(t / nnc);
+    const int64_t node = t - k * nnc;
+    const int64_t I = node % (Nc + 1), J = node / (Nc + 1);
+    const int64_t ilo = I >= 1 ? (I - 1) * Nf / Nc : 0, jlo = J >= 1 ? (J - 1) * Nf / Nc : 0;
+    int64_t ihi = ((I + 1) * Nf + Nc - 1) / Nc, jhi = ((J + 1) * Nf + Nc - 1) / Nc;
+    if (ihi > Nf) ihi = Nf;
+    if (jhi > Nf) jhi = Nf;
+    const double *r = rf + k * nnf;
+    double s = 0.0;
+    for (int64_t j = jlo; j <= jhi; ++j)
+        for (int64_t i = ilo; i <= ihi; ++i) {
+            const P1Loc L = p1_locate(Nf, Nc, i, j);
+            const double w = L.a == node ? L.wa : L.b == node ? L.wb : L.c == node ? L.wc : 0.0;
+            if (w != 0.0) s += w * r[j * (Nf + 1) + i];
+        }
+    bc[t] = s;
+}
+
+// P1 -> P2 on the same N-square: node dofs copy, face dofs take the mean of
+// their end nodes.  Each face is handled by the one cell whose CellFaceSign is
+// +1 for it (its first appearance).
+__global__ void k_mg_prolong_p2(int64_t N, const int32_t *__restrict__ cells, const int8_t *__restrict__ fsign,
+                                const double *__restrict__ x1, const uint8_t *__restrict__ fixed2,
+                                double *__restrict__ x2, const MgScal *sc)
+{
+    if (sc->done) return;
+    const int64_t nn = (N + 1) * (N + 1), nc = 2 * N * N;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nn && !fixed2[t]) x2[t] += x1[t];
+    if (t >= nc) return;
+    int64_t f[3];
+    sq_cellfaces(N, t, f);
+    for (int k = 0; k < 3; ++k) {
+        if (fsign[k * nc + t] <= 0) continue;
+        const int64_t dof = nn + f[k];
+        if (fixed2[dof]) continue;
+        const int32_t a = cells[k * nc + t], b = cells[((k + 1) % 3) * nc + t];
+        x2[dof] += 0.5 * (x1[a] + x1[b]);
+    }
+}
+
+// its transpose: node v gathers r2[v] and half of each incident face's entry
+// (faces: left / right horizontal, lower / upper vertical, lower-left /
+// upper-right diagonal)
+__global__ void k_mg_restrict_p2(int64_t N, const double *__restrict__ r2, const uint8_t *__restrict__ fixed1,
+                                 double *__restrict__ b1, const MgScal *sc)
+{
+    if (sc->done) return;
+    const int64_t nn = (N + 1) * (N + 1);
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nn) return;
+    if (fixed1[v]) {
+        b1[v] = 0.0;
+        return;
+    }
+    const int64_t i = v % (N + 1), j = v / (N + 1);
+    const double *rf = r2 + nn;
+    double s = 0.0;
+    if (i > 0) s += rf[sq_H(N, i - 1, j)];
+    if (i < N) s += rf[sq_H(N, i, j)];
+    if (j > 0) s += rf[sq_V(N, i, j - 1)];
+    if (j < N) s += rf[sq_V(N, i, j)];
+    if (i > 0 && j > 0) s += rf[sq_D(N, i - 1, j - 1)];
+    if (i < N && j < N) s += rf[sq_D(N, i, j)];
+    b1[v] = r2[v] + 0.5 * s;
+}
+
+// coarse coefficient: geometric mean of the fine cells whose centroid lies in
+// the coarse cell (centroids in units of 1 / (3 Nf); same integer test as the
+// fine-to-coarse location above).  A coarse cell without a fine centroid
+// (not met for Nf <= 2048) takes the fine cell holding its own centroid.
+__device__ __forceinline__ int64_t mg_cell_of(int64_t Nf, int64_t Nc, int64_t X, int64_t Y)
+{
+    // point (X, Y) / (3 Nf) in the Nc-triangulation
+    int64_t ic = X * Nc / (3 * Nf), jc = Y * Nc / (3 * Nf);
+    if (ic > Nc - 1) ic = Nc - 1;
+    if (jc > Nc - 1) jc = Nc - 1;
+    const int64_t un = X * Nc - ic * 3 * Nf, vn = Y * Nc - jc * 3 * Nf;
+    return 2 * (jc * Nc + ic) + (un >= vn ? 0 : 1);
+}
+
+__global__ void k_mg_coarse_kappa(int64_t Nf, int64_t Nc, const double *__restrict__ kf, double *__restrict__ kc)
+{
+    const int64_t cc = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cc >= 2 * Nc * Nc) return;
+    const int64_t sq = cc >> 1, jc = sq / Nc, ic = sq % Nc;
+    // fine squares overlapping the coarse square
+    const int64_t ilo = ic * Nf / Nc, jlo = jc * Nf / Nc;
+    int64_t ihi = ((ic + 1) * Nf + Nc - 1) / Nc, jhi = ((jc + 1) * Nf + Nc - 1) / Nc;
+    if (ihi > Nf) ihi = Nf;
+    if (jhi > Nf) jhi = Nf;
+    double s = 0.0;
+    int cnt = 0;
+    for (int64_t j = jlo; j < jhi; ++j)
+        for (int64_t i = ilo; i < ihi; ++i)
+            for (int up = 0; up < 2; ++up) {
+                const int64_t X = 3 * i + (up ? 1 : 2), Y = 3 * j + (up ? 2 : 1);
+                if (mg_cell_of(Nf, Nc, X, Y) != cc) continue;
+                s += log(kf[2 * (j * Nf + i) + up]);
+                ++cnt;
+            }
+    if (cnt == 0) {
+        // centroid of the coarse cell in units of 1 / (3 Nc), located among fine cells
+        const int64_t Xc = 3 * ic + ((cc & 1) ? 1 : 2), Yc = 3 * jc + ((cc & 1) ? 2 : 1);
+        kc[cc] = kf[mg_cell_of(Nc, Nf, Xc, Yc)];
+        return;
+    }
+    kc[cc] = exp(s / cnt);
+}
+
+// ------------------------------------------------------ coarsest level
+// Dense inverse of the coarsest matrix (fixed rows and columns replaced by
+// the identity), in-place Gauss-Jordan in LDS by one block.  The matrix is
+// symmetric positive definite, so no pivoting is needed.
+__global__ __launch_bounds__(256) void k_mg_dense_inv(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                      const double *__restrict__ val, const uint8_t *__restrict__ fixed,
+                                                      int n, double *__restrict__ inv)
+{
+    __shared__ double M[MG_DENSE_MAX * MG_DENSE_MAX];
+    __shared__ double f[MG_DENSE_MAX];
+    const int t = threadIdx.x;
+    for (int e = t; e < n * n; e += blockDim.x) M[e] = 0.0;
+    __syncthreads();
+    if (t < n) {
+        if (fixed[t]) {
+            M[t * n + t] = 1.0;
+        } else {
+            for (int32_t k = rp[t]; k < rp[t + 1]; ++k)
+                if (!fixed[ci[k]]) M[t * n + ci[k]] = val[k];
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        const double p = M[k * n + k];
+        __syncthreads();
+        if (t < n) M[k * n + t] = (t == k ? 1.0 : M[k * n + t]) / p;
+        __syncthreads();
+        if (t < n) f[t] = t == k ? 0.0 : M[t * n + k];
+        __syncthreads();
+        for (int e = t; e < n * n; e += blockDim.x) {
+            const int r = e / n, c = e - r * n;
+            if (r == k) continue;
+            M[e] = (c == k ? 0.0 : M[e]) - f[r] * M[k * n + c];
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < n * n; e += blockDim.x) inv[e] = M[e];
+}
+
+__global__ void k_mg_dense_apply(const double *__restrict__ inv, const double *__restrict__ b, int n,
+                                 double *__restrict__ x, const MgScal *sc)
+{
+    if (sc->done) return;
+    const int i = threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int j = 0; j < n; ++j) s += inv[i * n + j] * b[j];
+    x[i] = s;
+}
+
+// --------------------------------------------------------------- PCG
+__global__ void k_mg_dot(const double *__restrict__ a, const double *__restrict__ b, int64_t n, double *part,
+                         const MgScal *sc)
+{
+    if (sc->done) return;
+    const int64_t i = (int64_t)blockIdx.x * KB + threadIdx.x;
+    double pa = i < n ? a[i] * b[i] : 0.0, pb = 0.0;
+    block_sum2(pa, pb);
+    if (threadIdx.x == 0) part[blockIdx.x] = pa;
+}
+
+// x += alpha p, r -= alpha q; partials of r.r
+__global__ void k_mg_axpy(double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
+                          const double *__restrict__ q, int64_t n, double *part, const MgScal *sc)
+{
+    if (sc->done) return;
+    const double al = sc->alpha;
+    const int64_t i = (int64_t)blockIdx.x * KB + threadIdx.x;
+    double pa = 0.0, pb = 0.0;
+    if (i < n) {
+        x[i] += al * p[i];
+        const double ri = r[i] - al * q[i];
+        r[i] = ri;
+        pa = ri * ri;
+    }
+    block_sum2(pa, pb);
+    if (threadIdx.x == 0) part[blockIdx.x] = pa;
+}
+
+__global__ void k_mg_update_p(double *__restrict__ p, const double *__restrict__ z, int64_t n, const MgScal *sc)
+{
+    if (sc->done) return;
+    const double be = sc->beta;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = z[i] + be * p[i];
+}
+
+// one block: the fixed-order sum of nb partials, then the scalar step
+enum MgRed { RED_INIT = 0, RED_ALPHA = 1, RED_CONV = 2, RED_BETA = 3, RED_VERIFY = 4, RED_RESTART = 5 };
+template <int W>
+__global__ __launch_bounds__(KB) void k_mg_red(const double *__restrict__ part, int nb, const double *__restrict__ part2,
+                                               MgScal *sc)
+{
+    if (sc->done) return;
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < nb; k += KB) {
+        a += part[k];
+        if (W == RED_INIT) b += part2[k];
+    }
+    block_sum2(a, b);
+    if (threadIdx.x != 0) return;
+    if (W == RED_INIT) {  // a = r.z, b = r.r of the start residual
+        sc->rz = a;
+        sc->rr0 = b;
+        sc->rr = b;
+        sc->beta = 0.0;
+        sc->relres = b > 0.0 ? 1.0 : 0.0;
+        if (!(b > 0.0)) sc->done = 1;  // zero right-hand side: x = 0
+        else if (!(a > 0.0) || !isfinite(a)) { sc->status = GRMP_ERR_NONFINITE; sc->done = 1; }
+    } else if (W == RED_ALPHA) {
+        sc->pq = a;
+        if (!(a > 0.0) || !isfinite(a)) { sc->status = GRMP_ERR_NONFINITE; sc->done = 1; return; }
+        sc->alpha = sc->rz / a;
+    } else if (W == RED_CONV) {
+        sc->rr = a;
+        sc->it += 1;
+        sc->relres = sqrt(a / sc->rr0);
+        if (!isfinite(a)) { sc->status = GRMP_ERR_NONFINITE; sc->done = 1; }
+        else if (sc->relres <= sc->rtol) sc->done = 2;
+        else if (sc->it >= sc->maxit) { sc->status = GRMP_ERR_NOCONV; sc->done = 1; }
+    } else if (W == RED_VERIFY) {  // a = |b - A x|^2
+        const double tr = sqrt(a / sc->rr0);
+        sc->rr = a;
+        sc->relres = tr;
+        if (!isfinite(a)) { sc->status = GRMP_ERR_NONFINITE; sc->done = 1; }
+        else if (tr <= sc->rtol) sc->done = 1;
+        else if (sc->restarts >= 3 || (sc->restarts > 0 && tr > 0.5 * sc->best)) {
+            sc->status = GRMP_ERR_NOCONV;  // the true residual stopped improving
+            sc->done = 1;
+        } else {
+            sc->restarts += 1;
+            sc->best = sc->restarts == 1 || tr < sc->best ? tr : sc->best;
+        }
+    } else if (W == RED_RESTART) {  // a = r.z of the true residual: p = z
+        if (!(a > 0.0) || !isfinite(a)) { sc->status = GRMP_ERR_NONFINITE; sc->done = 1; return; }
+        sc->rz = a;
+        sc->beta = 0.0;
+    } else {
+        if (!isfinite(a)) { sc->status = GRMP_ERR_NONFINITE; sc->done = 1; return; }
+        sc->beta = a / sc->rz;
+        sc->rz = a;
+    }
+}
+
+// a check was requested: resume the kernels for it
+__global__ void k_mg_verify_begin(MgScal *sc)
+{
+    if (threadIdx.x == 0 && sc->done == 2) sc->done = 0;
+}
+
+// ------------------------------------------------------------- the job
+struct MgLevel {
+    MgLevelIn in;
+    int64_t n = 0, nnz = 0, nc = 0;
+    double *kappa = nullptr, *val = nullptr, *dinv = nullptr, *x = nullptr, *b = nullptr, *r = nullptr, *d = nullptr;
+    float *dinv32 = nullptr;
+    double *lam = nullptr;
+    DevForm *dforms = nullptr;
+    DevForm hforms[4];
+};
+
+struct MgJob {
+    grmp_ctx *ctx = nullptr;
+    hipStream_t st = nullptr;
+    int nlev = 0;
+    MgLevel lv[MG_MAX_LEVELS];
+    char *buf = nullptr;
+    size_t bytes = 0;
+    double *x = nullptr, *rhs = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+    double *part = nullptr, *part2 = nullptr, *inv = nullptr;
+    MgScal *sc = nullptr;
+    int nb0 = 0;
+    hipEvent_t ev = nullptr;
+    // profile: event pairs around every OP_RESID launch, with its algorithmic bytes
+    bool profile = false;
+    std::vector<hipEvent_t> pev;
+    std::vector<double> pbytes;
+    double prof_s = 0.0, prof_bytes = 0.0;
+    int64_t prof_launches = 0;
+};
+
+static inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static size_t mg_layout(MgJob *j, bool assign)
+{
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += al256(bytes); return o; };
+    const int64_t n0 = j->lv[0].n;
+    const int nb0 = (int)nblk(n0, KB);
+    size_t o_x = take(n0 * 8), o_rhs = take(n0 * 8), o_r = take(n0 * 8), o_z = take(n0 * 8), o_p = take(n0 * 8),
+           o_q = take(n0 * 8), o_part = take(nb0 * 8), o_part2 = take(nb0 * 8), o_sc = take(sizeof(MgScal));
+    const MgLevel &c = j->lv[j->nlev - 1];
+    size_t o_inv = take(c.n * c.n * 8);
+    std::vector<size_t> o(MG_MAX_LEVELS * 9);
+    for (int l = 0; l < j->nlev; ++l) {
+        const MgLevel &L = j->lv[l];
+        const bool own_kappa = l > 0 && L.in.N != j->lv[l - 1].in.N;
+        o[9 * l + 0] = own_kappa || l == 0 ? take(L.nc * 8) : 0;
+        o[9 * l + 1] = take(L.nnz * 8);
+        o[9 * l + 2] = take(L.n * 8);
+        o[9 * l + 3] = take(L.n * 4);
+        o[9 * l + 4] = l == 0 ? 0 : take(L.n * 8);  // x (level 0: z)
+        o[9 * l + 5] = l == 0 ? 0 : take(L.n * 8);  // b (level 0: r)
+        o[9 * l + 6] = take(L.n * 8);
+        o[9 * l + 7] = take(L.n * 8);
+        o[9 * l + 8] = take(8 + sizeof(DevForm) * 4);
+    }
+    if (!assign) return off;
+    char *B = j->buf;
+    j->x = (double *)(B + o_x);
+    j->rhs = (double *)(B + o_rhs);
+    j->r = (double *)(B + o_r);
+    j->z = (double *)(B + o_z);
+    j->p = (double *)(B + o_p);
+    j->q = (double *)(B + o_q);
+    j->part = (double *)(B + o_part);
+    j->part2 = (double *)(B + o_part2);
+    j->sc = (MgScal *)(B + o_sc);
+    j->inv = (double *)(B + o_inv);
+    j->nb0 = nb0;
+    for (int l = 0; l < j->nlev; ++l) {
+        MgLevel &L = j->lv[l];
+        const bool own_kappa = l > 0 && L.in.N != j->lv[l - 1].in.N;
+        L.kappa = (own_kappa || l == 0) ? (double *)(B + o[9 * l + 0]) : j->lv[l - 1].kappa;
+        L.val = (double *)(B + o[9 * l + 1]);
+        L.dinv = (double *)(B + o[9 * l + 2]);
+        L.dinv32 = (float *)(B + o[9 * l + 3]);
+        L.x = l == 0 ? j->z : (double *)(B + o[9 * l + 4]);
+        L.b = l == 0 ? j->r : (double *)(B + o[9 * l + 5]);
+        L.r = (double *)(B + o[9 * l + 6]);
+        L.d = (double *)(B + o[9 * l + 7]);
+        L.lam = (double *)(B + o[9 * l + 8]);
+        L.dforms = (DevForm *)(B + o[9 * l + 8] + 8);
+    }
+    return off;
+}
+
+int64_t mg_job_bytes(const MgLevelIn *in, int nlev)
+{
+    MgJob j;
+    j.nlev = nlev;
+    for (int l = 0; l < nlev; ++l) {
+        j.lv[l].in = in[l];
+        j.lv[l].n = in[l].A->n;
+        j.lv[l].nnz = in[l].A->nnz;
+        j.lv[l].nc = in[l].sp->mesh->ncells;
+    }
+    return (int64_t)mg_layout(&j, false);
+}
+
+int mg_job_create(grmp_ctx *ctx, hipStream_t st, const MgLevelIn *in, int nlev, bool profile, MgJob **out)
+{
+    GRMP_CHECK(ctx, nlev >= 1 && nlev <= MG_MAX_LEVELS, GRMP_ERR_ARG, "multigrid: %d levels", nlev);
+    for (int l = 0; l < nlev; ++l) {
+        const MgLevelIn &L = in[l];
+        GRMP_CHECK(ctx, L.A && L.sp && L.A->have_pattern && L.sp->mesh->kind == MESH_UNITSQUARE && L.sp->mesh->N == L.N &&
+                            L.A->n == L.sp->ndofs && L.A->n < (1ll << 31),
+                   GRMP_ERR_ARG, "multigrid: level %d is not an assembled unit-square pattern", l);
+        if (l > 0) {
+            const MgLevelIn &F = in[l - 1];
+            const bool p2 = F.fe == GRMP_FE_H1P2 && L.fe == GRMP_FE_H1P1 && L.N == F.N && L.ncomp == 1 && F.ncomp == 1;
+            const bool p1 = F.fe == GRMP_FE_H1P1 && L.fe == GRMP_FE_H1P1 && L.N == (F.N + 1) / 2 && L.ncomp == F.ncomp;
+            GRMP_CHECK(ctx, p2 || p1, GRMP_ERR_ARG, "multigrid: level %d does not coarsen level %d", l, l - 1);
+        }
+    }
+    GRMP_CHECK(ctx, in[nlev - 1].A->n <= MG_DENSE_MAX, GRMP_ERR_ARG, "multigrid: coarsest level has %lld > %d dofs",
+               (long long)in[nlev - 1].A->n, MG_DENSE_MAX);
+    MgJob *j = new MgJob();
+    j->ctx = ctx;
+    j->st = st;
+    j->nlev = nlev;
+    j->profile = profile;
+    for (int l = 0; l < nlev; ++l) {
+        j->lv[l].in = in[l];
+        j->lv[l].n = in[l].A->n;
+        j->lv[l].nnz = in[l].A->nnz;
+        j->lv[l].nc = in[l].sp->mesh->ncells;
+    }
+    j->bytes = mg_layout(j, false);
+    hipError_t e = hipMallocAsync((void **)&j->buf, j->bytes, st);
+    if (e != hipSuccess) {
+        delete j;
+        return grmp_fail(ctx, e == hipErrorOutOfMemory ? GRMP_ERR_OOM : GRMP_ERR_HIP, "multigrid: device buffer");
+    }
+    mg_layout(j, true);
+    if (hipEventCreateWithFlags(&j->ev, hipEventDisableTiming) != hipSuccess) {
+        (void)hipFreeAsync(j->buf, st);
+        delete j;
+        return grmp_fail(ctx, GRMP_ERR_HIP, "multigrid: event");
+    }
+    *out = j;
+    return GRMP_OK;
+}
+
+double *mg_job_kappa0(MgJob *j) { return j->lv[0].kappa; }
+double *mg_job_rhs(MgJob *j) { return j->rhs; }
+const double *mg_job_solution(const MgJob *j) { return j->x; }
+
+static void rows_args(MgRows &a, const MgLevel &L, const MgJob *j)
+{
+    memset(&a, 0, sizeof(a));
+    a.rp = L.in.A->rowptr;
+    a.ci = L.in.A->cols;
+    a.val = L.val;
+    a.fixed = L.in.A->fixed;
+    a.n = L.n;
+    a.dinv = L.dinv;
+    a.lam = L.lam;
+    a.sc = j->sc;
+}
+
+template <int OP>
+static void launch_rows(MgJob *j, const MgRows &a)
+{
+    if (OP == OP_RESID && j->profile) {
+        hipEvent_t e0, e1;
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        (void)hipEventRecord(e0, j->st);
+        hipLaunchKernelGGL(k_mg_rows<OP>, dim3(nblk(a.n, KB)), dim3(KB), 0, j->st, a);
+        (void)hipEventRecord(e1, j->st);
+        j->pev.push_back(e0);
+        j->pev.push_back(e1);
+        return;
+    }
+    hipLaunchKernelGGL(k_mg_rows<OP>, dim3(nblk(a.n, KB)), dim3(KB), 0, j->st, a);
+}
+
+// restriction of level l's residual r_l into b_{l+1}
+static void mg_restrict(MgJob *j, int l)
+{
+    const MgLevel &F = j->lv[l], &C = j->lv[l + 1];
+    if (F.in.fe == GRMP_FE_H1P2)
+        hipLaunchKernelGGL(k_mg_restrict_p2, dim3(nblk(C.n, 256)), dim3(256), 0, j->st, (int64_t)F.in.N, F.r,
+                           C.in.A->fixed, C.b, j->sc);
+    else
+        hipLaunchKernelGGL(k_mg_restrict_p1, dim3(nblk(C.n, 256)), dim3(256), 0, j->st, (int64_t)F.in.N,
+                           (int64_t)C.in.N, F.in.ncomp, F.r, C.in.A->fixed, C.b, j->sc);
+}
+
+static void mg_prolong(MgJob *j, int l)
+{
+    const MgLevel &F = j->lv[l], &C = j->lv[l + 1];
+    if (F.in.fe == GRMP_FE_H1P2) {
+        const grmp_mesh *m = F.in.sp->mesh;
+        const int64_t nn = m->nnodes, nc = m->ncells;
+        hipLaunchKernelGGL(k_mg_prolong_p2, dim3(nblk(std::max(nn, nc), 256)), dim3(256), 0, j->st, (int64_t)F.in.N,
+                           m->cells, m->fsign, C.x, F.in.A->fixed, F.x, j->sc);
+    } else {
+        hipLaunchKernelGGL(k_mg_prolong_p1, dim3(nblk(F.n, 256)), dim3(256), 0, j->st, (int64_t)F.in.N,
+                           (int64_t)C.in.N, F.in.ncomp, C.x, F.in.A->fixed, F.x, j->sc);
+    }
+}
+
+// x_0 (= z) = V(b_0 (= r)); at level 0 the last smoothing step also leaves
+// the block partials of r.z in part
+static void mg_vcycle(MgJob *j)
+{
+    const int L1 = j->nlev - 1;
+    for (int l = 0; l < L1; ++l) {
+        MgLevel &L = j->lv[l];
+        hipLaunchKernelGGL(k_mg_cheb_first, dim3(nblk(L.n, 256)), dim3(256), 0, j->st, L.b, L.dinv, L.in.A->fixed,
+                           L.lam, L.n, L.d, j->sc);
+        MgRows a;
+        rows_args(a, L, j);
+        a.v = L.d;
+        a.b = L.b;
+        a.o1 = L.x;
+        launch_rows<OP_CHEB_PRE>(j, a);
+        rows_args(a, L, j);
+        a.v = L.x;
+        a.b = L.b;
+        a.o1 = L.r;
+        launch_rows<OP_RESID>(j, a);
+        mg_restrict(j, l);
+    }
+    const MgLevel &C = j->lv[L1];
+    hipLaunchKernelGGL(k_mg_dense_apply, dim3(1), dim3(MG_DENSE_MAX), 0, j->st, j->inv, C.b, (int)C.n, C.x, j->sc);
+    for (int l = L1 - 1; l >= 0; --l) {
+        MgLevel &L = j->lv[l];
+        mg_prolong(j, l);
+        MgRows a;
+        rows_args(a, L, j);
+        a.v = L.x;
+        a.b = L.b;
+        a.o1 = L.d;
+        a.o2 = L.r;
+        launch_rows<OP_CHEB_POST1>(j, a);
+        rows_args(a, L, j);
+        a.v = L.r;
+        a.b = L.b;
+        a.d = L.d;
+        a.o1 = L.x;
+        a.part = l == 0 ? j->part : nullptr;
+        launch_rows<OP_CHEB_POST2>(j, a);
+    }
+    if (L1 == 0)  // a single level: z is the exact solve, r.z separately
+        hipLaunchKernelGGL(k_mg_dot, dim3(j->nb0), dim3(KB), 0, j->st, j->r, j->z, j->lv[0].n, j->part, j->sc);
+}
+
+int mg_job_setup(MgJob *j, double rtol, int maxit)
+{
+    grmp_ctx *ctx = j->ctx;
+    hipStream_t st = j->st;
+    MgScal s0;
+    memset(&s0, 0, sizeof(s0));
+    s0.rtol = rtol;
+    s0.maxit = maxit;
+    GRMP_HIP(ctx, hipMemcpyAsync(j->sc, &s0, sizeof(s0), hipMemcpyHostToDevice, st));
+    for (int l = 0; l < j->nlev; ++l) {
+        MgLevel &L = j->lv[l];
+        if (l > 0 && L.in.N != j->lv[l - 1].in.N)
+            hipLaunchKernelGGL(k_mg_coarse_kappa, dim3(nblk(L.nc, 256)), dim3(256), 0, st, (int64_t)j->lv[l - 1].in.N,
+                               (int64_t)L.in.N, j->lv[l - 1].kappa, L.kappa);
+        GRMP_TRY(assemble_values(L.in.A, L.kappa, L.dforms, L.hforms, L.val, nullptr, L.dinv, L.dinv32, st));
+        GRMP_HIP(ctx, hipMemsetAsync(L.lam, 0, 8, st));
+        hipLaunchKernelGGL(k_mg_gersh, dim3(nblk(L.n, 256)), dim3(256), 0, st, L.in.A->rowptr, L.val, L.dinv,
+                           L.in.A->fixed, L.n, (unsigned long long *)L.lam);
+    }
+    const MgLevel &C = j->lv[j->nlev - 1];
+    hipLaunchKernelGGL(k_mg_dense_inv, dim3(1), dim3(256), 0, st, C.in.A->rowptr, C.in.A->cols, C.val, C.in.A->fixed,
+                       (int)C.n, j->inv);
+    // x = 0, r = rhs (0 on fixed rows), z = V(r), p = 0 then p = z
+    const int64_t n = j->lv[0].n;
+    GRMP_HIP(ctx, hipMemsetAsync(j->x, 0, n * 8, st));
+    GRMP_HIP(ctx, hipMemsetAsync(j->p, 0, n * 8, st));
+    GRMP_HIP(ctx, hipMemcpyAsync(j->r, j->rhs, n * 8, hipMemcpyDeviceToDevice, st));
+    GRMP_TRY(zero_fixed_launch(j->lv[0].in.A, j->r, st));
+    mg_vcycle(j);
+    hipLaunchKernelGGL(k_mg_dot, dim3(j->nb0), dim3(KB), 0, st, j->r, j->r, n, j->part2, j->sc);
+    hipLaunchKernelGGL(k_mg_red<RED_INIT>, dim3(1), dim3(KB), 0, st, j->part, j->nb0, j->part2, j->sc);
+    hipLaunchKernelGGL(k_mg_update_p, dim3(nblk(n, 256)), dim3(256), 0, st, j->p, j->z, n, j->sc);
+    GRMP_HIP(ctx, hipGetLastError());
+    return GRMP_OK;
+}
+
+int mg_job_iterate(MgJob *j, int iters)
+{
+    const MgLevel &L0 = j->lv[0];
+    const int64_t n = L0.n;
+    for (int k = 0; k < iters; ++k) {
+        MgRows a;
+        rows_args(a, L0, j);
+        a.v = j->p;
+        a.o1 = j->q;
+        a.part = j->part;
+        launch_rows<OP_PCG_Q>(j, a);
+        hipLaunchKernelGGL(k_mg_red<RED_ALPHA>, dim3(1), dim3(KB), 0, j->st, j->part, j->nb0, nullptr, j->sc);
+        hipLaunchKernelGGL(k_mg_axpy, dim3(j->nb0), dim3(KB), 0, j->st, j->x, j->r, j->p, j->q, n, j->part, j->sc);
+        hipLaunchKernelGGL(k_mg_red<RED_CONV>, dim3(1), dim3(KB), 0, j->st, j->part, j->nb0, nullptr, j->sc);
+        mg_vcycle(j);
+        hipLaunchKernelGGL(k_mg_red<RED_BETA>, dim3(1), dim3(KB), 0, j->st, j->part, j->nb0, nullptr, j->sc);
+        hipLaunchKernelGGL(k_mg_update_p, dim3(nblk(n, 256)), dim3(256), 0, j->st, j->p, j->z, n, j->sc);
+    }
+    GRMP_HIP(j->ctx, hipGetLastError());
+    return GRMP_OK;
+}
+
+// The updated residual met rtol: replace it by the true residual b - A x;
+// accept if that meets rtol too, else restart CG from it (p = z), at most
+// three times and only while each check halves the previous one.
+int mg_job_verify(MgJob *j)
+{
+    const MgLevel &L0 = j->lv[0];
+    const int64_t n = L0.n;
+    hipLaunchKernelGGL(k_mg_verify_begin, dim3(1), dim3(1), 0, j->st, j->sc);
+    MgRows a;
+    rows_args(a, L0, j);
+    a.v = j->x;
+    a.b = j->rhs;
+    a.o1 = j->r;
+    hipLaunchKernelGGL(k_mg_rows<OP_RESID>, dim3(nblk(n, KB)), dim3(KB), 0, j->st, a);
+    hipLaunchKernelGGL(k_mg_dot, dim3(j->nb0), dim3(KB), 0, j->st, j->r, j->r, n, j->part, j->sc);
+    hipLaunchKernelGGL(k_mg_red<RED_VERIFY>, dim3(1), dim3(KB), 0, j->st, j->part, j->nb0, nullptr, j->sc);
+    mg_vcycle(j);
+    hipLaunchKernelGGL(k_mg_red<RED_RESTART>, dim3(1), dim3(KB), 0, j->st, j->part, j->nb0, nullptr, j->sc);
+    hipLaunchKernelGGL(k_mg_update_p, dim3(nblk(n, 256)), dim3(256), 0, j->st, j->p, j->z, n, j->sc);
+    GRMP_HIP(j->ctx, hipGetLastError());
+    return GRMP_OK;
+}
+
+int mg_job_status_async(MgJob *j, MgStatus *host)
+{
+    static_assert(sizeof(MgStatus) == 24, "MgStatus layout");
+    MgStatus *dst = host;
+    // relres, then it / done / status as laid out in MgScal
+    GRMP_HIP(j->ctx, hipMemcpyAsync(&dst->relres, &j->sc->relres, 8, hipMemcpyDeviceToHost, j->st));
+    GRMP_HIP(j->ctx, hipMemcpyAsync(&dst->it, &j->sc->it, 12, hipMemcpyDeviceToHost, j->st));
+    GRMP_HIP(j->ctx, hipEventRecord(j->ev, j->st));
+    return GRMP_OK;
+}
+
+bool mg_job_status_ready(MgJob *j) { return hipEventQuery(j->ev) != hipErrorNotReady; }
+
+int mg_job_wait(MgJob *j)
+{
+    GRMP_HIP(j->ctx, hipEventSynchronize(j->ev));
+    return GRMP_OK;
+}
+
+// profile totals (call after the job's stream has drained)
+void mg_job_profile(MgJob *j, double *seconds, double *bytes, int64_t *launches)
+{
+    *seconds = *bytes = 0.0;
+    *launches = 0;
+    if (!j->profile) return;
+    // OP_RESID launches alternate over levels 0 .. nlev-2 in V-cycle order
+    const int per = j->nlev - 1;
+    for (size_t k = 0; k + 1 < j->pev.size(); k += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, j->pev[k], j->pev[k + 1]) != hipSuccess) continue;
+        const MgLevel &L = j->lv[per > 0 ? (int)((k / 2) % per) : 0];
+        // values + columns, row pointers, x gathered once, b read, r written, fixed mask read
+        *bytes += (double)L.nnz * 12.0 + 4.0 * (L.n + 1) + 8.0 * L.n * 3 + 1.0 * L.n;
+        *seconds += ms * 1e-3;
+        *launches += 1;
+    }
+}
+
+void mg_job_destroy(MgJob *j)
+{
+    if (!j) return;
+    for (hipEvent_t e : j->pev) (void)hipEventDestroy(e);
+    if (j->ev) (void)hipEventDestroy(j->ev);
+    if (j->buf) (void)hipFreeAsync(j->buf, j->st);
+    delete j;
+}

@@ -17,6 +17,10 @@
 //     hipGraph.
 // Plain mode (the baseline): requests are served one at a time, each built
 // from scratch and solved by the host-driven FP64 PCG.
+// Multigrid mode: up to mg_slots requests at once, each on its own stream,
+// solved in FP64 by CG preconditioned with a geometric V-cycle whose levels
+// reuse the resident patterns of the coarser shapes; the host enqueues a few
+// iterations per request and polls a status copy between them.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -93,6 +97,10 @@ struct Job {
     int slot = -1;
     int64_t footprint = 0;
     int64_t n = 0;
+    // multigrid mode
+    MgJob *mg = nullptr;
+    std::vector<Template *> mtpl;
+    int mslot = -1;
 };
 
 struct grmp_server {
@@ -118,6 +126,11 @@ struct grmp_server {
     std::string fatal_msg;
     int64_t ntemplates = 0;
     double template_s = 0.0;
+    // multigrid mode: one context (stream) and one pinned status per slot
+    std::vector<grmp_ctx *> mg_ctx;
+    MgStatus *mg_host = nullptr;
+    double prof_s = 0.0, prof_bytes = 0.0;
+    int64_t prof_launches = 0;
 };
 
 static double now(const grmp_server *s) { return wall_seconds() - s->t0; }
@@ -439,6 +452,194 @@ static void served_loop(grmp_server *S)
     }
 }
 
+// ---------------------------------------------------------- multigrid loop
+static const int kMgChunk = 4;     // PCG iterations enqueued between status polls
+static const int kMgMaxIt = 500;
+
+static void mg_release(grmp_server *S, Job *j)
+{
+    mg_job_destroy(j->mg);
+    j->mg = nullptr;
+    for (Template *t : j->mtpl) t->refs--;
+    j->mtpl.clear();
+    S->mem_used -= j->footprint;
+}
+
+// Levels: the request's shape, P1 on the same square when it is P2, then P1
+// on ceil(N/2), ceil(N/4), ... down to N <= 4.
+static int mg_start(grmp_server *S, Job *j, int slot)
+{
+    grmp_ctx *c = S->mg_ctx[slot];
+    hipStream_t st = c->stream;
+    j->res.t_admit = now(S);
+    j->mslot = slot;
+    const grmp_request &r = j->req;
+    const int ncomp = r.kind == GRMP_REQ_ELASTICITY ? 2 : 1;
+    const int fe0 = r.kind == GRMP_REQ_ELASTICITY ? GRMP_FE_H1P1 : r.fe;
+    std::vector<std::pair<int, int>> shape;  // (fe, N)
+    shape.push_back({fe0, r.N});
+    if (fe0 == GRMP_FE_H1P2) shape.push_back({GRMP_FE_H1P1, r.N});
+    for (int N = r.N; N > 4;) {
+        N = (N + 1) / 2;
+        shape.push_back({GRMP_FE_H1P1, N});
+    }
+    std::vector<MgLevelIn> in(shape.size());
+    const int64_t built = S->ntemplates;
+    for (size_t l = 0; l < shape.size(); ++l) {
+        grmp_request rl = r;
+        rl.fe = shape[l].first;
+        rl.N = shape[l].second;
+        Template *t = nullptr;
+        GRMP_TRY(template_get(S, rl, &t));
+        t->refs++;
+        j->mtpl.push_back(t);
+        in[l] = MgLevelIn{t->A, t->space, rl.N, shape[l].first, ncomp};
+    }
+    // patterns built just now were made on the server stream
+    if (S->ntemplates != built) GRMP_HIP(c, hipStreamSynchronize(S->sctx->stream));
+    GRMP_TRY(mg_job_create(c, st, in.data(), (int)in.size(), S->cfg.profile != 0, &j->mg));
+    Template *t0 = j->mtpl[0];
+    j->n = t0->A->n;
+    grmp_vector kv{c, t0->mesh->ncells, mg_job_kappa0(j->mg)};
+    GRMP_TRY(grmp_vector_lognormal(&kv, r.seed, 0.0, r.sigma));
+    double *rhs = mg_job_rhs(j->mg);
+    GRMP_HIP(c, hipMemsetAsync(rhs, 0, j->n * 8, st));
+    GRMP_TRY(linearform_launch(c, t0->space, &t0->lf, rhs, st));
+    GRMP_TRY(zero_fixed_launch(t0->A, rhs, st));
+    GRMP_TRY(mg_job_setup(j->mg, r.rtol > 0 ? r.rtol : 1e-10, kMgMaxIt));
+    GRMP_TRY(mg_job_iterate(j->mg, kMgChunk));
+    GRMP_TRY(mg_job_status_async(j->mg, &S->mg_host[slot]));
+    j->state = 2;
+    return GRMP_OK;
+}
+
+static void mg_finish(grmp_server *S, Job *j)
+{
+    const MgStatus h = S->mg_host[j->mslot];
+    grmp_ctx *c = S->mg_ctx[j->mslot];
+    j->res.status = h.status;
+    j->res.iterations = h.it;
+    j->res.outer = 1;
+    j->res.relres = h.relres;
+    j->res.fp64_fallback = 0;
+    j->res.ndofs = j->n;
+    if (j->req.x_out)
+        (void)hipMemcpyAsync(j->req.x_out, mg_job_solution(j->mg), j->n * 8, hipMemcpyDeviceToHost, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    double sec = 0.0, bytes = 0.0;
+    int64_t nl = 0;
+    mg_job_profile(j->mg, &sec, &bytes, &nl);
+    std::lock_guard<std::mutex> g(S->mu);
+    S->prof_s += sec;
+    S->prof_bytes += bytes;
+    S->prof_launches += nl;
+    mg_release(S, j);
+    j->res.t_done = now(S);
+}
+
+static void mg_fatal(grmp_server *S, std::vector<Job *> &act, int rc)
+{
+    for (grmp_ctx *c : S->mg_ctx) (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(S->sctx->stream);
+    std::lock_guard<std::mutex> g(S->mu);
+    S->fatal = rc;
+    S->fatal_msg = S->sctx->err;
+    for (Job *&j : act) {
+        if (!j) continue;
+        mg_release(S, j);
+        j->res.status = rc;
+        j->res.t_done = now(S);
+        j->state = 3;
+        j = nullptr;
+    }
+    for (Job *j : S->queue) {
+        j->res.status = rc;
+        j->res.t_done = now(S);
+        j->state = 3;
+    }
+    S->queue.clear();
+    S->cv_done.notify_all();
+}
+
+static void mg_loop(grmp_server *S)
+{
+    const int M = (int)S->mg_ctx.size();
+    std::vector<Job *> act(M, nullptr);
+    for (;;) {
+        int nact = 0;
+        for (Job *j : act) nact += j != nullptr;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            if (nact == 0 && S->queue.empty()) {
+                if (S->stop) return;
+                S->cv.wait(lk, [&] { return S->stop || !S->queue.empty(); });
+                continue;
+            }
+        }
+        // admit into free slots, in FIFO order, within the memory budget
+        for (int s = 0; s < M; ++s) {
+            if (act[s]) continue;
+            Job *j = nullptr;
+            {
+                std::lock_guard<std::mutex> g(S->mu);
+                if (S->queue.empty()) break;
+                Job *f = S->queue.front();
+                if (S->mem_used > 0 && S->mem_used + f->footprint > S->cfg.memory_budget) break;
+                S->queue.pop_front();
+                S->mem_used += f->footprint;
+                j = f;
+            }
+            const int rc = mg_start(S, j, s);
+            if (rc == GRMP_ERR_HIP) {
+                act[s] = j;
+                mg_fatal(S, act, rc);
+                return;
+            }
+            if (rc) {
+                if (j->mg) (void)hipStreamSynchronize(S->mg_ctx[s]->stream);
+                std::lock_guard<std::mutex> g(S->mu);
+                mg_release(S, j);
+                j->res.status = rc;
+                j->res.t_done = now(S);
+                j->state = 3;
+                S->cv_done.notify_all();
+                continue;
+            }
+            act[s] = j;
+        }
+        // finished chunks: enqueue the next one or hand out the result
+        bool progress = false;
+        for (int s = 0; s < M; ++s) {
+            Job *j = act[s];
+            if (!j || !mg_job_status_ready(j->mg)) continue;
+            progress = true;
+            if (S->mg_host[s].done != 1) {
+                int rc = S->mg_host[s].done == 2 ? mg_job_verify(j->mg) : GRMP_OK;
+                if (!rc) rc = mg_job_iterate(j->mg, kMgChunk);
+                if (!rc) rc = mg_job_status_async(j->mg, &S->mg_host[s]);
+                if (rc) {
+                    mg_fatal(S, act, rc);
+                    return;
+                }
+                continue;
+            }
+            mg_finish(S, j);
+            std::lock_guard<std::mutex> g(S->mu);
+            j->state = 3;
+            act[s] = nullptr;
+            S->cv_done.notify_all();
+        }
+        if (!progress)
+            for (Job *j : act)
+                if (j) {
+                    (void)mg_job_wait(j->mg);
+                    break;
+                }
+        std::lock_guard<std::mutex> g(S->mu);
+        templates_trim(S, S->cfg.memory_budget / 2);
+    }
+}
+
 // ------------------------------------------------------------- plain loop
 static int plain_one(grmp_server *S, Job *j)
 {
@@ -506,6 +707,7 @@ extern "C" int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *cfg, 
     S->cfg.profile = 0;
     S->cfg.tile_rows = (int)kTileRows;
     S->cfg.fp64_first_sweep = 0;
+    S->cfg.mg_slots = 8;
     if (cfg) {
         if (cfg->mode) S->cfg.mode = cfg->mode;
         if (cfg->memory_budget > 0) S->cfg.memory_budget = cfg->memory_budget;
@@ -514,6 +716,12 @@ extern "C" int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *cfg, 
         S->cfg.profile = cfg->profile;
         if (cfg->tile_rows > 0) S->cfg.tile_rows = std::max(64, std::min(cfg->tile_rows, 8192));
         if (cfg->fp64_first_sweep >= 0) S->cfg.fp64_first_sweep = cfg->fp64_first_sweep;
+        if (cfg->mg_slots > 0) S->cfg.mg_slots = std::min(cfg->mg_slots, 64);
+    }
+    if (S->cfg.mode < 1 || S->cfg.mode > 3) {
+        const int mode = S->cfg.mode;
+        delete S;
+        return grmp_fail(ctx, GRMP_ERR_ARG, "grmp_server_create: mode %d", mode);
     }
     int rc = grmp_ctx_create(ctx->device, &S->sctx);
     if (rc) { delete S; return grmp_fail(ctx, rc, "grmp_server_create: no server context"); }
@@ -533,10 +741,29 @@ extern "C" int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *cfg, 
             return grmp_fail(ctx, rc, "grmp_server_create: engine init failed");
         }
     }
+    if (S->cfg.mode == 3) {
+        for (int k = 0; k < S->cfg.mg_slots && !rc; ++k) {
+            grmp_ctx *c = nullptr;
+            rc = grmp_ctx_create(ctx->device, &c);
+            if (!rc) S->mg_ctx.push_back(c);
+        }
+        if (!rc && hipHostMalloc((void **)&S->mg_host, sizeof(MgStatus) * S->cfg.mg_slots, hipHostMallocDefault) !=
+                       hipSuccess)
+            rc = GRMP_ERR_HIP;
+        if (rc) {
+            for (grmp_ctx *c : S->mg_ctx) grmp_ctx_destroy(c);
+            (void)hipStreamDestroy(S->st_solve);
+            grmp_ctx_destroy(S->sctx);
+            delete S;
+            return grmp_fail(ctx, rc, "grmp_server_create: multigrid streams");
+        }
+        memset(S->mg_host, 0, sizeof(MgStatus) * S->cfg.mg_slots);
+    }
     S->t0 = wall_seconds();
     S->worker = std::thread([S] {
         (void)hipSetDevice(S->sctx->device);
         if (S->cfg.mode == 1) served_loop(S);
+        else if (S->cfg.mode == 3) mg_loop(S);
         else plain_loop(S);
     });
     *out = S;
@@ -552,8 +779,9 @@ extern "C" int grmp_server_submit(grmp_server *S, const grmp_request *req, int64
     Job *j = new Job();
     j->req = *req;
     memset(&j->res, 0, sizeof(j->res));
-    j->footprint = grmp_request_footprint(req);
-    if (S->cfg.mode == 1 && j->footprint > S->cfg.memory_budget) {
+    // multigrid jobs hold the hierarchy's values and vectors besides the fine level's
+    j->footprint = grmp_request_footprint(req) * (S->cfg.mode == 3 ? 2 : 1);
+    if (S->cfg.mode != 2 && j->footprint > S->cfg.memory_budget) {
         delete j;
         return grmp_fail(S->user, GRMP_ERR_ADMISSION, "grmp_server_submit: footprint %lld B exceeds the budget %lld B",
                          (long long)grmp_request_footprint(req), (long long)S->cfg.memory_budget);
@@ -605,6 +833,12 @@ extern "C" int grmp_server_stats(grmp_server *S, grmp_kernel_stats *out)
     std::lock_guard<std::mutex> g(S->mu);
     out->templates = S->ntemplates;
     out->template_seconds = S->template_s;
+    if (S->cfg.mode == 3) {
+        out->spmv_seconds = S->prof_s;
+        out->spmv_launches = S->prof_launches;
+        out->spmv_bytes = S->prof_bytes;
+        return GRMP_OK;
+    }
     if (!S->eng) return GRMP_OK;
     return S->eng->stats(&out->spmv_seconds, &out->spmv_launches, &out->spmv_bytes, &out->steps);
 }
@@ -622,6 +856,8 @@ extern "C" int grmp_server_destroy(grmp_server *S)
     delete S->eng;
     for (auto &kv : S->templates) template_free(kv.second);
     for (auto &kv : S->jobs) delete kv.second;
+    for (grmp_ctx *c : S->mg_ctx) grmp_ctx_destroy(c);
+    if (S->mg_host) (void)hipHostFree(S->mg_host);
     (void)hipStreamDestroy(S->st_solve);
     grmp_ctx_destroy(S->sctx);
     delete S;

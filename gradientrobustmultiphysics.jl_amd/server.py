@@ -34,11 +34,19 @@ def footprint(kind, fe, N):
 
 
 class Server:
+    MODES = {"served": 1, "plain": 2, "mg": 3}
+
     def __init__(self, mode="served", memory_budget=64 << 30, max_active=256, graph=True, profile=False, ctx=None,
-                 tile_rows=0, fp64_first_sweep=0):
+                 tile_rows=0, fp64_first_sweep=0, mg_slots=0):
+        """mode "served": batched Jacobi-PCG engine with FP32 sweeps; "plain":
+        one request at a time, FP64 Jacobi-PCG; "mg": FP64 CG preconditioned
+        by a geometric multigrid V-cycle, ``mg_slots`` requests at once."""
         self.ctx = ctx or default_context()
         cfg = _abi.ServerConfig()
-        cfg.mode = 1 if mode == "served" else 2
+        if mode not in self.MODES:
+            raise ValueError(f"unknown server mode {mode!r}")
+        cfg.mode = self.MODES[mode]
+        cfg.mg_slots = int(mg_slots)
         cfg.memory_budget = int(memory_budget)
         cfg.max_active = int(max_active)
         cfg.graph = 1 if graph else 0

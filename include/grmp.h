@@ -282,7 +282,8 @@ typedef struct grmp_result {
 } grmp_result;
 
 typedef struct grmp_server_config {
-    int mode;                 /* 1 = batched + FP32_IR (served), 2 = plain one-at-a-time FP64 */
+    int mode;                 /* 1 = batched + FP32_IR (served), 2 = plain one-at-a-time FP64,
+                                 3 = multigrid-preconditioned CG in FP64, mg_slots requests at once */
     int64_t memory_budget;    /* bytes of HBM for resident problems */
     int max_active;           /* problems in flight */
     int graph;                /* capture the Krylov step as a hipGraph */
@@ -290,12 +291,17 @@ typedef struct grmp_server_config {
     int tile_rows;            /* engine rows per work tile (0 = 2048; 64 .. 8192) */
     int fp64_first_sweep;     /* > 0: a request whose first FP32 sweep takes more iterations
                                  continues in FP64 (0 = only on stalls) */
+    int mg_slots;             /* mode 3: requests solved concurrently, one HIP stream each (0 = 8) */
+    int pad_;
 } grmp_server_config;
 
 typedef struct grmp_kernel_stats {
-    double spmv_seconds;      /* summed durations of the engine SpMV launches (HIP events) */
+    double spmv_seconds;      /* summed durations of the timed SpMV launches (HIP events): the engine
+                                 SpMV (modes 1), the multigrid residual kernel (mode 3) */
     int64_t spmv_launches;
-    double spmv_bytes;        /* algorithmic bytes: nnz*(4+4) + 4(n+1) + 4n (x) + 4n (y) per row tile */
+    double spmv_bytes;        /* algorithmic bytes of those launches (engine: nnz*(4+4) + 4(n+1) + 4n (x)
+                                 + 4n (y) per row tile; multigrid residual: nnz*(8+4) + 4(n+1) + 8n (x)
+                                 + 8n (b) + 8n (r) + n (fixed mask)) */
     int64_t steps;            /* engine steps run */
     int64_t templates;        /* problem shapes built (mesh + dofmap + pattern) */
     double template_seconds;  /* host-observed time spent building them */
