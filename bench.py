@@ -50,10 +50,14 @@ def make_stream(nreq, rate, seed=2024):
 
 
 def serve(grmp, mode, stream, rtol, keep=(), profile=False, respect_arrivals=True, tile_rows=0, precision="fp32_ir",
-          fp64_first_sweep=0, max_active=512):
+          fp64_first_sweep=0, max_active=512, mg_slots=0, mg_threads=0, mg_block_rows=0, mg_graph=False,
+          mg_coarse_rows=0):
     """Submits the stream at its arrival times and waits for every result."""
+    graph = mg_graph if mode == "mg" else True
     S = grmp.Server(mode, profile=profile, memory_budget=96 << 30, max_active=max_active, tile_rows=tile_rows,
-                    fp64_first_sweep=fp64_first_sweep)
+                    graph=graph,
+                    fp64_first_sweep=fp64_first_sweep, mg_slots=mg_slots, mg_threads=mg_threads,
+                    mg_block_rows=mg_block_rows, mg_coarse_rows=mg_coarse_rows)
     keep = set(keep)
     tickets = []
     t0 = time.perf_counter()
@@ -304,7 +308,18 @@ def main():
     ap.add_argument("--fp64-first-sweep", type=int, default=0,
                     help="served requests whose first FP32 sweep takes more iterations continue in FP64 (0: off)")
     ap.add_argument("--precision", default="fp32_ir", choices=["fp32_ir", "fp64"],
-                    help="served requests: FP32 sweeps with FP64 refinement and fallback, or FP64 CG in the engine")
+                    help="engine requests: FP32 sweeps with FP64 refinement and fallback, or FP64 CG in the engine")
+    ap.add_argument("--solver", default="mg", choices=["mg", "engine"],
+                    help="served stream: FP64 CG with a multigrid V-cycle per request (mg), or the batched "
+                         "Jacobi-PCG engine (engine)")
+    ap.add_argument("--mg-slots", type=int, default=0, help="mg: requests solved concurrently (0 = library default)")
+    ap.add_argument("--mg-threads", type=int, default=0, help="mg: host threads launching (0 = library default)")
+    ap.add_argument("--mg-block-rows", type=int, default=0,
+                    help="mg: levels up to this many rows run in one workgroup (0 = library default)")
+    ap.add_argument("--mg-coarse-rows", type=int, default=0,
+                    help="mg: levels of larger requests up to this many rows run in one workgroup (0 = default)")
+    ap.add_argument("--mg-graph", action="store_true",
+                    help="mg: replay captured chunks of CG iterations (profiles every 4th request only)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the served stream shards by request "
@@ -317,10 +332,11 @@ def main():
         return
     stream = make_stream(args.requests, args.rate)
 
+    mode = "mg" if args.solver == "mg" else "served"
     # warmup: the first 256 requests of the stream, untimed
     for w in range(args.warmup):
         log(f"[bench] warmup {w}")
-        serve(grmp, "served", stream[:256], args.rtol, profile=True)
+        serve(grmp, mode, stream[:256], args.rtol, profile=True, mg_slots=args.mg_slots)
 
     # verification sample: spread over the size range, largest included
     order = np.argsort([r["N"] * (2 if r["fe"] == 2 or r["kind"] == 2 else 1) for r in stream])
@@ -330,9 +346,11 @@ def main():
     for s in range(args.steps):
         log(f"[bench] step {s}: {len(stream)} requests at {args.rate:g}/s")
         GE.load().default_context().sync()
-        runs.append(serve(grmp, "served", stream, args.rtol, keep=vidx, profile=True, tile_rows=args.tile_rows,
+        runs.append(serve(grmp, mode, stream, args.rtol, keep=vidx, profile=True, tile_rows=args.tile_rows,
                           precision=args.precision, fp64_first_sweep=args.fp64_first_sweep,
-                          max_active=args.max_active))
+                          max_active=args.max_active, mg_slots=args.mg_slots, mg_threads=args.mg_threads,
+                          mg_block_rows=args.mg_block_rows, mg_graph=args.mg_graph,
+                          mg_coarse_rows=args.mg_coarse_rows))
         GE.load().default_context().sync()
     run = runs[-1]
     walls = [r["wall"] for r in runs]
@@ -362,11 +380,13 @@ def main():
     st = run["stats"]
     achieved = st["spmv_bytes"] / st["spmv_seconds"] / 1e9 if st["spmv_seconds"] > 0 else 0.0
     avg_launch = st["spmv_seconds"] / max(st["spmv_launches"], 1)
-    roof = {"bound": "hbm", "kernel": "k_eng_spmv", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+    roof = {"bound": "hbm", "kernel": "k_mg_resid_fine" if mode == "mg" else "k_eng_spmv",
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "avg_launch_us": round(avg_launch * 1e6, 2), "launches": int(st["spmv_launches"]),
-            "traffic_over_algorithmic": 1.20,  # FETCH_SIZE + WRITE_SIZE passes, profiles/engine_spmv_hbm_traffic.md
             "bytes_per_launch": round(st["spmv_bytes"] / max(st["spmv_launches"], 1), 1)}
+    if mode == "served":
+        roof["traffic_over_algorithmic"] = 1.20  # FETCH_SIZE + WRITE_SIZE passes, profiles/engine_spmv_hbm_traffic.md
 
     cpu = None
     if not args.no_cpu:
@@ -385,7 +405,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 inner / f64 refined",
+        "dtype": "f64" if mode == "mg" else "f32 inner / f64 refined",
         "data": "synthetic",
         "config": {"workload": f"served stream: {args.requests} requests, Poisson arrivals {args.rate:g}/s, "
                                "Poisson(P1|P2)/elasticity(P1) 50:50, N log-uniform [16,768], log-normal sigma=1",
@@ -400,6 +420,8 @@ def main():
                   "p50_service_ms": round(1e3 * float(np.percentile(plain["service"], 50)), 2),
                   "p99_service_ms": round(1e3 * float(np.percentile(plain["service"], 99)), 2)},
         "served_vs_plain": round(value / plain_value, 3),
+        "solver": "FP64 CG + geometric multigrid V-cycle, one stream per request in flight" if mode == "mg"
+                  else "batched Jacobi-PCG engine, FP32 sweeps in FP64 refinement",
         "engine": {"steps": int(st["steps"]), "templates": int(st["templates"]),
                    "template_seconds": round(st["template_seconds"], 3),
                    "mean_outer_sweeps": round(float(run["outer"].mean()), 2),

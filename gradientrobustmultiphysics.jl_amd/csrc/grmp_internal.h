@@ -205,7 +205,11 @@ struct MgStatus {            // host copy of the job's scalars
 };
 struct MgJob;
 int64_t mg_job_bytes(const MgLevelIn *in, int nlev);
-int mg_job_create(grmp_ctx *ctx, hipStream_t st, const MgLevelIn *in, int nlev, bool profile, MgJob **out);
+// block_rows: a request with at most this many rows runs whole inside one
+// workgroup; coarse_rows: otherwise its levels with at most this many rows do;
+// graph: capture the grid-path chunk of iterations once and replay it
+int mg_job_create(grmp_ctx *ctx, hipStream_t st, const MgLevelIn *in, int nlev, bool profile, int block_rows,
+                  int coarse_rows, bool graph, MgJob **out);
 double *mg_job_kappa0(MgJob *j);     // level-0 cell coefficient, filled by the caller
 double *mg_job_rhs(MgJob *j);        // level-0 right-hand side, filled by the caller
 const double *mg_job_solution(const MgJob *j);
@@ -215,6 +219,7 @@ int mg_job_verify(MgJob *j);         // after done == 2: check the true residual
 int mg_job_status_async(MgJob *j, MgStatus *host);
 bool mg_job_status_ready(MgJob *j);
 int mg_job_wait(MgJob *j);
+void mg_job_collect(MgJob *j);       // after a status event: add the finished profiled launches
 void mg_job_profile(MgJob *j, double *seconds, double *bytes, int64_t *launches);
 void mg_job_destroy(MgJob *j);
 // timing

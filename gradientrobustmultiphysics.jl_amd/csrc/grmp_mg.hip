@@ -27,6 +27,8 @@
 #include <cstring>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "grmp_internal.h"
 #include "grmp_sparse.h"
 #include "grmp_square.h"
@@ -77,42 +79,64 @@ struct MgRows {
 //   PCG_Q   o1 = A v                                        [+ partials of v . o1]
 // Fixed rows get 0 (no residual, no correction).
 template <int OP>
-__global__ __launch_bounds__(KB) void k_mg_rows(MgRows a)
+__device__ __forceinline__ double mg_epilogue(int64_t i, double s, bool fx, const double *__restrict__ v,
+                                              const double *__restrict__ b, const double *__restrict__ dinv,
+                                              const double *__restrict__ d, const double *__restrict__ lam,
+                                              double *__restrict__ o1, double *__restrict__ o2)
 {
-    if (a.sc->done) return;
+    if (OP == OP_RESID) {
+        o1[i] = fx ? 0.0 : b[i] - s;
+    } else if (OP == OP_CHEB_PRE) {
+        const double l = *lam;
+        o1[i] = fx ? 0.0 : (1.0 + CHEB_C1) * v[i] + CHEB_C2(l) * dinv[i] * (b[i] - s);
+    } else if (OP == OP_CHEB_POST1) {
+        const double l = *lam;
+        const double dd = fx ? 0.0 : dinv[i] * (b[i] - s) * CHEB_ITHETA(l);
+        o1[i] = dd;
+        o2[i] = v[i] + dd;
+    } else if (OP == OP_CHEB_POST2) {
+        const double l = *lam;
+        const double z = fx ? 0.0 : v[i] + CHEB_C1 * d[i] + CHEB_C2(l) * dinv[i] * (b[i] - s);
+        o1[i] = z;
+        return b[i] * z;
+    } else {
+        const double q = fx ? 0.0 : s;
+        o1[i] = q;
+        return v[i] * q;
+    }
+    return 0.0;
+}
+
+template <int OP>
+__device__ __forceinline__ void mg_rows_body(const MgRows &a)
+{
     __shared__ double lds[SPMV_CAP];
     const int64_t r0 = (int64_t)blockIdx.x * KB;
     const int64_t r1 = r0 + KB < a.n ? r0 + KB : a.n;
     const double s = csr_rows_lds<double, double, double>(a.rp, a.ci, a.val, a.v, r0, r1, lds);
     const int64_t i = r0 + threadIdx.x;
     double pa = 0.0, pb = 0.0;
-    if (i < r1) {
-        const bool fx = a.fixed[i] != 0;
-        if (OP == OP_RESID) {
-            a.o1[i] = fx ? 0.0 : a.b[i] - s;
-        } else if (OP == OP_CHEB_PRE) {
-            const double l = *a.lam;
-            a.o1[i] = fx ? 0.0 : (1.0 + CHEB_C1) * a.v[i] + CHEB_C2(l) * a.dinv[i] * (a.b[i] - s);
-        } else if (OP == OP_CHEB_POST1) {
-            const double l = *a.lam;
-            const double d = fx ? 0.0 : a.dinv[i] * (a.b[i] - s) * CHEB_ITHETA(l);
-            a.o1[i] = d;
-            a.o2[i] = a.v[i] + d;
-        } else if (OP == OP_CHEB_POST2) {
-            const double l = *a.lam;
-            const double z = fx ? 0.0 : a.v[i] + CHEB_C1 * a.d[i] + CHEB_C2(l) * a.dinv[i] * (a.b[i] - s);
-            a.o1[i] = z;
-            pa = a.b[i] * z;
-        } else {
-            const double q = fx ? 0.0 : s;
-            a.o1[i] = q;
-            pa = a.v[i] * q;
-        }
-    }
+    if (i < r1) pa = mg_epilogue<OP>(i, s, a.fixed[i] != 0, a.v, a.b, a.dinv, a.d, a.lam, a.o1, a.o2);
     if ((OP == OP_CHEB_POST2 || OP == OP_PCG_Q) && a.part) {
         block_sum2(pa, pb);
         if (threadIdx.x == 0) a.part[blockIdx.x] = pa;
     }
+}
+
+template <int OP>
+__global__ __launch_bounds__(KB) void k_mg_rows(MgRows a)
+{
+    if (a.sc->done) return;
+    mg_rows_body<OP>(a);
+}
+
+// the residual on levels of at least MG_FINE_ROWS rows: the same rows as
+// k_mg_rows<OP_RESID> under its own name, the kernel the profiled runs time
+#define MG_FINE_ROWS (1 << 20)
+__global__ __launch_bounds__(KB) void k_mg_resid_fine(MgRows a)
+{
+    if (a.sc->done) return;
+    mg_rows_body<OP_RESID>(a);
 }
 
 // first pre-smoothing step from x0 = 0: d0 = D^-1 b / theta
@@ -122,8 +146,7 @@ __global__ void k_mg_cheb_first(const double *__restrict__ b, const double *__re
 {
     if (sc->done) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    d[i] = fixed[i] ? 0.0 : dinv[i] * b[i] * CHEB_ITHETA(*lam);
+    if (i < n) d[i] = fixed[i] ? 0.0 : dinv[i] * b[i] * CHEB_ITHETA(*lam);
 }
 
 // Gershgorin bound max_i sum_j |A_ij| / A_ii over free rows (a max, so the
@@ -144,99 +167,103 @@ __global__ void k_mg_gersh(const int32_t *__restrict__ rp, const double *__restr
 }
 
 // ------------------------------------------------------------- transfers
-// Fine node (i, j) of the Nf-square in the Nc-triangulation: coarse square
-// (ic, jc), offsets u = un / Nf, v = vn / Nf in [0, 1], and the interpolation
-// weights of the coarse triangle holding it (lower when u >= v).
-struct P1Loc {
-    int64_t a, b, c;  // coarse nodes
-    double wa, wb, wc;
-};
-
-__device__ __forceinline__ P1Loc p1_locate(int64_t Nf, int64_t Nc, int64_t i, int64_t j)
+// Fine node (i, j) of the Nf-square lies in coarse square (ic, jc) of the
+// Nc-triangulation at offsets u = un / Nf, v = vn / Nf in [0, 1] (integers:
+// i Nc = ic Nf + un), in its lower triangle when u >= v.  The interpolation
+// weight of coarse node (ic + dx, jc + dy) there is w / Nf with
+//   (0,0): Nf - max(un, vn)    (1,0): max(un - vn, 0)
+//   (1,1): min(un, vn)         (0,1): max(vn - un, 0)
+// (the barycentric coordinates of both triangles in one formula); prolongation
+// and restriction both evaluate it, so P^T is exactly the transpose of P.
+__device__ __forceinline__ void p1_coord(int Nf, int Nc, int i, int &ic, int &un)
 {
-    int64_t ic = i * Nc / Nf, jc = j * Nc / Nf;
+    ic = i * Nc / Nf;
     if (ic > Nc - 1) ic = Nc - 1;
-    if (jc > Nc - 1) jc = Nc - 1;
-    const int64_t un = i * Nc - ic * Nf, vn = j * Nc - jc * Nf;
-    const double inv = 1.0 / (double)Nf;
-    const int64_t n00 = jc * (Nc + 1) + ic;
-    P1Loc L;
-    if (un >= vn) {  // lower triangle (ic,jc), (ic+1,jc), (ic+1,jc+1)
-        L.a = n00;
-        L.b = n00 + 1;
-        L.c = n00 + Nc + 2;
-        L.wa = (double)(Nf - un) * inv;
-        L.wb = (double)(un - vn) * inv;
-        L.wc = (double)vn * inv;
-    } else {         // upper triangle (ic,jc), (ic+1,jc+1), (ic,jc+1)
-        L.a = n00;
-        L.b = n00 + Nc + 2;
-        L.c = n00 + Nc + 1;
-        L.wa = (double)(Nf - vn) * inv;
-        L.wb = (double)un * inv;
-        L.wc = (double)(vn - un) * inv;
-    }
-    return L;
+    un = i * Nc - ic * Nf;
 }
 
-// xf += P xc on free fine dofs (components blocked by node count)
-__global__ void k_mg_prolong_p1(int64_t Nf, int64_t Nc, int ncomp, const double *__restrict__ xc,
-                                const uint8_t *__restrict__ fixed_f, double *__restrict__ xf, const MgScal *sc)
+__device__ __forceinline__ int p1_weight(int Nf, int dx, int dy, int un, int vn)
 {
-    if (sc->done) return;
-    const int64_t nnf = (Nf + 1) * (Nf + 1), nnc = (Nc + 1) * (Nc + 1);
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nnf * ncomp) return;
-    const int k = (int)(t / nnf);
-    const int64_t node = t - k * nnf;
+    if (dx == 0 && dy == 0) return Nf - (un > vn ? un : vn);
+    if (dx == 1 && dy == 0) return un > vn ? un - vn : 0;
+    if (dx == 1 && dy == 1) return un < vn ? un : vn;
+    return vn > un ? vn - un : 0;  // (0,1)
+}
+
+// xf += P xc on free fine dofs (components blocked by node count); t < nnf * ncomp
+__device__ __forceinline__ void prolong_p1_one(int64_t Nf64, int64_t Nc64, int ncomp, const double *__restrict__ xc,
+                                               const uint8_t *__restrict__ fixed_f, double *__restrict__ xf, int64_t t)
+{
+    const int Nf = (int)Nf64, Nc = (int)Nc64;
+    const int64_t nnf = (int64_t)(Nf + 1) * (Nf + 1), nnc = (int64_t)(Nc + 1) * (Nc + 1);
     if (fixed_f[t]) return;
-    const P1Loc L = p1_locate(Nf, Nc, node % (Nf + 1), node / (Nf + 1));
-    const double *x = xc + k * nnc;
-    xf[t] += L.wa * x[L.a] + L.wb * x[L.b] + L.wc * x[L.c];
+    const int k = (int)(t / nnf);
+    const int node = (int)(t - k * nnf);
+    int ic, un, jc, vn;
+    p1_coord(Nf, Nc, node % (Nf + 1), ic, un);
+    p1_coord(Nf, Nc, node / (Nf + 1), jc, vn);
+    const double inv = 1.0 / (double)Nf;
+    const double *x = xc + k * nnc + (int64_t)jc * (Nc + 1) + ic;
+    double s = (double)p1_weight(Nf, 0, 0, un, vn) * inv * x[0];
+    s += (double)p1_weight(Nf, 1, 0, un, vn) * inv * x[1];
+    s += (double)p1_weight(Nf, 1, 1, un, vn) * inv * x[Nc + 2];
+    s += (double)p1_weight(Nf, 0, 1, un, vn) * inv * x[Nc + 1];
+    xf[t] += s;
 }
 
-// bc = P^T rf on free coarse dofs, 0 on fixed ones.  Coarse node (I, J)
-// gathers the fine nodes whose coarse triangle has it as a vertex; they lie in
-// the fine index box below, visited row by row.
-__global__ void k_mg_restrict_p1(int64_t Nf, int64_t Nc, int ncomp, const double *__restrict__ rf,
-                                 const uint8_t *__restrict__ fixed_c, double *__restrict__ bc, const MgScal *sc)
+// bc = P^T rf on free coarse dofs, 0 on fixed ones; t < nnc * ncomp.  Coarse
+// node (I, J) gathers the fine nodes of the index box around it whose coarse
+// square touches it (at most 7 x 7 for Nc = ceil(Nf / 2)), row by row.
+__device__ __forceinline__ void restrict_p1_one(int64_t Nf64, int64_t Nc64, int ncomp, const double *__restrict__ rf,
+                                                const uint8_t *__restrict__ fixed_c, double *__restrict__ bc,
+                                                int64_t t)
 {
-    if (sc->done) return;
-    const int64_t nnf = (Nf + 1) * (Nf + 1), nnc = (Nc + 1) * (Nc + 1);
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nnc * ncomp) return;
+    const int Nf = (int)Nf64, Nc = (int)Nc64;
+    const int64_t nnf = (int64_t)(Nf + 1) * (Nf + 1), nnc = (int64_t)(Nc + 1) * (Nc + 1);
     if (fixed_c[t]) {
         bc[t] = 0.0;
         return;
     }
     const int k = (int)(t / nnc);
-    const int64_t node = t - k * nnc;
-    const int64_t I = node % (Nc + 1), J = node / (Nc + 1);
-    const int64_t ilo = I >= 1 ? (I - 1) * Nf / Nc : 0, jlo = J >= 1 ? (J - 1) * Nf / Nc : 0;
-    int64_t ihi = ((I + 1) * Nf + Nc - 1) / Nc, jhi = ((J + 1) * Nf + Nc - 1) / Nc;
+    const int node = (int)(t - k * nnc);
+    const int I = node % (Nc + 1), J = node / (Nc + 1);
+    const int ilo = I >= 1 ? (I - 1) * Nf / Nc : 0, jlo = J >= 1 ? (J - 1) * Nf / Nc : 0;
+    int ihi = ((I + 1) * Nf + Nc - 1) / Nc, jhi = ((J + 1) * Nf + Nc - 1) / Nc;
     if (ihi > Nf) ihi = Nf;
     if (jhi > Nf) jhi = Nf;
+    int dxs[8], uns[8];
+    const int ni = ihi - ilo + 1 < 8 ? ihi - ilo + 1 : 8;
+    for (int a = 0; a < ni; ++a) {
+        int ic;
+        p1_coord(Nf, Nc, ilo + a, ic, uns[a]);
+        dxs[a] = I - ic;
+    }
+    const double inv = 1.0 / (double)Nf;
     const double *r = rf + k * nnf;
     double s = 0.0;
-    for (int64_t j = jlo; j <= jhi; ++j)
-        for (int64_t i = ilo; i <= ihi; ++i) {
-            const P1Loc L = p1_locate(Nf, Nc, i, j);
-            const double w = L.a == node ? L.wa : L.b == node ? L.wb : L.c == node ? L.wc : 0.0;
-            if (w != 0.0) s += w * r[j * (Nf + 1) + i];
+    for (int j = jlo; j <= jhi; ++j) {
+        int jc, vn;
+        p1_coord(Nf, Nc, j, jc, vn);
+        const int dy = J - jc;
+        if (dy < 0 || dy > 1) continue;
+        for (int a = 0; a < ni; ++a) {
+            const int dx = dxs[a];
+            if (dx < 0 || dx > 1) continue;
+            const int w = p1_weight(Nf, dx, dy, uns[a], vn);
+            if (w != 0) s += (double)w * inv * r[(int64_t)j * (Nf + 1) + ilo + a];
         }
+    }
     bc[t] = s;
 }
 
-// P1 -> P2 on the same N-square: node dofs copy, face dofs take the mean of
-// their end nodes.  Each face is handled by the one cell whose CellFaceSign is
-// +1 for it (its first appearance).
-__global__ void k_mg_prolong_p2(int64_t N, const int32_t *__restrict__ cells, const int8_t *__restrict__ fsign,
-                                const double *__restrict__ x1, const uint8_t *__restrict__ fixed2,
-                                double *__restrict__ x2, const MgScal *sc)
+// P1 -> P2 on the same N-square, added into x2: node dofs copy (t < nn),
+// face dofs take the mean of their end nodes (t < nc: each face is handled by
+// the one cell whose CellFaceSign is +1 for it, its first appearance)
+__device__ __forceinline__ void prolong_p2_one(int64_t N, const int32_t *__restrict__ cells,
+                                               const int8_t *__restrict__ fsign, const double *__restrict__ x1,
+                                               const uint8_t *__restrict__ fixed2, double *__restrict__ x2, int64_t t)
 {
-    if (sc->done) return;
     const int64_t nn = (N + 1) * (N + 1), nc = 2 * N * N;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nn && !fixed2[t]) x2[t] += x1[t];
     if (t >= nc) return;
     int64_t f[3];
@@ -250,16 +277,13 @@ __global__ void k_mg_prolong_p2(int64_t N, const int32_t *__restrict__ cells, co
     }
 }
 
-// its transpose: node v gathers r2[v] and half of each incident face's entry
-// (faces: left / right horizontal, lower / upper vertical, lower-left /
+// its transpose (v < nn): node v gathers r2[v] and half of each incident
+// face's entry (left / right horizontal, lower / upper vertical, lower-left /
 // upper-right diagonal)
-__global__ void k_mg_restrict_p2(int64_t N, const double *__restrict__ r2, const uint8_t *__restrict__ fixed1,
-                                 double *__restrict__ b1, const MgScal *sc)
+__device__ __forceinline__ void restrict_p2_one(int64_t N, const double *__restrict__ r2,
+                                                const uint8_t *__restrict__ fixed1, double *__restrict__ b1, int64_t v)
 {
-    if (sc->done) return;
     const int64_t nn = (N + 1) * (N + 1);
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nn) return;
     if (fixed1[v]) {
         b1[v] = 0.0;
         return;
@@ -274,6 +298,39 @@ __global__ void k_mg_restrict_p2(int64_t N, const double *__restrict__ r2, const
     if (i > 0 && j > 0) s += rf[sq_D(N, i - 1, j - 1)];
     if (i < N && j < N) s += rf[sq_D(N, i, j)];
     b1[v] = r2[v] + 0.5 * s;
+}
+
+__global__ void k_mg_prolong_p1(int64_t Nf, int64_t Nc, int ncomp, const double *__restrict__ xc,
+                                const uint8_t *__restrict__ fixed_f, double *__restrict__ xf, const MgScal *sc)
+{
+    if (sc->done) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (Nf + 1) * (Nf + 1) * ncomp) prolong_p1_one(Nf, Nc, ncomp, xc, fixed_f, xf, t);
+}
+
+__global__ void k_mg_restrict_p1(int64_t Nf, int64_t Nc, int ncomp, const double *__restrict__ rf,
+                                 const uint8_t *__restrict__ fixed_c, double *__restrict__ bc, const MgScal *sc)
+{
+    if (sc->done) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (Nc + 1) * (Nc + 1) * ncomp) restrict_p1_one(Nf, Nc, ncomp, rf, fixed_c, bc, t);
+}
+
+__global__ void k_mg_prolong_p2(int64_t N, const int32_t *__restrict__ cells, const int8_t *__restrict__ fsign,
+                                const double *__restrict__ x1, const uint8_t *__restrict__ fixed2,
+                                double *__restrict__ x2, const MgScal *sc)
+{
+    if (sc->done) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    prolong_p2_one(N, cells, fsign, x1, fixed2, x2, t);
+}
+
+__global__ void k_mg_restrict_p2(int64_t N, const double *__restrict__ r2, const uint8_t *__restrict__ fixed1,
+                                 double *__restrict__ b1, const MgScal *sc)
+{
+    if (sc->done) return;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < (N + 1) * (N + 1)) restrict_p2_one(N, r2, fixed1, b1, v);
 }
 
 // coarse coefficient: geometric mean of the fine cells whose centroid lies in
@@ -408,18 +465,10 @@ __global__ void k_mg_update_p(double *__restrict__ p, const double *__restrict__
 
 // one block: the fixed-order sum of nb partials, then the scalar step
 enum MgRed { RED_INIT = 0, RED_ALPHA = 1, RED_CONV = 2, RED_BETA = 3, RED_VERIFY = 4, RED_RESTART = 5 };
+// the scalar step of W on the reduced values a (and b for RED_INIT); one thread
 template <int W>
-__global__ __launch_bounds__(KB) void k_mg_red(const double *__restrict__ part, int nb, const double *__restrict__ part2,
-                                               MgScal *sc)
+__device__ __forceinline__ void mg_scalar(MgScal *sc, double a, double b)
 {
-    if (sc->done) return;
-    double a = 0.0, b = 0.0;
-    for (int k = threadIdx.x; k < nb; k += KB) {
-        a += part[k];
-        if (W == RED_INIT) b += part2[k];
-    }
-    block_sum2(a, b);
-    if (threadIdx.x != 0) return;
     if (W == RED_INIT) {  // a = r.z, b = r.r of the start residual
         sc->rz = a;
         sc->rr0 = b;
@@ -463,10 +512,204 @@ __global__ __launch_bounds__(KB) void k_mg_red(const double *__restrict__ part, 
     }
 }
 
+template <int W>
+__global__ __launch_bounds__(KB) void k_mg_red(const double *__restrict__ part, int nb, const double *__restrict__ part2,
+                                               MgScal *sc)
+{
+    if (sc->done) return;
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < nb; k += KB) {
+        a += part[k];
+        if (W == RED_INIT) b += part2[k];
+    }
+    block_sum2(a, b);
+    if (threadIdx.x == 0) mg_scalar<W>(sc, a, b);
+}
+
 // a check was requested: resume the kernels for it
 __global__ void k_mg_verify_begin(MgScal *sc)
 {
     if (threadIdx.x == 0 && sc->done == 2) sc->done = 0;
+}
+
+// ------------------------------------------------- one-workgroup kernels
+// Levels with at most block_rows rows run inside one workgroup of MG_BT
+// threads, phases separated by workgroup barriers: the coarse end of every
+// V-cycle in one launch, and for requests whose finest level is that small,
+// whole CG iterations in one launch.  Rows are summed in column order by one
+// thread each.
+#define MG_BT 1024
+
+struct MgBLevel {
+    const int32_t *rp, *ci;
+    const double *val, *dinv;
+    const uint8_t *fixed;
+    double *x, *b, *r, *d;
+    const double *lam;
+    int64_t n;
+    int N, fe, ncomp, pad;
+    const int32_t *cells;  // P2 level: its mesh, for the P1 -> P2 prolongation
+    const int8_t *fsign;
+};
+
+struct MgBDesc {
+    int first, nlev, nb0, pad;  // levels first .. nlev-1 run in the workgroup
+    MgBLevel lv[MG_MAX_LEVELS];
+    const double *inv;          // coarsest level's dense inverse
+    double *x, *r, *z, *p, *q;  // level-0 CG vectors
+    const double *rhs;
+    double *part;               // level-0 block partials (nb0 entries)
+};
+
+// sum over the workgroup, returned to every thread
+__device__ double blk_sum(double v)
+{
+    __shared__ double ws[MG_BT / 64];
+    v = wave_sum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < MG_BT / 64; ++k) t += ws[k];
+    return t;
+}
+
+// rows of level L through the epilogue OP; returns this thread's partial
+template <int OP>
+__device__ double blk_rows(const MgBLevel &L, const double *v, const double *b, const double *d, double *o1,
+                           double *o2)
+{
+    double part = 0.0;
+    for (int64_t i = threadIdx.x; i < L.n; i += MG_BT) {
+        double s = 0.0;
+        for (int32_t k = L.rp[i]; k < L.rp[i + 1]; ++k) s += L.val[k] * v[L.ci[k]];
+        part += mg_epilogue<OP>(i, s, L.fixed[i] != 0, v, b, L.dinv, d, L.lam, o1, o2);
+    }
+    return part;
+}
+
+// V-cycle over levels D.first .. D.nlev-1 (x_first = V(b_first)); returns the
+// workgroup total of b_0 . x_0 when D.first == 0
+__device__ double blk_vcycle(const MgBDesc &D)
+{
+    const int L1 = D.nlev - 1;
+    for (int l = D.first; l < L1; ++l) {
+        const MgBLevel &L = D.lv[l], &C = D.lv[l + 1];
+        for (int64_t i = threadIdx.x; i < L.n; i += MG_BT)
+            L.d[i] = L.fixed[i] ? 0.0 : L.dinv[i] * L.b[i] * CHEB_ITHETA(*L.lam);
+        __syncthreads();
+        blk_rows<OP_CHEB_PRE>(L, L.d, L.b, nullptr, L.x, nullptr);
+        __syncthreads();
+        blk_rows<OP_RESID>(L, L.x, L.b, nullptr, L.r, nullptr);
+        __syncthreads();
+        for (int64_t t = threadIdx.x; t < C.n; t += MG_BT) {
+            if (L.fe == GRMP_FE_H1P2) restrict_p2_one(L.N, L.r, C.fixed, C.b, t);
+            else restrict_p1_one(L.N, C.N, L.ncomp, L.r, C.fixed, C.b, t);
+        }
+        __syncthreads();
+    }
+    {
+        const MgBLevel &C = D.lv[L1];
+        const int n = (int)C.n;
+        for (int i = threadIdx.x; i < n; i += MG_BT) {
+            double s = 0.0;
+            for (int j = 0; j < n; ++j) s += D.inv[i * n + j] * C.b[j];
+            C.x[i] = s;
+        }
+        __syncthreads();
+    }
+    double rz = 0.0;
+    for (int l = L1 - 1; l >= D.first; --l) {
+        const MgBLevel &L = D.lv[l], &C = D.lv[l + 1];
+        if (L.fe == GRMP_FE_H1P2) {
+            const int64_t nn = (int64_t)(L.N + 1) * (L.N + 1), nc = 2ll * L.N * L.N;
+            for (int64_t t = threadIdx.x; t < (nn > nc ? nn : nc); t += MG_BT)
+                prolong_p2_one(L.N, L.cells, L.fsign, C.x, L.fixed, L.x, t);
+        } else {
+            for (int64_t t = threadIdx.x; t < L.n; t += MG_BT)
+                prolong_p1_one(L.N, C.N, L.ncomp, C.x, L.fixed, L.x, t);
+        }
+        __syncthreads();
+        blk_rows<OP_CHEB_POST1>(L, L.x, L.b, nullptr, L.d, L.r);
+        __syncthreads();
+        const double p = blk_rows<OP_CHEB_POST2>(L, L.r, L.b, L.d, L.x, nullptr);
+        __syncthreads();
+        if (l == 0) rz = blk_sum(p);
+    }
+    if (D.first == 0 && L1 == 0) {  // a single level: z is the exact solve
+        double p = 0.0;
+        for (int64_t i = threadIdx.x; i < D.lv[0].n; i += MG_BT) p += D.lv[0].b[i] * D.lv[0].x[i];
+        rz = blk_sum(p);
+    }
+    return rz;
+}
+
+// coarse end of a V-cycle (levels D.first ..); at D.first == 0 also the
+// level-0 partials of r.z (total in part[0], zeros after it)
+__global__ __launch_bounds__(MG_BT) void k_mg_vcycle_block(const MgBDesc *__restrict__ Dp, const MgScal *sc)
+{
+    if (sc->done) return;
+    const MgBDesc &D = *Dp;
+    const double rz = blk_vcycle(D);
+    if (D.first == 0)
+        for (int k = threadIdx.x; k < D.nb0; k += MG_BT) D.part[k] = k == 0 ? rz : 0.0;
+}
+
+// up to iters CG iterations of a request whose levels all fit the workgroup,
+// with the true-residual check and restart of mg_job_verify done in place
+__global__ __launch_bounds__(MG_BT) void k_mg_pcg_block(const MgBDesc *__restrict__ Dp, MgScal *sc, int iters)
+{
+    const MgBDesc &D = *Dp;
+    const MgBLevel &L0 = D.lv[0];
+    const int64_t n = L0.n;
+    for (int k = 0; k < iters; ++k) {
+        __syncthreads();
+        const int st = sc->done;
+        __syncthreads();
+        if (st == 1) return;
+        if (st == 2) {  // the updated residual met rtol: check b - A x
+            blk_rows<OP_RESID>(L0, D.x, D.rhs, nullptr, D.r, nullptr);
+            __syncthreads();
+            double p = 0.0;
+            for (int64_t i = threadIdx.x; i < n; i += MG_BT) p += D.r[i] * D.r[i];
+            const double rr = blk_sum(p);
+            if (threadIdx.x == 0) {
+                sc->done = 0;
+                mg_scalar<RED_VERIFY>(sc, rr, 0.0);
+            }
+            __syncthreads();
+            if (sc->done) continue;
+            const double rz = blk_vcycle(D);
+            if (threadIdx.x == 0) mg_scalar<RED_RESTART>(sc, rz, 0.0);
+            __syncthreads();
+            if (sc->done) continue;
+            for (int64_t i = threadIdx.x; i < n; i += MG_BT) D.p[i] = D.z[i];
+            continue;
+        }
+        const double pq = blk_sum(blk_rows<OP_PCG_Q>(L0, D.p, nullptr, nullptr, D.q, nullptr));
+        if (threadIdx.x == 0) mg_scalar<RED_ALPHA>(sc, pq, 0.0);
+        __syncthreads();
+        if (sc->done) continue;
+        const double al = sc->alpha;
+        double p = 0.0;
+        for (int64_t i = threadIdx.x; i < n; i += MG_BT) {
+            D.x[i] += al * D.p[i];
+            const double ri = D.r[i] - al * D.q[i];
+            D.r[i] = ri;
+            p += ri * ri;
+        }
+        const double rr = blk_sum(p);
+        if (threadIdx.x == 0) mg_scalar<RED_CONV>(sc, rr, 0.0);
+        __syncthreads();
+        if (sc->done) continue;
+        const double rz = blk_vcycle(D);
+        if (threadIdx.x == 0) mg_scalar<RED_BETA>(sc, rz, 0.0);
+        __syncthreads();
+        if (sc->done) continue;
+        const double be = sc->beta;
+        for (int64_t i = threadIdx.x; i < n; i += MG_BT) D.p[i] = D.z[i] + be * D.p[i];
+    }
 }
 
 // ------------------------------------------------------------- the job
@@ -491,7 +734,15 @@ struct MgJob {
     double *part = nullptr, *part2 = nullptr, *inv = nullptr;
     MgScal *sc = nullptr;
     int nb0 = 0;
+    int first_small = 0;      // first level that runs inside one workgroup
+    MgBDesc hdesc;
+    MgBDesc *ddesc = nullptr;
     hipEvent_t ev = nullptr;
+    // grid-path chunk of CG iterations captured once and replayed
+    bool use_graph = false;
+    int graph_iters = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
     // profile: event pairs around every OP_RESID launch, with its algorithmic bytes
     bool profile = false;
     std::vector<hipEvent_t> pev;
@@ -509,7 +760,8 @@ static size_t mg_layout(MgJob *j, bool assign)
     const int64_t n0 = j->lv[0].n;
     const int nb0 = (int)nblk(n0, KB);
     size_t o_x = take(n0 * 8), o_rhs = take(n0 * 8), o_r = take(n0 * 8), o_z = take(n0 * 8), o_p = take(n0 * 8),
-           o_q = take(n0 * 8), o_part = take(nb0 * 8), o_part2 = take(nb0 * 8), o_sc = take(sizeof(MgScal));
+           o_q = take(n0 * 8), o_part = take(nb0 * 8), o_part2 = take(nb0 * 8), o_sc = take(sizeof(MgScal)),
+           o_desc = take(sizeof(MgBDesc));
     const MgLevel &c = j->lv[j->nlev - 1];
     size_t o_inv = take(c.n * c.n * 8);
     std::vector<size_t> o(MG_MAX_LEVELS * 9);
@@ -537,6 +789,7 @@ static size_t mg_layout(MgJob *j, bool assign)
     j->part = (double *)(B + o_part);
     j->part2 = (double *)(B + o_part2);
     j->sc = (MgScal *)(B + o_sc);
+    j->ddesc = (MgBDesc *)(B + o_desc);
     j->inv = (double *)(B + o_inv);
     j->nb0 = nb0;
     for (int l = 0; l < j->nlev; ++l) {
@@ -569,7 +822,45 @@ int64_t mg_job_bytes(const MgLevelIn *in, int nlev)
     return (int64_t)mg_layout(&j, false);
 }
 
-int mg_job_create(grmp_ctx *ctx, hipStream_t st, const MgLevelIn *in, int nlev, bool profile, MgJob **out)
+static void fill_desc(MgJob *j)
+{
+    MgBDesc &D = j->hdesc;
+    memset(&D, 0, sizeof(D));
+    D.first = j->first_small;
+    D.nlev = j->nlev;
+    D.nb0 = j->nb0;
+    for (int l = 0; l < j->nlev; ++l) {
+        const MgLevel &L = j->lv[l];
+        MgBLevel &B = D.lv[l];
+        B.rp = L.in.A->rowptr;
+        B.ci = L.in.A->cols;
+        B.val = L.val;
+        B.dinv = L.dinv;
+        B.fixed = L.in.A->fixed;
+        B.x = L.x;
+        B.b = L.b;
+        B.r = L.r;
+        B.d = L.d;
+        B.lam = L.lam;
+        B.n = L.n;
+        B.N = L.in.N;
+        B.fe = L.in.fe;
+        B.ncomp = L.in.ncomp;
+        B.cells = L.in.sp->mesh->cells;
+        B.fsign = L.in.sp->mesh->fsign;
+    }
+    D.inv = j->inv;
+    D.x = j->x;
+    D.r = j->r;
+    D.z = j->z;
+    D.p = j->p;
+    D.q = j->q;
+    D.rhs = j->rhs;
+    D.part = j->part;
+}
+
+int mg_job_create(grmp_ctx *ctx, hipStream_t st, const MgLevelIn *in, int nlev, bool profile, int block_rows,
+                  int coarse_rows, bool graph, MgJob **out)
 {
     GRMP_CHECK(ctx, nlev >= 1 && nlev <= MG_MAX_LEVELS, GRMP_ERR_ARG, "multigrid: %d levels", nlev);
     for (int l = 0; l < nlev; ++l) {
@@ -591,6 +882,7 @@ int mg_job_create(grmp_ctx *ctx, hipStream_t st, const MgLevelIn *in, int nlev, 
     j->st = st;
     j->nlev = nlev;
     j->profile = profile;
+    j->use_graph = graph && !profile;  // events cannot be recorded inside a stream capture
     for (int l = 0; l < nlev; ++l) {
         j->lv[l].in = in[l];
         j->lv[l].n = in[l].A->n;
@@ -604,6 +896,14 @@ int mg_job_create(grmp_ctx *ctx, hipStream_t st, const MgLevelIn *in, int nlev, 
         return grmp_fail(ctx, e == hipErrorOutOfMemory ? GRMP_ERR_OOM : GRMP_ERR_HIP, "multigrid: device buffer");
     }
     mg_layout(j, true);
+    // a request with at most block_rows rows runs whole in one workgroup;
+    // otherwise its levels with at most coarse_rows rows do
+    coarse_rows = std::max(coarse_rows, MG_DENSE_MAX);
+    j->first_small = nlev - 1;
+    if (j->lv[0].n <= block_rows) j->first_small = 0;
+    else
+        while (j->first_small > 1 && j->lv[j->first_small - 1].n <= coarse_rows) --j->first_small;
+    fill_desc(j);
     if (hipEventCreateWithFlags(&j->ev, hipEventDisableTiming) != hipSuccess) {
         (void)hipFreeAsync(j->buf, st);
         delete j;
@@ -630,18 +930,25 @@ static void rows_args(MgRows &a, const MgLevel &L, const MgJob *j)
     a.sc = j->sc;
 }
 
+// Residuals on levels of at least MG_FINE_ROWS rows run as k_mg_resid_fine;
+// when profiled, each such launch carries a start / stop event pair bound to
+// its own dispatch (hipExtLaunchKernelGGL), kept with its algorithmic bytes:
+// values and columns, row pointers, x gathered once, b read, r written, mask.
 template <int OP>
-static void launch_rows(MgJob *j, const MgRows &a)
+static void launch_rows(MgJob *j, const MgRows &a, int64_t nnz = 0)
 {
-    if (OP == OP_RESID && j->profile) {
+    if (OP == OP_RESID && a.n >= MG_FINE_ROWS) {
+        if (!j->profile) {
+            hipLaunchKernelGGL(k_mg_resid_fine, dim3(nblk(a.n, KB)), dim3(KB), 0, j->st, a);
+            return;
+        }
         hipEvent_t e0, e1;
         (void)hipEventCreate(&e0);
         (void)hipEventCreate(&e1);
-        (void)hipEventRecord(e0, j->st);
-        hipLaunchKernelGGL(k_mg_rows<OP>, dim3(nblk(a.n, KB)), dim3(KB), 0, j->st, a);
-        (void)hipEventRecord(e1, j->st);
+        hipExtLaunchKernelGGL(k_mg_resid_fine, dim3(nblk(a.n, KB)), dim3(KB), 0, j->st, e0, e1, 0, a);
         j->pev.push_back(e0);
         j->pev.push_back(e1);
+        j->pbytes.push_back((double)nnz * 12.0 + 4.0 * (a.n + 1) + 24.0 * a.n + 1.0 * a.n);
         return;
     }
     hipLaunchKernelGGL(k_mg_rows<OP>, dim3(nblk(a.n, KB)), dim3(KB), 0, j->st, a);
@@ -673,12 +980,12 @@ static void mg_prolong(MgJob *j, int l)
     }
 }
 
-// x_0 (= z) = V(b_0 (= r)); at level 0 the last smoothing step also leaves
-// the block partials of r.z in part
+// x_0 (= z) = V(b_0 (= r)); the level-0 block partials of r.z land in part.
+// Levels below first_small run as grid kernels, the rest in one workgroup.
 static void mg_vcycle(MgJob *j)
 {
-    const int L1 = j->nlev - 1;
-    for (int l = 0; l < L1; ++l) {
+    const int F = j->first_small;
+    for (int l = 0; l < F; ++l) {
         MgLevel &L = j->lv[l];
         hipLaunchKernelGGL(k_mg_cheb_first, dim3(nblk(L.n, 256)), dim3(256), 0, j->st, L.b, L.dinv, L.in.A->fixed,
                            L.lam, L.n, L.d, j->sc);
@@ -692,12 +999,11 @@ static void mg_vcycle(MgJob *j)
         a.v = L.x;
         a.b = L.b;
         a.o1 = L.r;
-        launch_rows<OP_RESID>(j, a);
+        launch_rows<OP_RESID>(j, a, L.nnz);
         mg_restrict(j, l);
     }
-    const MgLevel &C = j->lv[L1];
-    hipLaunchKernelGGL(k_mg_dense_apply, dim3(1), dim3(MG_DENSE_MAX), 0, j->st, j->inv, C.b, (int)C.n, C.x, j->sc);
-    for (int l = L1 - 1; l >= 0; --l) {
+    hipLaunchKernelGGL(k_mg_vcycle_block, dim3(1), dim3(MG_BT), 0, j->st, j->ddesc, j->sc);
+    for (int l = F - 1; l >= 0; --l) {
         MgLevel &L = j->lv[l];
         mg_prolong(j, l);
         MgRows a;
@@ -715,8 +1021,6 @@ static void mg_vcycle(MgJob *j)
         a.part = l == 0 ? j->part : nullptr;
         launch_rows<OP_CHEB_POST2>(j, a);
     }
-    if (L1 == 0)  // a single level: z is the exact solve, r.z separately
-        hipLaunchKernelGGL(k_mg_dot, dim3(j->nb0), dim3(KB), 0, j->st, j->r, j->z, j->lv[0].n, j->part, j->sc);
 }
 
 int mg_job_setup(MgJob *j, double rtol, int maxit)
@@ -741,6 +1045,7 @@ int mg_job_setup(MgJob *j, double rtol, int maxit)
     const MgLevel &C = j->lv[j->nlev - 1];
     hipLaunchKernelGGL(k_mg_dense_inv, dim3(1), dim3(256), 0, st, C.in.A->rowptr, C.in.A->cols, C.val, C.in.A->fixed,
                        (int)C.n, j->inv);
+    GRMP_HIP(ctx, hipMemcpyAsync(j->ddesc, &j->hdesc, sizeof(MgBDesc), hipMemcpyHostToDevice, st));
     // x = 0, r = rhs (0 on fixed rows), z = V(r), p = 0 then p = z
     const int64_t n = j->lv[0].n;
     GRMP_HIP(ctx, hipMemsetAsync(j->x, 0, n * 8, st));
@@ -755,7 +1060,42 @@ int mg_job_setup(MgJob *j, double rtol, int maxit)
     return GRMP_OK;
 }
 
+static int mg_iterate_grid(MgJob *j, int iters);
+
 int mg_job_iterate(MgJob *j, int iters)
+{
+    if (j->first_small == 0) {  // every level fits one workgroup: whole iterations in one launch
+        hipLaunchKernelGGL(k_mg_pcg_block, dim3(1), dim3(MG_BT), 0, j->st, j->ddesc, j->sc, iters);
+        GRMP_HIP(j->ctx, hipGetLastError());
+        return GRMP_OK;
+    }
+    if (!j->use_graph) return mg_iterate_grid(j, iters);
+    if (!j->gexec || j->graph_iters != iters) {
+        if (j->gexec) (void)hipGraphExecDestroy(j->gexec);
+        if (j->graph) (void)hipGraphDestroy(j->graph);
+        j->gexec = nullptr;
+        j->graph = nullptr;
+        mg_job_collect(j);  // profiled launches made before the capture
+        for (hipEvent_t e : j->pev) (void)hipEventDestroy(e);
+        j->pev.clear();
+        j->pbytes.clear();
+        GRMP_HIP(j->ctx, hipStreamBeginCapture(j->st, hipStreamCaptureModeThreadLocal));
+        const int rc = mg_iterate_grid(j, iters);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(j->st, &g);
+        GRMP_TRY(rc);
+        GRMP_HIP(j->ctx, e);
+        j->graph = g;
+        GRMP_HIP(j->ctx, hipGraphInstantiate(&j->gexec, j->graph, nullptr, nullptr, 0));
+        j->graph_iters = iters;
+    }
+    GRMP_HIP(j->ctx, hipGraphLaunch(j->gexec, j->st));
+    return GRMP_OK;
+}
+
+// the grid-path iterations as individual launches (no memset or host sync:
+// the sequence can be captured)
+static int mg_iterate_grid(MgJob *j, int iters)
 {
     const MgLevel &L0 = j->lv[0];
     const int64_t n = L0.n;
@@ -782,6 +1122,15 @@ int mg_job_iterate(MgJob *j, int iters)
 // three times and only while each check halves the previous one.
 int mg_job_verify(MgJob *j)
 {
+    if (j->first_small == 0) return GRMP_OK;  // k_mg_pcg_block checks in place
+    // with a captured chunk, event pairs are kept for replays: the check's
+    // launches stay out of the profiled set then
+    struct Unprofiled {
+        MgJob *j;
+        bool was;
+        ~Unprofiled() { j->profile = was; }
+    } guard{j, j->profile};
+    if (j->gexec) j->profile = false;
     const MgLevel &L0 = j->lv[0];
     const int64_t n = L0.n;
     hipLaunchKernelGGL(k_mg_verify_begin, dim3(1), dim3(1), 0, j->st, j->sc);
@@ -790,7 +1139,7 @@ int mg_job_verify(MgJob *j)
     a.v = j->x;
     a.b = j->rhs;
     a.o1 = j->r;
-    hipLaunchKernelGGL(k_mg_rows<OP_RESID>, dim3(nblk(n, KB)), dim3(KB), 0, j->st, a);
+    launch_rows<OP_RESID>(j, a, L0.nnz);
     hipLaunchKernelGGL(k_mg_dot, dim3(j->nb0), dim3(KB), 0, j->st, j->r, j->r, n, j->part, j->sc);
     hipLaunchKernelGGL(k_mg_red<RED_VERIFY>, dim3(1), dim3(KB), 0, j->st, j->part, j->nb0, nullptr, j->sc);
     mg_vcycle(j);
@@ -819,28 +1168,37 @@ int mg_job_wait(MgJob *j)
     return GRMP_OK;
 }
 
-// profile totals (call after the job's stream has drained)
-void mg_job_profile(MgJob *j, double *seconds, double *bytes, int64_t *launches)
+// Adds the timings of the profiled launches whose work has completed (the
+// caller saw the status event of the chunk): a replayed graph keeps its event
+// pairs for the next replay, individually launched ones are released.
+void mg_job_collect(MgJob *j)
 {
-    *seconds = *bytes = 0.0;
-    *launches = 0;
     if (!j->profile) return;
-    // OP_RESID launches alternate over levels 0 .. nlev-2 in V-cycle order
-    const int per = j->nlev - 1;
     for (size_t k = 0; k + 1 < j->pev.size(); k += 2) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, j->pev[k], j->pev[k + 1]) != hipSuccess) continue;
-        const MgLevel &L = j->lv[per > 0 ? (int)((k / 2) % per) : 0];
-        // values + columns, row pointers, x gathered once, b read, r written, fixed mask read
-        *bytes += (double)L.nnz * 12.0 + 4.0 * (L.n + 1) + 8.0 * L.n * 3 + 1.0 * L.n;
-        *seconds += ms * 1e-3;
-        *launches += 1;
+        j->prof_s += ms * 1e-3;
+        j->prof_bytes += j->pbytes[k / 2];
+        j->prof_launches += 1;
     }
+    if (j->gexec) return;
+    for (hipEvent_t e : j->pev) (void)hipEventDestroy(e);
+    j->pev.clear();
+    j->pbytes.clear();
+}
+
+void mg_job_profile(MgJob *j, double *seconds, double *bytes, int64_t *launches)
+{
+    *seconds = j->prof_s;
+    *bytes = j->prof_bytes;
+    *launches = j->prof_launches;
 }
 
 void mg_job_destroy(MgJob *j)
 {
     if (!j) return;
+    if (j->gexec) (void)hipGraphExecDestroy(j->gexec);
+    if (j->graph) (void)hipGraphDestroy(j->graph);
     for (hipEvent_t e : j->pev) (void)hipEventDestroy(e);
     if (j->ev) (void)hipEventDestroy(j->ev);
     if (j->buf) (void)hipFreeAsync(j->buf, j->st);

@@ -126,8 +126,12 @@ struct grmp_server {
     std::string fatal_msg;
     int64_t ntemplates = 0;
     double template_s = 0.0;
-    // multigrid mode: one context (stream) and one pinned status per slot
+    // multigrid mode: one context (stream) and one pinned status per slot;
+    // slots are split over mg_threads host threads (thread t drives the slots
+    // s with s % mg_threads == t); tpl_mu guards the template cache for them
     std::vector<grmp_ctx *> mg_ctx;
+    std::vector<std::thread> mg_workers;
+    std::mutex tpl_mu;
     MgStatus *mg_host = nullptr;
     double prof_s = 0.0, prof_bytes = 0.0;
     int64_t prof_launches = 0;
@@ -455,12 +459,17 @@ static void served_loop(grmp_server *S)
 // ---------------------------------------------------------- multigrid loop
 static const int kMgChunk = 4;     // PCG iterations enqueued between status polls
 static const int kMgMaxIt = 500;
+static const int kMgProfileEvery = 4;
 
+// (caller holds S->mu; lock order S->mu before tpl_mu)
 static void mg_release(grmp_server *S, Job *j)
 {
     mg_job_destroy(j->mg);
     j->mg = nullptr;
-    for (Template *t : j->mtpl) t->refs--;
+    {
+        std::lock_guard<std::mutex> g(S->tpl_mu);
+        for (Template *t : j->mtpl) t->refs--;
+    }
     j->mtpl.clear();
     S->mem_used -= j->footprint;
 }
@@ -484,20 +493,29 @@ static int mg_start(grmp_server *S, Job *j, int slot)
         shape.push_back({GRMP_FE_H1P1, N});
     }
     std::vector<MgLevelIn> in(shape.size());
-    const int64_t built = S->ntemplates;
-    for (size_t l = 0; l < shape.size(); ++l) {
-        grmp_request rl = r;
-        rl.fe = shape[l].first;
-        rl.N = shape[l].second;
-        Template *t = nullptr;
-        GRMP_TRY(template_get(S, rl, &t));
-        t->refs++;
-        j->mtpl.push_back(t);
-        in[l] = MgLevelIn{t->A, t->space, rl.N, shape[l].first, ncomp};
+    {
+        std::lock_guard<std::mutex> g(S->tpl_mu);
+        const int64_t built = S->ntemplates;
+        for (size_t l = 0; l < shape.size(); ++l) {
+            grmp_request rl = r;
+            rl.fe = shape[l].first;
+            rl.N = shape[l].second;
+            Template *t = nullptr;
+            GRMP_TRY(template_get(S, rl, &t));
+            t->refs++;
+            j->mtpl.push_back(t);
+            in[l] = MgLevelIn{t->A, t->space, rl.N, shape[l].first, ncomp};
+        }
+        // patterns built just now were made on the server stream
+        if (S->ntemplates != built) GRMP_HIP(c, hipStreamSynchronize(S->sctx->stream));
     }
-    // patterns built just now were made on the server stream
-    if (S->ntemplates != built) GRMP_HIP(c, hipStreamSynchronize(S->sctx->stream));
-    GRMP_TRY(mg_job_create(c, st, in.data(), (int)in.size(), S->cfg.profile != 0, &j->mg));
+    // Profiling times every grid residual launch with an event pair.  Events
+    // cannot be recorded inside a capture, so with graphs on only every
+    // kMgProfileEvery-th request (tickets 1, 1 + k, ...) is profiled, and it
+    // launches its kernels one by one.
+    const bool prof = S->cfg.profile != 0 && (!S->cfg.graph || (j->ticket - 1) % kMgProfileEvery == 0);
+    GRMP_TRY(mg_job_create(c, st, in.data(), (int)in.size(), prof, S->cfg.mg_block_rows, S->cfg.mg_coarse_rows,
+                           S->cfg.graph != 0, &j->mg));
     Template *t0 = j->mtpl[0];
     j->n = t0->A->n;
     grmp_vector kv{c, t0->mesh->ncells, mg_job_kappa0(j->mg)};
@@ -561,10 +579,10 @@ static void mg_fatal(grmp_server *S, std::vector<Job *> &act, int rc)
     S->cv_done.notify_all();
 }
 
-static void mg_loop(grmp_server *S)
+static void mg_loop(grmp_server *S, int tid, int nthreads)
 {
     const int M = (int)S->mg_ctx.size();
-    std::vector<Job *> act(M, nullptr);
+    std::vector<Job *> act(M, nullptr);  // this thread uses the slots s % nthreads == tid
     for (;;) {
         int nact = 0;
         for (Job *j : act) nact += j != nullptr;
@@ -577,7 +595,7 @@ static void mg_loop(grmp_server *S)
             }
         }
         // admit into free slots, in FIFO order, within the memory budget
-        for (int s = 0; s < M; ++s) {
+        for (int s = tid; s < M; s += nthreads) {
             if (act[s]) continue;
             Job *j = nullptr;
             {
@@ -609,10 +627,11 @@ static void mg_loop(grmp_server *S)
         }
         // finished chunks: enqueue the next one or hand out the result
         bool progress = false;
-        for (int s = 0; s < M; ++s) {
+        for (int s = tid; s < M; s += nthreads) {
             Job *j = act[s];
             if (!j || !mg_job_status_ready(j->mg)) continue;
             progress = true;
+            mg_job_collect(j->mg);
             if (S->mg_host[s].done != 1) {
                 int rc = S->mg_host[s].done == 2 ? mg_job_verify(j->mg) : GRMP_OK;
                 if (!rc) rc = mg_job_iterate(j->mg, kMgChunk);
@@ -635,7 +654,7 @@ static void mg_loop(grmp_server *S)
                     (void)mg_job_wait(j->mg);
                     break;
                 }
-        std::lock_guard<std::mutex> g(S->mu);
+        std::lock_guard<std::mutex> g(S->tpl_mu);
         templates_trim(S, S->cfg.memory_budget / 2);
     }
 }
@@ -708,6 +727,9 @@ extern "C" int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *cfg, 
     S->cfg.tile_rows = (int)kTileRows;
     S->cfg.fp64_first_sweep = 0;
     S->cfg.mg_slots = 8;
+    S->cfg.mg_threads = 4;
+    S->cfg.mg_block_rows = 2048;
+    S->cfg.mg_coarse_rows = 2048;
     if (cfg) {
         if (cfg->mode) S->cfg.mode = cfg->mode;
         if (cfg->memory_budget > 0) S->cfg.memory_budget = cfg->memory_budget;
@@ -717,6 +739,9 @@ extern "C" int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *cfg, 
         if (cfg->tile_rows > 0) S->cfg.tile_rows = std::max(64, std::min(cfg->tile_rows, 8192));
         if (cfg->fp64_first_sweep >= 0) S->cfg.fp64_first_sweep = cfg->fp64_first_sweep;
         if (cfg->mg_slots > 0) S->cfg.mg_slots = std::min(cfg->mg_slots, 64);
+        if (cfg->mg_threads > 0) S->cfg.mg_threads = cfg->mg_threads;
+        if (cfg->mg_block_rows > 0) S->cfg.mg_block_rows = cfg->mg_block_rows;
+        if (cfg->mg_coarse_rows > 0) S->cfg.mg_coarse_rows = cfg->mg_coarse_rows;
     }
     if (S->cfg.mode < 1 || S->cfg.mode > 3) {
         const int mode = S->cfg.mode;
@@ -758,14 +783,20 @@ extern "C" int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *cfg, 
             return grmp_fail(ctx, rc, "grmp_server_create: multigrid streams");
         }
         memset(S->mg_host, 0, sizeof(MgStatus) * S->cfg.mg_slots);
+        S->cfg.mg_threads = std::max(1, std::min(S->cfg.mg_threads, S->cfg.mg_slots));
     }
     S->t0 = wall_seconds();
     S->worker = std::thread([S] {
         (void)hipSetDevice(S->sctx->device);
         if (S->cfg.mode == 1) served_loop(S);
-        else if (S->cfg.mode == 3) mg_loop(S);
+        else if (S->cfg.mode == 3) mg_loop(S, 0, S->cfg.mg_threads);
         else plain_loop(S);
     });
+    for (int t = 1; S->cfg.mode == 3 && t < S->cfg.mg_threads; ++t)
+        S->mg_workers.emplace_back([S, t] {
+            (void)hipSetDevice(S->sctx->device);
+            mg_loop(S, t, S->cfg.mg_threads);
+        });
     *out = S;
     return GRMP_OK;
 }
@@ -831,6 +862,7 @@ extern "C" int grmp_server_stats(grmp_server *S, grmp_kernel_stats *out)
     if (!S || !out) return GRMP_ERR_ARG;
     memset(out, 0, sizeof(*out));
     std::lock_guard<std::mutex> g(S->mu);
+    std::lock_guard<std::mutex> gt(S->tpl_mu);
     out->templates = S->ntemplates;
     out->template_seconds = S->template_s;
     if (S->cfg.mode == 3) {
@@ -852,6 +884,8 @@ extern "C" int grmp_server_destroy(grmp_server *S)
         S->cv.notify_all();
     }
     if (S->worker.joinable()) S->worker.join();
+    for (std::thread &w : S->mg_workers)
+        if (w.joinable()) w.join();
     (void)hipStreamSynchronize(S->st_solve);
     delete S->eng;
     for (auto &kv : S->templates) template_free(kv.second);

@@ -37,7 +37,8 @@ class Server:
     MODES = {"served": 1, "plain": 2, "mg": 3}
 
     def __init__(self, mode="served", memory_budget=64 << 30, max_active=256, graph=True, profile=False, ctx=None,
-                 tile_rows=0, fp64_first_sweep=0, mg_slots=0):
+                 tile_rows=0, fp64_first_sweep=0, mg_slots=0, mg_threads=0, mg_block_rows=0,
+                 mg_coarse_rows=0):
         """mode "served": batched Jacobi-PCG engine with FP32 sweeps; "plain":
         one request at a time, FP64 Jacobi-PCG; "mg": FP64 CG preconditioned
         by a geometric multigrid V-cycle, ``mg_slots`` requests at once."""
@@ -47,6 +48,9 @@ class Server:
             raise ValueError(f"unknown server mode {mode!r}")
         cfg.mode = self.MODES[mode]
         cfg.mg_slots = int(mg_slots)
+        cfg.mg_threads = int(mg_threads)
+        cfg.mg_block_rows = int(mg_block_rows)
+        cfg.mg_coarse_rows = int(mg_coarse_rows)
         cfg.memory_budget = int(memory_budget)
         cfg.max_active = int(max_active)
         cfg.graph = 1 if graph else 0

@@ -286,18 +286,24 @@ typedef struct grmp_server_config {
                                  3 = multigrid-preconditioned CG in FP64, mg_slots requests at once */
     int64_t memory_budget;    /* bytes of HBM for resident problems */
     int max_active;           /* problems in flight */
-    int graph;                /* capture the Krylov step as a hipGraph */
-    int profile;              /* time the engine SpMV kernel with HIP events */
+    int graph;                /* capture the Krylov step (mode 1) or a request's chunk of CG
+                                 iterations (mode 3) as a hipGraph */
+    int profile;              /* time the engine SpMV kernel with HIP events (mode 1); mode 3: the
+                                 grid residual kernel (of every 4th request when graph is on) */
     int tile_rows;            /* engine rows per work tile (0 = 2048; 64 .. 8192) */
     int fp64_first_sweep;     /* > 0: a request whose first FP32 sweep takes more iterations
                                  continues in FP64 (0 = only on stalls) */
     int mg_slots;             /* mode 3: requests solved concurrently, one HIP stream each (0 = 8) */
-    int pad_;
+    int mg_threads;           /* mode 3: host threads launching for those slots (0 = 4) */
+    int mg_block_rows;        /* mode 3: a request with at most this many rows is solved inside one
+                                 workgroup (0 = 2048) */
+    int mg_coarse_rows;       /* mode 3: larger requests run their multigrid levels with at most this
+                                 many rows inside one workgroup (0 = 2048) */
 } grmp_server_config;
 
 typedef struct grmp_kernel_stats {
     double spmv_seconds;      /* summed durations of the timed SpMV launches (HIP events): the engine
-                                 SpMV (modes 1), the multigrid residual kernel (mode 3) */
+                                 SpMV (mode 1), the multigrid residual on levels of >= 2^20 rows (mode 3) */
     int64_t spmv_launches;
     double spmv_bytes;        /* algorithmic bytes of those launches (engine: nnz*(4+4) + 4(n+1) + 4n (x)
                                  + 4n (y) per row tile; multigrid residual: nnz*(8+4) + 4(n+1) + 8n (x)

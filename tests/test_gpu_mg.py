@@ -37,9 +37,13 @@ SPECS = [(1, 1, 1, 3), (1, 1, 4, 4), (2, 1, 3, 5), (1, 2, 2, 6), (1, 1, 5, 7), (
          (1, 1, 37, 10), (1, 2, 21, 11), (2, 1, 30, 12)]
 
 
-def test_mg_matches_lu(grmp, O):
-    print("[case] multigrid vs LU", flush=True)
-    with grmp.Server("mg") as S:
+@pytest.mark.parametrize("block_rows", [0, 64])
+def test_mg_matches_lu(grmp, O, block_rows):
+    """block_rows 0: the default thresholds (these requests run whole inside
+    one workgroup, or with their coarse levels there); 64: every level above
+    the coarsest runs as grid kernels."""
+    print(f"[case] multigrid vs LU, block_rows {block_rows}", flush=True)
+    with grmp.Server("mg", mg_block_rows=block_rows, mg_coarse_rows=block_rows) as S:
         ts = [S.submit(*spec, rtol=1e-12) for spec in SPECS]
         for spec, t in zip(SPECS, ts):
             r = S.wait(t)
@@ -69,16 +73,17 @@ def test_mg_iterations_bounded(grmp):
             assert out[(kind, fe, 200)] <= out[(kind, fe, 25)] + 12
 
 
-def test_mg_concurrent_equals_alone_bitwise(grmp):
-    print("[case] multigrid concurrent vs alone", flush=True)
+@pytest.mark.parametrize("block_rows", [0, 64])
+def test_mg_concurrent_equals_alone_bitwise(grmp, block_rows):
+    print(f"[case] multigrid concurrent vs alone, block_rows {block_rows}", flush=True)
     specs = SPECS[4:]
     alone = {}
-    with grmp.Server("mg", mg_slots=1) as S:
+    with grmp.Server("mg", mg_slots=1, mg_block_rows=block_rows, mg_coarse_rows=block_rows) as S:
         for spec in specs:
             t = S.submit(*spec)
             r = S.wait(t)
             alone[spec] = (r.iterations, S.solution(t).copy())
-    with grmp.Server("mg", mg_slots=4) as S:
+    with grmp.Server("mg", mg_slots=4, mg_block_rows=block_rows, mg_coarse_rows=block_rows) as S:
         ts = [S.submit(*spec) for spec in specs + specs]
         for spec, t in zip(specs + specs, ts):
             r = S.wait(t)
@@ -105,7 +110,9 @@ def test_mg_failures_stay_local(grmp):
 
 
 def test_mg_profile_counts_residual_kernel(grmp):
-    specs = SPECS[5:8]
+    """Profiling times the residual kernel of levels with at least 2^20 rows
+    (P2 on 512 x 512 has 1,050,625); results equal the unprofiled ones."""
+    specs = [(1, 2, 512, 31), (1, 1, 40, 32)]
     with grmp.Server("mg") as S:
         ref = {spec: S.solution(t).copy() for spec in specs for t in [S.submit(*spec)] if S.wait(t).status == 0}
     with grmp.Server("mg", profile=True) as S:
