@@ -110,6 +110,29 @@ function device_mesh(xgrid::ExtendableGrid)
     end
 end
 
+"""
+    device_faces(xgrid) -> (CellFaces, CellFaceSigns, bface)
+
+Face adjacency of a triangle grid as the device builds it on upload:
+CellFaces [3, ncells] (1-based) and CellFaceSigns [3, ncells] in
+ExtendableGrids' layout, and a 0/1 flag per face for the boundary.  Faces are
+numbered by first appearance; a grid whose faces ExtendableGrids numbered
+differently keeps passing FES[CellDofs] through device_space.
+"""
+function device_faces(xgrid::ExtendableGrid)
+    m = device_mesh(xgrid)
+    nf = Ref{Int64}(0)
+    check(context(), ccall((:grmp_mesh_faces, libgrmp), Cint,
+        (Ptr{Cvoid}, Ref{Int64}, Ptr{Int32}, Ptr{Int8}, Ptr{UInt8}), m, nf, C_NULL, C_NULL, C_NULL))
+    nc = size(xgrid[CellNodes], 2)
+    cf = zeros(Int32, 3, nc)
+    cs = zeros(Int8, 3, nc)
+    bf = zeros(UInt8, nf[])
+    check(context(), ccall((:grmp_mesh_faces, libgrmp), Cint,
+        (Ptr{Cvoid}, Ref{Int64}, Ptr{Int32}, Ptr{Int8}, Ptr{UInt8}), m, nf, cf, cs, bf))
+    return cf, cs, bf
+end
+
 function device_space(FES::FESpace{Tv,Ti,FEType}) where {Tv,Ti,FEType}
     get!(SPACES, FES) do
         dofs = Int32.(Matrix(FES[CellDofs]))        # [ndofs_loc, ncells], 1-based

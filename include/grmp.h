@@ -303,7 +303,7 @@ typedef struct grmp_server_config {
 
 typedef struct grmp_kernel_stats {
     double spmv_seconds;      /* summed durations of the timed SpMV launches (HIP events): the engine
-                                 SpMV (mode 1), the multigrid residual on levels of >= 2^20 rows (mode 3) */
+                                 SpMV (mode 1), the multigrid residual on levels of >= 2^21 rows (mode 3) */
     int64_t spmv_launches;
     double spmv_bytes;        /* algorithmic bytes of those launches (engine: nnz*(4+4) + 4(n+1) + 4n (x)
                                  + 4n (y) per row tile; multigrid residual: nnz*(8+4) + 4(n+1) + 8n (x)

@@ -110,9 +110,9 @@ def test_mg_failures_stay_local(grmp):
 
 
 def test_mg_profile_counts_residual_kernel(grmp):
-    """Profiling times the residual kernel of levels with at least 2^20 rows
-    (P2 on 512 x 512 has 1,050,625); results equal the unprofiled ones."""
-    specs = [(1, 2, 512, 31), (1, 1, 40, 32)]
+    """Profiling times the residual kernel of levels with at least 2^21 rows
+    (P2 on 724 x 724 has 2,099,601); results equal the unprofiled ones."""
+    specs = [(1, 2, 724, 31), (1, 1, 40, 32)]
     with grmp.Server("mg") as S:
         ref = {spec: S.solution(t).copy() for spec in specs for t in [S.submit(*spec)] if S.wait(t).status == 0}
     with grmp.Server("mg", profile=True) as S:
