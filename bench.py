@@ -11,13 +11,20 @@ FP64 residual 1e-9 (the target is relative error <= 1e-8 against FP64
 solves, checked on 24 requests spanning the size range) and its solution
 copied back to the host.
 
+Solver (--solver): mg (default) serves every request with FP64 CG
+preconditioned by a geometric multigrid V-cycle, several requests at once on
+their own streams; engine serves them with the batched Jacobi-PCG engine
+(FP32 sweeps in FP64 refinement).
+
 Reported:
   value           DoF-solves/s of the served stream (sum of ndofs / wall)
   p50/p99         request latency (arrival -> solution on the host)
   plain           the same metric for plain one-at-a-time FP64 GPU solves,
                   measured on a systematic sample of the stream
-  roofline        engine SpMV kernel: algorithmic bytes / summed durations
-                  (HIP events captured around every launch of the timed run)
+  roofline        mg: the residual kernel on levels of >= 2^21 rows
+                  (k_mg_resid_fine); engine: the engine SpMV.  Algorithmic
+                  bytes / summed durations from HIP events around every
+                  launch of that kernel in the timed run
   cpu_baseline    the oracle (single-thread assembly + OpenMP PCG) on a
                   bounded sample of the same stream
 """
