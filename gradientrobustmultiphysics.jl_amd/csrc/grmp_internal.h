@@ -216,6 +216,7 @@ const double *mg_job_solution(const MgJob *j);
 int mg_job_setup(MgJob *j, double rtol, int maxit);
 int mg_job_iterate(MgJob *j, int iters);
 int mg_job_verify(MgJob *j);         // after done == 2: check the true residual
+bool mg_job_block(const MgJob *j);   // whole iterations in one workgroup (checks in place)
 int mg_job_status_async(MgJob *j, MgStatus *host);
 bool mg_job_status_ready(MgJob *j);
 int mg_job_wait(MgJob *j);

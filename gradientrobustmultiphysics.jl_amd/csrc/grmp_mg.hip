@@ -55,6 +55,58 @@ struct MgScal {
     int restarts, pad;
 };
 
+enum MgRed { RED_INIT = 0, RED_ALPHA = 1, RED_CONV = 2, RED_BETA = 3, RED_VERIFY = 4, RED_RESTART = 5 };
+// the scalar step of W on the reduced values a (and b for RED_INIT); one thread
+template <int W>
+__device__ __forceinline__ void mg_scalar(MgScal *sc, double a, double b)
+{
+    if (W == RED_INIT) {  // a = r.z, b = r.r of the start residual
+        sc->rz = a;
+        sc->rr0 = b;
+        sc->rr = b;
+        sc->beta = 0.0;
+        sc->relres = b > 0.0 ? 1.0 : 0.0;
+        if (!(b > 0.0)) sc->done = 1;  // zero right-hand side: x = 0
+        else if (!(a > 0.0) || !isfinite(a)) { sc->status = GRMP_ERR_NONFINITE; sc->done = 1; }
+    } else if (W == RED_ALPHA) {
+        sc->pq = a;
+        if (!(a > 0.0) || !isfinite(a)) { sc->status = GRMP_ERR_NONFINITE; sc->done = 1; return; }
+        sc->alpha = sc->rz / a;
+    } else if (W == RED_CONV) {
+        sc->rr = a;
+        sc->it += 1;
+        sc->relres = sqrt(a / sc->rr0);
+        if (!isfinite(a)) { sc->status = GRMP_ERR_NONFINITE; sc->done = 1; }
+        else if (sc->relres <= sc->rtol) sc->done = 2;
+        else if (sc->it >= sc->maxit) { sc->status = GRMP_ERR_NOCONV; sc->done = 1; }
+    } else if (W == RED_VERIFY) {  // a = |b - A x|^2
+        const double tr = sqrt(a / sc->rr0);
+        sc->rr = a;
+        sc->relres = tr;
+        if (!isfinite(a)) { sc->status = GRMP_ERR_NONFINITE; sc->done = 1; }
+        else if (tr <= sc->rtol) sc->done = 1;
+        else if (sc->restarts >= 3 || (sc->restarts > 0 && tr > 0.5 * sc->best)) {
+            // the true residual stopped improving: the FP64 floor of b - A x
+            // (1e-13 .. 1e-10 relative for these problems) is accepted within
+            // 100 rtol, as the engine does
+            if (!(tr <= 100.0 * sc->rtol)) sc->status = GRMP_ERR_NOCONV;
+            sc->done = 1;
+        } else {
+            sc->restarts += 1;
+            sc->best = sc->restarts == 1 || tr < sc->best ? tr : sc->best;
+        }
+    } else if (W == RED_RESTART) {  // a = r.z of the true residual: p = z
+        if (!(a > 0.0) || !isfinite(a)) { sc->status = GRMP_ERR_NONFINITE; sc->done = 1; return; }
+        sc->rz = a;
+        sc->beta = 0.0;
+    } else {
+        if (!isfinite(a)) { sc->status = GRMP_ERR_NONFINITE; sc->done = 1; return; }
+        sc->beta = a / sc->rz;
+        sc->rz = a;
+    }
+}
+
+
 // ------------------------------------------------------------ row kernels
 enum MgOp { OP_RESID = 0, OP_CHEB_PRE = 1, OP_CHEB_POST1 = 2, OP_CHEB_POST2 = 3, OP_PCG_Q = 4 };
 
@@ -123,6 +175,43 @@ __device__ __forceinline__ void mg_rows_body(const MgRows &a)
     }
 }
 
+// Chebyshev pre-smoothing from x0 = 0 in one pass: d0 = D^-1 b / theta is
+// formed on the fly in the staged row product (the same products as a stored
+// d0), then o1 = (1 + C1) d0 + C2 D^-1 (b - A d0).
+__global__ __launch_bounds__(KB) void k_mg_pre(MgRows a)
+{
+    if (a.sc->done) return;
+    __shared__ double lds[SPMV_CAP];
+    const double l = *a.lam, ith = CHEB_ITHETA(l);
+    const int64_t r0 = (int64_t)blockIdx.x * KB;
+    const int64_t r1 = r0 + KB < a.n ? r0 + KB : a.n;
+    const int t = threadIdx.x;
+    const int64_t r = r0 + t;
+    const int32_t base = a.rp[r0], end = a.rp[r1];
+    int32_t my0 = 0, my1 = 0;
+    if (r < r1) {
+        my0 = a.rp[r];
+        my1 = a.rp[r + 1];
+    }
+    double s = 0.0;
+    for (int32_t c0 = base; c0 < end; c0 += SPMV_CAP) {
+        const int32_t c1 = c0 + SPMV_CAP < end ? c0 + SPMV_CAP : end;
+        __syncthreads();
+        for (int32_t k = c0 + t; k < c1; k += KB) {
+            const int32_t c = a.ci[k];
+            lds[k - c0] = a.val[k] * (a.fixed[c] ? 0.0 : a.dinv[c] * a.b[c] * ith);
+        }
+        __syncthreads();
+        const int32_t lo = my0 > c0 ? my0 : c0, hi = my1 < c1 ? my1 : c1;
+        for (int32_t k = lo; k < hi; ++k) s += lds[k - c0];
+    }
+    if (r < r1) {
+        const bool fx = a.fixed[r] != 0;
+        const double d0 = fx ? 0.0 : a.dinv[r] * a.b[r] * ith;
+        a.o1[r] = fx ? 0.0 : (1.0 + CHEB_C1) * d0 + CHEB_C2(l) * a.dinv[r] * (a.b[r] - s);
+    }
+}
+
 template <int OP>
 __global__ __launch_bounds__(KB) void k_mg_rows(MgRows a)
 {
@@ -137,16 +226,6 @@ __global__ __launch_bounds__(KB) void k_mg_resid_fine(MgRows a)
 {
     if (a.sc->done) return;
     mg_rows_body<OP_RESID>(a);
-}
-
-// first pre-smoothing step from x0 = 0: d0 = D^-1 b / theta
-__global__ void k_mg_cheb_first(const double *__restrict__ b, const double *__restrict__ dinv,
-                                const uint8_t *__restrict__ fixed, const double *__restrict__ lam, int64_t n,
-                                double *__restrict__ d, const MgScal *sc)
-{
-    if (sc->done) return;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) d[i] = fixed[i] ? 0.0 : dinv[i] * b[i] * CHEB_ITHETA(*lam);
 }
 
 // Gershgorin bound max_i sum_j |A_ij| / A_ii over free rows (a max, so the
@@ -464,57 +543,6 @@ __global__ void k_mg_update_p(double *__restrict__ p, const double *__restrict__
 }
 
 // one block: the fixed-order sum of nb partials, then the scalar step
-enum MgRed { RED_INIT = 0, RED_ALPHA = 1, RED_CONV = 2, RED_BETA = 3, RED_VERIFY = 4, RED_RESTART = 5 };
-// the scalar step of W on the reduced values a (and b for RED_INIT); one thread
-template <int W>
-__device__ __forceinline__ void mg_scalar(MgScal *sc, double a, double b)
-{
-    if (W == RED_INIT) {  // a = r.z, b = r.r of the start residual
-        sc->rz = a;
-        sc->rr0 = b;
-        sc->rr = b;
-        sc->beta = 0.0;
-        sc->relres = b > 0.0 ? 1.0 : 0.0;
-        if (!(b > 0.0)) sc->done = 1;  // zero right-hand side: x = 0
-        else if (!(a > 0.0) || !isfinite(a)) { sc->status = GRMP_ERR_NONFINITE; sc->done = 1; }
-    } else if (W == RED_ALPHA) {
-        sc->pq = a;
-        if (!(a > 0.0) || !isfinite(a)) { sc->status = GRMP_ERR_NONFINITE; sc->done = 1; return; }
-        sc->alpha = sc->rz / a;
-    } else if (W == RED_CONV) {
-        sc->rr = a;
-        sc->it += 1;
-        sc->relres = sqrt(a / sc->rr0);
-        if (!isfinite(a)) { sc->status = GRMP_ERR_NONFINITE; sc->done = 1; }
-        else if (sc->relres <= sc->rtol) sc->done = 2;
-        else if (sc->it >= sc->maxit) { sc->status = GRMP_ERR_NOCONV; sc->done = 1; }
-    } else if (W == RED_VERIFY) {  // a = |b - A x|^2
-        const double tr = sqrt(a / sc->rr0);
-        sc->rr = a;
-        sc->relres = tr;
-        if (!isfinite(a)) { sc->status = GRMP_ERR_NONFINITE; sc->done = 1; }
-        else if (tr <= sc->rtol) sc->done = 1;
-        else if (sc->restarts >= 3 || (sc->restarts > 0 && tr > 0.5 * sc->best)) {
-            // the true residual stopped improving: the FP64 floor of b - A x
-            // (1e-13 .. 1e-10 relative for these problems) is accepted within
-            // 100 rtol, as the engine does
-            if (!(tr <= 100.0 * sc->rtol)) sc->status = GRMP_ERR_NOCONV;
-            sc->done = 1;
-        } else {
-            sc->restarts += 1;
-            sc->best = sc->restarts == 1 || tr < sc->best ? tr : sc->best;
-        }
-    } else if (W == RED_RESTART) {  // a = r.z of the true residual: p = z
-        if (!(a > 0.0) || !isfinite(a)) { sc->status = GRMP_ERR_NONFINITE; sc->done = 1; return; }
-        sc->rz = a;
-        sc->beta = 0.0;
-    } else {
-        if (!isfinite(a)) { sc->status = GRMP_ERR_NONFINITE; sc->done = 1; return; }
-        sc->beta = a / sc->rz;
-        sc->rz = a;
-    }
-}
-
 template <int W>
 __global__ __launch_bounds__(KB) void k_mg_red(const double *__restrict__ part, int nb, const double *__restrict__ part2,
                                                MgScal *sc)
@@ -990,14 +1018,11 @@ static void mg_vcycle(MgJob *j)
     const int F = j->first_small;
     for (int l = 0; l < F; ++l) {
         MgLevel &L = j->lv[l];
-        hipLaunchKernelGGL(k_mg_cheb_first, dim3(nblk(L.n, 256)), dim3(256), 0, j->st, L.b, L.dinv, L.in.A->fixed,
-                           L.lam, L.n, L.d, j->sc);
         MgRows a;
         rows_args(a, L, j);
-        a.v = L.d;
         a.b = L.b;
         a.o1 = L.x;
-        launch_rows<OP_CHEB_PRE>(j, a);
+        hipLaunchKernelGGL(k_mg_pre, dim3(nblk(L.n, KB)), dim3(KB), 0, j->st, a);
         rows_args(a, L, j);
         a.v = L.x;
         a.b = L.b;
@@ -1123,6 +1148,8 @@ static int mg_iterate_grid(MgJob *j, int iters)
 // The updated residual met rtol: replace it by the true residual b - A x;
 // accept if that meets rtol too, else restart CG from it (p = z), at most
 // three times and only while each check halves the previous one.
+bool mg_job_block(const MgJob *j) { return j->first_small == 0; }
+
 int mg_job_verify(MgJob *j)
 {
     if (j->first_small == 0) return GRMP_OK;  // k_mg_pcg_block checks in place

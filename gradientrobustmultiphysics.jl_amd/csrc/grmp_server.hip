@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <condition_variable>
 #include <cstring>
@@ -101,6 +102,8 @@ struct Job {
     MgJob *mg = nullptr;
     std::vector<Template *> mtpl;
     int mslot = -1;
+    int mg_it = 0;         // iteration count and residual at the previous status
+    double mg_rel = 0.0;
 };
 
 struct grmp_server {
@@ -531,6 +534,20 @@ static int mg_start(grmp_server *S, Job *j, int slot)
     return GRMP_OK;
 }
 
+// Iterations to enqueue before the next status: with replayed graphs the
+// captured chunk; otherwise what the last chunk's residual reduction rate
+// predicts to reach rtol, between 1 and 8, so few launches follow convergence.
+static int mg_chunk(const grmp_server *S, const Job *j, const MgStatus &h)
+{
+    if (S->cfg.graph) return kMgChunk;
+    const double rtol = j->req.rtol > 0 ? j->req.rtol : 1e-10;
+    if (j->mg_it <= 0 || h.it <= j->mg_it || !(h.relres > 0.0) || !(h.relres < j->mg_rel)) return kMgChunk;
+    const double rate = pow(h.relres / j->mg_rel, 1.0 / (h.it - j->mg_it));
+    const double need = ceil(log(rtol / h.relres) / log(rate));
+    if (!(need >= 1.0)) return 1;
+    return need > 8.0 ? 8 : (int)need;
+}
+
 static void mg_finish(grmp_server *S, Job *j)
 {
     const MgStatus h = S->mg_host[j->mslot];
@@ -632,9 +649,19 @@ static void mg_loop(grmp_server *S, int tid, int nthreads)
             if (!j || !mg_job_status_ready(j->mg)) continue;
             progress = true;
             mg_job_collect(j->mg);
-            if (S->mg_host[s].done != 1) {
-                int rc = S->mg_host[s].done == 2 ? mg_job_verify(j->mg) : GRMP_OK;
-                if (!rc) rc = mg_job_iterate(j->mg, kMgChunk);
+            const MgStatus h = S->mg_host[s];
+            if (h.done != 1) {
+                int rc = GRMP_OK;
+                if (h.done == 2) {
+                    // the true-residual check; a request checked in place by its
+                    // one-workgroup kernel does it at the start of its next launch
+                    rc = mg_job_verify(j->mg);
+                    if (!rc && mg_job_block(j->mg)) rc = mg_job_iterate(j->mg, 1);
+                } else {
+                    rc = mg_job_iterate(j->mg, mg_chunk(S, j, h));
+                }
+                j->mg_it = h.it;
+                j->mg_rel = h.relres;
                 if (!rc) rc = mg_job_status_async(j->mg, &S->mg_host[s]);
                 if (rc) {
                     mg_fatal(S, act, rc);
