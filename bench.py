@@ -58,13 +58,13 @@ def make_stream(nreq, rate, seed=2024):
 
 def serve(grmp, mode, stream, rtol, keep=(), profile=False, respect_arrivals=True, tile_rows=0, precision="fp32_ir",
           fp64_first_sweep=0, max_active=512, mg_slots=0, mg_threads=0, mg_block_rows=0, mg_graph=False,
-          mg_coarse_rows=0):
+          mg_coarse_rows=0, mg_batch=None):
     """Submits the stream at its arrival times and waits for every result."""
     graph = mg_graph if mode == "mg" else True
     S = grmp.Server(mode, profile=profile, memory_budget=96 << 30, max_active=max_active, tile_rows=tile_rows,
                     graph=graph,
                     fp64_first_sweep=fp64_first_sweep, mg_slots=mg_slots, mg_threads=mg_threads,
-                    mg_block_rows=mg_block_rows, mg_coarse_rows=mg_coarse_rows)
+                    mg_block_rows=mg_block_rows, mg_coarse_rows=mg_coarse_rows, mg_batch=mg_batch)
     keep = set(keep)
     tickets = []
     t0 = time.perf_counter()
@@ -325,6 +325,8 @@ def main():
                     help="mg: levels up to this many rows run in one workgroup (0 = library default)")
     ap.add_argument("--mg-coarse-rows", type=int, default=0,
                     help="mg: levels of larger requests up to this many rows run in one workgroup (0 = default)")
+    ap.add_argument("--mg-batch", type=int, default=None,
+                    help="mg: requests advanced in lockstep per launch (0: one stream per request; default: library)")
     ap.add_argument("--mg-graph", action="store_true",
                     help="mg: replay captured chunks of CG iterations (profiles every 4th request only)")
     args = ap.parse_args()
@@ -357,7 +359,7 @@ def main():
                           precision=args.precision, fp64_first_sweep=args.fp64_first_sweep,
                           max_active=args.max_active, mg_slots=args.mg_slots, mg_threads=args.mg_threads,
                           mg_block_rows=args.mg_block_rows, mg_graph=args.mg_graph,
-                          mg_coarse_rows=args.mg_coarse_rows))
+                          mg_coarse_rows=args.mg_coarse_rows, mg_batch=args.mg_batch))
         GE.load().default_context().sync()
     run = runs[-1]
     walls = [r["wall"] for r in runs]
@@ -387,7 +389,9 @@ def main():
     st = run["stats"]
     achieved = st["spmv_bytes"] / st["spmv_seconds"] / 1e9 if st["spmv_seconds"] > 0 else 0.0
     avg_launch = st["spmv_seconds"] / max(st["spmv_launches"], 1)
-    roof = {"bound": "hbm", "kernel": "k_mg_resid_fine" if mode == "mg" else "k_eng_spmv",
+    batched = mode == "mg" and (args.mg_batch is None or args.mg_batch > 0)
+    kernel = ("k_mgb_rows<OP_RESID>" if batched else "k_mg_resid_fine") if mode == "mg" else "k_eng_spmv"
+    roof = {"bound": "hbm", "kernel": kernel,
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "avg_launch_us": round(avg_launch * 1e6, 2), "launches": int(st["spmv_launches"]),
@@ -427,7 +431,8 @@ def main():
                   "p50_service_ms": round(1e3 * float(np.percentile(plain["service"], 50)), 2),
                   "p99_service_ms": round(1e3 * float(np.percentile(plain["service"], 99)), 2)},
         "served_vs_plain": round(value / plain_value, 3),
-        "solver": "FP64 CG + geometric multigrid V-cycle, one stream per request in flight" if mode == "mg"
+        "solver": ("FP64 CG + geometric multigrid V-cycle, " + ("requests in lockstep batches" if batched else
+                   "one stream per request in flight")) if mode == "mg"
                   else "batched Jacobi-PCG engine, FP32 sweeps in FP64 refinement",
         "engine": {"steps": int(st["steps"]), "templates": int(st["templates"]),
                    "template_seconds": round(st["template_seconds"], 3),

@@ -88,7 +88,8 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("memory_budget", I64), ("max_active", ctypes.c_int),
                 ("graph", ctypes.c_int), ("profile", ctypes.c_int), ("tile_rows", ctypes.c_int),
                 ("fp64_first_sweep", ctypes.c_int), ("mg_slots", ctypes.c_int), ("mg_threads", ctypes.c_int),
-                ("mg_block_rows", ctypes.c_int), ("mg_coarse_rows", ctypes.c_int)]
+                ("mg_block_rows", ctypes.c_int), ("mg_coarse_rows", ctypes.c_int),
+                ("mg_batch", ctypes.c_int), ("pad_", ctypes.c_int)]
 
 
 class KernelStats(ctypes.Structure):

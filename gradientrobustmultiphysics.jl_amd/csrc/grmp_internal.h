@@ -217,11 +217,20 @@ int mg_job_setup(MgJob *j, double rtol, int maxit);
 int mg_job_iterate(MgJob *j, int iters);
 int mg_job_verify(MgJob *j);         // after done == 2: check the true residual
 bool mg_job_block(const MgJob *j);   // whole iterations in one workgroup (checks in place)
+void mg_job_set_stream(MgJob *j, hipStream_t st);
 int mg_job_status_async(MgJob *j, MgStatus *host);
 bool mg_job_status_ready(MgJob *j);
 int mg_job_wait(MgJob *j);
 void mg_job_collect(MgJob *j);       // after a status event: add the finished profiled launches
 void mg_job_profile(MgJob *j, double *seconds, double *bytes, int64_t *launches);
 void mg_job_destroy(MgJob *j);
+// lockstep batches of jobs set up on one stream (grmp_mg.hip)
+struct MgBatch;
+int mg_batch_create(grmp_ctx *ctx, hipStream_t st, bool profile, MgBatch **out);
+int mg_batch_run(MgBatch *b, MgJob *const *jobs, int n, int iters);  // + status copy and event
+int mg_batch_wait(MgBatch *b);
+const MgStatus *mg_batch_status(const MgBatch *b);                  // per job, after the wait
+void mg_batch_profile(const MgBatch *b, double *seconds, double *bytes, int64_t *launches);
+void mg_batch_destroy(MgBatch *b);
 // timing
 double wall_seconds();

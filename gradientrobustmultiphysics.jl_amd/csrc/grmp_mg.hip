@@ -160,10 +160,9 @@ __device__ __forceinline__ double mg_epilogue(int64_t i, double s, bool fx, cons
 }
 
 template <int OP>
-__device__ __forceinline__ void mg_rows_body(const MgRows &a)
+__device__ __forceinline__ void mg_rows_body(const MgRows &a, int64_t r0, int pidx)
 {
     __shared__ double lds[SPMV_CAP];
-    const int64_t r0 = (int64_t)blockIdx.x * KB;
     const int64_t r1 = r0 + KB < a.n ? r0 + KB : a.n;
     const double s = csr_rows_lds<double, double, double>(a.rp, a.ci, a.val, a.v, r0, r1, lds);
     const int64_t i = r0 + threadIdx.x;
@@ -171,19 +170,17 @@ __device__ __forceinline__ void mg_rows_body(const MgRows &a)
     if (i < r1) pa = mg_epilogue<OP>(i, s, a.fixed[i] != 0, a.v, a.b, a.dinv, a.d, a.lam, a.o1, a.o2);
     if ((OP == OP_CHEB_POST2 || OP == OP_PCG_Q) && a.part) {
         block_sum2(pa, pb);
-        if (threadIdx.x == 0) a.part[blockIdx.x] = pa;
+        if (threadIdx.x == 0) a.part[pidx] = pa;
     }
 }
 
 // Chebyshev pre-smoothing from x0 = 0 in one pass: d0 = D^-1 b / theta is
 // formed on the fly in the staged row product (the same products as a stored
 // d0), then o1 = (1 + C1) d0 + C2 D^-1 (b - A d0).
-__global__ __launch_bounds__(KB) void k_mg_pre(MgRows a)
+__device__ __forceinline__ void mg_pre_body(const MgRows &a, int64_t r0)
 {
-    if (a.sc->done) return;
     __shared__ double lds[SPMV_CAP];
     const double l = *a.lam, ith = CHEB_ITHETA(l);
-    const int64_t r0 = (int64_t)blockIdx.x * KB;
     const int64_t r1 = r0 + KB < a.n ? r0 + KB : a.n;
     const int t = threadIdx.x;
     const int64_t r = r0 + t;
@@ -212,11 +209,17 @@ __global__ __launch_bounds__(KB) void k_mg_pre(MgRows a)
     }
 }
 
+__global__ __launch_bounds__(KB) void k_mg_pre(MgRows a)
+{
+    if (a.sc->done) return;
+    mg_pre_body(a, (int64_t)blockIdx.x * KB);
+}
+
 template <int OP>
 __global__ __launch_bounds__(KB) void k_mg_rows(MgRows a)
 {
     if (a.sc->done) return;
-    mg_rows_body<OP>(a);
+    mg_rows_body<OP>(a, (int64_t)blockIdx.x * KB, blockIdx.x);
 }
 
 // the residual on levels of at least MG_FINE_ROWS rows: the same rows as
@@ -225,7 +228,7 @@ __global__ __launch_bounds__(KB) void k_mg_rows(MgRows a)
 __global__ __launch_bounds__(KB) void k_mg_resid_fine(MgRows a)
 {
     if (a.sc->done) return;
-    mg_rows_body<OP_RESID>(a);
+    mg_rows_body<OP_RESID>(a, (int64_t)blockIdx.x * KB, blockIdx.x);
 }
 
 // Gershgorin bound max_i sum_j |A_ij| / A_ii over free rows (a max, so the
@@ -590,6 +593,7 @@ struct MgBDesc {
     double *x, *r, *z, *p, *q;  // level-0 CG vectors
     const double *rhs;
     double *part;               // level-0 block partials (nb0 entries)
+    MgScal *sc;
 };
 
 // sum over the workgroup, returned to every thread
@@ -689,9 +693,8 @@ __global__ __launch_bounds__(MG_BT) void k_mg_vcycle_block(const MgBDesc *__rest
 
 // up to iters CG iterations of a request whose levels all fit the workgroup,
 // with the true-residual check and restart of mg_job_verify done in place
-__global__ __launch_bounds__(MG_BT) void k_mg_pcg_block(const MgBDesc *__restrict__ Dp, MgScal *sc, int iters)
+__device__ void mg_pcg_block_body(const MgBDesc &D, MgScal *sc, int iters)
 {
-    const MgBDesc &D = *Dp;
     const MgBLevel &L0 = D.lv[0];
     const int64_t n = L0.n;
     for (int k = 0; k < iters; ++k) {
@@ -741,6 +744,186 @@ __global__ __launch_bounds__(MG_BT) void k_mg_pcg_block(const MgBDesc *__restric
         const double be = sc->beta;
         for (int64_t i = threadIdx.x; i < n; i += MG_BT) D.p[i] = D.z[i] + be * D.p[i];
     }
+}
+
+__global__ __launch_bounds__(MG_BT) void k_mg_pcg_block(const MgBDesc *__restrict__ Dp, MgScal *sc, int iters)
+{
+    mg_pcg_block_body(*Dp, sc, iters);
+}
+
+// ------------------------------------------------------ lockstep batches
+// Requests in flight advance together: every phase of a CG iteration and of
+// its V-cycle is one launch over a table of 256-row tiles of all requests
+// taking part, each tile naming its request and level.  Requests whose
+// levels all fit one workgroup run in one launch, a workgroup each.  The
+// reductions stay per request (one block each, the same fixed order), so a
+// request's iterates are the ones it has alone.
+struct MgTile {
+    int32_t req, level, row0, pad;
+};
+
+__device__ __forceinline__ MgRows mg_tile_rows(const MgBDesc &D, int level)
+{
+    const MgBLevel &L = D.lv[level];
+    MgRows a;
+    a.rp = L.rp;
+    a.ci = L.ci;
+    a.val = L.val;
+    a.fixed = L.fixed;
+    a.n = L.n;
+    a.v = nullptr;
+    a.b = nullptr;
+    a.dinv = L.dinv;
+    a.d = nullptr;
+    a.o1 = nullptr;
+    a.o2 = nullptr;
+    a.lam = L.lam;
+    a.sc = D.sc;
+    a.part = nullptr;
+    return a;
+}
+
+template <int OP>
+__global__ __launch_bounds__(KB) void k_mgb_rows(const MgTile *__restrict__ T, const MgBDesc *const *__restrict__ Dp)
+{
+    const MgTile t = T[blockIdx.x];
+    const MgBDesc &D = *Dp[t.req];
+    if (D.sc->done) return;
+    const MgBLevel &L = D.lv[t.level];
+    MgRows a = mg_tile_rows(D, t.level);
+    if (OP == OP_RESID) {
+        a.v = L.x;
+        a.b = L.b;
+        a.o1 = L.r;
+    } else if (OP == OP_CHEB_POST1) {
+        a.v = L.x;
+        a.b = L.b;
+        a.o1 = L.d;
+        a.o2 = L.r;
+    } else if (OP == OP_CHEB_POST2) {
+        a.v = L.r;
+        a.b = L.b;
+        a.d = L.d;
+        a.o1 = L.x;
+        a.part = t.level == 0 ? D.part : nullptr;
+    } else if (OP == OP_PCG_Q) {
+        a.v = D.p;
+        a.o1 = D.q;
+        a.part = D.part;
+    }
+    mg_rows_body<OP>(a, t.row0, t.row0 / KB);
+}
+
+__global__ __launch_bounds__(KB) void k_mgb_pre(const MgTile *__restrict__ T, const MgBDesc *const *__restrict__ Dp)
+{
+    const MgTile t = T[blockIdx.x];
+    const MgBDesc &D = *Dp[t.req];
+    if (D.sc->done) return;
+    const MgBLevel &L = D.lv[t.level];
+    MgRows a = mg_tile_rows(D, t.level);
+    a.b = L.b;
+    a.o1 = L.x;
+    mg_pre_body(a, t.row0);
+}
+
+// tiles over the rows of coarse level t.level (> 0)
+__global__ __launch_bounds__(KB) void k_mgb_restrict(const MgTile *__restrict__ T,
+                                                     const MgBDesc *const *__restrict__ Dp)
+{
+    const MgTile t = T[blockIdx.x];
+    const MgBDesc &D = *Dp[t.req];
+    if (D.sc->done) return;
+    const MgBLevel &F = D.lv[t.level - 1], &C = D.lv[t.level];
+    const int64_t i = t.row0 + threadIdx.x;
+    if (i >= C.n) return;
+    if (F.fe == GRMP_FE_H1P2) restrict_p2_one(F.N, F.r, C.fixed, C.b, i);
+    else restrict_p1_one(F.N, C.N, F.ncomp, F.r, C.fixed, C.b, i);
+}
+
+// items of fine level t.level: its rows, or for P2 max(nodes, cells)
+__global__ __launch_bounds__(KB) void k_mgb_prolong(const MgTile *__restrict__ T,
+                                                    const MgBDesc *const *__restrict__ Dp)
+{
+    const MgTile t = T[blockIdx.x];
+    const MgBDesc &D = *Dp[t.req];
+    if (D.sc->done) return;
+    const MgBLevel &F = D.lv[t.level], &C = D.lv[t.level + 1];
+    const int64_t i = t.row0 + threadIdx.x;
+    if (F.fe == GRMP_FE_H1P2) {
+        const int64_t nn = (int64_t)(F.N + 1) * (F.N + 1), nc = 2ll * F.N * F.N;
+        if (i < (nn > nc ? nn : nc)) prolong_p2_one(F.N, F.cells, F.fsign, C.x, F.fixed, F.x, i);
+    } else if (i < F.n) {
+        prolong_p1_one(F.N, C.N, F.ncomp, C.x, F.fixed, F.x, i);
+    }
+}
+
+// level-0 tiles: x += alpha p, r -= alpha q, partials of r.r
+__global__ __launch_bounds__(KB) void k_mgb_axpy(const MgTile *__restrict__ T, const MgBDesc *const *__restrict__ Dp)
+{
+    const MgTile t = T[blockIdx.x];
+    const MgBDesc &D = *Dp[t.req];
+    if (D.sc->done) return;
+    const double al = D.sc->alpha;
+    const int64_t i = t.row0 + threadIdx.x;
+    double pa = 0.0, pb = 0.0;
+    if (i < D.lv[0].n) {
+        D.x[i] += al * D.p[i];
+        const double ri = D.r[i] - al * D.q[i];
+        D.r[i] = ri;
+        pa = ri * ri;
+    }
+    block_sum2(pa, pb);
+    if (threadIdx.x == 0) D.part[t.row0 / KB] = pa;
+}
+
+__global__ __launch_bounds__(KB) void k_mgb_update_p(const MgTile *__restrict__ T,
+                                                     const MgBDesc *const *__restrict__ Dp)
+{
+    const MgTile t = T[blockIdx.x];
+    const MgBDesc &D = *Dp[t.req];
+    if (D.sc->done) return;
+    const double be = D.sc->beta;
+    const int64_t i = t.row0 + threadIdx.x;
+    if (i < D.lv[0].n) D.p[i] = D.z[i] + be * D.p[i];
+}
+
+// one block per request: the fixed-order sum of its partials, then step W
+template <int W>
+__global__ __launch_bounds__(KB) void k_mgb_red(const MgBDesc *const *__restrict__ Dp, const int32_t *__restrict__ reqs)
+{
+    const MgBDesc &D = *Dp[reqs[blockIdx.x]];
+    if (D.sc->done) return;
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < D.nb0; k += KB) a += D.part[k];
+    block_sum2(a, b);
+    if (threadIdx.x == 0) mg_scalar<W>(D.sc, a, b);
+}
+
+__global__ __launch_bounds__(MG_BT) void k_mgb_vcycle_block(const MgBDesc *const *__restrict__ Dp,
+                                                            const int32_t *__restrict__ reqs)
+{
+    const MgBDesc &D = *Dp[reqs[blockIdx.x]];
+    if (D.sc->done) return;
+    blk_vcycle(D);
+}
+
+__global__ __launch_bounds__(MG_BT) void k_mgb_pcg_block(const MgBDesc *const *__restrict__ Dp,
+                                                         const int32_t *__restrict__ reqs, int iters)
+{
+    const MgBDesc &D = *Dp[reqs[blockIdx.x]];
+    mg_pcg_block_body(D, D.sc, iters);
+}
+
+__global__ void k_mgb_status(const MgBDesc *const *__restrict__ Dp, int n, MgStatus *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const MgScal *sc = Dp[i]->sc;
+    out[i].relres = sc->relres;
+    out[i].it = sc->it;
+    out[i].done = sc->done;
+    out[i].status = sc->status;
+    out[i].pad = 0;
 }
 
 // ------------------------------------------------------------- the job
@@ -888,6 +1071,7 @@ static void fill_desc(MgJob *j)
     D.q = j->q;
     D.rhs = j->rhs;
     D.part = j->part;
+    D.sc = j->sc;
 }
 
 int mg_job_create(grmp_ctx *ctx, hipStream_t st, const MgLevelIn *in, int nlev, bool profile, int block_rows,
@@ -1150,6 +1334,10 @@ static int mg_iterate_grid(MgJob *j, int iters)
 // three times and only while each check halves the previous one.
 bool mg_job_block(const MgJob *j) { return j->first_small == 0; }
 
+// moves a set-up job to another stream (the caller has ordered it after the
+// job's work on the old one); its later launches and its release go there
+void mg_job_set_stream(MgJob *j, hipStream_t st) { j->st = st; }
+
 int mg_job_verify(MgJob *j)
 {
     if (j->first_small == 0) return GRMP_OK;  // k_mg_pcg_block checks in place
@@ -1233,4 +1421,224 @@ void mg_job_destroy(MgJob *j)
     if (j->ev) (void)hipEventDestroy(j->ev);
     if (j->buf) (void)hipFreeAsync(j->buf, j->st);
     delete j;
+}
+
+// ---------------------------------------------------- lockstep batch (host)
+struct MgSeg {
+    int64_t off = 0, cnt = 0;
+};
+
+struct MgBatch {
+    grmp_ctx *ctx = nullptr;
+    hipStream_t st = nullptr;
+    bool profile = false;
+    // device tables and their pinned staging (rewritten each run, after the previous run's wait)
+    MgTile *dtiles = nullptr, *htiles = nullptr;
+    size_t tcap = 0;
+    const MgBDesc **ddesc = nullptr, **hdesc = nullptr;
+    int32_t *dreq = nullptr, *hreq = nullptr;
+    MgStatus *dstat = nullptr, *hstat = nullptr;
+    int rcap = 0;
+    hipEvent_t ev = nullptr;
+    // profile: event pairs around each batched residual launch, with its bytes
+    std::vector<hipEvent_t> pev;
+    std::vector<double> pbytes;
+    double prof_s = 0.0, prof_bytes = 0.0;
+    int64_t prof_launches = 0;
+};
+
+int mg_batch_create(grmp_ctx *ctx, hipStream_t st, bool profile, MgBatch **out)
+{
+    MgBatch *b = new MgBatch();
+    b->ctx = ctx;
+    b->st = st;
+    b->profile = profile;
+    if (hipEventCreateWithFlags(&b->ev, hipEventDisableTiming) != hipSuccess) {
+        delete b;
+        return grmp_fail(ctx, GRMP_ERR_HIP, "multigrid batch: event");
+    }
+    *out = b;
+    return GRMP_OK;
+}
+
+static int batch_reserve(MgBatch *b, size_t tiles, int reqs)
+{
+    grmp_ctx *ctx = b->ctx;
+    if (tiles > b->tcap) {
+        const size_t cap = std::max(tiles, b->tcap * 2);
+        if (b->dtiles) (void)hipFreeAsync(b->dtiles, b->st);
+        if (b->htiles) {
+            (void)hipStreamSynchronize(b->st);
+            (void)hipHostFree(b->htiles);
+        }
+        b->dtiles = nullptr;
+        b->htiles = nullptr;
+        GRMP_HIP(ctx, hipMallocAsync((void **)&b->dtiles, cap * sizeof(MgTile), b->st));
+        GRMP_HIP(ctx, hipHostMalloc((void **)&b->htiles, cap * sizeof(MgTile), hipHostMallocDefault));
+        b->tcap = cap;
+    }
+    if (reqs > b->rcap) {
+        const int cap = std::max(reqs, 2 * b->rcap);
+        (void)hipStreamSynchronize(b->st);
+        if (b->ddesc) (void)hipFree((void *)b->ddesc);
+        if (b->dreq) (void)hipFree(b->dreq);
+        if (b->dstat) (void)hipFree(b->dstat);
+        if (b->hdesc) (void)hipHostFree((void *)b->hdesc);
+        if (b->hreq) (void)hipHostFree(b->hreq);
+        if (b->hstat) (void)hipHostFree(b->hstat);
+        GRMP_HIP(ctx, hipMalloc((void **)&b->ddesc, cap * sizeof(MgBDesc *)));
+        GRMP_HIP(ctx, hipMalloc((void **)&b->dreq, cap * sizeof(int32_t)));
+        GRMP_HIP(ctx, hipMalloc((void **)&b->dstat, cap * sizeof(MgStatus)));
+        GRMP_HIP(ctx, hipHostMalloc((void **)&b->hdesc, cap * sizeof(MgBDesc *), hipHostMallocDefault));
+        GRMP_HIP(ctx, hipHostMalloc((void **)&b->hreq, cap * sizeof(int32_t), hipHostMallocDefault));
+        GRMP_HIP(ctx, hipHostMalloc((void **)&b->hstat, cap * sizeof(MgStatus), hipHostMallocDefault));
+        b->rcap = cap;
+    }
+    return GRMP_OK;
+}
+
+static inline unsigned ublk(int64_t n) { return (unsigned)n; }
+
+// Enqueues iters lockstep iterations of the n jobs (all set up on the
+// batch's stream), then a status copy for each job into mg_batch_status().
+int mg_batch_run(MgBatch *b, MgJob *const *jobs, int n, int iters)
+{
+    grmp_ctx *ctx = b->ctx;
+    hipStream_t st = b->st;
+    std::vector<int> grid, blk;
+    int Fmax = 0;
+    for (int i = 0; i < n; ++i) {
+        if (jobs[i]->first_small == 0) blk.push_back(i);
+        else {
+            grid.push_back(i);
+            Fmax = std::max(Fmax, jobs[i]->first_small);
+        }
+    }
+    // tile segments: level-0 rows; per grid level l: its rows, the next
+    // level's rows (restriction), its prolongation items
+    std::vector<MgTile> T;
+    auto add = [&](MgSeg &sg, int lev, bool next, bool items) {
+        sg.off = (int64_t)T.size();
+        for (int g : grid) {
+            const MgJob *j = jobs[g];
+            if (j->first_small <= lev) continue;
+            const int L = next ? lev + 1 : lev;
+            int64_t cnt = j->lv[L].n;
+            if (items && j->lv[lev].in.fe == GRMP_FE_H1P2) {
+                const int64_t N = j->lv[lev].in.N;
+                cnt = std::max((N + 1) * (N + 1), 2 * N * N);
+            }
+            for (int64_t r0 = 0; r0 < cnt; r0 += KB) T.push_back(MgTile{g, L, (int32_t)r0, 0});
+        }
+        sg.cnt = (int64_t)T.size() - sg.off;
+    };
+    std::vector<MgSeg> lev(Fmax), res(Fmax), pro(Fmax);
+    for (int l = 0; l < Fmax; ++l) {
+        add(lev[l], l, false, false);
+        add(res[l], l, true, false);
+        add(pro[l], l, false, true);
+    }
+    GRMP_TRY(batch_reserve(b, std::max<size_t>(T.size(), 1), std::max(n, 1)));
+    memcpy(b->htiles, T.data(), T.size() * sizeof(MgTile));
+    for (int i = 0; i < n; ++i) b->hdesc[i] = jobs[i]->ddesc;
+    for (size_t k = 0; k < grid.size(); ++k) b->hreq[k] = grid[k];
+    for (size_t k = 0; k < blk.size(); ++k) b->hreq[grid.size() + k] = blk[k];
+    if (!T.empty())
+        GRMP_HIP(ctx, hipMemcpyAsync(b->dtiles, b->htiles, T.size() * sizeof(MgTile), hipMemcpyHostToDevice, st));
+    GRMP_HIP(ctx, hipMemcpyAsync((void *)b->ddesc, (const void *)b->hdesc, n * sizeof(MgBDesc *), hipMemcpyHostToDevice,
+                                 st));
+    GRMP_HIP(ctx, hipMemcpyAsync(b->dreq, b->hreq, n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    const MgBDesc *const *D = b->ddesc;
+    const int32_t *G = b->dreq, *Bk = b->dreq + grid.size();
+    const int ng = (int)grid.size(), nbk = (int)blk.size();
+    auto tiles = [&](const MgSeg &sg) { return b->dtiles + sg.off; };
+    for (int k = 0; k < iters && ng > 0; ++k) {
+        const MgSeg &L0 = lev[0];
+        hipLaunchKernelGGL(k_mgb_rows<OP_PCG_Q>, dim3(ublk(L0.cnt)), dim3(KB), 0, st, tiles(L0), D);
+        hipLaunchKernelGGL(k_mgb_red<RED_ALPHA>, dim3(ng), dim3(KB), 0, st, D, G);
+        hipLaunchKernelGGL(k_mgb_axpy, dim3(ublk(L0.cnt)), dim3(KB), 0, st, tiles(L0), D);
+        hipLaunchKernelGGL(k_mgb_red<RED_CONV>, dim3(ng), dim3(KB), 0, st, D, G);
+        for (int l = 0; l < Fmax; ++l) {
+            if (lev[l].cnt == 0) continue;
+            hipLaunchKernelGGL(k_mgb_pre, dim3(ublk(lev[l].cnt)), dim3(KB), 0, st, tiles(lev[l]), D);
+            if (b->profile) {
+                hipEvent_t e0, e1;
+                (void)hipEventCreate(&e0);
+                (void)hipEventCreate(&e1);
+                hipExtLaunchKernelGGL(k_mgb_rows<OP_RESID>, dim3(ublk(lev[l].cnt)), dim3(KB), 0, st, e0, e1, 0,
+                                      tiles(lev[l]), D);
+                b->pev.push_back(e0);
+                b->pev.push_back(e1);
+                double bytes = 0.0;
+                for (int g : grid)
+                    if (jobs[g]->first_small > l) {
+                        const MgLevel &Lv = jobs[g]->lv[l];
+                        bytes += (double)Lv.nnz * 12.0 + 4.0 * (Lv.n + 1) + 25.0 * Lv.n;
+                    }
+                b->pbytes.push_back(bytes);
+            } else {
+                hipLaunchKernelGGL(k_mgb_rows<OP_RESID>, dim3(ublk(lev[l].cnt)), dim3(KB), 0, st, tiles(lev[l]), D);
+            }
+            hipLaunchKernelGGL(k_mgb_restrict, dim3(ublk(res[l].cnt)), dim3(KB), 0, st, tiles(res[l]), D);
+        }
+        hipLaunchKernelGGL(k_mgb_vcycle_block, dim3(ng), dim3(MG_BT), 0, st, D, G);
+        for (int l = Fmax - 1; l >= 0; --l) {
+            if (lev[l].cnt == 0) continue;
+            hipLaunchKernelGGL(k_mgb_prolong, dim3(ublk(pro[l].cnt)), dim3(KB), 0, st, tiles(pro[l]), D);
+            hipLaunchKernelGGL(k_mgb_rows<OP_CHEB_POST1>, dim3(ublk(lev[l].cnt)), dim3(KB), 0, st, tiles(lev[l]), D);
+            hipLaunchKernelGGL(k_mgb_rows<OP_CHEB_POST2>, dim3(ublk(lev[l].cnt)), dim3(KB), 0, st, tiles(lev[l]), D);
+        }
+        hipLaunchKernelGGL(k_mgb_red<RED_BETA>, dim3(ng), dim3(KB), 0, st, D, G);
+        hipLaunchKernelGGL(k_mgb_update_p, dim3(ublk(L0.cnt)), dim3(KB), 0, st, tiles(L0), D);
+    }
+    if (nbk > 0) hipLaunchKernelGGL(k_mgb_pcg_block, dim3(nbk), dim3(MG_BT), 0, st, D, Bk, iters);
+    hipLaunchKernelGGL(k_mgb_status, dim3(nblk(n, 256)), dim3(256), 0, st, D, n, b->dstat);
+    GRMP_HIP(ctx, hipMemcpyAsync(b->hstat, b->dstat, n * sizeof(MgStatus), hipMemcpyDeviceToHost, st));
+    GRMP_HIP(ctx, hipEventRecord(b->ev, st));
+    GRMP_HIP(ctx, hipGetLastError());
+    return GRMP_OK;
+}
+
+int mg_batch_wait(MgBatch *b)
+{
+    GRMP_HIP(b->ctx, hipEventSynchronize(b->ev));
+    for (size_t k = 0; k + 1 < b->pev.size(); k += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, b->pev[k], b->pev[k + 1]) == hipSuccess) {
+            b->prof_s += ms * 1e-3;
+            b->prof_bytes += b->pbytes[k / 2];
+            b->prof_launches += 1;
+        }
+        (void)hipEventDestroy(b->pev[k]);
+        (void)hipEventDestroy(b->pev[k + 1]);
+    }
+    b->pev.clear();
+    b->pbytes.clear();
+    return GRMP_OK;
+}
+
+const MgStatus *mg_batch_status(const MgBatch *b) { return b->hstat; }
+
+void mg_batch_profile(const MgBatch *b, double *seconds, double *bytes, int64_t *launches)
+{
+    *seconds = b->prof_s;
+    *bytes = b->prof_bytes;
+    *launches = b->prof_launches;
+}
+
+void mg_batch_destroy(MgBatch *b)
+{
+    if (!b) return;
+    (void)hipStreamSynchronize(b->st);
+    if (b->dtiles) (void)hipFree(b->dtiles);
+    if (b->htiles) (void)hipHostFree(b->htiles);
+    if (b->ddesc) (void)hipFree((void *)b->ddesc);
+    if (b->dreq) (void)hipFree(b->dreq);
+    if (b->dstat) (void)hipFree(b->dstat);
+    if (b->hdesc) (void)hipHostFree((void *)b->hdesc);
+    if (b->hreq) (void)hipHostFree(b->hreq);
+    if (b->hstat) (void)hipHostFree(b->hstat);
+    for (hipEvent_t e : b->pev) (void)hipEventDestroy(e);
+    if (b->ev) (void)hipEventDestroy(b->ev);
+    delete b;
 }

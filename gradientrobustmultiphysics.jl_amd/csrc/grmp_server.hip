@@ -80,6 +80,8 @@ struct Template {
     int64_t bytes = 0;
     uint64_t last_use = 0;
     grmp_lform lf;
+    bool ready = true;   // false while a multigrid setup thread builds it
+    int build_rc = 0;
 };
 
 struct Job {
@@ -104,6 +106,7 @@ struct Job {
     int mslot = -1;
     int mg_it = 0;         // iteration count and residual at the previous status
     double mg_rel = 0.0;
+    hipEvent_t mg_ready = nullptr;  // lockstep batches: end of the job's setup
 };
 
 struct grmp_server {
@@ -135,6 +138,11 @@ struct grmp_server {
     std::vector<grmp_ctx *> mg_ctx;
     std::vector<std::thread> mg_workers;
     std::mutex tpl_mu;
+    // lockstep batches: setup threads move requests from queue to ready
+    std::deque<Job *> mg_readyq;
+    int mg_setting_up = 0, mg_in_batch = 0;
+    std::condition_variable cv_ready;
+    std::condition_variable cv_tpl;  // a shape under construction became ready
     MgStatus *mg_host = nullptr;
     double prof_s = 0.0, prof_bytes = 0.0;
     int64_t prof_launches = 0;
@@ -152,9 +160,8 @@ static void template_free(Template *t)
     delete t;
 }
 
-static int template_make(grmp_server *S, const grmp_request &r, Template **out)
+static int template_make(grmp_server *S, grmp_ctx *c, const grmp_request &r, Template **out)
 {
-    grmp_ctx *c = S->sctx;
     Template *t = new Template();
     t->kind = r.kind;
     t->fe = r.fe;
@@ -216,7 +223,8 @@ static void templates_trim(grmp_server *S, int64_t limit)
     while (S->tpl_bytes > limit) {
         Template *victim = nullptr;
         for (auto &kv : S->templates)
-            if (kv.second->refs == 0 && (!victim || kv.second->last_use < victim->last_use)) victim = kv.second;
+            if (kv.second->refs == 0 && kv.second->ready && (!victim || kv.second->last_use < victim->last_use))
+                victim = kv.second;
         if (!victim) return;
         S->templates.erase(std::make_tuple(victim->kind, victim->fe, victim->N));
         S->tpl_bytes -= victim->bytes;
@@ -231,7 +239,7 @@ static int template_get(grmp_server *S, const grmp_request &r, Template **out)
     if (it == S->templates.end()) {
         Template *t = nullptr;
         double t0 = wall_seconds();
-        GRMP_TRY(template_make(S, r, &t));
+        GRMP_TRY(template_make(S, S->sctx, r, &t));
         S->template_s += wall_seconds() - t0;
         S->ntemplates++;
         it = S->templates.emplace(key, t).first;
@@ -469,6 +477,8 @@ static void mg_release(grmp_server *S, Job *j)
 {
     mg_job_destroy(j->mg);
     j->mg = nullptr;
+    if (j->mg_ready) (void)hipEventDestroy(j->mg_ready);
+    j->mg_ready = nullptr;
     {
         std::lock_guard<std::mutex> g(S->tpl_mu);
         for (Template *t : j->mtpl) t->refs--;
@@ -477,14 +487,82 @@ static void mg_release(grmp_server *S, Job *j)
     S->mem_used -= j->footprint;
 }
 
+// A shape for a multigrid job, with a reference taken.  A missing shape is
+// built by the calling thread on its own context, outside the cache lock, and
+// made complete on the device before others use it; threads that need the
+// same shape meanwhile wait for it.
+static int mg_template(grmp_server *S, grmp_ctx *c, const grmp_request &r, Template **out)
+{
+    auto key = std::make_tuple(r.kind, r.kind == GRMP_REQ_ELASTICITY ? GRMP_FE_H1P1 : r.fe, r.N);
+    Template *t = nullptr;
+    {
+        std::unique_lock<std::mutex> lk(S->tpl_mu);
+        auto it = S->templates.find(key);
+        if (it != S->templates.end()) {
+            t = it->second;
+            t->refs++;
+            t->last_use = ++S->use_clock;
+            S->cv_tpl.wait(lk, [&] { return t->ready; });
+            if (t->build_rc) {  // the build failed; the entry already left the cache
+                const int rc = t->build_rc;
+                if (--t->refs == 0) delete t;
+                return rc;
+            }
+            *out = t;
+            return GRMP_OK;
+        }
+        t = new Template();  // placeholder: found by others, not ready
+        t->ready = false;
+        t->refs = 1;
+        t->last_use = ++S->use_clock;
+        S->templates.emplace(key, t);
+    }
+    const double t0 = wall_seconds();
+    Template *b = nullptr;
+    int rc = template_make(S, c, r, &b);
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = grmp_fail(c, GRMP_ERR_HIP, "multigrid shape build");
+    std::lock_guard<std::mutex> g(S->tpl_mu);
+    if (rc) {
+        // failed: waiting threads see the code; the entry leaves the cache
+        t->build_rc = rc;
+        t->ready = true;
+        t->refs--;
+        S->templates.erase(key);
+        S->cv_tpl.notify_all();
+        if (t->refs == 0) delete t;  // else the last waiter deletes it
+        if (b) template_free(b);
+        return rc;
+    }
+    // move the built objects into the placeholder others already hold
+    t->kind = b->kind;
+    t->fe = b->fe;
+    t->N = b->N;
+    t->mesh = b->mesh;
+    t->space = b->space;
+    t->A = b->A;
+    t->kappa0 = b->kappa0;
+    t->bytes = b->bytes;
+    t->lf = b->lf;
+    b->mesh = nullptr;
+    b->space = nullptr;
+    b->A = nullptr;
+    b->kappa0 = nullptr;
+    delete b;
+    t->ready = true;
+    S->tpl_bytes += t->bytes;
+    S->ntemplates++;
+    S->template_s += wall_seconds() - t0;
+    S->cv_tpl.notify_all();
+    *out = t;
+    return GRMP_OK;
+}
+
 // Levels: the request's shape, P1 on the same square when it is P2, then P1
 // on ceil(N/2), ceil(N/4), ... down to N <= 4.
-static int mg_start(grmp_server *S, Job *j, int slot)
+static int mg_prepare(grmp_server *S, Job *j, grmp_ctx *c, bool prof)
 {
-    grmp_ctx *c = S->mg_ctx[slot];
     hipStream_t st = c->stream;
     j->res.t_admit = now(S);
-    j->mslot = slot;
     const grmp_request &r = j->req;
     const int ncomp = r.kind == GRMP_REQ_ELASTICITY ? 2 : 1;
     const int fe0 = r.kind == GRMP_REQ_ELASTICITY ? GRMP_FE_H1P1 : r.fe;
@@ -496,29 +574,17 @@ static int mg_start(grmp_server *S, Job *j, int slot)
         shape.push_back({GRMP_FE_H1P1, N});
     }
     std::vector<MgLevelIn> in(shape.size());
-    {
-        std::lock_guard<std::mutex> g(S->tpl_mu);
-        const int64_t built = S->ntemplates;
-        for (size_t l = 0; l < shape.size(); ++l) {
-            grmp_request rl = r;
-            rl.fe = shape[l].first;
-            rl.N = shape[l].second;
-            Template *t = nullptr;
-            GRMP_TRY(template_get(S, rl, &t));
-            t->refs++;
-            j->mtpl.push_back(t);
-            in[l] = MgLevelIn{t->A, t->space, rl.N, shape[l].first, ncomp};
-        }
-        // patterns built just now were made on the server stream
-        if (S->ntemplates != built) GRMP_HIP(c, hipStreamSynchronize(S->sctx->stream));
+    for (size_t l = 0; l < shape.size(); ++l) {
+        grmp_request rl = r;
+        rl.fe = shape[l].first;
+        rl.N = shape[l].second;
+        Template *t = nullptr;
+        GRMP_TRY(mg_template(S, c, rl, &t));
+        j->mtpl.push_back(t);
+        in[l] = MgLevelIn{t->A, t->space, rl.N, shape[l].first, ncomp};
     }
-    // Profiling times every grid residual launch with an event pair.  Events
-    // cannot be recorded inside a capture, so with graphs on only every
-    // kMgProfileEvery-th request (tickets 1, 1 + k, ...) is profiled, and it
-    // launches its kernels one by one.
-    const bool prof = S->cfg.profile != 0 && (!S->cfg.graph || (j->ticket - 1) % kMgProfileEvery == 0);
     GRMP_TRY(mg_job_create(c, st, in.data(), (int)in.size(), prof, S->cfg.mg_block_rows, S->cfg.mg_coarse_rows,
-                           S->cfg.graph != 0, &j->mg));
+                           S->cfg.graph != 0 && S->cfg.mg_batch == 0, &j->mg));
     Template *t0 = j->mtpl[0];
     j->n = t0->A->n;
     grmp_vector kv{c, t0->mesh->ncells, mg_job_kappa0(j->mg)};
@@ -528,9 +594,21 @@ static int mg_start(grmp_server *S, Job *j, int slot)
     GRMP_TRY(linearform_launch(c, t0->space, &t0->lf, rhs, st));
     GRMP_TRY(zero_fixed_launch(t0->A, rhs, st));
     GRMP_TRY(mg_job_setup(j->mg, r.rtol > 0 ? r.rtol : 1e-10, kMgMaxIt));
+    j->state = 2;
+    return GRMP_OK;
+}
+
+static int mg_start(grmp_server *S, Job *j, int slot)
+{
+    j->mslot = slot;
+    // Profiling times every grid residual launch with an event pair.  Events
+    // cannot be recorded inside a capture, so with graphs on only every
+    // kMgProfileEvery-th request (tickets 1, 1 + k, ...) is profiled, and it
+    // launches its kernels one by one.
+    const bool prof = S->cfg.profile != 0 && (!S->cfg.graph || (j->ticket - 1) % kMgProfileEvery == 0);
+    GRMP_TRY(mg_prepare(S, j, S->mg_ctx[slot], prof));
     GRMP_TRY(mg_job_iterate(j->mg, kMgChunk));
     GRMP_TRY(mg_job_status_async(j->mg, &S->mg_host[slot]));
-    j->state = 2;
     return GRMP_OK;
 }
 
@@ -548,10 +626,8 @@ static int mg_chunk(const grmp_server *S, const Job *j, const MgStatus &h)
     return need > 8.0 ? 8 : (int)need;
 }
 
-static void mg_finish(grmp_server *S, Job *j)
+static void mg_finish(grmp_server *S, Job *j, const MgStatus h, grmp_ctx *c)
 {
-    const MgStatus h = S->mg_host[j->mslot];
-    grmp_ctx *c = S->mg_ctx[j->mslot];
     j->res.status = h.status;
     j->res.iterations = h.it;
     j->res.outer = 1;
@@ -669,7 +745,7 @@ static void mg_loop(grmp_server *S, int tid, int nthreads)
                 }
                 continue;
             }
-            mg_finish(S, j);
+            mg_finish(S, j, S->mg_host[s], S->mg_ctx[s]);
             std::lock_guard<std::mutex> g(S->mu);
             j->state = 3;
             act[s] = nullptr;
@@ -686,13 +762,142 @@ static void mg_loop(grmp_server *S, int tid, int nthreads)
     }
 }
 
+// Lockstep batches (mg_batch > 0).  mg_threads setup threads, each with its
+// own stream, take requests from the queue while the requests set up, ready
+// and in the batch stay within mg_batch; they build the shapes and levels and
+// hand the request on with an event.  The batch thread (one stream) adds
+// ready requests to the batch, which advances kMgChunk iterations per run;
+// requests leave between runs when done.
+static void mg_setup_loop(grmp_server *S, int tid)
+{
+    grmp_ctx *c = S->mg_ctx[tid];
+    for (;;) {
+        Job *j = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv.wait(lk, [&] {
+                if (S->stop && S->queue.empty()) return true;
+                if (S->queue.empty() || S->fatal) return S->stop;
+                if (S->mg_in_batch + (int)S->mg_readyq.size() + S->mg_setting_up >= S->cfg.mg_batch) return false;
+                Job *f = S->queue.front();
+                return !(S->mem_used > 0 && S->mem_used + f->footprint > S->cfg.memory_budget);
+            });
+            if (S->queue.empty() || S->fatal) return;
+            j = S->queue.front();
+            S->queue.pop_front();
+            S->mem_used += j->footprint;
+            S->mg_setting_up++;
+        }
+        int rc = mg_prepare(S, j, c, false);
+        if (!rc && hipEventCreateWithFlags(&j->mg_ready, hipEventDisableTiming) != hipSuccess) rc = GRMP_ERR_HIP;
+        if (!rc && hipEventRecord(j->mg_ready, c->stream) != hipSuccess) rc = GRMP_ERR_HIP;
+        if (rc) (void)hipStreamSynchronize(c->stream);
+        if (!rc && S->fatal) {  // the batch ended on a device error meanwhile
+            (void)hipStreamSynchronize(c->stream);
+            rc = S->fatal;
+        }
+        std::lock_guard<std::mutex> g(S->mu);
+        S->mg_setting_up--;
+        if (rc) {
+            mg_release(S, j);
+            j->res.status = rc;
+            j->res.t_done = now(S);
+            j->state = 3;
+            S->cv_done.notify_all();
+            S->cv.notify_all();
+            S->cv_ready.notify_all();
+            continue;
+        }
+        S->mg_readyq.push_back(j);
+        S->cv_ready.notify_all();
+    }
+}
+
+static void mg_batch_loop(grmp_server *S)
+{
+    grmp_ctx *c = S->mg_ctx[0];
+    MgBatch *B = nullptr;
+    std::vector<Job *> act;
+    int rc = mg_batch_create(c, c->stream, S->cfg.profile != 0, &B);
+    if (rc) {
+        mg_fatal(S, act, rc);
+        S->cv.notify_all();
+        return;
+    }
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            if (act.empty())
+                S->cv_ready.wait(lk, [&] {
+                    return !S->mg_readyq.empty() || (S->stop && S->queue.empty() && S->mg_setting_up == 0);
+                });
+            if (act.empty() && S->mg_readyq.empty()) break;
+            while (!S->mg_readyq.empty()) {
+                Job *j = S->mg_readyq.front();
+                S->mg_readyq.pop_front();
+                act.push_back(j);
+                S->mg_in_batch++;
+                // the batch stream waits for the job's setup, then owns it
+                (void)hipStreamWaitEvent(c->stream, j->mg_ready, 0);
+                mg_job_set_stream(j->mg, c->stream);
+            }
+        }
+        std::vector<MgJob *> jobs(act.size());
+        for (size_t k = 0; k < act.size(); ++k) jobs[k] = act[k]->mg;
+        rc = mg_batch_run(B, jobs.data(), (int)jobs.size(), kMgChunk);
+        if (!rc) rc = mg_batch_wait(B);
+        if (rc) break;
+        const MgStatus *st = mg_batch_status(B);
+        std::vector<Job *> keep;
+        for (size_t k = 0; k < act.size(); ++k) {
+            Job *j = act[k];
+            const MgStatus h = st[k];
+            if (h.done == 1) {
+                mg_finish(S, j, h, c);
+                std::lock_guard<std::mutex> g(S->mu);
+                j->state = 3;
+                S->mg_in_batch--;
+                S->cv_done.notify_all();
+                S->cv.notify_all();
+                continue;
+            }
+            if (h.done == 2 && (rc = mg_job_verify(j->mg))) break;
+            keep.push_back(j);
+        }
+        if (rc) break;
+        act.swap(keep);
+        std::lock_guard<std::mutex> g(S->tpl_mu);
+        templates_trim(S, S->cfg.memory_budget / 2);
+    }
+    if (rc) {
+        {
+            std::lock_guard<std::mutex> g(S->mu);
+            S->mg_in_batch -= (int)act.size();
+            for (Job *j : S->mg_readyq) act.push_back(j);  // set up, not yet in the batch
+            S->mg_readyq.clear();
+        }
+        mg_fatal(S, act, rc);
+        S->cv.notify_all();
+    }
+    double sec = 0.0, bytes = 0.0;
+    int64_t nl = 0;
+    mg_batch_profile(B, &sec, &bytes, &nl);
+    {
+        std::lock_guard<std::mutex> g(S->mu);
+        S->prof_s += sec;
+        S->prof_bytes += bytes;
+        S->prof_launches += nl;
+    }
+    mg_batch_destroy(B);
+}
+
 // ------------------------------------------------------------- plain loop
 static int plain_one(grmp_server *S, Job *j)
 {
     grmp_ctx *c = S->sctx;
     j->res.t_admit = now(S);
     Template *t = nullptr;
-    GRMP_TRY(template_make(S, j->req, &t));
+    GRMP_TRY(template_make(S, c, j->req, &t));
     // the pattern above used this request's own coefficient field (same seed)
     grmp_matrix *A = t->A;
     int rc = grmp_matrix_assemble(A, GRMP_VALUES_FP64);
@@ -757,6 +962,7 @@ extern "C" int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *cfg, 
     S->cfg.mg_threads = 4;
     S->cfg.mg_block_rows = 2048;
     S->cfg.mg_coarse_rows = 2048;
+    S->cfg.mg_batch = 256;
     if (cfg) {
         if (cfg->mode) S->cfg.mode = cfg->mode;
         if (cfg->memory_budget > 0) S->cfg.memory_budget = cfg->memory_budget;
@@ -769,6 +975,7 @@ extern "C" int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *cfg, 
         if (cfg->mg_threads > 0) S->cfg.mg_threads = cfg->mg_threads;
         if (cfg->mg_block_rows > 0) S->cfg.mg_block_rows = cfg->mg_block_rows;
         if (cfg->mg_coarse_rows > 0) S->cfg.mg_coarse_rows = cfg->mg_coarse_rows;
+        if (cfg->mg_batch >= 0) S->cfg.mg_batch = std::min(cfg->mg_batch, 4096);
     }
     if (S->cfg.mode < 1 || S->cfg.mode > 3) {
         const int mode = S->cfg.mode;
@@ -810,19 +1017,26 @@ extern "C" int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *cfg, 
             return grmp_fail(ctx, rc, "grmp_server_create: multigrid streams");
         }
         memset(S->mg_host, 0, sizeof(MgStatus) * S->cfg.mg_slots);
-        S->cfg.mg_threads = std::max(1, std::min(S->cfg.mg_threads, S->cfg.mg_slots));
+        S->cfg.mg_threads = std::max(1, std::min(S->cfg.mg_threads, S->cfg.mg_slots - (S->cfg.mg_batch > 0 ? 1 : 0)));
     }
     S->t0 = wall_seconds();
     S->worker = std::thread([S] {
         (void)hipSetDevice(S->sctx->device);
         if (S->cfg.mode == 1) served_loop(S);
+        else if (S->cfg.mode == 3 && S->cfg.mg_batch > 0) mg_batch_loop(S);
         else if (S->cfg.mode == 3) mg_loop(S, 0, S->cfg.mg_threads);
         else plain_loop(S);
     });
-    for (int t = 1; S->cfg.mode == 3 && t < S->cfg.mg_threads; ++t)
+    for (int t = 1; S->cfg.mode == 3 && S->cfg.mg_batch == 0 && t < S->cfg.mg_threads; ++t)
         S->mg_workers.emplace_back([S, t] {
             (void)hipSetDevice(S->sctx->device);
             mg_loop(S, t, S->cfg.mg_threads);
+        });
+    // lockstep batches: setup threads on slots 1 .. mg_threads (slot 0 is the batch's)
+    for (int t = 1; S->cfg.mode == 3 && S->cfg.mg_batch > 0 && t <= S->cfg.mg_threads; ++t)
+        S->mg_workers.emplace_back([S, t] {
+            (void)hipSetDevice(S->sctx->device);
+            mg_setup_loop(S, t);
         });
     *out = S;
     return GRMP_OK;
@@ -909,6 +1123,7 @@ extern "C" int grmp_server_destroy(grmp_server *S)
         std::lock_guard<std::mutex> g(S->mu);
         S->stop = true;
         S->cv.notify_all();
+        S->cv_ready.notify_all();
     }
     if (S->worker.joinable()) S->worker.join();
     for (std::thread &w : S->mg_workers)

@@ -38,7 +38,7 @@ class Server:
 
     def __init__(self, mode="served", memory_budget=64 << 30, max_active=256, graph=True, profile=False, ctx=None,
                  tile_rows=0, fp64_first_sweep=0, mg_slots=0, mg_threads=0, mg_block_rows=0,
-                 mg_coarse_rows=0):
+                 mg_coarse_rows=0, mg_batch=None):
         """mode "served": batched Jacobi-PCG engine with FP32 sweeps; "plain":
         one request at a time, FP64 Jacobi-PCG; "mg": FP64 CG preconditioned
         by a geometric multigrid V-cycle, ``mg_slots`` requests at once."""
@@ -51,6 +51,7 @@ class Server:
         cfg.mg_threads = int(mg_threads)
         cfg.mg_block_rows = int(mg_block_rows)
         cfg.mg_coarse_rows = int(mg_coarse_rows)
+        cfg.mg_batch = -1 if mg_batch is None else int(mg_batch)  # -1: library default
         cfg.memory_budget = int(memory_budget)
         cfg.max_active = int(max_active)
         cfg.graph = 1 if graph else 0

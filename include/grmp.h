@@ -299,6 +299,11 @@ typedef struct grmp_server_config {
                                  workgroup (0 = 2048) */
     int mg_coarse_rows;       /* mode 3: larger requests run their multigrid levels with at most this
                                  many rows inside one workgroup (0 = 2048) */
+    int mg_batch;             /* mode 3: > 0 (default 256; pass -1 for the default) advances up to this
+                                 many requests in lockstep, one launch per phase over all of them, with
+                                 mg_threads setup threads feeding it; 0 = one stream per request
+                                 (mg_slots, mg_threads) */
+    int pad_;
 } grmp_server_config;
 
 typedef struct grmp_kernel_stats {

@@ -37,13 +37,14 @@ SPECS = [(1, 1, 1, 3), (1, 1, 4, 4), (2, 1, 3, 5), (1, 2, 2, 6), (1, 1, 5, 7), (
          (1, 1, 37, 10), (1, 2, 21, 11), (2, 1, 30, 12)]
 
 
-@pytest.mark.parametrize("block_rows", [0, 64])
-def test_mg_matches_lu(grmp, O, block_rows):
+@pytest.mark.parametrize("block_rows,batch", [(0, 0), (64, 0), (0, 16), (64, 16)])
+def test_mg_matches_lu(grmp, O, block_rows, batch):
     """block_rows 0: the default thresholds (these requests run whole inside
     one workgroup, or with their coarse levels there); 64: every level above
-    the coarsest runs as grid kernels."""
-    print(f"[case] multigrid vs LU, block_rows {block_rows}", flush=True)
-    with grmp.Server("mg", mg_block_rows=block_rows, mg_coarse_rows=block_rows) as S:
+    the coarsest runs as grid kernels.  batch 16: the requests advance in
+    lockstep, one launch per phase."""
+    print(f"[case] multigrid vs LU, block_rows {block_rows}, batch {batch}", flush=True)
+    with grmp.Server("mg", mg_block_rows=block_rows, mg_coarse_rows=block_rows, mg_batch=batch) as S:
         ts = [S.submit(*spec, rtol=1e-12) for spec in SPECS]
         for spec, t in zip(SPECS, ts):
             r = S.wait(t)
@@ -90,6 +91,14 @@ def test_mg_concurrent_equals_alone_bitwise(grmp, block_rows):
             assert r.status == 0
             assert r.iterations == alone[spec][0]
             assert np.array_equal(S.solution(t), alone[spec][1]), spec
+    # lockstep batches run the same row bodies and fixed-order reductions
+    with grmp.Server("mg", mg_batch=16, mg_block_rows=block_rows, mg_coarse_rows=block_rows) as S:
+        ts = [S.submit(*spec) for spec in specs + specs]
+        for spec, t in zip(specs + specs, ts):
+            r = S.wait(t)
+            assert r.status == 0
+            assert r.iterations == alone[spec][0]
+            assert np.array_equal(S.solution(t), alone[spec][1]), spec
 
 
 def test_mg_failures_stay_local(grmp):
@@ -97,7 +106,12 @@ def test_mg_failures_stay_local(grmp):
     good = SPECS[4:7]
     with grmp.Server("mg", mg_slots=1) as S:
         alone = {spec: S.solution(t).copy() for spec in good for t in [S.submit(*spec)] if S.wait(t).status == 0}
-    with grmp.Server("mg", mg_slots=3) as S:
+    for kw in ({"mg_slots": 3}, {"mg_batch": 8}):
+        _failures(grmp, good, alone, kw)
+
+
+def _failures(grmp, good, alone, kw):
+    with grmp.Server("mg", **kw) as S:
         tnan = S.submit(1, 1, 16, 9, sigma=float("nan"))
         ts = [S.submit(*spec) for spec in good]
         tbad = S.submit(1, 1, 12, 10, rtol=1e-30)
