@@ -847,6 +847,15 @@ static void mg_batch_loop(grmp_server *S)
         rc = mg_batch_run(B, jobs.data(), (int)jobs.size(), kMgChunk);
         if (!rc) rc = mg_batch_wait(B);
         if (rc) break;
+        if (S->cfg.profile) {  // the batch is the only writer of the totals in this mode
+            double sec = 0.0, bytes = 0.0;
+            int64_t nl = 0;
+            mg_batch_profile(B, &sec, &bytes, &nl);
+            std::lock_guard<std::mutex> g(S->mu);
+            S->prof_s = sec;
+            S->prof_bytes = bytes;
+            S->prof_launches = nl;
+        }
         const MgStatus *st = mg_batch_status(B);
         std::vector<Job *> keep;
         for (size_t k = 0; k < act.size(); ++k) {
@@ -878,15 +887,6 @@ static void mg_batch_loop(grmp_server *S)
         }
         mg_fatal(S, act, rc);
         S->cv.notify_all();
-    }
-    double sec = 0.0, bytes = 0.0;
-    int64_t nl = 0;
-    mg_batch_profile(B, &sec, &bytes, &nl);
-    {
-        std::lock_guard<std::mutex> g(S->mu);
-        S->prof_s += sec;
-        S->prof_bytes += bytes;
-        S->prof_launches += nl;
     }
     mg_batch_destroy(B);
 }
@@ -972,7 +972,7 @@ extern "C" int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *cfg, 
         if (cfg->tile_rows > 0) S->cfg.tile_rows = std::max(64, std::min(cfg->tile_rows, 8192));
         if (cfg->fp64_first_sweep >= 0) S->cfg.fp64_first_sweep = cfg->fp64_first_sweep;
         if (cfg->mg_slots > 0) S->cfg.mg_slots = std::min(cfg->mg_slots, 64);
-        if (cfg->mg_threads > 0) S->cfg.mg_threads = cfg->mg_threads;
+        if (cfg->mg_threads > 0) S->cfg.mg_threads = std::min(cfg->mg_threads, 16);
         if (cfg->mg_block_rows > 0) S->cfg.mg_block_rows = cfg->mg_block_rows;
         if (cfg->mg_coarse_rows > 0) S->cfg.mg_coarse_rows = cfg->mg_coarse_rows;
         if (cfg->mg_batch >= 0) S->cfg.mg_batch = std::min(cfg->mg_batch, 4096);
@@ -1001,6 +1001,8 @@ extern "C" int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *cfg, 
         }
     }
     if (S->cfg.mode == 3) {
+        // lockstep batches use slot 0 for the batch and slots 1 .. mg_threads for setup
+        if (S->cfg.mg_batch > 0) S->cfg.mg_slots = std::max(S->cfg.mg_slots, S->cfg.mg_threads + 1);
         for (int k = 0; k < S->cfg.mg_slots && !rc; ++k) {
             grmp_ctx *c = nullptr;
             rc = grmp_ctx_create(ctx->device, &c);

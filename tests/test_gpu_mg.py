@@ -79,12 +79,12 @@ def test_mg_concurrent_equals_alone_bitwise(grmp, block_rows):
     print(f"[case] multigrid concurrent vs alone, block_rows {block_rows}", flush=True)
     specs = SPECS[4:]
     alone = {}
-    with grmp.Server("mg", mg_slots=1, mg_block_rows=block_rows, mg_coarse_rows=block_rows) as S:
+    with grmp.Server("mg", mg_slots=1, mg_batch=0, mg_block_rows=block_rows, mg_coarse_rows=block_rows) as S:
         for spec in specs:
             t = S.submit(*spec)
             r = S.wait(t)
             alone[spec] = (r.iterations, S.solution(t).copy())
-    with grmp.Server("mg", mg_slots=4, mg_block_rows=block_rows, mg_coarse_rows=block_rows) as S:
+    with grmp.Server("mg", mg_slots=4, mg_batch=0, mg_block_rows=block_rows, mg_coarse_rows=block_rows) as S:
         ts = [S.submit(*spec) for spec in specs + specs]
         for spec, t in zip(specs + specs, ts):
             r = S.wait(t)
@@ -104,9 +104,9 @@ def test_mg_concurrent_equals_alone_bitwise(grmp, block_rows):
 def test_mg_failures_stay_local(grmp):
     print("[case] multigrid failure isolation", flush=True)
     good = SPECS[4:7]
-    with grmp.Server("mg", mg_slots=1) as S:
+    with grmp.Server("mg", mg_slots=1, mg_batch=0) as S:
         alone = {spec: S.solution(t).copy() for spec in good for t in [S.submit(*spec)] if S.wait(t).status == 0}
-    for kw in ({"mg_slots": 3}, {"mg_batch": 8}):
+    for kw in ({"mg_slots": 3, "mg_batch": 0}, {"mg_batch": 8}, {"mg_batch": 4, "mg_slots": 1, "mg_threads": 1}):
         _failures(grmp, good, alone, kw)
 
 
