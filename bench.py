@@ -21,10 +21,11 @@ Reported:
   p50/p99         request latency (arrival -> solution on the host)
   plain           the same metric for plain one-at-a-time FP64 GPU solves,
                   measured on a systematic sample of the stream
-  roofline        mg: the residual kernel on levels of >= 2^21 rows
-                  (k_mg_resid_fine); engine: the engine SpMV.  Algorithmic
-                  bytes / summed durations from HIP events around every
-                  launch of that kernel in the timed run
+  roofline        mg: the batched residual over all requests' rows of a level
+                  (k_mgb_rows<OP_RESID>; with --mg-batch 0 the residual on
+                  levels of >= 2^21 rows, k_mg_resid_fine); engine: the engine
+                  SpMV.  Algorithmic bytes / summed durations from HIP events
+                  around every launch of that kernel in the timed run
   cpu_baseline    the oracle (single-thread assembly + OpenMP PCG) on a
                   bounded sample of the same stream
 """

@@ -825,8 +825,48 @@ static void mg_batch_loop(grmp_server *S)
         return;
     }
     for (;;) {
+        // without setup threads the batch thread sets requests up itself, between runs
+        while (S->cfg.mg_threads == 0 && (int)act.size() < S->cfg.mg_batch) {
+            Job *j = nullptr;
+            {
+                std::lock_guard<std::mutex> g(S->mu);
+                if (S->queue.empty() || S->fatal) break;
+                Job *f = S->queue.front();
+                if (S->mem_used > 0 && S->mem_used + f->footprint > S->cfg.memory_budget) break;
+                S->queue.pop_front();
+                S->mem_used += f->footprint;
+                S->mg_setting_up++;
+                j = f;
+            }
+            rc = mg_prepare(S, j, c, false);
+            std::lock_guard<std::mutex> g(S->mu);
+            S->mg_setting_up--;
+            if (rc == GRMP_ERR_HIP) {
+                act.push_back(j);
+                S->mg_in_batch++;
+                break;
+            }
+            if (rc) {
+                (void)hipStreamSynchronize(c->stream);
+                mg_release(S, j);
+                j->res.status = rc;
+                j->res.t_done = now(S);
+                j->state = 3;
+                S->cv_done.notify_all();
+                rc = GRMP_OK;
+                continue;
+            }
+            act.push_back(j);
+            S->mg_in_batch++;
+        }
+        if (rc) break;
         {
             std::unique_lock<std::mutex> lk(S->mu);
+            if (act.empty() && S->cfg.mg_threads == 0) {
+                if (S->stop && S->queue.empty()) break;
+                if (S->queue.empty()) S->cv.wait(lk, [&] { return S->stop || !S->queue.empty(); });
+                continue;
+            }
             if (act.empty())
                 S->cv_ready.wait(lk, [&] {
                     return !S->mg_readyq.empty() || (S->stop && S->queue.empty() && S->mg_setting_up == 0);
@@ -973,6 +1013,7 @@ extern "C" int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *cfg, 
         if (cfg->fp64_first_sweep >= 0) S->cfg.fp64_first_sweep = cfg->fp64_first_sweep;
         if (cfg->mg_slots > 0) S->cfg.mg_slots = std::min(cfg->mg_slots, 64);
         if (cfg->mg_threads > 0) S->cfg.mg_threads = std::min(cfg->mg_threads, 16);
+        if (cfg->mg_threads < 0) S->cfg.mg_threads = 0;  // batches: set up inline
         if (cfg->mg_block_rows > 0) S->cfg.mg_block_rows = cfg->mg_block_rows;
         if (cfg->mg_coarse_rows > 0) S->cfg.mg_coarse_rows = cfg->mg_coarse_rows;
         if (cfg->mg_batch >= 0) S->cfg.mg_batch = std::min(cfg->mg_batch, 4096);
@@ -1019,7 +1060,8 @@ extern "C" int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *cfg, 
             return grmp_fail(ctx, rc, "grmp_server_create: multigrid streams");
         }
         memset(S->mg_host, 0, sizeof(MgStatus) * S->cfg.mg_slots);
-        S->cfg.mg_threads = std::max(1, std::min(S->cfg.mg_threads, S->cfg.mg_slots - (S->cfg.mg_batch > 0 ? 1 : 0)));
+        S->cfg.mg_threads = S->cfg.mg_batch > 0 ? std::min(S->cfg.mg_threads, S->cfg.mg_slots - 1)
+                                                 : std::max(1, std::min(S->cfg.mg_threads, S->cfg.mg_slots));
     }
     S->t0 = wall_seconds();
     S->worker = std::thread([S] {

@@ -106,7 +106,8 @@ def test_mg_failures_stay_local(grmp):
     good = SPECS[4:7]
     with grmp.Server("mg", mg_slots=1, mg_batch=0) as S:
         alone = {spec: S.solution(t).copy() for spec in good for t in [S.submit(*spec)] if S.wait(t).status == 0}
-    for kw in ({"mg_slots": 3, "mg_batch": 0}, {"mg_batch": 8}, {"mg_batch": 4, "mg_slots": 1, "mg_threads": 1}):
+    for kw in ({"mg_slots": 3, "mg_batch": 0}, {"mg_batch": 8}, {"mg_batch": 4, "mg_slots": 1, "mg_threads": 1},
+               {"mg_batch": 4, "mg_threads": -1}):
         _failures(grmp, good, alone, kw)
 
 
