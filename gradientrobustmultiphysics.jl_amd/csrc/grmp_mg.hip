@@ -762,6 +762,13 @@ struct MgTile {
     int32_t req, level, row0, pad;
 };
 
+// a tile is MG_TILE_CHUNKS consecutive 256-row chunks of one level, handled
+// by one workgroup chunk after chunk: the tile and descriptor lookups that
+// start a workgroup are paid once per tile.  Partials stay per 256-row chunk.
+#ifndef MG_TILE_CHUNKS
+#define MG_TILE_CHUNKS 4
+#endif
+
 __device__ __forceinline__ MgRows mg_tile_rows(const MgBDesc &D, int level)
 {
     const MgBLevel &L = D.lv[level];
@@ -815,7 +822,11 @@ __global__ __launch_bounds__(KB) void k_mgb_rows(const MgTile *__restrict__ T, c
         a.o1 = D.q;
         a.part = D.part;
     }
-    mg_rows_body<OP>(a, t.row0, t.row0 / KB);
+    for (int c = 0; c < MG_TILE_CHUNKS; ++c) {
+        const int64_t r0 = t.row0 + (int64_t)c * KB;
+        if (r0 >= L.n) break;
+        mg_rows_body<OP>(a, r0, (int)(r0 / KB));
+    }
 }
 
 // d0 = D^-1 b / theta on the level's rows, the first step of the
@@ -828,8 +839,11 @@ __global__ __launch_bounds__(KB) void k_mgb_first(const MgTile *__restrict__ T, 
     const MgBDesc &D = *Dp[t.req];
     if (D.sc->done) return;
     const MgBLevel &L = D.lv[t.level];
-    const int64_t i = t.row0 + threadIdx.x;
-    if (i < L.n) L.d[i] = L.fixed[i] ? 0.0 : L.dinv[i] * L.b[i] * CHEB_ITHETA(*L.lam);
+    const double ith = CHEB_ITHETA(*L.lam);
+    for (int c = 0; c < MG_TILE_CHUNKS; ++c) {
+        const int64_t i = t.row0 + (int64_t)c * KB + threadIdx.x;
+        if (i < L.n) L.d[i] = L.fixed[i] ? 0.0 : L.dinv[i] * L.b[i] * ith;
+    }
 }
 
 // tiles over the rows of coarse level t.level (> 0)
@@ -840,10 +854,12 @@ __global__ __launch_bounds__(KB) void k_mgb_restrict(const MgTile *__restrict__ 
     const MgBDesc &D = *Dp[t.req];
     if (D.sc->done) return;
     const MgBLevel &F = D.lv[t.level - 1], &C = D.lv[t.level];
-    const int64_t i = t.row0 + threadIdx.x;
-    if (i >= C.n) return;
-    if (F.fe == GRMP_FE_H1P2) restrict_p2_one(F.N, F.r, C.fixed, C.b, i);
-    else restrict_p1_one(F.N, C.N, F.ncomp, F.r, C.fixed, C.b, i);
+    for (int c = 0; c < MG_TILE_CHUNKS; ++c) {
+        const int64_t i = t.row0 + (int64_t)c * KB + threadIdx.x;
+        if (i >= C.n) break;
+        if (F.fe == GRMP_FE_H1P2) restrict_p2_one(F.N, F.r, C.fixed, C.b, i);
+        else restrict_p1_one(F.N, C.N, F.ncomp, F.r, C.fixed, C.b, i);
+    }
 }
 
 // items of fine level t.level: its rows, or for P2 max(nodes, cells)
@@ -854,12 +870,14 @@ __global__ __launch_bounds__(KB) void k_mgb_prolong(const MgTile *__restrict__ T
     const MgBDesc &D = *Dp[t.req];
     if (D.sc->done) return;
     const MgBLevel &F = D.lv[t.level], &C = D.lv[t.level + 1];
-    const int64_t i = t.row0 + threadIdx.x;
-    if (F.fe == GRMP_FE_H1P2) {
-        const int64_t nn = (int64_t)(F.N + 1) * (F.N + 1), nc = 2ll * F.N * F.N;
-        if (i < (nn > nc ? nn : nc)) prolong_p2_one(F.N, F.cells, F.fsign, C.x, F.fixed, F.x, i);
-    } else if (i < F.n) {
-        prolong_p1_one(F.N, C.N, F.ncomp, C.x, F.fixed, F.x, i);
+    for (int c = 0; c < MG_TILE_CHUNKS; ++c) {
+        const int64_t i = t.row0 + (int64_t)c * KB + threadIdx.x;
+        if (F.fe == GRMP_FE_H1P2) {
+            const int64_t nn = (int64_t)(F.N + 1) * (F.N + 1), nc = 2ll * F.N * F.N;
+            if (i < (nn > nc ? nn : nc)) prolong_p2_one(F.N, F.cells, F.fsign, C.x, F.fixed, F.x, i);
+        } else if (i < F.n) {
+            prolong_p1_one(F.N, C.N, F.ncomp, C.x, F.fixed, F.x, i);
+        }
     }
 }
 
@@ -870,16 +888,21 @@ __global__ __launch_bounds__(KB) void k_mgb_axpy(const MgTile *__restrict__ T, c
     const MgBDesc &D = *Dp[t.req];
     if (D.sc->done) return;
     const double al = D.sc->alpha;
-    const int64_t i = t.row0 + threadIdx.x;
-    double pa = 0.0, pb = 0.0;
-    if (i < D.lv[0].n) {
-        D.x[i] += al * D.p[i];
-        const double ri = D.r[i] - al * D.q[i];
-        D.r[i] = ri;
-        pa = ri * ri;
+    const int64_t n = D.lv[0].n;
+    for (int c = 0; c < MG_TILE_CHUNKS; ++c) {
+        const int64_t r0 = t.row0 + (int64_t)c * KB;
+        if (r0 >= n) break;
+        const int64_t i = r0 + threadIdx.x;
+        double pa = 0.0, pb = 0.0;
+        if (i < n) {
+            D.x[i] += al * D.p[i];
+            const double ri = D.r[i] - al * D.q[i];
+            D.r[i] = ri;
+            pa = ri * ri;
+        }
+        block_sum2(pa, pb);
+        if (threadIdx.x == 0) D.part[r0 / KB] = pa;
     }
-    block_sum2(pa, pb);
-    if (threadIdx.x == 0) D.part[t.row0 / KB] = pa;
 }
 
 __global__ __launch_bounds__(KB) void k_mgb_update_p(const MgTile *__restrict__ T,
@@ -889,8 +912,10 @@ __global__ __launch_bounds__(KB) void k_mgb_update_p(const MgTile *__restrict__ 
     const MgBDesc &D = *Dp[t.req];
     if (D.sc->done) return;
     const double be = D.sc->beta;
-    const int64_t i = t.row0 + threadIdx.x;
-    if (i < D.lv[0].n) D.p[i] = D.z[i] + be * D.p[i];
+    for (int c = 0; c < MG_TILE_CHUNKS; ++c) {
+        const int64_t i = t.row0 + (int64_t)c * KB + threadIdx.x;
+        if (i < D.lv[0].n) D.p[i] = D.z[i] + be * D.p[i];
+    }
 }
 
 // one block per request: the fixed-order sum of its partials, then step W
@@ -1534,7 +1559,7 @@ int mg_batch_run(MgBatch *b, MgJob *const *jobs, int n, int iters)
                 const int64_t N = j->lv[lev].in.N;
                 cnt = std::max((N + 1) * (N + 1), 2 * N * N);
             }
-            for (int64_t r0 = 0; r0 < cnt; r0 += KB) T.push_back(MgTile{g, L, (int32_t)r0, 0});
+            for (int64_t r0 = 0; r0 < cnt; r0 += (int64_t)KB * MG_TILE_CHUNKS) T.push_back(MgTile{g, L, (int32_t)r0, 0});
         }
         sg.cnt = (int64_t)T.size() - sg.off;
     };
