@@ -413,6 +413,12 @@ def main():
             "bytes_per_launch": round(st["spmv_bytes"] / max(st["spmv_launches"], 1), 1)}
     if mode == "served":
         roof["traffic_over_algorithmic"] = 1.20  # FETCH_SIZE + WRITE_SIZE passes, profiles/engine_spmv_hbm_traffic.md
+    elif batched:
+        # FETCH_SIZE + WRITE_SIZE passes on the same kernel (profiles/mgb_resid_hbm_traffic.md): HBM bytes
+        # equal the algorithmic bytes, so the measured ratio scales this run's bytes per launch
+        roof["traffic_over_algorithmic"] = 1.00
+        roof["traffic"] = roof["bytes_per_launch"]
+        roof["traffic_basis"] = "PMC ratio 1.00 (256-request stream) x this run's algorithmic bytes per launch"
 
     cpu = None
     if not args.no_cpu:
