@@ -92,11 +92,11 @@ __global__ __launch_bounds__(KB) void k_mr_init(const int32_t *__restrict__ rp, 
                                                 T *__restrict__ v0, T *__restrict__ w0, T *__restrict__ w1, int64_t n,
                                                 double *__restrict__ part)
 {
-    __shared__ double lds[SPMV_CAP];
+    __shared__ double lds[SPMV_CAP64];
     double acc = 0.0, dummy = 0.0;
     for (int64_t r0 = (int64_t)blockIdx.x * KB; r0 < n; r0 += (int64_t)gridDim.x * KB) {
         const int64_t r1 = r0 + KB < n ? r0 + KB : n;
-        const double s = a ? csr_rows_lds<double>(rp, ci, a, x, r0, r1, lds) : 0.0;
+        const double s = a ? csr_rows_lds64(rp, ci, a, x, r0, r1, lds) : 0.0;
         const int64_t i = r0 + threadIdx.x;
         if (i < r1) {
             const double d = (double)dsg[i];
@@ -268,11 +268,11 @@ __global__ __launch_bounds__(KB) void k_mr_resid64(const int32_t *__restrict__ r
                                                    const double *__restrict__ x, double *__restrict__ r, int64_t n,
                                                    double *__restrict__ part)
 {
-    __shared__ double lds[SPMV_CAP];
+    __shared__ double lds[SPMV_CAP64];
     double acc = 0.0, dummy = 0.0;
     for (int64_t r0 = (int64_t)blockIdx.x * KB; r0 < n; r0 += (int64_t)gridDim.x * KB) {
         const int64_t r1 = r0 + KB < n ? r0 + KB : n;
-        const double s = csr_rows_lds<double>(rp, ci, a, x, r0, r1, lds);
+        const double s = csr_rows_lds64(rp, ci, a, x, r0, r1, lds);
         const int64_t i = r0 + threadIdx.x;
         if (i < r1) {
             const double d = dsg[i];

@@ -23,10 +23,10 @@ __global__ __launch_bounds__(KB) void k_spmv(const int32_t *__restrict__ rp, con
                                             const double *__restrict__ v, const double *__restrict__ x,
                                             double *__restrict__ y, int64_t n)
 {
-    __shared__ double lds[SPMV_CAP];
+    __shared__ double lds[SPMV_CAP64];
     for (int64_t r0 = (int64_t)blockIdx.x * KB; r0 < n; r0 += (int64_t)gridDim.x * KB) {
         const int64_t r1 = r0 + KB < n ? r0 + KB : n;
-        const double s = csr_rows_lds<double>(rp, ci, v, x, r0, r1, lds);
+        const double s = csr_rows_lds64(rp, ci, v, x, r0, r1, lds);
         if (r0 + threadIdx.x < r1) y[r0 + threadIdx.x] = s;
     }
 }
@@ -36,10 +36,10 @@ __global__ __launch_bounds__(KB) void k_spmv32(const int32_t *__restrict__ rp, c
                                               const float *__restrict__ v, const double *__restrict__ x,
                                               double *__restrict__ y, int64_t n)
 {
-    __shared__ double lds[SPMV_CAP];
+    __shared__ double lds[SPMV_CAP64];
     for (int64_t r0 = (int64_t)blockIdx.x * KB; r0 < n; r0 += (int64_t)gridDim.x * KB) {
         const int64_t r1 = r0 + KB < n ? r0 + KB : n;
-        const double s = csr_rows_lds<double>(rp, ci, v, x, r0, r1, lds);
+        const double s = csr_rows_lds64(rp, ci, v, x, r0, r1, lds);
         if (r0 + threadIdx.x < r1) y[r0 + threadIdx.x] = s;
     }
 }
@@ -97,11 +97,11 @@ __global__ __launch_bounds__(PB) void k_plain_resid(const int32_t *__restrict__ 
                                                    double *__restrict__ z, double *__restrict__ p, int64_t n,
                                                    double *__restrict__ part)
 {
-    __shared__ double lds[SPMV_CAP];
+    __shared__ double lds[SPMV_CAP64];
     double a0 = 0.0, a1 = 0.0;
     for (int64_t r0 = (int64_t)blockIdx.x * PB; r0 < n; r0 += (int64_t)gridDim.x * PB) {
         const int64_t r1 = r0 + PB < n ? r0 + PB : n;
-        const double s = csr_rows_lds<double>(rp, ci, v, x, r0, r1, lds);
+        const double s = csr_rows_lds64(rp, ci, v, x, r0, r1, lds);
         const int64_t i = r0 + threadIdx.x;
         if (i < r1) {
             double ri = fixed[i] ? 0.0 : b[i] - s;
@@ -126,11 +126,11 @@ __global__ __launch_bounds__(PB) void k_plain_spmv(const int32_t *__restrict__ r
                                                   const double *__restrict__ p, double *__restrict__ q, int64_t n,
                                                   double *__restrict__ part)
 {
-    __shared__ double lds[SPMV_CAP];
+    __shared__ double lds[SPMV_CAP64];
     double a0 = 0.0, a1 = 0.0;
     for (int64_t r0 = (int64_t)blockIdx.x * PB; r0 < n; r0 += (int64_t)gridDim.x * PB) {
         const int64_t r1 = r0 + PB < n ? r0 + PB : n;
-        const double s = csr_rows_lds<double>(rp, ci, v, p, r0, r1, lds);
+        const double s = csr_rows_lds64(rp, ci, v, p, r0, r1, lds);
         const int64_t i = r0 + threadIdx.x;
         if (i < r1) {
             double qi = fixed[i] ? 0.0 : s;
@@ -484,7 +484,7 @@ __device__ __forceinline__ double resid_tile(const KProb &pr, int row0, int row1
     double acc = 0.0;
     for (int r0 = row0; r0 < row1; r0 += KB) {
         const int r1 = r0 + KB < row1 ? r0 + KB : row1;
-        const double s = csr_rows_lds<double>(pr.rowptr, pr.cols, pr.val64, pr.x, r0, r1, lds);
+        const double s = csr_rows_lds64(pr.rowptr, pr.cols, pr.val64, pr.x, r0, r1, lds);
         const int i = r0 + (int)threadIdx.x;
         if (i < r1) {
             double ri = pr.fixed[i] ? 0.0 : pr.b[i] - s;
@@ -497,7 +497,7 @@ __device__ __forceinline__ double resid_tile(const KProb &pr, int row0, int row1
 
 __global__ __launch_bounds__(KB) void k_eng_resid(ENGINE_ARGS)
 {
-    __shared__ double lds[SPMV_CAP];
+    __shared__ double lds[SPMV_CAP64];
     const int nt = *ntiles;
     for (int t = blockIdx.x; t < nt; t += gridDim.x) {
         const KTile tl = T[t];
