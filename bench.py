@@ -197,6 +197,23 @@ def _assembly_rate(grmp, A, values, ncells, nloc, nnodes, dim, reps=3):
                 frac=round(gbs / HBM_PEAK_GBS, 4))
 
 
+def _timestep_rhs_rate(grmp, tcs, dt, reps=20):
+    """Wall time per step right-hand side b = f + M u / dt (one kernel,
+    back-to-back launches) and its algorithmic bytes
+    nnz_M (8 + 4) + 4 (n + 1) + 8 n (u) + 8 n (f) + 8 n (b) + n (fixed mask)."""
+    n, nnz, _ = tcs.AM.info()
+    tcs.rhs(dt)
+    tcs.X.ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        tcs.rhs(dt)
+    tcs.X.ctx.sync()
+    t = (time.perf_counter() - t0) / reps
+    nbytes = nnz * 12 + 4 * (n + 1) + 25 * n
+    gbs = nbytes / t / 1e9
+    return dict(us=round(t * 1e6, 1), bytes=int(nbytes), gbs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4))
+
+
 def run_workload(grmp, which, args, log):
     ctx = grmp.default_context()
     if args.rtol == 1e-9 and "--rtol" not in sys.argv:
@@ -302,6 +319,49 @@ def run_workload(grmp, which, args, log):
                 out["spmv_fp64"] = _spmv_rate(grmp, A, V64)
         res["rel_diff_fp32_ir_vs_fp64"] = float(np.linalg.norm(xs["fp32_ir"] - xs["fp64"]) / np.linalg.norm(xs["fp64"]))
         out["solve"] = res
+    elif which == "heat":
+        N = args.size or 2048
+        dt = args.dt
+        grid = grmp.grid_unitsquare(N)
+        fes = grmp.FESpace(grmp.H1P1(1), grid)
+        kap = grmp.DeviceVector(ctx, grid.ncells)
+        kap.lognormal(2024, 0.0, 1.0)
+        pde = grmp.PoissonProblem(cell_coefficient=grmp.CellCoefficient(kap))
+        pde.add_rhsdata(0, grmp.LinearForm(grmp.Identity, grmp.DataFunction([1.0])))
+        pde.add_boundarydata(0)
+        X = grmp.FEVector(fes)
+        out["config"] = {"workload": f"2D P1 heat equation u_t - div(kappa grad u) = 1, {N}x{N} triangulated square, "
+                                     f"log-normal per-cell kappa sigma=1, BackwardEuler dt={dt:g}, u0 = 0",
+                         "cells": grid.ncells, "ndofs": fes.ndofs}
+        log(f"[bench] heat: solver setup (mass matrix, rhs data), {fes.ndofs} dofs")
+        t0 = time.perf_counter()
+        tcs = grmp.TimeControlSolver(pde, X, grmp.BackwardEuler, timedependent_equations=[0],
+                                     precision=args.precision, rtol=args.rtol)
+        ctx.sync()
+        t_setup = time.perf_counter() - t0
+        log("[bench] heat: first step (system assembly + solve)")
+        t0 = time.perf_counter()
+        i1 = tcs.advance(dt)
+        ctx.sync()
+        t_first = time.perf_counter() - t0
+        log(f"[bench] heat: {args.timesteps} steps at fixed dt (right-hand side + solve per step)")
+        its, t0 = 0, time.perf_counter()
+        for _ in range(args.timesteps):
+            info = tcs.advance(dt)
+            its += int(info.iterations)
+        ctx.sync()
+        t_steps = time.perf_counter() - t0
+        per = t_steps / args.timesteps
+        n, nnz, _ = tcs.S.info()
+        out["config"].update(nnz=int(nnz), precision=args.precision, rtol=args.rtol)
+        out["setup_s"] = round(t_setup, 3)
+        out["first_step_s"] = round(t_first, 3)
+        out["first_step_iterations"] = int(i1.iterations)
+        out["steps"] = {"count": args.timesteps, "s_per_step": round(per, 5), "steps_per_s": round(1.0 / per, 2),
+                        "dof_steps_per_s": round(n / per, 1), "iterations_per_step": round(its / args.timesteps, 1),
+                        "system_assemblies": tcs.assemblies}
+        out["timestep_rhs"] = _timestep_rhs_rate(grmp, tcs, dt)
+        out["spmv_fp64_system"] = _spmv_rate(grmp, tcs.S, V64)
     else:
         raise SystemExit(f"unknown workload {which}")
     print(json.dumps(out), flush=True)
@@ -323,9 +383,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="served",
-                    help="served (the headline stream) | c2-p1 | c2-p2 | c3 | c4 (single large instances)")
+                    help="served (the headline stream) | c2-p1 | c2-p2 | c3 | c4 (single large instances) | "
+                         "heat (BackwardEuler time stepping)")
     ap.add_argument("--size", type=int, default=0, help="single-instance workloads: N (default 2048 / 160 / 512)")
     ap.add_argument("--solve", action="store_true", help="c3: also solve")
+    ap.add_argument("--dt", type=float, default=1e-4, help="heat: time step size")
+    ap.add_argument("--timesteps", type=int, default=20, help="heat: timed steps after the first")
     ap.add_argument("--tile-rows", type=int, default=0, help="engine rows per work tile (0 = library default)")
     ap.add_argument("--fp64-first-sweep", type=int, default=0,
                     help="served requests whose first FP32 sweep takes more iterations continue in FP64 (0: off)")

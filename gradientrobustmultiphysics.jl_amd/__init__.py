@@ -18,6 +18,7 @@ from .pde import (APT_BilinearForm, APT_SymmetricBilinearForm, BilinearForm, Cel
                   PoissonProblem, ReactionOperator, ReconstructionIdentity, SolverConfig, SymmetricGradient, assemble,
                   assemble_linearform, realize_constraints, solve, solve_system)
 
+from .timecontrol import BackwardEuler, CrankNicolson, TimeControlSolver  # noqa: F401
 from . import server  # noqa: F401
 from .server import Server  # noqa: F401
 

@@ -67,13 +67,55 @@ extern "C" int grmp_matrix_spmv(grmp_matrix *A, int values, const grmp_vector *x
     return spmv_launch(ctx, A, values, x->d, y->d, ctx->stream);
 }
 
+// ------------------------------------------- backward Euler right-hand side
+// b_i = penalty * u_i on the fixed rows of the system matrix, else
+// f_i + c (M u)_i: one pass over M's rows with u gathered through LDS.
+// b may be f (row i reads f_i before its thread writes b_i); u is gathered
+// by other rows and must be a different buffer.
+__global__ __launch_bounds__(KB) void k_timestep_rhs(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                    const double *__restrict__ v, const uint8_t *__restrict__ fixed,
+                                                    double penalty, double c, const double *f,
+                                                    const double *__restrict__ u, double *b, int64_t n)
+{
+    __shared__ double lds[SPMV_CAP64];
+    for (int64_t r0 = (int64_t)blockIdx.x * KB; r0 < n; r0 += (int64_t)gridDim.x * KB) {
+        const int64_t r1 = r0 + KB < n ? r0 + KB : n;
+        const double s = csr_rows_lds64(rp, ci, v, u, r0, r1, lds);
+        const int64_t i = r0 + threadIdx.x;
+        if (i < r1) b[i] = fixed[i] ? penalty * u[i] : f[i] + c * s;
+    }
+}
+
+extern "C" int grmp_timestep_rhs(const grmp_matrix *S, const grmp_matrix *M, const grmp_vector *f,
+                                 const grmp_vector *u, double inv_dt, grmp_vector *b)
+{
+    if (!S || !M || !f || !u || !b) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = S->ctx;
+    GRMP_CHECK(ctx, M->ctx == ctx && f->ctx == ctx && u->ctx == ctx && b->ctx == ctx, GRMP_ERR_ARG,
+               "grmp_timestep_rhs: objects of different contexts");
+    GRMP_CHECK(ctx, S->have_pattern && M->have_pattern && M->val64, GRMP_ERR_ARG,
+               "grmp_timestep_rhs: system pattern and FP64 mass matrix values needed");
+    GRMP_CHECK(ctx, M->n == S->n && f->n == S->n && u->n == S->n && b->n == S->n, GRMP_ERR_ARG,
+               "grmp_timestep_rhs: lengths differ from the system's %lld", (long long)S->n);
+    GRMP_CHECK(ctx, b->d != u->d, GRMP_ERR_ARG, "grmp_timestep_rhs: b must not alias u");
+    GRMP_CHECK(ctx, std::isfinite(inv_dt), GRMP_ERR_ARG, "grmp_timestep_rhs: 1/dt is not finite");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    const int64_t n = S->n;
+    dim3 grid((unsigned)std::min<int64_t>(nblk(n, KB), 1 << 20)), blk(KB);
+    hipLaunchKernelGGL(k_timestep_rhs, grid, blk, 0, ctx->stream, M->rowptr, M->cols, M->val64, S->fixed, S->penalty,
+                       inv_dt, f->d, u->d, b->d, n);
+    GRMP_HIP(ctx, hipGetLastError());
+    return GRMP_OK;
+}
+
 // ------------------------------------------------------- plain FP64 PCG
 // One problem, host-driven: every iteration launches its kernels and reads
 // the dot products back to the host, which computes alpha/beta and checks
 // convergence.  This is the conventional GPU CG and the baseline the
 // batched engine is measured against.
 #define PB 256
-#define PGRID 1024
+#define PGRID 1024        // grid of the streaming vector kernels
+#define PGRID_ROWS 16384  // grid cap of the row kernels: enough workgroups to fill every CU slot
 
 __global__ void k_plain_init(const int32_t *rp, const uint8_t *dpos, const double *val, const uint8_t *fixed,
                              const double *b, double *x, int64_t n)
@@ -172,16 +214,31 @@ __global__ void k_plain_pupd(double beta, const double *__restrict__ z, double *
     for (int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x; i < n; i += (int64_t)gridDim.x * PB) p[i] = z[i] + beta * p[i];
 }
 
-static int plain_sum(grmp_ctx *ctx, const double *dpart, double *hpart, int grid, double &a, double &b)
+// Sums a pass's per-workgroup partial pairs in a fixed order (thread t
+// takes pairs t, t + PB, ..., then the block tree) into out[0..1].
+__global__ __launch_bounds__(PB) void k_plain_fold(const double *__restrict__ part, int grid, double *__restrict__ out)
 {
-    GRMP_HIP(ctx, hipMemcpyAsync(hpart, dpart, sizeof(double) * 2 * grid, hipMemcpyDeviceToHost, ctx->stream));
-    GRMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    a = 0.0;
-    b = 0.0;
-    for (int k = 0; k < grid; ++k) {
-        a += hpart[2 * k];
-        b += hpart[2 * k + 1];
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < grid; k += PB) {
+        a += part[2 * k];
+        b += part[2 * k + 1];
     }
+    block_sum2(a, b);
+    if (threadIdx.x == 0) {
+        out[0] = a;
+        out[1] = b;
+    }
+}
+
+// folds the partials on the device and reads the pair back
+static int plain_sum(grmp_ctx *ctx, const double *dpart, double *dout, double *hpart, int grid, double &a, double &b)
+{
+    hipLaunchKernelGGL(k_plain_fold, dim3(1), dim3(PB), 0, ctx->stream, dpart, grid, dout);
+    GRMP_HIP(ctx, hipGetLastError());
+    GRMP_HIP(ctx, hipMemcpyAsync(hpart, dout, sizeof(double) * 2, hipMemcpyDeviceToHost, ctx->stream));
+    GRMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    a = hpart[0];
+    b = hpart[1];
     return GRMP_OK;
 }
 
@@ -196,17 +253,18 @@ int plain_cg_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solv
     GRMP_TRY(dalloc(ctx, &z, n));
     GRMP_TRY(dalloc(ctx, &p, n));
     GRMP_TRY(dalloc(ctx, &q, n));
-    GRMP_TRY(dalloc(ctx, &part, 2 * PGRID));
+    GRMP_TRY(dalloc(ctx, &part, 2 * PGRID_ROWS + 2));
+    double *fold = part + 2 * PGRID_ROWS;
     double *hpart = nullptr;
-    GRMP_HIP(ctx, hipHostMalloc((void **)&hpart, sizeof(double) * 2 * PGRID, hipHostMallocDefault));
-    const int grid = (int)std::min<int64_t>(PGRID, std::max<int64_t>(1, (n + PB - 1) / PB));
-    const int gride = grid;
+    GRMP_HIP(ctx, hipHostMalloc((void **)&hpart, sizeof(double) * 2, hipHostMallocDefault));
+    const int grid = (int)std::min<int64_t>(PGRID_ROWS, std::max<int64_t>(1, (n + PB - 1) / PB));
+    const int gride = (int)std::min<int64_t>(PGRID, grid);
     hipLaunchKernelGGL(k_plain_init, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, A->rowptr, A->diagpos, A->val64, A->fixed,
                        bd, xd, n);
 #define PL_SWITCH(KERNEL, ...) hipLaunchKernelGGL(KERNEL, dim3(grid), dim3(PB), 0, ctx->stream, __VA_ARGS__)
     PL_SWITCH(k_plain_resid, A->rowptr, A->cols, A->val64, A->fixed, A->dinv64, bd, xd, r, z, p, n, part);
     double rz, rr, pq, dummy;
-    int rc = plain_sum(ctx, part, hpart, grid, rz, rr);
+    int rc = plain_sum(ctx, part, fold, hpart, grid, rz, rr);
     double bb = rr;
     int64_t it = 0;
     int status = GRMP_OK;
@@ -216,11 +274,11 @@ int plain_cg_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solv
         if (rr <= o.rtol * o.rtol * bb || rr == 0.0) break;
         if (it >= o.maxit) { status = GRMP_ERR_NOCONV; break; }
         PL_SWITCH(k_plain_spmv, A->rowptr, A->cols, A->val64, A->fixed, p, q, n, part);
-        if ((rc = plain_sum(ctx, part, hpart, grid, pq, dummy))) break;
+        if ((rc = plain_sum(ctx, part, fold, hpart, grid, pq, dummy))) break;
         double alpha = rz / pq;
         hipLaunchKernelGGL(k_plain_axpy, dim3(gride), dim3(PB), 0, ctx->stream, alpha, p, q, A->dinv64, xd, r, z, n, part);
         double rz2;
-        if ((rc = plain_sum(ctx, part, hpart, gride, rz2, rr))) break;
+        if ((rc = plain_sum(ctx, part, fold, hpart, gride, rz2, rr))) break;
         double beta = rz2 / rz;
         rz = rz2;
         hipLaunchKernelGGL(k_plain_pupd, dim3(gride), dim3(PB), 0, ctx->stream, beta, z, p, n);

@@ -189,6 +189,23 @@ function hip_assemble!(A::FEMatrixBlock, kind::Cint, quadorder::Int; factor = 1.
     return nothing
 end
 
+# ---------------------------------------------------- time stepping
+"""
+    hip_timestep_rhs!(S, M, f, u, dt, b)
+
+Right-hand side update of `advance!` with `BackwardEuler`
+(src/solvers_timedependent.jl:339-359, :452-461) on device handles:
+`b = f + M u / dt`, and `b[d] = fixed_penalty * u[d]` on the fixed rows of the
+system matrix `S`.  `S`, `M` are `grmp_matrix` handles (M from the mass form
+of the time-dependent block), `f`, `u`, `b` are `grmp_vector` handles.
+"""
+function hip_timestep_rhs!(S::Ptr{Cvoid}, M::Ptr{Cvoid}, f::Ptr{Cvoid}, u::Ptr{Cvoid}, dt::Real, b::Ptr{Cvoid})
+    check(context(), ccall((:grmp_timestep_rhs, libgrmp), Cint,
+                           (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Ptr{Cvoid}),
+                           S, M, f, u, 1.0 / dt, b))
+    return b
+end
+
 # ---------------------------------------------------- LinearSolve plug-in
 """
     GRMPKrylov(; precision = :fp32_ir, rtol = 1e-10)

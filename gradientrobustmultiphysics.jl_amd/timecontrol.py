@@ -1,0 +1,143 @@
+"""TimeControlSolver with the BackwardEuler rule on the device.
+
+Python mirror of src/solvers_timedependent.jl for linear problems (one
+subiteration holding every unknown, time-independent data):
+
+* TimeControlSolver (:116-281): fixed dofs and homogeneous boundary values of
+  the initial state, the right-hand side f assembled once, the mass matrix M
+  of the time-dependent equations (Identity x Identity, dt_operator's default)
+  assembled once.
+* advance! (:293-594): the system S = A + M/dt is assembled on the first step
+  and whenever the step size changes (update_matrix, :337); every step builds
+  b = f + M u^n / dt with the penalty rows b_d = penalty * u^n_d
+  (:349-359, :452-461) in one device kernel (grmp_timestep_rhs), solves
+  S u^{n+1} = b with the device Krylov solver and realizes global
+  constraints (:578-581).
+* advance_until_stationarity! (:605-665): steps until the summed per-unknown
+  change norms drop below the threshold.
+
+CrankNicolson, nonlinear operators, dt_action / dt_lump and time-dependent
+data are not built and raise GrmpError(ERR_UNSUPPORTED).
+"""
+import numpy as np
+
+from . import _abi
+from ._abi import lib
+from .fe import FEMatrix, FEVector
+from .pde import (ReactionOperator, SolverConfig, _saddle_point, assemble_linearform, fixed_dofs,
+                  realize_constraints, solve_system)
+
+
+class BackwardEuler:
+    """(u^{n+1} - u^n)/dt + A u^{n+1} = f (src/solvers_timedependent.jl:8)."""
+
+
+class CrankNicolson:
+    """Marker of the Crank-Nicolson rule (src/solvers_timedependent.jl:9); not built."""
+
+
+class TimeControlSolver:
+    """TimeControlSolver(PDE, InitialValues, BackwardEuler;
+    timedependent_equations) (src/solvers_timedependent.jl:116-281).
+    ``InitialValues`` holds the initial state and, after each advance, the
+    current solution."""
+
+    def __init__(self, pde, InitialValues, rule=BackwardEuler, timedependent_equations=(), config=None, **kwargs):
+        if rule is not BackwardEuler:
+            raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, f"time integration rule {getattr(rule, '__name__', rule)} "
+                                 "is not built (BackwardEuler is)")
+        which = sorted(set(int(d) for d in timedependent_equations))
+        nu = len(InitialValues.FES)
+        if not which or which[0] < 0 or which[-1] >= nu:
+            raise _abi.GrmpError(_abi.ERR_ARG, "timedependent_equations must name at least one unknown "
+                                 f"in 0..{nu - 1}")
+        self.PDE, self.X, self.which = pde, InitialValues, which
+        self.SC = config or SolverConfig(**kwargs)
+        self.values = _abi.VALUES_FP64 | (_abi.VALUES_FP32 if self.SC.precision == "fp32_ir" else 0)
+        self.method = _abi.SOLVE_MINRES if _saddle_point(pde, InitialValues) else _abi.SOLVE_CG
+        self.fixed = fixed_dofs(pde, InitialValues)
+        # boundarydata!: homogeneous values on the boundary dofs of the initial state (:194-200)
+        bnd = [InitialValues.FES[j].boundary_dofs() + int(InitialValues.offsets[j])
+               for j in sorted(pde.BoundaryOperators)]
+        if bnd:
+            x = InitialValues.entries()
+            x[np.concatenate(bnd)] = 0.0
+            InitialValues.set_entries(x)
+        FES = InitialValues.FES
+        # mass matrix of the time derivative, assembled once (dt_is_nonlinear = false)
+        self.AM = FEMatrix(FES, name="mass matrix")
+        for d in which:
+            form, keep = ReactionOperator(1.0).form(FES[d], d, d, self.AM.ctx)
+            self.AM.add_form(form, keep)
+        self.AM.build_pattern()
+        self.AM.assemble(_abi.VALUES_FP64)
+        # right-hand side data, constant in time
+        self.f = FEVector(FES, name="rhs data")
+        for j, ops in sorted(pde.RHS.items()):
+            for op in ops:
+                assemble_linearform(self.f, j, op)
+        self.b = FEVector(FES, name="system rhs")
+        self.S = None
+        self.cstep, self.ctime, self.last_timestep = 0, 0.0, None
+        self.assemblies = 0
+        self.last_info = None
+
+    def _system(self, timestep):
+        """S = A + (1/dt) M with the fixed dofs penalized (:337-351)."""
+        FES = self.X.FES
+        S = FEMatrix(FES, name="system matrix")
+        for (i, j), ops in sorted(self.PDE.LHS.items()):
+            for op in ops:
+                form, keep = op.form(FES[i], i, j, S.ctx, FES[j])
+                S.add_form(form, keep)
+        for d in self.which:
+            form, keep = ReactionOperator(1.0 / timestep).form(FES[d], d, d, S.ctx)
+            S.add_form(form, keep)
+        S.set_fixed(self.fixed, self.SC.fixed_penalty)
+        S.build_pattern()
+        S.assemble(self.values)
+        self.assemblies += 1
+        return S
+
+    def rhs(self, timestep):
+        """b = f + M u^n / dt, b_d = penalty * u^n_d on fixed dofs (on the device)."""
+        X = self.X
+        X.ctx.check(lib().grmp_timestep_rhs(self.S.h, self.AM.h, self.f.vec.h, X.vec.h, 1.0 / timestep,
+                                            self.b.vec.h))
+        return self.b
+
+    def advance(self, timestep=1e-1):
+        """advance!(TCS, timestep) (src/solvers_timedependent.jl:293-594).
+        Returns the SolveInfo of the step's Krylov solve."""
+        timestep = float(timestep)
+        if not np.isfinite(timestep) or timestep <= 0.0:
+            raise _abi.GrmpError(_abi.ERR_ARG, f"advance: timestep {timestep} must be positive and finite")
+        self.cstep += 1
+        self.ctime += timestep
+        if self.S is None or timestep != self.last_timestep:
+            self.S = None
+            self.S = self._system(timestep)
+        self.rhs(timestep)
+        info = solve_system(self.S, self.b, self.X, self.SC, self.method)
+        realize_constraints(self.PDE, self.X)
+        self.last_timestep = timestep
+        self.last_info = info
+        return info
+
+    def advance_until_stationarity(self, timestep, stationarity_threshold=1e-11, maxtimesteps=100,
+                                   do_after_each_timestep=None):
+        """advance_until_stationarity! (src/solvers_timedependent.jl:605-665):
+        the change of a step is the sum over unknowns of ||u^{n+1} - u^n||
+        on that unknown's dofs.  Returns the number of steps taken."""
+        offs = self.X.offsets
+        for it in range(1, maxtimesteps + 1):
+            last = self.X.entries()
+            self.advance(timestep)
+            d = self.X.entries() - last
+            change = sum(float(np.linalg.norm(d[offs[j]:offs[j + 1]])) for j in range(len(offs) - 1))
+            self.last_change = change
+            if do_after_each_timestep is not None:
+                do_after_each_timestep(self)
+            if change < stationarity_threshold:
+                return it
+        return maxtimesteps

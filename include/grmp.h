@@ -224,6 +224,14 @@ int grmp_matrix_spmv(grmp_matrix *A, int values, const grmp_vector *x, grmp_vect
 /* seconds of the last assemble call's kernels (HIP events) */
 double grmp_matrix_last_assembly_seconds(const grmp_matrix *A);
 
+/* The right-hand side of one BackwardEuler step of advance!
+ * (src/solvers_timedependent.jl:339-359 with the penalty of :452-461):
+ * b = f + inv_dt * M u on the free rows of the system matrix S, and
+ * b_d = penalty * u_d on its fixed rows.  M: the time-derivative mass matrix
+ * (FP64 values); f, u, b of S's length; b may be f but not u. */
+int grmp_timestep_rhs(const grmp_matrix *S, const grmp_matrix *M, const grmp_vector *f, const grmp_vector *u,
+                      double inv_dt, grmp_vector *b);
+
 /* ---------------------------------------------------------- LinearForm */
 /* assemble!(b::FEVectorBlock, AP) for a LinearForm, added into b at offset
  * (src/assemblypatterns/linearform.jl:47-251) */

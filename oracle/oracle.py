@@ -12,6 +12,8 @@ the host-side bookkeeping the reference does in Julia:
 * ``assemble_system`` -- the system assemble + penalty of solve_direct!
   (src/solvers.jl:622-660; penalty src/solvers.jl:647-652 and
   src/fematrix.jl:349-355).
+* ``backward_euler`` -- the BackwardEuler steps of advance! for a linear
+  problem (src/solvers_timedependent.jl:337-359, :452-475), with sparse LU.
 """
 import ctypes
 import os
@@ -218,6 +220,41 @@ def direct_solve(A, b):
     for UMFPACK (src/solvers.jl:55,656-660)."""
     import scipy.sparse.linalg as spla
     return spla.spsolve(A.to_scipy().tocsc(), b)
+
+
+# ------------------------------------------------------- time stepping
+
+def backward_euler_rhs(M, f, u, dt, fixed, penalty=PENALTY):
+    """Right-hand side of one BackwardEuler step of advance!
+    (src/solvers_timedependent.jl:349-359: b = f + (1/dt) AM u; penalty rows
+    b_d = fixed_penalty * x_d at :452-461)."""
+    b = f + (M.to_scipy().tocsr() @ u) * (1.0 / dt)
+    b[fixed] = penalty * u[fixed]
+    return b
+
+
+def backward_euler_system(A, M, dt, fixed, penalty=PENALTY):
+    """S = A + (1/dt) AM (add!, src/solvers_timedependent.jl:345-346) with the
+    fixed diagonals overwritten by the penalty (:457-458); A unpenalized."""
+    S = (A.to_scipy() + M.to_scipy() * (1.0 / dt)).tolil()
+    for d in np.asarray(fixed).tolist():
+        S[d, d] = penalty
+    return S.tocsc()
+
+
+def backward_euler(A, M, f, u0, dts, fixed, penalty=PENALTY):
+    """advance! with BackwardEuler for a linear problem, one step per entry of
+    ``dts``; S is refactored only when the step size changes (:337).
+    Returns the list of states after each step (sparse LU solves)."""
+    import scipy.sparse.linalg as spla
+    u, out, lu, last = np.array(u0, dtype=np.float64), [], None, None
+    for dt in dts:
+        if lu is None or dt != last:
+            lu = spla.splu(backward_euler_system(A, M, dt, fixed, penalty))
+            last = dt
+        u = lu.solve(backward_euler_rhs(M, f, u, dt, fixed, penalty))
+        out.append(u.copy())
+    return out
 
 
 # ------------------------------------------------------------------ Stokes
