@@ -362,6 +362,42 @@ def run_workload(grmp, which, args, log):
                         "system_assemblies": tcs.assemblies}
         out["timestep_rhs"] = _timestep_rhs_rate(grmp, tcs, dt)
         out["spmv_fp64_system"] = _spmv_rate(grmp, tcs.S, V64)
+    elif which == "dirichlet":
+        N = args.size or 2048
+        grid = grmp.grid_unitsquare(N)
+        fes = grmp.FESpace(grmp.H1P2(1), grid)
+        kap = grmp.DeviceVector(ctx, grid.ncells)
+        kap.lognormal(2024, 0.0, 1.0)
+        coef = [0.3, -0.2, 0.5, 0.1, 0.7, -0.4, 1.1, -0.6, 0.9, 0.25]
+        res = {}
+        for name, kind in (("interpolate", grmp.InterpolateDirichletBoundary),
+                           ("bestapprox", grmp.BestapproxDirichletBoundary)):
+            pde = grmp.PoissonProblem(cell_coefficient=grmp.CellCoefficient(kap))
+            pde.add_rhsdata(0, grmp.LinearForm(grmp.Identity, grmp.DataFunction([1.0])))
+            pde.add_boundarydata(0, kind, data=grmp.PolynomialData(coef))
+            sol = grmp.FEVector(fes)
+            log(f"[bench] dirichlet: {name} boundary values")
+            grmp.boundary_values(pde, sol)
+            ctx.sync()
+            reps = 5
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                grmp.boundary_values(pde, sol)
+            ctx.sync()
+            tb = (time.perf_counter() - t0) / reps
+            log(f"[bench] dirichlet: {name} solve")
+            t0 = time.perf_counter()
+            info, A, b = grmp.solve(sol, pde, precision=args.precision, rtol=args.rtol)
+            ctx.sync()
+            res[name] = {"boundary_values_ms": round(tb * 1e3, 3), "solve_status": info.status,
+                         "solve_iterations": int(info.iterations), "solve_s": round(time.perf_counter() - t0, 3)}
+        nb = int(fes.boundary_dofs().size)
+        out["config"] = {"workload": f"2D P2 Poisson, {N}x{N} triangulated square, log-normal per-cell kappa sigma=1, "
+                                     "f=1, cubic polynomial Dirichlet data on all boundary faces",
+                         "cells": grid.ncells, "ndofs": fes.ndofs, "boundary_dofs": nb, "precision": args.precision,
+                         "rtol": args.rtol}
+        res["boundary_dofs_per_s_bestapprox"] = round(nb / (res["bestapprox"]["boundary_values_ms"] * 1e-3), 1)
+        out["dirichlet"] = res
     else:
         raise SystemExit(f"unknown workload {which}")
     print(json.dumps(out), flush=True)
@@ -384,7 +420,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="served",
                     help="served (the headline stream) | c2-p1 | c2-p2 | c3 | c4 (single large instances) | "
-                         "heat (BackwardEuler time stepping)")
+                         "heat (BackwardEuler time stepping) | dirichlet (inhomogeneous Dirichlet data)")
     ap.add_argument("--size", type=int, default=0, help="single-instance workloads: N (default 2048 / 160 / 512)")
     ap.add_argument("--solve", action="store_true", help="c3: also solve")
     ap.add_argument("--dt", type=float, default=1e-4, help="heat: time step size")

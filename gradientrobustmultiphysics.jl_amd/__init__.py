@@ -11,9 +11,10 @@ from ._abi import (Context, DeviceVector, GrmpError, default_context, lib)  # no
 from . import _abi  # noqa: F401
 from .fe import (ExtendableGrid, FEMatrix, FEType, FESpace, FEVector, H1BR, H1P1, H1P2, L2P0,  # noqa: F401
                  grid_unitcube, grid_unitsquare)
-from .pde import (APT_BilinearForm, APT_SymmetricBilinearForm, BilinearForm, CellCoefficient,  # noqa: F401
-                  DataFunction, Divergence, FixedIntegralMean, Gradient, GradientCurlData, HDIVRT0,
-                  HomogeneousDirichletBoundary, HookStiffnessOperator2D, Identity, IncompressibleNavierStokesProblem,
+from .pde import (APT_BilinearForm, APT_SymmetricBilinearForm, BestapproxDirichletBoundary,  # noqa: F401
+                  BilinearForm, CellCoefficient, DataFunction, Divergence, FixedIntegralMean, Gradient,
+                  GradientCurlData, HDIVRT0, HomogeneousDirichletBoundary, HookStiffnessOperator2D, Identity,
+                  IncompressibleNavierStokesProblem, InterpolateDirichletBoundary, PolynomialData, boundary_values,
                   LagrangeMultiplier, LaplaceOperator, LinearElasticityProblem, LinearForm, PDEDescription,
                   PoissonProblem, ReactionOperator, ReconstructionIdentity, SolverConfig, SymmetricGradient, assemble,
                   assemble_linearform, realize_constraints, solve, solve_system)

@@ -138,6 +138,10 @@ SIGNATURES = {
     "grmp_matrix_spmv": (ctypes.c_int, [VP, ctypes.c_int, VP, VP]),
     "grmp_matrix_last_assembly_seconds": (ctypes.c_double, [VP]),
     "grmp_timestep_rhs": (ctypes.c_int, [VP, VP, VP, VP, ctypes.c_double, VP]),
+    "grmp_boundary_interpolate": (ctypes.c_int, [VP, DP, ctypes.c_int, VP, I64]),
+    "grmp_boundary_bestapprox": (ctypes.c_int, [VP, DP, ctypes.c_int, ctypes.c_int, ctypes.c_double, VP, I64,
+                                                I64P, DP]),
+    "grmp_matrix_penalize_rhs": (ctypes.c_int, [VP, VP, VP]),
     "grmp_assemble_linearform": (ctypes.c_int, [VP, VP, ctypes.POINTER(LForm), VP, I64]),
     "grmp_integrate_identity": (ctypes.c_int, [VP, VP, ctypes.c_int, VP, I64, DP]),
     "grmp_solve_opts_default": (None, [ctypes.POINTER(SolveOpts)]),

@@ -1,0 +1,571 @@
+// Inhomogeneous Dirichlet data of libgrmp_hip.so: the boundary values of an
+// H1 P1/P2 space for polynomial data g, by interpolation or by the L2 best
+// approximation on the boundary faces, and the penalized right-hand side
+// that carries them into the solve.
+//
+// boundarydata! (src/boundarydata.jl:198-395):
+//  * InterpolateDirichletBoundary: the dofs of the boundary faces take the
+//    values of g at their points (vertices, P2 face midpoints).
+//  * BestapproxDirichletBoundary (:259-352): the mass matrix of the boundary
+//    faces (ON_BFACES, Identity x Identity) and the load of g on them are
+//    assembled, and the system compressed to the boundary dofs is solved
+//    (the reference factorizes it; here a Jacobi-PCG runs to a relative
+//    residual of rtol inside one workgroup).
+//  * solve_direct! then fixes b_d = penalty * Target_d (src/solvers.jl:647-652).
+//
+// Boundary faces are walked in cell order and local face order (0: nodes
+// 0-1, 1: 1-2, 2: 2-0), each face oriented as its cell's local face; a
+// boundary dof's row sums its faces in that order, so results do not depend
+// on scheduling.  Segment quadrature is Gauss-Legendre with
+// ceil((quadorder + 1) / 2) points, at least the 2 (P1) / 3 (P2) that
+// integrate the face mass matrix exactly.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "grmp_internal.h"
+#include "grmp_square.h"
+
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+#define BD_NCOEF 10  // g(x, y) = sum of c_k m_k, m = 1, x, y, x^2, xy, y^2, x^3, x^2 y, x y^2, y^3
+#define BD_MAXQ 4    // Gauss-Legendre points on a segment (exact to degree 7)
+#define BD_CG_T 1024
+
+struct BdData {
+    double c[BD_NCOEF];
+};
+
+struct BdMesh {
+    const double *coords;     // [2][nnodes]
+    const int32_t *cells;     // [3][ncells]
+    const int32_t *cellfaces; // [3][ncells] or nullptr (generated unit square)
+    const uint8_t *bface;     // [nfaces] or nullptr
+    const int32_t *celldofs;  // [nloc][ncells]
+    int64_t nnodes, ncells;
+    int N, fe;
+};
+
+__device__ __forceinline__ double bd_g(const BdData &g, double x, double y)
+{
+    double s = 0.0;
+    s += g.c[0];
+    s += g.c[1] * x;
+    s += g.c[2] * y;
+    s += g.c[3] * (x * x);
+    s += g.c[4] * (x * y);
+    s += g.c[5] * (y * y);
+    s += g.c[6] * (x * x * x);
+    s += g.c[7] * (x * x * y);
+    s += g.c[8] * (x * y * y);
+    s += g.c[9] * (y * y * y);
+    return s;
+}
+
+// Gauss-Legendre on [0, 1]: points s_q and weights w_q (sum 1)
+__device__ __forceinline__ void bd_gauss(int nq, int q, double &s, double &w)
+{
+    if (nq == 1) {
+        s = 0.5;
+        w = 1.0;
+    } else if (nq == 2) {
+        const double d = 0.28867513459481288225;  // 1 / (2 sqrt 3)
+        s = q == 0 ? 0.5 - d : 0.5 + d;
+        w = 0.5;
+    } else if (nq == 3) {
+        const double d = 0.38729833462074168852;  // sqrt(3/5) / 2
+        s = q == 0 ? 0.5 - d : (q == 1 ? 0.5 : 0.5 + d);
+        w = q == 1 ? 4.0 / 9.0 : 5.0 / 18.0;
+    } else {
+        const double d1 = 0.16999052179242813185, d2 = 0.43056815579702628810;  // roots / 2
+        const double w1 = 0.32607257743127307131, w2 = 0.17392742256872692869;  // weights / 2
+        s = q == 0 ? 0.5 - d2 : (q == 1 ? 0.5 - d1 : (q == 2 ? 0.5 + d1 : 0.5 + d2));
+        w = (q == 0 || q == 3) ? w2 : w1;
+    }
+}
+
+// 1D trace basis of the face (local dof 0 at the first node, 1 at the second, 2 the P2 midpoint)
+__device__ __forceinline__ double bd_phi(int fe, int a, double s)
+{
+    if (fe == GRMP_FE_H1P1) return a == 0 ? 1.0 - s : s;
+    if (a == 0) return (1.0 - s) * (1.0 - 2.0 * s);
+    if (a == 1) return s * (2.0 * s - 1.0);
+    return 4.0 * s * (1.0 - s);
+}
+
+__device__ __forceinline__ bool bd_is_boundary(const BdMesh &M, int64_t c, int k)
+{
+    if (M.cellfaces) return M.bface[M.cellfaces[k * M.ncells + c]] != 0;
+    const int64_t a = M.cells[k * M.ncells + c], b = M.cells[((k + 1) % 3) * M.ncells + c];
+    const int64_t W = M.N + 1;
+    const int64_t ia = a % W, ja = a / W, ib = b % W, jb = b / W;
+    return (ia == 0 && ib == 0) || (ia == M.N && ib == M.N) || (ja == 0 && jb == 0) || (ja == M.N && jb == M.N);
+}
+
+// local dofs of boundary face (c, k): global dof numbers
+__device__ __forceinline__ int bd_face_dofs(const BdMesh &M, int64_t c, int k, int32_t d[3])
+{
+    const int64_t nc = M.ncells;
+    d[0] = M.celldofs[k * nc + c];
+    d[1] = M.celldofs[((k + 1) % 3) * nc + c];
+    if (M.fe == GRMP_FE_H1P2) {
+        d[2] = M.celldofs[(3 + k) * nc + c];
+        return 3;
+    }
+    return 2;
+}
+
+__device__ __forceinline__ void bd_face_geom(const BdMesh &M, int64_t c, int k, double &x0, double &y0, double &x1,
+                                             double &y1)
+{
+    const int64_t nc = M.ncells, nn = M.nnodes;
+    const int32_t a = M.cells[k * nc + c], b = M.cells[((k + 1) % 3) * nc + c];
+    x0 = M.coords[a];
+    y0 = M.coords[nn + a];
+    x1 = M.coords[b];
+    y1 = M.coords[nn + b];
+}
+
+// ------------------------------------------------------------ boundary faces
+__global__ void k_bd_flag(BdMesh M, int32_t *__restrict__ flag)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * M.ncells) return;
+    const int64_t c = t / 3;
+    const int k = (int)(t - 3 * c);
+    flag[t] = bd_is_boundary(M, c, k) ? 1 : 0;
+}
+
+__global__ void k_bd_compact(const int32_t *__restrict__ flag, const int32_t *__restrict__ pos, int64_t n3,
+                             int32_t *__restrict__ bf)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n3 && flag[t]) bf[pos[t]] = (int32_t)t;  // t = 3 c + k
+}
+
+// boundary dof flags (int32 for the scan)
+__global__ void k_bd_dofflag(BdMesh M, const int32_t *__restrict__ bf, int64_t nbf, int32_t *__restrict__ dflag)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nbf) return;
+    const int64_t c = bf[i] / 3;
+    const int k = bf[i] - 3 * (int)c;
+    int32_t d[3];
+    const int nd = bd_face_dofs(M, c, k, d);
+    for (int a = 0; a < nd; ++a) dflag[d[a]] = 1;
+}
+
+__global__ void k_bd_rows(const int32_t *__restrict__ dflag, const int32_t *__restrict__ dpos, int64_t ndofs,
+                          int32_t *__restrict__ comp, int32_t *__restrict__ bdof)
+{
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= ndofs) return;
+    if (dflag[d]) {
+        comp[d] = dpos[d];
+        bdof[dpos[d]] = (int32_t)d;
+    } else {
+        comp[d] = -1;
+    }
+}
+
+// incidences (boundary face, local dof) per compressed row: count, fill, sort
+__global__ void k_bd_count(BdMesh M, const int32_t *__restrict__ bf, int64_t nbf, const int32_t *__restrict__ comp,
+                           int32_t *__restrict__ cnt)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nbf) return;
+    const int64_t c = bf[i] / 3;
+    const int k = bf[i] - 3 * (int)c;
+    int32_t d[3];
+    const int nd = bd_face_dofs(M, c, k, d);
+    for (int a = 0; a < nd; ++a) atomicAdd(&cnt[comp[d[a]]], 1);
+}
+
+__global__ void k_bd_fill(BdMesh M, const int32_t *__restrict__ bf, int64_t nbf, const int32_t *__restrict__ comp,
+                          const int32_t *__restrict__ ptr, int32_t *__restrict__ cur, int32_t *__restrict__ inc)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nbf) return;
+    const int64_t c = bf[i] / 3;
+    const int k = bf[i] - 3 * (int)c;
+    int32_t d[3];
+    const int nd = bd_face_dofs(M, c, k, d);
+    for (int a = 0; a < nd; ++a) {
+        const int32_t r = comp[d[a]];
+        const int32_t slot = ptr[r] + atomicAdd(&cur[r], 1);
+        inc[slot] = (int32_t)(i * 4 + a);  // boundary face index, local dof
+    }
+}
+
+__global__ void k_bd_sort(const int32_t *__restrict__ ptr, int64_t nb, int32_t *__restrict__ inc)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nb) return;
+    for (int32_t p = ptr[r] + 1; p < ptr[r + 1]; ++p) {
+        const int32_t v = inc[p];
+        int32_t q = p - 1;
+        while (q >= ptr[r] && inc[q] > v) {
+            inc[q + 1] = inc[q];
+            --q;
+        }
+        inc[q + 1] = v;
+    }
+}
+
+// ------------------------------------------------------ best approximation
+struct BdSys {
+    BdMesh M;
+    BdData g;
+    const int32_t *bf, *ptr, *inc, *comp;
+    int64_t nb;
+    int nq;
+};
+
+__device__ __forceinline__ void bd_face_len(const BdSys &S, int64_t i, double &len, double &x0, double &y0, double &x1,
+                                            double &y1)
+{
+    const int64_t c = S.bf[i] / 3;
+    const int k = S.bf[i] - 3 * (int)c;
+    bd_face_geom(S.M, c, k, x0, y0, x1, y1);
+    const double dx = x1 - x0, dy = y1 - y0;
+    len = sqrt(dx * dx + dy * dy);
+}
+
+// (M x)_r and the diagonal M_rr, row r summed over its faces in order
+__device__ __forceinline__ double bd_row_apply(const BdSys &S, int64_t r, const double *x, double *diag)
+{
+    double y = 0.0, dg = 0.0;
+    for (int32_t p = S.ptr[r]; p < S.ptr[r + 1]; ++p) {
+        const int64_t i = S.inc[p] >> 2;
+        const int a = S.inc[p] & 3;
+        double len, x0, y0, x1, y1;
+        bd_face_len(S, i, len, x0, y0, x1, y1);
+        const int64_t c = S.bf[i] / 3;
+        const int k = S.bf[i] - 3 * (int)c;
+        int32_t d[3];
+        const int nd = bd_face_dofs(S.M, c, k, d);
+        for (int b = 0; b < nd; ++b) {
+            double m = 0.0;
+            for (int q = 0; q < S.nq; ++q) {
+                double s, w;
+                bd_gauss(S.nq, q, s, w);
+                m += w * (bd_phi(S.M.fe, a, s) * bd_phi(S.M.fe, b, s));
+            }
+            m *= len;
+            if (x) y += m * x[S.comp[d[b]]];
+            if (b == a) dg += m;
+        }
+    }
+    if (diag) *diag = dg;
+    return y;
+}
+
+__device__ __forceinline__ double bd_row_load(const BdSys &S, int64_t r)
+{
+    double f = 0.0;
+    for (int32_t p = S.ptr[r]; p < S.ptr[r + 1]; ++p) {
+        const int64_t i = S.inc[p] >> 2;
+        const int a = S.inc[p] & 3;
+        double len, x0, y0, x1, y1;
+        bd_face_len(S, i, len, x0, y0, x1, y1);
+        double l = 0.0;
+        for (int q = 0; q < S.nq; ++q) {
+            double s, w;
+            bd_gauss(S.nq, q, s, w);
+            const double xq = x0 + s * (x1 - x0), yq = y0 + s * (y1 - y0);
+            l += w * (bd_g(S.g, xq, yq) * bd_phi(S.M.fe, a, s));
+        }
+        f += l * len;
+    }
+    return f;
+}
+
+__device__ __forceinline__ double bd_block_sum(double v, double *red)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int k = 0; k < BD_CG_T / 64; ++k) s += red[k];
+    return s;
+}
+
+// Jacobi-PCG on the compressed boundary system in one workgroup (the loop
+// condition is uniform: every thread holds the same block sums); x stays in
+// xv, status {iterations, relres} goes to st.
+__global__ __launch_bounds__(BD_CG_T) void k_bd_cg(BdSys S, double rtol, int maxit, double *__restrict__ xv,
+                                                   double *__restrict__ rv, double *__restrict__ zv,
+                                                   double *__restrict__ pv, double *__restrict__ dv,
+                                                   double *__restrict__ st)
+{
+    __shared__ double red[BD_CG_T / 64];
+    const int64_t nb = S.nb;
+    double rz = 0.0, bb = 0.0;
+    for (int64_t r = threadIdx.x; r < nb; r += BD_CG_T) {
+        double dg;
+        bd_row_apply(S, r, nullptr, &dg);
+        const double b = bd_row_load(S, r);
+        dv[r] = 1.0 / dg;
+        xv[r] = 0.0;
+        rv[r] = b;
+        zv[r] = b / dg;
+        pv[r] = b / dg;
+        rz += b * (b / dg);
+        bb += b * b;
+    }
+    rz = bd_block_sum(rz, red);
+    bb = bd_block_sum(bb, red);
+    double rr = bb;
+    int it = 0;
+    while (it < maxit && rr > rtol * rtol * bb && rr > 0.0) {
+        __syncthreads();  // p complete before the products
+        double pq = 0.0;
+        for (int64_t r = threadIdx.x; r < nb; r += BD_CG_T) {
+            const double q = bd_row_apply(S, r, pv, nullptr);
+            zv[r] = q;  // q kept in z until the update below
+            pq += pv[r] * q;
+        }
+        pq = bd_block_sum(pq, red);
+        const double alpha = rz / pq;
+        double rz2 = 0.0, rr2 = 0.0;
+        for (int64_t r = threadIdx.x; r < nb; r += BD_CG_T) {
+            xv[r] += alpha * pv[r];
+            const double ri = rv[r] - alpha * zv[r];
+            rv[r] = ri;
+            const double zi = dv[r] * ri;
+            zv[r] = zi;
+            rz2 += ri * zi;
+            rr2 += ri * ri;
+        }
+        rz2 = bd_block_sum(rz2, red);
+        rr = bd_block_sum(rr2, red);
+        const double beta = rz2 / rz;
+        rz = rz2;
+        for (int64_t r = threadIdx.x; r < nb; r += BD_CG_T) pv[r] = zv[r] + beta * pv[r];
+        ++it;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st[0] = (double)it;
+        st[1] = bb > 0.0 ? sqrt(rr / bb) : 0.0;
+    }
+}
+
+__global__ void k_bd_scatter(const double *__restrict__ xv, const int32_t *__restrict__ bdof, int64_t nb,
+                             double *__restrict__ target, int64_t offset)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nb) target[offset + bdof[r]] = xv[r];
+}
+
+// interpolation: g at the dof points of the boundary faces
+__global__ void k_bd_interp(BdMesh M, BdData g, const int32_t *__restrict__ bf, int64_t nbf,
+                            double *__restrict__ target, int64_t offset)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nbf) return;
+    const int64_t c = bf[i] / 3;
+    const int k = bf[i] - 3 * (int)c;
+    int32_t d[3];
+    const int nd = bd_face_dofs(M, c, k, d);
+    double x0, y0, x1, y1;
+    bd_face_geom(M, c, k, x0, y0, x1, y1);
+    // a dof shared by two faces gets the same value from both (its point)
+    target[offset + d[0]] = bd_g(g, x0, y0);
+    target[offset + d[1]] = bd_g(g, x1, y1);
+    if (nd == 3) target[offset + d[2]] = bd_g(g, 0.5 * (x0 + x1), 0.5 * (y0 + y1));
+}
+
+__global__ void k_penalize_rhs(const uint8_t *__restrict__ fixed, double penalty, const double *__restrict__ target,
+                               double *__restrict__ b, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && fixed[i]) b[i] = penalty * target[i];
+}
+
+// ------------------------------------------------------------------ host
+struct BdFaces {
+    int32_t *bf = nullptr;
+    int64_t nbf = 0;
+};
+
+static int bd_check(grmp_ctx *ctx, const grmp_space *s, const double *coef, int ncoef, const grmp_vector *target,
+                    int64_t offset)
+{
+    GRMP_CHECK(ctx, s->mesh->dim == 2 && s->mesh->nloc == 3, GRMP_ERR_UNSUPPORTED,
+               "Dirichlet data: triangle meshes only");
+    GRMP_CHECK(ctx, (s->fe == GRMP_FE_H1P1 || s->fe == GRMP_FE_H1P2) && s->ncomp == 1, GRMP_ERR_UNSUPPORTED,
+               "Dirichlet data: scalar H1P1 / H1P2 spaces only");
+    GRMP_CHECK(ctx, s->mesh->cellfaces || s->mesh->kind == MESH_UNITSQUARE, GRMP_ERR_UNSUPPORTED,
+               "Dirichlet data: the mesh has no face adjacency");
+    GRMP_CHECK(ctx, coef && ncoef >= 1 && ncoef <= BD_NCOEF, GRMP_ERR_ARG,
+               "Dirichlet data: 1..%d polynomial coefficients", BD_NCOEF);
+    GRMP_CHECK(ctx, target && target->ctx == ctx && offset >= 0 && offset + s->ndofs <= target->n, GRMP_ERR_ARG,
+               "Dirichlet data: target too short for the space at offset %lld", (long long)offset);
+    return GRMP_OK;
+}
+
+static BdMesh bd_mesh(const grmp_space *s)
+{
+    const grmp_mesh *m = s->mesh;
+    BdMesh M;
+    M.coords = m->coords;
+    M.cells = m->cells;
+    M.cellfaces = m->cellfaces;
+    M.bface = m->bface;
+    M.celldofs = s->celldofs;
+    M.nnodes = m->nnodes;
+    M.ncells = m->ncells;
+    M.N = m->N;
+    M.fe = s->fe;
+    return M;
+}
+
+static int bd_faces(grmp_ctx *ctx, const BdMesh &M, BdFaces &F)
+{
+    const int64_t n3 = 3 * M.ncells;
+    int32_t *flag = nullptr, *pos = nullptr;
+    int rc = GRMP_OK;
+    do {
+        if ((rc = dalloc(ctx, &flag, n3)) || (rc = dalloc(ctx, &pos, n3 + 1))) break;
+        hipLaunchKernelGGL(k_bd_flag, dim3(nblk(n3, 256)), dim3(256), 0, ctx->stream, M, flag);
+        if ((rc = exclusive_scan_i32(ctx, flag, pos, n3))) break;
+        int32_t nbf = 0;
+        if (hipMemcpyAsync(&nbf, pos + n3, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "Dirichlet data: boundary face count");
+            break;
+        }
+        F.nbf = nbf;
+        if ((rc = dalloc(ctx, &F.bf, std::max<int64_t>(nbf, 1)))) break;
+        hipLaunchKernelGGL(k_bd_compact, dim3(nblk(n3, 256)), dim3(256), 0, ctx->stream, flag, pos, n3, F.bf);
+        if (hipGetLastError() != hipSuccess) rc = grmp_fail(ctx, GRMP_ERR_HIP, "Dirichlet data: compaction");
+    } while (0);
+    dfree(ctx, flag);
+    dfree(ctx, pos);
+    return rc;
+}
+
+extern "C" int grmp_boundary_interpolate(const grmp_space *s, const double *coef, int ncoef, grmp_vector *target,
+                                         int64_t offset)
+{
+    if (!s) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = s->ctx;
+    GRMP_TRY(bd_check(ctx, s, coef, ncoef, target, offset));
+    std::lock_guard<std::mutex> g(ctx->mu);
+    BdData gd;
+    memset(&gd, 0, sizeof(gd));
+    for (int k = 0; k < ncoef; ++k) gd.c[k] = coef[k];
+    const BdMesh M = bd_mesh(s);
+    BdFaces F;
+    int rc = bd_faces(ctx, M, F);
+    if (!rc && F.nbf) {
+        hipLaunchKernelGGL(k_bd_interp, dim3(nblk(F.nbf, 256)), dim3(256), 0, ctx->stream, M, gd, F.bf, F.nbf,
+                           target->d, offset);
+        if (hipGetLastError() != hipSuccess) rc = grmp_fail(ctx, GRMP_ERR_HIP, "Dirichlet data: interpolation");
+    }
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = grmp_fail(ctx, GRMP_ERR_HIP, "Dirichlet data");
+    dfree(ctx, F.bf);
+    return rc;
+}
+
+extern "C" int grmp_boundary_bestapprox(const grmp_space *s, const double *coef, int ncoef, int quadorder,
+                                        double rtol, grmp_vector *target, int64_t offset, int64_t *iterations,
+                                        double *relres)
+{
+    if (!s) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = s->ctx;
+    GRMP_TRY(bd_check(ctx, s, coef, ncoef, target, offset));
+    GRMP_CHECK(ctx, quadorder >= 0 && quadorder <= 2 * BD_MAXQ - 1, GRMP_ERR_UNSUPPORTED,
+               "Dirichlet data: segment quadrature order %d above %d", quadorder, 2 * BD_MAXQ - 1);
+    GRMP_CHECK(ctx, rtol > 0.0 && rtol < 1.0, GRMP_ERR_ARG, "Dirichlet data: rtol must lie in (0, 1)");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    BdSys S;
+    memset(&S, 0, sizeof(S));
+    for (int k = 0; k < ncoef; ++k) S.g.c[k] = coef[k];
+    S.M = bd_mesh(s);
+    // the mass matrix needs degree 2 p exactly, the load p + the data's degree
+    S.nq = std::max((quadorder + 2) / 2, s->fe == GRMP_FE_H1P2 ? 3 : 2);
+    const int64_t nd = s->ndofs;
+    BdFaces F;
+    int32_t *dflag = nullptr, *dpos = nullptr, *comp = nullptr, *bdof = nullptr, *cnt = nullptr, *ptr = nullptr,
+            *cur = nullptr, *inc = nullptr;
+    double *work = nullptr;
+    int rc = GRMP_OK;
+    do {
+        if ((rc = bd_faces(ctx, S.M, F))) break;
+        if (F.nbf == 0) break;
+        const int per = s->fe == GRMP_FE_H1P2 ? 3 : 2;
+        if ((rc = dalloc(ctx, &dflag, nd)) || (rc = dalloc(ctx, &dpos, nd + 1)) || (rc = dalloc(ctx, &comp, nd)))
+            break;
+        GRMP_HIP(ctx, hipMemsetAsync(dflag, 0, sizeof(int32_t) * nd, ctx->stream));
+        hipLaunchKernelGGL(k_bd_dofflag, dim3(nblk(F.nbf, 256)), dim3(256), 0, ctx->stream, S.M, F.bf, F.nbf, dflag);
+        if ((rc = exclusive_scan_i32(ctx, dflag, dpos, nd))) break;
+        int32_t nb = 0;
+        if (hipMemcpyAsync(&nb, dpos + nd, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "Dirichlet data: boundary dof count");
+            break;
+        }
+        S.nb = nb;
+        const int64_t ninc = F.nbf * per;
+        if ((rc = dalloc(ctx, &bdof, nb)) || (rc = dalloc(ctx, &cnt, nb)) || (rc = dalloc(ctx, &ptr, nb + 1)) ||
+            (rc = dalloc(ctx, &cur, nb)) || (rc = dalloc(ctx, &inc, ninc)) || (rc = dalloc(ctx, &work, 5 * (size_t)nb + 2)))
+            break;
+        hipLaunchKernelGGL(k_bd_rows, dim3(nblk(nd, 256)), dim3(256), 0, ctx->stream, dflag, dpos, nd, comp, bdof);
+        GRMP_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(int32_t) * nb, ctx->stream));
+        GRMP_HIP(ctx, hipMemsetAsync(cur, 0, sizeof(int32_t) * nb, ctx->stream));
+        hipLaunchKernelGGL(k_bd_count, dim3(nblk(F.nbf, 256)), dim3(256), 0, ctx->stream, S.M, F.bf, F.nbf, comp, cnt);
+        if ((rc = exclusive_scan_i32(ctx, cnt, ptr, nb))) break;
+        hipLaunchKernelGGL(k_bd_fill, dim3(nblk(F.nbf, 256)), dim3(256), 0, ctx->stream, S.M, F.bf, F.nbf, comp, ptr,
+                           cur, inc);
+        hipLaunchKernelGGL(k_bd_sort, dim3(nblk(nb, 256)), dim3(256), 0, ctx->stream, ptr, (int64_t)nb, inc);
+        S.bf = F.bf;
+        S.ptr = ptr;
+        S.inc = inc;
+        S.comp = comp;
+        double *xv = work, *rv = work + nb, *zv = work + 2 * (size_t)nb, *pv = work + 3 * (size_t)nb,
+               *dv = work + 4 * (size_t)nb, *st = work + 5 * (size_t)nb;
+        const int maxit = (int)std::min<int64_t>(100000, 10 * (int64_t)nb + 100);
+        hipLaunchKernelGGL(k_bd_cg, dim3(1), dim3(BD_CG_T), 0, ctx->stream, S, rtol, maxit, xv, rv, zv, pv, dv, st);
+        hipLaunchKernelGGL(k_bd_scatter, dim3(nblk(nb, 256)), dim3(256), 0, ctx->stream, xv, bdof, (int64_t)nb,
+                           target->d, offset);
+        GRMP_HIP(ctx, hipGetLastError());
+        double hst[2] = {0.0, 0.0};
+        if (hipMemcpyAsync(hst, st, sizeof(hst), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "Dirichlet data: boundary solve");
+            break;
+        }
+        if (iterations) *iterations = (int64_t)hst[0];
+        if (relres) *relres = hst[1];
+        if (!(hst[1] <= rtol)) rc = grmp_fail(ctx, GRMP_ERR_NOCONV, "Dirichlet data: boundary solve reached relres %.3e",
+                                              hst[1]);
+    } while (0);
+    dfree(ctx, F.bf);
+    dfree(ctx, dflag);
+    dfree(ctx, dpos);
+    dfree(ctx, comp);
+    dfree(ctx, bdof);
+    dfree(ctx, cnt);
+    dfree(ctx, ptr);
+    dfree(ctx, cur);
+    dfree(ctx, inc);
+    dfree(ctx, work);
+    return rc;
+}
+
+extern "C" int grmp_matrix_penalize_rhs(const grmp_matrix *A, const grmp_vector *target, grmp_vector *b)
+{
+    if (!A || !target || !b) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = A->ctx;
+    GRMP_CHECK(ctx, A->fixed && target->n == A->n && b->n == A->n, GRMP_ERR_ARG,
+               "grmp_matrix_penalize_rhs: fixed dofs and vectors of the system's length %lld needed", (long long)A->n);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipLaunchKernelGGL(k_penalize_rhs, dim3(nblk(A->n, 256)), dim3(256), 0, ctx->stream, A->fixed, A->penalty,
+                       target->d, b->d, A->n);
+    GRMP_HIP(ctx, hipGetLastError());
+    return GRMP_OK;
+}

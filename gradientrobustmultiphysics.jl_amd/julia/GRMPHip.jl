@@ -189,6 +189,23 @@ function hip_assemble!(A::FEMatrixBlock, kind::Cint, quadorder::Int; factor = 1.
     return nothing
 end
 
+# ---------------------------------------------------- Dirichlet data
+"""
+    hip_boundary_bestapprox!(space, coef, quadorder, target, offset; rtol = 1e-14)
+
+BestapproxDirichletBoundary (src/boundarydata.jl:259-352) on the device for
+the polynomial data sum(coef[k] m_k), m = (1, x, y, x^2, xy, y^2, x^3, x^2 y,
+x y^2, y^3): the boundary values go to the `grmp_vector` handle `target` at
+`offset` (0-based dof numbering of the space).
+"""
+function hip_boundary_bestapprox!(space::Ptr{Cvoid}, coef::Vector{Float64}, quadorder::Int, target::Ptr{Cvoid},
+                                  offset::Int; rtol = 1e-14)
+    check(context(), ccall((:grmp_boundary_bestapprox, libgrmp), Cint,
+                           (Ptr{Cvoid}, Ptr{Cdouble}, Cint, Cint, Cdouble, Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Cdouble}),
+                           space, coef, length(coef), quadorder, rtol, target, offset, C_NULL, C_NULL))
+    return target
+end
+
 # ---------------------------------------------------- time stepping
 """
     hip_timestep_rhs!(S, M, f, u, dt, b)

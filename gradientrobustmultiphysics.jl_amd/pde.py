@@ -197,6 +197,37 @@ class HomogeneousDirichletBoundary:
     (src/boundarydata.jl:198-257)."""
 
 
+class InterpolateDirichletBoundary:
+    """Boundary dofs take the data's values at their points, vertices and P2
+    face midpoints (src/boundarydata.jl:198-257)."""
+
+
+class BestapproxDirichletBoundary:
+    """Boundary dofs from the L2 best approximation of the data on the
+    boundary faces (src/boundarydata.jl:259-352)."""
+
+
+class PolynomialData:
+    """Closed-form stand-in for a DataFunction of (x, y) on the device
+    (src/userdata.jl:3-63): g = sum_k c_k m_k over the monomials
+    1, x, y, x^2, xy, y^2, x^3, x^2 y, x y^2, y^3 (at most 10 coefficients);
+    bonus_quadorder defaults to the degree."""
+
+    MONOMIALS = ((0, 0), (1, 0), (0, 1), (2, 0), (1, 1), (0, 2), (3, 0), (2, 1), (1, 2), (0, 3))
+
+    def __init__(self, coeffs, bonus_quadorder=None):
+        c = np.asarray(coeffs, dtype=np.float64).ravel()
+        if c.size < 1 or c.size > 10:
+            raise _abi.GrmpError(_abi.ERR_ARG, "PolynomialData: 1..10 coefficients")
+        self.c = np.ascontiguousarray(c)
+        nz = [k for k in range(c.size) if c[k] != 0.0]
+        self.degree = max((sum(self.MONOMIALS[k]) for k in nz), default=0)
+        self.bonus_quadorder = self.degree if bonus_quadorder is None else int(bonus_quadorder)
+
+    def __call__(self, x, y):
+        return sum(ck * x ** i * y ** j for ck, (i, j) in zip(self.c, self.MONOMIALS))
+
+
 class FixedIntegralMean:
     """FixedIntegralMean(unknown, value) (src/globalconstraints.jl:23-44):
     the unknown's first dof is fixed during the solve (apply, :259-283) and
@@ -231,8 +262,13 @@ class PDEDescription:
     def add_rhsdata(self, j, op):
         self.RHS.setdefault(j, []).append(op)
 
-    def add_boundarydata(self, j, data=HomogeneousDirichletBoundary):
-        self.BoundaryOperators[j] = data
+    def add_boundarydata(self, j, kind=HomogeneousDirichletBoundary, data=None):
+        """add_boundarydata!(PDE, j, regions, kind; data) on all boundary
+        regions (src/pdedescription.jl); ``data`` is a PolynomialData for the
+        interpolated and best-approximated kinds."""
+        if kind is not HomogeneousDirichletBoundary and not isinstance(data, PolynomialData):
+            raise _abi.GrmpError(_abi.ERR_ARG, f"{kind.__name__} needs PolynomialData")
+        self.BoundaryOperators[j] = (kind, data)
 
     def add_constraint(self, c):
         self.GlobalConstraints.append(c)
@@ -317,6 +353,31 @@ def realize_constraints(pde, Solution):
         Solution.set_entries(x)
 
 
+def boundary_values(pde, Solution, rtol=1e-14):
+    """Target of boundarydata! (src/boundarydata.jl:198-352): a device vector
+    with the Dirichlet values on the boundary dofs of every unknown with
+    non-homogeneous data, or None when all data is homogeneous."""
+    target = None
+    for j, (kind, data) in sorted(pde.BoundaryOperators.items()):
+        if kind is HomogeneousDirichletBoundary:
+            continue
+        if target is None:
+            target = _abi.DeviceVector(Solution.ctx, Solution.n)
+            target.fill(0.0)
+        fes, off = Solution.FES[j], int(Solution.offsets[j])
+        if kind is InterpolateDirichletBoundary:
+            rc = lib().grmp_boundary_interpolate(fes.h, _abi.dptr(data.c), data.c.size, target.h, off)
+        elif kind is BestapproxDirichletBoundary:
+            it, rr = ctypes.c_int64(), ctypes.c_double()
+            q = fes.FEType.order + data.bonus_quadorder
+            rc = lib().grmp_boundary_bestapprox(fes.h, _abi.dptr(data.c), data.c.size, q, rtol, target.h, off,
+                                                ctypes.byref(it), ctypes.byref(rr))
+        else:
+            raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, f"boundary data {kind} is not built")
+        Solution.ctx.check(rc)
+    return target
+
+
 def _saddle_point(pde, Solution):
     """True when an unknown has no operator on its diagonal block (a Lagrange
     multiplier), which makes the system indefinite."""
@@ -341,6 +402,9 @@ def assemble(pde, Solution, values=_abi.VALUES_FP64, penalty=1e60):
             assemble_linearform(b, j, op)
     if fixed.size:
         b.vec.scatter(fixed + 1, penalty * 0.0)
+        target = boundary_values(pde, Solution)
+        if target is not None:
+            A.ctx.check(lib().grmp_matrix_penalize_rhs(A.h, target.h, b.vec.h))
     return A, b, fixed
 
 

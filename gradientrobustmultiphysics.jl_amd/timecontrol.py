@@ -17,15 +17,16 @@ subiteration holding every unknown, time-independent data):
   change norms drop below the threshold.
 
 CrankNicolson, nonlinear operators, dt_action / dt_lump and time-dependent
-data are not built and raise GrmpError(ERR_UNSUPPORTED).
+data and non-homogeneous Dirichlet data are not built and raise
+GrmpError(ERR_UNSUPPORTED).
 """
 import numpy as np
 
 from . import _abi
 from ._abi import lib
 from .fe import FEMatrix, FEVector
-from .pde import (ReactionOperator, SolverConfig, _saddle_point, assemble_linearform, fixed_dofs,
-                  realize_constraints, solve_system)
+from .pde import (HomogeneousDirichletBoundary, ReactionOperator, SolverConfig, _saddle_point, assemble_linearform,
+                  fixed_dofs, realize_constraints, solve_system)
 
 
 class BackwardEuler:
@@ -51,6 +52,8 @@ class TimeControlSolver:
         if not which or which[0] < 0 or which[-1] >= nu:
             raise _abi.GrmpError(_abi.ERR_ARG, "timedependent_equations must name at least one unknown "
                                  f"in 0..{nu - 1}")
+        if any(kind is not HomogeneousDirichletBoundary for kind, _ in pde.BoundaryOperators.values()):
+            raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "TimeControlSolver: only homogeneous Dirichlet data is built")
         self.PDE, self.X, self.which = pde, InitialValues, which
         self.SC = config or SolverConfig(**kwargs)
         self.values = _abi.VALUES_FP64 | (_abi.VALUES_FP32 if self.SC.precision == "fp32_ir" else 0)

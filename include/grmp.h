@@ -245,6 +245,25 @@ int grmp_assemble_linearform(grmp_ctx *ctx, const grmp_space *space, const grmp_
 int grmp_integrate_identity(grmp_ctx *ctx, const grmp_space *space, int quadorder,
                             const grmp_vector *u, int64_t offset, double *out);
 
+/* ------------------------------------------------------ Dirichlet data */
+/* Boundary values of a scalar H1P1 / H1P2 space (triangle meshes) for the
+ * polynomial data g = sum_k coef[k] m_k, m = (1, x, y, x^2, xy, y^2, x^3,
+ * x^2 y, x y^2, y^3), ncoef <= 10, written to target[offset + dof] for the
+ * dofs of all boundary faces (other entries untouched).
+ * InterpolateDirichletBoundary (src/boundarydata.jl:198-257): g at the dof
+ * points. */
+int grmp_boundary_interpolate(const grmp_space *space, const double *coef, int ncoef, grmp_vector *target,
+                              int64_t offset);
+/* BestapproxDirichletBoundary (src/boundarydata.jl:259-352): the L2
+ * projection of g onto the traces on the boundary faces (face mass matrix and
+ * load of g with a Gauss rule of order quadorder <= 7), solved to relative
+ * residual rtol; iterations / relres may be NULL. */
+int grmp_boundary_bestapprox(const grmp_space *space, const double *coef, int ncoef, int quadorder, double rtol,
+                             grmp_vector *target, int64_t offset, int64_t *iterations, double *relres);
+/* b_d = penalty * target_d on the fixed rows of A (solve_direct!,
+ * src/solvers.jl:647-652, with Target from boundarydata!). */
+int grmp_matrix_penalize_rhs(const grmp_matrix *A, const grmp_vector *target, grmp_vector *b);
+
 /* ---------------------------------------------------------------- solve */
 void grmp_solve_opts_default(grmp_solve_opts *o);
 /* Solves the penalized system A x = b (b[fixed] = penalty*value) as

@@ -222,6 +222,87 @@ def direct_solve(A, b):
     return spla.spsolve(A.to_scipy().tocsc(), b)
 
 
+# ------------------------------------------------------- Dirichlet data
+
+def poly_eval(c, x, y):
+    """sum_k c_k m_k(x, y) over 1, x, y, x^2, xy, y^2, x^3, x^2 y, x y^2, y^3."""
+    mons = (1.0, x, y, x * x, x * y, y * y, x * x * x, x * x * y, x * y * y, y * y * y)
+    return sum(ck * m for ck, m in zip(c, mons))
+
+
+def boundary_faces(cells, cellfaces, facecells):
+    """(cell, local face) of every boundary face, walked in cell order and
+    local face order; a face is on the boundary when one cell holds it."""
+    out = []
+    for c in range(cells.shape[0]):
+        for k in range(3):
+            if facecells[cellfaces[c, k], 1] < 0:
+                out.append((c, k))
+    return out
+
+
+def _face_dofs(fe, celldofs, c, k):
+    d = [celldofs[c, k], celldofs[c, (k + 1) % 3]]
+    if fe == FE_P2:
+        d.append(celldofs[c, 3 + k])
+    return d
+
+
+def _trace_basis(fe, s):
+    if fe == FE_P1:
+        return np.array([1.0 - s, s])
+    return np.array([(1.0 - s) * (1.0 - 2.0 * s), s * (2.0 * s - 1.0), 4.0 * s * (1.0 - s)])
+
+
+def boundary_interpolate(fe, coef, coords, cells, celldofs, cellfaces, facecells, ndofs):
+    """InterpolateDirichletBoundary (src/boundarydata.jl:198-257): the data at
+    the points of the boundary faces' dofs; NaN off the boundary."""
+    t = np.full(ndofs, np.nan)
+    for c, k in boundary_faces(cells, cellfaces, facecells):
+        a, b = cells[c, k], cells[c, (k + 1) % 3]
+        pts = [coords[a], coords[b]] + ([0.5 * (coords[a] + coords[b])] if fe == FE_P2 else [])
+        for d, p in zip(_face_dofs(fe, celldofs, c, k), pts):
+            t[d] = poly_eval(coef, p[0], p[1])
+    return t
+
+
+def boundary_bestapprox(fe, coef, quadorder, coords, cells, celldofs, cellfaces, facecells, ndofs):
+    """BestapproxDirichletBoundary (src/boundarydata.jl:259-352): mass matrix
+    and load of the data on the boundary faces (Gauss-Legendre with
+    ceil((quadorder+1)/2) points, at least 2 (P1) / 3 (P2)), compressed to
+    the boundary dofs and solved by sparse LU (:322-344).  Returns
+    (target with NaN off the boundary, compressed matrix, boundary dofs)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    nq = max((quadorder + 2) // 2, 3 if fe == FE_P2 else 2)
+    xg, wg = np.polynomial.legendre.leggauss(nq)
+    sq, wq = 0.5 * (xg + 1.0), 0.5 * wg
+    faces = boundary_faces(cells, cellfaces, facecells)
+    bd = sorted({d for c, k in faces for d in _face_dofs(fe, celldofs, c, k)})
+    comp = {d: r for r, d in enumerate(bd)}
+    rows, cols, vals = [], [], []
+    load = np.zeros(len(bd))
+    for c, k in faces:
+        a, b = cells[c, k], cells[c, (k + 1) % 3]
+        p0, p1 = coords[a], coords[b]
+        L = float(np.hypot(*(p1 - p0)))
+        dofs = _face_dofs(fe, celldofs, c, k)
+        phi = np.array([_trace_basis(fe, s) for s in sq])            # [nq, nloc]
+        Mloc = L * (phi.T * wq) @ phi
+        gq = np.array([poly_eval(coef, *(p0 + s * (p1 - p0))) for s in sq])
+        floc = L * (phi.T @ (wq * gq))
+        for i, di in enumerate(dofs):
+            load[comp[di]] += floc[i]
+            for j, dj in enumerate(dofs):
+                rows.append(comp[di])
+                cols.append(comp[dj])
+                vals.append(Mloc[i, j])
+    Mb = sp.csc_matrix((vals, (rows, cols)), shape=(len(bd), len(bd)))
+    t = np.full(ndofs, np.nan)
+    t[bd] = spla.spsolve(Mb, load)
+    return t, Mb, np.array(bd, dtype=np.int64)
+
+
 # ------------------------------------------------------- time stepping
 
 def backward_euler_rhs(M, f, u, dt, fixed, penalty=PENALTY):
