@@ -398,6 +398,13 @@ def run_workload(grmp, which, args, log):
                          "rtol": args.rtol}
         res["boundary_dofs_per_s_bestapprox"] = round(nb / (res["bestapprox"]["boundary_values_ms"] * 1e-3), 1)
         out["dirichlet"] = res
+        log("[bench] dirichlet: node values of the solution (export)")
+        grmp.nodevalues(sol[0])
+        t0 = time.perf_counter()
+        nv = grmp.nodevalues(sol[0])
+        te = time.perf_counter() - t0
+        out["nodevalues"] = {"ms": round(te * 1e3, 3), "nodes": int(nv.shape[1]),
+                             "includes": "node-cell adjacency, cell-order averages, copy to the host"}
     else:
         raise SystemExit(f"unknown workload {which}")
     print(json.dumps(out), flush=True)

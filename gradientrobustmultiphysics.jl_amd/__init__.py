@@ -19,6 +19,7 @@ from .pde import (APT_BilinearForm, APT_SymmetricBilinearForm, BestapproxDirichl
                   PoissonProblem, ReactionOperator, ReconstructionIdentity, SolverConfig, SymmetricGradient, assemble,
                   assemble_linearform, realize_constraints, solve, solve_system)
 
+from .dataexport import nodevalues, writeCSV  # noqa: F401
 from .timecontrol import BackwardEuler, CrankNicolson, TimeControlSolver  # noqa: F401
 from . import server  # noqa: F401
 from .server import Server  # noqa: F401

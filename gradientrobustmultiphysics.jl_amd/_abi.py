@@ -142,6 +142,7 @@ SIGNATURES = {
     "grmp_boundary_bestapprox": (ctypes.c_int, [VP, DP, ctypes.c_int, ctypes.c_int, ctypes.c_double, VP, I64,
                                                 I64P, DP]),
     "grmp_matrix_penalize_rhs": (ctypes.c_int, [VP, VP, VP]),
+    "grmp_nodevalues": (ctypes.c_int, [VP, VP, I64, DP]),
     "grmp_assemble_linearform": (ctypes.c_int, [VP, VP, ctypes.POINTER(LForm), VP, I64]),
     "grmp_integrate_identity": (ctypes.c_int, [VP, VP, ctypes.c_int, VP, I64, DP]),
     "grmp_solve_opts_default": (None, [ctypes.POINTER(SolveOpts)]),

@@ -1,7 +1,8 @@
 // Inhomogeneous Dirichlet data of libgrmp_hip.so: the boundary values of an
 // H1 P1/P2 space for polynomial data g, by interpolation or by the L2 best
 // approximation on the boundary faces, and the penalized right-hand side
-// that carries them into the solve.
+// that carries them into the solve.  At the end: the node values of a
+// solution for export.
 //
 // boundarydata! (src/boundarydata.jl:198-395):
 //  * InterpolateDirichletBoundary: the dofs of the boundary faces take the
@@ -568,4 +569,89 @@ extern "C" int grmp_matrix_penalize_rhs(const grmp_matrix *A, const grmp_vector 
                        target->d, b->d, A->n);
     GRMP_HIP(ctx, hipGetLastError());
     return GRMP_OK;
+}
+
+// ------------------------------------------------------------ node values
+// nodevalues!(target, source, FE, Identity) with continuous = false
+// (src/interpolations.jl:704-812; L2P0: src/fedefs/l2_p0.jl:52-68): per node
+// and component the values of the cells around the node at that vertex,
+// summed in cell order and divided by their number.  At a vertex every
+// basis function of P1 / P2 / BR is exactly 0 or 1 (the BR bubbles vanish),
+// so a cell's value there is the coefficient of the vertex's dof; for P0 it
+// is the cell's coefficient.
+__global__ void k_node_count(const int32_t *__restrict__ cells, int64_t nc, int32_t *__restrict__ cnt)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 3 * nc) atomicAdd(&cnt[cells[t]], 1);
+}
+
+__global__ void k_node_fill(const int32_t *__restrict__ cells, int64_t nc, const int32_t *__restrict__ ptr,
+                            int32_t *__restrict__ cur, int32_t *__restrict__ nc_inc)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * nc) return;
+    const int32_t n = cells[t];
+    const int64_t c = t % nc, k = t / nc;  // cells is SoA [3][nc]
+    nc_inc[ptr[n] + atomicAdd(&cur[n], 1)] = (int32_t)(c * 4 + k);
+}
+
+__global__ void k_nodevalues(const int32_t *__restrict__ ptr, const int32_t *__restrict__ nc_inc,
+                             const int32_t *__restrict__ celldofs, int64_t nc, int64_t nnodes, int fe, int ncomp,
+                             int per_comp, const double *__restrict__ u, double *__restrict__ out)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nnodes * ncomp) return;
+    const int64_t n = t / ncomp;
+    const int c = (int)(t - n * ncomp);
+    double acc = 0.0;
+    for (int32_t p = ptr[n]; p < ptr[n + 1]; ++p) {
+        const int64_t cell = nc_inc[p] >> 2;
+        const int k = nc_inc[p] & 3;
+        const int loc = fe == GRMP_FE_L2P0 ? c : c * per_comp + k;
+        acc += u[celldofs[(int64_t)loc * nc + cell]];
+    }
+    const int32_t cnt = ptr[n + 1] - ptr[n];
+    out[t] = cnt > 0 ? acc / (double)cnt : 0.0;
+}
+
+extern "C" int grmp_nodevalues(const grmp_space *s, const grmp_vector *u, int64_t offset, double *out)
+{
+    if (!s || !u || !out) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = s->ctx;
+    const grmp_mesh *m = s->mesh;
+    GRMP_CHECK(ctx, m->dim == 2 && m->nloc == 3, GRMP_ERR_UNSUPPORTED, "grmp_nodevalues: triangle meshes only");
+    GRMP_CHECK(ctx, s->fe == GRMP_FE_H1P1 || s->fe == GRMP_FE_H1P2 || s->fe == GRMP_FE_H1BR || s->fe == GRMP_FE_L2P0,
+               GRMP_ERR_UNSUPPORTED, "grmp_nodevalues: FE %d is not built", s->fe);
+    GRMP_CHECK(ctx, u->ctx == ctx && offset >= 0 && offset + s->ndofs <= u->n, GRMP_ERR_ARG,
+               "grmp_nodevalues: coefficient vector too short at offset %lld", (long long)offset);
+    const int ncomp = s->fe == GRMP_FE_H1BR ? 2 : s->ncomp;
+    const int per_comp = s->fe == GRMP_FE_H1P2 ? 6 : 3;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    const int64_t nc = m->ncells, nn = m->nnodes;
+    int32_t *cnt = nullptr, *ptr = nullptr, *cur = nullptr, *inc = nullptr;
+    double *dout = nullptr;
+    int rc = GRMP_OK;
+    do {
+        if ((rc = dalloc(ctx, &cnt, nn)) || (rc = dalloc(ctx, &ptr, nn + 1)) || (rc = dalloc(ctx, &cur, nn)) ||
+            (rc = dalloc(ctx, &inc, 3 * nc)) || (rc = dalloc(ctx, &dout, nn * ncomp)))
+            break;
+        GRMP_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(int32_t) * nn, ctx->stream));
+        GRMP_HIP(ctx, hipMemsetAsync(cur, 0, sizeof(int32_t) * nn, ctx->stream));
+        hipLaunchKernelGGL(k_node_count, dim3(nblk(3 * nc, 256)), dim3(256), 0, ctx->stream, m->cells, nc, cnt);
+        if ((rc = exclusive_scan_i32(ctx, cnt, ptr, nn))) break;
+        hipLaunchKernelGGL(k_node_fill, dim3(nblk(3 * nc, 256)), dim3(256), 0, ctx->stream, m->cells, nc, ptr, cur, inc);
+        hipLaunchKernelGGL(k_bd_sort, dim3(nblk(nn, 256)), dim3(256), 0, ctx->stream, ptr, nn, inc);
+        hipLaunchKernelGGL(k_nodevalues, dim3(nblk(nn * ncomp, 256)), dim3(256), 0, ctx->stream, ptr, inc,
+                           s->celldofs, nc, nn, s->fe, ncomp, per_comp, u->d + offset, dout);
+        GRMP_HIP(ctx, hipGetLastError());
+        if (hipMemcpyAsync(out, dout, sizeof(double) * nn * ncomp, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "grmp_nodevalues: download");
+    } while (0);
+    dfree(ctx, cnt);
+    dfree(ctx, ptr);
+    dfree(ctx, cur);
+    dfree(ctx, inc);
+    dfree(ctx, dout);
+    return rc;
 }

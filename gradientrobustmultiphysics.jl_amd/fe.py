@@ -170,8 +170,8 @@ class FESpace:
 # ------------------------------------------------------------------ FEVector
 
 class FEVectorBlock:
-    def __init__(self, parent, FES, offset):
-        self.parent, self.FES, self.offset = parent, FES, offset
+    def __init__(self, parent, FES, offset, name=""):
+        self.parent, self.FES, self.offset, self.name = parent, FES, offset, name
 
     def __len__(self):
         return self.FES.ndofs
@@ -190,7 +190,9 @@ class FEVector:
         self.offsets = np.concatenate([[0], np.cumsum([s.ndofs for s in spaces])]).astype(np.int64)
         self.n = int(self.offsets[-1])
         self.vec = _abi.DeviceVector(self.ctx, self.n)
-        self.blocks = [FEVectorBlock(self, s, int(o)) for s, o in zip(spaces, self.offsets[:-1])]
+        self.name = name
+        self.blocks = [FEVectorBlock(self, s, int(o), getattr(s, "name", "") or f"{name or 'u'}[{k + 1}]")
+                       for k, (s, o) in enumerate(zip(spaces, self.offsets[:-1]))]
 
     def __getitem__(self, i):
         return self.blocks[i]

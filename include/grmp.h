@@ -264,6 +264,13 @@ int grmp_boundary_bestapprox(const grmp_space *space, const double *coef, int nc
  * src/solvers.jl:647-652, with Target from boundarydata!). */
 int grmp_matrix_penalize_rhs(const grmp_matrix *A, const grmp_vector *target, grmp_vector *b);
 
+/* nodevalues!(target, source, FE, Identity) as writeCSV! uses it
+ * (src/dataexport.jl:22-77, src/interpolations.jl:704-812,
+ * src/fedefs/l2_p0.jl:52-68): per node the values at that vertex of the cells
+ * around it, summed in cell order and divided by their number.  H1P1, H1P2,
+ * H1BR, L2P0 on triangles; out [ncomp, nnodes] column-major (host). */
+int grmp_nodevalues(const grmp_space *space, const grmp_vector *u, int64_t offset, double *out);
+
 /* ---------------------------------------------------------------- solve */
 void grmp_solve_opts_default(grmp_solve_opts *o);
 /* Solves the penalized system A x = b (b[fixed] = penalty*value) as

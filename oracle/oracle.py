@@ -303,6 +303,27 @@ def boundary_bestapprox(fe, coef, quadorder, coords, cells, celldofs, cellfaces,
     return t, Mb, np.array(bd, dtype=np.int64)
 
 
+# ------------------------------------------------------------- export
+
+def nodevalues(fe, ncomp, u, cells, celldofs, nnodes):
+    """nodevalues!(target, source, FE, Identity) with continuous = false
+    (src/interpolations.jl:704-812; L2P0 src/fedefs/l2_p0.jl:52-68): per node
+    the cells' values at that vertex, summed in cell order, over their number.
+    Vertex values of P1/P2/BR are the vertex dof's coefficient, of P0 the
+    cell's.  Returns [ncomp, nnodes]."""
+    per = 6 if fe == FE_P2 else 3
+    out = np.zeros((ncomp, nnodes))
+    cnt = np.zeros(nnodes, dtype=np.int64)
+    for c in range(cells.shape[0]):
+        for k in range(3):
+            n = cells[c, k]
+            cnt[n] += 1
+            for m in range(ncomp):
+                loc = m if fe == FE_P0 else m * per + k
+                out[m, n] += u[celldofs[c, loc]]
+    return out / np.maximum(cnt, 1)
+
+
 # ------------------------------------------------------- time stepping
 
 def backward_euler_rhs(M, f, u, dt, fixed, penalty=PENALTY):
