@@ -30,11 +30,12 @@
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 #define BD_NCOEF 10  // g(x, y) = sum of c_k m_k, m = 1, x, y, x^2, xy, y^2, x^3, x^2 y, x y^2, y^3
+#define BD_MAXCOMP 2 // components of vector-valued data (one polynomial each)
 #define BD_MAXQ 4    // Gauss-Legendre points on a segment (exact to degree 7)
 #define BD_CG_T 1024
 
 struct BdData {
-    double c[BD_NCOEF];
+    double c[BD_MAXCOMP][BD_NCOEF];
 };
 
 struct BdMesh {
@@ -45,21 +46,23 @@ struct BdMesh {
     const int32_t *celldofs;  // [nloc][ncells]
     int64_t nnodes, ncells;
     int N, fe;
+    int ncomp, per;           // components; local dofs per component (3 P1, 6 P2)
 };
 
-__device__ __forceinline__ double bd_g(const BdData &g, double x, double y)
+__device__ __forceinline__ double bd_g(const BdData &g, int comp, double x, double y)
 {
+    const double *c = g.c[comp];
     double s = 0.0;
-    s += g.c[0];
-    s += g.c[1] * x;
-    s += g.c[2] * y;
-    s += g.c[3] * (x * x);
-    s += g.c[4] * (x * y);
-    s += g.c[5] * (y * y);
-    s += g.c[6] * (x * x * x);
-    s += g.c[7] * (x * x * y);
-    s += g.c[8] * (x * y * y);
-    s += g.c[9] * (y * y * y);
+    s += c[0];
+    s += c[1] * x;
+    s += c[2] * y;
+    s += c[3] * (x * x);
+    s += c[4] * (x * y);
+    s += c[5] * (y * y);
+    s += c[6] * (x * x * x);
+    s += c[7] * (x * x * y);
+    s += c[8] * (x * y * y);
+    s += c[9] * (y * y * y);
     return s;
 }
 
@@ -103,17 +106,28 @@ __device__ __forceinline__ bool bd_is_boundary(const BdMesh &M, int64_t c, int k
     return (ia == 0 && ib == 0) || (ia == M.N && ib == M.N) || (ja == 0 && jb == 0) || (ja == M.N && jb == M.N);
 }
 
-// local dofs of boundary face (c, k): global dof numbers
-__device__ __forceinline__ int bd_face_dofs(const BdMesh &M, int64_t c, int k, int32_t d[3])
+// local dofs of component comp on boundary face (c, k): global dof numbers
+__device__ __forceinline__ int bd_face_dofs(const BdMesh &M, int64_t c, int k, int comp, int32_t d[3])
 {
-    const int64_t nc = M.ncells;
-    d[0] = M.celldofs[k * nc + c];
-    d[1] = M.celldofs[((k + 1) % 3) * nc + c];
+    const int64_t nc = M.ncells, base = (int64_t)comp * M.per;
+    d[0] = M.celldofs[(base + k) * nc + c];
+    d[1] = M.celldofs[(base + (k + 1) % 3) * nc + c];
     if (M.fe == GRMP_FE_H1P2) {
-        d[2] = M.celldofs[(3 + k) * nc + c];
+        d[2] = M.celldofs[(base + 3 + k) * nc + c];
         return 3;
     }
     return 2;
+}
+
+// a boundary face of one component: v = i ncomp + comp over the face list
+__device__ __forceinline__ int bd_vface(const BdMesh &M, const int32_t *bf, int64_t v, int64_t &c, int &k, int32_t d[3],
+                                        int &comp)
+{
+    const int64_t i = v / M.ncomp;
+    comp = (int)(v - i * M.ncomp);
+    c = bf[i] / 3;
+    k = bf[i] - 3 * (int)c;
+    return bd_face_dofs(M, c, k, comp, d);
 }
 
 __device__ __forceinline__ void bd_face_geom(const BdMesh &M, int64_t c, int k, double &x0, double &y0, double &x1,
@@ -145,14 +159,14 @@ __global__ void k_bd_compact(const int32_t *__restrict__ flag, const int32_t *__
 }
 
 // boundary dof flags (int32 for the scan)
-__global__ void k_bd_dofflag(BdMesh M, const int32_t *__restrict__ bf, int64_t nbf, int32_t *__restrict__ dflag)
+__global__ void k_bd_dofflag(BdMesh M, const int32_t *__restrict__ bf, int64_t nv, int32_t *__restrict__ dflag)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nbf) return;
-    const int64_t c = bf[i] / 3;
-    const int k = bf[i] - 3 * (int)c;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    int64_t c;
+    int k, comp;
     int32_t d[3];
-    const int nd = bd_face_dofs(M, c, k, d);
+    const int nd = bd_vface(M, bf, v, c, k, d, comp);
     for (int a = 0; a < nd; ++a) dflag[d[a]] = 1;
 }
 
@@ -170,31 +184,31 @@ __global__ void k_bd_rows(const int32_t *__restrict__ dflag, const int32_t *__re
 }
 
 // incidences (boundary face, local dof) per compressed row: count, fill, sort
-__global__ void k_bd_count(BdMesh M, const int32_t *__restrict__ bf, int64_t nbf, const int32_t *__restrict__ comp,
+__global__ void k_bd_count(BdMesh M, const int32_t *__restrict__ bf, int64_t nv, const int32_t *__restrict__ comp,
                            int32_t *__restrict__ cnt)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nbf) return;
-    const int64_t c = bf[i] / 3;
-    const int k = bf[i] - 3 * (int)c;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    int64_t c;
+    int k, cp;
     int32_t d[3];
-    const int nd = bd_face_dofs(M, c, k, d);
+    const int nd = bd_vface(M, bf, v, c, k, d, cp);
     for (int a = 0; a < nd; ++a) atomicAdd(&cnt[comp[d[a]]], 1);
 }
 
-__global__ void k_bd_fill(BdMesh M, const int32_t *__restrict__ bf, int64_t nbf, const int32_t *__restrict__ comp,
+__global__ void k_bd_fill(BdMesh M, const int32_t *__restrict__ bf, int64_t nv, const int32_t *__restrict__ comp,
                           const int32_t *__restrict__ ptr, int32_t *__restrict__ cur, int32_t *__restrict__ inc)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nbf) return;
-    const int64_t c = bf[i] / 3;
-    const int k = bf[i] - 3 * (int)c;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    int64_t c;
+    int k, cp;
     int32_t d[3];
-    const int nd = bd_face_dofs(M, c, k, d);
+    const int nd = bd_vface(M, bf, v, c, k, d, cp);
     for (int a = 0; a < nd; ++a) {
         const int32_t r = comp[d[a]];
         const int32_t slot = ptr[r] + atomicAdd(&cur[r], 1);
-        inc[slot] = (int32_t)(i * 4 + a);  // boundary face index, local dof
+        inc[slot] = (int32_t)(v * 4 + a);  // component face index, local dof
     }
 }
 
@@ -237,14 +251,14 @@ __device__ __forceinline__ double bd_row_apply(const BdSys &S, int64_t r, const 
 {
     double y = 0.0, dg = 0.0;
     for (int32_t p = S.ptr[r]; p < S.ptr[r + 1]; ++p) {
-        const int64_t i = S.inc[p] >> 2;
+        const int64_t v = S.inc[p] >> 2;
         const int a = S.inc[p] & 3;
         double len, x0, y0, x1, y1;
-        bd_face_len(S, i, len, x0, y0, x1, y1);
-        const int64_t c = S.bf[i] / 3;
-        const int k = S.bf[i] - 3 * (int)c;
+        bd_face_len(S, v / S.M.ncomp, len, x0, y0, x1, y1);
+        int64_t c;
+        int k, comp;
         int32_t d[3];
-        const int nd = bd_face_dofs(S.M, c, k, d);
+        const int nd = bd_vface(S.M, S.bf, v, c, k, d, comp);
         for (int b = 0; b < nd; ++b) {
             double m = 0.0;
             for (int q = 0; q < S.nq; ++q) {
@@ -265,16 +279,17 @@ __device__ __forceinline__ double bd_row_load(const BdSys &S, int64_t r)
 {
     double f = 0.0;
     for (int32_t p = S.ptr[r]; p < S.ptr[r + 1]; ++p) {
-        const int64_t i = S.inc[p] >> 2;
+        const int64_t v = S.inc[p] >> 2;
         const int a = S.inc[p] & 3;
+        const int comp = (int)(v % S.M.ncomp);
         double len, x0, y0, x1, y1;
-        bd_face_len(S, i, len, x0, y0, x1, y1);
+        bd_face_len(S, v / S.M.ncomp, len, x0, y0, x1, y1);
         double l = 0.0;
         for (int q = 0; q < S.nq; ++q) {
             double s, w;
             bd_gauss(S.nq, q, s, w);
             const double xq = x0 + s * (x1 - x0), yq = y0 + s * (y1 - y0);
-            l += w * (bd_g(S.g, xq, yq) * bd_phi(S.M.fe, a, s));
+            l += w * (bd_g(S.g, comp, xq, yq) * bd_phi(S.M.fe, a, s));
         }
         f += l * len;
     }
@@ -362,21 +377,21 @@ __global__ void k_bd_scatter(const double *__restrict__ xv, const int32_t *__res
 }
 
 // interpolation: g at the dof points of the boundary faces
-__global__ void k_bd_interp(BdMesh M, BdData g, const int32_t *__restrict__ bf, int64_t nbf,
+__global__ void k_bd_interp(BdMesh M, BdData g, const int32_t *__restrict__ bf, int64_t nv,
                             double *__restrict__ target, int64_t offset)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nbf) return;
-    const int64_t c = bf[i] / 3;
-    const int k = bf[i] - 3 * (int)c;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    int64_t c;
+    int k, comp;
     int32_t d[3];
-    const int nd = bd_face_dofs(M, c, k, d);
+    const int nd = bd_vface(M, bf, v, c, k, d, comp);
     double x0, y0, x1, y1;
     bd_face_geom(M, c, k, x0, y0, x1, y1);
     // a dof shared by two faces gets the same value from both (its point)
-    target[offset + d[0]] = bd_g(g, x0, y0);
-    target[offset + d[1]] = bd_g(g, x1, y1);
-    if (nd == 3) target[offset + d[2]] = bd_g(g, 0.5 * (x0 + x1), 0.5 * (y0 + y1));
+    target[offset + d[0]] = bd_g(g, comp, x0, y0);
+    target[offset + d[1]] = bd_g(g, comp, x1, y1);
+    if (nd == 3) target[offset + d[2]] = bd_g(g, comp, 0.5 * (x0 + x1), 0.5 * (y0 + y1));
 }
 
 __global__ void k_penalize_rhs(const uint8_t *__restrict__ fixed, double penalty, const double *__restrict__ target,
@@ -397,8 +412,8 @@ static int bd_check(grmp_ctx *ctx, const grmp_space *s, const double *coef, int 
 {
     GRMP_CHECK(ctx, s->mesh->dim == 2 && s->mesh->nloc == 3, GRMP_ERR_UNSUPPORTED,
                "Dirichlet data: triangle meshes only");
-    GRMP_CHECK(ctx, (s->fe == GRMP_FE_H1P1 || s->fe == GRMP_FE_H1P2) && s->ncomp == 1, GRMP_ERR_UNSUPPORTED,
-               "Dirichlet data: scalar H1P1 / H1P2 spaces only");
+    GRMP_CHECK(ctx, (s->fe == GRMP_FE_H1P1 || s->fe == GRMP_FE_H1P2) && s->ncomp >= 1 && s->ncomp <= BD_MAXCOMP,
+               GRMP_ERR_UNSUPPORTED, "Dirichlet data: H1P1 / H1P2 spaces with 1 or 2 components only");
     GRMP_CHECK(ctx, s->mesh->cellfaces || s->mesh->kind == MESH_UNITSQUARE, GRMP_ERR_UNSUPPORTED,
                "Dirichlet data: the mesh has no face adjacency");
     GRMP_CHECK(ctx, coef && ncoef >= 1 && ncoef <= BD_NCOEF, GRMP_ERR_ARG,
@@ -421,6 +436,8 @@ static BdMesh bd_mesh(const grmp_space *s)
     M.ncells = m->ncells;
     M.N = m->N;
     M.fe = s->fe;
+    M.ncomp = s->ncomp;
+    M.per = s->fe == GRMP_FE_H1P2 ? 6 : 3;
     return M;
 }
 
@@ -458,13 +475,15 @@ extern "C" int grmp_boundary_interpolate(const grmp_space *s, const double *coef
     std::lock_guard<std::mutex> g(ctx->mu);
     BdData gd;
     memset(&gd, 0, sizeof(gd));
-    for (int k = 0; k < ncoef; ++k) gd.c[k] = coef[k];
+    for (int m = 0; m < s->ncomp; ++m)
+        for (int k = 0; k < ncoef; ++k) gd.c[m][k] = coef[m * ncoef + k];
     const BdMesh M = bd_mesh(s);
     BdFaces F;
     int rc = bd_faces(ctx, M, F);
-    if (!rc && F.nbf) {
-        hipLaunchKernelGGL(k_bd_interp, dim3(nblk(F.nbf, 256)), dim3(256), 0, ctx->stream, M, gd, F.bf, F.nbf,
-                           target->d, offset);
+    const int64_t nv = F.nbf * s->ncomp;
+    if (!rc && nv) {
+        hipLaunchKernelGGL(k_bd_interp, dim3(nblk(nv, 256)), dim3(256), 0, ctx->stream, M, gd, F.bf, nv, target->d,
+                           offset);
         if (hipGetLastError() != hipSuccess) rc = grmp_fail(ctx, GRMP_ERR_HIP, "Dirichlet data: interpolation");
     }
     if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = grmp_fail(ctx, GRMP_ERR_HIP, "Dirichlet data");
@@ -485,7 +504,8 @@ extern "C" int grmp_boundary_bestapprox(const grmp_space *s, const double *coef,
     std::lock_guard<std::mutex> g(ctx->mu);
     BdSys S;
     memset(&S, 0, sizeof(S));
-    for (int k = 0; k < ncoef; ++k) S.g.c[k] = coef[k];
+    for (int m = 0; m < s->ncomp; ++m)
+        for (int k = 0; k < ncoef; ++k) S.g.c[m][k] = coef[m * ncoef + k];
     S.M = bd_mesh(s);
     // the mass matrix needs degree 2 p exactly, the load p + the data's degree
     S.nq = std::max((quadorder + 2) / 2, s->fe == GRMP_FE_H1P2 ? 3 : 2);
@@ -499,10 +519,11 @@ extern "C" int grmp_boundary_bestapprox(const grmp_space *s, const double *coef,
         if ((rc = bd_faces(ctx, S.M, F))) break;
         if (F.nbf == 0) break;
         const int per = s->fe == GRMP_FE_H1P2 ? 3 : 2;
+        const int64_t nv = F.nbf * s->ncomp;  // boundary faces of every component
         if ((rc = dalloc(ctx, &dflag, nd)) || (rc = dalloc(ctx, &dpos, nd + 1)) || (rc = dalloc(ctx, &comp, nd)))
             break;
         GRMP_HIP(ctx, hipMemsetAsync(dflag, 0, sizeof(int32_t) * nd, ctx->stream));
-        hipLaunchKernelGGL(k_bd_dofflag, dim3(nblk(F.nbf, 256)), dim3(256), 0, ctx->stream, S.M, F.bf, F.nbf, dflag);
+        hipLaunchKernelGGL(k_bd_dofflag, dim3(nblk(nv, 256)), dim3(256), 0, ctx->stream, S.M, F.bf, nv, dflag);
         if ((rc = exclusive_scan_i32(ctx, dflag, dpos, nd))) break;
         int32_t nb = 0;
         if (hipMemcpyAsync(&nb, dpos + nd, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
@@ -511,16 +532,16 @@ extern "C" int grmp_boundary_bestapprox(const grmp_space *s, const double *coef,
             break;
         }
         S.nb = nb;
-        const int64_t ninc = F.nbf * per;
+        const int64_t ninc = nv * per;
         if ((rc = dalloc(ctx, &bdof, nb)) || (rc = dalloc(ctx, &cnt, nb)) || (rc = dalloc(ctx, &ptr, nb + 1)) ||
             (rc = dalloc(ctx, &cur, nb)) || (rc = dalloc(ctx, &inc, ninc)) || (rc = dalloc(ctx, &work, 5 * (size_t)nb + 2)))
             break;
         hipLaunchKernelGGL(k_bd_rows, dim3(nblk(nd, 256)), dim3(256), 0, ctx->stream, dflag, dpos, nd, comp, bdof);
         GRMP_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(int32_t) * nb, ctx->stream));
         GRMP_HIP(ctx, hipMemsetAsync(cur, 0, sizeof(int32_t) * nb, ctx->stream));
-        hipLaunchKernelGGL(k_bd_count, dim3(nblk(F.nbf, 256)), dim3(256), 0, ctx->stream, S.M, F.bf, F.nbf, comp, cnt);
+        hipLaunchKernelGGL(k_bd_count, dim3(nblk(nv, 256)), dim3(256), 0, ctx->stream, S.M, F.bf, nv, comp, cnt);
         if ((rc = exclusive_scan_i32(ctx, cnt, ptr, nb))) break;
-        hipLaunchKernelGGL(k_bd_fill, dim3(nblk(F.nbf, 256)), dim3(256), 0, ctx->stream, S.M, F.bf, F.nbf, comp, ptr,
+        hipLaunchKernelGGL(k_bd_fill, dim3(nblk(nv, 256)), dim3(256), 0, ctx->stream, S.M, F.bf, nv, comp, ptr,
                            cur, inc);
         hipLaunchKernelGGL(k_bd_sort, dim3(nblk(nb, 256)), dim3(256), 0, ctx->stream, ptr, (int64_t)nb, inc);
         S.bf = F.bf;

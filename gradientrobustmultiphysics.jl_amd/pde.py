@@ -211,21 +211,25 @@ class PolynomialData:
     """Closed-form stand-in for a DataFunction of (x, y) on the device
     (src/userdata.jl:3-63): g = sum_k c_k m_k over the monomials
     1, x, y, x^2, xy, y^2, x^3, x^2 y, x y^2, y^3 (at most 10 coefficients);
-    bonus_quadorder defaults to the degree."""
+    a (ncomp, ncoef) array gives vector-valued data with 2 components at
+    most.  bonus_quadorder defaults to the degree."""
 
     MONOMIALS = ((0, 0), (1, 0), (0, 1), (2, 0), (1, 1), (0, 2), (3, 0), (2, 1), (1, 2), (0, 3))
 
     def __init__(self, coeffs, bonus_quadorder=None):
-        c = np.asarray(coeffs, dtype=np.float64).ravel()
-        if c.size < 1 or c.size > 10:
-            raise _abi.GrmpError(_abi.ERR_ARG, "PolynomialData: 1..10 coefficients")
-        self.c = np.ascontiguousarray(c)
-        nz = [k for k in range(c.size) if c[k] != 0.0]
+        c = np.asarray(coeffs, dtype=np.float64)
+        c = c.reshape(1, -1) if c.ndim <= 1 else c
+        if c.ndim != 2 or not 1 <= c.shape[0] <= 2 or not 1 <= c.shape[1] <= 10:
+            raise _abi.GrmpError(_abi.ERR_ARG, "PolynomialData: 1..10 coefficients for each of 1 or 2 components")
+        self.ncomp, self.ncoef = c.shape
+        self.c = np.ascontiguousarray(c.ravel())
+        nz = [k for k in range(self.ncoef) if np.any(c[:, k] != 0.0)]
         self.degree = max((sum(self.MONOMIALS[k]) for k in nz), default=0)
         self.bonus_quadorder = self.degree if bonus_quadorder is None else int(bonus_quadorder)
 
-    def __call__(self, x, y):
-        return sum(ck * x ** i * y ** j for ck, (i, j) in zip(self.c, self.MONOMIALS))
+    def __call__(self, x, y, comp=0):
+        row = self.c[comp * self.ncoef:(comp + 1) * self.ncoef]
+        return sum(ck * x ** i * y ** j for ck, (i, j) in zip(row, self.MONOMIALS))
 
 
 class FixedIntegralMean:
@@ -365,12 +369,15 @@ def boundary_values(pde, Solution, rtol=1e-14):
             target = _abi.DeviceVector(Solution.ctx, Solution.n)
             target.fill(0.0)
         fes, off = Solution.FES[j], int(Solution.offsets[j])
+        if data.ncomp != fes.FEType.ncomp:
+            raise _abi.GrmpError(_abi.ERR_ARG, f"boundary data with {data.ncomp} components for a space with "
+                                 f"{fes.FEType.ncomp}")
         if kind is InterpolateDirichletBoundary:
-            rc = lib().grmp_boundary_interpolate(fes.h, _abi.dptr(data.c), data.c.size, target.h, off)
+            rc = lib().grmp_boundary_interpolate(fes.h, _abi.dptr(data.c), data.ncoef, target.h, off)
         elif kind is BestapproxDirichletBoundary:
             it, rr = ctypes.c_int64(), ctypes.c_double()
             q = fes.FEType.order + data.bonus_quadorder
-            rc = lib().grmp_boundary_bestapprox(fes.h, _abi.dptr(data.c), data.c.size, q, rtol, target.h, off,
+            rc = lib().grmp_boundary_bestapprox(fes.h, _abi.dptr(data.c), data.ncoef, q, rtol, target.h, off,
                                                 ctypes.byref(it), ctypes.byref(rr))
         else:
             raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, f"boundary data {kind} is not built")

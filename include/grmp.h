@@ -246,10 +246,11 @@ int grmp_integrate_identity(grmp_ctx *ctx, const grmp_space *space, int quadorde
                             const grmp_vector *u, int64_t offset, double *out);
 
 /* ------------------------------------------------------ Dirichlet data */
-/* Boundary values of a scalar H1P1 / H1P2 space (triangle meshes) for the
- * polynomial data g = sum_k coef[k] m_k, m = (1, x, y, x^2, xy, y^2, x^3,
- * x^2 y, x y^2, y^3), ncoef <= 10, written to target[offset + dof] for the
- * dofs of all boundary faces (other entries untouched).
+/* Boundary values of an H1P1 / H1P2 space with 1 or 2 components (triangle
+ * meshes) for the polynomial data g_c = sum_k coef[c ncoef + k] m_k,
+ * m = (1, x, y, x^2, xy, y^2, x^3, x^2 y, x y^2, y^3), ncoef <= 10, coef
+ * holding ncomp * ncoef values; written to target[offset + dof] for the dofs
+ * of all boundary faces (other entries untouched).
  * InterpolateDirichletBoundary (src/boundarydata.jl:198-257): g at the dof
  * points. */
 int grmp_boundary_interpolate(const grmp_space *space, const double *coef, int ncoef, grmp_vector *target,

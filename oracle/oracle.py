@@ -303,6 +303,24 @@ def boundary_bestapprox(fe, coef, quadorder, coords, cells, celldofs, cellfaces,
     return t, Mb, np.array(bd, dtype=np.int64)
 
 
+def boundary_values_vec(kind, fe, ncomp, coef, quadorder, coords, cells, celldofs, cellfaces, facecells, ndofs):
+    """Vector-valued data: each component's dofs (the component's block of
+    the local dofs) get the scalar routine with that component's polynomial;
+    the boundary systems of the components are independent."""
+    per = 6 if fe == FE_P2 else 3
+    coef = np.atleast_2d(coef)
+    t = np.full(ndofs, np.nan)
+    for m in range(ncomp):
+        cd = np.ascontiguousarray(celldofs[:, m * per:(m + 1) * per])
+        if kind == "bestapprox":
+            tm = boundary_bestapprox(fe, coef[m], quadorder, coords, cells, cd, cellfaces, facecells, ndofs)[0]
+        else:
+            tm = boundary_interpolate(fe, coef[m], coords, cells, cd, cellfaces, facecells, ndofs)
+        sel = ~np.isnan(tm)
+        t[sel] = tm[sel]
+    return t
+
+
 # ------------------------------------------------------------- export
 
 def nodevalues(fe, ncomp, u, cells, celldofs, nnodes):

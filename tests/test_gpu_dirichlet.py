@@ -5,8 +5,9 @@ against the oracle.
 Bar: best-approximated boundary values within 1e-12 of max|target| of the
 oracle's (sparse LU of the same boundary system; the device PCG stops at
 relative residual 1e-14); interpolated values within 1e-14; entries off the
-boundary untouched; P2 (P1) Poisson solutions with quadratic (linear) data
-and a quadratic (linear) exact solution reproduce it at every dof to 1e-9."""
+boundary untouched; scalar and two-component spaces; P2 (P1) Poisson and P1{2}
+elasticity solutions with quadratic (linear) data and a quadratic (linear)
+exact solution reproduce it at every dof to 1e-9."""
 import ctypes
 
 import numpy as np
@@ -49,10 +50,10 @@ def test_boundary_values_match_oracle(grmp, O, fe, N, seed):
         t = grmp.DeviceVector(grid.ctx, nd + 5, sentinel)
         if kind == "bestapprox":
             it, rr = ctypes.c_int64(), ctypes.c_double()
-            rc = grmp.lib().grmp_boundary_bestapprox(fes.h, grmp._abi.dptr(data.c), data.c.size, q, 1e-14, t.h, 5,
+            rc = grmp.lib().grmp_boundary_bestapprox(fes.h, grmp._abi.dptr(data.c), data.ncoef, q, 1e-14, t.h, 5,
                                                      ctypes.byref(it), ctypes.byref(rr))
         else:
-            rc = grmp.lib().grmp_boundary_interpolate(fes.h, grmp._abi.dptr(data.c), data.c.size, t.h, 5)
+            rc = grmp.lib().grmp_boundary_interpolate(fes.h, grmp._abi.dptr(data.c), data.ncoef, t.h, 5)
         grid.ctx.check(rc)
         v = t.download()
         assert np.all(v[:5] == 7.0)
@@ -93,12 +94,63 @@ def test_poisson_with_dirichlet_data(grmp, O, fe, kind):
     assert err <= 1e-9
 
 
+@pytest.mark.parametrize("fe", [1, 2])
+def test_vector_boundary_values(grmp, O, fe):
+    """Two components with their own cubic polynomials on a shuffled mesh."""
+    N, seed = 6, 8
+    print(f"[case] vector boundary values fe={fe}", flush=True)
+    grid, coords, cells = _grid(grmp, O, N, seed)
+    fes = grmp.FESpace(grmp.H1P1(2) if fe == 1 else grmp.H1P2(2), grid)
+    cf, fn, fc = O.tri_faces(cells, coords.shape[0])
+    coef = np.array([CUBIC, CUBIC[::-1]])
+    data = grmp.PolynomialData(coef)
+    q = fe + data.bonus_quadorder
+    nd = fes.ndofs
+    for kind in ("bestapprox", "interpolate"):
+        ref = O.boundary_values_vec(kind, fe, 2, coef, q, coords, cells, fes.celldofs(), cf, fc, nd)
+        bd = np.nonzero(~np.isnan(ref))[0]
+        assert np.array_equal(bd, fes.boundary_dofs())
+        t = grmp.DeviceVector(grid.ctx, nd, np.zeros(nd))
+        if kind == "bestapprox":
+            rc = grmp.lib().grmp_boundary_bestapprox(fes.h, grmp._abi.dptr(data.c), data.ncoef, q, 1e-14, t.h, 0,
+                                                     None, None)
+        else:
+            rc = grmp.lib().grmp_boundary_interpolate(fes.h, grmp._abi.dptr(data.c), data.ncoef, t.h, 0)
+        grid.ctx.check(rc)
+        v = t.download()
+        err = np.max(np.abs(v[bd] - ref[bd])) / np.max(np.abs(ref[bd]))
+        print(f"  {kind}: max rel {err:.2e}")
+        assert err <= (1e-12 if kind == "bestapprox" else 1e-14)
+
+
+def test_elasticity_with_linear_displacement(grmp, O):
+    """A linear displacement has constant stress, so with f = 0 and its
+    boundary values the P1{2} elasticity solution reproduces it."""
+    N, seed = 8, 9
+    print("[case] elasticity with linear Dirichlet displacement", flush=True)
+    grid, coords, cells = _grid(grmp, O, N, seed)
+    fes = grmp.FESpace(grmp.H1P1(2), grid)
+    coef = np.array([[0.1, 0.5, -0.3], [-0.2, 0.25, 0.4]])
+    pde = grmp.LinearElasticityProblem(1.0, 0.3)
+    pde.add_rhsdata(0, grmp.LinearForm(grmp.Identity, grmp.DataFunction([0.0, 0.0])))
+    pde.add_boundarydata(0, grmp.BestapproxDirichletBoundary, data=grmp.PolynomialData(coef))
+    sol = grmp.FEVector([fes])
+    info, A, b = grmp.solve(sol, pde, precision="fp64", rtol=1e-13)
+    assert info.status == 0
+    x = sol.entries()
+    nn = coords.shape[0]
+    u = np.concatenate([O.poly_eval(coef[m], coords[:, 0], coords[:, 1]) for m in range(2)])
+    err = np.max(np.abs(x - u)) / np.max(np.abs(u))
+    print(f"  {info.iterations} iterations, max rel err vs exact {err:.2e}")
+    assert x.size == 2 * nn and err <= 1e-9
+
+
 def test_dirichlet_data_errors(grmp):
     print("[case] dirichlet data argument errors", flush=True)
     grid = grmp.grid_unitsquare(4)
-    fesv = grmp.FESpace(grmp.H1P1(2), grid)
+    fesv = grmp.FESpace(grmp.H1BR(2), grid)
     t = grmp.DeviceVector(grid.ctx, fesv.ndofs)
-    c = np.array([1.0])
+    c = np.array([1.0, 1.0])
     with pytest.raises(grmp.GrmpError):
         grid.ctx.check(grmp.lib().grmp_boundary_interpolate(fesv.h, grmp._abi.dptr(c), 1, t.h, 0))
     fes = grmp.FESpace(grmp.H1P1(1), grid)
@@ -109,3 +161,7 @@ def test_dirichlet_data_errors(grmp):
         grmp.PolynomialData(np.ones(11))
     with pytest.raises(grmp.GrmpError):
         grmp.PoissonProblem().add_boundarydata(0, grmp.BestapproxDirichletBoundary)
+    pde = grmp.PoissonProblem()
+    pde.add_boundarydata(0, grmp.BestapproxDirichletBoundary, data=grmp.PolynomialData([[1.0], [2.0]]))
+    with pytest.raises(grmp.GrmpError):  # two-component data on a scalar space
+        grmp.boundary_values(pde, grmp.FEVector([fes]))
