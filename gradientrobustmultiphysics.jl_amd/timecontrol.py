@@ -3,8 +3,8 @@
 Python mirror of src/solvers_timedependent.jl for linear problems (one
 subiteration holding every unknown, time-independent data):
 
-* TimeControlSolver (:116-281): fixed dofs and homogeneous boundary values of
-  the initial state, the right-hand side f assembled once, the mass matrix M
+* TimeControlSolver (:116-281): fixed dofs and the Dirichlet values of the
+  initial state, the right-hand side f assembled once, the mass matrix M
   of the time-dependent equations (Identity x Identity, dt_operator's default)
   assembled once.
 * advance! (:293-594): the system S = A + M/dt is assembled on the first step
@@ -16,17 +16,17 @@ subiteration holding every unknown, time-independent data):
 * advance_until_stationarity! (:605-665): steps until the summed per-unknown
   change norms drop below the threshold.
 
-CrankNicolson, nonlinear operators, dt_action / dt_lump and time-dependent
-data and non-homogeneous Dirichlet data are not built and raise
-GrmpError(ERR_UNSUPPORTED).
+Dirichlet data (homogeneous, interpolated or best-approximated) is fixed in
+time.  CrankNicolson, nonlinear operators, dt_action / dt_lump and
+time-dependent data are not built and raise GrmpError(ERR_UNSUPPORTED).
 """
 import numpy as np
 
 from . import _abi
 from ._abi import lib
 from .fe import FEMatrix, FEVector
-from .pde import (HomogeneousDirichletBoundary, ReactionOperator, SolverConfig, _saddle_point, assemble_linearform,
-                  fixed_dofs, realize_constraints, solve_system)
+from .pde import (ReactionOperator, SolverConfig, _saddle_point, assemble_linearform, boundary_values, fixed_dofs,
+                  realize_constraints, solve_system)
 
 
 class BackwardEuler:
@@ -52,19 +52,20 @@ class TimeControlSolver:
         if not which or which[0] < 0 or which[-1] >= nu:
             raise _abi.GrmpError(_abi.ERR_ARG, "timedependent_equations must name at least one unknown "
                                  f"in 0..{nu - 1}")
-        if any(kind is not HomogeneousDirichletBoundary for kind, _ in pde.BoundaryOperators.values()):
-            raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "TimeControlSolver: only homogeneous Dirichlet data is built")
         self.PDE, self.X, self.which = pde, InitialValues, which
         self.SC = config or SolverConfig(**kwargs)
         self.values = _abi.VALUES_FP64 | (_abi.VALUES_FP32 if self.SC.precision == "fp32_ir" else 0)
         self.method = _abi.SOLVE_MINRES if _saddle_point(pde, InitialValues) else _abi.SOLVE_CG
         self.fixed = fixed_dofs(pde, InitialValues)
-        # boundarydata!: homogeneous values on the boundary dofs of the initial state (:194-200)
+        # boundarydata!: the Dirichlet values on the boundary dofs of the initial state (:194-200);
+        # the penalty rows of every step keep them
         bnd = [InitialValues.FES[j].boundary_dofs() + int(InitialValues.offsets[j])
                for j in sorted(pde.BoundaryOperators)]
         if bnd:
+            bnd = np.concatenate(bnd)
+            target = boundary_values(pde, InitialValues)
             x = InitialValues.entries()
-            x[np.concatenate(bnd)] = 0.0
+            x[bnd] = 0.0 if target is None else target.download()[bnd]
             InitialValues.set_entries(x)
         FES = InitialValues.FES
         # mass matrix of the time derivative, assembled once (dt_is_nonlinear = false)

@@ -1,7 +1,7 @@
 """BackwardEuler time stepping on the device (TimeControlSolver and
 advance!, src/solvers_timedependent.jl:116-594) against the oracle's steps
 on the P1 heat equation u_t - div(kappa grad u) = 1 with homogeneous
-Dirichlet data.
+Dirichlet data, and with best-approximated linear data and f = 0.
 
 Bar:
 * mass matrix: pattern and values bitwise equal to the oracle's;
@@ -10,7 +10,9 @@ Bar:
 * every state within 1e-9 (max norm, relative to the oracle state's) of the
   oracle's sparse-LU steps, the device CG stopping at relative residual 1e-13;
 * the system matrix is reassembled on the first step and on each change of
-  the step size only."""
+  the step size only;
+* inhomogeneous data: boundary values kept to 1e-13 at every step, and the
+  stationary limit equal to the harmonic (linear) data to 1e-9."""
 import numpy as np
 import pytest
 
@@ -122,6 +124,33 @@ def test_uploaded_mesh_until_stationarity(grmp, O):
     print(f"  {nsteps} steps, vs oracle steps {err:.2e}, vs stationary {errs:.2e}")
     assert err <= 1e-9
     assert errs <= 1e-8
+
+
+def test_dirichlet_data_kept_and_stationary_limit(grmp, O):
+    """Best-approximated linear Dirichlet data g = 0.5 + 2x - y with f = 0:
+    the boundary values hold at every step and the state tends to the
+    harmonic P1 function, which is g itself."""
+    N = 12
+    print(f"[case] heat with Dirichlet data N={N}", flush=True)
+    grid = grmp.grid_unitsquare(N)
+    fes = grmp.FESpace(grmp.H1P1(1), grid)
+    coef = [0.5, 2.0, -1.0]
+    pde = grmp.PoissonProblem(1.0)
+    pde.add_rhsdata(0, grmp.LinearForm(grmp.Identity, grmp.DataFunction([0.0])))
+    pde.add_boundarydata(0, grmp.BestapproxDirichletBoundary, data=grmp.PolynomialData(coef))
+    X = grmp.FEVector([fes])
+    X.set_entries(np.random.default_rng(3).standard_normal(fes.ndofs))
+    tcs = grmp.TimeControlSolver(pde, X, grmp.BackwardEuler, timedependent_equations=[0], rtol=1e-13)
+    coords, _ = grid.download()
+    g = O.poly_eval(coef, coords[:, 0], coords[:, 1])
+    bd = fes.boundary_dofs()
+    assert np.max(np.abs(X.entries()[bd] - g[bd])) < 1e-14
+    tcs.advance(0.01)
+    assert np.max(np.abs(X.entries()[bd] - g[bd])) < 1e-13
+    n = tcs.advance_until_stationarity(1.0, stationarity_threshold=1e-11, maxtimesteps=100)
+    err = np.max(np.abs(X.entries() - g)) / np.max(np.abs(g))
+    print(f"  {n} steps to stationarity, max rel err vs g {err:.2e}")
+    assert n < 100 and err <= 1e-9
 
 
 def test_argument_errors(grmp):
