@@ -336,7 +336,7 @@ def run_workload(grmp, which, args, log):
         log(f"[bench] heat: solver setup (mass matrix, rhs data), {fes.ndofs} dofs")
         t0 = time.perf_counter()
         tcs = grmp.TimeControlSolver(pde, X, grmp.BackwardEuler, timedependent_equations=[0],
-                                     precision=args.precision, rtol=args.rtol)
+                                     precision=args.precision, rtol=args.rtol, solver=args.ts_solver)
         ctx.sync()
         t_setup = time.perf_counter() - t0
         log("[bench] heat: first step (system assembly + solve)")
@@ -353,7 +353,8 @@ def run_workload(grmp, which, args, log):
         t_steps = time.perf_counter() - t0
         per = t_steps / args.timesteps
         n, nnz, _ = tcs.S.info()
-        out["config"].update(nnz=int(nnz), precision=args.precision, rtol=args.rtol)
+        out["config"].update(nnz=int(nnz), precision=args.precision if tcs.solver == "krylov" else "fp64",
+                             rtol=args.rtol, step_solver=tcs.solver)
         out["setup_s"] = round(t_setup, 3)
         out["first_step_s"] = round(t_first, 3)
         out["first_step_iterations"] = int(i1.iterations)
@@ -432,6 +433,8 @@ def main():
     ap.add_argument("--solve", action="store_true", help="c3: also solve")
     ap.add_argument("--dt", type=float, default=1e-4, help="heat: time step size")
     ap.add_argument("--timesteps", type=int, default=20, help="heat: timed steps after the first")
+    ap.add_argument("--ts-solver", default="auto", choices=["auto", "krylov", "mg"],
+                    help="heat: step solver (auto = multigrid V-cycle CG where it applies)")
     ap.add_argument("--tile-rows", type=int, default=0, help="engine rows per work tile (0 = library default)")
     ap.add_argument("--fp64-first-sweep", type=int, default=0,
                     help="served requests whose first FP32 sweep takes more iterations continue in FP64 (0: off)")

@@ -1679,3 +1679,91 @@ void mg_batch_destroy(MgBatch *b)
     if (b->ev) (void)hipEventDestroy(b->ev);
     delete b;
 }
+
+// ------------------------------------------------- one multigrid solve
+// The served path's V-cycle CG for one caller-built hierarchy: level 0 is
+// the caller's system (its forms are assembled by the job with the job's
+// own cell coefficient), coarser levels are rediscretizations on the unit
+// squares of ceil(N/2) down to N <= 4.  The solution is 0 on fixed rows
+// (homogeneous Dirichlet data).  Iterations are chunked as in the server:
+// status after each chunk, the true-residual check when the scalars ask.
+extern "C" int grmp_mg_solve(grmp_ctx *ctx, int nlev, const grmp_matrix *const *levels,
+                             const grmp_space *const *spaces, const grmp_vector *kappa0, const grmp_vector *b,
+                             grmp_vector *x, double rtol, int64_t maxit, grmp_solve_info *info)
+{
+    if (!ctx || !levels || !spaces || !kappa0 || !b || !x) return GRMP_ERR_ARG;
+    GRMP_CHECK(ctx, nlev >= 1 && nlev <= MG_MAX_LEVELS, GRMP_ERR_ARG, "grmp_mg_solve: %d levels", nlev);
+    for (int l = 0; l < nlev; ++l)
+        GRMP_CHECK(ctx, levels[l] && spaces[l] && levels[l]->ctx == ctx && levels[l]->nblocks == 1 &&
+                            levels[l]->spaces[0] == spaces[l],
+                   GRMP_ERR_ARG, "grmp_mg_solve: level %d is not a one-block matrix over its space", l);
+    const grmp_mesh *m0 = spaces[0]->mesh;
+    GRMP_CHECK(ctx, kappa0->ctx == ctx && kappa0->n == m0->ncells, GRMP_ERR_ARG,
+               "grmp_mg_solve: kappa0 needs one value per level-0 cell (%lld)", (long long)m0->ncells);
+    GRMP_CHECK(ctx, b->n == levels[0]->n && x->n == levels[0]->n && b->d != x->d, GRMP_ERR_ARG,
+               "grmp_mg_solve: b and x need the level-0 length and distinct storage");
+    GRMP_CHECK(ctx, rtol > 0.0 && rtol < 1.0 && maxit >= 1 && maxit <= (1 << 30), GRMP_ERR_ARG,
+               "grmp_mg_solve: rtol in (0, 1) and maxit >= 1 needed");
+    std::vector<MgLevelIn> in(nlev);
+    for (int l = 0; l < nlev; ++l)
+        in[l] = MgLevelIn{levels[l], spaces[l], spaces[l]->mesh->N, spaces[l]->fe, spaces[l]->ncomp};
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipStream_t st = ctx->stream;
+    const double t0 = wall_seconds();
+    MgJob *j = nullptr;
+    MgStatus *h = nullptr;
+    int rc = GRMP_OK;
+    do {
+        if ((rc = mg_job_create(ctx, st, in.data(), nlev, false, 2048, 2048, false, &j))) break;
+        if (hipHostMalloc((void **)&h, sizeof(MgStatus), hipHostMallocDefault) != hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "grmp_mg_solve: status buffer");
+            break;
+        }
+        const int64_t n = levels[0]->n;
+        if (hipMemcpyAsync(mg_job_kappa0(j), kappa0->d, sizeof(double) * m0->ncells, hipMemcpyDeviceToDevice, st) !=
+                hipSuccess ||
+            hipMemcpyAsync(mg_job_rhs(j), b->d, sizeof(double) * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "grmp_mg_solve: inputs");
+            break;
+        }
+        if ((rc = zero_fixed_launch(levels[0], mg_job_rhs(j), st))) break;
+        if ((rc = mg_job_setup(j, rtol, (int)maxit))) break;
+        if ((rc = mg_job_iterate(j, 4)) || (rc = mg_job_status_async(j, h)) || (rc = mg_job_wait(j))) break;
+        while (h->done != 1) {
+            if (h->done == 2) {
+                rc = mg_job_verify(j);
+                if (!rc && mg_job_block(j)) rc = mg_job_iterate(j, 1);
+            } else {
+                rc = mg_job_iterate(j, 4);
+            }
+            if (rc || (rc = mg_job_status_async(j, h)) || (rc = mg_job_wait(j))) break;
+        }
+        if (rc) break;
+        if (hipMemcpyAsync(x->d, mg_job_solution(j), sizeof(double) * n, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "grmp_mg_solve: solution");
+            break;
+        }
+    } while (0);
+    grmp_solve_info inf;
+    memset(&inf, 0, sizeof(inf));
+    if (h) {
+        inf.status = h->status;
+        inf.iterations = h->it;
+        inf.relres = h->relres;
+    }
+    inf.outer = 1;
+    inf.seconds = wall_seconds() - t0;
+    if (info) *info = inf;
+    if (j) {
+        (void)hipStreamSynchronize(st);
+        mg_job_destroy(j);
+        (void)hipStreamSynchronize(st);
+    }
+    if (h) (void)hipHostFree(h);
+    if (rc) return rc;
+    if (inf.status == GRMP_ERR_NOCONV)
+        return grmp_fail(ctx, GRMP_ERR_NOCONV, "grmp_mg_solve: no convergence, relres %.3e after %lld iterations",
+                         inf.relres, (long long)inf.iterations);
+    return inf.status;
+}

@@ -16,17 +16,26 @@ subiteration holding every unknown, time-independent data):
 * advance_until_stationarity! (:605-665): steps until the summed per-unknown
   change norms drop below the threshold.
 
+Each step's system is solved by the served path's multigrid V-cycle CG
+(FP64, grmp_mg_solve) when the problem is one scalar H1P1 unknown on a
+generated unit square with homogeneous Dirichlet data: the coarse levels
+rediscretize A + M/dt on squares of ceil(N/2) down to N <= 4.  Otherwise,
+or with solver="krylov", the device Krylov solver of solve_system runs.
+
 Dirichlet data (homogeneous, interpolated or best-approximated) is fixed in
 time.  CrankNicolson, nonlinear operators, dt_action / dt_lump and
 time-dependent data are not built and raise GrmpError(ERR_UNSUPPORTED).
 """
+import copy
+import ctypes
+
 import numpy as np
 
 from . import _abi
 from ._abi import lib
-from .fe import FEMatrix, FEVector
-from .pde import (ReactionOperator, SolverConfig, _saddle_point, assemble_linearform, boundary_values, fixed_dofs,
-                  realize_constraints, solve_system)
+from .fe import FEMatrix, FESpace, FEVector, H1P1, grid_unitsquare
+from .pde import (CellCoefficient, HomogeneousDirichletBoundary, ReactionOperator, SolverConfig, _saddle_point, assemble_linearform,
+                  boundary_values, fixed_dofs, realize_constraints, solve_system)
 
 
 class BackwardEuler:
@@ -43,7 +52,8 @@ class TimeControlSolver:
     ``InitialValues`` holds the initial state and, after each advance, the
     current solution."""
 
-    def __init__(self, pde, InitialValues, rule=BackwardEuler, timedependent_equations=(), config=None, **kwargs):
+    def __init__(self, pde, InitialValues, rule=BackwardEuler, timedependent_equations=(), config=None,
+                 solver="auto", **kwargs):
         if rule is not BackwardEuler:
             raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, f"time integration rule {getattr(rule, '__name__', rule)} "
                                  "is not built (BackwardEuler is)")
@@ -81,10 +91,84 @@ class TimeControlSolver:
             for op in ops:
                 assemble_linearform(self.f, j, op)
         self.b = FEVector(FES, name="system rhs")
+        # step solver: the multigrid V-cycle CG where the served hierarchy applies, else the Krylov solver
+        if solver not in ("auto", "krylov", "mg"):
+            raise _abi.GrmpError(_abi.ERR_ARG, f"solver {solver!r}: 'auto', 'krylov' or 'mg'")
+        mg_ok = self._mg_eligible()
+        if solver == "mg" and not mg_ok:
+            raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "multigrid steps need one scalar H1P1 unknown on a generated "
+                                 "unit square, homogeneous Dirichlet data and no global constraints")
+        self.solver = "mg" if (solver == "mg" or (solver == "auto" and mg_ok)) else "krylov"
+        self.levels = None
         self.S = None
         self.cstep, self.ctime, self.last_timestep = 0, 0.0, None
         self.assemblies = 0
         self.last_info = None
+
+    def _mg_eligible(self):
+        X, pde = self.X, self.PDE
+        if len(X.FES) != 1 or pde.GlobalConstraints:
+            return False
+        fes = X.FES[0]
+        if fes.FEType.code != _abi.FE_H1P1 or fes.FEType.ncomp != 1 or fes.xgrid.kind != "unitsquare":
+            return False
+        if any(kind is not HomogeneousDirichletBoundary for kind, _ in pde.BoundaryOperators.values()):
+            return False
+        if not pde.BoundaryOperators:
+            return False
+        ops = pde.LHS.get((0, 0), [])
+        return sum(op.action is not None for op in ops) <= 1 and all(k == (0, 0) for k in pde.LHS)
+
+    def _mg_hierarchy(self, timestep):
+        """Level 0 is S; coarser levels carry the same operators (a cell
+        coefficient as a placeholder: the solve restricts the fine one) plus
+        the mass term, with their boundary dofs fixed, on the unit squares of
+        ceil(N/2) down to N <= 4."""
+        fes0 = self.X.FES[0]
+        ctx = fes0.ctx
+        Ns = [fes0.xgrid.N]
+        while Ns[-1] > 4:
+            Ns.append((Ns[-1] + 1) // 2)
+        levels = [(self.S, fes0)]
+        for N in Ns[1:]:
+            grid = grid_unitsquare(N, ctx)
+            fes = FESpace(H1P1(1), grid)
+            A = FEMatrix([fes])
+            for op in self.PDE.LHS[(0, 0)]:
+                if op.action is not None:
+                    ph = _abi.DeviceVector(ctx, grid.ncells)
+                    ph.fill(1.0)
+                    op = copy.copy(op)
+                    op.action = CellCoefficient(ph, bonus_quadorder=op.action.bonus_quadorder)
+                    form, keep = op.form(fes, 0, 0, ctx)
+                    A.add_form(form, keep)
+                else:
+                    form, keep = op.form(fes, 0, 0, ctx)
+                    A.add_form(form, keep)
+            form, keep = ReactionOperator(1.0 / timestep).form(fes, 0, 0, ctx)
+            A.add_form(form, keep)
+            A.set_fixed(fes.boundary_dofs(), self.SC.fixed_penalty)
+            A.build_pattern()
+            levels.append((A, fes, grid))
+        act = [op.action for op in self.PDE.LHS[(0, 0)] if op.action is not None]
+        if act:
+            kappa0 = act[0].device(ctx, fes0.xgrid.ncells)
+        else:
+            kappa0 = _abi.DeviceVector(ctx, fes0.xgrid.ncells)
+            kappa0.fill(1.0)
+        return levels, kappa0
+
+    def _mg_solve(self):
+        levels, kappa0 = self.levels
+        n = len(levels)
+        mats = (_abi.VP * n)(*[lv[0].h for lv in levels])
+        sps = (_abi.VP * n)(*[lv[1].h for lv in levels])
+        info = _abi.SolveInfo()
+        maxit = int(min(self.SC.maxiterations, 500))  # V-cycle CG iterations (the served path's cap)
+        rc = lib().grmp_mg_solve(self.X.ctx.h, n, mats, sps, kappa0.h, self.b.vec.h, self.X.vec.h,
+                                 float(self.SC.rtol), maxit, ctypes.byref(info))
+        self.X.ctx.check(rc)
+        return info
 
     def _system(self, timestep):
         """S = A + (1/dt) M with the fixed dofs penalized (:337-351)."""
@@ -119,10 +203,15 @@ class TimeControlSolver:
         self.cstep += 1
         self.ctime += timestep
         if self.S is None or timestep != self.last_timestep:
-            self.S = None
+            self.S = self.levels = None
             self.S = self._system(timestep)
+            if self.solver == "mg":
+                self.levels = self._mg_hierarchy(timestep)
         self.rhs(timestep)
-        info = solve_system(self.S, self.b, self.X, self.SC, self.method)
+        if self.solver == "mg":
+            info = self._mg_solve()
+        else:
+            info = solve_system(self.S, self.b, self.X, self.SC, self.method)
         realize_constraints(self.PDE, self.X)
         self.last_timestep = timestep
         self.last_info = info

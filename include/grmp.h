@@ -279,6 +279,20 @@ void grmp_solve_opts_default(grmp_solve_opts *o);
 int grmp_solve(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, const grmp_solve_opts *opts,
                grmp_solve_info *info);
 
+/* Multigrid-preconditioned FP64 CG (V-cycle with Chebyshev smoothing, the
+ * served solver) on a caller-built hierarchy of generated unit squares:
+ * levels[0] is the system over spaces[0] (H1P1 or H1P1{2} on N x N, or H1P2
+ * followed by H1P1 on the same N), each next level a matrix with the same
+ * forms, fixed boundary dofs and pattern on ceil(N/2) x ceil(N/2), down to
+ * at most 64 dofs.  The job assembles every level itself: forms with
+ * GRMP_COEF_CELL take kappa0 on level 0 and its geometric means on the
+ * coarser cells.  Homogeneous Dirichlet data (x = 0 on fixed rows).  No
+ * counterpart in the reference, which factorizes (src/solvers.jl:55,
+ * 656-660); used for the systems of TimeControlSolver. */
+int grmp_mg_solve(grmp_ctx *ctx, int nlev, const grmp_matrix *const *levels, const grmp_space *const *spaces,
+                  const grmp_vector *kappa0, const grmp_vector *b, grmp_vector *x, double rtol, int64_t maxit,
+                  grmp_solve_info *info);
+
 /* Solver plug-in: solves LinearProblem(A.entries.cscmatrix, b.entries) as the
  * linsolver of solve! (src/solvers.jl:55, 89-101, 656-660).  colptr[n+1],
  * rowval[nnz] 1-based (Int64, as SparseMatrixCSC{Float64,Int64}); fixed[nfixed]

@@ -8,9 +8,12 @@ Bar:
 * step right-hand side b = f + M u / dt: within 1e-13 of max|b| on free rows,
   penalty rows b_d = 1e60 u_d bitwise;
 * every state within 1e-9 (max norm, relative to the oracle state's) of the
-  oracle's sparse-LU steps, the device CG stopping at relative residual 1e-13;
+  oracle's sparse-LU steps, the device solve stopping at relative residual
+  1e-13 (Krylov) or 1e-12 (multigrid);
 * the system matrix is reassembled on the first step and on each change of
   the step size only;
+* both step solvers: the Krylov solver, and the multigrid V-cycle CG on the
+  rediscretized unit-square hierarchy (odd N included);
 * inhomogeneous data: boundary values kept to 1e-13 at every step, and the
   stationary limit equal to the harmonic (linear) data to 1e-9."""
 import numpy as np
@@ -72,9 +75,10 @@ def test_step_rhs_and_mass(grmp, O):
     assert np.array_equal(b[fixed], bref[fixed])
 
 
-@pytest.mark.parametrize("N", [2, 24])
-def test_steps_match_oracle(grmp, O, N):
-    print(f"[case] backward euler steps N={N}", flush=True)
+@pytest.mark.parametrize("solver", ["krylov", "mg"])
+@pytest.mark.parametrize("N", [2, 24, 37])
+def test_steps_match_oracle(grmp, O, N, solver):
+    print(f"[case] backward euler steps N={N} {solver}", flush=True)
     grid = grmp.grid_unitsquare(N)
     kap = grmp.DeviceVector(grid.ctx, grid.ncells)
     kap.lognormal(7, 0.0, 1.0)
@@ -82,7 +86,9 @@ def test_steps_match_oracle(grmp, O, N):
     X = grmp.FEVector([fes])
     rng = np.random.default_rng(N)
     X.set_entries(rng.standard_normal(fes.ndofs))
-    tcs = grmp.TimeControlSolver(pde, X, grmp.BackwardEuler, timedependent_equations=[0], rtol=1e-13)
+    tcs = grmp.TimeControlSolver(pde, X, grmp.BackwardEuler, timedependent_equations=[0],
+                                 rtol=1e-13 if solver == "krylov" else 1e-12, solver=solver)
+    assert tcs.solver == solver
     u0 = X.entries()
     coords, cells = grid.download()
     A, M, f, fixed = _oracle(O, coords, cells, kap=kap.download())
@@ -141,6 +147,7 @@ def test_dirichlet_data_kept_and_stationary_limit(grmp, O):
     X = grmp.FEVector([fes])
     X.set_entries(np.random.default_rng(3).standard_normal(fes.ndofs))
     tcs = grmp.TimeControlSolver(pde, X, grmp.BackwardEuler, timedependent_equations=[0], rtol=1e-13)
+    assert tcs.solver == "krylov"  # inhomogeneous data
     coords, _ = grid.download()
     g = O.poly_eval(coef, coords[:, 0], coords[:, 1])
     bd = fes.boundary_dofs()
@@ -165,6 +172,11 @@ def test_argument_errors(grmp):
     tcs = grmp.TimeControlSolver(pde, X, grmp.BackwardEuler, timedependent_equations=[0])
     with pytest.raises(grmp.GrmpError):
         tcs.advance(0.0)
+    with pytest.raises(grmp.GrmpError):  # multigrid needs a generated unit square
+        up = grmp.ExtendableGrid.from_arrays(*grid.download())
+        fu, pu = _heat(grmp, up)
+        grmp.TimeControlSolver(pu, grmp.FEVector([fu]), grmp.BackwardEuler, timedependent_equations=[0],
+                               solver="mg")
     other = grmp.FEVector([fes])
     with pytest.raises(grmp.GrmpError):  # b aliasing u
         tcs.S = tcs._system(0.1)
