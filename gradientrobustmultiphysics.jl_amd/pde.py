@@ -10,12 +10,13 @@ sum of the polynomial orders of the FE types, minus one per derivative
 (QuadratureOrderShift4Operator, src/functionoperators.jl:294), plus the
 action's bonus order.
 """
+import copy
 import ctypes
 
 import numpy as np
 
 from . import _abi
-from .fe import FEMatrix, FEVector
+from .fe import FEMatrix, FESpace, FEVector, H1P1, grid_unitsquare
 from ._abi import lib
 
 Identity, Gradient, SymmetricGradient, Divergence = "Identity", "Gradient", "SymmetricGradient", "Divergence"
@@ -415,14 +416,95 @@ def assemble(pde, Solution, values=_abi.VALUES_FP64, penalty=1e60):
     return A, b, fixed
 
 
-def solve(Solution, pde, config=None, **kwargs):
+def mg_eligible(pde, Solution):
+    """True when grmp_mg_solve applies: one unknown, H1P1 (1 or 2
+    components) or scalar H1P2 on a generated unit square, homogeneous
+    Dirichlet data, no global constraints, at most one cell-coefficient
+    operator."""
+    if len(Solution.FES) != 1 or pde.GlobalConstraints or list(pde.LHS) != [(0, 0)]:
+        return False
+    fes = Solution.FES[0]
+    ft = fes.FEType
+    if fes.xgrid.kind != "unitsquare" or not ((ft.code == _abi.FE_H1P1 and ft.ncomp in (1, 2)) or
+                                              (ft.code == _abi.FE_H1P2 and ft.ncomp == 1)):
+        return False
+    if not pde.BoundaryOperators or any(kind is not HomogeneousDirichletBoundary
+                                        for kind, _ in pde.BoundaryOperators.values()):
+        return False
+    return sum(getattr(op, "action", None) is not None for op in pde.LHS[(0, 0)]) <= 1
+
+
+def mg_hierarchy(pde, fes0, A0, extra_ops=(), penalty=1e60):
+    """Levels for grmp_mg_solve: level 0 is the assembled system A0 over
+    fes0; then H1P1 on the same square when fes0 is H1P2, and H1P1 on the
+    squares of ceil(N/2) down to N <= 4, each with pde's operators (a cell
+    coefficient as a placeholder: the solve restricts the fine one) and
+    ``extra_ops``, its boundary dofs fixed.  Returns (levels, kappa0)."""
+    ctx = fes0.ctx
+    ncomp = fes0.FEType.ncomp
+    N = fes0.xgrid.N
+    Ns = [N] if fes0.FEType.code == _abi.FE_H1P2 else []
+    while N > 4:
+        N = (N + 1) // 2
+        Ns.append(N)
+    ops = list(pde.LHS[(0, 0)]) + list(extra_ops)
+    levels = [(A0, fes0)]
+    for Nl in Ns:
+        grid = fes0.xgrid if Nl == fes0.xgrid.N else grid_unitsquare(Nl, ctx)
+        fes = FESpace(H1P1(ncomp), grid)
+        A = FEMatrix([fes])
+        for op in ops:
+            if getattr(op, "action", None) is not None:
+                ph = _abi.DeviceVector(ctx, grid.ncells)
+                ph.fill(1.0)
+                op = copy.copy(op)
+                op.action = CellCoefficient(ph, bonus_quadorder=op.action.bonus_quadorder)
+            form, keep = op.form(fes, 0, 0, ctx)
+            A.add_form(form, keep)
+        A.set_fixed(fes.boundary_dofs(), penalty)
+        A.build_pattern()
+        levels.append((A, fes, grid))
+    act = [op.action for op in ops if getattr(op, "action", None) is not None]
+    if act:
+        kappa0 = act[0].device(ctx, fes0.xgrid.ncells)
+    else:
+        kappa0 = _abi.DeviceVector(ctx, fes0.xgrid.ncells)
+        kappa0.fill(1.0)
+    return levels, kappa0
+
+
+def mg_solve(levels, kappa0, b, x, rtol, maxit=500):
+    """grmp_mg_solve on a hierarchy from mg_hierarchy: x = S^{-1} b."""
+    n = len(levels)
+    mats = (_abi.VP * n)(*[lv[0].h for lv in levels])
+    sps = (_abi.VP * n)(*[lv[1].h for lv in levels])
+    info = _abi.SolveInfo()
+    rc = lib().grmp_mg_solve(x.ctx.h, n, mats, sps, kappa0.h, b.vec.h, x.vec.h, float(rtol), int(maxit),
+                             ctypes.byref(info))
+    x.ctx.check(rc)
+    return info
+
+
+def solve(Solution, pde, config=None, solver="krylov", **kwargs):
     """solve!(Solution, PDE) for linear stationary problems
     (src/solvers.jl:1354-1402 -> solve_direct! :622-686).  Returns the
-    SolveInfo of the device Krylov solve."""
+    SolveInfo of the device solve.  solver="krylov": the device Krylov
+    solvers (CG / MINRES, FP64 or FP32 inner); "mg": the multigrid V-cycle CG
+    (FP64) where mg_eligible holds; "auto": multigrid where it applies."""
     cfg = config or SolverConfig(**kwargs)
-    fp32 = cfg.precision == "fp32_ir"
+    if solver not in ("auto", "krylov", "mg"):
+        raise _abi.GrmpError(_abi.ERR_ARG, f"solver {solver!r}: 'auto', 'krylov' or 'mg'")
+    use_mg = solver != "krylov" and mg_eligible(pde, Solution)
+    if solver == "mg" and not use_mg:
+        raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "multigrid needs one H1P1 / H1P2 unknown on a generated unit "
+                             "square, homogeneous Dirichlet data and no global constraints")
+    fp32 = cfg.precision == "fp32_ir" and not use_mg
     values = _abi.VALUES_FP64 | (_abi.VALUES_FP32 if fp32 else 0)
     A, b, fixed = assemble(pde, Solution, values, cfg.fixed_penalty)
+    if use_mg:
+        levels, kappa0 = mg_hierarchy(pde, Solution.FES[0], A, penalty=cfg.fixed_penalty)
+        info = mg_solve(levels, kappa0, b, Solution, cfg.rtol, min(cfg.maxiterations, 500))
+        return info, A, b
     method = _abi.SOLVE_MINRES if _saddle_point(pde, Solution) else _abi.SOLVE_CG
     info = solve_system(A, b, Solution, cfg, method)
     realize_constraints(pde, Solution)

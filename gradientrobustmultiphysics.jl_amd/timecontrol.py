@@ -26,16 +26,13 @@ Dirichlet data (homogeneous, interpolated or best-approximated) is fixed in
 time.  CrankNicolson, nonlinear operators, dt_action / dt_lump and
 time-dependent data are not built and raise GrmpError(ERR_UNSUPPORTED).
 """
-import copy
-import ctypes
-
 import numpy as np
 
 from . import _abi
 from ._abi import lib
-from .fe import FEMatrix, FESpace, FEVector, H1P1, grid_unitsquare
-from .pde import (CellCoefficient, HomogeneousDirichletBoundary, ReactionOperator, SolverConfig, _saddle_point, assemble_linearform,
-                  boundary_values, fixed_dofs, realize_constraints, solve_system)
+from .fe import FEMatrix, FEVector
+from .pde import (ReactionOperator, SolverConfig, _saddle_point, assemble_linearform, boundary_values, fixed_dofs,
+                  mg_eligible, mg_hierarchy, mg_solve, realize_constraints, solve_system)
 
 
 class BackwardEuler:
@@ -106,69 +103,13 @@ class TimeControlSolver:
         self.last_info = None
 
     def _mg_eligible(self):
-        X, pde = self.X, self.PDE
-        if len(X.FES) != 1 or pde.GlobalConstraints:
-            return False
-        fes = X.FES[0]
-        if fes.FEType.code != _abi.FE_H1P1 or fes.FEType.ncomp != 1 or fes.xgrid.kind != "unitsquare":
-            return False
-        if any(kind is not HomogeneousDirichletBoundary for kind, _ in pde.BoundaryOperators.values()):
-            return False
-        if not pde.BoundaryOperators:
-            return False
-        ops = pde.LHS.get((0, 0), [])
-        return sum(op.action is not None for op in ops) <= 1 and all(k == (0, 0) for k in pde.LHS)
-
-    def _mg_hierarchy(self, timestep):
-        """Level 0 is S; coarser levels carry the same operators (a cell
-        coefficient as a placeholder: the solve restricts the fine one) plus
-        the mass term, with their boundary dofs fixed, on the unit squares of
-        ceil(N/2) down to N <= 4."""
-        fes0 = self.X.FES[0]
-        ctx = fes0.ctx
-        Ns = [fes0.xgrid.N]
-        while Ns[-1] > 4:
-            Ns.append((Ns[-1] + 1) // 2)
-        levels = [(self.S, fes0)]
-        for N in Ns[1:]:
-            grid = grid_unitsquare(N, ctx)
-            fes = FESpace(H1P1(1), grid)
-            A = FEMatrix([fes])
-            for op in self.PDE.LHS[(0, 0)]:
-                if op.action is not None:
-                    ph = _abi.DeviceVector(ctx, grid.ncells)
-                    ph.fill(1.0)
-                    op = copy.copy(op)
-                    op.action = CellCoefficient(ph, bonus_quadorder=op.action.bonus_quadorder)
-                    form, keep = op.form(fes, 0, 0, ctx)
-                    A.add_form(form, keep)
-                else:
-                    form, keep = op.form(fes, 0, 0, ctx)
-                    A.add_form(form, keep)
-            form, keep = ReactionOperator(1.0 / timestep).form(fes, 0, 0, ctx)
-            A.add_form(form, keep)
-            A.set_fixed(fes.boundary_dofs(), self.SC.fixed_penalty)
-            A.build_pattern()
-            levels.append((A, fes, grid))
-        act = [op.action for op in self.PDE.LHS[(0, 0)] if op.action is not None]
-        if act:
-            kappa0 = act[0].device(ctx, fes0.xgrid.ncells)
-        else:
-            kappa0 = _abi.DeviceVector(ctx, fes0.xgrid.ncells)
-            kappa0.fill(1.0)
-        return levels, kappa0
+        """mg_eligible, with the mass matrix's restriction to scalar H1P1."""
+        fes = self.X.FES[0]
+        return mg_eligible(self.PDE, self.X) and fes.FEType.code == _abi.FE_H1P1 and fes.FEType.ncomp == 1
 
     def _mg_solve(self):
         levels, kappa0 = self.levels
-        n = len(levels)
-        mats = (_abi.VP * n)(*[lv[0].h for lv in levels])
-        sps = (_abi.VP * n)(*[lv[1].h for lv in levels])
-        info = _abi.SolveInfo()
-        maxit = int(min(self.SC.maxiterations, 500))  # V-cycle CG iterations (the served path's cap)
-        rc = lib().grmp_mg_solve(self.X.ctx.h, n, mats, sps, kappa0.h, self.b.vec.h, self.X.vec.h,
-                                 float(self.SC.rtol), maxit, ctypes.byref(info))
-        self.X.ctx.check(rc)
-        return info
+        return mg_solve(levels, kappa0, self.b, self.X, self.SC.rtol, min(self.SC.maxiterations, 500))
 
     def _system(self, timestep):
         """S = A + (1/dt) M with the fixed dofs penalized (:337-351)."""
@@ -206,7 +147,8 @@ class TimeControlSolver:
             self.S = self.levels = None
             self.S = self._system(timestep)
             if self.solver == "mg":
-                self.levels = self._mg_hierarchy(timestep)
+                self.levels = mg_hierarchy(self.PDE, self.X.FES[0], self.S, [ReactionOperator(1.0 / timestep)],
+                                           self.SC.fixed_penalty)
         self.rhs(timestep)
         if self.solver == "mg":
             info = self._mg_solve()

@@ -66,6 +66,61 @@ def test_elasticity_solve(grmp, O):
     assert info.status == 0 and err <= 1e-8, err
 
 
+@pytest.mark.parametrize("N,fe", [(24, 1), (37, 1), (24, 2), (3, 2)])
+def test_poisson_solve_multigrid(grmp, O, N, fe):
+    """solve(..., solver="mg"): the multigrid V-cycle CG on the rediscretized
+    unit-square hierarchy, against sparse LU."""
+    print(f"[case] multigrid solve N={N} fe={fe}", flush=True)
+    grid, fes, pde, xref = _poisson(grmp, O, N, fe, 23)
+    sol = grmp.FEVector([fes])
+    info, A, b = grmp.solve(sol, pde, solver="mg", rtol=1e-12)
+    err = np.linalg.norm(sol.entries() - xref) / np.linalg.norm(xref)
+    print(f"  {info.iterations} iterations, rel err vs LU {err:.2e}")
+    assert info.status == 0 and info.iterations <= 60
+    assert err <= 1e-8, err
+
+
+def test_elasticity_solve_multigrid(grmp, O):
+    N = 20
+    print(f"[case] multigrid elasticity N={N}", flush=True)
+    grid = grmp.grid_unitsquare(N)
+    fes = grmp.FESpace(grmp.H1P1(2), grid)
+    kap = grmp.DeviceVector(grid.ctx, grid.ncells)
+    kap.lognormal(5, 0.0, 1.0)
+    pde = grmp.LinearElasticityProblem(1.0, 0.3, grmp.CellCoefficient(kap))
+    pde.add_rhsdata(0, grmp.LinearForm(grmp.Identity, grmp.DataFunction([0.0, -1.0])))
+    pde.add_boundarydata(0, grmp.HomogeneousDirichletBoundary)
+    sol = grmp.FEVector([fes])
+    info, A, b = grmp.solve(sol, pde, solver="mg", rtol=1e-12)
+    S = O.Space(N, O.FE_P1, ncomp=2)
+    mu, lam = 1.0 / 2.6, 0.3 / (1.3 * 0.4)
+    ref = O.assemble_matrix(S.ndofs, [dict(kind=O.BLF_HOOKE2D, fe=O.FE_P1, ncomp=2, quadorder=0, factor=1.0,
+                                          coef=O.COEF_CELL, kappa=kap.download(), params=np.array([lam, mu]),
+                                          coords=S.coords, cells=S.cells, celldofs=S.celldofs)], fixed=S.fixed)
+    bb = np.zeros(S.ndofs)
+    O.assemble_lf_const(bb, O.FE_P1, 2, 1, [0.0, -1.0], S.coords, S.cells, S.celldofs)
+    xref = O.direct_solve(ref, O.penalized_rhs(bb, S.fixed))
+    err = np.linalg.norm(sol.entries() - xref) / np.linalg.norm(xref)
+    print(f"  {info.iterations} iterations, rel err vs LU {err:.2e}")
+    assert info.status == 0 and err <= 1e-8, err
+
+
+def test_multigrid_not_applicable(grmp, O):
+    """solver="mg" on an uploaded mesh is refused; "auto" falls back."""
+    print("[case] multigrid not applicable", flush=True)
+    grid, fes, pde, xref = _poisson(grmp, O, 6, 1, 3)
+    up = grmp.ExtendableGrid.from_arrays(*grid.download())
+    fu = grmp.FESpace(grmp.H1P1(1), up)
+    pu = grmp.PoissonProblem(1.0)
+    pu.add_rhsdata(0, grmp.LinearForm(grmp.Identity, grmp.DataFunction([1.0])))
+    pu.add_boundarydata(0, grmp.HomogeneousDirichletBoundary)
+    assert not grmp.mg_eligible(pu, grmp.FEVector([fu]))
+    with pytest.raises(grmp.GrmpError):
+        grmp.solve(grmp.FEVector([fu]), pu, solver="mg")
+    info, A, b = grmp.solve(grmp.FEVector([fu]), pu, solver="auto", rtol=1e-12)
+    assert info.status == 0
+
+
 def test_nonconvergence_reported(grmp, O):
     """An unreachable tolerance with few iterations returns GRMP_ERR_NOCONV
     (the reference warns, src/solvers.jl:1398-1400)."""
