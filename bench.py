@@ -214,6 +214,28 @@ def _timestep_rhs_rate(grmp, tcs, dt, reps=20):
     return dict(us=round(t * 1e6, 1), bytes=int(nbytes), gbs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4))
 
 
+def _heat_cpu_baseline(dt, N=512, steps=5):
+    """The oracle's BackwardEuler steps (the reference's scheme: S = A + M/dt
+    factorized once while dt is fixed, then one triangular solve per step;
+    SuperLU in place of UMFPACK) on the heat problem at N x N, timed on one
+    host thread after the assembly."""
+    O = GE.load_oracle()
+    S = O.Space(N, O.FE_P1)
+    kap = np.exp(np.random.default_rng(2024).standard_normal(S.cells.shape[0]))
+    base = dict(fe=O.FE_P1, coords=S.coords, cells=S.cells, celldofs=S.celldofs)
+    A = O.assemble_matrix(S.ndofs, [dict(base, kind=O.BLF_LAPLACE, quadorder=0, factor=1.0, coef=O.COEF_CELL,
+                                         kappa=kap)])
+    M = O.assemble_matrix(S.ndofs, [dict(base, kind=O.BLF_MASS, quadorder=2, factor=1.0)])
+    f = np.zeros(S.ndofs)
+    O.assemble_lf_const(f, O.FE_P1, 1, 1, [1.0], S.coords, S.cells, S.celldofs)
+    t0 = time.perf_counter()
+    O.backward_euler(A, M, f, np.zeros(S.ndofs), [dt] * steps, S.fixed)
+    t = time.perf_counter() - t0
+    return dict(value=round(S.ndofs * steps / t, 1), unit="DoF-steps/s", cores=1, kind="port",
+                sample=f"{N}x{N} P1 heat ({S.ndofs} dofs), {steps} steps including one sparse-LU factorization, "
+                       f"{t:.1f} s")
+
+
 def run_workload(grmp, which, args, log):
     ctx = grmp.default_context()
     if args.rtol == 1e-9 and "--rtol" not in sys.argv:
@@ -379,6 +401,9 @@ def run_workload(grmp, which, args, log):
                         "system_assemblies": tcs.assemblies}
         out["timestep_rhs"] = _timestep_rhs_rate(grmp, tcs, dt)
         out["spmv_fp64_system"] = _spmv_rate(grmp, tcs.S, V64)
+        if not args.no_cpu:
+            log("[bench] heat: cpu baseline (oracle: sparse LU of S once, then one solve per step)")
+            out["cpu_baseline"] = _heat_cpu_baseline(dt)
     elif which == "dirichlet":
         N = args.size or 2048
         grid = grmp.grid_unitsquare(N)
