@@ -447,6 +447,18 @@ def run_workload(grmp, which, args, log):
         te = time.perf_counter() - t0
         out["nodevalues"] = {"ms": round(te * 1e3, 3), "nodes": int(nv.shape[1]),
                              "includes": "node-cell adjacency, cell-order averages, copy to the host"}
+        if not args.no_cpu:
+            log("[bench] dirichlet: cpu baseline (oracle best approximation at 256x256)")
+            O = GE.load_oracle()
+            S2 = O.Space(256, O.FE_P2)
+            t0 = time.perf_counter()
+            _, _, bdo = O.boundary_bestapprox(O.FE_P2, coef, 2 + 3, S2.coords, S2.cells, S2.celldofs, S2.cellfaces,
+                                              S2.facecells, S2.ndofs)
+            tc = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": round(bdo.size / tc, 1), "unit": "boundary dofs/s (best approximation)",
+                                   "cores": 1, "kind": "port",
+                                   "sample": f"256x256 P2 ({bdo.size} boundary dofs): boundary walk, face matrices, "
+                                             f"sparse LU; {tc:.2f} s"}
     else:
         raise SystemExit(f"unknown workload {which}")
     print(json.dumps(out), flush=True)
