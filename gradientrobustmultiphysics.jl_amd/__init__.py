@@ -17,8 +17,8 @@ from .pde import (APT_BilinearForm, APT_SymmetricBilinearForm, BestapproxDirichl
                   IncompressibleNavierStokesProblem, InterpolateDirichletBoundary, PolynomialData, boundary_values,
                   LagrangeMultiplier, LaplaceOperator, LinearElasticityProblem, LinearForm, PDEDescription,
                   PoissonProblem, ReactionOperator, ReconstructionIdentity, SolverConfig, SymmetricGradient, assemble,
-                  assemble_linearform, mg_eligible, mg_hierarchy, mg_solve, realize_constraints, solve,
-                  solve_system)
+                  assemble_linearform, MultigridSolver, mg_eligible, mg_hierarchy, mg_solve, realize_constraints,
+                  solve, solve_system)
 
 from .dataexport import nodevalues, writeCSV  # noqa: F401
 from .timecontrol import BackwardEuler, CrankNicolson, TimeControlSolver  # noqa: F401

@@ -145,6 +145,10 @@ SIGNATURES = {
     "grmp_nodevalues": (ctypes.c_int, [VP, VP, I64, DP]),
     "grmp_mg_solve": (ctypes.c_int, [VP, ctypes.c_int, ctypes.POINTER(VP), ctypes.POINTER(VP), VP, VP, VP,
                                      ctypes.c_double, I64, ctypes.POINTER(SolveInfo)]),
+    "grmp_mg_create": (ctypes.c_int, [VP, ctypes.c_int, ctypes.POINTER(VP), ctypes.POINTER(VP), VP,
+                                      ctypes.POINTER(VP)]),
+    "grmp_mg_apply": (ctypes.c_int, [VP, VP, VP, ctypes.c_double, I64, ctypes.POINTER(SolveInfo)]),
+    "grmp_mg_destroy": (ctypes.c_int, [VP]),
     "grmp_assemble_linearform": (ctypes.c_int, [VP, VP, ctypes.POINTER(LForm), VP, I64]),
     "grmp_integrate_identity": (ctypes.c_int, [VP, VP, ctypes.c_int, VP, I64, DP]),
     "grmp_solve_opts_default": (None, [ctypes.POINTER(SolveOpts)]),

@@ -1271,15 +1271,11 @@ static void mg_vcycle(MgJob *j)
     }
 }
 
-int mg_job_setup(MgJob *j, double rtol, int maxit)
+// level values, smoother bounds and the coarsest inverse: once per operator
+static int mg_job_assemble(MgJob *j)
 {
     grmp_ctx *ctx = j->ctx;
     hipStream_t st = j->st;
-    MgScal s0;
-    memset(&s0, 0, sizeof(s0));
-    s0.rtol = rtol;
-    s0.maxit = maxit;
-    GRMP_HIP(ctx, hipMemcpyAsync(j->sc, &s0, sizeof(s0), hipMemcpyHostToDevice, st));
     for (int l = 0; l < j->nlev; ++l) {
         MgLevel &L = j->lv[l];
         if (l > 0 && L.in.N != j->lv[l - 1].in.N)
@@ -1294,7 +1290,20 @@ int mg_job_setup(MgJob *j, double rtol, int maxit)
     hipLaunchKernelGGL(k_mg_dense_inv, dim3(1), dim3(256), 0, st, C.in.A->rowptr, C.in.A->cols, C.val, C.in.A->fixed,
                        (int)C.n, j->inv);
     GRMP_HIP(ctx, hipMemcpyAsync(j->ddesc, &j->hdesc, sizeof(MgBDesc), hipMemcpyHostToDevice, st));
-    // x = 0, r = rhs (0 on fixed rows), z = V(r), p = 0 then p = z
+    return GRMP_OK;
+}
+
+// Krylov state for the right-hand side in mg_job_rhs: scalars reset,
+// x = 0, r = rhs (0 on fixed rows), z = V(r), p = 0 then p = z
+static int mg_job_start(MgJob *j, double rtol, int maxit)
+{
+    grmp_ctx *ctx = j->ctx;
+    hipStream_t st = j->st;
+    MgScal s0;
+    memset(&s0, 0, sizeof(s0));
+    s0.rtol = rtol;
+    s0.maxit = maxit;
+    GRMP_HIP(ctx, hipMemcpyAsync(j->sc, &s0, sizeof(s0), hipMemcpyHostToDevice, st));
     const int64_t n = j->lv[0].n;
     GRMP_HIP(ctx, hipMemsetAsync(j->x, 0, n * 8, st));
     GRMP_HIP(ctx, hipMemsetAsync(j->p, 0, n * 8, st));
@@ -1306,6 +1315,12 @@ int mg_job_setup(MgJob *j, double rtol, int maxit)
     hipLaunchKernelGGL(k_mg_update_p, dim3(nblk(n, 256)), dim3(256), 0, st, j->p, j->z, n, j->sc);
     GRMP_HIP(ctx, hipGetLastError());
     return GRMP_OK;
+}
+
+int mg_job_setup(MgJob *j, double rtol, int maxit)
+{
+    GRMP_TRY(mg_job_assemble(j));
+    return mg_job_start(j, rtol, maxit);
 }
 
 static int mg_iterate_grid(MgJob *j, int iters);
@@ -1685,49 +1700,96 @@ void mg_batch_destroy(MgBatch *b)
 // the caller's system (its forms are assembled by the job with the job's
 // own cell coefficient), coarser levels are rediscretizations on the unit
 // squares of ceil(N/2) down to N <= 4.  The solution is 0 on fixed rows
-// (homogeneous Dirichlet data).  Iterations are chunked as in the server:
-// status after each chunk, the true-residual check when the scalars ask.
-extern "C" int grmp_mg_solve(grmp_ctx *ctx, int nlev, const grmp_matrix *const *levels,
-                             const grmp_space *const *spaces, const grmp_vector *kappa0, const grmp_vector *b,
-                             grmp_vector *x, double rtol, int64_t maxit, grmp_solve_info *info)
+// (homogeneous Dirichlet data).  A handle keeps the assembled hierarchy for
+// further right-hand sides, as the reference keeps its factorization while
+// the matrix is unchanged.  Iterations are chunked as in the server: status
+// after each chunk, the true-residual check when the scalars ask.
+struct grmp_mg {
+    grmp_ctx *ctx;
+    MgJob *job;
+    MgStatus *h;  // pinned
+    const grmp_matrix *A0;
+    int64_t n;
+};
+
+extern "C" int grmp_mg_destroy(grmp_mg *m)
 {
-    if (!ctx || !levels || !spaces || !kappa0 || !b || !x) return GRMP_ERR_ARG;
-    GRMP_CHECK(ctx, nlev >= 1 && nlev <= MG_MAX_LEVELS, GRMP_ERR_ARG, "grmp_mg_solve: %d levels", nlev);
+    if (!m) return GRMP_OK;
+    if (m->job) {
+        (void)hipStreamSynchronize(m->ctx->stream);
+        mg_job_destroy(m->job);
+        (void)hipStreamSynchronize(m->ctx->stream);
+    }
+    if (m->h) (void)hipHostFree(m->h);
+    delete m;
+    return GRMP_OK;
+}
+
+extern "C" int grmp_mg_create(grmp_ctx *ctx, int nlev, const grmp_matrix *const *levels,
+                              const grmp_space *const *spaces, const grmp_vector *kappa0, grmp_mg **out)
+{
+    if (!ctx || !levels || !spaces || !kappa0 || !out) return GRMP_ERR_ARG;
+    *out = nullptr;
+    GRMP_CHECK(ctx, nlev >= 1 && nlev <= MG_MAX_LEVELS, GRMP_ERR_ARG, "grmp_mg_create: %d levels", nlev);
     for (int l = 0; l < nlev; ++l)
         GRMP_CHECK(ctx, levels[l] && spaces[l] && levels[l]->ctx == ctx && levels[l]->nblocks == 1 &&
                             levels[l]->spaces[0] == spaces[l],
-                   GRMP_ERR_ARG, "grmp_mg_solve: level %d is not a one-block matrix over its space", l);
+                   GRMP_ERR_ARG, "grmp_mg_create: level %d is not a one-block matrix over its space", l);
     const grmp_mesh *m0 = spaces[0]->mesh;
     GRMP_CHECK(ctx, kappa0->ctx == ctx && kappa0->n == m0->ncells, GRMP_ERR_ARG,
-               "grmp_mg_solve: kappa0 needs one value per level-0 cell (%lld)", (long long)m0->ncells);
-    GRMP_CHECK(ctx, b->n == levels[0]->n && x->n == levels[0]->n && b->d != x->d, GRMP_ERR_ARG,
-               "grmp_mg_solve: b and x need the level-0 length and distinct storage");
-    GRMP_CHECK(ctx, rtol > 0.0 && rtol < 1.0 && maxit >= 1 && maxit <= (1 << 30), GRMP_ERR_ARG,
-               "grmp_mg_solve: rtol in (0, 1) and maxit >= 1 needed");
+               "grmp_mg_create: kappa0 needs one value per level-0 cell (%lld)", (long long)m0->ncells);
     std::vector<MgLevelIn> in(nlev);
     for (int l = 0; l < nlev; ++l)
         in[l] = MgLevelIn{levels[l], spaces[l], spaces[l]->mesh->N, spaces[l]->fe, spaces[l]->ncomp};
     std::lock_guard<std::mutex> g(ctx->mu);
     hipStream_t st = ctx->stream;
-    const double t0 = wall_seconds();
-    MgJob *j = nullptr;
-    MgStatus *h = nullptr;
+    grmp_mg *m = new grmp_mg{ctx, nullptr, nullptr, levels[0], levels[0]->n};
     int rc = GRMP_OK;
     do {
-        if ((rc = mg_job_create(ctx, st, in.data(), nlev, false, 2048, 2048, false, &j))) break;
-        if (hipHostMalloc((void **)&h, sizeof(MgStatus), hipHostMallocDefault) != hipSuccess) {
-            rc = grmp_fail(ctx, GRMP_ERR_HIP, "grmp_mg_solve: status buffer");
+        if ((rc = mg_job_create(ctx, st, in.data(), nlev, false, 2048, 2048, false, &m->job))) break;
+        if (hipHostMalloc((void **)&m->h, sizeof(MgStatus), hipHostMallocDefault) != hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "grmp_mg_create: status buffer");
             break;
         }
-        const int64_t n = levels[0]->n;
-        if (hipMemcpyAsync(mg_job_kappa0(j), kappa0->d, sizeof(double) * m0->ncells, hipMemcpyDeviceToDevice, st) !=
-                hipSuccess ||
-            hipMemcpyAsync(mg_job_rhs(j), b->d, sizeof(double) * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
-            rc = grmp_fail(ctx, GRMP_ERR_HIP, "grmp_mg_solve: inputs");
+        if (hipMemcpyAsync(mg_job_kappa0(m->job), kappa0->d, sizeof(double) * m0->ncells, hipMemcpyDeviceToDevice,
+                           st) != hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "grmp_mg_create: coefficient");
             break;
         }
-        if ((rc = zero_fixed_launch(levels[0], mg_job_rhs(j), st))) break;
-        if ((rc = mg_job_setup(j, rtol, (int)maxit))) break;
+        if ((rc = mg_job_assemble(m->job))) break;
+        if (hipStreamSynchronize(st) != hipSuccess) rc = grmp_fail(ctx, GRMP_ERR_HIP, "grmp_mg_create: assembly");
+    } while (0);
+    if (rc) {
+        grmp_mg_destroy(m);
+        return rc;
+    }
+    *out = m;
+    return GRMP_OK;
+}
+
+extern "C" int grmp_mg_apply(grmp_mg *m, const grmp_vector *b, grmp_vector *x, double rtol, int64_t maxit,
+                             grmp_solve_info *info)
+{
+    if (!m || !b || !x) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = m->ctx;
+    GRMP_CHECK(ctx, b->ctx == ctx && x->ctx == ctx && b->n == m->n && x->n == m->n && b->d != x->d, GRMP_ERR_ARG,
+               "grmp_mg_apply: b and x need the level-0 length %lld and distinct storage", (long long)m->n);
+    GRMP_CHECK(ctx, rtol > 0.0 && rtol < 1.0 && maxit >= 1 && maxit <= (1 << 30), GRMP_ERR_ARG,
+               "grmp_mg_apply: rtol in (0, 1) and maxit >= 1 needed");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipStream_t st = ctx->stream;
+    const double t0 = wall_seconds();
+    MgJob *j = m->job;
+    MgStatus *h = m->h;
+    memset(h, 0, sizeof(MgStatus));
+    int rc = GRMP_OK;
+    do {
+        if (hipMemcpyAsync(mg_job_rhs(j), b->d, sizeof(double) * m->n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "grmp_mg_apply: right-hand side");
+            break;
+        }
+        if ((rc = zero_fixed_launch(m->A0, mg_job_rhs(j), st))) break;
+        if ((rc = mg_job_start(j, rtol, (int)maxit))) break;
         if ((rc = mg_job_iterate(j, 4)) || (rc = mg_job_status_async(j, h)) || (rc = mg_job_wait(j))) break;
         while (h->done != 1) {
             if (h->done == 2) {
@@ -1739,31 +1801,36 @@ extern "C" int grmp_mg_solve(grmp_ctx *ctx, int nlev, const grmp_matrix *const *
             if (rc || (rc = mg_job_status_async(j, h)) || (rc = mg_job_wait(j))) break;
         }
         if (rc) break;
-        if (hipMemcpyAsync(x->d, mg_job_solution(j), sizeof(double) * n, hipMemcpyDeviceToDevice, st) != hipSuccess ||
-            hipStreamSynchronize(st) != hipSuccess) {
-            rc = grmp_fail(ctx, GRMP_ERR_HIP, "grmp_mg_solve: solution");
-            break;
-        }
+        if (hipMemcpyAsync(x->d, mg_job_solution(j), sizeof(double) * m->n, hipMemcpyDeviceToDevice, st) !=
+                hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "grmp_mg_apply: solution");
     } while (0);
     grmp_solve_info inf;
     memset(&inf, 0, sizeof(inf));
-    if (h) {
-        inf.status = h->status;
-        inf.iterations = h->it;
-        inf.relres = h->relres;
-    }
+    inf.status = h->status;
+    inf.iterations = h->it;
+    inf.relres = h->relres;
     inf.outer = 1;
     inf.seconds = wall_seconds() - t0;
     if (info) *info = inf;
-    if (j) {
+    if (rc) {
         (void)hipStreamSynchronize(st);
-        mg_job_destroy(j);
-        (void)hipStreamSynchronize(st);
+        return rc;
     }
-    if (h) (void)hipHostFree(h);
-    if (rc) return rc;
     if (inf.status == GRMP_ERR_NOCONV)
-        return grmp_fail(ctx, GRMP_ERR_NOCONV, "grmp_mg_solve: no convergence, relres %.3e after %lld iterations",
+        return grmp_fail(ctx, GRMP_ERR_NOCONV, "grmp_mg_apply: no convergence, relres %.3e after %lld iterations",
                          inf.relres, (long long)inf.iterations);
     return inf.status;
+}
+
+extern "C" int grmp_mg_solve(grmp_ctx *ctx, int nlev, const grmp_matrix *const *levels,
+                             const grmp_space *const *spaces, const grmp_vector *kappa0, const grmp_vector *b,
+                             grmp_vector *x, double rtol, int64_t maxit, grmp_solve_info *info)
+{
+    grmp_mg *m = nullptr;
+    GRMP_TRY(grmp_mg_create(ctx, nlev, levels, spaces, kappa0, &m));
+    const int rc = grmp_mg_apply(m, b, x, rtol, maxit, info);
+    grmp_mg_destroy(m);
+    return rc;
 }

@@ -473,6 +473,35 @@ def mg_hierarchy(pde, fes0, A0, extra_ops=(), penalty=1e60):
     return levels, kappa0
 
 
+class MultigridSolver:
+    """A hierarchy from mg_hierarchy assembled once (grmp_mg_create), then
+    solved for any number of right-hand sides (grmp_mg_apply), as the
+    reference reuses its factorization while the matrix is unchanged."""
+
+    def __init__(self, levels, kappa0):
+        self.levels, self.kappa0 = levels, kappa0  # kept alive: the handle reads them
+        self.ctx = levels[0][1].ctx
+        n = len(levels)
+        mats = (_abi.VP * n)(*[lv[0].h for lv in levels])
+        sps = (_abi.VP * n)(*[lv[1].h for lv in levels])
+        h = _abi.VP()
+        self.ctx.check(lib().grmp_mg_create(self.ctx.h, n, mats, sps, kappa0.h, ctypes.byref(h)))
+        self.h = h
+
+    def solve(self, b, x, rtol, maxit=500):
+        info = _abi.SolveInfo()
+        self.ctx.check(lib().grmp_mg_apply(self.h, b.vec.h, x.vec.h, float(rtol), int(maxit), ctypes.byref(info)))
+        return info
+
+    def __del__(self):
+        try:
+            if self.h and _abi.alive():
+                lib().grmp_mg_destroy(self.h)
+            self.h = None
+        except Exception:
+            pass
+
+
 def mg_solve(levels, kappa0, b, x, rtol, maxit=500):
     """grmp_mg_solve on a hierarchy from mg_hierarchy: x = S^{-1} b."""
     n = len(levels)

@@ -32,7 +32,7 @@ from . import _abi
 from ._abi import lib
 from .fe import FEMatrix, FEVector
 from .pde import (ReactionOperator, SolverConfig, _saddle_point, assemble_linearform, boundary_values, fixed_dofs,
-                  mg_eligible, mg_hierarchy, mg_solve, realize_constraints, solve_system)
+                  MultigridSolver, mg_eligible, mg_hierarchy, realize_constraints, solve_system)
 
 
 class BackwardEuler:
@@ -108,8 +108,7 @@ class TimeControlSolver:
         return mg_eligible(self.PDE, self.X) and fes.FEType.code == _abi.FE_H1P1 and fes.FEType.ncomp == 1
 
     def _mg_solve(self):
-        levels, kappa0 = self.levels
-        return mg_solve(levels, kappa0, self.b, self.X, self.SC.rtol, min(self.SC.maxiterations, 500))
+        return self.levels.solve(self.b, self.X, self.SC.rtol, min(self.SC.maxiterations, 500))
 
     def _system(self, timestep):
         """S = A + (1/dt) M with the fixed dofs penalized (:337-351)."""
@@ -146,9 +145,10 @@ class TimeControlSolver:
         if self.S is None or timestep != self.last_timestep:
             self.S = self.levels = None
             self.S = self._system(timestep)
-            if self.solver == "mg":
-                self.levels = mg_hierarchy(self.PDE, self.X.FES[0], self.S, [ReactionOperator(1.0 / timestep)],
-                                           self.SC.fixed_penalty)
+            if self.solver == "mg":  # assembled once per step size, reused by every step
+                self.levels = MultigridSolver(*mg_hierarchy(self.PDE, self.X.FES[0], self.S,
+                                                            [ReactionOperator(1.0 / timestep)],
+                                                            self.SC.fixed_penalty))
         self.rhs(timestep)
         if self.solver == "mg":
             info = self._mg_solve()

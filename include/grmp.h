@@ -80,6 +80,7 @@ typedef struct grmp_space grmp_space;
 typedef struct grmp_vector grmp_vector;
 typedef struct grmp_matrix grmp_matrix;
 typedef struct grmp_server grmp_server;
+typedef struct grmp_mg grmp_mg;
 
 typedef struct grmp_form {
     int kind;                 /* GRMP_BLF_* */
@@ -292,6 +293,15 @@ int grmp_solve(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, const grmp_
 int grmp_mg_solve(grmp_ctx *ctx, int nlev, const grmp_matrix *const *levels, const grmp_space *const *spaces,
                   const grmp_vector *kappa0, const grmp_vector *b, grmp_vector *x, double rtol, int64_t maxit,
                   grmp_solve_info *info);
+/* The same in parts, for many right-hand sides with one operator (the reuse
+ * of the factorization in advance!, src/solvers_timedependent.jl:337,
+ * :463-466): create assembles the hierarchy (levels, spaces and kappa0 must
+ * outlive the handle), apply solves for one b, destroy frees the handle. */
+int grmp_mg_create(grmp_ctx *ctx, int nlev, const grmp_matrix *const *levels, const grmp_space *const *spaces,
+                   const grmp_vector *kappa0, grmp_mg **out);
+int grmp_mg_apply(grmp_mg *m, const grmp_vector *b, grmp_vector *x, double rtol, int64_t maxit,
+                  grmp_solve_info *info);
+int grmp_mg_destroy(grmp_mg *m);
 
 /* Solver plug-in: solves LinearProblem(A.entries.cscmatrix, b.entries) as the
  * linsolver of solve! (src/solvers.jl:55, 89-101, 656-660).  colptr[n+1],

@@ -105,6 +105,33 @@ def test_elasticity_solve_multigrid(grmp, O):
     assert info.status == 0 and err <= 1e-8, err
 
 
+def test_multigrid_handle_reuse(grmp, O):
+    """One assembled hierarchy, several right-hand sides: each solve equals,
+    bit for bit, a fresh one-shot solve of the same right-hand side."""
+    print("[case] multigrid handle reuse", flush=True)
+    grid, fes, pde, xref = _poisson(grmp, O, 30, 1, 11)
+    sol = grmp.FEVector([fes])
+    A, b, fixed = grmp.assemble(pde, sol)
+    levels, kappa0 = grmp.mg_hierarchy(pde, fes, A)
+    M = grmp.MultigridSolver(levels, kappa0)
+    b2 = grmp.FEVector([fes])
+    v = np.random.default_rng(1).standard_normal(fes.ndofs)
+    v[fixed] = 0.0
+    b2.set_entries(v)
+    outs = []
+    for rhs in (b, b2, b):
+        x = grmp.FEVector([fes])
+        info = M.solve(rhs, x, 1e-12)
+        assert info.status == 0
+        outs.append(x.entries())
+    x1 = grmp.FEVector([fes])
+    grmp.mg_solve(levels, kappa0, b, x1, 1e-12)
+    assert np.array_equal(outs[0], outs[2])
+    assert np.array_equal(outs[0], x1.entries())
+    err = np.linalg.norm(outs[0] - xref) / np.linalg.norm(xref)
+    assert err <= 1e-8 and not np.array_equal(outs[0], outs[1])
+
+
 def test_multigrid_not_applicable(grmp, O):
     """solver="mg" on an uploaded mesh is refused; "auto" falls back."""
     print("[case] multigrid not applicable", flush=True)
