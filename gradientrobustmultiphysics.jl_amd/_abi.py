@@ -149,6 +149,7 @@ SIGNATURES = {
                                       ctypes.POINTER(VP)]),
     "grmp_mg_apply": (ctypes.c_int, [VP, VP, VP, ctypes.c_double, I64, ctypes.POINTER(SolveInfo)]),
     "grmp_mg_destroy": (ctypes.c_int, [VP]),
+    "grmp_vector_axpy": (ctypes.c_int, [ctypes.c_double, VP, VP]),
     "grmp_assemble_linearform": (ctypes.c_int, [VP, VP, ctypes.POINTER(LForm), VP, I64]),
     "grmp_integrate_identity": (ctypes.c_int, [VP, VP, ctypes.c_int, VP, I64, DP]),
     "grmp_solve_opts_default": (None, [ctypes.POINTER(SolveOpts)]),

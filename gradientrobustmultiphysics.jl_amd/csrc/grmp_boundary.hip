@@ -676,3 +676,23 @@ extern "C" int grmp_nodevalues(const grmp_space *s, const grmp_vector *u, int64_
     dfree(ctx, dout);
     return rc;
 }
+
+// ------------------------------------------------------------------ axpy
+__global__ void k_axpy(double a, const double *__restrict__ x, double *__restrict__ y, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += a * x[i];
+}
+
+extern "C" int grmp_vector_axpy(double a, const grmp_vector *x, grmp_vector *y)
+{
+    if (!x || !y) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = y->ctx;
+    GRMP_CHECK(ctx, x->ctx == ctx && x->n == y->n, GRMP_ERR_ARG, "grmp_vector_axpy: lengths %lld and %lld differ",
+               (long long)x->n, (long long)y->n);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (y->n == 0) return GRMP_OK;
+    hipLaunchKernelGGL(k_axpy, dim3(nblk(y->n, 256)), dim3(256), 0, ctx->stream, a, x->d, y->d, y->n);
+    GRMP_HIP(ctx, hipGetLastError());
+    return GRMP_OK;
+}

@@ -418,8 +418,9 @@ def assemble(pde, Solution, values=_abi.VALUES_FP64, penalty=1e60):
 
 def mg_eligible(pde, Solution):
     """True when grmp_mg_solve applies: one unknown, H1P1 (1 or 2
-    components) or scalar H1P2 on a generated unit square, homogeneous
-    Dirichlet data, no global constraints, at most one cell-coefficient
+    components) or scalar H1P2 on a generated unit square, Dirichlet data on
+    the whole boundary (inhomogeneous data is lifted: the solve runs with
+    homogeneous data), no global constraints, at most one cell-coefficient
     operator."""
     if len(Solution.FES) != 1 or pde.GlobalConstraints or list(pde.LHS) != [(0, 0)]:
         return False
@@ -428,8 +429,7 @@ def mg_eligible(pde, Solution):
     if fes.xgrid.kind != "unitsquare" or not ((ft.code == _abi.FE_H1P1 and ft.ncomp in (1, 2)) or
                                               (ft.code == _abi.FE_H1P2 and ft.ncomp == 1)):
         return False
-    if not pde.BoundaryOperators or any(kind is not HomogeneousDirichletBoundary
-                                        for kind, _ in pde.BoundaryOperators.values()):
+    if not pde.BoundaryOperators:
         return False
     return sum(getattr(op, "action", None) is not None for op in pde.LHS[(0, 0)]) <= 1
 
@@ -502,6 +502,14 @@ class MultigridSolver:
             pass
 
 
+def lift_dirichlet(A, b, target, out):
+    """out = b - A target on the free rows of A (penalty rows as target
+    times the penalty; the multigrid solve zeroes them): the right-hand side
+    for the correction x - target, which has homogeneous data."""
+    A.ctx.check(lib().grmp_timestep_rhs(A.h, A.h, b.vec.h, target.h, -1.0, out.vec.h))
+    return out
+
+
 def mg_solve(levels, kappa0, b, x, rtol, maxit=500):
     """grmp_mg_solve on a hierarchy from mg_hierarchy: x = S^{-1} b."""
     n = len(levels)
@@ -526,13 +534,17 @@ def solve(Solution, pde, config=None, solver="krylov", **kwargs):
     use_mg = solver != "krylov" and mg_eligible(pde, Solution)
     if solver == "mg" and not use_mg:
         raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "multigrid needs one H1P1 / H1P2 unknown on a generated unit "
-                             "square, homogeneous Dirichlet data and no global constraints")
+                             "square, Dirichlet data and no global constraints")
     fp32 = cfg.precision == "fp32_ir" and not use_mg
     values = _abi.VALUES_FP64 | (_abi.VALUES_FP32 if fp32 else 0)
     A, b, fixed = assemble(pde, Solution, values, cfg.fixed_penalty)
     if use_mg:
         levels, kappa0 = mg_hierarchy(pde, Solution.FES[0], A, penalty=cfg.fixed_penalty)
-        info = mg_solve(levels, kappa0, b, Solution, cfg.rtol, min(cfg.maxiterations, 500))
+        target = boundary_values(pde, Solution)
+        rhs = b if target is None else lift_dirichlet(A, b, target, FEVector(Solution.FES))
+        info = mg_solve(levels, kappa0, rhs, Solution, cfg.rtol, min(cfg.maxiterations, 500))
+        if target is not None:
+            A.ctx.check(lib().grmp_vector_axpy(1.0, target.h, Solution.vec.h))
         return info, A, b
     method = _abi.SOLVE_MINRES if _saddle_point(pde, Solution) else _abi.SOLVE_CG
     info = solve_system(A, b, Solution, cfg, method)

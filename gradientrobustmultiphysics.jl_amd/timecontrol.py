@@ -17,8 +17,9 @@ subiteration holding every unknown, time-independent data):
   change norms drop below the threshold.
 
 Each step's system is solved by the served path's multigrid V-cycle CG
-(FP64, grmp_mg_solve) when the problem is one scalar H1P1 unknown on a
-generated unit square with homogeneous Dirichlet data: the coarse levels
+(FP64, grmp_mg_create / grmp_mg_apply) when the problem is one scalar H1P1
+unknown on a generated unit square with Dirichlet data (inhomogeneous values
+lifted): the coarse levels
 rediscretize A + M/dt on squares of ceil(N/2) down to N <= 4.  Otherwise,
 or with solver="krylov", the device Krylov solver of solve_system runs.
 
@@ -32,7 +33,7 @@ from . import _abi
 from ._abi import lib
 from .fe import FEMatrix, FEVector
 from .pde import (ReactionOperator, SolverConfig, _saddle_point, assemble_linearform, boundary_values, fixed_dofs,
-                  MultigridSolver, mg_eligible, mg_hierarchy, realize_constraints, solve_system)
+                  MultigridSolver, lift_dirichlet, mg_eligible, mg_hierarchy, realize_constraints, solve_system)
 
 
 class BackwardEuler:
@@ -68,11 +69,12 @@ class TimeControlSolver:
         # the penalty rows of every step keep them
         bnd = [InitialValues.FES[j].boundary_dofs() + int(InitialValues.offsets[j])
                for j in sorted(pde.BoundaryOperators)]
+        self.target = None
         if bnd:
             bnd = np.concatenate(bnd)
-            target = boundary_values(pde, InitialValues)
+            self.target = boundary_values(pde, InitialValues)
             x = InitialValues.entries()
-            x[bnd] = 0.0 if target is None else target.download()[bnd]
+            x[bnd] = 0.0 if self.target is None else self.target.download()[bnd]
             InitialValues.set_entries(x)
         FES = InitialValues.FES
         # mass matrix of the time derivative, assembled once (dt_is_nonlinear = false)
@@ -108,7 +110,17 @@ class TimeControlSolver:
         return mg_eligible(self.PDE, self.X) and fes.FEType.code == _abi.FE_H1P1 and fes.FEType.ncomp == 1
 
     def _mg_solve(self):
-        return self.levels.solve(self.b, self.X, self.SC.rtol, min(self.SC.maxiterations, 500))
+        """The step's solve; inhomogeneous Dirichlet values are lifted (the
+        multigrid solve has homogeneous data) and added back."""
+        maxit = min(self.SC.maxiterations, 500)
+        if self.target is None:
+            return self.levels.solve(self.b, self.X, self.SC.rtol, maxit)
+        if getattr(self, "_lifted", None) is None:
+            self._lifted = FEVector(self.X.FES, name="lifted rhs")
+        lift_dirichlet(self.S, self.b, self.target, self._lifted)
+        info = self.levels.solve(self._lifted, self.X, self.SC.rtol, maxit)
+        self.X.ctx.check(lib().grmp_vector_axpy(1.0, self.target.h, self.X.vec.h))
+        return info
 
     def _system(self, timestep):
         """S = A + (1/dt) M with the fixed dofs penalized (:337-351)."""

@@ -151,6 +151,9 @@ int grmp_vector_scatter(grmp_vector *v, int64_t count, const int64_t *idx, const
 /* seeded log-normal field exp(mu + sigma Z), Z ~ N(0,1) from Philox-4x32-10
  * keyed by seed and counted by entry index (Box-Muller) */
 int grmp_vector_lognormal(grmp_vector *v, uint64_t seed, double mu, double sigma);
+/* y += a x (lifting of Dirichlet values around a solve with homogeneous
+ * data; no counterpart of its own in the reference) */
+int grmp_vector_axpy(double a, const grmp_vector *x, grmp_vector *y);
 
 /* --------------------------------------------------------------- meshes */
 /* ExtendableGrid upload: Coordinates [dim,nnodes], CellNodes [nloc,ncells]

@@ -67,6 +67,27 @@ def test_boundary_values_match_oracle(grmp, O, fe, N, seed):
         assert err <= (1e-12 if kind == "bestapprox" else 1e-14)
 
 
+def test_multigrid_solve_with_dirichlet_data(grmp, O):
+    """solve(solver="mg") lifts best-approximated data: P2 on a generated
+    square reproduces the quadratic solution."""
+    N = 16
+    print(f"[case] multigrid solve with dirichlet data N={N}", flush=True)
+    grid = grmp.grid_unitsquare(N)
+    coords, cells = grid.download()
+    fes = grmp.FESpace(grmp.H1P2(1), grid)
+    pde = grmp.PoissonProblem(1.0)
+    pde.add_rhsdata(0, grmp.LinearForm(grmp.Identity, grmp.DataFunction([-4.0])))
+    pde.add_boundarydata(0, grmp.BestapproxDirichletBoundary, data=grmp.PolynomialData(Q2))
+    sol = grmp.FEVector([fes])
+    info, A, b = grmp.solve(sol, pde, solver="mg", rtol=1e-12)
+    cf, fn, fc = O.tri_faces(cells, coords.shape[0])
+    P = np.vstack([coords, 0.5 * (coords[fn[:, 0]] + coords[fn[:, 1]])])
+    u = O.poly_eval(Q2, P[:, 0], P[:, 1])
+    err = np.max(np.abs(sol.entries() - u)) / np.max(np.abs(u))
+    print(f"  {info.iterations} iterations, max rel err vs exact {err:.2e}")
+    assert info.status == 0 and err <= 1e-9
+
+
 @pytest.mark.parametrize("fe,kind", [(2, "bestapprox"), (1, "interpolate"), (1, "bestapprox")])
 def test_poisson_with_dirichlet_data(grmp, O, fe, kind):
     """u = 1 + x - 2y + x^2/2 + xy + 3y^2/2 (P2, -Delta u = -4) or
