@@ -460,6 +460,15 @@ def run_workload(grmp, which, args, log):
                                    "cores": 1, "kind": "port",
                                    "sample": f"256x256 P2 ({bdo.size} boundary dofs): boundary walk, face matrices, "
                                              f"sparse LU; {tc:.2f} s"}
+            S1 = O.Space(256, O.FE_P1)
+            u1 = np.random.default_rng(3).standard_normal(S1.ndofs)
+            t0 = time.perf_counter()
+            O.nodevalues(O.FE_P1, 1, u1, S1.cells, S1.celldofs, S1.nnodes)
+            tn = time.perf_counter() - t0
+            out["cpu_baseline_nodevalues"] = {"value": round(S1.nnodes / tn, 1), "unit": "nodes/s", "cores": 1,
+                                              "kind": "port",
+                                              "sample": f"256x256 P1 ({S1.nnodes} nodes), cell-order averages; "
+                                                        f"{tn:.2f} s"}
     else:
         raise SystemExit(f"unknown workload {which}")
     print(json.dumps(out), flush=True)
