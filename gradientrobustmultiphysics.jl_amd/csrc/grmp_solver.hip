@@ -162,12 +162,23 @@ __global__ __launch_bounds__(PB) void k_plain_resid(const int32_t *__restrict__ 
     }
 }
 
+// Scalars of the plain CG on the device: alpha and beta are formed there
+// and the convergence test runs there, so the host reads the state back
+// once per chunk of iterations instead of twice per iteration.  Kernels of a
+// chunk launched after convergence return at once (done is uniform).
+struct PlainScal {
+    double rz, rr, bb, rtol2, alpha, beta;
+    long long it, maxit;
+    int done, status;
+};
+
 // q = A p on free rows; partial p.q
 __global__ __launch_bounds__(PB) void k_plain_spmv(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                                                   const double *__restrict__ v, const uint8_t *__restrict__ fixed,
                                                   const double *__restrict__ p, double *__restrict__ q, int64_t n,
-                                                  double *__restrict__ part)
+                                                  double *__restrict__ part, const PlainScal *__restrict__ S)
 {
+    if (S->done) return;
     __shared__ double lds[SPMV_CAP64];
     double a0 = 0.0, a1 = 0.0;
     for (int64_t r0 = (int64_t)blockIdx.x * PB; r0 < n; r0 += (int64_t)gridDim.x * PB) {
@@ -188,10 +199,12 @@ __global__ __launch_bounds__(PB) void k_plain_spmv(const int32_t *__restrict__ r
 }
 
 // x += alpha p; r -= alpha q; z = dinv r; partials (r.z, r.r)
-__global__ void k_plain_axpy(double alpha, const double *__restrict__ p, const double *__restrict__ q,
-                             const double *__restrict__ dinv, double *__restrict__ x, double *__restrict__ r,
-                             double *__restrict__ z, int64_t n, double *__restrict__ part)
+__global__ void k_plain_axpy(const PlainScal *__restrict__ S, const double *__restrict__ p,
+                             const double *__restrict__ q, const double *__restrict__ dinv, double *__restrict__ x,
+                             double *__restrict__ r, double *__restrict__ z, int64_t n, double *__restrict__ part)
 {
+    if (S->done) return;
+    const double alpha = S->alpha;
     double a0 = 0.0, a1 = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x; i < n; i += (int64_t)gridDim.x * PB) {
         x[i] += alpha * p[i];
@@ -209,9 +222,53 @@ __global__ void k_plain_axpy(double alpha, const double *__restrict__ p, const d
     }
 }
 
-__global__ void k_plain_pupd(double beta, const double *__restrict__ z, double *__restrict__ p, int64_t n)
+__global__ void k_plain_pupd(const PlainScal *__restrict__ S, const double *__restrict__ z, double *__restrict__ p,
+                             int64_t n)
 {
+    if (S->done) return;
+    const double beta = S->beta;
     for (int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x; i < n; i += (int64_t)gridDim.x * PB) p[i] = z[i] + beta * p[i];
+}
+
+// alpha = rz / (p.q) from the product's partials
+__global__ __launch_bounds__(PB) void k_plain_alpha(const double *__restrict__ part, int grid, PlainScal *S)
+{
+    if (S->done) return;
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < grid; k += PB) {
+        a += part[2 * k];
+        b += part[2 * k + 1];
+    }
+    block_sum2(a, b);
+    if (threadIdx.x == 0) S->alpha = S->rz / a;
+}
+
+// beta = rz' / rz, the iteration count and the tests the host loop made
+// before each iteration: non-finite, converged, iteration cap
+__global__ __launch_bounds__(PB) void k_plain_beta(const double *__restrict__ part, int grid, PlainScal *S)
+{
+    if (S->done) return;
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < grid; k += PB) {
+        a += part[2 * k];
+        b += part[2 * k + 1];
+    }
+    block_sum2(a, b);
+    if (threadIdx.x == 0) {
+        S->beta = a / S->rz;
+        S->rz = a;
+        S->rr = b;
+        S->it += 1;
+        if (!(b == b) || isinf(b)) {
+            S->done = 1;
+            S->status = GRMP_ERR_NONFINITE;
+        } else if (b <= S->rtol2 * S->bb || b == 0.0) {
+            S->done = 1;
+        } else if (S->it >= S->maxit) {
+            S->done = 1;
+            S->status = GRMP_ERR_NOCONV;
+        }
+    }
 }
 
 // Sums a pass's per-workgroup partial pairs in a fixed order (thread t
@@ -249,54 +306,81 @@ int plain_cg_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solv
     GRMP_CHECK(ctx, A->rowptr && A->cols && A->val64 && A->dinv64 && A->fixed && A->diagpos && bd && xd, GRMP_ERR_ARG,
                "plain CG: matrix not assembled in FP64");
     double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *part = nullptr;
+    PlainScal *dsc = nullptr, *hsc = nullptr;
     GRMP_TRY(dalloc(ctx, &r, n));
     GRMP_TRY(dalloc(ctx, &z, n));
     GRMP_TRY(dalloc(ctx, &p, n));
     GRMP_TRY(dalloc(ctx, &q, n));
     GRMP_TRY(dalloc(ctx, &part, 2 * PGRID_ROWS + 2));
+    GRMP_TRY(dalloc(ctx, &dsc, 1));
     double *fold = part + 2 * PGRID_ROWS;
-    double *hpart = nullptr;
-    GRMP_HIP(ctx, hipHostMalloc((void **)&hpart, sizeof(double) * 2, hipHostMallocDefault));
+    GRMP_HIP(ctx, hipHostMalloc((void **)&hsc, sizeof(PlainScal), hipHostMallocDefault));
+    double *hpart = &hsc->alpha;  // two doubles for the first fold
     const int grid = (int)std::min<int64_t>(PGRID_ROWS, std::max<int64_t>(1, (n + PB - 1) / PB));
     const int gride = (int)std::min<int64_t>(PGRID, grid);
     hipLaunchKernelGGL(k_plain_init, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, A->rowptr, A->diagpos, A->val64, A->fixed,
                        bd, xd, n);
-#define PL_SWITCH(KERNEL, ...) hipLaunchKernelGGL(KERNEL, dim3(grid), dim3(PB), 0, ctx->stream, __VA_ARGS__)
-    PL_SWITCH(k_plain_resid, A->rowptr, A->cols, A->val64, A->fixed, A->dinv64, bd, xd, r, z, p, n, part);
-    double rz, rr, pq, dummy;
+    hipLaunchKernelGGL(k_plain_resid, dim3(grid), dim3(PB), 0, ctx->stream, A->rowptr, A->cols, A->val64, A->fixed,
+                       A->dinv64, bd, xd, r, z, p, n, part);
+    double rz, rr;
     int rc = plain_sum(ctx, part, fold, hpart, grid, rz, rr);
-    double bb = rr;
+    const double bb = rr;
     int64_t it = 0;
     int status = GRMP_OK;
-    const int every = std::max(1, o.check_every);
-    while (rc == GRMP_OK) {
-        if (!(rr == rr) || std::isinf(rr)) { status = GRMP_ERR_NONFINITE; break; }
-        if (rr <= o.rtol * o.rtol * bb || rr == 0.0) break;
-        if (it >= o.maxit) { status = GRMP_ERR_NOCONV; break; }
-        PL_SWITCH(k_plain_spmv, A->rowptr, A->cols, A->val64, A->fixed, p, q, n, part);
-        if ((rc = plain_sum(ctx, part, fold, hpart, grid, pq, dummy))) break;
-        double alpha = rz / pq;
-        hipLaunchKernelGGL(k_plain_axpy, dim3(gride), dim3(PB), 0, ctx->stream, alpha, p, q, A->dinv64, xd, r, z, n, part);
-        double rz2;
-        if ((rc = plain_sum(ctx, part, fold, hpart, gride, rz2, rr))) break;
-        double beta = rz2 / rz;
-        rz = rz2;
-        hipLaunchKernelGGL(k_plain_pupd, dim3(gride), dim3(PB), 0, ctx->stream, beta, z, p, n);
-        ++it;
-        (void)every;
+    // the tests of the first iteration on the host, the rest on the device
+    if (rc == GRMP_OK) {
+        if (!(rr == rr) || std::isinf(rr)) status = GRMP_ERR_NONFINITE;
+        else if (rr <= o.rtol * o.rtol * bb || rr == 0.0) status = GRMP_OK;
+        else if (it >= o.maxit) status = GRMP_ERR_NOCONV;
+        else {
+            PlainScal s0;
+            memset(&s0, 0, sizeof(s0));
+            s0.rz = rz;
+            s0.rr = rr;
+            s0.bb = bb;
+            s0.rtol2 = o.rtol * o.rtol;
+            s0.maxit = o.maxit;
+            *hsc = s0;
+            if (hipMemcpyAsync(dsc, hsc, sizeof(PlainScal), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+                rc = grmp_fail(ctx, GRMP_ERR_HIP, "plain CG: scalars");
+            const int chunk = 8;
+            while (rc == GRMP_OK) {
+                for (int k = 0; k < chunk; ++k) {
+                    hipLaunchKernelGGL(k_plain_spmv, dim3(grid), dim3(PB), 0, ctx->stream, A->rowptr, A->cols,
+                                       A->val64, A->fixed, p, q, n, part, dsc);
+                    hipLaunchKernelGGL(k_plain_alpha, dim3(1), dim3(PB), 0, ctx->stream, part, grid, dsc);
+                    hipLaunchKernelGGL(k_plain_axpy, dim3(gride), dim3(PB), 0, ctx->stream, dsc, p, q, A->dinv64, xd,
+                                       r, z, n, part);
+                    hipLaunchKernelGGL(k_plain_beta, dim3(1), dim3(PB), 0, ctx->stream, part, gride, dsc);
+                    hipLaunchKernelGGL(k_plain_pupd, dim3(gride), dim3(PB), 0, ctx->stream, dsc, z, p, n);
+                }
+                if (hipGetLastError() != hipSuccess ||
+                    hipMemcpyAsync(hsc, dsc, sizeof(PlainScal), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+                    hipStreamSynchronize(ctx->stream) != hipSuccess) {
+                    rc = grmp_fail(ctx, GRMP_ERR_HIP, "plain CG: iteration chunk");
+                    break;
+                }
+                if (hsc->done) break;
+            }
+            if (rc == GRMP_OK) {
+                it = hsc->it;
+                rr = hsc->rr;
+                status = hsc->status;
+            }
+        }
     }
-#undef PL_SWITCH
     GRMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     info.iterations = it;
     info.outer = 1;
     info.relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
     info.status = rc ? rc : status;
-    (void)hipHostFree(hpart);
+    (void)hipHostFree(hsc);
     dfree(ctx, r);
     dfree(ctx, z);
     dfree(ctx, p);
     dfree(ctx, q);
     dfree(ctx, part);
+    dfree(ctx, dsc);
     return rc;
 }
 
