@@ -86,3 +86,58 @@ def test_faces_first_appearance(O):
         # every interior face is shared by exactly two cells
         cnt = np.bincount(S.cellfaces.ravel(), minlength=S.nfaces)
         assert set(cnt.tolist()) <= {1, 2}
+
+
+def _csc_close(A, g, rtol):
+    """pattern identical; every entry within rtol of the golden's (exact) value"""
+    assert np.array_equal(A.colptr, np.array(g["colptr"]))
+    assert np.array_equal(A.rowval, np.array(g["rowval"]))
+    ref = np.array(g["nzval"])
+    err = np.abs(A.nzval - ref)
+    assert np.all(err <= rtol * np.abs(ref)), float(np.max(err / np.maximum(np.abs(ref), 1e-300)))
+
+
+@pytest.mark.parametrize("N", [2, 4])
+def test_p1_mass_matrix(O, N):
+    """P1 mass |T|/12 (1 + delta_ij) (the 3-point rule is exact on P1 x P1;
+    a one-point or lumped rule fails the off-diagonal values)."""
+    g = _load(f"mass_p1_N{N}.json")
+    S = O.Space(N, O.FE_P1)
+    M = O.assemble_matrix(S.ndofs, [dict(kind=O.BLF_MASS, fe=O.FE_P1, quadorder=2, factor=1.0,
+                                        coords=S.coords, cells=S.cells, celldofs=S.celldofs)])
+    # the rule's weight 1/3 is not dyadic: a few ulps
+    _csc_close(M, g, 4e-16)
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_tet_p1_stiffness(O, n):
+    """P1 stiffness on the Kuhn cube: the 7-point stencil with its exact
+    values (off-path couplings of each tetrahedron are exactly zero, so the
+    reference's zero test drops them)."""
+    g = _load(f"tet_p1_n{n}.json")
+    coords = np.array(g["coords"])
+    cells = np.array(g["cells"], dtype=np.int32)
+    A = O.assemble_matrix(g["ndofs"], [dict(kind=O.BLF_LAPLACE, geom=3, fe=O.FE_P1, quadorder=0, factor=1.0,
+                                           coords=coords, cells=cells, celldofs=cells)])
+    # |det|/6 is not dyadic: a few ulps
+    _csc_close(A, g, 8e-16)
+    # 7-point stencil at the interior node: 6/n, -1/n on the axis neighbours
+    if n == 3:
+        n1 = n + 1
+        r = (1 * n1 + 1) * n1 + 1
+        M = A.to_scipy().tocsr()
+        row = M.getrow(r)
+        assert sorted(row.indices.tolist()) == sorted([r, r - 1, r + 1, r - n1, r + n1, r - n1 * n1, r + n1 * n1])
+        assert abs(M[r, r] - 6.0 / n) < 1e-14 and all(abs(M[r, c] + 1.0 / n) < 1e-14 for c in row.indices if c != r)
+
+
+@pytest.mark.parametrize("N", [2, 4])
+def test_br_stokes_blocks(O, N):
+    """H1BR{2} x L2P0: the viscous block nu (grad u, grad v), the divergence
+    block -(div v, q) and its transposed copy, with the no-slip penalty,
+    against exact arithmetic over Q(sqrt 2)."""
+    g = _load(f"stokes_br_N{N}.json")
+    S = O.StokesSpace(N)
+    assert S.n == g["n"] and np.array_equal(S.fixed, np.array(g["fixed"]))
+    A = S.matrix(float(Fraction(g["nu"])))
+    _csc_close(A, g, 2e-15)
