@@ -152,6 +152,8 @@ SIGNATURES = {
     "grmp_vector_axpy": (ctypes.c_int, [ctypes.c_double, VP, VP]),
     "grmp_assemble_linearform": (ctypes.c_int, [VP, VP, ctypes.POINTER(LForm), VP, I64]),
     "grmp_integrate_identity": (ctypes.c_int, [VP, VP, ctypes.c_int, VP, I64, DP]),
+    "grmp_integrate_l2error": (ctypes.c_int, [VP, VP, VP, I64, DP, ctypes.c_int, ctypes.c_int, ctypes.c_double, DP,
+                                              ctypes.POINTER(ctypes.c_double)]),
     "grmp_solve_opts_default": (None, [ctypes.POINTER(SolveOpts)]),
     "grmp_solve": (ctypes.c_int, [VP, VP, VP, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
     "grmp_solve_csc": (ctypes.c_int, [VP, I64, I64P, I64P, DP, DP, DP, I64, I64P, ctypes.c_int, ctypes.c_double]),

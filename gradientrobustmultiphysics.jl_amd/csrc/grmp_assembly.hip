@@ -14,7 +14,6 @@
 // copy of its row at a precomputed position (the 8-bit slot map).  Every
 // entry is therefore summed in cell order -- the reference's order -- with no
 // atomics, and each row is written to HBM once, coalesced across the block.
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -1094,6 +1093,16 @@ extern "C" int grmp_matrix_build_pattern(grmp_matrix *A)
         }
     }
     A->max_row = flags[1];
+    // the row pointers are int32: check the total in 64 bits before the scan
+    int64_t total = 0;
+    int rc64 = sum_i32_as_i64(ctx, count, A->n, &total);
+    if (!rc64 && total > GRMP_MAX_NNZ)
+        rc64 = grmp_fail(ctx, GRMP_ERR_UNSUPPORTED, "grmp_matrix_build_pattern: %lld nonzeros exceed the int32 CSR "
+                         "limit %lld", (long long)total, (long long)GRMP_MAX_NNZ);
+    if (rc64) {
+        dfree(ctx, count);
+        return rc64;
+    }
     GRMP_TRY(exclusive_scan_i32(ctx, count, A->rowptr, A->n));
     int32_t nnz32 = 0;
     GRMP_HIP(ctx, hipMemcpyAsync(&nnz32, A->rowptr + A->n, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -1710,6 +1719,201 @@ extern "C" int grmp_integrate_identity(grmp_ctx *ctx, const grmp_space *s, int q
     GRMP_HIP(ctx, hipMemcpyAsync(out, d, sizeof(double) * m->ncells * s->ncomp, hipMemcpyDeviceToHost, ctx->stream));
     GRMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     dfree(ctx, d);
+    return GRMP_OK;
+}
+
+// ----------------------------------------------------- L2ErrorIntegrator
+// L2ErrorIntegrator(g, Identity; quadorder, factor) and L2NormIntegrator
+// (src/assemblypatterns/itemintegrator.jl:40-95): evaluate! (:243-307) with
+// the action L2error_function, per cell
+//   out[c] = sum_q (sum_k (g_k(x_q) - u_h,k(x_q) * factor)^2) * w_q * vol,
+// u_h,k(x_q) = sum_dofs u_dof * cvals[k, dof, q] (eval_febe!, src/feevaluator.jl:445-452),
+// x_q = x_1 + A xref_q.  g is a polynomial of degree <= 3 per component
+// (ncoef 0: g = 0, the L2 norm).  P1, P2 (1 or 2 components), H1BR{2}
+// (Identity with coefficients: face bubbles times the face normal,
+// src/feevaluator_h1.jl:31-42) and L2P0 on triangles.
+#define L2E_MAXQ RECO_MAXQ
+struct L2eDev {
+    double xq[L2E_MAXQ][2], wq[L2E_MAXQ];
+    double g[2][10];
+    double factor;
+    int nq, fe, ncomp, per;
+    const double *coords;
+    const int32_t *cells, *celldofs;
+    const int8_t *fsign;
+    int64_t nnodes, ncells;
+};
+
+__device__ __forceinline__ double l2e_poly(const double *c, double x, double y)
+{
+    double s = 0.0;
+    s += c[0];
+    s += c[1] * x;
+    s += c[2] * y;
+    s += c[3] * (x * x);
+    s += c[4] * (x * y);
+    s += c[5] * (y * y);
+    s += c[6] * (x * x * x);
+    s += c[7] * (x * x * y);
+    s += c[8] * (x * y * y);
+    s += c[9] * (y * y * y);
+    return s;
+}
+
+// value of scalar shape function sf of the element at (x0, x1)
+__device__ __forceinline__ double l2e_shape(int fe, int sf, double x0, double x1)
+{
+    if (fe == GRMP_FE_H1P1) return p1tri_val(sf, x0, x1);
+    if (fe == GRMP_FE_H1P2) return p2tri_val(sf, x0, x1);
+    // H1BR{2}: P1 nodes, then bubbles 6 x l, 6 y x, 6 l y (src/fedefs/h1v_br.jl:117-130)
+    double l = 1.0;
+    l -= x0;
+    l -= x1;
+    if (sf < 3) return sf == 0 ? l : (sf == 1 ? x0 : x1);
+    if (sf == 3) return 6.0 * x0 * l;
+    if (sf == 4) return 6.0 * x1 * x0;
+    return 6.0 * l * x1;
+}
+
+__global__ void k_l2error(L2eDev L, const double *__restrict__ u, double *__restrict__ out)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= L.ncells) return;
+    const int64_t nc = L.ncells, nn = L.nnodes;
+    const int32_t v0 = L.cells[e], v1 = L.cells[nc + e], v2 = L.cells[2 * nc + e];
+    const double X0 = L.coords[v0], Y0 = L.coords[nn + v0];
+    const double A11 = L.coords[v1] - X0, A12 = L.coords[v2] - X0;
+    const double A21 = L.coords[nn + v1] - Y0, A22 = L.coords[nn + v2] - Y0;
+    const double vol = fabs(A11 * A22 - A12 * A21) * 0.5;
+    double n[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    if (L.fe == GRMP_FE_H1BR) {
+        const int32_t vv[3] = {v0, v1, v2};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const bool own = L.fsign[k * nc + e] > 0;
+            const int32_t a = own ? vv[k] : vv[(k + 1) % 3], b = own ? vv[(k + 1) % 3] : vv[k];
+            const double tx = L.coords[b] - L.coords[a], ty = L.coords[nn + b] - L.coords[nn + a];
+            const double len = sqrt(tx * tx + ty * ty);
+            n[k][0] = ty / len;
+            n[k][1] = -tx / len;
+        }
+    }
+    double acc = 0.0;
+    for (int q = 0; q < L.nq; ++q) {
+        const double x0 = L.xq[q][0], x1 = L.xq[q][1];
+        double in[2] = {0.0, 0.0};
+        if (L.fe == GRMP_FE_L2P0) {
+            in[0] += u[e] * 1.0;
+        } else if (L.fe == GRMP_FE_H1BR) {
+            for (int dof = 0; dof < 9; ++dof) {
+                const double ud = u[L.celldofs[(int64_t)dof * nc + e]];
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    double cv;
+                    if (dof < 6) cv = (dof / 3 == k) ? l2e_shape(L.fe, dof % 3, x0, x1) * 1.0 : 0.0;
+                    else cv = l2e_shape(L.fe, dof - 3, x0, x1) * n[dof - 6][k];
+                    in[k] += ud * cv * 1.0;
+                }
+            }
+        } else {
+            for (int c = 0; c < L.ncomp; ++c)
+                for (int s = 0; s < L.per; ++s) {
+                    const double ud = u[L.celldofs[(int64_t)(c * L.per + s) * nc + e]];
+                    for (int k = 0; k < L.ncomp; ++k) in[k] += ud * (c == k ? l2e_shape(L.fe, s, x0, x1) : 0.0) * 1.0;
+                }
+        }
+        double xp = 0.0, yp = 0.0;
+        xp += A11 * x0;
+        xp += A12 * x1;
+        xp += X0;
+        yp += A21 * x0;
+        yp += A22 * x1;
+        yp += Y0;
+        double r = 0.0;
+        for (int k = 0; k < L.ncomp; ++k) {
+            double val = l2e_poly(L.g[k], xp, yp);
+            val -= in[k] * L.factor;
+            r += val * val;
+        }
+        acc += r * L.wq[q] * vol;
+    }
+    out[e] = acc;
+}
+
+// fixed-order sum: each block sums its 1024 entries by a fixed tree, then one
+// block sums the block partials the same way (deterministic for a given n)
+__device__ __forceinline__ double l2e_block_sum(double v, double *red)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += red[k];
+    return s;
+}
+
+__global__ __launch_bounds__(1024) void k_sum_blocks(const double *__restrict__ x, int64_t n, double *__restrict__ part)
+{
+    __shared__ double red[16];
+    const int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    const double s = l2e_block_sum(i < n ? x[i] : 0.0, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(1024) void k_sum_final(const double *__restrict__ part, int64_t np, double *__restrict__ out)
+{
+    __shared__ double red[16];
+    double v = 0.0;
+    for (int64_t i = threadIdx.x; i < np; i += 1024) v += part[i];
+    const double s = l2e_block_sum(v, red);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+extern "C" int grmp_integrate_l2error(grmp_ctx *ctx, const grmp_space *s, const grmp_vector *u, int64_t offset,
+                                      const double *coef, int ncoef, int quadorder, double factor, double *cellvals,
+                                      double *total)
+{
+    GRMP_CHECK(ctx, ctx && s && u && (cellvals || total), GRMP_ERR_ARG, "grmp_integrate_l2error: null argument");
+    GRMP_CHECK(ctx, offset >= 0 && offset + s->ndofs <= u->n, GRMP_ERR_ARG, "grmp_integrate_l2error: vector too short");
+    GRMP_CHECK(ctx, ncoef >= 0 && ncoef <= 10 && (ncoef == 0 || coef), GRMP_ERR_ARG,
+               "grmp_integrate_l2error: 0..10 polynomial coefficients per component");
+    const grmp_mesh *m = s->mesh;
+    GRMP_CHECK(ctx, m->dim == 2 && m->nloc == 3, GRMP_ERR_UNSUPPORTED, "grmp_integrate_l2error: triangle meshes only");
+    GRMP_CHECK(ctx, ((s->fe == GRMP_FE_H1P1 || s->fe == GRMP_FE_H1P2) && s->ncomp <= 2) || s->fe == GRMP_FE_L2P0 ||
+                        (s->fe == GRMP_FE_H1BR && m->fsign),
+               GRMP_ERR_UNSUPPORTED, "grmp_integrate_l2error: H1P1/H1P2 (1-2 components), H1BR{2}, L2P0 only");
+    L2eDev L;
+    memset(&L, 0, sizeof(L));
+    L.nq = tri_rule(quadorder, L.xq, L.wq);
+    GRMP_CHECK(ctx, L.nq > 0, GRMP_ERR_UNSUPPORTED, "grmp_integrate_l2error: quadrature order %d too high", quadorder);
+    L.fe = s->fe;
+    L.ncomp = s->ncomp;
+    L.per = s->fe == GRMP_FE_H1P2 ? 6 : 3;
+    for (int k = 0; k < L.ncomp; ++k)
+        for (int j = 0; j < ncoef; ++j) L.g[k][j] = coef[k * ncoef + j];
+    L.factor = factor;
+    L.coords = m->coords;
+    L.cells = m->cells;
+    L.celldofs = s->celldofs;
+    L.fsign = m->fsign;
+    L.nnodes = m->nnodes;
+    L.ncells = m->ncells;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    const int64_t nc = m->ncells, nb = (nc + 1023) / 1024;
+    double *d = nullptr;
+    GRMP_TRY(dalloc(ctx, &d, nc + nb + 1));
+    hipLaunchKernelGGL(k_l2error, dim3(nblk(nc, 256)), dim3(256), 0, ctx->stream, L, u->d + offset, d);
+    hipLaunchKernelGGL(k_sum_blocks, dim3((unsigned)nb), dim3(1024), 0, ctx->stream, d, nc, d + nc);
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(1024), 0, ctx->stream, d + nc, nb, d + nc + nb);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && cellvals)
+        e = hipMemcpyAsync(cellvals, d, sizeof(double) * nc, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && total)
+        e = hipMemcpyAsync(total, d + nc + nb, sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dfree(ctx, d);
+    GRMP_HIP(ctx, e);
     return GRMP_OK;
 }
 

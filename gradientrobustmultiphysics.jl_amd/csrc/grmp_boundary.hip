@@ -1,8 +1,8 @@
 // Inhomogeneous Dirichlet data of libgrmp_hip.so: the boundary values of an
-// H1 P1/P2 space for polynomial data g, by interpolation or by the L2 best
-// approximation on the boundary faces, and the penalized right-hand side
-// that carries them into the solve.  At the end: the node values of a
-// solution for export.
+// H1 P1/P2 or H1BR{2} space for polynomial data g, by interpolation or by
+// the L2 best approximation on the boundary faces, and the penalized
+// right-hand side that carries them into the solve.  At the end: the node
+// values of a solution for export.
 //
 // boundarydata! (src/boundarydata.jl:198-395):
 //  * InterpolateDirichletBoundary: the dofs of the boundary faces take the
@@ -13,6 +13,12 @@
 //    (the reference factorizes it; here a Jacobi-PCG runs to a relative
 //    residual of rtol inside one workgroup).
 //  * solve_direct! then fixes b_d = penalty * Target_d (src/solvers.jl:647-652).
+//  * H1BR{2} (src/fedefs/h1v_br.jl:104-113,186-194): a boundary face carries
+//    five dofs, the P1 node dofs of both components ("N1i1") and the face
+//    bubble 6 s (1 - s) times the face normal, so the components couple; a
+//    face is one unit of the loops below.  Its interpolation keeps the data's
+//    normal flux through the face (:37-81): bubble = sum_c (int_F g_c -
+//    int_F I g_c) n_c / |F|.
 //
 // Boundary faces are walked in cell order and local face order (0: nodes
 // 0-1, 1: 1-2, 2: 2-0), each face oriented as its cell's local face; a
@@ -44,9 +50,10 @@ struct BdMesh {
     const int32_t *cellfaces; // [3][ncells] or nullptr (generated unit square)
     const uint8_t *bface;     // [nfaces] or nullptr
     const int32_t *celldofs;  // [nloc][ncells]
+    const int8_t *fsign;      // [3][ncells] CellFaceSigns (H1BR{2})
     int64_t nnodes, ncells;
     int N, fe;
-    int ncomp, per;           // components; local dofs per component (3 P1, 6 P2)
+    int ncomp, per;           // components of a face unit (H1BR{2}: 1 unit of both); local dofs per component
 };
 
 __device__ __forceinline__ double bd_g(const BdData &g, int comp, double x, double y)
@@ -106,10 +113,20 @@ __device__ __forceinline__ bool bd_is_boundary(const BdMesh &M, int64_t c, int k
     return (ia == 0 && ib == 0) || (ia == M.N && ib == M.N) || (ja == 0 && jb == 0) || (ja == M.N && jb == M.N);
 }
 
+#define BD_MAXD 5  // dofs of a face unit (H1BR{2}: 2 nodes x 2 components + the bubble)
+
 // local dofs of component comp on boundary face (c, k): global dof numbers
-__device__ __forceinline__ int bd_face_dofs(const BdMesh &M, int64_t c, int k, int comp, int32_t d[3])
+__device__ __forceinline__ int bd_face_dofs(const BdMesh &M, int64_t c, int k, int comp, int32_t d[BD_MAXD])
 {
     const int64_t nc = M.ncells, base = (int64_t)comp * M.per;
+    if (M.fe == GRMP_FE_H1BR) {
+        d[0] = M.celldofs[k * nc + c];
+        d[1] = M.celldofs[((k + 1) % 3) * nc + c];
+        d[2] = M.celldofs[(3 + k) * nc + c];
+        d[3] = M.celldofs[(3 + (k + 1) % 3) * nc + c];
+        d[4] = M.celldofs[(6 + k) * nc + c];
+        return 5;
+    }
     d[0] = M.celldofs[(base + k) * nc + c];
     d[1] = M.celldofs[(base + (k + 1) % 3) * nc + c];
     if (M.fe == GRMP_FE_H1P2) {
@@ -120,8 +137,8 @@ __device__ __forceinline__ int bd_face_dofs(const BdMesh &M, int64_t c, int k, i
 }
 
 // a boundary face of one component: v = i ncomp + comp over the face list
-__device__ __forceinline__ int bd_vface(const BdMesh &M, const int32_t *bf, int64_t v, int64_t &c, int &k, int32_t d[3],
-                                        int &comp)
+__device__ __forceinline__ int bd_vface(const BdMesh &M, const int32_t *bf, int64_t v, int64_t &c, int &k,
+                                        int32_t d[BD_MAXD], int &comp)
 {
     const int64_t i = v / M.ncomp;
     comp = (int)(v - i * M.ncomp);
@@ -139,6 +156,34 @@ __device__ __forceinline__ void bd_face_geom(const BdMesh &M, int64_t c, int k, 
     y0 = M.coords[nn + a];
     x1 = M.coords[b];
     y1 = M.coords[nn + b];
+}
+
+// FaceNormals of boundary face (c, k) in its global orientation (CellFaceSigns)
+__device__ __forceinline__ void bd_face_normal(const BdMesh &M, int64_t c, int k, double n[2])
+{
+    const int64_t nc = M.ncells, nn = M.nnodes;
+    const int32_t a0 = M.cells[k * nc + c], b0 = M.cells[((k + 1) % 3) * nc + c];
+    const bool own = M.fsign[k * nc + c] > 0;
+    const int32_t a = own ? a0 : b0, b = own ? b0 : a0;
+    const double tx = M.coords[b] - M.coords[a], ty = M.coords[nn + b] - M.coords[nn + a];
+    const double len = sqrt(tx * tx + ty * ty);
+    n[0] = ty / len;
+    n[1] = -tx / len;
+}
+
+// H1BR{2} trace basis on the face: P1 per component, then the normal bubble
+__device__ __forceinline__ void bd_br_val(int a, double s, const double n[2], double v[2])
+{
+    const double l = 1.0 - s;
+    if (a < 4) {
+        const double p = (a & 1) ? s : l;
+        v[0] = a < 2 ? p : 0.0;
+        v[1] = a < 2 ? 0.0 : p;
+    } else {
+        const double bub = 6.0 * s * l;
+        v[0] = bub * n[0];
+        v[1] = bub * n[1];
+    }
 }
 
 // ------------------------------------------------------------ boundary faces
@@ -165,7 +210,7 @@ __global__ void k_bd_dofflag(BdMesh M, const int32_t *__restrict__ bf, int64_t n
     if (v >= nv) return;
     int64_t c;
     int k, comp;
-    int32_t d[3];
+    int32_t d[BD_MAXD];
     const int nd = bd_vface(M, bf, v, c, k, d, comp);
     for (int a = 0; a < nd; ++a) dflag[d[a]] = 1;
 }
@@ -191,7 +236,7 @@ __global__ void k_bd_count(BdMesh M, const int32_t *__restrict__ bf, int64_t nv,
     if (v >= nv) return;
     int64_t c;
     int k, cp;
-    int32_t d[3];
+    int32_t d[BD_MAXD];
     const int nd = bd_vface(M, bf, v, c, k, d, cp);
     for (int a = 0; a < nd; ++a) atomicAdd(&cnt[comp[d[a]]], 1);
 }
@@ -203,12 +248,12 @@ __global__ void k_bd_fill(BdMesh M, const int32_t *__restrict__ bf, int64_t nv, 
     if (v >= nv) return;
     int64_t c;
     int k, cp;
-    int32_t d[3];
+    int32_t d[BD_MAXD];
     const int nd = bd_vface(M, bf, v, c, k, d, cp);
     for (int a = 0; a < nd; ++a) {
         const int32_t r = comp[d[a]];
         const int32_t slot = ptr[r] + atomicAdd(&cur[r], 1);
-        inc[slot] = (int32_t)(v * 4 + a);  // component face index, local dof
+        inc[slot] = (int32_t)(v * 8 + a);  // component face index, local dof
     }
 }
 
@@ -251,20 +296,32 @@ __device__ __forceinline__ double bd_row_apply(const BdSys &S, int64_t r, const 
 {
     double y = 0.0, dg = 0.0;
     for (int32_t p = S.ptr[r]; p < S.ptr[r + 1]; ++p) {
-        const int64_t v = S.inc[p] >> 2;
-        const int a = S.inc[p] & 3;
+        const int64_t v = S.inc[p] >> 3;
+        const int a = S.inc[p] & 7;
         double len, x0, y0, x1, y1;
         bd_face_len(S, v / S.M.ncomp, len, x0, y0, x1, y1);
         int64_t c;
         int k, comp;
-        int32_t d[3];
+        int32_t d[BD_MAXD];
         const int nd = bd_vface(S.M, S.bf, v, c, k, d, comp);
+        double n[2] = {0.0, 0.0};
+        if (S.M.fe == GRMP_FE_H1BR) bd_face_normal(S.M, c, k, n);
         for (int b = 0; b < nd; ++b) {
             double m = 0.0;
             for (int q = 0; q < S.nq; ++q) {
                 double s, w;
                 bd_gauss(S.nq, q, s, w);
-                m += w * (bd_phi(S.M.fe, a, s) * bd_phi(S.M.fe, b, s));
+                if (S.M.fe == GRMP_FE_H1BR) {
+                    double va[2], vb[2];
+                    bd_br_val(a, s, n, va);
+                    bd_br_val(b, s, n, vb);
+                    double t = 0.0;
+                    t += va[0] * vb[0];
+                    t += va[1] * vb[1];
+                    m += w * t;
+                } else {
+                    m += w * (bd_phi(S.M.fe, a, s) * bd_phi(S.M.fe, b, s));
+                }
             }
             m *= len;
             if (x) y += m * x[S.comp[d[b]]];
@@ -279,17 +336,31 @@ __device__ __forceinline__ double bd_row_load(const BdSys &S, int64_t r)
 {
     double f = 0.0;
     for (int32_t p = S.ptr[r]; p < S.ptr[r + 1]; ++p) {
-        const int64_t v = S.inc[p] >> 2;
-        const int a = S.inc[p] & 3;
+        const int64_t v = S.inc[p] >> 3;
+        const int a = S.inc[p] & 7;
         const int comp = (int)(v % S.M.ncomp);
         double len, x0, y0, x1, y1;
         bd_face_len(S, v / S.M.ncomp, len, x0, y0, x1, y1);
+        double n[2] = {0.0, 0.0};
+        if (S.M.fe == GRMP_FE_H1BR) {
+            const int64_t i = v / S.M.ncomp, c = S.bf[i] / 3;
+            bd_face_normal(S.M, c, S.bf[i] - 3 * (int)c, n);
+        }
         double l = 0.0;
         for (int q = 0; q < S.nq; ++q) {
             double s, w;
             bd_gauss(S.nq, q, s, w);
             const double xq = x0 + s * (x1 - x0), yq = y0 + s * (y1 - y0);
-            l += w * (bd_g(S.g, comp, xq, yq) * bd_phi(S.M.fe, a, s));
+            if (S.M.fe == GRMP_FE_H1BR) {
+                double va[2];
+                bd_br_val(a, s, n, va);
+                double t = 0.0;
+                t += bd_g(S.g, 0, xq, yq) * va[0];
+                t += bd_g(S.g, 1, xq, yq) * va[1];
+                l += w * t;
+            } else {
+                l += w * (bd_g(S.g, comp, xq, yq) * bd_phi(S.M.fe, a, s));
+            }
         }
         f += l * len;
     }
@@ -384,10 +455,38 @@ __global__ void k_bd_interp(BdMesh M, BdData g, const int32_t *__restrict__ bf, 
     if (v >= nv) return;
     int64_t c;
     int k, comp;
-    int32_t d[3];
+    int32_t d[BD_MAXD];
     const int nd = bd_vface(M, bf, v, c, k, d, comp);
     double x0, y0, x1, y1;
     bd_face_geom(M, c, k, x0, y0, x1, y1);
+    if (M.fe == GRMP_FE_H1BR) {
+        // nodes, then the bubble that keeps the normal flux of g through the face
+        const double g0[2] = {bd_g(g, 0, x0, y0), bd_g(g, 1, x0, y0)};
+        const double g1[2] = {bd_g(g, 0, x1, y1), bd_g(g, 1, x1, y1)};
+        target[offset + d[0]] = g0[0];
+        target[offset + d[1]] = g1[0];
+        target[offset + d[2]] = g0[1];
+        target[offset + d[3]] = g1[1];
+        double n[2];
+        bd_face_normal(M, c, k, n);
+        const double dx = x1 - x0, dy = y1 - y0, len = sqrt(dx * dx + dy * dy);
+        double val = 0.0;
+        for (int cc = 0; cc < 2; ++cc) {
+            double mean = 0.0;  // int_F g_c: 4-point Gauss (exact to degree 7)
+            for (int q = 0; q < BD_MAXQ; ++q) {
+                double s, w;
+                bd_gauss(BD_MAXQ, q, s, w);
+                mean += w * bd_g(g, cc, x0 + s * dx, y0 + s * dy);
+            }
+            mean *= len;
+            double p1 = 0.0;
+            p1 += g0[cc] * len / 2.0;
+            p1 += g1[cc] * len / 2.0;
+            val += (mean - p1) * n[cc];
+        }
+        target[offset + d[4]] = val / len;
+        return;
+    }
     // a dof shared by two faces gets the same value from both (its point)
     target[offset + d[0]] = bd_g(g, comp, x0, y0);
     target[offset + d[1]] = bd_g(g, comp, x1, y1);
@@ -412,8 +511,9 @@ static int bd_check(grmp_ctx *ctx, const grmp_space *s, const double *coef, int 
 {
     GRMP_CHECK(ctx, s->mesh->dim == 2 && s->mesh->nloc == 3, GRMP_ERR_UNSUPPORTED,
                "Dirichlet data: triangle meshes only");
-    GRMP_CHECK(ctx, (s->fe == GRMP_FE_H1P1 || s->fe == GRMP_FE_H1P2) && s->ncomp >= 1 && s->ncomp <= BD_MAXCOMP,
-               GRMP_ERR_UNSUPPORTED, "Dirichlet data: H1P1 / H1P2 spaces with 1 or 2 components only");
+    GRMP_CHECK(ctx, ((s->fe == GRMP_FE_H1P1 || s->fe == GRMP_FE_H1P2) && s->ncomp >= 1 && s->ncomp <= BD_MAXCOMP) ||
+                        (s->fe == GRMP_FE_H1BR && s->ncomp == 2 && s->mesh->fsign),
+               GRMP_ERR_UNSUPPORTED, "Dirichlet data: H1P1 / H1P2 spaces with 1 or 2 components and H1BR{2} only");
     GRMP_CHECK(ctx, s->mesh->cellfaces || s->mesh->kind == MESH_UNITSQUARE, GRMP_ERR_UNSUPPORTED,
                "Dirichlet data: the mesh has no face adjacency");
     GRMP_CHECK(ctx, coef && ncoef >= 1 && ncoef <= BD_NCOEF, GRMP_ERR_ARG,
@@ -432,11 +532,12 @@ static BdMesh bd_mesh(const grmp_space *s)
     M.cellfaces = m->cellfaces;
     M.bface = m->bface;
     M.celldofs = s->celldofs;
+    M.fsign = m->fsign;
     M.nnodes = m->nnodes;
     M.ncells = m->ncells;
     M.N = m->N;
     M.fe = s->fe;
-    M.ncomp = s->ncomp;
+    M.ncomp = s->fe == GRMP_FE_H1BR ? 1 : s->ncomp;
     M.per = s->fe == GRMP_FE_H1P2 ? 6 : 3;
     return M;
 }
@@ -480,7 +581,7 @@ extern "C" int grmp_boundary_interpolate(const grmp_space *s, const double *coef
     const BdMesh M = bd_mesh(s);
     BdFaces F;
     int rc = bd_faces(ctx, M, F);
-    const int64_t nv = F.nbf * s->ncomp;
+    const int64_t nv = F.nbf * M.ncomp;
     if (!rc && nv) {
         hipLaunchKernelGGL(k_bd_interp, dim3(nblk(nv, 256)), dim3(256), 0, ctx->stream, M, gd, F.bf, nv, target->d,
                            offset);
@@ -508,7 +609,7 @@ extern "C" int grmp_boundary_bestapprox(const grmp_space *s, const double *coef,
         for (int k = 0; k < ncoef; ++k) S.g.c[m][k] = coef[m * ncoef + k];
     S.M = bd_mesh(s);
     // the mass matrix needs degree 2 p exactly, the load p + the data's degree
-    S.nq = std::max((quadorder + 2) / 2, s->fe == GRMP_FE_H1P2 ? 3 : 2);
+    S.nq = std::max((quadorder + 2) / 2, s->fe == GRMP_FE_H1P1 ? 2 : 3);
     const int64_t nd = s->ndofs;
     BdFaces F;
     int32_t *dflag = nullptr, *dpos = nullptr, *comp = nullptr, *bdof = nullptr, *cnt = nullptr, *ptr = nullptr,
@@ -518,8 +619,8 @@ extern "C" int grmp_boundary_bestapprox(const grmp_space *s, const double *coef,
     do {
         if ((rc = bd_faces(ctx, S.M, F))) break;
         if (F.nbf == 0) break;
-        const int per = s->fe == GRMP_FE_H1P2 ? 3 : 2;
-        const int64_t nv = F.nbf * s->ncomp;  // boundary faces of every component
+        const int per = s->fe == GRMP_FE_H1BR ? 5 : (s->fe == GRMP_FE_H1P2 ? 3 : 2);
+        const int64_t nv = F.nbf * S.M.ncomp;  // boundary face units
         if ((rc = dalloc(ctx, &dflag, nd)) || (rc = dalloc(ctx, &dpos, nd + 1)) || (rc = dalloc(ctx, &comp, nd)))
             break;
         GRMP_HIP(ctx, hipMemsetAsync(dflag, 0, sizeof(int32_t) * nd, ctx->stream));

@@ -5,7 +5,9 @@
 // (src/finiteelements.jl:72-112, src/dofmaps.jl:201-363).  Everything is
 // stored structure-of-arrays so that a thread per cell or per dof reads its
 // entries with coalesced loads.
-#include <hipcub/hipcub.hpp>
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -91,11 +93,28 @@ int exclusive_scan_i32(grmp_ctx *ctx, const int32_t *in, int32_t *out, int64_t n
 {
     // out[0] = 0, out[i+1] = sum in[0..i]
     size_t tmp = 0;
-    GRMP_HIP(ctx, hipcub::DeviceScan::InclusiveSum(nullptr, tmp, in, out + 1, (int)n, ctx->stream));
+    GRMP_HIP(ctx, rocprim::inclusive_scan(nullptr, tmp, in, out + 1, (size_t)n, rocprim::plus<int32_t>(),
+                                          ctx->stream));
     GRMP_TRY(ensure_scratch(ctx, tmp));
     GRMP_HIP(ctx, hipMemsetAsync(out, 0, sizeof(int32_t), ctx->stream));
-    GRMP_HIP(ctx, hipcub::DeviceScan::InclusiveSum(ctx->scratch, tmp, in, out + 1, (int)n, ctx->stream));
+    GRMP_HIP(ctx, rocprim::inclusive_scan(ctx->scratch, tmp, in, out + 1, (size_t)n, rocprim::plus<int32_t>(),
+                                          ctx->stream));
     return GRMP_OK;
+}
+
+int sum_i32_as_i64(grmp_ctx *ctx, const int32_t *in, int64_t n, int64_t *total)
+{
+    int64_t *d = nullptr;
+    GRMP_TRY(dalloc(ctx, &d, 1));
+    size_t tmp = 0;
+    hipError_t e = rocprim::reduce(nullptr, tmp, in, d, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), ctx->stream);
+    int rc = e == hipSuccess ? ensure_scratch(ctx, tmp) : GRMP_ERR_HIP;
+    if (!rc) e = rocprim::reduce(ctx->scratch, tmp, in, d, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), ctx->stream);
+    if (!rc && e == hipSuccess) e = hipMemcpyAsync(total, d, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream);
+    if (!rc && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dfree(ctx, d);
+    if (!rc && e != hipSuccess) rc = grmp_fail(ctx, GRMP_ERR_HIP, "64-bit sum: %s", hipGetErrorString(e));
+    return rc;
 }
 
 // ------------------------------------------------------------ Philox RNG
@@ -374,14 +393,14 @@ int mesh_build_faces(grmp_ctx *ctx, grmp_mesh *m)
         int bits = 1;
         while (bits < 64 && ((uint64_t)1 << bits) < (uint64_t)m->nnodes * (uint64_t)m->nnodes) ++bits;
         size_t tmp = 0;
-        if (hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, k0, k1, a0, a1, (int)M, 0, bits, ctx->stream) !=
+        if (rocprim::radix_sort_pairs(nullptr, tmp, k0, k1, a0, a1, (size_t)M, 0u, (unsigned)bits, ctx->stream) !=
             hipSuccess) {
             rc = grmp_fail(ctx, GRMP_ERR_HIP, "mesh faces: sort setup");
             break;
         }
         if ((rc = ensure_scratch(ctx, tmp))) break;
-        if (hipcub::DeviceRadixSort::SortPairs(ctx->scratch, tmp, k0, k1, a0, a1, (int)M, 0, bits, ctx->stream) !=
-            hipSuccess) {
+        if (rocprim::radix_sort_pairs(ctx->scratch, tmp, k0, k1, a0, a1, (size_t)M, 0u, (unsigned)bits,
+                                      ctx->stream) != hipSuccess) {
             rc = grmp_fail(ctx, GRMP_ERR_HIP, "mesh faces: sort");
             break;
         }
@@ -1000,15 +1019,14 @@ static int boundary_dofs_faces(grmp_ctx *ctx, const grmp_space *s, int64_t *coun
         hipLaunchKernelGGL(k_mark_bdofs, dim3(nblk(3 * nc, 256)), dim3(256), 0, ctx->stream, m->cells, m->cellfaces,
                            m->bface, nc, m->nnodes, s->fe, s->ncomp, coff, mark);
         // stream compaction keeps index order, so the list comes out sorted
-        hipcub::CountingInputIterator<int64_t> idx(1);  // 1-based dof numbers
+        rocprim::counting_iterator<int64_t> idx(1);  // 1-based dof numbers
         size_t tmp = 0;
-        if (hipcub::DeviceSelect::Flagged(nullptr, tmp, idx, mark, sel, nsel, (int)nd, ctx->stream) != hipSuccess) {
+        if (rocprim::select(nullptr, tmp, idx, mark, sel, nsel, (size_t)nd, ctx->stream) != hipSuccess) {
             rc = grmp_fail(ctx, GRMP_ERR_HIP, "boundary dofs: select setup");
             break;
         }
         if ((rc = ensure_scratch(ctx, tmp))) break;
-        if (hipcub::DeviceSelect::Flagged(ctx->scratch, tmp, idx, mark, sel, nsel, (int)nd, ctx->stream) !=
-            hipSuccess) {
+        if (rocprim::select(ctx->scratch, tmp, idx, mark, sel, nsel, (size_t)nd, ctx->stream) != hipSuccess) {
             rc = grmp_fail(ctx, GRMP_ERR_HIP, "boundary dofs: select");
             break;
         }

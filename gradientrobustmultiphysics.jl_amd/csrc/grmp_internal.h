@@ -174,6 +174,9 @@ struct grmp_matrix {
 // ------------------------------------------------------------- internals
 int build_incidence(grmp_ctx *ctx, grmp_space *s);
 int exclusive_scan_i32(grmp_ctx *ctx, const int32_t *in, int32_t *out, int64_t n);  // out[n+1]
+int sum_i32_as_i64(grmp_ctx *ctx, const int32_t *in, int64_t n, int64_t *total);   // host total, synchronizes
+// largest nonzero count of a CSR pattern with int32 row pointers and columns
+constexpr int64_t GRMP_MAX_NNZ = 2147483647;
 int form_resolve(grmp_matrix *A, FormRec &r);
 void fill_devform(const grmp_matrix *A, const FormRec &r, DevForm &d);
 

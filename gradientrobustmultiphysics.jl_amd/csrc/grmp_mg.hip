@@ -35,6 +35,24 @@
 
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// Start / stop events of a profiled launch come from a pool that collecting
+// refills, so hipEventCreate runs only while the pool grows; every creation
+// is checked, and a launch whose pair cannot be created runs unprofiled.
+static bool ev_take(std::vector<hipEvent_t> &pool, hipEvent_t &e0, hipEvent_t &e1)
+{
+    e0 = e1 = nullptr;
+    while (pool.size() < 2) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess || e == nullptr) return false;
+        pool.push_back(e);
+    }
+    e1 = pool.back();
+    pool.pop_back();
+    e0 = pool.back();
+    pool.pop_back();
+    return true;
+}
+
 #define MG_MAX_LEVELS 16
 #define MG_DENSE_MAX 64  // dofs of the coarsest level (P1 x 2 on 4 x 4: 50)
 
@@ -995,7 +1013,7 @@ struct MgJob {
     hipGraphExec_t gexec = nullptr;
     // profile: event pairs around every OP_RESID launch, with its algorithmic bytes
     bool profile = false;
-    std::vector<hipEvent_t> pev;
+    std::vector<hipEvent_t> pev, pool;  // profiled launches in flight; free events
     std::vector<double> pbytes;
     double prof_s = 0.0, prof_bytes = 0.0;
     int64_t prof_launches = 0;
@@ -1194,8 +1212,10 @@ static void launch_rows(MgJob *j, const MgRows &a, int64_t nnz = 0)
             return;
         }
         hipEvent_t e0, e1;
-        (void)hipEventCreate(&e0);
-        (void)hipEventCreate(&e1);
+        if (!ev_take(j->pool, e0, e1)) {
+            hipLaunchKernelGGL(k_mg_resid_fine, dim3(nblk(a.n, KB)), dim3(KB), 0, j->st, a);
+            return;
+        }
         hipExtLaunchKernelGGL(k_mg_resid_fine, dim3(nblk(a.n, KB)), dim3(KB), 0, j->st, e0, e1, 0, a);
         j->pev.push_back(e0);
         j->pev.push_back(e1);
@@ -1339,7 +1359,7 @@ int mg_job_iterate(MgJob *j, int iters)
         j->gexec = nullptr;
         j->graph = nullptr;
         mg_job_collect(j);  // profiled launches made before the capture
-        for (hipEvent_t e : j->pev) (void)hipEventDestroy(e);
+        for (hipEvent_t e : j->pev) j->pool.push_back(e);
         j->pev.clear();
         j->pbytes.clear();
         GRMP_HIP(j->ctx, hipStreamBeginCapture(j->st, hipStreamCaptureModeThreadLocal));
@@ -1451,7 +1471,7 @@ void mg_job_collect(MgJob *j)
         j->prof_launches += 1;
     }
     if (j->gexec) return;
-    for (hipEvent_t e : j->pev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : j->pev) j->pool.push_back(e);
     j->pev.clear();
     j->pbytes.clear();
 }
@@ -1469,6 +1489,7 @@ void mg_job_destroy(MgJob *j)
     if (j->gexec) (void)hipGraphExecDestroy(j->gexec);
     if (j->graph) (void)hipGraphDestroy(j->graph);
     for (hipEvent_t e : j->pev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : j->pool) (void)hipEventDestroy(e);
     if (j->ev) (void)hipEventDestroy(j->ev);
     if (j->buf) (void)hipFreeAsync(j->buf, j->st);
     delete j;
@@ -1492,7 +1513,7 @@ struct MgBatch {
     int rcap = 0;
     hipEvent_t ev = nullptr;
     // profile: event pairs around each batched residual launch, with its bytes
-    std::vector<hipEvent_t> pev;
+    std::vector<hipEvent_t> pev, pool;  // profiled launches in flight; free events
     std::vector<double> pbytes;
     double prof_s = 0.0, prof_bytes = 0.0;
     int64_t prof_launches = 0;
@@ -1613,10 +1634,8 @@ int mg_batch_run(MgBatch *b, MgJob *const *jobs, int n, int iters)
             if (lev[l].cnt == 0) continue;
             hipLaunchKernelGGL(k_mgb_first, dim3(ublk(lev[l].cnt)), dim3(KB), 0, st, tiles(lev[l]), D);
             hipLaunchKernelGGL(k_mgb_rows<OP_CHEB_PRE>, dim3(ublk(lev[l].cnt)), dim3(KB), 0, st, tiles(lev[l]), D);
-            if (b->profile) {
-                hipEvent_t e0, e1;
-                (void)hipEventCreate(&e0);
-                (void)hipEventCreate(&e1);
+            hipEvent_t e0, e1;
+            if (b->profile && ev_take(b->pool, e0, e1)) {
                 hipExtLaunchKernelGGL(k_mgb_rows<OP_RESID>, dim3(ublk(lev[l].cnt)), dim3(KB), 0, st, e0, e1, 0,
                                       tiles(lev[l]), D);
                 b->pev.push_back(e0);
@@ -1661,8 +1680,8 @@ int mg_batch_wait(MgBatch *b)
             b->prof_bytes += b->pbytes[k / 2];
             b->prof_launches += 1;
         }
-        (void)hipEventDestroy(b->pev[k]);
-        (void)hipEventDestroy(b->pev[k + 1]);
+        b->pool.push_back(b->pev[k]);
+        b->pool.push_back(b->pev[k + 1]);
     }
     b->pev.clear();
     b->pbytes.clear();
@@ -1691,6 +1710,7 @@ void mg_batch_destroy(MgBatch *b)
     if (b->hreq) (void)hipHostFree(b->hreq);
     if (b->hstat) (void)hipHostFree(b->hstat);
     for (hipEvent_t e : b->pev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : b->pool) (void)hipEventDestroy(e);
     if (b->ev) (void)hipEventDestroy(b->ev);
     delete b;
 }

@@ -305,16 +305,32 @@ int plain_cg_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solv
     const int64_t n = A->n;
     GRMP_CHECK(ctx, A->rowptr && A->cols && A->val64 && A->dinv64 && A->fixed && A->diagpos && bd && xd, GRMP_ERR_ARG,
                "plain CG: matrix not assembled in FP64");
-    double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *part = nullptr;
-    PlainScal *dsc = nullptr, *hsc = nullptr;
-    GRMP_TRY(dalloc(ctx, &r, n));
-    GRMP_TRY(dalloc(ctx, &z, n));
-    GRMP_TRY(dalloc(ctx, &p, n));
-    GRMP_TRY(dalloc(ctx, &q, n));
-    GRMP_TRY(dalloc(ctx, &part, 2 * PGRID_ROWS + 2));
-    GRMP_TRY(dalloc(ctx, &dsc, 1));
+    // every exit path releases the work vectors and the scalar records
+    struct Bufs {
+        grmp_ctx *ctx;
+        double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *part = nullptr;
+        PlainScal *dsc = nullptr, *hsc = nullptr;
+        ~Bufs()
+        {
+            if (hsc) (void)hipHostFree(hsc);
+            dfree(ctx, r);
+            dfree(ctx, z);
+            dfree(ctx, p);
+            dfree(ctx, q);
+            dfree(ctx, part);
+            dfree(ctx, dsc);
+        }
+    } B{ctx};
+    GRMP_TRY(dalloc(ctx, &B.r, n));
+    GRMP_TRY(dalloc(ctx, &B.z, n));
+    GRMP_TRY(dalloc(ctx, &B.p, n));
+    GRMP_TRY(dalloc(ctx, &B.q, n));
+    GRMP_TRY(dalloc(ctx, &B.part, 2 * PGRID_ROWS + 2));
+    GRMP_TRY(dalloc(ctx, &B.dsc, 1));
+    GRMP_HIP(ctx, hipHostMalloc((void **)&B.hsc, sizeof(PlainScal), hipHostMallocDefault));
+    double *r = B.r, *z = B.z, *p = B.p, *q = B.q, *part = B.part;
+    PlainScal *dsc = B.dsc, *hsc = B.hsc;
     double *fold = part + 2 * PGRID_ROWS;
-    GRMP_HIP(ctx, hipHostMalloc((void **)&hsc, sizeof(PlainScal), hipHostMallocDefault));
     double *hpart = &hsc->alpha;  // two doubles for the first fold
     const int grid = (int)std::min<int64_t>(PGRID_ROWS, std::max<int64_t>(1, (n + PB - 1) / PB));
     const int gride = (int)std::min<int64_t>(PGRID, grid);
@@ -374,13 +390,6 @@ int plain_cg_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solv
     info.outer = 1;
     info.relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
     info.status = rc ? rc : status;
-    (void)hipHostFree(hsc);
-    dfree(ctx, r);
-    dfree(ctx, z);
-    dfree(ctx, p);
-    dfree(ctx, q);
-    dfree(ctx, part);
-    dfree(ctx, dsc);
     return rc;
 }
 

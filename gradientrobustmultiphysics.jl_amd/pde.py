@@ -233,6 +233,61 @@ class PolynomialData:
         return sum(ck * x ** i * y ** j for ck, (i, j) in zip(row, self.MONOMIALS))
 
 
+class L2ErrorIntegrator:
+    """L2ErrorIntegrator(compare_data, Identity; quadorder = "auto", factor = 1)
+    (src/assemblypatterns/itemintegrator.jl:40-89): integrates
+    |compare_data(x) - factor u_h(x)|^2 over the cells; ``evaluate`` returns
+    the sum (the squared L2 error), ``evaluate_cells`` the per-cell values of
+    evaluate!.  compare_data is a PolynomialData (a Julia closure cannot run
+    on the device).  The quadrature order is the element's polynomial order
+    plus quadorder, "auto" meaning twice the data's bonus_quadorder
+    (src/assemblypatterns.jl:559-563)."""
+
+    def __init__(self, compare_data, operator=Identity, quadorder="auto", factor=1.0, name="auto"):
+        if operator != Identity:
+            raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, f"L2ErrorIntegrator: operator {operator} is not built")
+        self.data = compare_data
+        self.ncomp = compare_data.ncomp if compare_data is not None else None
+        self.quadorder = 2 * compare_data.bonus_quadorder if quadorder == "auto" else int(quadorder)
+        self.factor = float(factor)
+        self.name = f"L2 error" if name == "auto" else name
+
+    def _run(self, block, cells):
+        fes = block.FES
+        if self.ncomp is not None and self.ncomp != fes.FEType.ncomp:
+            raise _abi.GrmpError(_abi.ERR_ARG, f"{self.name}: data with {self.ncomp} components for a space with "
+                                 f"{fes.FEType.ncomp}")
+        coef = self.data.c if self.data is not None else np.zeros(1)
+        ncoef = self.data.ncoef if self.data is not None else 0
+        out = np.empty(fes.xgrid.ncells) if cells else None
+        tot = ctypes.c_double()
+        block.parent.ctx.check(lib().grmp_integrate_l2error(
+            block.parent.ctx.h, fes.h, block.parent.vec.h, block.offset, _abi.dptr(coef), ncoef,
+            fes.FEType.order + self.quadorder, self.factor, _abi.dptr(out) if cells else None, ctypes.byref(tot)))
+        return out if cells else tot.value
+
+
+def L2NormIntegrator(ncomponents, operator=Identity, quadorder=2, name="L2 norm"):
+    """L2NormIntegrator(ncomponents, Identity; quadorder = 2)
+    (src/assemblypatterns/itemintegrator.jl:96-128): the squared L2 norm,
+    the L2 error against zero data."""
+    I = L2ErrorIntegrator(None, operator, quadorder=quadorder, name=name)
+    I.ncomp = int(ncomponents)
+    return I
+
+
+def evaluate(integrator, block):
+    """evaluate(ItemIntegrator, FEVectorBlock) (itemintegrator.jl:323-360):
+    the sum over all cells."""
+    return integrator._run(block, cells=False)
+
+
+def evaluate_cells(integrator, block):
+    """evaluate!(b, ItemIntegrator, FEVectorBlock) (itemintegrator.jl:169-307):
+    one value per cell."""
+    return integrator._run(block, cells=True)
+
+
 class FixedIntegralMean:
     """FixedIntegralMean(unknown, value) (src/globalconstraints.jl:23-44):
     the unknown's first dof is fixed during the solve (apply, :259-283) and

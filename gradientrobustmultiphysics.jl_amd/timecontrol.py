@@ -1,4 +1,4 @@
-"""TimeControlSolver with the BackwardEuler rule on the device.
+"""TimeControlSolver with the BackwardEuler and CrankNicolson rules on the device.
 
 Python mirror of src/solvers_timedependent.jl for linear problems (one
 subiteration holding every unknown, time-independent data):
@@ -13,8 +13,15 @@ subiteration holding every unknown, time-independent data):
   (:349-359, :452-461) in one device kernel (grmp_timestep_rhs), solves
   S u^{n+1} = b with the device Krylov solver and realizes global
   constraints (:578-581).
+* CrankNicolson (:255-258, :376-433): S = A + (2/dt) M is kept beside the
+  spatial matrix A; every step builds rhs = 2 f - A u^n + (2/dt) M u^n
+  (time-independent f: b^n = b^{n+1} = f) with the penalty rows, as two
+  passes of the step kernel.  Every unknown must be time-dependent (the
+  reference's weighting of algebraic constraints, :384-394, is not built).
 * advance_until_stationarity! (:605-665): steps until the summed per-unknown
   change norms drop below the threshold.
+* advance_until_time! (:683-760): steps of the given size while the time is
+  below finaltime - finaltime_tolerance; warns when the end time is missed.
 
 Each step's system is solved by the served path's multigrid V-cycle CG
 (FP64, grmp_mg_create / grmp_mg_apply) when the problem is one scalar H1P1
@@ -24,9 +31,11 @@ rediscretize A + M/dt on squares of ceil(N/2) down to N <= 4.  Otherwise,
 or with solver="krylov", the device Krylov solver of solve_system runs.
 
 Dirichlet data (homogeneous, interpolated or best-approximated) is fixed in
-time.  CrankNicolson, nonlinear operators, dt_action / dt_lump and
-time-dependent data are not built and raise GrmpError(ERR_UNSUPPORTED).
+time.  Nonlinear operators, dt_action / dt_lump and time-dependent data are
+not built and raise GrmpError(ERR_UNSUPPORTED).
 """
+import warnings
+
 import numpy as np
 
 from . import _abi
@@ -41,7 +50,8 @@ class BackwardEuler:
 
 
 class CrankNicolson:
-    """Marker of the Crank-Nicolson rule (src/solvers_timedependent.jl:9); not built."""
+    """(u^{n+1} - u^n)/dt + (A u^{n+1} + A u^n)/2 = (f^{n+1} + f^n)/2
+    (src/solvers_timedependent.jl:9), scaled by 2 as the reference does."""
 
 
 class TimeControlSolver:
@@ -52,14 +62,18 @@ class TimeControlSolver:
 
     def __init__(self, pde, InitialValues, rule=BackwardEuler, timedependent_equations=(), config=None,
                  solver="auto", **kwargs):
-        if rule is not BackwardEuler:
+        if rule is not BackwardEuler and rule is not CrankNicolson:
             raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, f"time integration rule {getattr(rule, '__name__', rule)} "
-                                 "is not built (BackwardEuler is)")
+                                 "is not built (BackwardEuler and CrankNicolson are)")
         which = sorted(set(int(d) for d in timedependent_equations))
         nu = len(InitialValues.FES)
         if not which or which[0] < 0 or which[-1] >= nu:
             raise _abi.GrmpError(_abi.ERR_ARG, "timedependent_equations must name at least one unknown "
                                  f"in 0..{nu - 1}")
+        if rule is CrankNicolson and len(which) != nu:
+            raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "CrankNicolson needs every unknown time-dependent "
+                                 "(algebraic constraints are not built)")
+        self.rule = rule
         self.PDE, self.X, self.which = pde, InitialValues, which
         self.SC = config or SolverConfig(**kwargs)
         self.values = _abi.VALUES_FP64 | (_abi.VALUES_FP32 if self.SC.precision == "fp32_ir" else 0)
@@ -90,13 +104,28 @@ class TimeControlSolver:
             for op in ops:
                 assemble_linearform(self.f, j, op)
         self.b = FEVector(FES, name="system rhs")
+        if rule is CrankNicolson:
+            # the spatial matrix A (no penalty: its fixed rows are overwritten) and 2 f, kept for every step
+            self.A = FEMatrix(FES, name="spatial matrix")
+            for (i, j), ops in sorted(pde.LHS.items()):
+                for op in ops:
+                    form, keep = op.form(FES[i], i, j, self.A.ctx, FES[j])
+                    self.A.add_form(form, keep)
+            self.A.build_pattern()
+            self.A.assemble(_abi.VALUES_FP64)
+            self.f2 = FEVector(FES, name="2 f")
+            self.f2.vec.fill(0.0)
+            for _ in range(2):
+                self.f2.ctx.check(lib().grmp_vector_axpy(1.0, self.f.vec.h, self.f2.vec.h))
+            self._tmp = FEVector(FES, name="2 f - A u")
         # step solver: the multigrid V-cycle CG where the served hierarchy applies, else the Krylov solver
         if solver not in ("auto", "krylov", "mg"):
             raise _abi.GrmpError(_abi.ERR_ARG, f"solver {solver!r}: 'auto', 'krylov' or 'mg'")
         mg_ok = self._mg_eligible()
         if solver == "mg" and not mg_ok:
             raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "multigrid steps need one scalar H1P1 unknown on a generated "
-                                 "unit square, homogeneous Dirichlet data and no global constraints")
+                                 "unit square with Dirichlet data on its whole boundary (inhomogeneous values are "
+                                 "lifted) and no global constraints")
         self.solver = "mg" if (solver == "mg" or (solver == "auto" and mg_ok)) else "krylov"
         self.levels = None
         self.S = None
@@ -122,8 +151,13 @@ class TimeControlSolver:
         self.X.ctx.check(lib().grmp_vector_axpy(1.0, self.target.h, self.X.vec.h))
         return info
 
+    def _mfactor(self, timestep):
+        """the mass matrix's factor in S: 1/dt (BackwardEuler), 2/dt (CrankNicolson)"""
+        return (2.0 if self.rule is CrankNicolson else 1.0) / timestep
+
     def _system(self, timestep):
-        """S = A + (1/dt) M with the fixed dofs penalized (:337-351)."""
+        """S = A + (1/dt) M (BackwardEuler, :337-351) or A + (2/dt) M
+        (CrankNicolson, :408-413) with the fixed dofs penalized."""
         FES = self.X.FES
         S = FEMatrix(FES, name="system matrix")
         for (i, j), ops in sorted(self.PDE.LHS.items()):
@@ -131,7 +165,7 @@ class TimeControlSolver:
                 form, keep = op.form(FES[i], i, j, S.ctx, FES[j])
                 S.add_form(form, keep)
         for d in self.which:
-            form, keep = ReactionOperator(1.0 / timestep).form(FES[d], d, d, S.ctx)
+            form, keep = ReactionOperator(self._mfactor(timestep)).form(FES[d], d, d, S.ctx)
             S.add_form(form, keep)
         S.set_fixed(self.fixed, self.SC.fixed_penalty)
         S.build_pattern()
@@ -140,8 +174,14 @@ class TimeControlSolver:
         return S
 
     def rhs(self, timestep):
-        """b = f + M u^n / dt, b_d = penalty * u^n_d on fixed dofs (on the device)."""
+        """BackwardEuler: b = f + M u^n / dt; CrankNicolson: b = 2 f - A u^n +
+        (2/dt) M u^n; b_d = penalty * u^n_d on fixed dofs (on the device)."""
         X = self.X
+        if self.rule is CrankNicolson:
+            X.ctx.check(lib().grmp_timestep_rhs(self.S.h, self.A.h, self.f2.vec.h, X.vec.h, -1.0, self._tmp.vec.h))
+            X.ctx.check(lib().grmp_timestep_rhs(self.S.h, self.AM.h, self._tmp.vec.h, X.vec.h, 2.0 / timestep,
+                                                self.b.vec.h))
+            return self.b
         X.ctx.check(lib().grmp_timestep_rhs(self.S.h, self.AM.h, self.f.vec.h, X.vec.h, 1.0 / timestep,
                                             self.b.vec.h))
         return self.b
@@ -159,7 +199,7 @@ class TimeControlSolver:
             self.S = self._system(timestep)
             if self.solver == "mg":  # assembled once per step size, reused by every step
                 self.levels = MultigridSolver(*mg_hierarchy(self.PDE, self.X.FES[0], self.S,
-                                                            [ReactionOperator(1.0 / timestep)],
+                                                            [ReactionOperator(self._mfactor(timestep))],
                                                             self.SC.fixed_penalty))
         self.rhs(timestep)
         if self.solver == "mg":
@@ -188,3 +228,19 @@ class TimeControlSolver:
             if change < stationarity_threshold:
                 return it
         return maxtimesteps
+
+    def advance_until_time(self, timestep, finaltime, finaltime_tolerance=1e-15, do_after_each_timestep=None):
+        """advance_until_time!(TCS, timestep, finaltime; finaltime_tolerance,
+        do_after_each_timestep) (src/solvers_timedependent.jl:683-760): steps
+        of size ``timestep`` while the time is below finaltime minus the
+        tolerance; warns when the final time is not met within it.  Returns
+        the number of steps taken."""
+        steps = 0
+        while self.ctime < finaltime - finaltime_tolerance:
+            self.advance(timestep)
+            steps += 1
+            if do_after_each_timestep is not None:
+                do_after_each_timestep(self)
+        if abs(self.ctime - finaltime) > finaltime_tolerance:
+            warnings.warn("final time not reached within tolerance! (consider another timestep)")
+        return steps

@@ -249,6 +249,20 @@ int grmp_assemble_linearform(grmp_ctx *ctx, const grmp_space *space, const grmp_
 int grmp_integrate_identity(grmp_ctx *ctx, const grmp_space *space, int quadorder,
                             const grmp_vector *u, int64_t offset, double *out);
 
+/* evaluate(L2ErrorIntegrator(g, Identity; quadorder, factor), u) and, with
+ * ncoef = 0, evaluate(L2NormIntegrator(ncomp, Identity), u)
+ * (src/assemblypatterns/itemintegrator.jl:40-95, evaluate! :169-307,
+ * evaluate :323-360).  Per cell sum_q w_q |T| sum_k (g_k(x_q) - factor
+ * u_k(x_q))^2 with g_k the polynomial coef[k*ncoef .. k*ncoef+ncoef) over the
+ * monomials 1, x, y, x^2, xy, y^2, x^3, x^2 y, x y^2, y^3.  cellvals
+ * ([ncells], optional) gets evaluate!'s per-item values, *total (optional)
+ * their sum, the squared L2 error.  H1P1/H1P2 (1-2 components), H1BR{2},
+ * L2P0 on triangles; quadorder as the reference's triangle rules (<= 2
+ * closed form, else Stroud). */
+int grmp_integrate_l2error(grmp_ctx *ctx, const grmp_space *space, const grmp_vector *u, int64_t offset,
+                           const double *coef, int ncoef, int quadorder, double factor, double *cellvals,
+                           double *total);
+
 /* ------------------------------------------------------ Dirichlet data */
 /* Boundary values of an H1P1 / H1P2 space with 1 or 2 components (triangle
  * meshes) for the polynomial data g_c = sum_k coef[c ncoef + k] m_k,

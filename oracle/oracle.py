@@ -222,6 +222,24 @@ def direct_solve(A, b):
     return spla.spsolve(A.to_scipy().tocsc(), b)
 
 
+
+def direct_solve_fixed(A, b, fixed, values):
+    """The penalized solve with the fixed dofs eliminated: x_d = values_d,
+    A_ff x_f = b_f - A_fd values (sparse LU).  Equal to the penalized
+    system's solution up to O(1/penalty) in exact arithmetic; used where the
+    1e60 rows with inhomogeneous values leave SuperLU's pivoting without
+    accuracy in the saddle point's pressure."""
+    import scipy.sparse.linalg as spla
+    M = A.to_scipy().tocsr()
+    n = M.shape[0]
+    fixed = np.asarray(fixed, dtype=np.int64)
+    free = np.setdiff1d(np.arange(n), fixed)
+    x = np.zeros(n)
+    x[fixed] = values
+    r = b[free] - M[free][:, fixed] @ x[fixed]
+    x[free] = spla.spsolve(M[free][:, free].tocsc(), r)
+    return x
+
 # ------------------------------------------------------- Dirichlet data
 
 def poly_eval(c, x, y):
@@ -471,3 +489,212 @@ class StokesSpace:
         y = x.copy()
         y[self.nU:] -= mean
         return y
+
+
+# ------------------------------------------------------ L2ErrorIntegrator
+
+def tri_rule(order):
+    """The reference's triangle rule (src/quadrature.jl:173-195): midpoint,
+    face midpoints, else the Stroud rule; points (nq, 2), weights (sum 1)."""
+    if order <= 1:
+        return np.array([[1.0 / 3.0, 1.0 / 3.0]]), np.array([1.0])
+    if order == 2:
+        return np.array([[0.5, 0.5], [0.0, 0.5], [0.5, 0.0]]), np.full(3, 1.0 / 3.0)
+    return quad_stroud(order)
+
+
+def _shape_vals(fe, x0, x1):
+    """scalar shape functions at one reference point, op order of the
+    reference's get_basis (P1 h1_p1.jl:64-75, P2 h1_p2.jl:134-145, BR
+    h1v_br.jl:117-130: P1 then 6 x l, 6 y x, 6 l y)"""
+    l = 1.0
+    l -= x0
+    l -= x1
+    if fe == FE_P1:
+        return [l, x0, x1]
+    if fe == FE_P2:
+        l2 = 1.0 - x0 - x1
+        return [2.0 * l2 * (l2 - 0.5), 2.0 * x0 * (x0 - 0.5), 2.0 * x1 * (x1 - 0.5), 4.0 * l2 * x0, 4.0 * x0 * x1,
+                4.0 * x1 * l2]
+    return [l, x0, x1, 6.0 * x0 * l, 6.0 * x1 * x0, 6.0 * l * x1]
+
+
+def l2error_cells(fe, ncomp, u, coef, quadorder, factor, coords, cells, celldofs, normals=None):
+    """evaluate!(b, L2ErrorIntegrator(g, Identity; quadorder, factor), u)
+    (src/assemblypatterns/itemintegrator.jl:40-89, :243-307) per cell:
+    sum_q (sum_k (g_k(x_q) - factor u_k(x_q))^2) w_q vol, with
+    u_k(x_q) = sum_dofs u_dof cvals[k, dof, q] in dof order
+    (src/feevaluator.jl:445-452) and x_q = x_1 + A xref_q.  coef: (ncomp,
+    ncoef) polynomial coefficients (None: zero data).  normals: (ncells, 3,
+    2) face normals of H1BR{2}."""
+    nc = cells.shape[0]
+    X = coords[cells]
+    X0, Y0 = X[:, 0, 0], X[:, 0, 1]
+    A11, A12 = X[:, 1, 0] - X0, X[:, 2, 0] - X0
+    A21, A22 = X[:, 1, 1] - Y0, X[:, 2, 1] - Y0
+    vol = np.abs(A11 * A22 - A12 * A21) * 0.5
+    xq, wq = tri_rule(quadorder)
+    c = np.zeros((ncomp, 10)) if coef is None else np.atleast_2d(np.asarray(coef, dtype=np.float64))
+    acc = np.zeros(nc)
+    for (x0, x1), w in zip(xq, wq):
+        sv = _shape_vals(fe, x0, x1)
+        inp = [np.zeros(nc) for _ in range(ncomp)]
+        if fe == FE_P0:
+            inp[0] += u[celldofs[:, 0]] * 1.0
+        elif fe == FE_BR:
+            for dof in range(9):
+                ud = u[celldofs[:, dof]]
+                for k in range(2):
+                    if dof < 6:
+                        cv = sv[dof % 3] * 1.0 if dof // 3 == k else 0.0
+                    else:
+                        cv = sv[dof - 3] * normals[:, dof - 6, k]
+                    inp[k] += ud * cv * 1.0
+        else:
+            per = 6 if fe == FE_P2 else 3
+            for cc in range(ncomp):
+                for s in range(per):
+                    ud = u[celldofs[:, cc * per + s]]
+                    for k in range(ncomp):
+                        inp[k] += ud * (sv[s] if cc == k else 0.0) * 1.0
+        xp = np.zeros(nc)
+        xp += A11 * x0
+        xp += A12 * x1
+        xp += X0
+        yp = np.zeros(nc)
+        yp += A21 * x0
+        yp += A22 * x1
+        yp += Y0
+        r = np.zeros(nc)
+        for k in range(ncomp):
+            val = _poly_ordered(c[k], xp, yp)
+            val -= inp[k] * factor
+            r += val * val
+        acc += r * w * vol
+    return acc
+
+
+def _poly_ordered(c, x, y):
+    """g(x, y) summed term by term in monomial order (as the device does)"""
+    c = list(c) + [0.0] * (10 - len(c))
+    s = np.zeros_like(x)
+    s += c[0]
+    s += c[1] * x
+    s += c[2] * y
+    s += c[3] * (x * x)
+    s += c[4] * (x * y)
+    s += c[5] * (y * y)
+    s += c[6] * (x * x * x)
+    s += c[7] * (x * x * y)
+    s += c[8] * (x * y * y)
+    s += c[9] * (y * y * y)
+    return s
+
+
+# ------------------------------------------ H1BR{2} Dirichlet data
+
+def _br_face_units(S):
+    """boundary faces of a StokesSpace in cell order: (cell, local face,
+    5 dofs [a_x, b_x, a_y, b_y, bubble], face normal)"""
+    out = []
+    nrm = S.fnormal.reshape(-1, 2)
+    for c, k in boundary_faces(S.cells, S.cellfaces, S.facecells):
+        d = S.celldofs[c]
+        dofs = [d[k], d[(k + 1) % 3], d[3 + k], d[3 + (k + 1) % 3], d[6 + k]]
+        out.append((c, k, dofs, nrm[S.cellfaces[c, k]]))
+    return out
+
+
+def _br_trace(s, n):
+    """vector values of the five face dofs of H1BR{2} at s
+    (src/fedefs/h1v_br.jl:104-113: P1{2} on the edge, then 6 s (1-s) n)"""
+    b = 6.0 * s * (1.0 - s)
+    return np.array([[1.0 - s, 0.0], [s, 0.0], [0.0, 1.0 - s], [0.0, s], [b * n[0], b * n[1]]])
+
+
+def boundary_bestapprox_br(S, coef, quadorder):
+    """BestapproxDirichletBoundary on H1BR{2} (src/boundarydata.jl:259-352):
+    the boundary mass matrix of the vector-valued trace basis and the load
+    of g = (g_x, g_y) on the boundary faces, solved by sparse LU.  Returns
+    (target with NaN off the boundary, boundary dofs)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    coef = np.atleast_2d(coef)
+    nq = max((quadorder + 2) // 2, 3)
+    xg, wg = np.polynomial.legendre.leggauss(nq)
+    sq, wq = 0.5 * (xg + 1.0), 0.5 * wg
+    units = _br_face_units(S)
+    bd = sorted({d for _, _, dofs, _ in units for d in dofs})
+    comp = {d: r for r, d in enumerate(bd)}
+    rows, cols, vals = [], [], []
+    load = np.zeros(len(bd))
+    for c, k, dofs, n in units:
+        a, b = S.cells[c, k], S.cells[c, (k + 1) % 3]
+        p0, p1 = S.coords[a], S.coords[b]
+        L = float(np.hypot(*(p1 - p0)))
+        for s, w in zip(sq, wq):
+            V = _br_trace(s, n)
+            x, y = p0 + s * (p1 - p0)
+            g = np.array([poly_eval(coef[0], x, y), poly_eval(coef[1], x, y)])
+            for i, di in enumerate(dofs):
+                load[comp[di]] += L * w * float(V[i] @ g)
+                for j, dj in enumerate(dofs):
+                    rows.append(comp[di])
+                    cols.append(comp[dj])
+                    vals.append(L * w * float(V[i] @ V[j]))
+    Mb = sp.csc_matrix((vals, (rows, cols)), shape=(len(bd), len(bd)))
+    t = np.full(S.n, np.nan)
+    t[bd] = spla.spsolve(Mb, load)
+    return t, np.array(bd, dtype=np.int64)
+
+
+def boundary_interpolate_br(S, coef):
+    """InterpolateDirichletBoundary on H1BR{2} (src/fedefs/h1v_br.jl:37-81):
+    node values of g, and the bubble that keeps the normal flux of g through
+    each boundary face: sum_c (int_F g_c - int_F I g_c) n_c / |F|."""
+    coef = np.atleast_2d(coef)
+    xg, wg = np.polynomial.legendre.leggauss(6)
+    sq, wq = 0.5 * (xg + 1.0), 0.5 * wg
+    t = np.full(S.n, np.nan)
+    for c, k, dofs, n in _br_face_units(S):
+        a, b = S.cells[c, k], S.cells[c, (k + 1) % 3]
+        p0, p1 = S.coords[a], S.coords[b]
+        L = float(np.hypot(*(p1 - p0)))
+        g0 = [poly_eval(coef[m], *p0) for m in range(2)]
+        g1 = [poly_eval(coef[m], *p1) for m in range(2)]
+        t[dofs[0]], t[dofs[1]], t[dofs[2]], t[dofs[3]] = g0[0], g1[0], g0[1], g1[1]
+        val = 0.0
+        for m in range(2):
+            mean = L * sum(w * poly_eval(coef[m], *(p0 + s * (p1 - p0))) for s, w in zip(sq, wq))
+            val += (mean - 0.5 * L * (g0[m] + g1[m])) * n[m]
+        t[dofs[4]] = val / L
+    return t
+
+
+def crank_nicolson_rhs(A, M, f, u, dt, fixed, penalty=PENALTY):
+    """Right-hand side of one CrankNicolson step of advance! for
+    time-independent data (src/solvers_timedependent.jl:376-433):
+    rhs = b^n - A u^n + b^{n+1} + (2/dt) AM u^n with b^n = b^{n+1} = f;
+    penalty rows rhs_d = fixed_penalty * x_d (:452-461)."""
+    Ms, As = M.to_scipy().tocsr(), A.to_scipy().tocsr()
+    b = f - As @ u + f + (Ms @ u) * (2.0 / dt)
+    b[fixed] = penalty * u[fixed]
+    return b
+
+
+def crank_nicolson(A, M, f, u0, dts, fixed, penalty=PENALTY):
+    """advance! with CrankNicolson for a linear problem without algebraic
+    constraints: S = A + (2/dt) AM (:408-413), refactored when the step
+    size changes; returns the states after each step (sparse LU)."""
+    import scipy.sparse.linalg as spla
+    u, out, lu, last = np.array(u0, dtype=np.float64), [], None, None
+    for dt in dts:
+        if lu is None or dt != last:
+            S = (A.to_scipy() + M.to_scipy() * (2.0 / dt)).tolil()
+            for d in np.asarray(fixed).tolist():
+                S[d, d] = penalty
+            lu = spla.splu(S.tocsc())
+            last = dt
+        u = lu.solve(crank_nicolson_rhs(A, M, f, u, dt, fixed, penalty))
+        out.append(u.copy())
+    return out

@@ -186,3 +186,37 @@ def test_dirichlet_data_errors(grmp):
     pde.add_boundarydata(0, grmp.BestapproxDirichletBoundary, data=grmp.PolynomialData([[1.0], [2.0]]))
     with pytest.raises(grmp.GrmpError):  # two-component data on a scalar space
         grmp.boundary_values(pde, grmp.FEVector([fes]))
+
+
+@pytest.mark.parametrize("N,seed", [(6, None), (7, 5)])
+def test_br_boundary_values_match_oracle(grmp, O, N, seed):
+    """H1BR{2}: best approximation (5 dofs per face, components coupled by the
+    normal bubble) within 1e-12 of the oracle's sparse-LU solve;
+    interpolation (node values, flux-keeping bubble) within 1e-14."""
+    print(f"[case] BR boundary values N={N} seed={seed}", flush=True)
+    grid, coords, cells = _grid(grmp, O, N, seed)
+    fes = grmp.FESpace(grmp.H1BR(2), grid)
+    S = O.StokesSpace(N, coords, cells) if seed is not None else O.StokesSpace(N)
+    assert np.array_equal(S.celldofs, fes.celldofs())
+    g2 = np.array([CUBIC, [0.1, 0.4, -0.3, 0.2, -0.5, 0.6, -0.7, 0.3, 0.2, -0.1]])
+    data = grmp.PolynomialData(g2)
+    q = 2 + data.bonus_quadorder
+    ref, bd = O.boundary_bestapprox_br(S, g2, q)
+    refi = O.boundary_interpolate_br(S, g2)
+    assert np.array_equal(bd, fes.boundary_dofs())
+    nd = fes.ndofs
+    for kind in ("bestapprox", "interpolate"):
+        t = grmp.DeviceVector(grid.ctx, nd, np.full(nd, 7.0))
+        if kind == "bestapprox":
+            it, rr = ctypes.c_int64(), ctypes.c_double()
+            rc = grmp.lib().grmp_boundary_bestapprox(fes.h, grmp._abi.dptr(data.c), data.ncoef, q, 1e-14, t.h, 0,
+                                                     ctypes.byref(it), ctypes.byref(rr))
+        else:
+            rc = grmp.lib().grmp_boundary_interpolate(fes.h, grmp._abi.dptr(data.c), data.ncoef, t.h, 0)
+        grid.ctx.check(rc)
+        v = t.download()
+        off = np.setdiff1d(np.arange(nd), bd)
+        assert np.all(v[off] == 7.0)
+        r = ref if kind == "bestapprox" else refi
+        tol = 1e-12 if kind == "bestapprox" else 1e-14
+        assert np.max(np.abs(v[bd] - r[bd])) <= tol * np.max(np.abs(r[bd])), kind

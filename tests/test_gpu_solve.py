@@ -156,6 +156,29 @@ def test_nonconvergence_reported(grmp, O):
     with pytest.raises(grmp.GrmpError) as ei:
         grmp.solve(sol, pde, precision="fp64", rtol=1e-30, maxiterations=5)
     assert ei.value.code == grmp._abi.ERR_NOCONV
+    # the device stops after exactly maxit iterations (its scalars and the
+    # stop test live on the device, the host reads them every 8 iterations)
+    A, b, fixed = grmp.assemble(pde, sol)
+    for maxit in (5, 8, 13):
+        info = grmp.solve_system(A, b, sol, grmp.SolverConfig(precision="fp64", rtol=1e-30, maxiterations=maxit))
+        assert info.status == grmp._abi.ERR_NOCONV and info.iterations == maxit, (maxit, info.iterations)
+
+
+def test_plain_cg_iterations_pinned(grmp, O):
+    """The device plain FP64 CG takes the iterations of the oracle's host
+    Jacobi-PCG on the same penalized system (same recurrences; the dot
+    products are summed in other orders, so the count may differ by one) and
+    stops within the tolerance."""
+    grid, fes, pde, xref = _poisson(grmp, O, 32, 1, 3)
+    sol = grmp.FEVector([fes])
+    info, A, b = grmp.solve(sol, pde, precision="fp64", rtol=1e-10)
+    S = O.Space(32, O.FE_P1)
+    colptr, rowval, nzval = A.csc()
+    Ao = O.CSC(S.ndofs, colptr.astype(np.int64), rowval.astype(np.int32), nzval)
+    x, it, rr = O.pcg(Ao, b.entries(), S.fixed, rtol=1e-10)
+    assert info.status == 0 and info.relres <= 1e-10 and rr <= 1e-10
+    assert abs(info.iterations - it) <= 1, (info.iterations, it)
+    assert np.linalg.norm(sol.entries() - x) / np.linalg.norm(x) < 1e-8
 
 
 @pytest.mark.parametrize("precision", [1, 2])

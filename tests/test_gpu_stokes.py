@@ -142,3 +142,31 @@ def test_stokes_csc_plugin_picks_minres(grmp, O):
     ctx.check(rc)
     err = np.linalg.norm(x - xref) / np.linalg.norm(xref)
     assert err <= 1e-8, err
+
+
+@pytest.mark.parametrize("seed", [None, 8])
+def test_stokes_exactness_br_p0(grmp, O, seed):
+    """The reference's Stokes test for H1BR{2} x L2P0, orders (1, 0)
+    (test/runtests.jl:521-540,581,606-640): u = (y + 1, x - 1), p = 0,
+    f = 0, viscosity 1, BestapproxDirichletBoundary with u on all regions,
+    FixedIntegralMean; L2 errors of velocity (quadorder 1) and pressure
+    (quadorder 0) below the reference's tolerance 6e-12 after MINRES."""
+    from meshes import shuffled_square
+    print(f"[case] stokes exactness seed={seed}", flush=True)
+    if seed is None:
+        grid = grmp.grid_unitsquare(8)
+    else:
+        coords, cells = shuffled_square(O, 8, seed)
+        grid = grmp.ExtendableGrid.from_arrays(coords, cells)
+    V = grmp.FESpace(grmp.H1BR(2), grid)
+    Q = grmp.FESpace(grmp.L2P0(1), grid)
+    pde = grmp.IncompressibleNavierStokesProblem(2, viscosity=1.0)
+    lin = grmp.PolynomialData(np.array([[1.0, 0.0, 1.0], [-1.0, 1.0, 0.0]]), bonus_quadorder=1)
+    pde.add_boundarydata(0, grmp.BestapproxDirichletBoundary, data=lin)
+    sol = grmp.FEVector([V, Q])   # f = 0: no right-hand side operator
+    info, A, b = grmp.solve(sol, pde, precision="fp64", rtol=1e-14)
+    assert info.status == 0, info.status
+    eV = np.sqrt(grmp.evaluate(grmp.L2ErrorIntegrator(lin, quadorder=1), sol[0]))
+    eP = np.sqrt(grmp.evaluate(grmp.L2ErrorIntegrator(grmp.PolynomialData([0.0]), quadorder=0), sol[1]))
+    print(f"  MINRES {info.iterations} it, errorV {eV:.2e}, errorP {eP:.2e}")
+    assert max(eV, eP) < 6e-12

@@ -15,7 +15,10 @@ Bar:
 * both step solvers: the Krylov solver, and the multigrid V-cycle CG on the
   rediscretized unit-square hierarchy (odd N included);
 * inhomogeneous data: boundary values kept to 1e-13 at every step, and the
-  stationary limit equal to the harmonic (linear) data to 1e-9."""
+  stationary limit equal to the harmonic (linear) data to 1e-9;
+* CrankNicolson (src/solvers_timedependent.jl:376-433): every state within
+  1e-9 of the oracle's CrankNicolson steps, both step solvers, and
+  advance_until_time! stopping at the final time."""
 import numpy as np
 import pytest
 
@@ -168,7 +171,7 @@ def test_argument_errors(grmp):
     fes, pde = _heat(grmp, grid)
     X = grmp.FEVector([fes])
     with pytest.raises(grmp.GrmpError):
-        grmp.TimeControlSolver(pde, X, grmp.CrankNicolson, timedependent_equations=[0])
+        grmp.TimeControlSolver(pde, X, grmp.CrankNicolson, timedependent_equations=[])
     with pytest.raises(grmp.GrmpError):
         grmp.TimeControlSolver(pde, X, grmp.BackwardEuler)
     tcs = grmp.TimeControlSolver(pde, X, grmp.BackwardEuler, timedependent_equations=[0])
@@ -183,3 +186,81 @@ def test_argument_errors(grmp):
     with pytest.raises(grmp.GrmpError):  # b aliasing u
         tcs.S = tcs._system(0.1)
         X.ctx.check(grmp.lib().grmp_timestep_rhs(tcs.S.h, tcs.AM.h, tcs.f.vec.h, other.vec.h, 10.0, other.vec.h))
+
+
+@pytest.mark.parametrize("solver", ["krylov", "mg"])
+def test_crank_nicolson_matches_oracle(grmp, O, solver):
+    N = 24
+    print(f"[case] crank nicolson steps N={N} {solver}", flush=True)
+    grid = grmp.grid_unitsquare(N)
+    kap = grmp.DeviceVector(grid.ctx, grid.ncells)
+    kap.lognormal(9, 0.0, 1.0)
+    fes, pde = _heat(grmp, grid, kap=kap)
+    X = grmp.FEVector([fes])
+    X.set_entries(np.random.default_rng(2).standard_normal(fes.ndofs))
+    tcs = grmp.TimeControlSolver(pde, X, grmp.CrankNicolson, timedependent_equations=[0],
+                                 rtol=1e-13 if solver == "krylov" else 1e-12, solver=solver)
+    assert tcs.solver == solver
+    u0 = X.entries()
+    coords, cells = grid.download()
+    A, M, f, fixed = _oracle(O, coords, cells, kap=kap.download())
+    dts = [0.01, 0.01, 0.03, 0.03, 0.01]
+    ref = O.crank_nicolson(A, M, f, u0, dts, fixed)
+    for k, dt in enumerate(dts):
+        info = tcs.advance(dt)
+        assert info.status == 0
+        err = np.max(np.abs(X.entries() - ref[k])) / np.max(np.abs(ref[k]))
+        print(f"  step {k} dt={dt} iterations {info.iterations} max rel {err:.2e}")
+        assert err <= 1e-9, (k, err)
+    assert tcs.assemblies == 3
+
+
+def test_advance_until_time_crank_nicolson_uploaded(grmp, O):
+    """advance_until_time! on a shuffled mesh: 0.05 steps to t = 0.4, the
+    states equal the oracle's, the time is met, and the callback runs every
+    step."""
+    coords, cells = shuffled_square(O, 9, 6)
+    print("[case] advance until time (CN, uploaded mesh)", flush=True)
+    grid = grmp.ExtendableGrid.from_arrays(coords, cells)
+    fes, pde = _heat(grmp, grid, diffusion=0.9)
+    X = grmp.FEVector([fes])
+    tcs = grmp.TimeControlSolver(pde, X, grmp.CrankNicolson, timedependent_equations=[0], rtol=1e-13)
+    seen = []
+    n = tcs.advance_until_time(0.05, 0.4, finaltime_tolerance=1e-12, do_after_each_timestep=lambda t: seen.append(t.cstep))
+    assert n == 8 and seen == list(range(1, 9)) and abs(tcs.ctime - 0.4) < 1e-12
+    A, M, f, fixed = _oracle(O, coords, cells, diffusion=0.9)
+    ref = O.crank_nicolson(A, M, f, np.zeros(fes.ndofs), [0.05] * 8, fixed)[-1]
+    err = np.max(np.abs(X.entries() - ref)) / np.max(np.abs(ref))
+    print(f"  vs oracle {err:.2e}")
+    assert err <= 1e-9
+
+
+def test_auto_solver_inhomogeneous_data_tight_tolerance(grmp, O):
+    """solver="auto" with inhomogeneous Dirichlet data picks the lifted
+    multigrid steps; at rtol 1e-13 its states match the oracle's to 1e-9 and
+    keep the boundary values to 1e-13."""
+    N = 16
+    print(f"[case] auto solver, inhomogeneous data, rtol 1e-13, N={N}", flush=True)
+    grid = grmp.grid_unitsquare(N)
+    fes = grmp.FESpace(grmp.H1P1(1), grid)
+    coef = [0.5, 2.0, -1.0, 0.3]
+    pde = grmp.PoissonProblem(1.0)
+    pde.add_rhsdata(0, grmp.LinearForm(grmp.Identity, grmp.DataFunction([1.0])))
+    pde.add_boundarydata(0, grmp.InterpolateDirichletBoundary, data=grmp.PolynomialData(coef))
+    X = grmp.FEVector([fes])
+    tcs = grmp.TimeControlSolver(pde, X, grmp.BackwardEuler, timedependent_equations=[0], rtol=1e-13)
+    assert tcs.solver == "mg"
+    coords, cells = grid.download()
+    A, M, f, fixed = _oracle(O, coords, cells)
+    g = O.poly_eval(coef, coords[:, 0], coords[:, 1])
+    u0 = X.entries()
+    assert np.max(np.abs(u0[fixed] - g[fixed])) < 1e-15
+    dts = [0.02, 0.02, 0.05]
+    ref = O.backward_euler(A, M, f, u0, dts, fixed)
+    for k, dt in enumerate(dts):
+        info = tcs.advance(dt)
+        assert info.status == 0, info.status
+        u = X.entries()
+        err = np.max(np.abs(u - ref[k])) / np.max(np.abs(ref[k]))
+        print(f"  step {k} iterations {info.iterations} relres {info.relres:.2e} max rel {err:.2e}")
+        assert err <= 1e-9 and np.max(np.abs(u[fixed] - g[fixed])) < 1e-13

@@ -102,3 +102,60 @@ def test_p2_poisson_with_quadratic_data(O):
     P = _points(coords, fn, 2)
     u = O.poly_eval(Q2, P[:, 0], P[:, 1])
     assert np.max(np.abs(x - u)) < 1e-11
+
+
+def _br_space(O, N, seed):
+    if seed is None:
+        return O.StokesSpace(N)
+    coords, cells = shuffled_square(O, N, seed)
+    return O.StokesSpace(N, coords, cells)
+
+
+def test_br_linear_data_reproduced(O):
+    """H1BR{2}: linear data lies in the trace space (P1 part, zero bubble):
+    the best approximation and the interpolation both reproduce it."""
+    lin = [[1.0, 0.0, 1.0], [-1.0, 1.0, 0.0]]     # (y + 1, x - 1): the reference's Stokes test data
+    for seed in (None, 6):
+        S = _br_space(O, 5, seed)
+        t, bd = O.boundary_bestapprox_br(S, lin, 3)
+        ti = O.boundary_interpolate_br(S, lin)
+        nn = S.nnodes
+        assert np.array_equal(np.isnan(ti), np.isnan(t))
+        nodes = bd[bd < nn]
+        X, Y = S.coords[nodes, 0], S.coords[nodes, 1]
+        assert np.max(np.abs(t[nodes] - (Y + 1))) < 1e-14
+        assert np.max(np.abs(t[nodes + nn] - (X - 1))) < 1e-14
+        bub = bd[bd >= 2 * nn]
+        assert bub.size == 4 * S.N and np.max(np.abs(t[bub])) < 1e-14
+        assert np.max(np.abs(ti[bd] - t[bd])) < 1e-14
+
+
+def test_br_galerkin_orthogonality(O):
+    """Cubic data: (g - g_h, v_i) = 0 on the boundary for all five face
+    functions (8-point Gauss per face), and the interpolation keeps each
+    face's normal flux int_F g.n."""
+    xg, wg = np.polynomial.legendre.leggauss(8)
+    s8, w8 = 0.5 * (xg + 1.0), 0.5 * wg
+    g2 = [CUBIC, [0.1, 0.4, -0.3, 0.2, -0.5, 0.6, -0.7, 0.3, 0.2, -0.1]]
+    S = _br_space(O, 4, 7)
+    t, bd = O.boundary_bestapprox_br(S, g2, 5)
+    ti = O.boundary_interpolate_br(S, g2)
+    res = {d: 0.0 for d in bd.tolist()}
+    for c, k, dofs, n in O._br_face_units(S):
+        a, b = S.cells[c, k], S.cells[c, (k + 1) % 3]
+        p0, p1 = S.coords[a], S.coords[b]
+        L = np.hypot(*(p1 - p0))
+        flux_g = flux_i = 0.0
+        for s, w in zip(s8, w8):
+            x, y = p0 + s * (p1 - p0)
+            bub = 6 * s * (1 - s)
+            V = [np.array([1 - s, 0]), np.array([s, 0]), np.array([0, 1 - s]), np.array([0, s]), bub * n]
+            g = np.array([sum(ck * x ** i * y ** j for ck, (i, j) in zip(gc, _MONO)) for gc in g2])
+            gh = sum(t[d] * v for d, v in zip(dofs, V))
+            gi = sum(ti[d] * v for d, v in zip(dofs, V))
+            for d, v in zip(dofs, V):
+                res[d] += L * w * float((g - gh) @ v)
+            flux_g += L * w * float(g @ n)
+            flux_i += L * w * float(gi @ n)
+        assert abs(flux_g - flux_i) < 1e-14
+    assert max(abs(v) for v in res.values()) < 1e-14

@@ -95,3 +95,66 @@ def test_integrate_identity_tet_exact(O):
     ref = vol * u[cells].mean(axis=1)
     assert abs(vol.sum() - 1.0) < 1e-13
     assert np.max(np.abs(got - ref)) <= 1e-15 * np.max(np.abs(ref)) * 8
+
+
+def _cell_integrals(coords, cells, fvals):
+    """sum over an exact-to-degree-11 conical rule of fvals(l1, l2, x, y)"""
+    P, W = _tri_rule(6)
+    X = coords[cells]
+    det = ((X[:, 1, 0] - X[:, 0, 0]) * (X[:, 2, 1] - X[:, 0, 1])
+           - (X[:, 2, 0] - X[:, 0, 0]) * (X[:, 1, 1] - X[:, 0, 1]))
+    out = np.zeros(cells.shape[0])
+    for (a, b), w in zip(P, W):
+        p = X[:, 0] + a * (X[:, 1] - X[:, 0]) + b * (X[:, 2] - X[:, 0])
+        out += w * fvals(a, b, p[:, 0], p[:, 1])
+    return out * np.abs(det)
+
+
+def test_l2error_p1_interpolant(O):
+    """L2ErrorIntegrator of the P1 interpolant of a quadratic g, quadorder
+    3 (+1 for P1 = the Stroud rule of order 4, exact for the degree-4
+    integrand): equal to the exact per-cell integral of (g - I g)^2; zero
+    for linear data."""
+    coords, cells, cellfaces, facenodes = _jittered_square(O, 6, 3)
+    nn = coords.shape[0]
+    cd, ndofs = O.celldofs(O.FE_P1, 1, cells, nn)
+    coef = [1.0, 2.0, -0.5, 3.0, -1.5, 0.75]
+    u = _poly(coords[:, 0], coords[:, 1])
+    got = O.l2error_cells(O.FE_P1, 1, u, [coef], 4, 1.0, coords, cells, cd)
+
+    def err2(l1, l2, x, y):
+        uh = (1 - l1 - l2) * u[cells[:, 0]] + l1 * u[cells[:, 1]] + l2 * u[cells[:, 2]]
+        return (_poly(x, y) - uh) ** 2
+    ref = _cell_integrals(coords, cells, err2)
+    assert np.all(ref > 0)
+    assert np.max(np.abs(got - ref) / ref) < 1e-12
+    lin = [0.5, -1.0, 2.0]
+    ul = 0.5 - coords[:, 0] + 2.0 * coords[:, 1]
+    z = O.l2error_cells(O.FE_P1, 1, ul, [lin], 2, 1.0, coords, cells, cd)
+    assert np.max(z) < 1e-28
+
+
+def test_l2norm_br_identity(O):
+    """L2NormIntegrator of a Bernardi-Raugel function on a jittered mesh:
+    P1 parts per component plus face bubbles 6 l_a l_b times the face
+    normal, against the conical rule (degree 4 integrand, Stroud order 4)."""
+    coords, cells, cellfaces, facenodes = _jittered_square(O, 5, 9)
+    S = O.StokesSpace(5, coords, cells)
+    rng = np.random.default_rng(2)
+    u = rng.standard_normal(S.nU)
+    nrm = S.fnormal.reshape(-1, 2)[S.cellfaces]           # (nc, 3, 2)
+    got = O.l2error_cells(O.FE_BR, 2, u, None, 4, -1.0, S.coords, S.cells, S.celldofs, normals=nrm)
+    cdf = S.celldofs
+
+    def sq(l1, l2, x, y):
+        lam = [1 - l1 - l2, l1, l2]
+        val = np.zeros((cells.shape[0], 2))
+        for k in range(2):
+            for a in range(3):
+                val[:, k] += u[cdf[:, 3 * k + a]] * lam[a]
+        for f in range(3):
+            b = 6 * lam[f] * lam[(f + 1) % 3]
+            val += (u[cdf[:, 6 + f]] * b)[:, None] * nrm[:, f, :]
+        return (val ** 2).sum(axis=1)
+    ref = _cell_integrals(S.coords, S.cells, sq)
+    assert np.max(np.abs(got - ref) / ref) < 1e-12

@@ -57,3 +57,31 @@ def test_stokes_rhs_divergence_free_part(O):
     b[S.fixed] = 0.0
     x = O.direct_solve(A, b)
     assert np.max(np.abs(x[:S.nU])) > 1e-4
+
+
+def test_stokes_exactness_br_p0(O):
+    """The reference's Stokes test for H1BR{2} x L2P0 with orders (1, 0)
+    (test/runtests.jl:521-540,581,606-640): u = (y + 1, x - 1), p = 0, f = 0,
+    viscosity 1, best-approximated Dirichlet data, FixedIntegralMean; the
+    L2 errors of u_h (quadorder 2 + 1) and p_h (0 + 0) are below the
+    reference's tolerance 6e-12.  On the structured square and on a shuffled
+    mesh (the reference uses ExtendableGrids' refined unit square, absent
+    here)."""
+    from meshes import shuffled_square
+    lin = np.array([[1.0, 0.0, 1.0], [-1.0, 1.0, 0.0]])
+    for seed in (None, 8):
+        if seed is None:
+            S = O.StokesSpace(8)
+        else:
+            coords, cells = shuffled_square(O, 8, seed)
+            S = O.StokesSpace(8, coords, cells)
+        A = S.matrix(1.0)
+        t, bd = O.boundary_bestapprox_br(S, lin, 2 + 1)
+        target = np.zeros(S.n)
+        target[bd] = t[bd]
+        x = S.pressure_mean_shift(O.direct_solve_fixed(A, np.zeros(S.n), S.fixed, target[S.fixed]))
+        nrm = S.fnormal.reshape(-1, 2)[S.cellfaces]
+        eu = O.l2error_cells(O.FE_BR, 2, x[:S.nU], lin, 3, 1.0, S.coords, S.cells, S.celldofs, normals=nrm).sum()
+        ep = O.l2error_cells(O.FE_P0, 1, x[S.nU:], None, 0, 1.0, S.coords, S.cells,
+                             np.arange(S.nc, dtype=np.int32)[:, None]).sum()
+        assert np.sqrt(eu) < 6e-12 and np.sqrt(ep) < 6e-12, (seed, np.sqrt(eu), np.sqrt(ep))

@@ -66,3 +66,37 @@ def test_fixed_values_kept(O):
     assert np.array_equal(b[S.fixed], O.PENALTY * u0[S.fixed])
     u1 = O.backward_euler(A, M, np.zeros(S.ndofs), u0, [0.1], S.fixed)[0]
     assert np.allclose(u1[S.fixed], u0[S.fixed], rtol=1e-12, atol=0)
+
+
+def test_crank_nicolson_eigenvector_decay(O):
+    """CrankNicolson: a generalized eigenvector decays by exactly
+    (1 - lambda dt / 2) / (1 + lambda dt / 2) per step, across a step size
+    change (fails for a BackwardEuler factor, a missing A u^n or a wrong
+    2/dt)."""
+    S, A, M, free = _heat(O, 8, kappa=0.7)
+    Af = A.to_scipy().toarray()[np.ix_(free, free)]
+    Mf = M.to_scipy().toarray()[np.ix_(free, free)]
+    lam, V = sla.eigh(Af, Mf)
+    for k in (0, 5, 20):
+        u0 = np.zeros(S.ndofs)
+        u0[free] = V[:, k]
+        dts = [0.05, 0.05, 0.01]
+        out = O.crank_nicolson(A, M, np.zeros(S.ndofs), u0, dts, S.fixed)
+        g = 1.0
+        for dt, u in zip(dts, out):
+            g *= (1.0 - 0.5 * lam[k] * dt) / (1.0 + 0.5 * lam[k] * dt)
+            assert np.max(np.abs(u[free] - g * V[:, k])) <= 1e-10 * np.max(np.abs(V[:, k])), (k, dt)
+
+
+def test_crank_nicolson_stationary_and_fixed(O):
+    S, A, M, free = _heat(O, 9)
+    f = np.zeros(S.ndofs)
+    O.assemble_lf_const(f, O.FE_P1, 1, 1, [1.0], S.coords, S.cells, S.celldofs)
+    us = np.zeros(S.ndofs)
+    us[free] = np.linalg.solve(A.to_scipy().toarray()[np.ix_(free, free)], f[free])
+    for dt in (1e-3, 0.1, 10.0):
+        u1 = O.crank_nicolson(A, M, f, us, [dt], S.fixed)[0]
+        assert np.max(np.abs(u1 - us)) <= 1e-12 * np.max(np.abs(us)), dt
+    u0 = np.random.default_rng(4).standard_normal(S.ndofs)
+    b = O.crank_nicolson_rhs(A, M, f, u0, 0.1, S.fixed)
+    assert np.array_equal(b[S.fixed], O.PENALTY * u0[S.fixed])
