@@ -7,9 +7,12 @@ linear-elasticity problem (50:50; E=1, nu=0.3) on an N x N triangulated unit
 square with N = round(exp(U(ln 16, ln 768))) and a per-request seeded
 log-normal cell coefficient (sigma = 1).  Poisson requests use H1P1 or H1P2
 (50:50), elasticity uses H1P1{2}.  Every request is solved to relative
-FP64 residual 1e-9 (the target is relative error <= 1e-8 against FP64
-solves, checked on 24 requests spanning the size range) and its solution
-copied back to the host.
+FP64 residual 1e-9 and its solution copied into page-locked host memory
+before the request counts as done (the copy is inside the timed region and
+inside each latency).  The error target (relative error <= 1e-8) is checked
+against the oracle's FP64 solves of the same requests (same seeded
+coefficients) and against device FP64 solves to 1e-13 on requests spanning
+the size range.
 
 Solver (--solver): mg (default) serves every request with FP64 CG
 preconditioned by a geometric multigrid V-cycle, several requests at once on
@@ -19,8 +22,12 @@ their own streams; engine serves them with the batched Jacobi-PCG engine
 Reported:
   value           DoF-solves/s of the served stream (sum of ndofs / wall)
   p50/p99         request latency (arrival -> solution on the host)
-  plain           the same metric for plain one-at-a-time FP64 GPU solves,
+  plain           the same metric for plain one-at-a-time FP64 GPU solves
+                  (host-driven Jacobi-PCG, every request built from scratch),
                   measured on a systematic sample of the stream
+  plain_mg        the same sample one at a time with the served solver
+                  (multigrid V-cycle CG, one request in flight, no batching)
+  rate_sweep      the first 512 requests at 100 ... 2000 req/s
   roofline        mg: the batched residual over all requests' rows of a level
                   (k_mgb_rows<OP_RESID>; with --mg-batch 0 the residual on
                   levels of >= 2^21 rows, k_mg_resid_fine); engine: the engine
@@ -57,10 +64,17 @@ def make_stream(nreq, rate, seed=2024):
             for i in range(nreq)]
 
 
+def stream_ndofs(grmp, stream):
+    return np.array([grmp.server.ndofs(r["kind"], r["fe"], r["N"]) for r in stream], dtype=np.int64)
+
+
 def serve(grmp, mode, stream, rtol, keep=(), profile=False, respect_arrivals=True, tile_rows=0, precision="fp32_ir",
           fp64_first_sweep=0, max_active=512, mg_slots=0, mg_threads=0, mg_block_rows=0, mg_graph=False,
-          mg_coarse_rows=0, mg_batch=None):
-    """Submits the stream at its arrival times and waits for every result."""
+          mg_coarse_rows=0, mg_batch=None, pinned=None):
+    """Submits the stream at its arrival times and waits for every result.
+    With ``pinned`` (a PinnedBuffer of at least the stream's total dofs),
+    every request's solution is copied into its slice of it before the
+    request completes; otherwise only the ``keep`` requests are copied."""
     graph = mg_graph if mode == "mg" else True
     S = grmp.Server(mode, profile=profile, memory_budget=96 << 30, max_active=max_active, tile_rows=tile_rows,
                     graph=graph,
@@ -68,14 +82,20 @@ def serve(grmp, mode, stream, rtol, keep=(), profile=False, respect_arrivals=Tru
                     mg_block_rows=mg_block_rows, mg_coarse_rows=mg_coarse_rows, mg_batch=mg_batch)
     keep = set(keep)
     tickets = []
+    offs = None
+    if pinned is not None:
+        nd = stream_ndofs(grmp, stream)
+        offs = np.concatenate([[0], np.cumsum(nd)])
+        assert offs[-1] <= pinned.n, "pinned buffer too small for the stream"
     t0 = time.perf_counter()
     for k, r in enumerate(stream):
         if respect_arrivals:
             dt = r["t"] - (time.perf_counter() - t0)
             if dt > 0:
                 time.sleep(dt)
+        out = pinned.view(int(offs[k]), int(offs[k + 1] - offs[k])) if pinned is not None else None
         tickets.append(S.submit(r["kind"], r["fe"], r["N"], r["seed"], rtol=rtol, keep_solution=k in keep,
-                                precision=precision))
+                                precision=precision, out=out))
     res = []
     for k, t in enumerate(tickets):
         res.append(S.wait(t))
@@ -95,10 +115,19 @@ def serve(grmp, mode, stream, rtol, keep=(), profile=False, respect_arrivals=Tru
                 service=np.array([r.t_done - r.t_admit for r in res]))
 
 
-def _oracle_request(O, r, rng, rtol):
+def request_kappa(grmp, r, sigma=1.0):
+    """the request's per-cell coefficient, generated as the server does
+    (Philox log-normal keyed by the request's seed)"""
+    kv = grmp.DeviceVector(grmp.default_context(), 2 * r["N"] * r["N"])
+    kv.lognormal(r["seed"], 0.0, sigma)
+    return kv.download()
+
+
+def _oracle_request(O, r, kap, rtol):
+    """assembles and solves one request with the oracle (single-thread
+    assembly, OpenMP FP64 Jacobi-PCG); returns (ndofs, solution)"""
     if r["kind"] == 1:
         S = O.Space(r["N"], r["fe"])
-        kap = np.exp(rng.standard_normal(S.cells.shape[0]))
         A = O.assemble_matrix(S.ndofs, [dict(kind=O.BLF_LAPLACE, fe=r["fe"], quadorder=2 * r["fe"] - 2,
                                             factor=1.0, coef=O.COEF_CELL, kappa=kap, coords=S.coords,
                                             cells=S.cells, celldofs=S.celldofs)], fixed=S.fixed)
@@ -106,7 +135,6 @@ def _oracle_request(O, r, rng, rtol):
         O.assemble_lf_const(b, r["fe"], 1, r["fe"], [1.0], S.coords, S.cells, S.celldofs)
     else:
         S = O.Space(r["N"], O.FE_P1, ncomp=2)
-        kap = np.exp(rng.standard_normal(S.cells.shape[0]))
         A = O.assemble_matrix(S.ndofs, [dict(kind=O.BLF_HOOKE2D, fe=O.FE_P1, ncomp=2, quadorder=0, factor=1.0,
                                             coef=O.COEF_CELL, kappa=kap,
                                             params=np.array([0.3 / (1.3 * 0.4), 1.0 / 2.6]),
@@ -115,17 +143,29 @@ def _oracle_request(O, r, rng, rtol):
         b = np.zeros(S.ndofs)
         O.assemble_lf_const(b, O.FE_P1, 2, 1, [0.0, -1.0], S.coords, S.cells, S.celldofs)
     b = O.penalized_rhs(b, S.fixed)
-    O.pcg(A, b, S.fixed, rtol=rtol)
-    return S.ndofs
+    x, it, rr = O.pcg(A, b, S.fixed, rtol=rtol)
+    return S.ndofs, x
 
 
-def cpu_baseline(stream, budget_s=15.0, rtol=1e-9):
+def oracle_errors(grmp, stream, sols, idx, rtol=1e-12):
+    """relative errors of served solutions against the oracle's FP64 solves
+    of the same requests (same coefficient fields) to relative residual rtol"""
+    O = GE.load_oracle()
+    errs = []
+    for k in idx:
+        r = stream[k]
+        _, x = _oracle_request(O, r, request_kappa(grmp, r), rtol)
+        errs.append(float(np.linalg.norm(sols[k] - x) / np.linalg.norm(x)))
+    return errs
+
+
+def cpu_baseline(grmp, stream, budget_s=15.0, rtol=1e-9):
     """The oracle (single-thread assembly + OpenMP FP64 Jacobi-PCG) on a
     bounded sample of the stream: requests in a fixed random order, each
     taken only if its predicted time (cost ~ ndofs * N, calibrated on the
-    requests already run) fits the remaining budget."""
+    requests already run) fits the remaining budget.  The coefficient field
+    is generated outside the timing."""
     O = GE.load_oracle()
-    rng = np.random.default_rng(7)
     threads = O.lib().or_num_threads()
     order = np.random.default_rng(11).permutation(len(stream))
     done_dofs, t_all, n, unit = 0, 0.0, 0, None
@@ -140,8 +180,9 @@ def cpu_baseline(stream, budget_s=15.0, rtol=1e-9):
             continue
         if unit is None and r["N"] > 64:
             continue  # calibrate on a small request first
+        kap = request_kappa(grmp, r)
         t0 = time.perf_counter()
-        _oracle_request(O, r, rng, rtol)
+        _oracle_request(O, r, kap, rtol)
         dt = time.perf_counter() - t0
         t_all += dt
         done_dofs += nd
@@ -485,6 +526,10 @@ def main():
                     help="relative FP64 residual; the served error target is 1e-8 and is verified")
     ap.add_argument("--plain-every", type=int, default=32, help="plain leg: every k-th request of the stream")
     ap.add_argument("--verify", type=int, default=24, help="requests re-solved in FP64 for the error check")
+    ap.add_argument("--oracle-verify", type=int, default=12,
+                    help="requests (N <= 256) re-solved by the oracle on the CPU for the error check")
+    ap.add_argument("--sweep-requests", type=int, default=512, help="rate sweep: requests per rate (0: off)")
+    ap.add_argument("--sweep-rates", default="100,200,500,1000,2000", help="rate sweep: arrival rates (req/s)")
     ap.add_argument("--max-active", type=int, default=512, help="served problems in flight")
     ap.add_argument("--minres-inner", type=float, default=1e-4, help="c4: FP32 inner MINRES relative tolerance")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -527,27 +572,35 @@ def main():
         run_workload(grmp, args.workload, args, log)
         return
     stream = make_stream(args.requests, args.rate)
+    ctx = grmp.default_context()
+    # one page-locked arena for every request's solution, allocated before the timed runs
+    nd_stream = stream_ndofs(grmp, stream)
+    pinned = grmp._abi.PinnedBuffer(ctx, int(nd_stream.sum()))
 
     mode = "mg" if args.solver == "mg" else "served"
     # warmup: the first 256 requests of the stream, untimed
     for w in range(args.warmup):
         log(f"[bench] warmup {w}")
-        serve(grmp, mode, stream[:256], args.rtol, profile=True, mg_slots=args.mg_slots)
+        serve(grmp, mode, stream[:256], args.rtol, profile=True, mg_slots=args.mg_slots, pinned=pinned)
 
     # verification sample: spread over the size range, largest included
     order = np.argsort([r["N"] * (2 if r["fe"] == 2 or r["kind"] == 2 else 1) for r in stream])
     vidx = sorted(set(order[np.linspace(0, len(order) - 1, args.verify).astype(int)].tolist()))
+    # oracle sample: the same spread over the requests the CPU solves in seconds (N <= 256)
+    small = [k for k in order.tolist() if stream[k]["N"] <= 256]
+    oidx = sorted(set(small[j] for j in np.linspace(0, len(small) - 1, args.oracle_verify).astype(int)))
+    offs = np.concatenate([[0], np.cumsum(nd_stream)])
 
     runs = []
     for s in range(args.steps):
         log(f"[bench] step {s}: {len(stream)} requests at {args.rate:g}/s")
-        GE.load().default_context().sync()
-        runs.append(serve(grmp, mode, stream, args.rtol, keep=vidx, profile=True, tile_rows=args.tile_rows,
+        ctx.sync()
+        runs.append(serve(grmp, mode, stream, args.rtol, profile=True, tile_rows=args.tile_rows,
                           precision=args.precision, fp64_first_sweep=args.fp64_first_sweep,
                           max_active=args.max_active, mg_slots=args.mg_slots, mg_threads=args.mg_threads,
                           mg_block_rows=args.mg_block_rows, mg_graph=args.mg_graph,
-                          mg_coarse_rows=args.mg_coarse_rows, mg_batch=args.mg_batch))
-        GE.load().default_context().sync()
+                          mg_coarse_rows=args.mg_coarse_rows, mg_batch=args.mg_batch, pinned=pinned))
+        ctx.sync()
     run = runs[-1]
     walls = [r["wall"] for r in runs]
     if not run["ok"].all():
@@ -557,21 +610,47 @@ def main():
                 f"status {run['status'][k]} iterations {run['iters'][k]} outer {run['outer'][k]}")
         raise SystemExit(f"{int((~run['ok']).sum())} requests failed")
     value = float(np.mean([r["dofs"].sum() / r["wall"] for r in runs]))
+    # the last run's solutions, as delivered to the host
+    sols = {k: pinned.view(int(offs[k]), int(nd_stream[k])).copy() for k in set(vidx) | set(oidx)}
 
-    log("[bench] verification against FP64 solves")
+    log("[bench] verification against device FP64 solves (rel. residual 1e-13)")
     vstream = [dict(stream[k], t=0.0) for k in vidx]
     ref = serve(grmp, "plain", vstream, 1e-13, keep=range(len(vidx)), respect_arrivals=False)
     errs = []
     for j, k in enumerate(vidx):
-        xr, xs = ref["sols"][j], run["sols"][k]
+        xr, xs = ref["sols"][j], sols[k]
         errs.append(float(np.linalg.norm(xs - xr) / np.linalg.norm(xr)))
+    oerrs = []
+    if not args.no_cpu:
+        log(f"[bench] verification against the oracle's FP64 solves ({len(oidx)} requests, N <= 256)")
+        oerrs = oracle_errors(grmp, stream, sols, oidx)
 
-    log("[bench] plain one-at-a-time FP64 leg")
+    log("[bench] plain one-at-a-time FP64 leg (Jacobi-PCG, every request built from scratch)")
     pstream = [dict(r, t=0.0) for r in stream[::args.plain_every]]
-    plain = serve(grmp, "plain", pstream, args.rtol, respect_arrivals=False)
+    plain = serve(grmp, "plain", pstream, args.rtol, respect_arrivals=False, pinned=pinned)
     plain_value = float(plain["dofs"].sum() / plain["wall"])
-    served_sample_mask = np.zeros(len(stream), bool)
-    served_sample_mask[::args.plain_every] = True
+    plain_mg = None
+    if mode == "mg":
+        log("[bench] plain one-at-a-time leg with the served solver (multigrid CG, one request in flight)")
+        pm = serve(grmp, "mg", pstream, args.rtol, respect_arrivals=False, mg_slots=1, mg_threads=1, mg_batch=0,
+                   pinned=pinned)
+        plain_mg = float(pm["dofs"].sum() / pm["wall"])
+
+    sweep = []
+    if args.sweep_requests > 0:
+        for rate in [float(x) for x in args.sweep_rates.split(",") if x]:
+            sstream = make_stream(args.sweep_requests, rate)
+            log(f"[bench] rate sweep: {len(sstream)} requests at {rate:g}/s")
+            sr = serve(grmp, mode, sstream, args.rtol, max_active=args.max_active, mg_batch=args.mg_batch,
+                       pinned=pinned)
+            if not sr["ok"].all():
+                raise SystemExit(f"rate sweep at {rate:g}/s: {int((~sr['ok']).sum())} requests failed")
+            sweep.append({"rate_per_s": rate, "requests": len(sstream), "wall_s": round(sr["wall"], 3),
+                          "arrival_span_s": round(sstream[-1]["t"], 3),
+                          "solves_per_s": round(len(sstream) / sr["wall"], 2),
+                          "dof_solves_per_s": round(float(sr["dofs"].sum() / sr["wall"]), 1),
+                          "p50_latency_ms": round(1e3 * float(np.percentile(sr["latency"], 50)), 2),
+                          "p99_latency_ms": round(1e3 * float(np.percentile(sr["latency"], 99)), 2)})
 
     st = run["stats"]
     achieved = st["spmv_bytes"] / st["spmv_seconds"] / 1e9 if st["spmv_seconds"] > 0 else 0.0
@@ -595,7 +674,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle)")
-        cpu = cpu_baseline(stream, args.cpu_seconds, args.rtol)
+        cpu = cpu_baseline(grmp, stream, args.cpu_seconds, args.rtol)
 
     lat = run["latency"]
     out = {
@@ -614,16 +693,28 @@ def main():
         "config": {"workload": f"served stream: {args.requests} requests, Poisson arrivals {args.rate:g}/s, "
                                "Poisson(P1|P2)/elasticity(P1) 50:50, N log-uniform [16,768], log-normal sigma=1",
                    "requests": args.requests, "rate_per_s": args.rate, "rtol": args.rtol,
-                   "total_dofs": int(run["dofs"].sum())},
+                   "total_dofs": int(run["dofs"].sum()),
+                   "delivery": "every solution copied into page-locked host memory before its request completes "
+                               "(inside the timed region and the latency)"},
         "solves_per_s": round(len(stream) / float(np.mean(walls)), 2),
         "p50_latency_ms": round(1e3 * float(np.percentile(lat, 50)), 2),
         "p99_latency_ms": round(1e3 * float(np.percentile(lat, 99)), 2),
         "max_rel_err_vs_fp64": max(errs),
+        "max_rel_err_vs_oracle": max(oerrs) if oerrs else None,
+        "oracle_verified": {"requests": len(oerrs), "rtol": 1e-12,
+                            "sample": "requests with N <= 256 spread over sizes and kinds; the oracle assembles and "
+                                      "solves each with the server's coefficient field"} if oerrs else None,
         "plain": {"value": round(plain_value, 1), "unit": "DoF-solves/s", "requests": len(pstream),
-                  "sample": f"every {args.plain_every}th request of the stream, served one at a time",
+                  "sample": f"every {args.plain_every}th request of the stream, served one at a time "
+                            "(host-driven FP64 Jacobi-PCG, mesh, dofmap and pattern built per request)",
                   "p50_service_ms": round(1e3 * float(np.percentile(plain["service"], 50)), 2),
                   "p99_service_ms": round(1e3 * float(np.percentile(plain["service"], 99)), 2)},
         "served_vs_plain": round(value / plain_value, 3),
+        "plain_mg": {"value": round(plain_mg, 1), "unit": "DoF-solves/s", "requests": len(pstream),
+                     "sample": "the same requests one at a time with the served solver (multigrid V-cycle CG, one "
+                               "request in flight, no lockstep batching)"} if plain_mg else None,
+        "served_vs_plain_same_solver": round(value / plain_mg, 3) if plain_mg else None,
+        "rate_sweep": sweep,
         "solver": ("FP64 CG + geometric multigrid V-cycle, " + ("requests in lockstep batches" if batched else
                    "one stream per request in flight")) if mode == "mg"
                   else "batched Jacobi-PCG engine, FP32 sweeps in FP64 refinement",

@@ -27,11 +27,11 @@ ERR_NAMES = {0: "ok", 1: "bad argument", 2: "out of device memory", 3: "HIP erro
              5: "non-finite values", 6: "admission refused", 7: "unsupported", 8: "busy"}
 
 FE_H1P1, FE_H1P2, FE_H1BR, FE_L2P0 = 1, 2, 3, 4
-BLF_LAPLACE, BLF_HOOKE2D, BLF_MASS, BLF_DIVP0 = 1, 2, 3, 4
+BLF_LAPLACE, BLF_HOOKE2D, BLF_MASS, BLF_DIVP0, BLF_NLDIFF = 1, 2, 3, 4, 5
 COEF_FACTOR, COEF_CELL = 0, 1
-LF_CONST, LF_RECO_RT0 = 1, 2
+LF_CONST, LF_RECO_RT0, LF_POLY, LF_NEWTON = 1, 2, 3, 4
 VALUES_FP64, VALUES_FP32 = 1, 2
-SOLVE_CG, SOLVE_MINRES = 1, 2
+SOLVE_CG, SOLVE_MINRES, SOLVE_BICGSTAB = 1, 2, 3
 PREC_JACOBI = 1
 PRECISION_FP64, PRECISION_FP32_IR = 1, 2
 REQ_POISSON, REQ_ELASTICITY = 1, 2
@@ -52,13 +52,14 @@ class GrmpError(RuntimeError):
 class Form(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("coef_mode", ctypes.c_int), ("quadorder", ctypes.c_int),
                 ("row_block", ctypes.c_int), ("col_block", ctypes.c_int), ("factor", ctypes.c_double),
-                ("params", ctypes.c_double * 4), ("cell_coef", VP)]
+                ("params", ctypes.c_double * 4), ("cell_coef", VP), ("current", VP), ("current_offset", I64)]
 
 
 class LForm(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("quadorder", ctypes.c_int), ("factor", ctypes.c_double),
                 ("f", ctypes.c_double * 4), ("nmodes", ctypes.c_int), ("pad_", ctypes.c_int), ("modes", DP),
-                ("poly", ctypes.c_double * 4)]
+                ("poly", ctypes.c_double * 4), ("current", VP), ("current_offset", I64), ("coef", DP),
+                ("ncoef", ctypes.c_int), ("pad2_", ctypes.c_int)]
 
 
 class SolveOpts(ctypes.Structure):
@@ -152,6 +153,13 @@ SIGNATURES = {
     "grmp_vector_axpy": (ctypes.c_int, [ctypes.c_double, VP, VP]),
     "grmp_assemble_linearform": (ctypes.c_int, [VP, VP, ctypes.POINTER(LForm), VP, I64]),
     "grmp_integrate_identity": (ctypes.c_int, [VP, VP, ctypes.c_int, VP, I64, DP]),
+    "grmp_host_alloc": (ctypes.c_int, [VP, ctypes.c_size_t, ctypes.POINTER(VP)]),
+    "grmp_matrix_set_tune": (ctypes.c_int, [VP, VP]),
+    "grmp_matrix_tune_key": (ctypes.c_int, [VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                            ctypes.POINTER(ctypes.c_int)]),
+    "grmp_tune_load": (ctypes.c_int, [VP, ctypes.c_char_p]),
+    "grmp_tune_count": (ctypes.c_int, [VP]),
+    "grmp_host_free": (ctypes.c_int, [VP]),
     "grmp_integrate_l2error": (ctypes.c_int, [VP, VP, VP, I64, DP, ctypes.c_int, ctypes.c_int, ctypes.c_double, DP,
                                               ctypes.POINTER(ctypes.c_double)]),
     "grmp_solve_opts_default": (None, [ctypes.POINTER(SolveOpts)]),
@@ -237,6 +245,33 @@ def default_context():
     if _default is None:
         _default = Context(0)
     return _default
+
+
+class AsmTune(ctypes.Structure):
+    _fields_ = [("variant", ctypes.c_int32), ("rows_per_block", ctypes.c_int32), ("xcd_remap", ctypes.c_int32)]
+
+
+class PinnedBuffer:
+    """Page-locked host memory (grmp_host_alloc) holding ``n`` doubles;
+    ``view(off, m)`` is a numpy array over part of it."""
+
+    def __init__(self, ctx, n):
+        self.ctx, self.n = ctx, int(n)
+        p = VP()
+        ctx.check(lib().grmp_host_alloc(ctx.h, 8 * max(self.n, 1), ctypes.byref(p)))
+        self.p = p
+        self.array = np.ctypeslib.as_array((ctypes.c_double * max(self.n, 1)).from_address(p.value))[:self.n]
+
+    def view(self, off, m):
+        return self.array[off:off + m]
+
+    def __del__(self):
+        try:
+            if self.p and alive():
+                lib().grmp_host_free(self.p)
+            self.p = None
+        except Exception:
+            pass
 
 
 class DeviceVector:

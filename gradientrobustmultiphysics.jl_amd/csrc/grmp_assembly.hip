@@ -23,6 +23,9 @@
 
 #pragma clang fp contract(off)
 
+#define RECO_MAXQ 25  // points of the largest triangle rule a kernel takes (Stroud order 9)
+static int tri_rule(int order, double (*xq)[2], double *wq);
+
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 #define MAXFORMS 4
@@ -462,6 +465,95 @@ __device__ __forceinline__ void row_divt_p0br(const DevForm &F, int64_t e, int a
     }
 }
 
+// ------------------------------------------------------------ Newton form
+// NonlinearForm(Gradient, [Identity, Gradient], [1,1], kernel, [2,3]) with
+// kernel(u, g) = alpha(u) g, alpha(u) = c0 + c1 u + c2 u^2
+// (src/pdeoperators.jl:590-635): full_assemble! (src/assemblypatterns/
+// nonlinearform.jl:44-246) evaluates the iterate's [u, grad u] at each
+// point (eval_febe!, Identity then Gradient), the kernel's Jacobian
+//   J = [alpha'(u) g_0, alpha, 0; alpha'(u) g_1, 0, alpha],
+// and adds (J [phi_b, grad phi_b]) . grad phi_a w for ansatz b, test a;
+// transposed_assembly puts it in row a (the test function), column b.
+__device__ __forceinline__ void nl_point(const double *uc, int nd, int fe, const double M[2][2], double x0, double x1,
+                                         double in[3])
+{
+    in[0] = in[1] = in[2] = 0.0;
+    for (int d = 0; d < nd; ++d) in[0] += uc[d] * (fe == 1 ? p1tri_val(d, x0, x1) : p2tri_val(d, x0, x1)) * 1.0;
+    for (int d = 0; d < nd; ++d) {
+        double g[2];
+        if (fe == 1) grad_tri<1>(M, d, x0, x1, g);
+        else grad_tri<2>(M, d, x0, x1, g);
+        in[1] += uc[d] * g[0] * 1.0;
+        in[2] += uc[d] * g[1] * 1.0;
+    }
+}
+
+// J and kernel value at input in = [u, g0, g1]
+__device__ __forceinline__ void nl_jac(const double c[3], const double in[3], double J[2][3], double val[2])
+{
+    const double u = in[0];
+    double alpha = c[0];
+    alpha += c[1] * u;
+    alpha += c[2] * (u * u);
+    double dalpha = c[1];
+    dalpha += 2.0 * c[2] * u;
+    J[0][0] = dalpha * in[1];
+    J[0][1] = alpha;
+    J[0][2] = 0.0;
+    J[1][0] = dalpha * in[2];
+    J[1][1] = 0.0;
+    J[1][2] = alpha;
+    val[0] = alpha * in[1];
+    val[1] = alpha * in[2];
+}
+
+template <int FE>
+__device__ __forceinline__ void row_nldiff_tri(const DevForm &F, int64_t e, int a, double *v)
+{
+    constexpr int ND = (FE == 1) ? 3 : 6;
+    if (!F.uold) {  // pattern: every local (test, ansatz) pair
+#pragma unroll
+        for (int b = 0; b < ND; ++b) v[b] = 1.0;
+        return;
+    }
+    double M[2][2], vol;
+    trafo_tri(F, e, M, vol);
+    const int64_t nc = F.ncells;
+    double uc[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) uc[d] = F.uold[F.rdofs[(int64_t)d * nc + e]];
+    double L[ND];
+#pragma unroll
+    for (int b = 0; b < ND; ++b) L[b] = 0.0;
+    for (int q = 0; q < F.nqt; ++q) {
+        const double x0 = F.qtab[3 * q], x1 = F.qtab[3 * q + 1], w = F.qtab[3 * q + 2];
+        double in[3], J[2][3], val[2];
+        nl_point(uc, ND, FE, M, x0, x1, in);
+        nl_jac(F.nlc, in, J, val);
+        double ga[2];
+        grad_tri<FE>(M, a, x0, x1, ga);
+#pragma unroll
+        for (int b = 0; b < ND; ++b) {
+            double gb[2];
+            grad_tri<FE>(M, b, x0, x1, gb);
+            const double in2[3] = {(FE == 1 ? p1tri_val(b, x0, x1) : p2tri_val(b, x0, x1)), gb[0], gb[1]};
+            double temp = 0.0;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                double ar = 0.0;
+                ar += J[k][0] * in2[0];
+                ar += J[k][1] * in2[1];
+                ar += J[k][2] * in2[2];
+                temp += ar * ga[k];
+            }
+            L[b] += temp * w;
+        }
+    }
+    const double itemfactor = vol * F.factor * 1.0;
+#pragma unroll
+    for (int b = 0; b < ND; ++b) v[b] = L[b] * itemfactor;
+}
+
 // form-kind traits: number of column dofs and the row function
 template <int CN_, void (*ROW)(const DevForm &, int64_t, int, double *)>
 struct Kind {
@@ -479,6 +571,8 @@ using K_MASS_P1T3 = Kind<3, row_mass_tri<1, 3>>;
 using K_LAP_BR = Kind<9, row_laplace_br<3>>;
 using K_DIV_BRP0 = Kind<1, row_div_brp0>;
 using K_DIVT_P0BR = Kind<9, row_divt_p0br>;
+using K_NLDIFF_P1 = Kind<3, row_nldiff_tri<1>>;
+using K_NLDIFF_P2 = Kind<6, row_nldiff_tri<2>>;
 
 // runs the statement(s) with KT bound to the kind type of form F
 #define GRMP_KCASE(CODE, TYPE, ...) \
@@ -499,6 +593,8 @@ using K_DIVT_P0BR = Kind<9, row_divt_p0br>;
         GRMP_KCASE(FK_LAPLACE_BR * 4 + 1, K_LAP_BR, __VA_ARGS__)                  \
         GRMP_KCASE(FK_DIV_BRP0 * 4 + 0, K_DIV_BRP0, __VA_ARGS__)                  \
         GRMP_KCASE(FK_DIVT_P0BR * 4 + 0, K_DIVT_P0BR, __VA_ARGS__)                \
+        GRMP_KCASE(FK_NLDIFF_P1TRI * 4 + 0, K_NLDIFF_P1, __VA_ARGS__)             \
+        GRMP_KCASE(FK_NLDIFF_P2TRI * 4 + 0, K_NLDIFF_P2, __VA_ARGS__)             \
     default: break;                                                               \
     }
 
@@ -643,11 +739,24 @@ __device__ __forceinline__ void assemble_row(const DevForm &F, int64_t i, double
     }
 }
 
+// The row block a workgroup owns.  With remap, consecutive row blocks go to
+// one XCD: the hardware hands workgroup b to XCD b % 8, so workgroup b takes
+// block i of XCD b % 8's contiguous share (the first nb % 8 XCDs get one
+// block more); the cells and nodes that neighbouring rows share then meet in
+// one L2 instead of eight.
+__device__ __forceinline__ int64_t row_block_of(int remap)
+{
+    const int64_t b = blockIdx.x, nb = gridDim.x;
+    if (!remap || nb < 16) return b;
+    const int64_t x = b & 7, i = b >> 3, per = nb >> 3, rem = nb & 7;
+    return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
+
 // One thread per row; the block's rows are staged contiguously in LDS
 // (dynamic shared memory of rowptr[end]-rowptr[start] doubles).  ``add``
 // accumulates the forms of the row block into a row (FP64, reference order).
 template <class AddRow>
-__device__ __forceinline__ void assemble_block(AddRow add, int64_t g0, int64_t g1, int64_t rblock,
+__device__ __forceinline__ void assemble_block(AddRow add, int64_t blk, int64_t g0, int64_t g1, int64_t rblock,
                                                const int32_t *__restrict__ rowptr, const uint8_t *__restrict__ diagpos,
                                                const uint8_t *__restrict__ fixed, double penalty,
                                                double *__restrict__ val64, float *__restrict__ val32,
@@ -655,7 +764,7 @@ __device__ __forceinline__ void assemble_block(AddRow add, int64_t g0, int64_t g
 {
     extern __shared__ double buf[];
     const int B = blockDim.x;
-    const int64_t rs = g0 + (int64_t)blockIdx.x * B;
+    const int64_t rs = g0 + blk * B;
     const int64_t re = min(rs + B, g1);
     const int32_t base = rowptr[rs];
     const int32_t len = rowptr[re] - base;
@@ -702,7 +811,7 @@ __global__ void k_assemble(FormPack P, int64_t g0, int64_t g1, int64_t rblock, c
                 GRMP_DISPATCH(F, assemble_row<KT>(F, i, row));
             }
         },
-        g0, g1, rblock, rowptr, diagpos, fixed, penalty, val64, val32, dinv64, dinv32);
+        (int64_t)blockIdx.x, g0, g1, rblock, rowptr, diagpos, fixed, penalty, val64, val32, dinv64, dinv32);
 }
 
 // One form of a known kind, passed by value (kernel arguments stay scalar),
@@ -717,7 +826,8 @@ __global__ void k_assemble(FormPack P, int64_t g0, int64_t g1, int64_t rblock, c
 #define ASM_NT 256
 #define ASM_STAGE 2048  // FP64 staging entries (16 KB)
 template <class K>
-__global__ __launch_bounds__(ASM_NT) void k_assemble1(DevForm F, int rb, int64_t g0, int64_t g1, int64_t rblock,
+__global__ __launch_bounds__(ASM_NT) void k_assemble1(DevForm F, int rb, int remap, int64_t g0, int64_t g1,
+                                                      int64_t rblock,
                                                       const int32_t *__restrict__ rowptr,
                                                       const uint8_t *__restrict__ diagpos,
                                                       const uint8_t *__restrict__ fixed, double penalty,
@@ -729,7 +839,7 @@ __global__ __launch_bounds__(ASM_NT) void k_assemble1(DevForm F, int rb, int64_t
     double *stage = shm;             // SCAP * CN local values
     double *buf = shm + ASM_STAGE;   // the block's rows
     const int t = threadIdx.x;
-    const int64_t rs = g0 + (int64_t)blockIdx.x * rb;
+    const int64_t rs = g0 + row_block_of(remap) * rb;
     const int64_t re = min(rs + rb, g1);
     const int32_t base = rowptr[rs];
     const int32_t len = rowptr[re] - base;
@@ -797,45 +907,76 @@ __global__ __launch_bounds__(ASM_NT) void k_assemble1(DevForm F, int rb, int64_t
 // One form of a known kind, one thread per row walking its cells serially:
 // the better choice when rows have few incidences (triangles)
 template <class K>
-__global__ void k_assemble1s(DevForm F, int64_t g0, int64_t g1, int64_t rblock, const int32_t *__restrict__ rowptr,
-                             const uint8_t *__restrict__ diagpos, const uint8_t *__restrict__ fixed, double penalty,
-                             double *__restrict__ val64, float *__restrict__ val32, double *__restrict__ dinv64,
-                             float *__restrict__ dinv32)
+__global__ void k_assemble1s(DevForm F, int remap, int64_t g0, int64_t g1, int64_t rblock,
+                             const int32_t *__restrict__ rowptr, const uint8_t *__restrict__ diagpos,
+                             const uint8_t *__restrict__ fixed, double penalty, double *__restrict__ val64,
+                             float *__restrict__ val32, double *__restrict__ dinv64, float *__restrict__ dinv32)
 {
-    assemble_block([&](int64_t i, double *row) { assemble_row<K>(F, i, row); }, g0, g1, rblock, rowptr, diagpos, fixed,
-                   penalty, val64, val32, dinv64, dinv32);
+    assemble_block([&](int64_t i, double *row) { assemble_row<K>(F, i, row); }, row_block_of(remap), g0, g1, rblock,
+                   rowptr, diagpos, fixed, penalty, val64, val32, dinv64, dinv32);
 }
 
-// Launches the value assembly of row block [g0, g1): the per-kind kernel when
+int nnz_row_bucket(int max_row)
+{
+    return max_row <= 8 ? 0 : max_row <= 16 ? 1 : max_row <= 32 ? 2 : max_row <= 64 ? 3 : 4;
+}
+
+// the launch shape for a single-form row block of kind hf: the matrix's own,
+// else the context's tuning table, else the built-in choice (variant and rows
+// 0, remap -1 are filled in by launch_assemble)
+static AsmTune asm_tune(const grmp_matrix *A, const DevForm &hf)
+{
+    if (A->tune_set) return A->tune;
+    const int bucket = nnz_row_bucket(A->max_row);
+    for (const TuneEntry &e : A->ctx->tune)
+        if (e.kind == hf.kind && e.nq == hf.nq && e.bucket == bucket) return e.t;
+    return AsmTune();
+}
+
+static int asm_block_size(int max_row);
+
+// Launches the value assembly of row block [g0, g1): a per-kind kernel when
 // the block has one form (hf = its host copy), the generic one otherwise.
+// Built-in choice for one form: the serial row walk up to 12 cell incidences
+// per row (triangles), the staged kernel above (tetrahedra), blocks in the
+// hardware's order.
 static void launch_assemble(const grmp_matrix *A, const grmp_space *sp, const DevForm *hf, FormPack P, int64_t g0,
                             int64_t g1, int B, size_t shm, double *val64, float *val32, double *dinv64, float *dinv32,
                             hipStream_t st)
 {
-    const dim3 grid(nblk(g1 - g0, B)), blk(B);
     const double inc_per_row = (double)sp->mesh->ncells * sp->nloc / (double)std::max<int64_t>(sp->ndofs, 1);
-    if (P.n == 1 && inc_per_row <= 12.0) {
-        const DevForm F = *hf;
-        GRMP_DISPATCH(F, hipLaunchKernelGGL((k_assemble1s<KT>), grid, blk, shm, st, F, g0, g1, g0, A->rowptr, A->diagpos,
-                                            A->fixed, A->penalty, val64, val32, dinv64, dinv32));
-        return;
-    }
+    const int mr = A->max_row > 0 ? A->max_row : 1;
     if (P.n == 1) {
+        const AsmTune T = asm_tune(A, *hf);
+        const int variant = T.variant ? T.variant : (inc_per_row <= 12.0 ? 1 : 2);
+        const int remap = T.remap >= 0 ? T.remap : 0;
+        if (variant == 1) {
+            // rows per block: a multiple of 64 whose rows fit the 48 KB of LDS
+            int Bs = asm_block_size(mr);
+            if (T.rows >= 64 && T.rows <= Bs) Bs = T.rows & ~63;
+            const DevForm F = *hf;
+            const size_t shs = (size_t)Bs * mr * sizeof(double);
+            GRMP_DISPATCH(F, hipLaunchKernelGGL((k_assemble1s<KT>), dim3(nblk(g1 - g0, Bs)), dim3(Bs), shs, st, F, remap, g0,
+                                                g1, g0, A->rowptr, A->diagpos, A->fixed, A->penalty, val64, val32,
+                                                dinv64, dinv32));
+            return;
+        }
         // rows per block: about 6 incidences per thread in a chunk, and the
         // block's rows (rb * max_row doubles) within 48 KB next to the 16 KB stage
         int rb = ASM_NT;
-        while (rb > 32 && ((double)rb * inc_per_row > 6.0 * ASM_NT || (size_t)rb * A->max_row * 8 > 48 * 1024)) rb >>= 1;
-        if ((size_t)rb * A->max_row * 8 <= 48 * 1024) {
+        while (rb > 32 && ((double)rb * inc_per_row > 6.0 * ASM_NT || (size_t)rb * mr * 8 > 48 * 1024)) rb >>= 1;
+        if (T.rows >= 8 && T.rows <= ASM_NT && (size_t)T.rows * mr * 8 <= 48 * 1024) rb = T.rows;
+        if ((size_t)rb * mr * 8 <= 48 * 1024) {
             const DevForm F = *hf;
-            const dim3 grid1(nblk(g1 - g0, rb)), blk1(ASM_NT);
-            const size_t shm1 = (ASM_STAGE + (size_t)rb * (A->max_row > 0 ? A->max_row : 1)) * sizeof(double);
-            GRMP_DISPATCH(F, hipLaunchKernelGGL((k_assemble1<KT>), grid1, blk1, shm1, st, F, rb, g0, g1, g0, A->rowptr,
-                                                A->diagpos, A->fixed, A->penalty, val64, val32, dinv64, dinv32));
+            const size_t shm1 = (ASM_STAGE + (size_t)rb * mr) * sizeof(double);
+            GRMP_DISPATCH(F, hipLaunchKernelGGL((k_assemble1<KT>), dim3(nblk(g1 - g0, rb)), dim3(ASM_NT), shm1, st, F, rb,
+                                                remap, g0, g1, g0, A->rowptr, A->diagpos, A->fixed, A->penalty, val64,
+                                                val32, dinv64, dinv32));
             return;
         }
     }
-    hipLaunchKernelGGL(k_assemble, grid, blk, shm, st, P, g0, g1, g0, A->rowptr, A->diagpos, A->fixed, A->penalty, val64,
-                       val32, dinv64, dinv32);
+    hipLaunchKernelGGL(k_assemble, dim3(nblk(g1 - g0, B)), dim3(B), shm, st, P, g0, g1, g0, A->rowptr, A->diagpos,
+                       A->fixed, A->penalty, val64, val32, dinv64, dinv32);
 }
 
 // ------------------------------------------------------------ matrix API
@@ -869,6 +1010,24 @@ int form_resolve(grmp_matrix *A, FormRec &r)
                f.quadorder <= 1 && f.coef_mode == GRMP_COEF_FACTOR) {
         r.kind = FK_DIV_BRP0;
         r.nq = 1;
+    } else if (f.kind == GRMP_BLF_NLDIFF && rs == cs && rs->ncomp == 1 && dim == 2 &&
+               (rs->fe == GRMP_FE_H1P1 || rs->fe == GRMP_FE_H1P2) && f.coef_mode == GRMP_COEF_FACTOR) {
+        GRMP_CHECK(ctx, f.current && f.current_offset >= 0 && f.current_offset + rs->ndofs <= f.current->n,
+                   GRMP_ERR_ARG, "Newton form: the iterate vector is missing or too short");
+        double xq[RECO_MAXQ][2], wq[RECO_MAXQ];
+        const int nqt = tri_rule(f.quadorder, xq, wq);
+        GRMP_CHECK(ctx, nqt > 0, GRMP_ERR_UNSUPPORTED, "Newton form: quadrature order %d too high", f.quadorder);
+        std::vector<double> tab(3 * nqt);
+        for (int q = 0; q < nqt; ++q) {
+            tab[3 * q] = xq[q][0];
+            tab[3 * q + 1] = xq[q][1];
+            tab[3 * q + 2] = wq[q];
+        }
+        GRMP_HIP(ctx, hipMalloc((void **)&r.qtab, sizeof(double) * tab.size()));
+        GRMP_HIP(ctx, hipMemcpy(r.qtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+        r.nqt = nqt;
+        r.kind = rs->fe == GRMP_FE_H1P1 ? FK_NLDIFF_P1TRI : FK_NLDIFF_P2TRI;
+        r.nq = 1;  // dispatch key; the points are in the table
     }
     GRMP_CHECK(ctx, (r.kind != FK_LAPLACE_BR && r.kind != FK_DIV_BRP0) || rs->mesh->fsign, GRMP_ERR_UNSUPPORTED,
                "H1BR forms need the mesh's CellFaceSigns (generated meshes carry them)");
@@ -909,6 +1068,12 @@ void fill_devform(const grmp_matrix *A, const FormRec &r, DevForm &d)
     d.ro = A->offs[r.f.row_block];
     d.co = A->offs[r.f.col_block];
     d.pos = r.pos;
+    if (r.kind == FK_NLDIFF_P1TRI || r.kind == FK_NLDIFF_P2TRI) {
+        d.uold = r.f.current->d + r.f.current_offset;
+        d.qtab = r.qtab;
+        d.nqt = r.nqt;
+        for (int k = 0; k < 3; ++k) d.nlc[k] = r.f.params[k];
+    }
 }
 
 extern "C" int grmp_matrix_create(grmp_ctx *ctx, int nblocks, const grmp_space *const *spaces, grmp_matrix **out)
@@ -949,6 +1114,8 @@ extern "C" int grmp_matrix_destroy(grmp_matrix *A)
     if (!A) return GRMP_OK;
     std::lock_guard<std::mutex> g(A->ctx->mu);
     matrix_free_pattern(A);
+    for (auto &r : A->forms)
+        if (r.qtab) (void)hipFree(r.qtab);
     dfree(A->ctx, A->fixed);
     delete A;
     return GRMP_OK;
@@ -1011,7 +1178,9 @@ extern "C" int grmp_matrix_set_fixed(grmp_matrix *A, int64_t nfixed, const int64
 
 // uploads the DevForms of every block row (in registration order) and
 // returns the device pack of block row rb
-static int upload_forms(grmp_matrix *A, const double *kappa_override = nullptr)
+// the pattern pass uploads Newton forms without their iterate: every local
+// pair then enters the pattern, whatever the iterate's values
+static int upload_forms(grmp_matrix *A, bool pattern, const double *kappa_override = nullptr)
 {
     grmp_ctx *ctx = A->ctx;
     std::vector<DevForm> h;
@@ -1023,6 +1192,7 @@ static int upload_forms(grmp_matrix *A, const double *kappa_override = nullptr)
                 DevForm d;
                 fill_devform(A, r, d);
                 if (kappa_override && d.coef_cell) d.kappa = kappa_override;
+                if (pattern) d.uold = nullptr;
                 h.push_back(d);
             }
     }
@@ -1073,7 +1243,7 @@ extern "C" int grmp_matrix_build_pattern(grmp_matrix *A)
         GRMP_HIP(ctx, hipMemsetAsync(A->fixed, 0, A->n, ctx->stream));
     }
     for (auto &r : A->forms) GRMP_TRY(dalloc(ctx, &r.pos, (size_t)r.rs->mesh->ncells * r.rs->nloc * r.words));
-    GRMP_TRY(upload_forms(A));
+    GRMP_TRY(upload_forms(A, true));
     int32_t *count = nullptr;
     GRMP_TRY(dalloc(ctx, &count, A->n));
     GRMP_TRY(dalloc(ctx, &A->rowptr, A->n + 1));
@@ -1135,6 +1305,11 @@ extern "C" int grmp_matrix_assemble(grmp_matrix *A, int values)
     if ((values & GRMP_VALUES_FP32) && !A->val32) GRMP_TRY(dalloc(ctx, &A->val32, A->nnz));
     if (!A->dinv64) GRMP_TRY(dalloc(ctx, &A->dinv64, A->n));
     if (!A->dinv32) GRMP_TRY(dalloc(ctx, &A->dinv32, A->n));
+    // Newton forms read the iterate: the device copies of the forms made for
+    // the pattern (without it) are refreshed for the multi-form kernel
+    bool newton = false;
+    for (auto &r : A->forms) newton |= r.qtab != nullptr;
+    if (newton) GRMP_TRY(upload_forms(A, false));
     const int B = asm_block_size(A->max_row);
     std::vector<DevForm> hf(A->forms.size() + 1);
     {
@@ -1214,6 +1389,37 @@ int assemble_values(const grmp_matrix *A, const double *kappa, DevForm *dforms, 
 }
 
 extern "C" double grmp_matrix_last_assembly_seconds(const grmp_matrix *A) { return A ? A->last_asm_ms * 1e-3 : -1.0; }
+
+extern "C" int grmp_matrix_set_tune(grmp_matrix *A, const grmp_asm_tune *t)
+{
+    if (!A) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = A->ctx;
+    if (!t) {
+        A->tune_set = false;
+        return GRMP_OK;
+    }
+    GRMP_CHECK(ctx, t->variant >= 0 && t->variant <= 2 && t->rows_per_block >= 0 && t->rows_per_block <= 1024 &&
+                        t->xcd_remap >= -1 && t->xcd_remap <= 1,
+               GRMP_ERR_ARG, "grmp_matrix_set_tune: variant 0..2, rows_per_block 0..1024, xcd_remap -1..1");
+    A->tune.variant = t->variant;
+    A->tune.rows = t->rows_per_block;
+    A->tune.remap = t->xcd_remap;
+    A->tune_set = true;
+    return GRMP_OK;
+}
+
+extern "C" int grmp_matrix_tune_key(const grmp_matrix *A, int *kind, int *nq, int *bucket)
+{
+    if (!A) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = A->ctx;
+    GRMP_CHECK(ctx, A->have_pattern, GRMP_ERR_ARG, "grmp_matrix_tune_key: no pattern built");
+    GRMP_CHECK(ctx, A->forms.size() == 1, GRMP_ERR_UNSUPPORTED,
+               "grmp_matrix_tune_key: launch shapes are tuned for single-form matrices");
+    if (kind) *kind = A->forms[0].kind;
+    if (nq) *nq = A->forms[0].nq;
+    if (bucket) *bucket = nnz_row_bucket(A->max_row);
+    return GRMP_OK;
+}
 
 extern "C" int grmp_matrix_info(const grmp_matrix *A, int64_t *n, int64_t *nnz, int *max_row)
 {
@@ -1391,13 +1597,171 @@ int linearform_launch(grmp_ctx *ctx, const grmp_space *s, const grmp_lform *form
     return GRMP_OK;
 }
 
+// ------------------------- polynomial data and the Newton right-hand side
+// LinearForm(Identity, f) with f a polynomial of degree <= 4 per component
+// (src/assemblypatterns/linearform.jl:181-220: per point the action's value
+// times the test function, summed over the points, then times
+// itemfactor = factor * vol), and the right-hand side full_assemble! adds for
+// the Newton form (src/assemblypatterns/nonlinearform.jl:190-200, 218-224):
+// (J [u, grad u] - kernel) . grad phi_a per point.  One thread per dof
+// walking its cells in cell order.
+#define LP_NCOEF 15
+struct LpDev {
+    int kind, fe, ncomp, per, nloc, nqt;
+    double factor;
+    double xq[RECO_MAXQ][2], wq[RECO_MAXQ];
+    double g[2][LP_NCOEF];
+    double nlc[3];
+    const double *coords;
+    const int32_t *cells, *celldofs;
+    int64_t nnodes, ncells;
+    const int32_t *inc_ptr, *inc;
+    const double *uold;
+};
+
+__device__ __forceinline__ double lp_poly(const double *c, double x, double y)
+{
+    double s = 0.0;
+    s += c[0];
+    s += c[1] * x;
+    s += c[2] * y;
+    s += c[3] * (x * x);
+    s += c[4] * (x * y);
+    s += c[5] * (y * y);
+    s += c[6] * (x * x * x);
+    s += c[7] * (x * x * y);
+    s += c[8] * (x * y * y);
+    s += c[9] * (y * y * y);
+    s += c[10] * (x * x * x * x);
+    s += c[11] * (x * x * x * y);
+    s += c[12] * (x * x * y * y);
+    s += c[13] * (x * y * y * y);
+    s += c[14] * (y * y * y * y);
+    return s;
+}
+
+template <int FE>
+__device__ __forceinline__ double lp_local(const LpDev &L, int64_t e, int a)
+{
+    constexpr int ND = (FE == 1) ? 3 : 6;
+    const int64_t nc = L.ncells, nn = L.nnodes;
+    const int32_t v0 = L.cells[e], v1 = L.cells[nc + e], v2 = L.cells[2 * nc + e];
+    const double X0 = L.coords[v0], Y0 = L.coords[nn + v0];
+    const double A11 = L.coords[v1] - X0, A12 = L.coords[v2] - X0;
+    const double A21 = L.coords[nn + v1] - Y0, A22 = L.coords[nn + v2] - Y0;
+    const double det = A11 * A22 - A12 * A21;
+    const double vol = fabs(det) * 0.5;
+    double lb = 0.0;
+    if (L.kind == GRMP_LF_POLY) {
+        const int c = a / L.per, s = a - c * L.per;
+        for (int q = 0; q < L.nqt; ++q) {
+            const double x0 = L.xq[q][0], x1 = L.xq[q][1];
+            double xp = 0.0, yp = 0.0;
+            xp += A11 * x0;
+            xp += A12 * x1;
+            xp += X0;
+            yp += A21 * x0;
+            yp += A22 * x1;
+            yp += Y0;
+            const double phi = FE == 1 ? p1tri_val(s, x0, x1) : p2tri_val(s, x0, x1);
+            double temp = 0.0;
+            for (int k = 0; k < L.ncomp; ++k) temp += lp_poly(L.g[k], xp, yp) * (k == c ? phi : 0.0);
+            lb += temp * L.wq[q];
+        }
+    } else {
+        // Newton right-hand side: A^{-T} as the matrix kernels form it
+        double M[2][2];
+        M[0][0] = A22 / det;
+        M[0][1] = -A21 / det;
+        M[1][0] = -A12 / det;
+        M[1][1] = A11 / det;
+        double uc[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) uc[d] = L.uold[L.celldofs[(int64_t)d * nc + e]];
+        for (int q = 0; q < L.nqt; ++q) {
+            const double x0 = L.xq[q][0], x1 = L.xq[q][1];
+            double in[3], J[2][3], val[2];
+            nl_point(uc, ND, FE, M, x0, x1, in);
+            nl_jac(L.nlc, in, J, val);
+            double ga[2];
+            grad_tri<FE>(M, a, x0, x1, ga);
+            double temp = 0.0;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                double ar = 0.0;
+                ar += J[k][0] * in[0];
+                ar += J[k][1] * in[1];
+                ar += J[k][2] * in[2];
+                temp += (ar - val[k]) * ga[k];
+            }
+            lb += temp * L.wq[q];
+        }
+    }
+    return lb * (L.factor * vol * 1.0);
+}
+
+template <int FE>
+__global__ void k_linearform_lp(LpDev L, double *__restrict__ b, int64_t ndofs)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndofs) return;
+    double acc = b[i];
+    for (int32_t t = L.inc_ptr[i]; t < L.inc_ptr[i + 1]; ++t) {
+        const int32_t pk = L.inc[t];
+        const int64_t e = pk / L.nloc;
+        acc += lp_local<FE>(L, e, pk - (int)e * L.nloc);
+    }
+    b[i] = acc;
+}
+
+static int linearform_lp_launch(grmp_ctx *ctx, const grmp_space *s, const grmp_lform *form, double *bp, hipStream_t st)
+{
+    const grmp_mesh *m = s->mesh;
+    GRMP_CHECK(ctx, m->dim == 2 && m->nloc == 3 && (s->fe == GRMP_FE_H1P1 || s->fe == GRMP_FE_H1P2), GRMP_ERR_UNSUPPORTED,
+               "linear form kind %d: P1/P2 on triangles only", form->kind);
+    GRMP_CHECK(ctx, s->inc_ptr && s->inc, GRMP_ERR_ARG, "linear form: space without dof-cell incidence");
+    LpDev L;
+    memset(&L, 0, sizeof(L));
+    L.kind = form->kind;
+    L.fe = s->fe;
+    L.ncomp = s->ncomp;
+    L.per = s->fe == GRMP_FE_H1P2 ? 6 : 3;
+    L.nloc = s->nloc;
+    L.nqt = tri_rule(form->quadorder, L.xq, L.wq);
+    GRMP_CHECK(ctx, L.nqt > 0, GRMP_ERR_UNSUPPORTED, "linear form: quadrature order %d too high", form->quadorder);
+    L.factor = form->factor;
+    if (form->kind == GRMP_LF_POLY) {
+        GRMP_CHECK(ctx, s->ncomp <= 2 && form->coef && form->ncoef >= 1 && form->ncoef <= LP_NCOEF, GRMP_ERR_ARG,
+                   "GRMP_LF_POLY: 1..%d coefficients per component, at most 2 components", LP_NCOEF);
+        for (int k = 0; k < s->ncomp; ++k)
+            for (int j = 0; j < form->ncoef; ++j) L.g[k][j] = form->coef[k * form->ncoef + j];
+    } else {
+        GRMP_CHECK(ctx, s->ncomp == 1 && form->current && form->current_offset >= 0 &&
+                            form->current_offset + s->ndofs <= form->current->n,
+                   GRMP_ERR_ARG, "GRMP_LF_NEWTON: a scalar space and its iterate vector are needed");
+        L.uold = form->current->d + form->current_offset;
+        for (int k = 0; k < 3; ++k) L.nlc[k] = form->f[k];
+    }
+    L.coords = m->coords;
+    L.cells = m->cells;
+    L.celldofs = s->celldofs;
+    L.nnodes = m->nnodes;
+    L.ncells = m->ncells;
+    L.inc_ptr = s->inc_ptr;
+    L.inc = s->inc;
+    const dim3 grid(nblk(s->ndofs, 256)), blk(256);
+    if (s->fe == GRMP_FE_H1P1) hipLaunchKernelGGL(k_linearform_lp<1>, grid, blk, 0, st, L, bp, s->ndofs);
+    else hipLaunchKernelGGL(k_linearform_lp<2>, grid, blk, 0, st, L, bp, s->ndofs);
+    GRMP_HIP(ctx, hipGetLastError());
+    return GRMP_OK;
+}
+
 // ------------------------------------------ reconstructed Stokes right-hand side
 // LinearForm(ReconstructionIdentity{HDIVRT0{2}}, f) on H1BR{2}
 // (src/feevaluator_h1.jl:342-381, src/reconstructions.jl:381-403): the test
 // function is replaced by its RT0 reconstruction sum_f rc[a][f] Piola(RT0_f),
 // rc = |F|/2 n_k(F) for the node dofs of face F, |F| for its bubble.
 
-#define RECO_MAXQ 25
 #define RECO_MAXMODES 16
 struct RecoDev {
     int nq, K;
@@ -1633,6 +1997,8 @@ extern "C" int grmp_assemble_linearform(grmp_ctx *ctx, const grmp_space *s, cons
     GRMP_CHECK(ctx, offset >= 0 && offset + s->ndofs <= b->n, GRMP_ERR_ARG, "grmp_assemble_linearform: vector too short");
     std::lock_guard<std::mutex> g(ctx->mu);
     if (form->kind == GRMP_LF_RECO_RT0) return linearform_reco_launch(ctx, s, form, b->d + offset, ctx->stream);
+    if (form->kind == GRMP_LF_POLY || form->kind == GRMP_LF_NEWTON)
+        return linearform_lp_launch(ctx, s, form, b->d + offset, ctx->stream);
     return linearform_launch(ctx, s, form, b->d + offset, ctx->stream);
 }
 
@@ -1870,11 +2236,10 @@ __global__ __launch_bounds__(1024) void k_sum_final(const double *__restrict__ p
     if (threadIdx.x == 0) out[0] = s;
 }
 
-extern "C" int grmp_integrate_l2error(grmp_ctx *ctx, const grmp_space *s, const grmp_vector *u, int64_t offset,
-                                      const double *coef, int ncoef, int quadorder, double factor, double *cellvals,
-                                      double *total)
+static int l2e_setup(grmp_ctx *ctx, const grmp_space *s, const grmp_vector *u, int64_t offset, const double *coef,
+                     int ncoef, int quadorder, double factor, L2eDev &L)
 {
-    GRMP_CHECK(ctx, ctx && s && u && (cellvals || total), GRMP_ERR_ARG, "grmp_integrate_l2error: null argument");
+    GRMP_CHECK(ctx, ctx && s && u, GRMP_ERR_ARG, "grmp_integrate_l2error: null argument");
     GRMP_CHECK(ctx, offset >= 0 && offset + s->ndofs <= u->n, GRMP_ERR_ARG, "grmp_integrate_l2error: vector too short");
     GRMP_CHECK(ctx, ncoef >= 0 && ncoef <= 10 && (ncoef == 0 || coef), GRMP_ERR_ARG,
                "grmp_integrate_l2error: 0..10 polynomial coefficients per component");
@@ -1883,7 +2248,6 @@ extern "C" int grmp_integrate_l2error(grmp_ctx *ctx, const grmp_space *s, const 
     GRMP_CHECK(ctx, ((s->fe == GRMP_FE_H1P1 || s->fe == GRMP_FE_H1P2) && s->ncomp <= 2) || s->fe == GRMP_FE_L2P0 ||
                         (s->fe == GRMP_FE_H1BR && m->fsign),
                GRMP_ERR_UNSUPPORTED, "grmp_integrate_l2error: H1P1/H1P2 (1-2 components), H1BR{2}, L2P0 only");
-    L2eDev L;
     memset(&L, 0, sizeof(L));
     L.nq = tri_rule(quadorder, L.xq, L.wq);
     GRMP_CHECK(ctx, L.nq > 0, GRMP_ERR_UNSUPPORTED, "grmp_integrate_l2error: quadrature order %d too high", quadorder);
@@ -1899,11 +2263,16 @@ extern "C" int grmp_integrate_l2error(grmp_ctx *ctx, const grmp_space *s, const 
     L.fsign = m->fsign;
     L.nnodes = m->nnodes;
     L.ncells = m->ncells;
-    std::lock_guard<std::mutex> g(ctx->mu);
-    const int64_t nc = m->ncells, nb = (nc + 1023) / 1024;
-    double *d = nullptr;
-    GRMP_TRY(dalloc(ctx, &d, nc + nb + 1));
-    hipLaunchKernelGGL(k_l2error, dim3(nblk(nc, 256)), dim3(256), 0, ctx->stream, L, u->d + offset, d);
+    return GRMP_OK;
+}
+
+// per-cell values into d[0..nc), block partials into d[nc..nc+nb), the total
+// into d[nc+nb], then both copied to the host; every element of d is written
+// before it is read, so d needs no clearing
+static int l2e_run(grmp_ctx *ctx, const L2eDev &L, const double *u, double *d, double *cellvals, double *total)
+{
+    const int64_t nc = L.ncells, nb = (nc + 1023) / 1024;
+    hipLaunchKernelGGL(k_l2error, dim3(nblk(nc, 256)), dim3(256), 0, ctx->stream, L, u, d);
     hipLaunchKernelGGL(k_sum_blocks, dim3((unsigned)nb), dim3(1024), 0, ctx->stream, d, nc, d + nc);
     hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(1024), 0, ctx->stream, d + nc, nb, d + nc + nb);
     hipError_t e = hipGetLastError();
@@ -1912,9 +2281,24 @@ extern "C" int grmp_integrate_l2error(grmp_ctx *ctx, const grmp_space *s, const 
     if (e == hipSuccess && total)
         e = hipMemcpyAsync(total, d + nc + nb, sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    dfree(ctx, d);
     GRMP_HIP(ctx, e);
     return GRMP_OK;
+}
+
+extern "C" int grmp_integrate_l2error(grmp_ctx *ctx, const grmp_space *s, const grmp_vector *u, int64_t offset,
+                                      const double *coef, int ncoef, int quadorder, double factor, double *cellvals,
+                                      double *total)
+{
+    GRMP_CHECK(ctx, ctx && (cellvals || total), GRMP_ERR_ARG, "grmp_integrate_l2error: no output");
+    L2eDev L;
+    GRMP_TRY(l2e_setup(ctx, s, u, offset, coef, ncoef, quadorder, factor, L));
+    std::lock_guard<std::mutex> g(ctx->mu);
+    const int64_t nc = L.ncells, nb = (nc + 1023) / 1024;
+    double *d = nullptr;
+    GRMP_TRY(dalloc(ctx, &d, nc + nb + 1));
+    const int rc = l2e_run(ctx, L, u->d + offset, d, cellvals, total);
+    dfree(ctx, d);
+    return rc;
 }
 
 // ------------------------------------------------------------ CellVolumes

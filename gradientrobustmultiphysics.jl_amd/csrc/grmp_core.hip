@@ -5,7 +5,9 @@
 // (src/finiteelements.jl:72-112, src/dofmaps.jl:201-363).  Everything is
 // stored structure-of-arrays so that a thread per cell or per dof reads its
 // entries with coalesced loads.
+#include <cstdlib>
 #include <cstring>
+#include <dlfcn.h>
 
 #include <rocprim/rocprim.hpp>
 
@@ -46,6 +48,85 @@ int ensure_scratch(grmp_ctx *ctx, size_t bytes)
 
 extern "C" int grmp_version(void) { return 100; }
 
+// ------------------------------------------------------------ tuning table
+// The autotuner writes one JSON object per entry, with integer fields
+// "kind", "nq", "bucket", "variant", "rows", "remap" (other fields, such as
+// names and timings, are ignored).  The reader takes every {...} object
+// after "entries" that holds the three key fields.
+static bool json_int(const std::string &obj, const char *key, long &out)
+{
+    const std::string k = std::string("\"") + key + "\"";
+    size_t p = obj.find(k);
+    if (p == std::string::npos) return false;
+    p = obj.find(':', p + k.size());
+    if (p == std::string::npos) return false;
+    char *end = nullptr;
+    out = std::strtol(obj.c_str() + p + 1, &end, 10);
+    return end != obj.c_str() + p + 1;
+}
+
+static int tune_parse(grmp_ctx *ctx, const std::string &text, std::vector<TuneEntry> &out)
+{
+    size_t p = text.find("\"entries\"");
+    GRMP_CHECK(ctx, p != std::string::npos, GRMP_ERR_ARG, "tuning table: no \"entries\" list");
+    while ((p = text.find('{', p)) != std::string::npos) {
+        const size_t q = text.find('}', p);
+        if (q == std::string::npos) break;
+        const std::string obj = text.substr(p, q - p + 1);
+        long kind, nq, bucket, v = 0, rows = 0, remap = -1;
+        if (json_int(obj, "kind", kind) && json_int(obj, "nq", nq) && json_int(obj, "bucket", bucket)) {
+            json_int(obj, "variant", v);
+            json_int(obj, "rows", rows);
+            json_int(obj, "remap", remap);
+            GRMP_CHECK(ctx, v >= 0 && v <= 2 && rows >= 0 && rows <= 1024 && remap >= -1 && remap <= 1, GRMP_ERR_ARG,
+                       "tuning table: entry out of range in %s", obj.c_str());
+            TuneEntry e;
+            e.kind = (int)kind;
+            e.nq = (int)nq;
+            e.bucket = (int)bucket;
+            e.t.variant = (int)v;
+            e.t.rows = (int)rows;
+            e.t.remap = (int)remap;
+            out.push_back(e);
+        }
+        p = q + 1;
+    }
+    return GRMP_OK;
+}
+
+// the table shipped beside the library (tuning_gfx950.json), or $GRMP_TUNING
+static std::string tune_default_path()
+{
+    if (const char *env = std::getenv("GRMP_TUNING")) return env;
+    Dl_info info;
+    if (dladdr((void *)&grmp_version, &info) && info.dli_fname) {
+        std::string so = info.dli_fname;
+        const size_t s = so.rfind('/');
+        return (s == std::string::npos ? std::string(".") : so.substr(0, s)) + "/tuning_gfx950.json";
+    }
+    return "tuning_gfx950.json";
+}
+
+extern "C" int grmp_tune_load(grmp_ctx *ctx, const char *path)
+{
+    GRMP_CHECK(ctx, ctx, GRMP_ERR_ARG, "grmp_tune_load: null context");
+    const std::string fn = path ? std::string(path) : tune_default_path();
+    FILE *f = std::fopen(fn.c_str(), "rb");
+    GRMP_CHECK(ctx, f, GRMP_ERR_ARG, "grmp_tune_load: cannot open %s", fn.c_str());
+    std::string text;
+    char buf[4096];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, n);
+    std::fclose(f);
+    std::vector<TuneEntry> t;
+    GRMP_TRY(tune_parse(ctx, text, t));
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->tune = t;
+    return GRMP_OK;
+}
+
+extern "C" int grmp_tune_count(const grmp_ctx *ctx) { return ctx ? (int)ctx->tune.size() : -1; }
+
 extern "C" int grmp_ctx_create(int device, grmp_ctx **out)
 {
     if (!out) return GRMP_ERR_ARG;
@@ -63,6 +144,12 @@ extern "C" int grmp_ctx_create(int device, grmp_ctx **out)
         (void)hipStreamDestroy(c->stream);
         delete c;
         return GRMP_ERR_OOM;
+    }
+    // the shipped tuning table, when present (a missing file keeps the built-in launch shapes)
+    const std::string fn = tune_default_path();
+    if (FILE *f = std::fopen(fn.c_str(), "rb")) {
+        std::fclose(f);
+        if (grmp_tune_load(c, fn.c_str()) != GRMP_OK) c->tune.clear();
     }
     *out = c;
     return GRMP_OK;
@@ -193,6 +280,22 @@ extern "C" int grmp_vector_download(const grmp_vector *v, double *host)
     grmp_ctx *ctx = v->ctx;
     GRMP_HIP(ctx, hipMemcpyAsync(host, v->d, sizeof(double) * v->n, hipMemcpyDeviceToHost, ctx->stream));
     GRMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GRMP_OK;
+}
+
+// page-locked host memory: device-to-host copies into it run as DMA without
+// a staging copy, so a served solution lands on the host at link speed
+extern "C" int grmp_host_alloc(grmp_ctx *ctx, size_t bytes, void **out)
+{
+    GRMP_CHECK(ctx, ctx && out, GRMP_ERR_ARG, "grmp_host_alloc: null argument");
+    *out = nullptr;
+    GRMP_HIP(ctx, hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault));
+    return GRMP_OK;
+}
+
+extern "C" int grmp_host_free(void *p)
+{
+    if (p && hipHostFree(p) != hipSuccess) return GRMP_ERR_HIP;
     return GRMP_OK;
 }
 

@@ -10,6 +10,25 @@
 
 #include "../../include/grmp.h"
 
+// ------------------------------------------------------------------ tuning
+// Launch shape of the element assembly of one form kind: the kernel variant
+// (1: one thread walks a row's cells; 2: the block's cell work staged in LDS
+// over all threads), the rows one workgroup owns, and whether consecutive row
+// blocks go to one XCD (1) or round-robin over the XCDs (0, the hardware's
+// dispatch order).  Zero fields mean the built-in choice.  Results do not
+// depend on it: every entry is summed in the same order by every variant.
+struct AsmTune {
+    int variant = 0;
+    int rows = 0;
+    int remap = -1;   // -1: built-in
+};
+// tuning table entry, keyed by (form kind, quadrature points, nnz/row bucket)
+struct TuneEntry {
+    int kind = 0, nq = 0, bucket = 0;
+    AsmTune t;
+};
+int nnz_row_bucket(int max_row);  // 0: <= 8, 1: <= 16, 2: <= 32, 3: <= 64, 4: above
+
 // ------------------------------------------------------------------ errors
 struct grmp_ctx {
     int device = 0;
@@ -20,6 +39,7 @@ struct grmp_ctx {
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     int *dflag = nullptr;  // device error/overflow flag
+    std::vector<TuneEntry> tune;  // assembly launch shapes (tuning table)
 };
 
 int grmp_fail(grmp_ctx *ctx, int code, const char *fmt, ...);
@@ -111,6 +131,8 @@ enum FormKind {
     FK_LAPLACE_BR = 7,   // (nu grad u, grad v) on H1BR{2}
     FK_DIV_BRP0 = 8,     // LagrangeMultiplier(Divergence): rows BR, columns P0
     FK_DIVT_P0BR = 9,    // its transposed copy: rows P0, columns BR
+    FK_NLDIFF_P1TRI = 10, // Newton matrix of alpha(u) grad u on P1 triangles
+    FK_NLDIFF_P2TRI = 11, // ... on P2 triangles
 };
 
 struct DevForm {
@@ -132,6 +154,13 @@ struct DevForm {
     const int32_t *cdofs;
     int64_t ro, co;      // global offsets of row / column block
     uint32_t *pos;       // slot map (uint8 positions relative to the row start)
+    // Newton forms: the iterate at this space's dofs (nullptr while the
+    // pattern is built: then every local pair counts), the quadrature table
+    // (x0, x1, w per point) and alpha's coefficients
+    const double *uold;
+    const double *qtab;
+    int nqt, pad2_;
+    double nlc[3];
 };
 
 struct FormRec {
@@ -141,6 +170,8 @@ struct FormRec {
     int nq;
     uint32_t *pos = nullptr;
     int words = 0;
+    double *qtab = nullptr;  // Newton forms: device quadrature table
+    int nqt = 0;
 };
 
 struct grmp_matrix {
@@ -169,6 +200,8 @@ struct grmp_matrix {
     float last_asm_ms = 0.f;
     DevForm *dforms = nullptr;      // device copy of the forms, grouped by block row
     std::vector<int> form_first;    // first form of each block row in dforms
+    bool tune_set = false;          // a launch shape set on this matrix overrides the table
+    AsmTune tune;
 };
 
 // ------------------------------------------------------------- internals

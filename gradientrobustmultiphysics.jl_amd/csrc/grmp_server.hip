@@ -347,10 +347,9 @@ static void job_finish(grmp_server *S, Job *j, const KProb &q)
     j->res.relres = q.relres;
     j->res.fp64_fallback = q.fp64;
     j->res.ndofs = j->n;
-    if (j->req.x_out) {
-        (void)hipMemcpyAsync(j->req.x_out, j->x, j->n * 8, hipMemcpyDeviceToHost, S->st_solve);
-        (void)hipStreamSynchronize(S->st_solve);
-    }
+    if (j->req.x_out && (hipMemcpyAsync(j->req.x_out, j->x, j->n * 8, hipMemcpyDeviceToHost, S->st_solve) != hipSuccess ||
+                         hipStreamSynchronize(S->st_solve) != hipSuccess) && j->res.status == GRMP_OK)
+        j->res.status = GRMP_ERR_HIP;
     S->eng->remove(j->slot);
     job_release(S, j);
     j->res.t_done = now(S);
@@ -634,9 +633,12 @@ static void mg_finish(grmp_server *S, Job *j, const MgStatus h, grmp_ctx *c)
     j->res.relres = h.relres;
     j->res.fp64_fallback = 0;
     j->res.ndofs = j->n;
-    if (j->req.x_out)
-        (void)hipMemcpyAsync(j->req.x_out, mg_job_solution(j->mg), j->n * 8, hipMemcpyDeviceToHost, c->stream);
-    (void)hipStreamSynchronize(c->stream);
+    // the solution's copy to the host is part of the request: a failed copy fails it
+    if (j->req.x_out &&
+        hipMemcpyAsync(j->req.x_out, mg_job_solution(j->mg), j->n * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess &&
+        j->res.status == GRMP_OK)
+        j->res.status = GRMP_ERR_HIP;
+    if (hipStreamSynchronize(c->stream) != hipSuccess && j->res.status == GRMP_OK) j->res.status = GRMP_ERR_HIP;
     double sec = 0.0, bytes = 0.0;
     int64_t nl = 0;
     mg_job_profile(j->mg, &sec, &bytes, &nl);

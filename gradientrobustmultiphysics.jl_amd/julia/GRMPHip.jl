@@ -3,10 +3,15 @@ GRMPHip.jl -- Julia binding of libgrmp_hip.so for GradientRobustMultiPhysics.jl.
 
 Loaded next to GradientRobustMultiPhysics, it
   * adds `GRMPKrylov`, a LinearSolve algorithm that solve! accepts through its
-    `linsolver` keyword (src/solvers.jl:55, 89-101, 656-660), and
-  * adds `hip_assemble!` methods for the pattern-level assemble! of the
-    BilinearForm / LinearForm kinds the library builds
-    (src/assemblypatterns/bilinearform.jl:384-400, linearform.jl:239-251).
+    `linsolver` keyword (src/solvers.jl:55, 89-101, 656-660): `init` builds a
+    LinearCache with it (`init_cacheval`), and `LinearSolve.solve(cache)`
+    reaches `SciMLBase.solve!(cache, alg::GRMPKrylov)` below;
+  * adds `HIPOperator <: AbstractPDEOperator` (src/pdeoperators.jl:23-56), an
+    operator a PDEDescription holds like LaplaceOperator: `assemble!`,
+    `check_PDEoperator` and `check_dependency` are implemented, so
+    solve!/assemble! of the PDE (src/solvers.jl:533,568-573,595) assemble its
+    block on the GPU;
+  * keeps `hip_assemble!` as the block-level call both use.
 
 Every call passes plain pointers: xgrid[Coordinates] ([dim, nnodes]),
 xgrid[CellNodes] ([nloc, ncells]) and FES[CellDofs] ([ndofs_loc, ncells]) in
@@ -179,13 +184,57 @@ function hip_assemble!(A::FEMatrixBlock, kind::Cint, quadorder::Int; factor = 1.
                      m[], colptr, rowval, nzval))
     ccall((:grmp_matrix_destroy, libgrmp), Cint, (Ptr{Cvoid},), m[])
     kv[] != C_NULL && ccall((:grmp_vector_destroy, libgrmp), Cint, (Ptr{Cvoid},), kv[])
-    # add into the block (same entries the reference's _addnz would create)
-    B = SparseMatrixCSC(n[], n[], colptr, rowval, nzval)
-    rows, cols, vals = findnz(B)
-    for k in eachindex(vals)
-        rawupdateindex!(A.entries, +, vals[k], rows[k] + A.offsetX, cols[k] + A.offsetY)
-    end
+    # add into the block: the device's pattern is the one _addnz would create;
+    # one sparse addition at the CSC level (shifted into the block's offsets)
+    # instead of an update call per entry
     flush!(A.entries)
+    N = size(A.entries.cscmatrix, 1)
+    rows, cols, vals = findnz(SparseMatrixCSC(n[], n[], colptr, rowval, nzval))
+    B = sparse(rows .+ A.offsetX, cols .+ A.offsetY, vals, N, size(A.entries.cscmatrix, 2))
+    A.entries.cscmatrix = A.entries.cscmatrix + B
+    return nothing
+end
+
+# ------------------------------------------ operator plug-in (PDEDescription)
+"""
+    HIPOperator(kind; factor = 1.0, kappa = nothing, params = (0,0,0,0), name)
+
+An AbstractPDEOperator (src/pdeoperators.jl:23-56) whose matrix block is
+assembled by libgrmp_hip.so.  `kind`: `:laplace` (LaplaceOperator(factor),
+or the per-cell action BilinearForm([Gradient, Gradient], kappa) when
+`kappa` is a vector over the cells), `:reaction` (ReactionOperator(factor)),
+`:hooke2d` (HookStiffnessOperator2D with params = (lambda, mu, 0, 0),
+optionally scaled per cell by `kappa`).  Linear and time-independent:
+assembled once (AssemblyInitial).
+"""
+struct HIPOperator <: GradientRobustMultiPhysics.AbstractPDEOperator
+    name::String
+    kind::Symbol
+    factor::Float64
+    kappa::Union{Nothing,Vector{Float64}}
+    params::NTuple{4,Float64}
+end
+HIPOperator(kind::Symbol; factor = 1.0, kappa = nothing, params = (0.0, 0.0, 0.0, 0.0), name = "HIP $(kind)") =
+    HIPOperator(name, kind, factor, kappa === nothing ? nothing : Vector{Float64}(kappa), params)
+
+GradientRobustMultiPhysics.check_PDEoperator(O::HIPOperator, involved_equations) =
+    (false, false, GradientRobustMultiPhysics.AssemblyInitial)
+GradientRobustMultiPhysics.check_dependency(O::HIPOperator, arg::Int) = false
+
+# quadrature order of prepare_assembly! (src/assemblypatterns.jl:559-563):
+# polynomial orders of both arguments plus the operators' shifts (-1 per gradient)
+function _quadorder(O::HIPOperator, FES)
+    EG = FES.xgrid[UniqueCellGeometries][1]
+    p = GradientRobustMultiPhysics.get_polynomialorder(eltype(FES), EG)
+    return O.kind == :reaction ? 2 * p : 2 * (p - 1)
+end
+
+function GradientRobustMultiPhysics.assemble!(A::FEMatrixBlock, SC, j::Int, k::Int, o::Int, O::HIPOperator,
+                                              CurrentSolution::FEVector; time = 0, At = nothing)
+    kind = O.kind == :laplace ? GRMP_BLF_LAPLACE : O.kind == :reaction ? GRMP_BLF_MASS :
+           O.kind == :hooke2d ? GRMP_BLF_HOOKE2D : error("HIPOperator: unknown kind $(O.kind)")
+    hip_assemble!(A, kind, _quadorder(O, A.FESX); factor = O.factor, kappa = O.kappa, params = O.params)
+    At === nothing || error("HIPOperator: transposed copies are not built")
     return nothing
 end
 
@@ -238,10 +287,16 @@ struct GRMPKrylov <: LinearSolve.SciMLLinearSolveAlgorithm
 end
 GRMPKrylov(; precision = :fp32_ir, rtol = 1e-10) = GRMPKrylov(precision, rtol)
 
-function LinearSolve.solve(cache::LinearSolve.LinearCache, alg::GRMPKrylov; kwargs...)
+# no factorization or Krylov state lives on the host: the cache value is empty
+LinearSolve.init_cacheval(alg::GRMPKrylov, A, b, u, Pl, Pr, maxiters::Int, abstol, reltol, verbose::Bool,
+                          assumptions::LinearSolve.OperatorAssumptions) = nothing
+
+# LinearSolve 2 dispatches solve(cache) -> SciMLBase.solve!(cache, cache.alg)
+function LinearSolve.SciMLBase.solve!(cache::LinearSolve.LinearCache, alg::GRMPKrylov; kwargs...)
     A::SparseMatrixCSC{Float64,Int64} = cache.A
     b::Vector{Float64} = cache.b
-    x = zeros(length(b))
+    x = cache.u
+    fill!(x, 0.0)
     n = size(A, 1)
     fixed = [j for j in 1:n if A[j, j] >= 1e59]
     # CSC of A is CSR of A^T; the systems solved here are symmetric
@@ -252,5 +307,7 @@ function LinearSolve.solve(cache::LinearSolve.LinearCache, alg::GRMPKrylov; kwar
     check(context(), rc)
     return LinearSolve.SciMLBase.build_linear_solution(alg, x, nothing, cache)
 end
+
+export GRMPKrylov, HIPOperator
 
 end # module

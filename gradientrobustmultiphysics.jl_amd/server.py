@@ -63,9 +63,15 @@ class Server:
         self.h = h
         self._outs = {}
 
-    def submit(self, kind, fe, N, seed, sigma=1.0, rtol=1e-10, keep_solution=True, **kw):
-        """Returns a ticket; the solution lands in a host array on completion."""
-        x = np.zeros(ndofs(kind, fe, N)) if keep_solution else None
+    def submit(self, kind, fe, N, seed, sigma=1.0, rtol=1e-10, keep_solution=True, out=None, **kw):
+        """Returns a ticket; the solution lands in a host array on completion:
+        ``out`` (e.g. a PinnedBuffer view of ndofs doubles) or a new array."""
+        if out is not None:
+            if out.dtype != np.float64 or out.size != ndofs(kind, fe, N) or not out.flags.c_contiguous:
+                raise _abi.GrmpError(_abi.ERR_ARG, "submit: out must be a contiguous float64 array of ndofs entries")
+            x = out
+        else:
+            x = np.zeros(ndofs(kind, fe, N)) if keep_solution else None
         r = request(kind, fe, N, seed, sigma, rtol, x_out=x, **kw)
         t = ctypes.c_int64()
         self.ctx.check(lib().grmp_server_submit(self.h, ctypes.byref(r), ctypes.byref(t)))

@@ -52,6 +52,12 @@ extern "C" {
 #define GRMP_BLF_MASS 3         /* (kappa u, v): ReactionOperator :176-203 */
 #define GRMP_BLF_DIVP0 4        /* LagrangeMultiplier(Divergence) :214-226 (with
                                    transposed copy) */
+#define GRMP_BLF_NLDIFF 5       /* NonlinearForm(Gradient, [Identity, Gradient], [1,1], kernel, [2,3])
+                                   (:590-635) with kernel(u, grad u) = alpha(u) grad u,
+                                   alpha(u) = params[0] + params[1] u + params[2] u^2: the Newton matrix
+                                   of full_assemble! (src/assemblypatterns/nonlinearform.jl:44-246, rows
+                                   = test functions as transposed_assembly sets it) at the iterate
+                                   `current`; its pattern holds every local (test, ansatz) pair */
 
 /* how the coefficient enters (src/assemblypatterns/bilinearform.jl:306-313) */
 #define GRMP_COEF_FACTOR 0      /* NoAction: constant coefficient multiplies itemfactor */
@@ -61,6 +67,11 @@ extern "C" {
 #define GRMP_LF_CONST 1         /* LinearForm(Identity, DataFunction(f)) :694-705 */
 #define GRMP_LF_RECO_RT0 2      /* LinearForm(ReconstructionIdentity{HDIVRT0{2}}, f) on H1BR{2},
                                    f = grad(phi) + curl(psi) (x-dependent, see grmp_lform) */
+#define GRMP_LF_POLY 3          /* LinearForm(Identity, f) with f a polynomial of degree <= 4 per
+                                   component (coef, ncoef) on P1/P2 triangles */
+#define GRMP_LF_NEWTON 4        /* the right-hand side full_assemble! adds for GRMP_BLF_NLDIFF at
+                                   `current` (nonlinearform.jl:190-200, 218-224):
+                                   (J [u, grad u] - kernel(u, grad u)) . grad v, alpha from f[0..2] */
 
 /* value precision flags of an assembled matrix */
 #define GRMP_VALUES_FP64 1
@@ -69,6 +80,7 @@ extern "C" {
 /* solver choices (replaces the LinearSolve/UMFPACK call, src/solvers.jl:656-660) */
 #define GRMP_SOLVE_CG 1
 #define GRMP_SOLVE_MINRES 2
+#define GRMP_SOLVE_BICGSTAB 3   /* nonsymmetric systems (Newton matrices), FP64, Jacobi */
 #define GRMP_PREC_JACOBI 1
 #define GRMP_PRECISION_FP64 1   /* plain FP64 Krylov */
 #define GRMP_PRECISION_FP32_IR 2/* FP32 matrix/SpMV/preconditioner inside FP64 refinement */
@@ -90,6 +102,8 @@ typedef struct grmp_form {
     double factor;            /* PDEOperator factor (NoAction coefficient) */
     double params[4];         /* Hooke: lambda, mu */
     const grmp_vector *cell_coef; /* per-cell kappa for GRMP_COEF_CELL, else NULL */
+    const grmp_vector *current;   /* GRMP_BLF_NLDIFF: the iterate (read at every assemble) */
+    int64_t current_offset;       /* its block offset */
 } grmp_form;
 
 typedef struct grmp_lform {
@@ -102,12 +116,19 @@ typedef struct grmp_lform {
     const double *modes;      /* [2*nmodes][4] = (amplitude, p, q, phase): A sin(2 pi (p x + q y) + t),
                                  phi's modes first; host pointer, copied */
     double poly[4];           /* phi += poly0 x^3 + poly1 x y^2 + poly2 y^2 */
+    const grmp_vector *current;   /* GRMP_LF_NEWTON: the iterate */
+    int64_t current_offset;
+    const double *coef;       /* GRMP_LF_POLY: [ncomp][ncoef] over 1, x, y, x^2, xy, y^2, x^3, x^2 y,
+                                 x y^2, y^3, x^4, x^3 y, x^2 y^2, x y^3, y^4; host pointer, copied */
+    int ncoef;                /* <= 15 */
+    int pad2_;
 } grmp_lform;
 
 typedef struct grmp_solve_opts {
     int method;               /* GRMP_SOLVE_CG (symmetric positive definite systems) /
                                  GRMP_SOLVE_MINRES (saddle points: free rows without a diagonal
-                                 are constraint rows, negated to make the system symmetric) */
+                                 are constraint rows, negated to make the system symmetric) /
+                                 GRMP_SOLVE_BICGSTAB (nonsymmetric, FP64 only) */
     int precond;              /* GRMP_PREC_JACOBI (MINRES: 1/|A_ii|, and on constraint rows
                                  1 / sum_j A_ij^2 / A_jj) */
     int precision;            /* GRMP_PRECISION_FP64 / GRMP_PRECISION_FP32_IR */
@@ -145,6 +166,11 @@ int64_t grmp_vector_length(const grmp_vector *v);
 int grmp_vector_upload(grmp_vector *v, const double *host);
 int grmp_vector_download(const grmp_vector *v, double *host);
 int grmp_vector_fill(grmp_vector *v, double value);
+/* page-locked host buffers for results (the solution arrays a Julia caller
+ * wraps with unsafe_wrap): device-to-host copies into them need no staging.
+ * No counterpart in the reference, which keeps every array on the host. */
+int grmp_host_alloc(grmp_ctx *ctx, size_t bytes, void **out);
+int grmp_host_free(void *p);
 /* v[idx[k]] = values[k] (idx 1-based); the b[fixed] = penalty*target write of
  * solve_direct! (src/solvers.jl:649-650) */
 int grmp_vector_scatter(grmp_vector *v, int64_t count, const int64_t *idx, const double *values);
@@ -227,6 +253,28 @@ int grmp_matrix_export_csc(const grmp_matrix *A, int64_t *colptr, int64_t *rowva
 int grmp_matrix_spmv(grmp_matrix *A, int values, const grmp_vector *x, grmp_vector *y);
 /* seconds of the last assemble call's kernels (HIP events) */
 double grmp_matrix_last_assembly_seconds(const grmp_matrix *A);
+
+/* Launch shape of the element assembly (no counterpart in the reference,
+ * whose element loop is serial): variant 1 walks each row's cells in one
+ * thread, 2 stages the row block's cell work over all threads of a
+ * workgroup; rows_per_block the rows one workgroup owns; xcd_remap 1 gives
+ * each XCD a contiguous share of the row blocks.  0 (-1 for xcd_remap)
+ * keeps the built-in choice.  Values are bitwise the same for every shape.
+ * grmp_matrix_set_tune overrides the context's table for one matrix (NULL
+ * resets); grmp_matrix_tune_key returns the table key of a single-form
+ * matrix: form kind, quadrature points and nnz/row bucket (<= 8, 16, 32, 64,
+ * above).  grmp_tune_load reads a table written by the autotuner (JSON; path
+ * NULL: tuning_gfx950.json beside the library or $GRMP_TUNING), which
+ * grmp_ctx_create also loads when present; grmp_tune_count is its size. */
+typedef struct grmp_asm_tune {
+    int32_t variant;
+    int32_t rows_per_block;
+    int32_t xcd_remap;
+} grmp_asm_tune;
+int grmp_matrix_set_tune(grmp_matrix *A, const grmp_asm_tune *t);
+int grmp_matrix_tune_key(const grmp_matrix *A, int *kind, int *nq, int *bucket);
+int grmp_tune_load(grmp_ctx *ctx, const char *path);
+int grmp_tune_count(const grmp_ctx *ctx);
 
 /* The right-hand side of one BackwardEuler step of advance!
  * (src/solvers_timedependent.jl:339-359 with the penalty of :452-461):

@@ -158,10 +158,16 @@ def test_nonconvergence_reported(grmp, O):
     assert ei.value.code == grmp._abi.ERR_NOCONV
     # the device stops after exactly maxit iterations (its scalars and the
     # stop test live on the device, the host reads them every 8 iterations)
+    import ctypes
     A, b, fixed = grmp.assemble(pde, sol)
     for maxit in (5, 8, 13):
-        info = grmp.solve_system(A, b, sol, grmp.SolverConfig(precision="fp64", rtol=1e-30, maxiterations=maxit))
-        assert info.status == grmp._abi.ERR_NOCONV and info.iterations == maxit, (maxit, info.iterations)
+        o = grmp._abi.SolveOpts()
+        grmp.lib().grmp_solve_opts_default(ctypes.byref(o))
+        o.method, o.precision = grmp._abi.SOLVE_CG, grmp._abi.PRECISION_FP64
+        o.rtol, o.maxit, o.check_every = 1e-30, maxit, 1
+        info = grmp._abi.SolveInfo()
+        rc = grmp.lib().grmp_solve(A.h, b.vec.h, sol.vec.h, ctypes.byref(o), ctypes.byref(info))
+        assert rc == grmp._abi.ERR_NOCONV and info.iterations == maxit, (maxit, rc, info.iterations)
 
 
 def test_plain_cg_iterations_pinned(grmp, O):
