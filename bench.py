@@ -510,6 +510,83 @@ def run_workload(grmp, which, args, log):
                                               "kind": "port",
                                               "sample": f"256x256 P1 ({S1.nnodes} nodes), cell-order averages; "
                                                         f"{tn:.2f} s"}
+    elif which == "nonlinear":
+        # -div((1 + u^2) grad u) = f with u = x y (examples/Example205_NonlinearPoisson2D.jl, q = 1) on the
+        # N x N square: Newton steps of the fixpoint loop, each a Newton-matrix and right-hand-side
+        # assembly at the iterate plus a BiCGStab solve
+        N = args.size or 256
+        fe = 2
+        grid = grmp.grid_unitsquare(N)
+        fes = grmp.FESpace(grmp.H1P2(1), grid)
+        pde = grmp.PDEDescription("nonlinear Poisson problem")
+        pde.add_unknown("u")
+        nlf = grmp.NonlinearForm(grmp.Gradient, [grmp.Identity, grmp.Gradient], [0, 0],
+                                 grmp.NonlinearDiffusion(1.0, 0.0, 1.0), [2, 3], bonus_quadorder=2)
+        pde.add_operator((0, 0), nlf)
+        pde.add_boundarydata(0, grmp.InterpolateDirichletBoundary, data=grmp.PolynomialData([0, 0, 0, 0, 1.0]))
+        F = [0.0] * 15
+        F[11], F[13] = -2.0, -2.0
+        pde.add_rhsdata(0, grmp.LinearForm(grmp.Identity, grmp.PolynomialData(F)))
+        sol = grmp.FEVector([fes])
+        log(f"[bench] nonlinear: P2 {N}x{N}, {fes.ndofs} dofs")
+        t0 = time.perf_counter()
+        info, A, b = grmp.solve(sol, pde, target_residual=1e-10, nonlinear_maxiterations=12, rtol=1e-12)
+        ctx.sync()
+        t = time.perf_counter() - t0
+        A.assemble()
+        ts = []
+        for _ in range(3):
+            A.assemble()
+            ts.append(A.last_assembly_seconds())
+        n, nnz, _ = A.info()
+        coords, _ = grid.download()
+        err = float(grmp.evaluate(grmp.L2ErrorIntegrator(grmp.PolynomialData([0, 0, 0, 0, 1.0]), quadorder=2),
+                                  sol[0])) ** 0.5
+        out["config"] = {"workload": f"nonlinear Poisson -div((1+u^2) grad u) = f, u = xy, P2 on {N}x{N}",
+                         "ndofs": int(n), "nnz": int(nnz), "cells": grid.ncells}
+        out["nonlinear"] = {"status": info.status, "newton_steps": info.iterations,
+                            "residuals": [float("%.3e" % r) for r in info.residuals],
+                            "bicgstab_iterations": info.linear_iterations, "s": round(t, 3),
+                            "newton_matrix_ms": round(min(ts) * 1e3, 3),
+                            "newton_matrix_elements_per_s": round(grid.ncells / min(ts), 1),
+                            "l2_error_vs_xy": err}
+        if not args.no_cpu:
+            log("[bench] nonlinear: cpu baseline (oracle Newton loop, P2 on 48x48)")
+            O = GE.load_oracle()
+            S = O.Space(48, O.FE_P2)
+            P = np.vstack([S.coords, 0.5 * (S.coords[S.facenodes[:, 0]] + S.coords[S.facenodes[:, 1]])])
+            t0 = time.perf_counter()
+            xo, res = O.newton_solve(O.FE_P2, S.coords, S.cells, S.celldofs, S.ndofs, S.fixed, P[:, 0] * P[:, 1],
+                                     (1.0, 0.0, 1.0), F, quadorder=6, lf_quadorder=6, maxit=12,
+                                     target_residual=1e-10)
+            tc = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": round(S.ndofs / tc, 1), "unit": "DoF-solves/s (whole Newton loop)",
+                                   "cores": 1, "kind": "port",
+                                   "sample": f"P2 48x48 ({S.ndofs} dofs), {len(res)} Newton steps with sparse LU, "
+                                             f"{tc:.2f} s"}
+            out["dof_solves_per_s"] = round(n / t, 1)
+    elif which == "copy":
+        # the streaming ceiling the kernels' rates are read against, and the
+        # calibration kernel of the FETCH_SIZE / WRITE_SIZE counters: a plain
+        # device copy of a buffer far larger than the 256 MB MALL
+        import torch
+        nbytes = (args.size or 2048) << 20
+        src_t = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst_t = torch.empty_like(src_t)
+        for _ in range(3):
+            dst_t.copy_(src_t)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 20
+        e0.record()
+        for _ in range(reps):
+            dst_t.copy_(src_t)
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) * 1e-3 / reps
+        out["config"] = {"workload": f"device copy of {nbytes >> 20} MiB (FP64)", "bytes_per_copy": 2 * nbytes}
+        out["copy"] = {"us": round(t * 1e6, 1), "gbs": round(2 * nbytes / t / 1e9, 1),
+                       "frac": round(2 * nbytes / t / 1e9 / HBM_PEAK_GBS, 4), "reps": reps}
     else:
         raise SystemExit(f"unknown workload {which}")
     print(json.dumps(out), flush=True)
@@ -536,7 +613,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="served",
                     help="served (the headline stream) | c2-p1 | c2-p2 | c3 | c4 (single large instances) | "
-                         "heat (BackwardEuler time stepping) | dirichlet (inhomogeneous Dirichlet data)")
+                         "heat (BackwardEuler time stepping) | dirichlet (inhomogeneous Dirichlet data) | "
+                         "nonlinear (Newton fixpoint loop) | copy (the streaming ceiling)")
     ap.add_argument("--size", type=int, default=0, help="single-instance workloads: N (default 2048 / 160 / 512)")
     ap.add_argument("--solve", action="store_true", help="c3: also solve")
     ap.add_argument("--dt", type=float, default=1e-4, help="heat: time step size")

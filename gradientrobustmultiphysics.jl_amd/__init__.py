@@ -15,7 +15,8 @@ from .pde import (APT_BilinearForm, APT_SymmetricBilinearForm, BestapproxDirichl
                   BilinearForm, CellCoefficient, DataFunction, Divergence, FixedIntegralMean, Gradient,
                   GradientCurlData, HDIVRT0, HomogeneousDirichletBoundary, HookStiffnessOperator2D, Identity,
                   IncompressibleNavierStokesProblem, InterpolateDirichletBoundary, PolynomialData, boundary_values,
-                  L2ErrorIntegrator, L2NormIntegrator, evaluate, evaluate_cells,
+                  L2ErrorIntegrator, L2NormIntegrator, evaluate, evaluate_cells, NonlinearDiffusion,
+                  NonlinearForm, solve_fixpoint, is_nonlinear,
                   LagrangeMultiplier, LaplaceOperator, LinearElasticityProblem, LinearForm, PDEDescription,
                   PoissonProblem, ReactionOperator, ReconstructionIdentity, SolverConfig, SymmetricGradient, assemble,
                   assemble_linearform, MultigridSolver, mg_eligible, mg_hierarchy, mg_solve, realize_constraints,

@@ -393,6 +393,205 @@ int plain_cg_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solv
     return rc;
 }
 
+// ------------------------------------------------------ BiCGStab (FP64)
+// For the nonsymmetric Newton matrices of NonlinearForms: BiCGStab (van der
+// Vorst 1992) with right Jacobi preconditioning, fixed dofs eliminated as in
+// the plain CG (x_d = b_d / A_dd, rows masked).  Host-driven: the dot
+// products are folded on the device in a fixed order and read back.
+
+// y = A x on free rows (0 on fixed rows); partials (w.y, y.y)
+__global__ __launch_bounds__(PB) void k_bic_spmv(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                const double *__restrict__ v, const uint8_t *__restrict__ fixed,
+                                                const double *__restrict__ x, const double *__restrict__ w,
+                                                double *__restrict__ y, int64_t n, double *__restrict__ part)
+{
+    __shared__ double lds[SPMV_CAP64];
+    double a0 = 0.0, a1 = 0.0;
+    for (int64_t r0 = (int64_t)blockIdx.x * PB; r0 < n; r0 += (int64_t)gridDim.x * PB) {
+        const int64_t r1 = r0 + PB < n ? r0 + PB : n;
+        const double s = csr_rows_lds64(rp, ci, v, x, r0, r1, lds);
+        const int64_t i = r0 + threadIdx.x;
+        if (i < r1) {
+            const double yi = fixed[i] ? 0.0 : s;
+            y[i] = yi;
+            a0 += w[i] * yi;
+            a1 += yi * yi;
+        }
+    }
+    block_sum2(a0, a1);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = a0;
+        part[2 * blockIdx.x + 1] = a1;
+    }
+}
+
+// r = b - A x on free rows, rh = p = r; partials (r.r, r.r)
+__global__ __launch_bounds__(PB) void k_bic_resid(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                 const double *__restrict__ v, const uint8_t *__restrict__ fixed,
+                                                 const double *__restrict__ b, const double *__restrict__ x,
+                                                 double *__restrict__ r, double *__restrict__ rh,
+                                                 double *__restrict__ p, int64_t n, double *__restrict__ part)
+{
+    __shared__ double lds[SPMV_CAP64];
+    double a0 = 0.0, a1 = 0.0;
+    for (int64_t r0 = (int64_t)blockIdx.x * PB; r0 < n; r0 += (int64_t)gridDim.x * PB) {
+        const int64_t r1 = r0 + PB < n ? r0 + PB : n;
+        const double s = csr_rows_lds64(rp, ci, v, x, r0, r1, lds);
+        const int64_t i = r0 + threadIdx.x;
+        if (i < r1) {
+            const double ri = fixed[i] ? 0.0 : b[i] - s;
+            r[i] = ri;
+            rh[i] = ri;
+            p[i] = ri;
+            a0 += ri * ri;
+            a1 += ri * ri;
+        }
+    }
+    block_sum2(a0, a1);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = a0;
+        part[2 * blockIdx.x + 1] = a1;
+    }
+}
+
+__global__ void k_bic_scale(const double *__restrict__ d, const double *__restrict__ in, double *__restrict__ out,
+                            int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x; i < n; i += (int64_t)gridDim.x * PB) out[i] = d[i] * in[i];
+}
+
+// s = r - alpha v; partials (s.s, 0)
+__global__ __launch_bounds__(PB) void k_bic_s(const double *__restrict__ r, const double *__restrict__ v, double alpha,
+                                             double *__restrict__ s, int64_t n, double *__restrict__ part)
+{
+    double a0 = 0.0, a1 = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x; i < n; i += (int64_t)gridDim.x * PB) {
+        const double si = r[i] - alpha * v[i];
+        s[i] = si;
+        a0 += si * si;
+    }
+    block_sum2(a0, a1);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = a0;
+        part[2 * blockIdx.x + 1] = a1;
+    }
+}
+
+// x += alpha ph + omega sh; r = s - omega t; partials (rh.r, r.r)
+__global__ __launch_bounds__(PB) void k_bic_update(double *__restrict__ x, const double *__restrict__ ph,
+                                                  const double *__restrict__ sh, double alpha, double omega,
+                                                  const double *__restrict__ s, const double *__restrict__ t,
+                                                  double *__restrict__ r, const double *__restrict__ rh, int64_t n,
+                                                  double *__restrict__ part)
+{
+    double a0 = 0.0, a1 = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x; i < n; i += (int64_t)gridDim.x * PB) {
+        x[i] += alpha * ph[i] + omega * sh[i];
+        const double ri = s[i] - omega * t[i];
+        r[i] = ri;
+        a0 += rh[i] * ri;
+        a1 += ri * ri;
+    }
+    block_sum2(a0, a1);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = a0;
+        part[2 * blockIdx.x + 1] = a1;
+    }
+}
+
+__global__ void k_bic_p(const double *__restrict__ r, const double *__restrict__ v, double beta, double omega,
+                        double *__restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x; i < n; i += (int64_t)gridDim.x * PB)
+        p[i] = r[i] + beta * (p[i] - omega * v[i]);
+}
+
+int bicgstab_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solve_opts &o, grmp_solve_info &info)
+{
+    grmp_ctx *ctx = A->ctx;
+    const int64_t n = A->n;
+    GRMP_CHECK(ctx, A->rowptr && A->cols && A->val64 && A->dinv64 && A->fixed && A->diagpos && bd && xd, GRMP_ERR_ARG,
+               "BiCGStab: matrix not assembled in FP64");
+    struct Bufs {
+        grmp_ctx *ctx;
+        double *w = nullptr;
+        double *hp = nullptr;
+        ~Bufs()
+        {
+            if (hp) (void)hipHostFree(hp);
+            dfree(ctx, w);
+        }
+    } B{ctx};
+    GRMP_TRY(dalloc(ctx, &B.w, 8 * (size_t)n + 2 * PGRID_ROWS + 2));
+    GRMP_HIP(ctx, hipHostMalloc((void **)&B.hp, 2 * sizeof(double), hipHostMallocDefault));
+    double *r = B.w, *rh = r + n, *p = rh + n, *v = p + n, *s = v + n, *t = s + n, *ph = t + n, *sh = ph + n;
+    double *part = sh + n, *fold = part + 2 * PGRID_ROWS;
+    const int grid = (int)std::min<int64_t>(PGRID_ROWS, std::max<int64_t>(1, (n + PB - 1) / PB));
+    const int gride = (int)std::min<int64_t>(PGRID, grid);
+    hipLaunchKernelGGL(k_plain_init, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, A->rowptr, A->diagpos, A->val64,
+                       A->fixed, bd, xd, n);
+    hipLaunchKernelGGL(k_bic_resid, dim3(grid), dim3(PB), 0, ctx->stream, A->rowptr, A->cols, A->val64, A->fixed, bd, xd,
+                       r, rh, p, n, part);
+    double rho, rr;
+    GRMP_TRY(plain_sum(ctx, part, fold, B.hp, grid, rho, rr));
+    const double bb = rr, tol2 = o.rtol * o.rtol * bb;
+    int64_t it = 0;
+    int status = GRMP_OK;
+    while (true) {
+        if (!(rr == rr) || std::isinf(rr)) {
+            status = GRMP_ERR_NONFINITE;
+            break;
+        }
+        if (rr <= tol2 || rr == 0.0) break;
+        if (it >= o.maxit || rho == 0.0) {
+            status = GRMP_ERR_NOCONV;
+            break;
+        }
+        double rv, vv, ss, dummy, ts, tt;
+        hipLaunchKernelGGL(k_bic_scale, dim3(gride), dim3(PB), 0, ctx->stream, A->dinv64, p, ph, n);
+        hipLaunchKernelGGL(k_bic_spmv, dim3(grid), dim3(PB), 0, ctx->stream, A->rowptr, A->cols, A->val64, A->fixed, ph,
+                           rh, v, n, part);
+        GRMP_TRY(plain_sum(ctx, part, fold, B.hp, grid, rv, vv));
+        if (rv == 0.0) {
+            status = GRMP_ERR_NOCONV;
+            break;
+        }
+        const double alpha = rho / rv;
+        hipLaunchKernelGGL(k_bic_s, dim3(gride), dim3(PB), 0, ctx->stream, r, v, alpha, s, n, part);
+        GRMP_TRY(plain_sum(ctx, part, fold, B.hp, gride, ss, dummy));
+        ++it;
+        if (ss <= tol2) {  // x += alpha ph; the residual is s
+            hipLaunchKernelGGL(k_bic_update, dim3(gride), dim3(PB), 0, ctx->stream, xd, ph, ph, alpha, 0.0, s, s, r, rh,
+                               n, part);
+            GRMP_TRY(plain_sum(ctx, part, fold, B.hp, gride, dummy, rr));
+            break;
+        }
+        hipLaunchKernelGGL(k_bic_scale, dim3(gride), dim3(PB), 0, ctx->stream, A->dinv64, s, sh, n);
+        hipLaunchKernelGGL(k_bic_spmv, dim3(grid), dim3(PB), 0, ctx->stream, A->rowptr, A->cols, A->val64, A->fixed, sh,
+                           s, t, n, part);
+        GRMP_TRY(plain_sum(ctx, part, fold, B.hp, grid, ts, tt));
+        const double omega = tt > 0.0 ? ts / tt : 0.0;
+        hipLaunchKernelGGL(k_bic_update, dim3(gride), dim3(PB), 0, ctx->stream, xd, ph, sh, alpha, omega, s, t, r, rh,
+                           n, part);
+        double rho2;
+        GRMP_TRY(plain_sum(ctx, part, fold, B.hp, gride, rho2, rr));
+        if (omega == 0.0) {
+            status = rr <= tol2 ? GRMP_OK : GRMP_ERR_NOCONV;
+            break;
+        }
+        const double beta = (rho2 / rho) * (alpha / omega);
+        rho = rho2;
+        hipLaunchKernelGGL(k_bic_p, dim3(gride), dim3(PB), 0, ctx->stream, r, v, beta, omega, p, n);
+    }
+    GRMP_HIP(ctx, hipGetLastError());
+    GRMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    info.iterations = it;
+    info.outer = 1;
+    info.relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+    info.status = status;
+    return GRMP_OK;
+}
+
 __global__ void k_zero_fixed(const uint8_t *fixed, double *b, int64_t n)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1132,12 +1331,15 @@ extern "C" int grmp_solve(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, 
     memset(&inf, 0, sizeof(inf));
     GRMP_CHECK(ctx, A->have_pattern && A->val64, GRMP_ERR_ARG, "grmp_solve: matrix not assembled in FP64");
     GRMP_CHECK(ctx, b->n == A->n && x->n == A->n, GRMP_ERR_ARG, "grmp_solve: vector length != %lld", (long long)A->n);
-    GRMP_CHECK(ctx, o.method == GRMP_SOLVE_CG || o.method == GRMP_SOLVE_MINRES, GRMP_ERR_UNSUPPORTED,
-               "grmp_solve: method %d is not built", o.method);
+    GRMP_CHECK(ctx, o.method == GRMP_SOLVE_CG || o.method == GRMP_SOLVE_MINRES || o.method == GRMP_SOLVE_BICGSTAB,
+               GRMP_ERR_UNSUPPORTED, "grmp_solve: method %d is not built", o.method);
+    GRMP_CHECK(ctx, o.method != GRMP_SOLVE_BICGSTAB || o.precision == GRMP_PRECISION_FP64, GRMP_ERR_UNSUPPORTED,
+               "grmp_solve: BiCGStab runs in FP64 only");
     std::lock_guard<std::mutex> g(ctx->mu);
     double t0 = wall_seconds();
     int rc;
     if (o.method == GRMP_SOLVE_MINRES) rc = minres_solve(A, b->d, x->d, o, inf);
+    else if (o.method == GRMP_SOLVE_BICGSTAB) rc = bicgstab_solve(A, b->d, x->d, o, inf);
     else if (o.precision == GRMP_PRECISION_FP32_IR) rc = solve_engine(A, b, x, o, inf);
     else rc = plain_cg_solve(A, b->d, x->d, o, inf);
     inf.seconds = wall_seconds() - t0;

@@ -156,16 +156,17 @@ class GradientCurlData:
 
 
 class LinearForm(PDEOperator):
-    """LinearForm(Identity, DataFunction(f)) (src/pdeoperators.jl:694-705),
-    and LinearForm(ReconstructionIdentity(HDIVRT0), GradientCurlData) on
+    """LinearForm(Identity, DataFunction(f)) (src/pdeoperators.jl:694-705)
+    with constant or polynomial (PolynomialData, degree <= 4) data, and
+    LinearForm(ReconstructionIdentity(HDIVRT0), GradientCurlData) on
     H1BR{2}, the pressure-robust Stokes right-hand side."""
 
     def __init__(self, operator, data, factor=1.0, name="auto"):
         reco = isinstance(operator, ReconstructionIdentity)
-        if not ((operator == Identity and isinstance(data, DataFunction)) or
+        if not ((operator == Identity and isinstance(data, (DataFunction, PolynomialData))) or
                 (reco and isinstance(data, GradientCurlData))):
             raise _abi.GrmpError(_abi.ERR_UNSUPPORTED,
-                                 "LinearForm(Identity, constant DataFunction) and "
+                                 "LinearForm(Identity, DataFunction or PolynomialData) and "
                                  "LinearForm(ReconstructionIdentity(HDIVRT0), GradientCurlData) are built")
         self.operator, self.data, self.factor, self.name = operator, data, factor, name
 
@@ -179,10 +180,83 @@ class LinearForm(PDEOperator):
             lf.modes = self.data.modes.ctypes.data_as(_abi.DP)
             for k, v in enumerate(self.data.poly[:4]):
                 lf.poly[k] = v
+        elif isinstance(self.data, PolynomialData):
+            if self.data.ncomp != fes.FEType.ncomp:
+                raise _abi.GrmpError(_abi.ERR_ARG, "LinearForm: data components differ from the space's")
+            lf.kind = _abi.LF_POLY
+            lf.coef = self.data.c.ctypes.data_as(_abi.DP)
+            lf.ncoef = self.data.ncoef
         else:
             lf.kind = _abi.LF_CONST
             for k, v in enumerate(self.data.c[:4]):
                 lf.f[k] = float(v)
+        return lf
+
+
+class NonlinearDiffusion:
+    """The kernel (result, input) -> result = alpha(u) grad u, input =
+    [u, grad u], with alpha(u) = c0 + c1 u + c2 u^2: a closed-form stand-in
+    for the Julia closure of examples/Example205_NonlinearPoisson2D.jl:57-61,
+    72-76 (q = 1: alpha(u) = 1 + u^2), whose Jacobian ForwardDiff forms there."""
+
+    def __init__(self, c0=1.0, c1=0.0, c2=1.0):
+        self.c = (float(c0), float(c1), float(c2))
+
+    def alpha(self, u):
+        return self.c[0] + self.c[1] * u + self.c[2] * u * u
+
+
+class NonlinearForm(PDEOperator):
+    """NonlinearForm(Gradient, [Identity, Gradient], [0, 0], kernel, [2, 3];
+    bonus_quadorder, factor) (src/pdeoperators.jl:590-635) with a
+    NonlinearDiffusion kernel: assembled by full_assemble! with Newton's
+    linearization at the current iterate (src/assemblypatterns/nonlinearform.jl:
+    44-246): the matrix (J [phi, grad phi]) . grad v and the right-hand side
+    (J [u, grad u] - kernel) . grad v.  Unknown indices are 0-based."""
+
+    nonlinear = True
+
+    def __init__(self, operator_test, operators_current, coeff_from, action_kernel, argsizes, bonus_quadorder=0,
+                 factor=1.0, name="auto"):
+        if operator_test != Gradient or list(operators_current) != [Identity, Gradient] or \
+                not isinstance(action_kernel, NonlinearDiffusion) or list(argsizes)[:2] != [2, 3]:
+            raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "NonlinearForm(Gradient, [Identity, Gradient], [j, j], "
+                                 "NonlinearDiffusion, [2, 3]) is built")
+        if len(set(int(c) for c in coeff_from)) != 1:
+            raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "NonlinearForm: both arguments from the same unknown")
+        self.coeff_from = [int(c) for c in coeff_from]
+        self.kernel, self.bonus_quadorder, self.factor, self.name = action_kernel, int(bonus_quadorder), factor, name
+
+    def quadorder(self, fes, fes_col=None):
+        """polynomial orders of Identity, Gradient and the test Gradient plus
+        the bonus (src/assemblypatterns.jl:559-563)"""
+        p = fes.FEType.order
+        return p + (p - 1) + (p - 1) + self.bonus_quadorder
+
+    def form(self, fes, rb, cb, ctx, fes_col=None, current=None):
+        if current is None:
+            raise _abi.GrmpError(_abi.ERR_ARG, "NonlinearForm: the current iterate is needed")
+        f = _abi.Form()
+        f.kind = _abi.BLF_NLDIFF
+        f.quadorder = self.quadorder(fes)
+        f.row_block, f.col_block = rb, cb
+        f.factor = float(self.factor)
+        for k in range(3):
+            f.params[k] = self.kernel.c[k]
+        f.coef_mode = _abi.COEF_FACTOR
+        f.current = current.vec.h
+        f.current_offset = int(current.offsets[self.coeff_from[0]])
+        return f, None
+
+    def newton_lform(self, fes, current):
+        lf = _abi.LForm()
+        lf.kind = _abi.LF_NEWTON
+        lf.quadorder = self.quadorder(fes)
+        lf.factor = float(self.factor)
+        for k in range(3):
+            lf.f[k] = self.kernel.c[k]
+        lf.current = current.vec.h
+        lf.current_offset = int(current.offsets[self.coeff_from[0]])
         return lf
 
 
@@ -211,17 +285,19 @@ class BestapproxDirichletBoundary:
 class PolynomialData:
     """Closed-form stand-in for a DataFunction of (x, y) on the device
     (src/userdata.jl:3-63): g = sum_k c_k m_k over the monomials
-    1, x, y, x^2, xy, y^2, x^3, x^2 y, x y^2, y^3 (at most 10 coefficients);
-    a (ncomp, ncoef) array gives vector-valued data with 2 components at
-    most.  bonus_quadorder defaults to the degree."""
+    1, x, y, x^2, xy, y^2, x^3, x^2 y, x y^2, y^3, x^4, x^3 y, x^2 y^2, x y^3,
+    y^4 (at most 15 coefficients; boundary data takes up to degree 3, 10
+    coefficients); a (ncomp, ncoef) array gives vector-valued data with 2
+    components at most.  bonus_quadorder defaults to the degree."""
 
-    MONOMIALS = ((0, 0), (1, 0), (0, 1), (2, 0), (1, 1), (0, 2), (3, 0), (2, 1), (1, 2), (0, 3))
+    MONOMIALS = ((0, 0), (1, 0), (0, 1), (2, 0), (1, 1), (0, 2), (3, 0), (2, 1), (1, 2), (0, 3),
+                 (4, 0), (3, 1), (2, 2), (1, 3), (0, 4))
 
     def __init__(self, coeffs, bonus_quadorder=None):
         c = np.asarray(coeffs, dtype=np.float64)
         c = c.reshape(1, -1) if c.ndim <= 1 else c
-        if c.ndim != 2 or not 1 <= c.shape[0] <= 2 or not 1 <= c.shape[1] <= 10:
-            raise _abi.GrmpError(_abi.ERR_ARG, "PolynomialData: 1..10 coefficients for each of 1 or 2 components")
+        if c.ndim != 2 or not 1 <= c.shape[0] <= 2 or not 1 <= c.shape[1] <= 15:
+            raise _abi.GrmpError(_abi.ERR_ARG, "PolynomialData: 1..15 coefficients for each of 1 or 2 components")
         self.ncomp, self.ncoef = c.shape
         self.c = np.ascontiguousarray(c.ravel())
         nz = [k for k in range(self.ncoef) if np.any(c[:, k] != 0.0)]
@@ -378,10 +454,14 @@ class SolverConfig:
     """Keyword configuration of solve (src/solvers.jl:38-59)."""
 
     def __init__(self, precision="fp64", rtol=1e-10, maxiterations=1000000, inner_rtol=0.1,
-                 fixed_penalty=1e60, minres_inner_rtol=1e-4):
+                 fixed_penalty=1e60, minres_inner_rtol=1e-4, target_residual=1e-12, nonlinear_maxiterations=10,
+                 damping=0.0):
         self.precision, self.rtol, self.maxiterations = precision, rtol, maxiterations
         self.inner_rtol, self.fixed_penalty = inner_rtol, fixed_penalty
         self.minres_inner_rtol = minres_inner_rtol
+        # fixpoint iterations of nonlinear problems (default_solver_kwargs, src/solvers.jl:37-59)
+        self.target_residual, self.nonlinear_maxiterations = target_residual, nonlinear_maxiterations
+        self.damping = damping
 
 
 def fixed_dofs(pde, Solution):
@@ -586,6 +666,10 @@ def solve(Solution, pde, config=None, solver="krylov", **kwargs):
     cfg = config or SolverConfig(**kwargs)
     if solver not in ("auto", "krylov", "mg"):
         raise _abi.GrmpError(_abi.ERR_ARG, f"solver {solver!r}: 'auto', 'krylov' or 'mg'")
+    if is_nonlinear(pde):
+        if solver == "mg":
+            raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "nonlinear problems are solved by the Newton fixpoint loop")
+        return solve_fixpoint(Solution, pde, cfg)
     use_mg = solver != "krylov" and mg_eligible(pde, Solution)
     if solver == "mg" and not use_mg:
         raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "multigrid needs one H1P1 / H1P2 unknown on a generated unit "
@@ -621,3 +705,96 @@ def solve_system(A, b, Solution, cfg, method=_abi.SOLVE_CG):
     if rc not in (_abi.OK,):
         A.ctx.check(rc)
     return info
+
+
+# ------------------------------------------------------------ nonlinear
+def is_nonlinear(pde):
+    return any(getattr(op, "nonlinear", False) for ops in pde.LHS.values() for op in ops)
+
+
+class FixpointInfo:
+    """Statistics of solve_fixpoint: iterations, the nonlinear residual of
+    each, and the linear solves' iterations."""
+
+    def __init__(self):
+        self.status, self.iterations, self.residuals, self.linear_iterations = 0, 0, [], []
+
+    @property
+    def relres(self):
+        return self.residuals[-1] if self.residuals else float("nan")
+
+
+def _nl_system(pde, Solution, A):
+    """values of A and the right-hand side at the current iterate: linear
+    forms, then the Newton right-hand sides of the nonlinear forms"""
+    A.assemble(_abi.VALUES_FP64)
+    b = FEVector(Solution.FES)
+    b.vec.fill(0.0)
+    for j, ops in sorted(pde.RHS.items()):
+        for op in ops:
+            assemble_linearform(b, j, op)
+    for (i, j), ops in sorted(pde.LHS.items()):
+        for op in ops:
+            if getattr(op, "nonlinear", False):
+                lf = op.newton_lform(Solution.FES[i], Solution)
+                b.ctx.check(lib().grmp_assemble_linearform(b.ctx.h, Solution.FES[i].h, ctypes.byref(lf), b.vec.h,
+                                                           int(b.offsets[i])))
+    return b
+
+
+def solve_fixpoint(Solution, pde, cfg):
+    """solve_fixpoint_full! (src/solvers.jl:812-1017) for PDEs with
+    NonlinearForms: assemble at the initial iterate, set the boundary data,
+    then per iteration penalize the fixed dofs, solve the Newton system on
+    the device (BiCGStab: the Newton matrix is nonsymmetric), optionally
+    damp, realize constraints, reassemble at the new iterate and stop when
+    the nonlinear residual ||A x - b|| (fixed rows zeroed) drops below
+    target_residual.  Anderson acceleration is not built.  Returns
+    (FixpointInfo, A, b)."""
+    if cfg.precision != "fp64":
+        raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "nonlinear problems are solved in FP64")
+    A = FEMatrix(Solution.FES)
+    for (i, j), ops in sorted(pde.LHS.items()):
+        for op in ops:
+            if getattr(op, "nonlinear", False):
+                form, keep = op.form(Solution.FES[i], i, j, A.ctx, Solution.FES[j], current=Solution)
+            else:
+                form, keep = op.form(Solution.FES[i], i, j, A.ctx, Solution.FES[j])
+            A.add_form(form, keep)
+    fixed = fixed_dofs(pde, Solution)
+    A.set_fixed(fixed, cfg.fixed_penalty)
+    A.build_pattern()
+    b = _nl_system(pde, Solution, A)
+    if fixed.size:
+        target = boundary_values(pde, Solution)
+        x = Solution.entries()
+        x[fixed] = 0.0 if target is None else target.download()[fixed]
+        Solution.set_entries(x)
+    info = FixpointInfo()
+    last = Solution.entries() if cfg.damping > 0 else None
+    res = FEVector(Solution.FES)
+    mask = np.zeros(Solution.n, dtype=bool)
+    mask[fixed] = True
+    for it in range(1, cfg.nonlinear_maxiterations + 1):
+        if fixed.size:  # b[fixed] = penalty * Target[fixed]
+            A.ctx.check(lib().grmp_matrix_penalize_rhs(A.h, Solution.vec.h, b.vec.h))
+        li = solve_system(A, b, Solution, cfg, _abi.SOLVE_BICGSTAB)
+        info.linear_iterations.append(int(li.iterations))
+        if cfg.damping > 0:
+            x = cfg.damping * last + (1.0 - cfg.damping) * Solution.entries()
+            Solution.set_entries(x)
+            last = x
+        realize_constraints(pde, Solution)
+        b = _nl_system(pde, Solution, A)
+        # residual A x - b on the free rows (the step kernel forms b - A x)
+        A.ctx.check(lib().grmp_timestep_rhs(A.h, A.h, b.vec.h, Solution.vec.h, -1.0, res.vec.h))
+        r = res.entries()
+        r[mask] = 0.0
+        resnorm = float(np.linalg.norm(r))
+        info.residuals.append(resnorm)
+        info.iterations = it
+        if resnorm < cfg.target_residual:
+            break
+    else:
+        info.status = _abi.ERR_NOCONV
+    return info, A, b

@@ -698,3 +698,113 @@ def crank_nicolson(A, M, f, u0, dts, fixed, penalty=PENALTY):
         u = lu.solve(crank_nicolson_rhs(A, M, f, u, dt, fixed, penalty))
         out.append(u.copy())
     return out
+
+
+# ------------------------------------------------------- NonlinearForm (Newton)
+
+def _tri_shape(fe, x0, x1):
+    """values and reference derivatives of the P1/P2 basis at one point"""
+    l = 1.0 - x0 - x1
+    if fe == FE_P1:
+        return np.array([l, x0, x1]), np.array([[-1.0, -1.0], [1.0, 0.0], [0.0, 1.0]])
+    v = np.array([2 * l * (l - 0.5), 2 * x0 * (x0 - 0.5), 2 * x1 * (x1 - 0.5), 4 * l * x0, 4 * x0 * x1, 4 * x1 * l])
+    d = np.array([[-(4 * l - 1), -(4 * l - 1)], [4 * x0 - 1, 0.0], [0.0, 4 * x1 - 1],
+                  [4 * (l - x0), -4 * x0], [4 * x1, 4 * x0], [-4 * x1, 4 * (l - x1)]])
+    return v, d
+
+
+def _tri_geometry(coords, cells):
+    X = coords[cells]
+    A11, A12 = X[:, 1, 0] - X[:, 0, 0], X[:, 2, 0] - X[:, 0, 0]
+    A21, A22 = X[:, 1, 1] - X[:, 0, 1], X[:, 2, 1] - X[:, 0, 1]
+    det = A11 * A22 - A12 * A21
+    M = np.empty((cells.shape[0], 2, 2))
+    M[:, 0, 0], M[:, 0, 1], M[:, 1, 0], M[:, 1, 1] = A22 / det, -A21 / det, -A12 / det, A11 / det
+    return M, np.abs(det) * 0.5, X[:, 0], np.stack([[A11, A12], [A21, A22]]).transpose(2, 0, 1)
+
+
+def newton_assemble(fe, coords, cells, celldofs, ndofs, u, c, quadorder, factor=1.0):
+    """full_assemble! of NonlinearForm(Gradient, [Identity, Gradient], [1,1],
+    kernel, [2,3]) with kernel = alpha(u) grad u, alpha = c0 + c1 u + c2 u^2
+    (src/assemblypatterns/nonlinearform.jl:44-246, transposed_assembly as
+    src/pdeoperators.jl:630 sets it): returns (A, b) with
+    A[test a, ansatz b] = sum_T vol sum_q w (J [phi_b, grad phi_b]) . grad phi_a,
+    b[a] = sum_T vol sum_q w ((J [u, grad u]) - kernel) . grad phi_a; A keeps
+    every local pair (also exact zeros)."""
+    import scipy.sparse as sp
+    nd = 3 if fe == FE_P1 else 6
+    xq, wq = tri_rule(quadorder)
+    M, vol, _, _ = _tri_geometry(coords, cells)
+    uc = u[celldofs[:, :nd]]
+    nc = cells.shape[0]
+    L = np.zeros((nc, nd, nd))      # [cell, test a, ansatz b]
+    lb = np.zeros((nc, nd))
+    for (x0, x1), w in zip(xq, wq):
+        phi, dref = _tri_shape(fe, x0, x1)
+        g = np.einsum("ckj,dj->cdk", M, dref)          # [cell, dof, k] gradients
+        uu = uc @ phi
+        gu = np.einsum("cd,cdk->ck", uc, g)
+        alpha = c[0] + c[1] * uu + c[2] * uu * uu
+        dalpha = c[1] + 2.0 * c[2] * uu
+        # J [phi_b, grad phi_b] = dalpha grad u phi_b + alpha grad phi_b
+        Jb = dalpha[:, None, None] * gu[:, None, :] * phi[None, :, None] + alpha[:, None, None] * g
+        L += w * np.einsum("cbk,cak->cab", Jb, g)
+        Jin = dalpha[:, None] * gu * uu[:, None] + alpha[:, None] * gu
+        val = alpha[:, None] * gu
+        lb += w * np.einsum("ck,cak->ca", Jin - val, g)
+    itf = vol * factor
+    rows = np.repeat(celldofs[:, :nd], nd, axis=1).ravel()
+    cols = np.tile(celldofs[:, :nd], (1, nd)).ravel()
+    A = sp.csr_matrix(((L * itf[:, None, None]).ravel(), (rows, cols)), shape=(ndofs, ndofs))
+    b = np.zeros(ndofs)
+    np.add.at(b, celldofs[:, :nd].ravel(), (lb * itf[:, None]).ravel())
+    return A, b
+
+
+def assemble_lf_poly(fe, coef, quadorder, coords, cells, celldofs, ndofs, factor=1.0):
+    """LinearForm(Identity, f) with polynomial f (degree <= 4, monomials
+    1, x, y, x^2, xy, y^2, x^3, x^2 y, x y^2, y^3, x^4, x^3 y, x^2 y^2, x y^3,
+    y^4) on scalar P1/P2 (src/assemblypatterns/linearform.jl:181-220)."""
+    mons = ((0, 0), (1, 0), (0, 1), (2, 0), (1, 1), (0, 2), (3, 0), (2, 1), (1, 2), (0, 3),
+            (4, 0), (3, 1), (2, 2), (1, 3), (0, 4))
+    nd = 3 if fe == FE_P1 else 6
+    xq, wq = tri_rule(quadorder)
+    M, vol, X0, Amat = _tri_geometry(coords, cells)
+    lb = np.zeros((cells.shape[0], nd))
+    for (x0, x1), w in zip(xq, wq):
+        phi, _ = _tri_shape(fe, x0, x1)
+        xp = X0[:, 0] + Amat[:, 0, 0] * x0 + Amat[:, 0, 1] * x1
+        yp = X0[:, 1] + Amat[:, 1, 0] * x0 + Amat[:, 1, 1] * x1
+        f = sum(ck * xp ** i * yp ** j for ck, (i, j) in zip(coef, mons))
+        lb += w * f[:, None] * phi[None, :]
+    b = np.zeros(ndofs)
+    np.add.at(b, celldofs[:, :nd].ravel(), (lb * (factor * vol)[:, None]).ravel())
+    return b
+
+
+def newton_solve(fe, coords, cells, celldofs, ndofs, fixed, target, c, f_coef, quadorder, lf_quadorder,
+                 maxit=10, target_residual=1e-12, u0=None):
+    """solve_fixpoint_full! (src/solvers.jl:812-1017) for the nonlinear
+    Poisson problem: assemble at the initial iterate, set the boundary
+    values, then per iteration: penalized direct solve (fixed dofs
+    eliminated), reassemble, residual ||A x - b|| on the free rows.
+    Returns (x, residuals)."""
+    import scipy.sparse.linalg as spla
+    u = np.zeros(ndofs) if u0 is None else np.array(u0, dtype=np.float64)
+    f = assemble_lf_poly(fe, f_coef, lf_quadorder, coords, cells, celldofs, ndofs)
+    A, bn = newton_assemble(fe, coords, cells, celldofs, ndofs, u, c, quadorder)
+    b = f + bn
+    u[fixed] = target[fixed]
+    free = np.setdiff1d(np.arange(ndofs), fixed)
+    res = []
+    for _ in range(maxit):
+        rhs = b[free] - A[free][:, fixed] @ u[fixed]
+        u[free] = spla.spsolve(A[free][:, free].tocsc(), rhs)
+        A, bn = newton_assemble(fe, coords, cells, celldofs, ndofs, u, c, quadorder)
+        b = f + bn
+        r = A @ u - b
+        r[fixed] = 0.0
+        res.append(float(np.linalg.norm(r)))
+        if res[-1] < target_residual:
+            break
+    return u, res

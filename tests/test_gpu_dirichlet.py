@@ -179,7 +179,7 @@ def test_dirichlet_data_errors(grmp):
     with pytest.raises(grmp.GrmpError):
         grid.ctx.check(grmp.lib().grmp_boundary_interpolate(fes.h, grmp._abi.dptr(c), 1, short.h, 0))
     with pytest.raises(grmp.GrmpError):
-        grmp.PolynomialData(np.ones(11))
+        grmp.PolynomialData(np.ones(16))
     with pytest.raises(grmp.GrmpError):
         grmp.PoissonProblem().add_boundarydata(0, grmp.BestapproxDirichletBoundary)
     pde = grmp.PoissonProblem()
