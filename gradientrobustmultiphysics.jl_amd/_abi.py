@@ -162,6 +162,8 @@ SIGNATURES = {
     "grmp_host_free": (ctypes.c_int, [VP]),
     "grmp_integrate_l2error": (ctypes.c_int, [VP, VP, VP, I64, DP, ctypes.c_int, ctypes.c_int, ctypes.c_double, DP,
                                               ctypes.POINTER(ctypes.c_double)]),
+    "grmp_integrate_l2error_noalloc": (ctypes.c_int, [VP, VP, VP, I64, DP, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                      DP, ctypes.POINTER(ctypes.c_double)]),
     "grmp_solve_opts_default": (None, [ctypes.POINTER(SolveOpts)]),
     "grmp_solve": (ctypes.c_int, [VP, VP, VP, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
     "grmp_solve_csc": (ctypes.c_int, [VP, I64, I64P, I64P, DP, DP, DP, I64, I64P, ctypes.c_int, ctypes.c_double]),

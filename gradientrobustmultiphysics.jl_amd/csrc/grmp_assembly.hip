@@ -2301,6 +2301,57 @@ extern "C" int grmp_integrate_l2error(grmp_ctx *ctx, const grmp_space *s, const 
     return rc;
 }
 
+// ------------------------------- L2ErrorIntegrator without device allocation
+// grmp_integrate_l2error_noalloc keeps its per-cell values and partials in
+// this file-scope array instead of allocating them.  Each GPU holds its own
+// copy, at its own address, until the process ends.  The host never uses the
+// array's name as an address: the per-device address comes from
+// hipGetSymbolAddress, taken once per device on the entry point's first call
+// there with that device current.
+#define L2E_NA_MAXCELLS (1ll << 22)
+#define L2E_NA_WORDS (L2E_NA_MAXCELLS + (L2E_NA_MAXCELLS + 1023) / 1024 + 1)
+static __device__ __align__(256) double scratch[L2E_NA_WORDS];
+
+#define L2E_NA_MAXDEV 64
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[L2E_NA_MAXDEV];
+
+static int l2e_scratch(grmp_ctx *ctx, double **out)
+{
+    const int dev = ctx->device;
+    GRMP_CHECK(ctx, dev >= 0 && dev < L2E_NA_MAXDEV, GRMP_ERR_UNSUPPORTED, "no-allocation scratch: device %d", dev);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int prev = 0;
+        GRMP_HIP(ctx, hipGetDevice(&prev));
+        GRMP_HIP(ctx, hipSetDevice(dev));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        (void)hipSetDevice(prev);
+        GRMP_HIP(ctx, e);
+        GRMP_CHECK(ctx, p != nullptr, GRMP_ERR_HIP, "no-allocation scratch: no device address");
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return GRMP_OK;
+}
+
+extern "C" int grmp_integrate_l2error_noalloc(grmp_ctx *ctx, const grmp_space *s, const grmp_vector *u,
+                                              int64_t offset, const double *coef, int ncoef, int quadorder,
+                                              double factor, double *cellvals, double *total)
+{
+    GRMP_CHECK(ctx, ctx && (cellvals || total), GRMP_ERR_ARG, "grmp_integrate_l2error_noalloc: no output");
+    L2eDev L;
+    GRMP_TRY(l2e_setup(ctx, s, u, offset, coef, ncoef, quadorder, factor, L));
+    GRMP_CHECK(ctx, L.ncells <= L2E_NA_MAXCELLS, GRMP_ERR_ARG,
+               "grmp_integrate_l2error_noalloc: %lld cells exceed the scratch limit of %lld cells",
+               (long long)L.ncells, (long long)L2E_NA_MAXCELLS);
+    double *d = nullptr;
+    GRMP_TRY(l2e_scratch(ctx, &d));
+    std::lock_guard<std::mutex> g(ctx->mu);
+    return l2e_run(ctx, L, u->d + offset, d, cellvals, total);
+}
+
 // ------------------------------------------------------------ CellVolumes
 __global__ void k_cellvolumes(const double *coords, const int32_t *cells, int64_t nn, int64_t nc, int dim, double *out)
 {

@@ -311,6 +311,17 @@ int grmp_integrate_l2error(grmp_ctx *ctx, const grmp_space *space, const grmp_ve
                            const double *coef, int ncoef, int quadorder, double factor, double *cellvals,
                            double *total);
 
+/* The same, allocating nothing: the per-cell values and partials live in a
+ * file-scope device array of the library (one per GPU, held until the
+ * process ends), so callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc) can
+ * evaluate errors.  No counterpart in the reference.  At most 4,194,304
+ * cells (GRMP_ERR_ARG above); calls that overlap on one GPU share the array
+ * and race. */
+int grmp_integrate_l2error_noalloc(grmp_ctx *ctx, const grmp_space *space, const grmp_vector *u, int64_t offset,
+                                   const double *coef, int ncoef, int quadorder, double factor, double *cellvals,
+                                   double *total);
+
 /* ------------------------------------------------------ Dirichlet data */
 /* Boundary values of an H1P1 / H1P2 space with 1 or 2 components (triangle
  * meshes) for the polynomial data g_c = sum_k coef[c ncoef + k] m_k,
