@@ -1388,6 +1388,117 @@ int assemble_values(const grmp_matrix *A, const double *kappa, DevForm *dforms, 
     return GRMP_OK;
 }
 
+// ------------------------------------------------ batched assembly (many meshes)
+// Many single-form matrices -- the levels of many served requests, or any
+// set of meshes -- assembled by one launch: a persistent kernel whose
+// workgroups walk a queue of (matrix, 256-row block) tiles and run, per tile,
+// the serial row walk of k_assemble1s with that matrix's form.  Every row is
+// summed in the order of the per-matrix kernels, so the values are the same
+// bits.  Tiles go to workgroups in a fixed stride (no atomics).
+struct AsmItemDev {
+    DevForm F;
+    const int32_t *rowptr;
+    const uint8_t *diagpos, *fixed;
+    double penalty;
+    double *val64, *dinv64;
+    float *dinv32;
+    int64_t n;
+};
+struct AsmTile {
+    int32_t item, blk;
+};
+#define ASMB_ROWS 256
+
+__global__ __launch_bounds__(ASMB_ROWS) void k_assemble_queue(const AsmItemDev *__restrict__ items,
+                                                               const AsmTile *__restrict__ tiles, int ntiles)
+{
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const AsmTile T = tiles[t];
+        const AsmItemDev &I = items[T.item];
+        const DevForm F = I.F;
+        GRMP_DISPATCH(F, assemble_block([&](int64_t i, double *row) { assemble_row<KT>(F, i, row); }, (int64_t)T.blk,
+                                        0, I.n, 0, I.rowptr, I.diagpos, I.fixed, I.penalty, I.val64, nullptr,
+                                        I.dinv64, I.dinv32));
+        __syncthreads();  // the next tile reuses the LDS rows
+    }
+}
+
+int assemble_batch(grmp_ctx *ctx, hipStream_t st, const AsmBatchItem *items, int n)
+{
+    if (n <= 0) return GRMP_OK;
+    std::vector<AsmItemDev> hi(n);
+    std::vector<AsmTile> ht;
+    int maxrow = 1;
+    for (int k = 0; k < n; ++k) {
+        const grmp_matrix *A = items[k].A;
+        GRMP_CHECK(ctx, A && A->have_pattern && A->forms.size() == 1 && A->nblocks == 1 && items[k].val64 &&
+                            items[k].dinv64 && items[k].dinv32,
+                   GRMP_ERR_ARG, "batched assembly: item %d is not a single-form matrix with a pattern and outputs", k);
+        AsmItemDev &d = hi[k];
+        memset(&d, 0, sizeof(d));
+        fill_devform(A, A->forms[0], d.F);
+        if (items[k].kappa && d.F.coef_cell) d.F.kappa = items[k].kappa;
+        GRMP_CHECK(ctx, d.F.kappa || !d.F.coef_cell, GRMP_ERR_ARG, "batched assembly: item %d has no coefficient", k);
+        d.rowptr = A->rowptr;
+        d.diagpos = A->diagpos;
+        d.fixed = A->fixed;
+        d.penalty = A->penalty;
+        d.val64 = items[k].val64;
+        d.dinv64 = items[k].dinv64;
+        d.dinv32 = items[k].dinv32;
+        d.n = A->n;
+        maxrow = std::max(maxrow, A->max_row);
+        for (int64_t b = 0; b * ASMB_ROWS < A->n; ++b) ht.push_back(AsmTile{k, (int32_t)b});
+    }
+    const size_t shm = (size_t)ASMB_ROWS * maxrow * sizeof(double);
+    GRMP_CHECK(ctx, shm <= 64 * 1024, GRMP_ERR_UNSUPPORTED,
+               "batched assembly: rows of %d entries exceed a tile's LDS (at most 32)", maxrow);
+    const size_t bi = sizeof(AsmItemDev) * hi.size(), bt = sizeof(AsmTile) * ht.size();
+    char *dbuf = nullptr;
+    GRMP_HIP(ctx, hipMallocAsync((void **)&dbuf, bi + bt, st));
+    hipError_t e = hipMemcpyAsync(dbuf, hi.data(), bi, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dbuf + bi, ht.data(), bt, hipMemcpyHostToDevice, st);
+    const int ntiles = (int)ht.size();
+    const int grid = std::min(ntiles, 2048);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_assemble_queue, dim3(grid), dim3(ASMB_ROWS), shm, st, (const AsmItemDev *)dbuf,
+                           (const AsmTile *)(dbuf + bi), ntiles);
+        e = hipGetLastError();
+    }
+    // the host arrays are the copies' sources: finish them before returning
+    const hipError_t e2 = hipStreamSynchronize(st);
+    (void)hipFreeAsync(dbuf, st);
+    GRMP_HIP(ctx, e);
+    GRMP_HIP(ctx, e2);
+    return GRMP_OK;
+}
+
+extern "C" int grmp_matrix_assemble_batch(int count, grmp_matrix *const *mats)
+{
+    GRMP_CHECK(nullptr, count >= 0 && (count == 0 || mats), GRMP_ERR_ARG, "grmp_matrix_assemble_batch: no matrices");
+    if (count == 0) return GRMP_OK;
+    grmp_ctx *ctx = mats[0]->ctx;
+    std::vector<AsmBatchItem> items(count);
+    for (int k = 0; k < count; ++k) {
+        grmp_matrix *A = mats[k];
+        GRMP_CHECK(ctx, A && A->ctx == ctx, GRMP_ERR_ARG, "grmp_matrix_assemble_batch: matrix %d of another context", k);
+        if (!A->have_pattern) GRMP_TRY(grmp_matrix_build_pattern(A));
+    }
+    std::lock_guard<std::mutex> g(ctx->mu);
+    for (int k = 0; k < count; ++k) {
+        grmp_matrix *A = mats[k];
+        GRMP_CHECK(ctx, A->forms.size() == 1 && A->nblocks == 1, GRMP_ERR_UNSUPPORTED,
+                   "grmp_matrix_assemble_batch: matrix %d has more than one form or block", k);
+        if (!A->val64) GRMP_TRY(dalloc(ctx, &A->val64, A->nnz));
+        if (!A->dinv64) GRMP_TRY(dalloc(ctx, &A->dinv64, A->n));
+        if (!A->dinv32) GRMP_TRY(dalloc(ctx, &A->dinv32, A->n));
+        items[k] = AsmBatchItem{A, nullptr, A->val64, A->dinv64, A->dinv32};
+    }
+    GRMP_TRY(assemble_batch(ctx, ctx->stream, items.data(), count));
+    for (int k = 0; k < count; ++k) mats[k]->values |= GRMP_VALUES_FP64;
+    return GRMP_OK;
+}
+
 extern "C" double grmp_matrix_last_assembly_seconds(const grmp_matrix *A) { return A ? A->last_asm_ms * 1e-3 : -1.0; }
 
 extern "C" int grmp_matrix_set_tune(grmp_matrix *A, const grmp_asm_tune *t)

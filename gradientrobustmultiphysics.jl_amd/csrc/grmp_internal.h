@@ -213,6 +213,21 @@ constexpr int64_t GRMP_MAX_NNZ = 2147483647;
 int form_resolve(grmp_matrix *A, FormRec &r);
 void fill_devform(const grmp_matrix *A, const FormRec &r, DevForm &d);
 
+// one matrix of a batched assembly (assemble_batch): a single-form matrix with
+// its pattern, the cell coefficient replacing its form's (or nullptr), and
+// where its FP64 values and inverse diagonals go
+struct AsmBatchItem {
+    const grmp_matrix *A;
+    const double *kappa;
+    double *val64;
+    double *dinv64;
+    float *dinv32;
+};
+// Assembles every item in one launch of a persistent kernel over a queue of
+// (item, row block) tiles; values are bitwise those of per-matrix assembly.
+// The staging is synchronous: the call returns after the upload.
+int assemble_batch(grmp_ctx *ctx, hipStream_t st, const AsmBatchItem *items, int n);
+
 // assembly internals shared with the server
 int assemble_values(const grmp_matrix *A, const double *kappa, DevForm *dforms, DevForm *hforms, double *val64,
                     float *val32, double *dinv64, float *dinv32, hipStream_t st);
@@ -250,6 +265,8 @@ double *mg_job_kappa0(MgJob *j);     // level-0 cell coefficient, filled by the 
 double *mg_job_rhs(MgJob *j);        // level-0 right-hand side, filled by the caller
 const double *mg_job_solution(const MgJob *j);
 int mg_job_setup(MgJob *j, double rtol, int maxit);
+// mg_job_setup for n jobs on one stream with their level assembly in one launch
+int mg_jobs_setup_batched(MgJob *const *jobs, int n, hipStream_t st, double *const rtol, int maxit);
 int mg_job_iterate(MgJob *j, int iters);
 int mg_job_verify(MgJob *j);         // after done == 2: check the true residual
 bool mg_job_block(const MgJob *j);   // whole iterations in one workgroup (checks in place)

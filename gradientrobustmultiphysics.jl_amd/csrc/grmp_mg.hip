@@ -1343,6 +1343,46 @@ int mg_job_setup(MgJob *j, double rtol, int maxit)
     return mg_job_start(j, rtol, maxit);
 }
 
+// The setup of many jobs with their level assembly batched: coarse
+// coefficients of every job, one launch assembling every level of every job
+// (assemble_batch, the same values as mg_job_assemble's per-level launches),
+// then per job the Gershgorin bounds, the coarsest inverse and the first
+// V-cycle.  The jobs share the stream st.
+int mg_jobs_setup_batched(MgJob *const *jobs, int n, hipStream_t st, double *const rtol, int maxit)
+{
+    if (n <= 0) return GRMP_OK;
+    grmp_ctx *ctx = jobs[0]->ctx;
+    std::vector<AsmBatchItem> items;
+    for (int k = 0; k < n; ++k) {
+        MgJob *j = jobs[k];
+        GRMP_CHECK(ctx, j->st == st, GRMP_ERR_ARG, "batched multigrid setup: jobs on different streams");
+        for (int l = 0; l < j->nlev; ++l) {
+            MgLevel &L = j->lv[l];
+            if (l > 0 && L.in.N != j->lv[l - 1].in.N)
+                hipLaunchKernelGGL(k_mg_coarse_kappa, dim3(nblk(L.nc, 256)), dim3(256), 0, st, (int64_t)j->lv[l - 1].in.N,
+                                   (int64_t)L.in.N, j->lv[l - 1].kappa, L.kappa);
+            items.push_back(AsmBatchItem{L.in.A, L.kappa, L.val, L.dinv, L.dinv32});
+        }
+    }
+    GRMP_HIP(ctx, hipGetLastError());
+    GRMP_TRY(assemble_batch(ctx, st, items.data(), (int)items.size()));
+    for (int k = 0; k < n; ++k) {
+        MgJob *j = jobs[k];
+        for (int l = 0; l < j->nlev; ++l) {
+            MgLevel &L = j->lv[l];
+            GRMP_HIP(ctx, hipMemsetAsync(L.lam, 0, 8, st));
+            hipLaunchKernelGGL(k_mg_gersh, dim3(nblk(L.n, 256)), dim3(256), 0, st, L.in.A->rowptr, L.val, L.dinv,
+                               L.in.A->fixed, L.n, (unsigned long long *)L.lam);
+        }
+        const MgLevel &C = j->lv[j->nlev - 1];
+        hipLaunchKernelGGL(k_mg_dense_inv, dim3(1), dim3(256), 0, st, C.in.A->rowptr, C.in.A->cols, C.val,
+                           C.in.A->fixed, (int)C.n, j->inv);
+        GRMP_HIP(ctx, hipMemcpyAsync(j->ddesc, &j->hdesc, sizeof(MgBDesc), hipMemcpyHostToDevice, st));
+        GRMP_TRY(mg_job_start(j, rtol[k], maxit));
+    }
+    return GRMP_OK;
+}
+
 static int mg_iterate_grid(MgJob *j, int iters);
 
 int mg_job_iterate(MgJob *j, int iters)

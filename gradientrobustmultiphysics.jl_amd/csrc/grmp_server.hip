@@ -469,6 +469,7 @@ static void served_loop(grmp_server *S)
 // ---------------------------------------------------------- multigrid loop
 static const int kMgChunk = 4;     // PCG iterations enqueued between status polls
 static const int kMgMaxIt = 500;
+static const int kMgSetupGroup = 16;  // requests a setup thread sets up together (one assembly launch)
 static const int kMgProfileEvery = 4;
 
 // (caller holds S->mu; lock order S->mu before tpl_mu)
@@ -557,8 +558,11 @@ static int mg_template(grmp_server *S, grmp_ctx *c, const grmp_request &r, Templ
 }
 
 // Levels: the request's shape, P1 on the same square when it is P2, then P1
-// on ceil(N/2), ceil(N/4), ... down to N <= 4.
-static int mg_prepare(grmp_server *S, Job *j, grmp_ctx *c, bool prof)
+// on ceil(N/2), ceil(N/4), ... down to N <= 4.  mg_build makes the shapes,
+// the job, its coefficient and right-hand side; the level assembly and the
+// first V-cycle follow (mg_prepare: for this job alone; the setup threads:
+// for a group of jobs with one assembly launch).
+static int mg_build(grmp_server *S, Job *j, grmp_ctx *c, bool prof)
 {
     hipStream_t st = c->stream;
     j->res.t_admit = now(S);
@@ -592,7 +596,13 @@ static int mg_prepare(grmp_server *S, Job *j, grmp_ctx *c, bool prof)
     GRMP_HIP(c, hipMemsetAsync(rhs, 0, j->n * 8, st));
     GRMP_TRY(linearform_launch(c, t0->space, &t0->lf, rhs, st));
     GRMP_TRY(zero_fixed_launch(t0->A, rhs, st));
-    GRMP_TRY(mg_job_setup(j->mg, r.rtol > 0 ? r.rtol : 1e-10, kMgMaxIt));
+    return GRMP_OK;
+}
+
+static int mg_prepare(grmp_server *S, Job *j, grmp_ctx *c, bool prof)
+{
+    GRMP_TRY(mg_build(S, j, c, prof));
+    GRMP_TRY(mg_job_setup(j->mg, j->req.rtol > 0 ? j->req.rtol : 1e-10, kMgMaxIt));
     j->state = 2;
     return GRMP_OK;
 }
@@ -770,47 +780,85 @@ static void mg_loop(grmp_server *S, int tid, int nthreads)
 // hand the request on with an event.  The batch thread (one stream) adds
 // ready requests to the batch, which advances kMgChunk iterations per run;
 // requests leave between runs when done.
+static void mg_setup_fail(grmp_server *S, Job *j, int rc)
+{
+    // with S->mu held
+    mg_release(S, j);
+    j->res.status = rc;
+    j->res.t_done = now(S);
+    j->state = 3;
+}
+
 static void mg_setup_loop(grmp_server *S, int tid)
 {
     grmp_ctx *c = S->mg_ctx[tid];
+    auto room = [&](Job *f) {
+        return S->mg_in_batch + (int)S->mg_readyq.size() + S->mg_setting_up < S->cfg.mg_batch &&
+               !(S->mem_used > 0 && S->mem_used + f->footprint > S->cfg.memory_budget);
+    };
     for (;;) {
-        Job *j = nullptr;
+        // a group of queued requests, set up together: their levels are
+        // assembled by one launch (mg_jobs_setup_batched)
+        std::vector<Job *> grp;
         {
             std::unique_lock<std::mutex> lk(S->mu);
             S->cv.wait(lk, [&] {
                 if (S->stop && S->queue.empty()) return true;
                 if (S->queue.empty() || S->fatal) return S->stop;
-                if (S->mg_in_batch + (int)S->mg_readyq.size() + S->mg_setting_up >= S->cfg.mg_batch) return false;
-                Job *f = S->queue.front();
-                return !(S->mem_used > 0 && S->mem_used + f->footprint > S->cfg.memory_budget);
+                return room(S->queue.front());
             });
             if (S->queue.empty() || S->fatal) return;
-            j = S->queue.front();
-            S->queue.pop_front();
-            S->mem_used += j->footprint;
-            S->mg_setting_up++;
+            while (!S->queue.empty() && (int)grp.size() < kMgSetupGroup && room(S->queue.front())) {
+                Job *j = S->queue.front();
+                S->queue.pop_front();
+                S->mem_used += j->footprint;
+                S->mg_setting_up++;
+                grp.push_back(j);
+            }
         }
-        int rc = mg_prepare(S, j, c, false);
-        if (!rc && hipEventCreateWithFlags(&j->mg_ready, hipEventDisableTiming) != hipSuccess) rc = GRMP_ERR_HIP;
-        if (!rc && hipEventRecord(j->mg_ready, c->stream) != hipSuccess) rc = GRMP_ERR_HIP;
-        if (rc) (void)hipStreamSynchronize(c->stream);
-        if (!rc && S->fatal) {  // the batch ended on a device error meanwhile
+        std::vector<int> rcs(grp.size(), GRMP_OK);
+        std::vector<MgJob *> ok;
+        std::vector<double> rtol;
+        for (size_t k = 0; k < grp.size(); ++k) {
+            rcs[k] = mg_build(S, grp[k], c, false);
+            if (!rcs[k]) {
+                ok.push_back(grp[k]->mg);
+                rtol.push_back(grp[k]->req.rtol > 0 ? grp[k]->req.rtol : 1e-10);
+            }
+        }
+        int grc = mg_jobs_setup_batched(ok.data(), (int)ok.size(), c->stream, rtol.data(), kMgMaxIt);
+        for (size_t k = 0; k < grp.size(); ++k) {
+            Job *j = grp[k];
+            if (rcs[k]) continue;
+            if (grc) {
+                rcs[k] = grc;
+                continue;
+            }
+            j->state = 2;
+            if (hipEventCreateWithFlags(&j->mg_ready, hipEventDisableTiming) != hipSuccess ||
+                hipEventRecord(j->mg_ready, c->stream) != hipSuccess)
+                rcs[k] = GRMP_ERR_HIP;
+        }
+        bool any_fail = false;
+        for (int rc : rcs) any_fail |= rc != GRMP_OK;
+        if (any_fail) (void)hipStreamSynchronize(c->stream);
+        if (S->fatal) {  // the batch ended on a device error meanwhile
             (void)hipStreamSynchronize(c->stream);
-            rc = S->fatal;
+            for (int &rc : rcs)
+                if (!rc) rc = S->fatal;
         }
         std::lock_guard<std::mutex> g(S->mu);
-        S->mg_setting_up--;
-        if (rc) {
-            mg_release(S, j);
-            j->res.status = rc;
-            j->res.t_done = now(S);
-            j->state = 3;
-            S->cv_done.notify_all();
-            S->cv.notify_all();
-            S->cv_ready.notify_all();
-            continue;
+        for (size_t k = 0; k < grp.size(); ++k) {
+            Job *j = grp[k];
+            S->mg_setting_up--;
+            if (rcs[k]) {
+                mg_setup_fail(S, j, rcs[k]);
+                continue;
+            }
+            S->mg_readyq.push_back(j);
         }
-        S->mg_readyq.push_back(j);
+        S->cv_done.notify_all();
+        S->cv.notify_all();
         S->cv_ready.notify_all();
     }
 }

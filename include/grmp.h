@@ -251,6 +251,14 @@ int grmp_matrix_info(const grmp_matrix *A, int64_t *n, int64_t *nnz, int *max_ro
 int grmp_matrix_export_csc(const grmp_matrix *A, int64_t *colptr, int64_t *rowval, double *nzval);
 /* y = A x with the FP64 or FP32 values (x, y FP64 device vectors) */
 int grmp_matrix_spmv(grmp_matrix *A, int values, const grmp_vector *x, grmp_vector *y);
+/* Batched assembly of many meshes: the FP64 values of `count` single-form
+ * matrices (any meshes, spaces and form kinds) in one launch of a persistent
+ * kernel over a queue of (matrix, row block) tiles, bitwise equal to
+ * assembling each (bilinearform.jl:226-377 per item).  Builds missing
+ * patterns first.  No counterpart in the reference, which assembles one
+ * operator at a time. */
+int grmp_matrix_assemble_batch(int count, grmp_matrix *const *mats);
+
 /* seconds of the last assemble call's kernels (HIP events) */
 double grmp_matrix_last_assembly_seconds(const grmp_matrix *A);
 
