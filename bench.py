@@ -319,7 +319,7 @@ def run_workload(grmp, which, args, log):
         log(f"[bench] {which}: spmv")
         out["spmv_fp64"] = _spmv_rate(grmp, A, V64)
         out["spmv_fp32"] = _spmv_rate(grmp, A, V32)
-        if which != "c3" or args.solve:
+        if (which != "c3" or args.solve) and not args.assembly_only:
             log(f"[bench] {which}: FP64 CG")
             x64 = grmp.FEVector(fes)
             i64 = grmp.solve_system(A, b, x64, grmp.SolverConfig(precision="fp64", rtol=args.rtol))
@@ -546,7 +546,8 @@ def run_workload(grmp, which, args, log):
                          "ndofs": int(n), "nnz": int(nnz), "cells": grid.ncells}
         out["nonlinear"] = {"status": info.status, "newton_steps": info.iterations,
                             "residuals": [float("%.3e" % r) for r in info.residuals],
-                            "bicgstab_iterations": info.linear_iterations, "s": round(t, 3),
+                            "bicgstab_iterations": info.linear_iterations,
+                            "bicgstab_restarts": info.linear_restarts, "s": round(t, 3),
                             "newton_matrix_ms": round(min(ts) * 1e3, 3),
                             "newton_matrix_elements_per_s": round(grid.ncells / min(ts), 1),
                             "l2_error_vs_xy": err}
@@ -567,26 +568,30 @@ def run_workload(grmp, which, args, log):
             out["dof_solves_per_s"] = round(n / t, 1)
     elif which == "copy":
         # the streaming ceiling the kernels' rates are read against, and the
-        # calibration kernel of the FETCH_SIZE / WRITE_SIZE counters: a plain
-        # device copy of a buffer far larger than the 256 MB MALL
-        import torch
+        # calibration kernel of the FETCH_SIZE / WRITE_SIZE counters: the
+        # library's y += a x over two buffers far larger than the 256 MB MALL
+        # (reads 2 n doubles, writes n doubles per launch)
         nbytes = (args.size or 2048) << 20
-        src_t = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
-        dst_t = torch.empty_like(src_t)
+        n = nbytes // 8
+        x = grmp.DeviceVector(ctx, n)
+        y = grmp.DeviceVector(ctx, n)
+        x.fill(1.0)
+        y.fill(0.0)
+        L = grmp.lib()
         for _ in range(3):
-            dst_t.copy_(src_t)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ctx.check(L.grmp_vector_axpy(1.0, x.h, y.h))
+        ctx.sync()
         reps = 20
-        e0.record()
+        t0 = time.perf_counter()
         for _ in range(reps):
-            dst_t.copy_(src_t)
-        e1.record()
-        torch.cuda.synchronize()
-        t = e0.elapsed_time(e1) * 1e-3 / reps
-        out["config"] = {"workload": f"device copy of {nbytes >> 20} MiB (FP64)", "bytes_per_copy": 2 * nbytes}
-        out["copy"] = {"us": round(t * 1e6, 1), "gbs": round(2 * nbytes / t / 1e9, 1),
-                       "frac": round(2 * nbytes / t / 1e9 / HBM_PEAK_GBS, 4), "reps": reps}
+            ctx.check(L.grmp_vector_axpy(1.0, x.h, y.h))
+        ctx.sync()
+        t = (time.perf_counter() - t0) / reps
+        moved = 3 * nbytes
+        out["config"] = {"workload": f"y += a x over two {nbytes >> 20} MiB FP64 vectors (k_axpy)",
+                         "bytes_per_launch": moved, "read_bytes": 2 * nbytes, "write_bytes": nbytes}
+        out["copy"] = {"us": round(t * 1e6, 1), "gbs": round(moved / t / 1e9, 1),
+                       "frac": round(moved / t / 1e9 / HBM_PEAK_GBS, 4), "reps": reps}
     else:
         raise SystemExit(f"unknown workload {which}")
     print(json.dumps(out), flush=True)
@@ -617,6 +622,7 @@ def main():
                          "nonlinear (Newton fixpoint loop) | copy (the streaming ceiling)")
     ap.add_argument("--size", type=int, default=0, help="single-instance workloads: N (default 2048 / 160 / 512)")
     ap.add_argument("--solve", action="store_true", help="c3: also solve")
+    ap.add_argument("--assembly-only", action="store_true", help="c2-p1/c2-p2/c3: assembly and SpMV rates only")
     ap.add_argument("--dt", type=float, default=1e-4, help="heat: time step size")
     ap.add_argument("--timesteps", type=int, default=20, help="heat: timed steps after the first")
     ap.add_argument("--ts-solver", default="auto", choices=["auto", "krylov", "mg"],

@@ -65,7 +65,8 @@ class LForm(ctypes.Structure):
 class SolveOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("precond", ctypes.c_int), ("precision", ctypes.c_int),
                 ("rtol", ctypes.c_double), ("maxit", I64), ("inner_rtol", ctypes.c_double),
-                ("inner_maxit", I64), ("check_every", ctypes.c_int)]
+                ("inner_maxit", I64), ("check_every", ctypes.c_int),
+                ("initial_guess", ctypes.c_int)]
 
 
 class SolveInfo(ctypes.Structure):

@@ -3,10 +3,10 @@
 // Many independent sparse systems ("problems") are solved together by one
 // sequence of kernel launches: every kernel walks a table of row tiles, each
 // tile belonging to one problem, and a per-problem phase machine decides what
-// a tile does (FP64 residual, FP32 inner CG step, FP64 update).  Scalars of
-// a problem are reduced by the last block that finishes one of its tiles, in
-// a fixed order, so a problem's iterates do not depend on which other
-// problems share the launch.  The launch sequence of one step never changes,
+// a tile does (FP64 residual, FP32 inner CG step, FP64 update).  Each tile
+// writes its partial dot products to its own slot; a separate reduce kernel
+// (k_eng_reduce) then sums a problem's slots in tile order, so a problem's
+// iterates do not depend on which other problems share the launch.  The launch sequence of one step never changes,
 // so it is captured once as a hipGraph and replayed; problems join and leave
 // between replays by rewriting the tile table.
 #pragma once

@@ -398,6 +398,10 @@ int plain_cg_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solv
 // Vorst 1992) with right Jacobi preconditioning, fixed dofs eliminated as in
 // the plain CG (x_d = b_d / A_dd, rows masked).  Host-driven: the dot
 // products are folded on the device in a fixed order and read back.
+// Near-breakdowns (the shadow residual nearly orthogonal to r or to A p, or
+// omega = 0) restart the recurrence from the true residual with a new
+// shadow residual; on Newton matrices of P2 forms at 10^5 dofs and more
+// they happen every few hundred iterations.
 
 // y = A x on free rows (0 on fixed rows); partials (w.y, y.y)
 __global__ __launch_bounds__(PB) void k_bic_spmv(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
@@ -452,6 +456,14 @@ __global__ __launch_bounds__(PB) void k_bic_resid(const int32_t *__restrict__ rp
         part[2 * blockIdx.x] = a0;
         part[2 * blockIdx.x + 1] = a1;
     }
+}
+
+// x = g on free rows (the initial guess), fixed rows keep x_d = b_d / A_dd
+__global__ void k_bic_guess(const uint8_t *__restrict__ fixed, const double *__restrict__ g, double *__restrict__ x,
+                            int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x; i < n; i += (int64_t)gridDim.x * PB)
+        if (!fixed[i]) x[i] = g[i];
 }
 
 __global__ void k_bic_scale(const double *__restrict__ d, const double *__restrict__ in, double *__restrict__ out,
@@ -528,6 +540,7 @@ int bicgstab_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solv
     double *part = sh + n, *fold = part + 2 * PGRID_ROWS;
     const int grid = (int)std::min<int64_t>(PGRID_ROWS, std::max<int64_t>(1, (n + PB - 1) / PB));
     const int gride = (int)std::min<int64_t>(PGRID, grid);
+    if (o.initial_guess) GRMP_HIP(ctx, hipMemcpyAsync(sh, xd, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     hipLaunchKernelGGL(k_plain_init, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, A->rowptr, A->diagpos, A->val64,
                        A->fixed, bd, xd, n);
     hipLaunchKernelGGL(k_bic_resid, dim3(grid), dim3(PB), 0, ctx->stream, A->rowptr, A->cols, A->val64, A->fixed, bd, xd,
@@ -535,6 +548,20 @@ int bicgstab_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solv
     double rho, rr;
     GRMP_TRY(plain_sum(ctx, part, fold, B.hp, grid, rho, rr));
     const double bb = rr, tol2 = o.rtol * o.rtol * bb;
+    // r = rh = p = b - A x at the current x
+    auto restart = [&](double &rho_, double &rr_) -> int {
+        hipLaunchKernelGGL(k_bic_resid, dim3(grid), dim3(PB), 0, ctx->stream, A->rowptr, A->cols, A->val64, A->fixed, bd,
+                           xd, r, rh, p, n, part);
+        return plain_sum(ctx, part, fold, B.hp, grid, rho_, rr_);
+    };
+    if (o.initial_guess) {
+        hipLaunchKernelGGL(k_bic_guess, dim3(gride), dim3(PB), 0, ctx->stream, A->fixed, sh, xd, n);
+        GRMP_TRY(restart(rho, rr));
+    }
+    double rhh = rr;              // rh.rh
+    const double brk = 1e-15;     // near-breakdown: |cosine| below this
+    const int max_restarts = 200;
+    int restarts = 0;
     int64_t it = 0;
     int status = GRMP_OK;
     while (true) {
@@ -543,18 +570,32 @@ int bicgstab_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solv
             break;
         }
         if (rr <= tol2 || rr == 0.0) break;
-        if (it >= o.maxit || rho == 0.0) {
+        if (it >= o.maxit) {
             status = GRMP_ERR_NOCONV;
             break;
+        }
+        if (std::fabs(rho) <= brk * std::sqrt(rhh * rr)) {
+            if (++restarts > max_restarts) {
+                status = GRMP_ERR_NOCONV;
+                break;
+            }
+            GRMP_TRY(restart(rho, rr));
+            rhh = rr;
+            continue;
         }
         double rv, vv, ss, dummy, ts, tt;
         hipLaunchKernelGGL(k_bic_scale, dim3(gride), dim3(PB), 0, ctx->stream, A->dinv64, p, ph, n);
         hipLaunchKernelGGL(k_bic_spmv, dim3(grid), dim3(PB), 0, ctx->stream, A->rowptr, A->cols, A->val64, A->fixed, ph,
                            rh, v, n, part);
         GRMP_TRY(plain_sum(ctx, part, fold, B.hp, grid, rv, vv));
-        if (rv == 0.0) {
-            status = GRMP_ERR_NOCONV;
-            break;
+        if (std::fabs(rv) <= brk * std::sqrt(rhh * vv)) {
+            if (++restarts > max_restarts) {
+                status = GRMP_ERR_NOCONV;
+                break;
+            }
+            GRMP_TRY(restart(rho, rr));
+            rhh = rr;
+            continue;
         }
         const double alpha = rho / rv;
         hipLaunchKernelGGL(k_bic_s, dim3(gride), dim3(PB), 0, ctx->stream, r, v, alpha, s, n, part);
@@ -575,9 +616,15 @@ int bicgstab_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solv
                            n, part);
         double rho2;
         GRMP_TRY(plain_sum(ctx, part, fold, B.hp, gride, rho2, rr));
-        if (omega == 0.0) {
-            status = rr <= tol2 ? GRMP_OK : GRMP_ERR_NOCONV;
-            break;
+        if (omega == 0.0) {   // x and r advanced by alpha only: restart from there
+            if (rr <= tol2) break;
+            if (++restarts > max_restarts) {
+                status = GRMP_ERR_NOCONV;
+                break;
+            }
+            GRMP_TRY(restart(rho, rr));
+            rhh = rr;
+            continue;
         }
         const double beta = (rho2 / rho) * (alpha / omega);
         rho = rho2;
@@ -586,7 +633,7 @@ int bicgstab_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solv
     GRMP_HIP(ctx, hipGetLastError());
     GRMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     info.iterations = it;
-    info.outer = 1;
+    info.outer = 1 + restarts;
     info.relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
     info.status = status;
     return GRMP_OK;
@@ -1257,6 +1304,7 @@ extern "C" void grmp_solve_opts_default(grmp_solve_opts *o)
     o->inner_rtol = 0.1;
     o->inner_maxit = 100000;
     o->check_every = 1;
+    o->initial_guess = 0;
 }
 
 static int solve_engine(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, const grmp_solve_opts &o, grmp_solve_info &info)
@@ -1335,6 +1383,8 @@ extern "C" int grmp_solve(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, 
                GRMP_ERR_UNSUPPORTED, "grmp_solve: method %d is not built", o.method);
     GRMP_CHECK(ctx, o.method != GRMP_SOLVE_BICGSTAB || o.precision == GRMP_PRECISION_FP64, GRMP_ERR_UNSUPPORTED,
                "grmp_solve: BiCGStab runs in FP64 only");
+    GRMP_CHECK(ctx, o.initial_guess == 0 || o.method == GRMP_SOLVE_BICGSTAB, GRMP_ERR_UNSUPPORTED,
+               "grmp_solve: only BiCGStab starts from an initial guess");
     std::lock_guard<std::mutex> g(ctx->mu);
     double t0 = wall_seconds();
     int rc;

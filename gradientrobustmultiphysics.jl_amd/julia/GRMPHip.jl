@@ -67,6 +67,7 @@ struct GrmpSolveOpts
     inner_rtol::Cdouble
     inner_maxit::Int64
     check_every::Cint
+    initial_guess::Cint
 end
 
 mutable struct GrmpSolveInfo

@@ -691,7 +691,7 @@ def solve(Solution, pde, config=None, solver="krylov", **kwargs):
     return info, A, b
 
 
-def solve_system(A, b, Solution, cfg, method=_abi.SOLVE_CG):
+def solve_system(A, b, Solution, cfg, method=_abi.SOLVE_CG, initial_guess=False):
     o = _abi.SolveOpts()
     lib().grmp_solve_opts_default(ctypes.byref(o))
     o.method = method
@@ -700,6 +700,7 @@ def solve_system(A, b, Solution, cfg, method=_abi.SOLVE_CG):
     o.maxit = int(cfg.maxiterations)
     o.inner_rtol = cfg.inner_rtol if method == _abi.SOLVE_CG else cfg.minres_inner_rtol
     o.check_every = 16 if method == _abi.SOLVE_MINRES else 1
+    o.initial_guess = 1 if initial_guess else 0
     info = _abi.SolveInfo()
     rc = lib().grmp_solve(A.h, b.vec.h, Solution.vec.h, ctypes.byref(o), ctypes.byref(info))
     if rc not in (_abi.OK,):
@@ -718,6 +719,7 @@ class FixpointInfo:
 
     def __init__(self):
         self.status, self.iterations, self.residuals, self.linear_iterations = 0, 0, [], []
+        self.linear_restarts = []  # BiCGStab restarts after near-breakdowns, per Newton step
 
     @property
     def relres(self):
@@ -778,8 +780,10 @@ def solve_fixpoint(Solution, pde, cfg):
     for it in range(1, cfg.nonlinear_maxiterations + 1):
         if fixed.size:  # b[fixed] = penalty * Target[fixed]
             A.ctx.check(lib().grmp_matrix_penalize_rhs(A.h, Solution.vec.h, b.vec.h))
-        li = solve_system(A, b, Solution, cfg, _abi.SOLVE_BICGSTAB)
+        # BiCGStab from the current iterate (the Newton step is the full system for the new iterate)
+        li = solve_system(A, b, Solution, cfg, _abi.SOLVE_BICGSTAB, initial_guess=True)
         info.linear_iterations.append(int(li.iterations))
+        info.linear_restarts.append(int(li.outer) - 1)
         if cfg.damping > 0:
             x = cfg.damping * last + (1.0 - cfg.damping) * Solution.entries()
             Solution.set_entries(x)

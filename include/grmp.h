@@ -141,6 +141,9 @@ typedef struct grmp_solve_opts {
     int64_t inner_maxit;      /* FP32_IR MINRES: iteration cap of each inner solve */
     int check_every;          /* MINRES: status poll period in iterations (results do not
                                  depend on it) */
+    int initial_guess;        /* BiCGStab: 1 = start from the free entries of x (the previous
+                                 Newton iterate); the target stays rtol times the residual of
+                                 the zero start.  CG and MINRES start from zero and reject 1 */
 } grmp_solve_opts;
 
 typedef struct grmp_solve_info {

@@ -114,3 +114,65 @@ def test_newton_p1_matches_oracle(grmp, O):
           f"max rel diff {err:.2e}")
     assert info.status == 0 and err <= 1e-9
     assert abs(info.iterations - len(res)) <= 1
+
+
+def test_bicgstab_initial_guess(grmp, O):
+    """a solution passed back as the initial guess is accepted at once; CG
+    and MINRES refuse an initial guess"""
+    print("[case] bicgstab initial guess", flush=True)
+    import ctypes
+    grid, fes, coords, cells, pts = _setup(grmp, O, 9, 2, seed=5)
+    pde, nlf = _pde(grmp)
+    X = grmp.FEVector([fes])
+    X.set_entries(np.random.default_rng(3).standard_normal(fes.ndofs))
+    A = grmp.FEMatrix([fes])
+    form, keep = nlf.form(fes, 0, 0, grid.ctx, current=X)
+    A.add_form(form, keep)
+    A.set_fixed(fes.boundary_dofs())
+    A.build_pattern()
+    A.assemble(grmp._abi.VALUES_FP64)
+    b = grmp.FEVector([fes])
+    b.set_entries(np.random.default_rng(4).standard_normal(fes.ndofs))
+    cfg = grmp.SolverConfig(rtol=1e-12)
+    x = grmp.FEVector([fes])
+    i1 = grmp.solve_system(A, b, x, cfg, grmp._abi.SOLVE_BICGSTAB)
+    x1 = x.entries()
+    i2 = grmp.solve_system(A, b, x, cfg, grmp._abi.SOLVE_BICGSTAB, initial_guess=True)
+    print(f"  cold {i1.iterations} iterations ({i1.outer - 1} restarts), warm {i2.iterations}")
+    assert i1.status == 0 and i2.status == 0
+    assert i2.iterations == 0 and np.array_equal(x.entries(), x1)
+    # against a sparse LU of the free block
+    K = A.to_scipy().tocsr()
+    fixed = fes.boundary_dofs()
+    free = np.setdiff1d(np.arange(fes.ndofs), fixed)
+    import scipy.sparse.linalg as spla
+    bb = b.entries()
+    xr = np.zeros(fes.ndofs)
+    xr[fixed] = bb[fixed] / K.diagonal()[fixed]
+    xr[free] = spla.spsolve(K[free][:, free].tocsc(), bb[free] - K[free][:, fixed] @ xr[fixed])
+    assert np.max(np.abs(x1 - xr)) <= 1e-9 * np.max(np.abs(xr))
+    o = grmp._abi.SolveOpts()
+    grmp.lib().grmp_solve_opts_default(ctypes.byref(o))
+    o.initial_guess = 1
+    for m in (grmp._abi.SOLVE_CG, grmp._abi.SOLVE_MINRES):
+        o.method = m
+        inf = grmp._abi.SolveInfo()
+        assert grmp.lib().grmp_solve(A.h, b.vec.h, x.vec.h, ctypes.byref(o), ctypes.byref(inf)) == \
+            grmp._abi.ERR_UNSUPPORTED
+
+
+def test_newton_p2_large(grmp):
+    """the fixpoint loop at 148,225 P2 dofs, where BiCGStab's recurrence
+    meets near-breakdowns and restarts: ||u_h - x y||_L2 below 1e-9"""
+    print("[case] newton P2 192x192", flush=True)
+    N = 192
+    grid = grmp.grid_unitsquare(N)
+    fes = grmp.FESpace(grmp.H1P2(1), grid)
+    pde, nlf = _pde(grmp)
+    sol = grmp.FEVector([fes])
+    info, A, b = grmp.solve(sol, pde, target_residual=1e-10, nonlinear_maxiterations=12, rtol=1e-12)
+    err = float(grmp.evaluate(grmp.L2ErrorIntegrator(grmp.PolynomialData([0, 0, 0, 0, 1.0]), quadorder=2),
+                              sol[0])) ** 0.5
+    print(f"  {info.iterations} Newton steps {['%.1e' % r for r in info.residuals]}, BiCGStab "
+          f"{info.linear_iterations}, restarts {info.linear_restarts}, L2 error {err:.2e}")
+    assert info.status == 0 and err < 1e-9
