@@ -3,7 +3,8 @@
 For each single-form kind of the element loop (P1 / P2 Laplace on
 triangles, P1 Laplace on tetrahedra, P1{2} Hooke, P1 mass) it measures every
 launch shape -- kernel variant (1: one thread walks a row's cells; 2: the
-row block's cell work staged over the workgroup), rows per workgroup, and
+row block's cell work staged over the workgroup; 3: as 2 with several
+cells' gathers in flight per thread), rows per workgroup, and
 XCD-contiguous row blocks on or off -- on a mesh of the benchmark's scale,
 checks on a small mesh that each shape gives bitwise the values of the
 built-in shape, and writes the fastest per (form kind, quadrature points,
@@ -82,6 +83,8 @@ def _shapes():
             out.append((1, rows, remap))
         for rows in (32, 64, 128, 256):
             out.append((2, rows, remap))
+        for rows in (32, 64, 128, 256):
+            out.append((3, rows, remap))
     return out
 
 
@@ -118,7 +121,8 @@ def tune(g, quick=False, reps=3, log=print):
         A, keep = build(False)
         kind, nq, bucket = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         A.ctx.check(g.lib().grmp_matrix_tune_key(A.h, ctypes.byref(kind), ctypes.byref(nq), ctypes.byref(bucket)))
-        _set(g, A, None)
+        # the library's built-in shape, whatever table the context loaded
+        _set(g, A, (0, 0, -1))
         A.assemble(g._abi.VALUES_FP64)
         t_builtin = _time(g, A, reps)
         best, t_best = None, t_builtin

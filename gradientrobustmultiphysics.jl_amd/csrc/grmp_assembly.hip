@@ -94,13 +94,9 @@ __device__ __forceinline__ double p2tri_der(int a, int j, double x0, double x1)
 // Triangle: A = [x2-x1, x3-x1], det = A11 A22 - A12 A21,
 // M = A^{-T} = [A22 -A21; -A12 A11]/det, vol = |det|/2
 // (ExtendableGrids update_trafo!/mapderiv!, called at src/feevaluator.jl:371-380).
-__device__ __forceinline__ void trafo_tri(const DevForm &F, int64_t e, double M[2][2], double &vol)
+__device__ __forceinline__ void trafo_xy(double x0, double y0, double x1, double y1, double x2, double y2,
+                                         double M[2][2], double &vol)
 {
-    const int64_t nc = F.ncells, nn = F.nnodes;
-    int32_t v0 = F.cells[e], v1 = F.cells[nc + e], v2 = F.cells[2 * nc + e];
-    double x0 = F.coords[v0], y0 = F.coords[nn + v0];
-    double x1 = F.coords[v1], y1 = F.coords[nn + v1];
-    double x2 = F.coords[v2], y2 = F.coords[nn + v2];
     double A11 = x1 - x0, A12 = x2 - x0, A21 = y1 - y0, A22 = y2 - y0;
     double det = A11 * A22 - A12 * A21;
     M[0][0] = A22 / det;
@@ -110,16 +106,15 @@ __device__ __forceinline__ void trafo_tri(const DevForm &F, int64_t e, double M[
     vol = fabs(det) * 0.5;
 }
 
-__device__ __forceinline__ void trafo_tet(const DevForm &F, int64_t e, double M[3][3], double &vol)
+__device__ __forceinline__ void trafo_tri(const DevForm &F, int64_t e, double M[2][2], double &vol)
 {
     const int64_t nc = F.ncells, nn = F.nnodes;
-    double X[4][3];
-    for (int v = 0; v < 4; ++v) {
-        int32_t n = F.cells[v * nc + e];
-        X[v][0] = F.coords[n];
-        X[v][1] = F.coords[nn + n];
-        X[v][2] = F.coords[2 * nn + n];
-    }
+    int32_t v0 = F.cells[e], v1 = F.cells[nc + e], v2 = F.cells[2 * nc + e];
+    trafo_xy(F.coords[v0], F.coords[nn + v0], F.coords[v1], F.coords[nn + v1], F.coords[v2], F.coords[nn + v2], M, vol);
+}
+
+__device__ __forceinline__ void trafo_x(const double X[4][3], double M[3][3], double &vol)
+{
     double A[3][3];
     for (int r = 0; r < 3; ++r)
         for (int k = 0; k < 3; ++k) A[r][k] = X[k + 1][r] - X[0][r];
@@ -137,6 +132,61 @@ __device__ __forceinline__ void trafo_tet(const DevForm &F, int64_t e, double M[
     M[1][0] = c10 / det; M[1][1] = c11 / det; M[1][2] = c12 / det;
     M[2][0] = c20 / det; M[2][1] = c21 / det; M[2][2] = c22 / det;
     vol = fabs(det) / 6.0;
+}
+
+__device__ __forceinline__ void trafo_tet(const DevForm &F, int64_t e, double M[3][3], double &vol)
+{
+    const int64_t nc = F.ncells, nn = F.nnodes;
+    double X[4][3];
+    for (int v = 0; v < 4; ++v) {
+        int32_t n = F.cells[v * nc + e];
+        X[v][0] = F.coords[n];
+        X[v][1] = F.coords[nn + n];
+        X[v][2] = F.coords[2 * nn + n];
+    }
+    trafo_x(X, M, vol);
+}
+
+// Corner coordinates and cell coefficient of one cell, gathered in two
+// dependent steps (node ids, then coordinates and coefficient) so that a
+// thread can keep the gathers of several cells in flight at once.  The
+// row functions below take them and compute exactly what they compute from
+// (F, e).
+struct TriGeo {
+    double x[3], y[3], kap;
+};
+struct TetGeo {
+    double X[4][3], kap;
+};
+
+template <int NV>
+__device__ __forceinline__ void cell_nodes(const DevForm &F, int64_t e, int32_t *nv)
+{
+#pragma unroll
+    for (int v = 0; v < NV; ++v) nv[v] = F.cells[(int64_t)v * F.ncells + e];
+}
+
+__device__ __forceinline__ void geo_load(const DevForm &F, int64_t e, const int32_t *nv, TriGeo &g)
+{
+    const int64_t nn = F.nnodes;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        g.x[v] = F.coords[nv[v]];
+        g.y[v] = F.coords[nn + nv[v]];
+    }
+    g.kap = F.coef_cell ? F.kappa[e] : 1.0;
+}
+
+__device__ __forceinline__ void geo_load(const DevForm &F, int64_t e, const int32_t *nv, TetGeo &g)
+{
+    const int64_t nn = F.nnodes;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        g.X[v][0] = F.coords[nv[v]];
+        g.X[v][1] = F.coords[nn + nv[v]];
+        g.X[v][2] = F.coords[2 * nn + nv[v]];
+    }
+    g.kap = F.coef_cell ? F.kappa[e] : 1.0;
 }
 
 // ----------------------------------------------------------- local rows
@@ -165,12 +215,12 @@ __device__ __forceinline__ void grad_tri(const double M[2][2], int d, double x0,
 
 // (kappa grad u, grad v) with P1/P2 on triangles; NQ quadrature points.
 template <int FE, int NQ>
-__device__ __forceinline__ void row_laplace_tri(const DevForm &F, int64_t e, int a, double *v)
+__device__ __forceinline__ void row_laplace_tri_g(const DevForm &F, const TriGeo &G, int a, double *v)
 {
     constexpr int ND = (FE == 1) ? 3 : 6;
     double M[2][2], vol;
-    trafo_tri(F, e, M, vol);
-    const double kap = F.coef_cell ? F.kappa[e] : 1.0;
+    trafo_xy(G.x[0], G.y[0], G.x[1], G.y[1], G.x[2], G.y[2], M, vol);
+    const double kap = G.kap;
     const double itemfactor = vol * F.factor * 1.0;
     double ga[NQ][2];
 #pragma unroll
@@ -217,11 +267,11 @@ __device__ __forceinline__ void grad_tet(const double M[3][3], int d, double g[3
     }
 }
 
-__device__ __forceinline__ void row_laplace_tet(const DevForm &F, int64_t e, int a, double *v)
+__device__ __forceinline__ void row_laplace_tet_g(const DevForm &F, const TetGeo &G, int a, double *v)
 {
     double M[3][3], vol;
-    trafo_tet(F, e, M, vol);
-    const double kap = F.coef_cell ? F.kappa[e] : 1.0;
+    trafo_x(G.X, M, vol);
+    const double kap = G.kap;
     const double itemfactor = vol * F.factor * 1.0;
     double ga[3];
     grad_tet(M, a, ga);
@@ -250,11 +300,11 @@ __device__ __forceinline__ void row_laplace_tet(const DevForm &F, int64_t e, int
 // compression [1,3,3,2]): a dof of component 0 gives [gx, 0, gy], of
 // component 1 gives [0, gy, gx].
 template <int NQ>
-__device__ __forceinline__ void row_hooke_tri(const DevForm &F, int64_t e, int a, double *v)
+__device__ __forceinline__ void row_hooke_tri_g(const DevForm &F, const TriGeo &G, int a, double *v)
 {
     double M[2][2], vol;
-    trafo_tri(F, e, M, vol);
-    double kap = F.coef_cell ? F.kappa[e] : 1.0;
+    trafo_xy(G.x[0], G.y[0], G.x[1], G.y[1], G.x[2], G.y[2], M, vol);
+    double kap = G.kap;
     double lam = kap * F.lam, mu = kap * F.mu;
     const int ca = a / 3, sa = a - 3 * ca;
     double ga[2];
@@ -287,12 +337,12 @@ __device__ __forceinline__ void row_hooke_tri(const DevForm &F, int64_t e, int a
 
 // (kappa u, v) with P1/P2 on triangles (ReactionOperator, symmetric).
 template <int FE, int NQ>
-__device__ __forceinline__ void row_mass_tri(const DevForm &F, int64_t e, int a, double *v)
+__device__ __forceinline__ void row_mass_tri_g(const DevForm &F, const TriGeo &G, int a, double *v)
 {
     constexpr int ND = (FE == 1) ? 3 : 6;
     double M[2][2], vol;
-    trafo_tri(F, e, M, vol);
-    const double kap = F.coef_cell ? F.kappa[e] : 1.0;
+    trafo_xy(G.x[0], G.y[0], G.x[1], G.y[1], G.x[2], G.y[2], M, vol);
+    const double kap = G.kap;
     const double itemfactor = vol * F.factor * 1.0;
 #pragma unroll
     for (int b = 0; b < ND; ++b) {
@@ -558,16 +608,38 @@ __device__ __forceinline__ void row_nldiff_tri(const DevForm &F, int64_t e, int 
 template <int CN_, void (*ROW)(const DevForm &, int64_t, int, double *)>
 struct Kind {
     static constexpr int CN = CN_;
+    static constexpr bool SPLIT = false;  // no separate gather step
+    static constexpr int NV = 1;
+    using Geo = int;
     __device__ static __forceinline__ void row(const DevForm &F, int64_t e, int a, double *v) { ROW(F, e, a, v); }
+    __device__ static __forceinline__ void rowg(const DevForm &, const Geo &, int, double *) {}
 };
-using K_LAP_P1T1 = Kind<3, row_laplace_tri<1, 1>>;
-using K_LAP_P1T3 = Kind<3, row_laplace_tri<1, 3>>;
-using K_LAP_P2T3 = Kind<6, row_laplace_tri<2, 3>>;
-using K_LAP_P1TET = Kind<4, row_laplace_tet>;
-using K_HOOKE1 = Kind<6, row_hooke_tri<1>>;
-using K_HOOKE3 = Kind<6, row_hooke_tri<3>>;
-using K_MASS_P1T1 = Kind<3, row_mass_tri<1, 1>>;
-using K_MASS_P1T3 = Kind<3, row_mass_tri<1, 3>>;
+// kinds whose rows depend on the cell only through its corners and
+// coefficient: row() gathers them, rowg() computes from a gathered cell
+template <int CN_, class GEO, int NV_, void (*ROWG)(const DevForm &, const GEO &, int, double *)>
+struct KindG {
+    static constexpr int CN = CN_;
+    static constexpr bool SPLIT = true;
+    static constexpr int NV = NV_;
+    using Geo = GEO;
+    __device__ static __forceinline__ void rowg(const DevForm &F, const Geo &g, int a, double *v) { ROWG(F, g, a, v); }
+    __device__ static __forceinline__ void row(const DevForm &F, int64_t e, int a, double *v)
+    {
+        int32_t nv[NV];
+        cell_nodes<NV>(F, e, nv);
+        Geo g;
+        geo_load(F, e, nv, g);
+        ROWG(F, g, a, v);
+    }
+};
+using K_LAP_P1T1 = KindG<3, TriGeo, 3, row_laplace_tri_g<1, 1>>;
+using K_LAP_P1T3 = KindG<3, TriGeo, 3, row_laplace_tri_g<1, 3>>;
+using K_LAP_P2T3 = KindG<6, TriGeo, 3, row_laplace_tri_g<2, 3>>;
+using K_LAP_P1TET = KindG<4, TetGeo, 4, row_laplace_tet_g>;
+using K_HOOKE1 = KindG<6, TriGeo, 3, row_hooke_tri_g<1>>;
+using K_HOOKE3 = KindG<6, TriGeo, 3, row_hooke_tri_g<3>>;
+using K_MASS_P1T1 = KindG<3, TriGeo, 3, row_mass_tri_g<1, 1>>;
+using K_MASS_P1T3 = KindG<3, TriGeo, 3, row_mass_tri_g<1, 3>>;
 using K_LAP_BR = Kind<9, row_laplace_br<3>>;
 using K_DIV_BRP0 = Kind<1, row_div_brp0>;
 using K_DIVT_P0BR = Kind<9, row_divt_p0br>;
@@ -904,6 +976,128 @@ __global__ __launch_bounds__(ASM_NT) void k_assemble1(DevForm F, int rb, int rem
     }
 }
 
+// k_assemble1 with the cell gathers batched: in step 1 each thread takes NB
+// incidences of the chunk (j = c0 + t + k ASM_NT) and issues all their
+// loads level by level -- incidence words and slot words, node ids, corner
+// coordinates and coefficients -- before computing any local row, so a
+// thread waits for NB cells' memory once instead of NB times.  Step 1 also
+// stages the slot words in LDS, so step 2 (each row's thread adding its
+// incidences in incidence order) reads LDS only.  Summation order per entry
+// is the serial walk's, so the values are the same bits.  Kinds with a
+// separate gather step only.
+#define ASMG_STAGE 3072  // FP64 staging entries (24 KB)
+template <class K, int NB>
+__global__ __launch_bounds__(ASM_NT) void k_assemble1g(DevForm F, int rb, int remap, int64_t g0, int64_t g1,
+                                                       int64_t rblock, const int32_t *__restrict__ rowptr,
+                                                       const uint8_t *__restrict__ diagpos,
+                                                       const uint8_t *__restrict__ fixed, double penalty,
+                                                       double *__restrict__ val64, float *__restrict__ val32,
+                                                       double *__restrict__ dinv64, float *__restrict__ dinv32)
+{
+    constexpr int CN = K::CN, W = (CN + 3) / 4, NV = K::NV;
+    constexpr int SCAP = (ASMG_STAGE / CN < NB * ASM_NT) ? ASMG_STAGE / CN : NB * ASM_NT;
+    extern __shared__ double shm[];
+    double *stage = shm;                                            // SCAP * CN local values
+    uint32_t *wst = reinterpret_cast<uint32_t *>(shm + ASMG_STAGE);  // SCAP * W slot words
+    double *buf = shm + ASMG_STAGE + (SCAP * W + 1) / 2;            // the block's rows
+    const int t = threadIdx.x;
+    const int64_t rs = g0 + row_block_of(remap) * rb;
+    const int64_t re = min(rs + rb, g1);
+    const int32_t base = rowptr[rs];
+    const int32_t len = rowptr[re] - base;
+    for (int32_t k = t; k < len; k += ASM_NT) buf[k] = 0.0;
+    const int64_t g = rs + t;
+    const bool mine = t < rb && g < re;
+    int32_t off = 0, rl = 0, my0 = 0, my1 = 0;
+    if (mine) {
+        off = rowptr[g] - base;
+        rl = rowptr[g + 1] - rowptr[g];
+        my0 = F.inc_ptr[g - rblock];
+        my1 = F.inc_ptr[g - rblock + 1];
+    }
+    const int32_t i0 = F.inc_ptr[rs - rblock], i1 = F.inc_ptr[re - rblock];
+    for (int32_t c0 = i0; c0 < i1; c0 += SCAP) {
+        const int32_t c1 = min(c0 + SCAP, i1);
+        __syncthreads();
+        for (int32_t jb = c0 + t; jb < c1; jb += NB * ASM_NT) {
+            // past the chunk's end a lane repeats the chunk's last incidence and stores nothing
+            int32_t pk[NB];
+            uint32_t w[NB][W];
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int32_t j = min(jb + k * ASM_NT, c1 - 1);
+                pk[k] = F.inc[j];
+#pragma unroll
+                for (int q = 0; q < W; ++q) w[k][q] = F.pos[(int64_t)j * W + q];
+            }
+            int64_t e[NB];
+            int32_t nv[NB][NV];
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                e[k] = pk[k] / F.rnloc;
+                cell_nodes<NV>(F, e[k], nv[k]);
+            }
+            typename K::Geo G[NB];
+#pragma unroll
+            for (int k = 0; k < NB; ++k) geo_load(F, e[k], nv[k], G[k]);
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int32_t j = jb + k * ASM_NT;
+                if (j < c1) {
+                    const int a = pk[k] - (int)e[k] * F.rnloc;
+                    double v[CN];
+                    K::rowg(F, G[k], a, v);
+#pragma unroll
+                    for (int b = 0; b < CN; ++b) stage[(j - c0) * CN + b] = v[b];
+#pragma unroll
+                    for (int q = 0; q < W; ++q) wst[(j - c0) * W + q] = w[k][q];
+                }
+            }
+        }
+        __syncthreads();
+        if (mine) {
+            const int32_t ja = max(my0, c0), jb = min(my1, c1);
+            for (int32_t j = ja; j < jb; ++j) {
+#pragma unroll
+                for (int b = 0; b < CN; ++b) {
+                    const uint32_t p = (wst[(j - c0) * W + (b >> 2)] >> (8 * (b & 3))) & 255u;
+                    if (p != 255u) buf[off + p] += stage[(j - c0) * CN + b];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    bool fx = false;
+    if (mine) {
+        fx = fixed[g] != 0;
+        const uint8_t dp = diagpos[g];
+        if (fx && dp != 255) buf[off + dp] = penalty;
+        const double d = (dp != 255) ? buf[off + dp] : 0.0;
+        const double di = (fx || d == 0.0) ? 0.0 : 1.0 / d;
+        if (dinv64) dinv64[g] = di;
+        if (dinv32) dinv32[g] = (float)di;
+    }
+    __syncthreads();
+    if (val64)
+        for (int32_t k = t; k < len; k += ASM_NT) val64[base + k] = buf[k];
+    if (val32) {
+        __syncthreads();
+        if (mine && fx)
+            for (int32_t k = 0; k < rl; ++k) buf[off + k] = 0.0;
+        __syncthreads();
+        for (int32_t k = t; k < len; k += ASM_NT) val32[base + k] = (float)buf[k];
+    }
+}
+
+// the LDS bytes of k_assemble1g<K, NB> with rb rows of at most mr entries
+template <class K, int NB>
+static size_t asmg_lds(int rb, int mr)
+{
+    constexpr int CN = K::CN, W = (CN + 3) / 4;
+    constexpr int SCAP = (ASMG_STAGE / CN < NB * ASM_NT) ? ASMG_STAGE / CN : NB * ASM_NT;
+    return (ASMG_STAGE + (SCAP * W + 1) / 2 + (size_t)rb * mr) * sizeof(double);
+}
+
 // One form of a known kind, one thread per row walking its cells serially:
 // the better choice when rows have few incidences (triangles)
 template <class K>
@@ -935,11 +1129,30 @@ static AsmTune asm_tune(const grmp_matrix *A, const DevForm &hf)
 
 static int asm_block_size(int max_row);
 
+// variant 3 for kinds with a separate gather step (NB cells in flight per
+// thread: the stage holds NB * ASM_NT local rows); false for the others
+template <class K>
+static bool launch_assemble1g(const DevForm &F, int rb, int remap, int64_t g0, int64_t g1, const grmp_matrix *A,
+                              int mr, double *val64, float *val32, double *dinv64, float *dinv32, hipStream_t st)
+{
+    if constexpr (K::SPLIT) {
+        constexpr int NB = ASMG_STAGE / (K::CN * ASM_NT) > 0 ? ASMG_STAGE / (K::CN * ASM_NT) : 1;
+        const size_t lds = asmg_lds<K, NB>(rb, mr);
+        if (lds > 64 * 1024) return false;
+        hipLaunchKernelGGL((k_assemble1g<K, NB>), dim3(nblk(g1 - g0, rb)), dim3(ASM_NT), lds, st, F, rb, remap, g0, g1,
+                           g0, A->rowptr, A->diagpos, A->fixed, A->penalty, val64, val32, dinv64, dinv32);
+        return true;
+    } else {
+        return false;
+    }
+}
+
 // Launches the value assembly of row block [g0, g1): a per-kind kernel when
 // the block has one form (hf = its host copy), the generic one otherwise.
 // Built-in choice for one form: the serial row walk up to 12 cell incidences
-// per row (triangles), the staged kernel above (tetrahedra), blocks in the
-// hardware's order.
+// per row (triangles), the staged kernel with batched gathers above
+// (tetrahedra; the staged kernel for kinds without a gather step), blocks in
+// the hardware's order.
 static void launch_assemble(const grmp_matrix *A, const grmp_space *sp, const DevForm *hf, FormPack P, int64_t g0,
                             int64_t g1, int B, size_t shm, double *val64, float *val32, double *dinv64, float *dinv32,
                             hipStream_t st)
@@ -948,7 +1161,7 @@ static void launch_assemble(const grmp_matrix *A, const grmp_space *sp, const De
     const int mr = A->max_row > 0 ? A->max_row : 1;
     if (P.n == 1) {
         const AsmTune T = asm_tune(A, *hf);
-        const int variant = T.variant ? T.variant : (inc_per_row <= 12.0 ? 1 : 2);
+        const int variant = T.variant ? T.variant : (inc_per_row <= 12.0 ? 1 : 3);
         const int remap = T.remap >= 0 ? T.remap : 0;
         if (variant == 1) {
             // rows per block: a multiple of 64 whose rows fit the 48 KB of LDS
@@ -966,6 +1179,12 @@ static void launch_assemble(const grmp_matrix *A, const grmp_space *sp, const De
         int rb = ASM_NT;
         while (rb > 32 && ((double)rb * inc_per_row > 6.0 * ASM_NT || (size_t)rb * mr * 8 > 48 * 1024)) rb >>= 1;
         if (T.rows >= 8 && T.rows <= ASM_NT && (size_t)T.rows * mr * 8 <= 48 * 1024) rb = T.rows;
+        if (variant == 3 && (size_t)rb * mr * 8 <= 40 * 1024) {
+            const DevForm F = *hf;
+            bool done = false;
+            GRMP_DISPATCH(F, done = launch_assemble1g<KT>(F, rb, remap, g0, g1, A, mr, val64, val32, dinv64, dinv32, st));
+            if (done) return;
+        }
         if ((size_t)rb * mr * 8 <= 48 * 1024) {
             const DevForm F = *hf;
             const size_t shm1 = (ASM_STAGE + (size_t)rb * mr) * sizeof(double);
@@ -1509,9 +1728,9 @@ extern "C" int grmp_matrix_set_tune(grmp_matrix *A, const grmp_asm_tune *t)
         A->tune_set = false;
         return GRMP_OK;
     }
-    GRMP_CHECK(ctx, t->variant >= 0 && t->variant <= 2 && t->rows_per_block >= 0 && t->rows_per_block <= 1024 &&
+    GRMP_CHECK(ctx, t->variant >= 0 && t->variant <= 3 && t->rows_per_block >= 0 && t->rows_per_block <= 1024 &&
                         t->xcd_remap >= -1 && t->xcd_remap <= 1,
-               GRMP_ERR_ARG, "grmp_matrix_set_tune: variant 0..2, rows_per_block 0..1024, xcd_remap -1..1");
+               GRMP_ERR_ARG, "grmp_matrix_set_tune: variant 0..3, rows_per_block 0..1024, xcd_remap -1..1");
     A->tune.variant = t->variant;
     A->tune.rows = t->rows_per_block;
     A->tune.remap = t->xcd_remap;

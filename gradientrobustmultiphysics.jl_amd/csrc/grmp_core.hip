@@ -78,7 +78,7 @@ static int tune_parse(grmp_ctx *ctx, const std::string &text, std::vector<TuneEn
             json_int(obj, "variant", v);
             json_int(obj, "rows", rows);
             json_int(obj, "remap", remap);
-            GRMP_CHECK(ctx, v >= 0 && v <= 2 && rows >= 0 && rows <= 1024 && remap >= -1 && remap <= 1, GRMP_ERR_ARG,
+            GRMP_CHECK(ctx, v >= 0 && v <= 3 && rows >= 0 && rows <= 1024 && remap >= -1 && remap <= 1, GRMP_ERR_ARG,
                        "tuning table: entry out of range in %s", obj.c_str());
             TuneEntry e;
             e.kind = (int)kind;

@@ -13,7 +13,8 @@
 // ------------------------------------------------------------------ tuning
 // Launch shape of the element assembly of one form kind: the kernel variant
 // (1: one thread walks a row's cells; 2: the block's cell work staged in LDS
-// over all threads), the rows one workgroup owns, and whether consecutive row
+// over all threads; 3: as 2 with each thread's cell gathers batched), the
+// rows one workgroup owns, and whether consecutive row
 // blocks go to one XCD (1) or round-robin over the XCDs (0, the hardware's
 // dispatch order).  Zero fields mean the built-in choice.  Results do not
 // depend on it: every entry is summed in the same order by every variant.

@@ -268,7 +268,9 @@ double grmp_matrix_last_assembly_seconds(const grmp_matrix *A);
 /* Launch shape of the element assembly (no counterpart in the reference,
  * whose element loop is serial): variant 1 walks each row's cells in one
  * thread, 2 stages the row block's cell work over all threads of a
- * workgroup; rows_per_block the rows one workgroup owns; xcd_remap 1 gives
+ * workgroup, 3 is 2 with several cells' gathers in flight per thread (form
+ * kinds whose rows need only the cell's corners and coefficient; others run
+ * 2); rows_per_block the rows one workgroup owns; xcd_remap 1 gives
  * each XCD a contiguous share of the row blocks.  0 (-1 for xcd_remap)
  * keeps the built-in choice.  Values are bitwise the same for every shape.
  * grmp_matrix_set_tune overrides the context's table for one matrix (NULL
