@@ -243,7 +243,9 @@ int init_x_launch(const grmp_matrix *A, const double *val64, const double *b, do
 // b[i] = 0 on fixed rows (homogeneous Dirichlet, penalty * 0)
 int zero_fixed_launch(const grmp_matrix *A, double *b, hipStream_t st);
 int plain_cg_solve(grmp_matrix *A, const double *b, double *x, const grmp_solve_opts &o, grmp_solve_info &info);
-int minres_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solve_opts &o, grmp_solve_info &info);
+// vel_mg: the multigrid block of the block preconditioner (nullptr: diagonal)
+int minres_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solve_opts &o, grmp_solve_info &info,
+                 grmp_mg *vel_mg = nullptr, int vel_ncomp = 0);
 
 // multigrid-preconditioned CG on unit-square hierarchies (grmp_mg.hip)
 struct MgLevelIn {
@@ -268,6 +270,10 @@ const double *mg_job_solution(const MgJob *j);
 int mg_job_setup(MgJob *j, double rtol, int maxit);
 // mg_job_setup for n jobs on one stream with their level assembly in one launch
 int mg_jobs_setup_batched(MgJob *const *jobs, int n, hipStream_t st, double *const rtol, int maxit);
+// one V-cycle of a created hierarchy, z = V v on the hierarchy's stream (the
+// velocity block of the Stokes preconditioner); level-0 rows of the hierarchy
+int mg_precond_apply(grmp_mg *m, const double *v, double *z, hipStream_t st);
+int64_t mg_level0_rows(const grmp_mg *m);
 int mg_job_iterate(MgJob *j, int iters);
 int mg_job_verify(MgJob *j);         // after done == 2: check the true residual
 bool mg_job_block(const MgJob *j);   // whole iterations in one workgroup (checks in place)

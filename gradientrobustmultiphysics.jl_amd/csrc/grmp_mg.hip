@@ -1884,6 +1884,38 @@ extern "C" int grmp_mg_apply(grmp_mg *m, const grmp_vector *b, grmp_vector *x, d
     return inf.status;
 }
 
+// ------------------------------------------- one V-cycle as a preconditioner
+// r_0 = v with the fixed rows zeroed; the device scalars are reset so the
+// V-cycle kernels run (a finished CG leaves done set)
+__global__ void k_mg_pc_in(const double *__restrict__ v, const uint8_t *__restrict__ fixed, double *__restrict__ r,
+                           MgScal *__restrict__ sc, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        sc->done = 0;
+        sc->status = 0;
+    }
+    if (i < n) r[i] = fixed[i] ? 0.0 : v[i];
+}
+
+int64_t mg_level0_rows(const grmp_mg *m) { return m->n; }
+
+// z = V v: one V-cycle of the hierarchy from a zero start on stream st (the
+// hierarchy's own stream): symmetric and positive definite, so it can
+// precondition MINRES as it does CG.  z and v are n-vectors of the level-0
+// numbering; fixed rows of z are 0.
+int mg_precond_apply(grmp_mg *m, const double *v, double *z, hipStream_t st)
+{
+    MgJob *j = m->job;
+    const int64_t n = m->n;
+    GRMP_CHECK(m->ctx, st == j->st, GRMP_ERR_ARG, "multigrid preconditioner: not the hierarchy's stream");
+    hipLaunchKernelGGL(k_mg_pc_in, dim3(nblk(n, 256)), dim3(256), 0, st, v, m->A0->fixed, j->r, j->sc, n);
+    mg_vcycle(j);
+    GRMP_HIP(m->ctx, hipMemcpyAsync(z, j->z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    GRMP_HIP(m->ctx, hipGetLastError());
+    return GRMP_OK;
+}
+
 extern "C" int grmp_mg_solve(grmp_ctx *ctx, int nlev, const grmp_matrix *const *levels,
                              const grmp_space *const *spaces, const grmp_vector *kappa0, const grmp_vector *b,
                              grmp_vector *x, double rtol, int64_t maxit, grmp_solve_info *info)

@@ -15,6 +15,12 @@
 //   constraint rows:        1 / sum_j A_ij^2 / A_jj  (the diagonal of
 //                            B diag(K)^-1 B^T, a Schur-complement estimate)
 // stored as one signed array: its sign is the row's sign flip.
+// Block variant (grmp_solve_minres_mg): the node rows of each velocity
+// component -- the P1 part of H1BR{2}, on which the viscous block is nu
+// times the P1 Laplacian -- get one multigrid V-cycle of that Laplacian in
+// place of the diagonal; the face-bubble rows keep 1 / |A_ii| and the
+// constraint rows the Schur estimate.  Block diagonal and symmetric positive
+// definite, so MINRES's recurrence is unchanged.
 //
 // The recurrence is the preconditioned Lanczos / Givens form of MINRES
 // (Paige & Saunders 1975, as written in Elman, Silvester & Wathen, "Finite
@@ -24,6 +30,7 @@
 // status word every few iterations and results do not depend on the polling.
 // Fixed dofs are eliminated as in the CG solvers (x_d = b_d / A_dd).
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 
@@ -260,6 +267,16 @@ __global__ __launch_bounds__(KB) void k_mr_givens(MrScal *__restrict__ S, const 
     *S = s;
 }
 
+// partial z.v after the block preconditioner has rewritten z
+__global__ __launch_bounds__(KB) void k_mr_dot(const double *__restrict__ z, const double *__restrict__ v, int64_t n,
+                                              double *__restrict__ part)
+{
+    double acc = 0.0, dummy = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * KB + threadIdx.x; i < n; i += (int64_t)gridDim.x * KB) acc += z[i] * v[i];
+    block_sum2(acc, dummy);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
 // ------------------------------------------------------- FP64 refinement
 // r = S (b - A x) on free rows (FP64), partial r.|d| r
 __global__ __launch_bounds__(KB) void k_mr_resid64(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
@@ -337,17 +354,37 @@ struct MrHost {
     int grid, gride;
 };
 
+// the multigrid block of the block preconditioner: components 0..ncomp-1
+// occupy rows [k nv, (k+1) nv)
+struct MrPrec {
+    grmp_mg *mg = nullptr;
+    int ncomp = 0;
+    int64_t nv = 0;
+};
+
+// z = P v on the multigrid rows (the diagonal part is already in z), then the
+// np partials of z.v
+static int mr_precondition(MrHost &H, const MrPrec &pc, const double *v, double *z, int64_t n, int np)
+{
+    for (int k = 0; k < pc.ncomp; ++k) GRMP_TRY(mg_precond_apply(pc.mg, v + k * pc.nv, z + k * pc.nv, H.st));
+    hipLaunchKernelGGL(k_mr_dot, dim3(np), dim3(KB), 0, H.st, z, v, n, H.part);
+    return GRMP_OK;
+}
+
 
 // one MINRES run on S A y = rhs from y = y0 (the residual of y0 against rhs
 // is formed when Ainit != nullptr, else y0 must be 0); returns via hS
 template <class T, class V, class X, class R>
 int mr_run(MrHost &H, const grmp_matrix *A, const V *vals, const T *dsg, const R *rhs, X *y, bool y_nonzero,
-           MrBufs<T> &B, double rtol, int64_t maxit, int check_every, int64_t *iters_out)
+           MrBufs<T> &B, double rtol, int64_t maxit, int check_every, int64_t *iters_out, const MrPrec *pc = nullptr)
 {
     const int64_t n = A->n;
     const V *ainit = y_nonzero ? vals : nullptr;
     hipLaunchKernelGGL((k_mr_init<T, V, X, R>), dim3(H.grid), dim3(KB), 0, H.st, A->rowptr, A->cols, ainit, A->fixed,
                        dsg, rhs, y, B.V[1], B.Z[1], B.V[0], B.W[0], B.W[1], n, H.part);
+    if constexpr (std::is_same<T, double>::value) {
+        if (pc) GRMP_TRY(mr_precondition(H, *pc, B.V[1], B.Z[1], n, H.grid));
+    }
     hipLaunchKernelGGL(k_mr_start, dim3(1), dim3(KB), 0, H.st, H.dS, H.part, H.grid, rtol, maxit);
     GRMP_HIP(H.ctx, hipGetLastError());
     const int every = std::max(1, check_every);
@@ -360,6 +397,10 @@ int mr_run(MrHost &H, const grmp_matrix *A, const V *vals, const T *dsg, const R
                            dsg, z, zprev, wnew, wm2, wm1, y, B.q, n, H.dS, H.part);
         hipLaunchKernelGGL(k_mr_delta, dim3(1), dim3(KB), 0, H.st, H.dS, H.part, H.grid);
         hipLaunchKernelGGL(k_mr_lanczos<T>, dim3(H.gride), dim3(KB), 0, H.st, B.q, vj, vm1, dsg, znew, n, H.dS, H.part);
+        if constexpr (std::is_same<T, double>::value) {
+            // after the last iteration z is not read again; the extra V-cycles until the next poll are wasted work
+            if (pc) GRMP_TRY(mr_precondition(H, *pc, vm1, znew, n, H.gride));
+        }
         hipLaunchKernelGGL(k_mr_givens, dim3(1), dim3(KB), 0, H.st, H.dS, H.part, H.gride);
         if (j % every == 0) {
             GRMP_HIP(H.ctx, hipGetLastError());
@@ -374,7 +415,8 @@ int mr_run(MrHost &H, const grmp_matrix *A, const V *vals, const T *dsg, const R
 
 }  // namespace
 
-int minres_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solve_opts &o, grmp_solve_info &info)
+int minres_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solve_opts &o, grmp_solve_info &info,
+                 grmp_mg *vel_mg, int vel_ncomp)
 {
     grmp_ctx *ctx = A->ctx;
     const int64_t n = A->n;
@@ -382,6 +424,16 @@ int minres_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solve_
     GRMP_CHECK(ctx, A->rowptr && A->cols && A->val64 && A->dinv64 && A->fixed && A->diagpos && bd && xd, GRMP_ERR_ARG,
                "MINRES: matrix not assembled in FP64");
     GRMP_CHECK(ctx, !ir || A->val32, GRMP_ERR_ARG, "MINRES FP32_IR: FP32 values not assembled");
+    GRMP_CHECK(ctx, !vel_mg || !ir, GRMP_ERR_UNSUPPORTED, "MINRES with a multigrid block runs in FP64");
+    MrPrec pc;
+    if (vel_mg) {
+        pc.mg = vel_mg;
+        pc.ncomp = vel_ncomp;
+        pc.nv = mg_level0_rows(vel_mg);
+        GRMP_CHECK(ctx, vel_ncomp >= 1 && vel_ncomp <= 3 && pc.nv * vel_ncomp <= n, GRMP_ERR_ARG,
+                   "MINRES: %d velocity components of %lld rows exceed the system's %lld", vel_ncomp,
+                   (long long)pc.nv, (long long)n);
+    }
     MrHost H;
     H.ctx = ctx;
     H.st = ctx->stream;
@@ -423,7 +475,8 @@ int minres_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solve_
             MrBufs<double> B;
             if ((rc = mr_alloc(ctx, B, &blk64, n))) break;
             if ((rc = mr_run<double, double, double, double>(H, A, A->val64, dsg, bd, xd, true, B, o.rtol, o.maxit,
-                                                             o.check_every, &iters)))
+                                                             vel_mg ? std::min(o.check_every, 4) : o.check_every,
+                                                             &iters, vel_mg ? &pc : nullptr)))
                 break;
             status = H.hS->fail;
             relres = H.hS->g1 > 0 ? fabs(H.hS->eta) / H.hS->g1 : 0.0;

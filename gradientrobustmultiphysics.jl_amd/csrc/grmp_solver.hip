@@ -1402,6 +1402,39 @@ extern "C" int grmp_solve(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, 
     return inf.status;
 }
 
+// MINRES for the Stokes saddle point with the velocity's node block
+// preconditioned by one multigrid V-cycle per component
+extern "C" int grmp_solve_minres_mg(grmp_matrix *A, const grmp_vector *b, grmp_vector *x,
+                                    const grmp_solve_opts *opts, grmp_mg *velocity, int ncomp,
+                                    grmp_solve_info *info)
+{
+    if (!A || !b || !x || !velocity) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = A->ctx;
+    grmp_solve_opts o;
+    grmp_solve_opts_default(&o);
+    if (opts) o = *opts;
+    o.method = GRMP_SOLVE_MINRES;
+    grmp_solve_info inf;
+    memset(&inf, 0, sizeof(inf));
+    GRMP_CHECK(ctx, A->have_pattern && A->val64, GRMP_ERR_ARG, "grmp_solve_minres_mg: matrix not assembled in FP64");
+    GRMP_CHECK(ctx, b->n == A->n && x->n == A->n, GRMP_ERR_ARG, "grmp_solve_minres_mg: vector length != %lld",
+               (long long)A->n);
+    GRMP_CHECK(ctx, o.precision == GRMP_PRECISION_FP64 && o.initial_guess == 0, GRMP_ERR_UNSUPPORTED,
+               "grmp_solve_minres_mg: FP64 from a zero start only");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    const double t0 = wall_seconds();
+    const int rc = minres_solve(A, b->d, x->d, o, inf, velocity, ncomp);
+    inf.seconds = wall_seconds() - t0;
+    if (info) *info = inf;
+    if (rc) return rc;
+    if (inf.status == GRMP_ERR_NOCONV)
+        return grmp_fail(ctx, GRMP_ERR_NOCONV, "grmp_solve_minres_mg: no convergence, relres %.3e after %lld iterations",
+                         inf.relres, (long long)inf.iterations);
+    if (inf.status == GRMP_ERR_NONFINITE)
+        return grmp_fail(ctx, GRMP_ERR_NONFINITE, "grmp_solve_minres_mg: non-finite values in the Krylov loop");
+    return inf.status;
+}
+
 // ------------------------------------------------------- CSC plug-in solve
 extern "C" int grmp_solve_csc(grmp_ctx *ctx, int64_t n, const int64_t *colptr, const int64_t *rowval, const double *nzval,
                               const double *b, double *x, int64_t nfixed, const int64_t *fixed, int precision, double rtol)

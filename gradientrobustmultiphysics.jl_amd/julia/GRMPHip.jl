@@ -273,6 +273,42 @@ function hip_timestep_rhs!(S::Ptr{Cvoid}, M::Ptr{Cvoid}, f::Ptr{Cvoid}, u::Ptr{C
     return b
 end
 
+# ---------------------------------------------------- ItemIntegrators
+"""
+    hip_l2error(space, u, offset, coef, ncoef, quadorder; factor = 1.0, cellvals = nothing)
+
+evaluate(L2ErrorIntegrator(g, Identity; quadorder, factor), u) for polynomial
+data g (per component `ncoef` coefficients over 1, x, y, x^2, xy, y^2, x^3,
+x^2 y, x y^2, y^3; `ncoef = 0` gives L2NormIntegrator) on the device
+(src/assemblypatterns/itemintegrator.jl:40-95, 169-360).  `u` is a
+`grmp_vector` handle holding the FEVector's entries, `offset` the block's.
+Returns the squared L2 error; `cellvals` (length ncells) gets evaluate!'s
+per-item values.
+"""
+function hip_l2error(space::Ptr{Cvoid}, u::Ptr{Cvoid}, offset::Int, coef::Vector{Float64}, ncoef::Int,
+                     quadorder::Int; factor = 1.0, cellvals = nothing)
+    total = Ref{Cdouble}(0.0)
+    check(context(), ccall((:grmp_integrate_l2error, libgrmp), Cint,
+                           (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Cdouble}, Cint, Cint, Cdouble, Ptr{Cdouble},
+                            Ref{Cdouble}),
+                           context(), space, u, offset, coef, ncoef, quadorder, factor,
+                           cellvals === nothing ? C_NULL : cellvals, total))
+    return total[]
+end
+
+# ---------------------------------------------------- batched assembly
+"""
+    hip_assemble_batch!(mats::Vector{Ptr{Cvoid}})
+
+FP64 values of many single-form `grmp_matrix` handles (any meshes, spaces
+and form kinds) in one launch of the persistent assembly kernel; each equals
+assembling it alone bit for bit.  For callers that assemble many small
+problems, e.g. one operator per mesh of a parameter sweep.
+"""
+hip_assemble_batch!(mats::Vector{Ptr{Cvoid}}) =
+    check(context(), ccall((:grmp_matrix_assemble_batch, libgrmp), Cint, (Cint, Ptr{Ptr{Cvoid}}),
+                           length(mats), mats))
+
 # ---------------------------------------------------- LinearSolve plug-in
 """
     GRMPKrylov(; precision = :fp32_ir, rtol = 1e-10)
@@ -309,6 +345,6 @@ function LinearSolve.SciMLBase.solve!(cache::LinearSolve.LinearCache, alg::GRMPK
     return LinearSolve.SciMLBase.build_linear_solution(alg, x, nothing, cache)
 end
 
-export GRMPKrylov, HIPOperator
+export GRMPKrylov, HIPOperator, hip_l2error, hip_assemble_batch!
 
 end # module

@@ -569,6 +569,41 @@ def mg_eligible(pde, Solution):
     return sum(getattr(op, "action", None) is not None for op in pde.LHS[(0, 0)]) <= 1
 
 
+def stokes_mg_eligible(pde, Solution):
+    """True when grmp_solve_minres_mg applies: velocity H1BR{2} on a
+    generated unit square and a pressure, the velocity block one
+    LaplaceOperator with a constant factor (the viscosity), no-slip data."""
+    if len(Solution.FES) != 2 or not _saddle_point(pde, Solution):
+        return False
+    V = Solution.FES[0]
+    if V.FEType.code != _abi.FE_H1BR or V.xgrid.kind != "unitsquare":
+        return False
+    ops = pde.LHS.get((0, 0), [])
+    if len(ops) != 1 or ops[0].kind != _abi.BLF_LAPLACE or ops[0].action is not None:
+        return False
+    bd = pde.BoundaryOperators.get(0)
+    return bd is not None and bd[0] is HomogeneousDirichletBoundary
+
+
+def stokes_velocity_mg(pde, V, penalty=1e60):
+    """The velocity block's preconditioner hierarchy: the viscosity times the
+    scalar P1 Laplacian on the velocity's square (the node part of H1BR{2}),
+    boundary dofs fixed, and its coarser levels; assembled once."""
+    nu = float(pde.LHS[(0, 0)][0].factor)
+    fes = FESpace(H1P1(1), V.xgrid)
+    A0 = FEMatrix([fes])
+    op = LaplaceOperator(nu)
+    form, keep = op.form(fes, 0, 0, fes.ctx)
+    A0.add_form(form, keep)
+    A0.set_fixed(fes.boundary_dofs(), penalty)
+    A0.build_pattern()
+
+    class _Velocity:
+        LHS = {(0, 0): [op]}
+    levels, kappa0 = mg_hierarchy(_Velocity, fes, A0, penalty=penalty)
+    return MultigridSolver(levels, kappa0)
+
+
 def mg_hierarchy(pde, fes0, A0, extra_ops=(), penalty=1e60):
     """Levels for grmp_mg_solve: level 0 is the assembled system A0 over
     fes0; then H1P1 on the same square when fes0 is H1P2, and H1P1 on the
@@ -662,7 +697,9 @@ def solve(Solution, pde, config=None, solver="krylov", **kwargs):
     (src/solvers.jl:1354-1402 -> solve_direct! :622-686).  Returns the
     SolveInfo of the device solve.  solver="krylov": the device Krylov
     solvers (CG / MINRES, FP64 or FP32 inner); "mg": the multigrid V-cycle CG
-    (FP64) where mg_eligible holds; "auto": multigrid where it applies."""
+    (FP64) where mg_eligible holds, and for the Stokes system MINRES with the
+    multigrid velocity block (stokes_mg_eligible); "auto": multigrid where it
+    applies."""
     cfg = config or SolverConfig(**kwargs)
     if solver not in ("auto", "krylov", "mg"):
         raise _abi.GrmpError(_abi.ERR_ARG, f"solver {solver!r}: 'auto', 'krylov' or 'mg'")
@@ -671,7 +708,7 @@ def solve(Solution, pde, config=None, solver="krylov", **kwargs):
             raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "nonlinear problems are solved by the Newton fixpoint loop")
         return solve_fixpoint(Solution, pde, cfg)
     use_mg = solver != "krylov" and mg_eligible(pde, Solution)
-    if solver == "mg" and not use_mg:
+    if solver == "mg" and not use_mg and not stokes_mg_eligible(pde, Solution):
         raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "multigrid needs one H1P1 / H1P2 unknown on a generated unit "
                              "square, Dirichlet data and no global constraints")
     fp32 = cfg.precision == "fp32_ir" and not use_mg
@@ -686,12 +723,17 @@ def solve(Solution, pde, config=None, solver="krylov", **kwargs):
             A.ctx.check(lib().grmp_vector_axpy(1.0, target.h, Solution.vec.h))
         return info, A, b
     method = _abi.SOLVE_MINRES if _saddle_point(pde, Solution) else _abi.SOLVE_CG
-    info = solve_system(A, b, Solution, cfg, method)
+    if solver != "krylov" and method == _abi.SOLVE_MINRES and cfg.precision == "fp64" and \
+            stokes_mg_eligible(pde, Solution):
+        mg = stokes_velocity_mg(pde, Solution.FES[0], cfg.fixed_penalty)
+        info = solve_system(A, b, Solution, cfg, method, velocity_mg=mg)
+    else:
+        info = solve_system(A, b, Solution, cfg, method)
     realize_constraints(pde, Solution)
     return info, A, b
 
 
-def solve_system(A, b, Solution, cfg, method=_abi.SOLVE_CG, initial_guess=False):
+def solve_system(A, b, Solution, cfg, method=_abi.SOLVE_CG, initial_guess=False, velocity_mg=None):
     o = _abi.SolveOpts()
     lib().grmp_solve_opts_default(ctypes.byref(o))
     o.method = method
@@ -702,7 +744,12 @@ def solve_system(A, b, Solution, cfg, method=_abi.SOLVE_CG, initial_guess=False)
     o.check_every = 16 if method == _abi.SOLVE_MINRES else 1
     o.initial_guess = 1 if initial_guess else 0
     info = _abi.SolveInfo()
-    rc = lib().grmp_solve(A.h, b.vec.h, Solution.vec.h, ctypes.byref(o), ctypes.byref(info))
+    if velocity_mg is not None:  # MINRES with the multigrid velocity block (Stokes)
+        ncomp = Solution.FES[0].FEType.ncomp
+        rc = lib().grmp_solve_minres_mg(A.h, b.vec.h, Solution.vec.h, ctypes.byref(o), velocity_mg.h, ncomp,
+                                        ctypes.byref(info))
+    else:
+        rc = lib().grmp_solve(A.h, b.vec.h, Solution.vec.h, ctypes.byref(o), ctypes.byref(info))
     if rc not in (_abi.OK,):
         A.ctx.check(rc)
     return info

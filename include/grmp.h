@@ -392,6 +392,20 @@ int grmp_mg_apply(grmp_mg *m, const grmp_vector *b, grmp_vector *x, double rtol,
                   grmp_solve_info *info);
 int grmp_mg_destroy(grmp_mg *m);
 
+/* The Stokes system of grmp_solve's MINRES (src/solvers.jl:622-686 replaced
+ * for IncompressibleNavierStokesProblem(2; nonlinear=false),
+ * src/pdeprototypes.jl:17-55) with a block-diagonal preconditioner: one
+ * V-cycle of `velocity` -- a hierarchy (grmp_mg_create) of the viscosity
+ * times the scalar P1 Laplacian on the velocity's mesh, with its boundary
+ * dofs fixed -- for the node rows of each of the ncomp velocity components
+ * (rows [k n, (k+1) n), n the hierarchy's level-0 size: the P1 part of
+ * H1BR{2}, on which the viscous block is exactly that Laplacian), 1/|A_ii| on
+ * the other velocity rows (face bubbles) and the Schur-complement diagonal on
+ * the constraint rows.  FP64, zero start.  No counterpart in the reference,
+ * which factorizes. */
+int grmp_solve_minres_mg(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, const grmp_solve_opts *opts,
+                         grmp_mg *velocity, int ncomp, grmp_solve_info *info);
+
 /* Solver plug-in: solves LinearProblem(A.entries.cscmatrix, b.entries) as the
  * linsolver of solve! (src/solvers.jl:55, 89-101, 656-660).  colptr[n+1],
  * rowval[nnz] 1-based (Int64, as SparseMatrixCSC{Float64,Int64}); fixed[nfixed]

@@ -14,6 +14,22 @@ the host-side bookkeeping the reference does in Julia:
   src/fematrix.jl:349-355).
 * ``backward_euler`` -- the BackwardEuler steps of advance! for a linear
   problem (src/solvers_timedependent.jl:337-359, :452-475), with sparse LU.
+* ``crank_nicolson`` -- the CrankNicolson steps of the same loop
+  (src/solvers_timedependent.jl:255, :376, :521).
+* ``boundary_interpolate`` / ``boundary_bestapprox`` (and the ``_br``
+  variants for H1BR{2}) -- boundarydata! (src/boundarydata.jl:198-352).
+* ``l2error_cells`` -- evaluate! of L2ErrorIntegrator / L2NormIntegrator
+  (src/assemblypatterns/itemintegrator.jl:40-95, :169-307).
+* ``nodevalues`` -- nodevalues! with continuous = false
+  (src/interpolations.jl:704-812).
+* ``newton_assemble`` / ``newton_solve`` / ``assemble_lf_poly`` -- the
+  Newton system of NonlinearForm (src/assemblypatterns/nonlinearform.jl:44-246)
+  and the fixpoint loop of solve_fixpoint_full! (src/solvers.jl:812-1017),
+  and polynomial LinearForms (src/assemblypatterns/linearform.jl:181-220).
+
+What pins each function independently of the device code is in the tests
+that exercise it (tests/test_oracle_*.py): exact-rational fixtures, closed
+forms, invariants and brute force.
 """
 import ctypes
 import os
