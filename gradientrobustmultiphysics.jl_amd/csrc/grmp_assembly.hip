@@ -1485,9 +1485,10 @@ extern "C" int grmp_matrix_build_pattern(grmp_matrix *A)
     // the row pointers are int32: check the total in 64 bits before the scan
     int64_t total = 0;
     int rc64 = sum_i32_as_i64(ctx, count, A->n, &total);
-    if (!rc64 && total > GRMP_MAX_NNZ)
-        rc64 = grmp_fail(ctx, GRMP_ERR_UNSUPPORTED, "grmp_matrix_build_pattern: %lld nonzeros exceed the int32 CSR "
-                         "limit %lld", (long long)total, (long long)GRMP_MAX_NNZ);
+    const int64_t limit = std::min<int64_t>(GRMP_MAX_NNZ, ctx->max_nnz);
+    if (!rc64 && total > limit)
+        rc64 = grmp_fail(ctx, GRMP_ERR_UNSUPPORTED, "grmp_matrix_build_pattern: %lld nonzeros exceed the limit %lld "
+                         "(int32 CSR: %lld)", (long long)total, (long long)limit, (long long)GRMP_MAX_NNZ);
     if (rc64) {
         dfree(ctx, count);
         return rc64;

@@ -176,6 +176,15 @@ extern "C" int grmp_ctx_synchronize(grmp_ctx *ctx)
     return GRMP_OK;
 }
 
+extern "C" int grmp_ctx_set_max_nnz(grmp_ctx *ctx, int64_t max_nnz)
+{
+    if (!ctx) return GRMP_ERR_ARG;
+    GRMP_CHECK(ctx, max_nnz >= 1, GRMP_ERR_ARG, "grmp_ctx_set_max_nnz: the limit must be positive");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->max_nnz = max_nnz < 2147483647 ? max_nnz : 2147483647;
+    return GRMP_OK;
+}
+
 int exclusive_scan_i32(grmp_ctx *ctx, const int32_t *in, int32_t *out, int64_t n)
 {
     // out[0] = 0, out[i+1] = sum in[0..i]

@@ -41,6 +41,7 @@ struct grmp_ctx {
     size_t scratch_bytes = 0;
     int *dflag = nullptr;  // device error/overflow flag
     std::vector<TuneEntry> tune;  // assembly launch shapes (tuning table)
+    int64_t max_nnz = 2147483647; // pattern builds above this many nonzeros fail (<= the int32 CSR limit)
 };
 
 int grmp_fail(grmp_ctx *ctx, int code, const char *fmt, ...);

@@ -159,6 +159,11 @@ int grmp_ctx_create(int device, grmp_ctx **out);
 int grmp_ctx_destroy(grmp_ctx *ctx);
 const char *grmp_last_error(const grmp_ctx *ctx);
 int grmp_ctx_synchronize(grmp_ctx *ctx);
+/* Pattern builds of more than max_nnz nonzeros fail with GRMP_ERR_UNSUPPORTED
+ * (the total is summed in 64 bits before the int32 row-pointer scan); the
+ * limit is at most and by default 2^31 - 1, the int32 CSR bound.  A memory
+ * guard with no counterpart in the reference (Int64 CSC). */
+int grmp_ctx_set_max_nnz(grmp_ctx *ctx, int64_t max_nnz);
 /* library build/ABI version: major*10000 + minor*100 + patch */
 int grmp_version(void);
 

@@ -208,3 +208,30 @@ def test_spmv_matches_scipy(grmp, O):
     M32.data = M32.data.astype(np.float32).astype(np.float64)
     yr32 = M32 @ x
     assert np.max(np.abs(y32 - yr32)[free]) <= 1e-5 * np.max(np.abs(yr32[free]))
+
+
+def test_nnz_limit_guard(grmp):
+    """The pattern total is summed in 64 bits before the int32 row-pointer
+    scan; a build above the context's limit fails cleanly and leaves the
+    context usable (the limit stands in for the 2^31 - 1 bound)."""
+    print("[case] nnz limit guard", flush=True)
+    grid = grmp.grid_unitsquare(8)
+    fes = grmp.FESpace(grmp.H1P1(1), grid)
+    ctx = grid.ctx
+    L = grmp.lib()
+    A = grmp.FEMatrix([fes])
+    form, keep = grmp.LaplaceOperator(1.0).form(fes, 0, 0, ctx)
+    A.add_form(form, keep)
+    try:
+        ctx.check(L.grmp_ctx_set_max_nnz(ctx.h, 100))
+        with pytest.raises(grmp.GrmpError) as ei:
+            A.build_pattern()
+        assert ei.value.code == grmp._abi.ERR_UNSUPPORTED and "exceed the limit 100" in str(ei.value)
+    finally:
+        ctx.check(L.grmp_ctx_set_max_nnz(ctx.h, 1 << 40))
+    assert L.grmp_ctx_set_max_nnz(ctx.h, 0) == grmp._abi.ERR_ARG
+    B = grmp.FEMatrix([fes])
+    form, keep2 = grmp.LaplaceOperator(1.0).form(fes, 0, 0, ctx)
+    B.add_form(form, keep2)
+    B.build_pattern()
+    assert B.info()[1] == 81 + 4 * 8 * 9  # the 5-point pattern: the diagonal couplings are exactly 0
