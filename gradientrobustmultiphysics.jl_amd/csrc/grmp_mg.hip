@@ -126,11 +126,21 @@ __device__ __forceinline__ void mg_scalar(MgScal *sc, double a, double b)
 
 
 // ------------------------------------------------------------ row kernels
-enum MgOp { OP_RESID = 0, OP_CHEB_PRE = 1, OP_CHEB_POST1 = 2, OP_CHEB_POST2 = 3, OP_PCG_Q = 4 };
+// The V-cycle's row products (RESID, CHEB_*) read FP32 copies of the level
+// values and accumulate in FP64: the preconditioner moves 8 instead of 12
+// bytes per nonzero.  The CG product (PCG_Q) and the true residual of the
+// convergence check (TRUE_RESID) read the FP64 values, so the accepted
+// solutions meet rtol against the FP64 operator.
+enum MgOp { OP_RESID = 0, OP_CHEB_PRE = 1, OP_CHEB_POST1 = 2, OP_CHEB_POST2 = 3, OP_PCG_Q = 4, OP_TRUE_RESID = 5 };
+template <int OP>
+struct MgF64 {
+    static constexpr bool value = OP == OP_PCG_Q || OP == OP_TRUE_RESID;
+};
 
 struct MgRows {
     const int32_t *rp, *ci;
     const double *val;
+    const float *val32;  // FP32 copy (fixed rows 0) for the V-cycle's products
     const uint8_t *fixed;
     int64_t n;
     const double *v;     // the vector A multiplies
@@ -154,7 +164,7 @@ __device__ __forceinline__ double mg_epilogue(int64_t i, double s, bool fx, cons
                                               const double *__restrict__ d, const double *__restrict__ lam,
                                               double *__restrict__ o1, double *__restrict__ o2)
 {
-    if (OP == OP_RESID) {
+    if (OP == OP_RESID || OP == OP_TRUE_RESID) {
         o1[i] = fx ? 0.0 : b[i] - s;
     } else if (OP == OP_CHEB_PRE) {
         const double l = *lam;
@@ -187,7 +197,9 @@ __device__ __forceinline__ void mg_rows_body(const MgRows &a, int64_t r0, int pi
 {
     __shared__ double lds[MG_ROWS_CAP];
     const int64_t r1 = r0 + KB < a.n ? r0 + KB : a.n;
-    const double s = csr_rows_lds<double, double, double, MG_ROWS_CAP>(a.rp, a.ci, a.val, a.v, r0, r1, lds);
+    double s;
+    if constexpr (MgF64<OP>::value) s = csr_rows_lds<double, double, double, MG_ROWS_CAP>(a.rp, a.ci, a.val, a.v, r0, r1, lds);
+    else s = csr_rows_lds<double, float, double, MG_ROWS_CAP>(a.rp, a.ci, a.val32, a.v, r0, r1, lds);
     const int64_t i = r0 + threadIdx.x;
     double pa = 0.0, pb = 0.0;
     if (i < r1) pa = mg_epilogue<OP>(i, s, a.fixed[i] != 0, a.v, a.b, a.dinv, a.d, a.lam, a.o1, a.o2);
@@ -219,7 +231,7 @@ __device__ __forceinline__ void mg_pre_body(const MgRows &a, int64_t r0)
         __syncthreads();
         for (int32_t k = c0 + t; k < c1; k += KB) {
             const int32_t c = a.ci[k];
-            lds[k - c0] = a.val[k] * (a.fixed[c] ? 0.0 : a.dinv[c] * a.b[c] * ith);
+            lds[k - c0] = (double)a.val32[k] * (a.fixed[c] ? 0.0 : a.dinv[c] * a.b[c] * ith);
         }
         __syncthreads();
         const int32_t lo = my0 > c0 ? my0 : c0, hi = my1 < c1 ? my1 : c1;
@@ -252,6 +264,17 @@ __global__ __launch_bounds__(KB) void k_mg_resid_fine(MgRows a)
 {
     if (a.sc->done) return;
     mg_rows_body<OP_RESID>(a, (int64_t)blockIdx.x * KB, blockIdx.x);
+}
+
+// the V-cycle's FP32 copy of a level's values; fixed rows 0 (their 1e60
+// penalty has no FP32 value, and their products are not used)
+__global__ void k_mg_val32(const int32_t *__restrict__ rp, const double *__restrict__ val,
+                           const uint8_t *__restrict__ fixed, float *__restrict__ val32, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool fx = fixed[i] != 0;
+    for (int32_t k = rp[i]; k < rp[i + 1]; ++k) val32[k] = fx ? 0.f : (float)val[k];
 }
 
 // Gershgorin bound max_i sum_j |A_ij| / A_ii over free rows (a max, so the
@@ -600,6 +623,7 @@ __global__ void k_mg_verify_begin(MgScal *sc)
 struct MgBLevel {
     const int32_t *rp, *ci;
     const double *val, *dinv;
+    const float *val32;
     const uint8_t *fixed;
     double *x, *b, *r, *d;
     const double *lam;
@@ -641,7 +665,10 @@ __device__ double blk_rows(const MgBLevel &L, const double *v, const double *b, 
     double part = 0.0;
     for (int64_t i = threadIdx.x; i < L.n; i += MG_BT) {
         double s = 0.0;
-        for (int32_t k = L.rp[i]; k < L.rp[i + 1]; ++k) s += L.val[k] * v[L.ci[k]];
+        if constexpr (MgF64<OP>::value)
+            for (int32_t k = L.rp[i]; k < L.rp[i + 1]; ++k) s += L.val[k] * v[L.ci[k]];
+        else
+            for (int32_t k = L.rp[i]; k < L.rp[i + 1]; ++k) s += (double)L.val32[k] * v[L.ci[k]];
         part += mg_epilogue<OP>(i, s, L.fixed[i] != 0, v, b, L.dinv, d, L.lam, o1, o2);
     }
     return part;
@@ -726,7 +753,7 @@ __device__ void mg_pcg_block_body(const MgBDesc &D, MgScal *sc, int iters)
         __syncthreads();
         if (st == 1) return;
         if (st == 2) {  // the updated residual met rtol: check b - A x
-            blk_rows<OP_RESID>(L0, D.x, D.rhs, nullptr, D.r, nullptr);
+            blk_rows<OP_TRUE_RESID>(L0, D.x, D.rhs, nullptr, D.r, nullptr);
             __syncthreads();
             double p = 0.0;
             for (int64_t i = threadIdx.x; i < n; i += MG_BT) p += D.r[i] * D.r[i];
@@ -799,6 +826,7 @@ __device__ __forceinline__ MgRows mg_tile_rows(const MgBDesc &D, int level)
     a.rp = L.rp;
     a.ci = L.ci;
     a.val = L.val;
+    a.val32 = L.val32;
     a.fixed = L.fixed;
     a.n = L.n;
     a.v = nullptr;
@@ -986,6 +1014,7 @@ struct MgLevel {
     int64_t n = 0, nnz = 0, nc = 0;
     double *kappa = nullptr, *val = nullptr, *dinv = nullptr, *x = nullptr, *b = nullptr, *r = nullptr, *d = nullptr;
     float *dinv32 = nullptr;
+    float *val32 = nullptr;  // FP32 copy of val for the V-cycle, fixed rows 0
     double *lam = nullptr;
     DevForm *dforms = nullptr;
     DevForm hforms[4];
@@ -1032,19 +1061,20 @@ static size_t mg_layout(MgJob *j, bool assign)
            o_desc = take(sizeof(MgBDesc));
     const MgLevel &c = j->lv[j->nlev - 1];
     size_t o_inv = take(c.n * c.n * 8);
-    std::vector<size_t> o(MG_MAX_LEVELS * 9);
+    std::vector<size_t> o(MG_MAX_LEVELS * 10);
     for (int l = 0; l < j->nlev; ++l) {
         const MgLevel &L = j->lv[l];
         const bool own_kappa = l > 0 && L.in.N != j->lv[l - 1].in.N;
-        o[9 * l + 0] = own_kappa || l == 0 ? take(L.nc * 8) : 0;
-        o[9 * l + 1] = take(L.nnz * 8);
-        o[9 * l + 2] = take(L.n * 8);
-        o[9 * l + 3] = take(L.n * 4);
-        o[9 * l + 4] = l == 0 ? 0 : take(L.n * 8);  // x (level 0: z)
-        o[9 * l + 5] = l == 0 ? 0 : take(L.n * 8);  // b (level 0: r)
-        o[9 * l + 6] = take(L.n * 8);
-        o[9 * l + 7] = take(L.n * 8);
-        o[9 * l + 8] = take(8 + sizeof(DevForm) * 4);
+        o[10 * l + 0] = own_kappa || l == 0 ? take(L.nc * 8) : 0;
+        o[10 * l + 1] = take(L.nnz * 8);
+        o[10 * l + 2] = take(L.n * 8);
+        o[10 * l + 3] = take(L.n * 4);
+        o[10 * l + 4] = l == 0 ? 0 : take(L.n * 8);  // x (level 0: z)
+        o[10 * l + 5] = l == 0 ? 0 : take(L.n * 8);  // b (level 0: r)
+        o[10 * l + 6] = take(L.n * 8);
+        o[10 * l + 7] = take(L.n * 8);
+        o[10 * l + 8] = take(8 + sizeof(DevForm) * 4);
+        o[10 * l + 9] = take(L.nnz * 4);
     }
     if (!assign) return off;
     char *B = j->buf;
@@ -1063,16 +1093,17 @@ static size_t mg_layout(MgJob *j, bool assign)
     for (int l = 0; l < j->nlev; ++l) {
         MgLevel &L = j->lv[l];
         const bool own_kappa = l > 0 && L.in.N != j->lv[l - 1].in.N;
-        L.kappa = (own_kappa || l == 0) ? (double *)(B + o[9 * l + 0]) : j->lv[l - 1].kappa;
-        L.val = (double *)(B + o[9 * l + 1]);
-        L.dinv = (double *)(B + o[9 * l + 2]);
-        L.dinv32 = (float *)(B + o[9 * l + 3]);
-        L.x = l == 0 ? j->z : (double *)(B + o[9 * l + 4]);
-        L.b = l == 0 ? j->r : (double *)(B + o[9 * l + 5]);
-        L.r = (double *)(B + o[9 * l + 6]);
-        L.d = (double *)(B + o[9 * l + 7]);
-        L.lam = (double *)(B + o[9 * l + 8]);
-        L.dforms = (DevForm *)(B + o[9 * l + 8] + 8);
+        L.kappa = (own_kappa || l == 0) ? (double *)(B + o[10 * l + 0]) : j->lv[l - 1].kappa;
+        L.val = (double *)(B + o[10 * l + 1]);
+        L.dinv = (double *)(B + o[10 * l + 2]);
+        L.dinv32 = (float *)(B + o[10 * l + 3]);
+        L.x = l == 0 ? j->z : (double *)(B + o[10 * l + 4]);
+        L.b = l == 0 ? j->r : (double *)(B + o[10 * l + 5]);
+        L.r = (double *)(B + o[10 * l + 6]);
+        L.d = (double *)(B + o[10 * l + 7]);
+        L.lam = (double *)(B + o[10 * l + 8]);
+        L.dforms = (DevForm *)(B + o[10 * l + 8] + 8);
+        L.val32 = (float *)(B + o[10 * l + 9]);
     }
     return off;
 }
@@ -1103,6 +1134,7 @@ static void fill_desc(MgJob *j)
         B.rp = L.in.A->rowptr;
         B.ci = L.in.A->cols;
         B.val = L.val;
+        B.val32 = L.val32;
         B.dinv = L.dinv;
         B.fixed = L.in.A->fixed;
         B.x = L.x;
@@ -1192,6 +1224,7 @@ static void rows_args(MgRows &a, const MgLevel &L, const MgJob *j)
     a.rp = L.in.A->rowptr;
     a.ci = L.in.A->cols;
     a.val = L.val;
+    a.val32 = L.val32;
     a.fixed = L.in.A->fixed;
     a.n = L.n;
     a.dinv = L.dinv;
@@ -1219,7 +1252,8 @@ static void launch_rows(MgJob *j, const MgRows &a, int64_t nnz = 0)
         hipExtLaunchKernelGGL(k_mg_resid_fine, dim3(nblk(a.n, KB)), dim3(KB), 0, j->st, e0, e1, 0, a);
         j->pev.push_back(e0);
         j->pev.push_back(e1);
-        j->pbytes.push_back((double)nnz * 12.0 + 4.0 * (a.n + 1) + 24.0 * a.n + 1.0 * a.n);
+        // FP32 values + int32 columns, row pointers, x gathered once, b, r, mask
+        j->pbytes.push_back((double)nnz * 8.0 + 4.0 * (a.n + 1) + 24.0 * a.n + 1.0 * a.n);
         return;
     }
     hipLaunchKernelGGL(k_mg_rows<OP>, dim3(nblk(a.n, KB)), dim3(KB), 0, j->st, a);
@@ -1302,6 +1336,8 @@ static int mg_job_assemble(MgJob *j)
             hipLaunchKernelGGL(k_mg_coarse_kappa, dim3(nblk(L.nc, 256)), dim3(256), 0, st, (int64_t)j->lv[l - 1].in.N,
                                (int64_t)L.in.N, j->lv[l - 1].kappa, L.kappa);
         GRMP_TRY(assemble_values(L.in.A, L.kappa, L.dforms, L.hforms, L.val, nullptr, L.dinv, L.dinv32, st));
+        hipLaunchKernelGGL(k_mg_val32, dim3(nblk(L.n, 256)), dim3(256), 0, st, L.in.A->rowptr, L.val, L.in.A->fixed,
+                           L.val32, L.n);
         GRMP_HIP(ctx, hipMemsetAsync(L.lam, 0, 8, st));
         hipLaunchKernelGGL(k_mg_gersh, dim3(nblk(L.n, 256)), dim3(256), 0, st, L.in.A->rowptr, L.val, L.dinv,
                            L.in.A->fixed, L.n, (unsigned long long *)L.lam);
@@ -1370,6 +1406,8 @@ int mg_jobs_setup_batched(MgJob *const *jobs, int n, hipStream_t st, double *con
         MgJob *j = jobs[k];
         for (int l = 0; l < j->nlev; ++l) {
             MgLevel &L = j->lv[l];
+            hipLaunchKernelGGL(k_mg_val32, dim3(nblk(L.n, 256)), dim3(256), 0, st, L.in.A->rowptr, L.val,
+                               L.in.A->fixed, L.val32, L.n);
             GRMP_HIP(ctx, hipMemsetAsync(L.lam, 0, 8, st));
             hipLaunchKernelGGL(k_mg_gersh, dim3(nblk(L.n, 256)), dim3(256), 0, st, L.in.A->rowptr, L.val, L.dinv,
                                L.in.A->fixed, L.n, (unsigned long long *)L.lam);
@@ -1468,7 +1506,7 @@ int mg_job_verify(MgJob *j)
     a.v = j->x;
     a.b = j->rhs;
     a.o1 = j->r;
-    launch_rows<OP_RESID>(j, a, L0.nnz);
+    launch_rows<OP_TRUE_RESID>(j, a, L0.nnz);
     hipLaunchKernelGGL(k_mg_dot, dim3(j->nb0), dim3(KB), 0, j->st, j->r, j->r, n, j->part, j->sc);
     hipLaunchKernelGGL(k_mg_red<RED_VERIFY>, dim3(1), dim3(KB), 0, j->st, j->part, j->nb0, nullptr, j->sc);
     mg_vcycle(j);
@@ -1684,7 +1722,7 @@ int mg_batch_run(MgBatch *b, MgJob *const *jobs, int n, int iters)
                 for (int g : grid)
                     if (jobs[g]->first_small > l) {
                         const MgLevel &Lv = jobs[g]->lv[l];
-                        bytes += (double)Lv.nnz * 12.0 + 4.0 * (Lv.n + 1) + 25.0 * Lv.n;
+                        bytes += (double)Lv.nnz * 8.0 + 4.0 * (Lv.n + 1) + 25.0 * Lv.n;
                     }
                 b->pbytes.push_back(bytes);
             } else {
