@@ -334,6 +334,21 @@ def run_workload(grmp, which, args, log):
                             "fp32_ir": {"status": i32.status, "iterations": int(i32.iterations), "outer": int(i32.outer),
                                         "s": round(i32.seconds, 3), "dof_solves_per_s": round(n / i32.seconds, 1)},
                             "rel_diff_fp32_ir_vs_fp64": float(np.linalg.norm(c - a) / np.linalg.norm(a))}
+            # the same assembled system with the algebraic multigrid CG (any mesh): hierarchy, then solve
+            log(f"[bench] {which}: algebraic multigrid CG")
+            t0 = time.perf_counter()
+            amg = grmp.AlgebraicMultigridSolver(A)
+            ctx.sync()
+            t_setup = time.perf_counter() - t0
+            xa = grmp.FEVector(fes)
+            ia = amg.solve(b, xa, args.rtol)
+            nl, coarse, cx = amg.info()
+            out["solve"]["amg"] = {"status": int(ia.status), "iterations": int(ia.iterations),
+                                   "setup_s": round(t_setup, 3), "s": round(ia.seconds, 3),
+                                   "dof_solves_per_s": round(n / ia.seconds, 1), "levels": nl,
+                                   "operator_complexity": round(cx, 3),
+                                   "rel_diff_vs_fp64": float(np.linalg.norm(xa.entries() - a) / np.linalg.norm(a))}
+            del amg
             if which != "c3":
                 # the same problem (same seeded coefficient field) as one multigrid request:
                 # shapes of every level, level assembly, V-cycle CG and the copy back, end to end

@@ -20,7 +20,8 @@ from .pde import (APT_BilinearForm, APT_SymmetricBilinearForm, BestapproxDirichl
                   LagrangeMultiplier, LaplaceOperator, LinearElasticityProblem, LinearForm, PDEDescription,
                   PoissonProblem, ReactionOperator, ReconstructionIdentity, SolverConfig, SymmetricGradient, assemble,
                   assemble_linearform, MultigridSolver, mg_eligible, mg_hierarchy, mg_solve, realize_constraints,
-                  solve, solve_system, stokes_mg_eligible, stokes_velocity_mg)
+                  solve, solve_system, stokes_mg_eligible, stokes_velocity_mg, AlgebraicMultigridSolver,
+                  amg_eligible)
 
 from .dataexport import nodevalues, writeCSV  # noqa: F401
 from .timecontrol import BackwardEuler, CrankNicolson, TimeControlSolver  # noqa: F401

@@ -159,6 +159,11 @@ SIGNATURES = {
     "grmp_matrix_assemble_batch": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(VP)]),
     "grmp_solve_minres_mg": (ctypes.c_int, [VP, VP, VP, VP, VP, ctypes.c_int, VP]),
     "grmp_ctx_set_max_nnz": (ctypes.c_int, [VP, I64]),
+    "grmp_amg_create": (ctypes.c_int, [VP, ctypes.POINTER(VP)]),
+    "grmp_amg_apply": (ctypes.c_int, [VP, VP, VP, ctypes.c_double, I64, VP]),
+    "grmp_amg_info": (ctypes.c_int, [VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(I64),
+                                     ctypes.POINTER(ctypes.c_double)]),
+    "grmp_amg_destroy": (ctypes.c_int, [VP]),
     "grmp_matrix_tune_key": (ctypes.c_int, [VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                             ctypes.POINTER(ctypes.c_int)]),
     "grmp_tune_load": (ctypes.c_int, [VP, ctypes.c_char_p]),

@@ -213,6 +213,57 @@ int sum_i32_as_i64(grmp_ctx *ctx, const int32_t *in, int64_t n, int64_t *total)
     return rc;
 }
 
+// ------------------------------------------ sorts and run sums (rocPRIM)
+// Stable LSD radix sorts on the low `bits` key bits and the sums of runs of
+// equal keys, used by the algebraic multigrid setup.  They run on the
+// context's stream with its shared temporary storage.
+int sort_pairs_u64_f64(grmp_ctx *ctx, const uint64_t *kin, uint64_t *kout, const double *vin, double *vout,
+                       int64_t n, int bits)
+{
+    if (n <= 0) return GRMP_OK;
+    size_t tmp = 0;
+    GRMP_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tmp, kin, kout, vin, vout, (size_t)n, 0u, (unsigned)bits,
+                                            ctx->stream));
+    GRMP_TRY(ensure_scratch(ctx, tmp));
+    GRMP_HIP(ctx, rocprim::radix_sort_pairs(ctx->scratch, tmp, kin, kout, vin, vout, (size_t)n, 0u, (unsigned)bits,
+                                            ctx->stream));
+    return GRMP_OK;
+}
+
+int sort_pairs_i32_i32(grmp_ctx *ctx, const int32_t *kin, int32_t *kout, const int32_t *vin, int32_t *vout,
+                       int64_t n, int bits)
+{
+    if (n <= 0) return GRMP_OK;
+    size_t tmp = 0;
+    GRMP_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tmp, kin, kout, vin, vout, (size_t)n, 0u, (unsigned)bits,
+                                            ctx->stream));
+    GRMP_TRY(ensure_scratch(ctx, tmp));
+    GRMP_HIP(ctx, rocprim::radix_sort_pairs(ctx->scratch, tmp, kin, kout, vin, vout, (size_t)n, 0u, (unsigned)bits,
+                                            ctx->stream));
+    return GRMP_OK;
+}
+
+int reduce_runs_u64_f64(grmp_ctx *ctx, const uint64_t *keys, const double *vals, int64_t n, uint64_t *ukeys,
+                        double *sums, int64_t *count)
+{
+    *count = 0;
+    if (n <= 0) return GRMP_OK;
+    int64_t *dcount = nullptr;
+    GRMP_TRY(dalloc(ctx, &dcount, 1));
+    size_t tmp = 0;
+    hipError_t e = rocprim::reduce_by_key(nullptr, tmp, keys, vals, (size_t)n, ukeys, sums, dcount,
+                                          rocprim::plus<double>(), rocprim::equal_to<uint64_t>(), ctx->stream);
+    int rc = e == hipSuccess ? ensure_scratch(ctx, tmp) : GRMP_ERR_HIP;
+    if (!rc)
+        e = rocprim::reduce_by_key(ctx->scratch, tmp, keys, vals, (size_t)n, ukeys, sums, dcount,
+                                   rocprim::plus<double>(), rocprim::equal_to<uint64_t>(), ctx->stream);
+    if (!rc && e == hipSuccess) e = hipMemcpyAsync(count, dcount, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream);
+    if (!rc && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dfree(ctx, dcount);
+    if (!rc && e != hipSuccess) rc = grmp_fail(ctx, GRMP_ERR_HIP, "run sums: %s", hipGetErrorString(e));
+    return rc;
+}
+
 // ------------------------------------------------------------ Philox RNG
 // Philox-4x32-10 (Salmon et al., SC'11): counter = (index, 0, 0, 0),
 // key = (seed low, seed high).

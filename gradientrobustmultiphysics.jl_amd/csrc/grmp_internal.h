@@ -210,6 +210,14 @@ struct grmp_matrix {
 int build_incidence(grmp_ctx *ctx, grmp_space *s);
 int exclusive_scan_i32(grmp_ctx *ctx, const int32_t *in, int32_t *out, int64_t n);  // out[n+1]
 int sum_i32_as_i64(grmp_ctx *ctx, const int32_t *in, int64_t n, int64_t *total);   // host total, synchronizes
+// stable radix sorts of (key, value) pairs on the low `bits` key bits, and the
+// sums of runs of equal keys (host count, synchronizes); context stream
+int sort_pairs_u64_f64(grmp_ctx *ctx, const uint64_t *kin, uint64_t *kout, const double *vin, double *vout,
+                       int64_t n, int bits);
+int sort_pairs_i32_i32(grmp_ctx *ctx, const int32_t *kin, int32_t *kout, const int32_t *vin, int32_t *vout,
+                       int64_t n, int bits);
+int reduce_runs_u64_f64(grmp_ctx *ctx, const uint64_t *keys, const double *vals, int64_t n, uint64_t *ukeys,
+                        double *sums, int64_t *count);
 // largest nonzero count of a CSR pattern with int32 row pointers and columns
 constexpr int64_t GRMP_MAX_NNZ = 2147483647;
 int form_resolve(grmp_matrix *A, FormRec &r);

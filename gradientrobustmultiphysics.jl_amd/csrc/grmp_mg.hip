@@ -1964,3 +1964,758 @@ extern "C" int grmp_mg_solve(grmp_ctx *ctx, int nlev, const grmp_matrix *const *
     grmp_mg_destroy(m);
     return rc;
 }
+
+// ================================================================ AMG
+// Algebraic multigrid for symmetric positive definite systems on any mesh
+// (the geometric hierarchy above needs generated unit squares).
+//   aggregation   two passes of pairwise matching per level: a free row pairs
+//                 with its strongest free neighbour (the largest positive
+//                 -a_ij / sqrt(a_ii a_jj), ties to the smaller index) when the
+//                 choice is mutual, in three handshake rounds; a row left over
+//                 joins the pair of its strongest matched neighbour, or stays
+//                 single when it has none.
+//                 No absolute threshold: on coarse 3D levels with 20-30
+//                 neighbours per row every normalized coupling is small.
+//   coarse level  the Galerkin product P^T A P of the piecewise-constant P:
+//                 every entry keyed (agg i, agg j), stably sorted, runs summed.
+//   cycle         V(1,1) with the Chebyshev smoothing of the geometric levels
+//                 (FP32-stored values, FP64 vectors), the coarse correction
+//                 scaled by AMG_SCALE (plain aggregation under-corrects smooth
+//                 error; the scaled cycle stays symmetric and positive
+//                 definite for a scale below 2), the dense inverse on the
+//                 coarsest level (at most MG_DENSE_MAX rows).
+//   Krylov        FP64 CG with the device scalars and the true-residual check
+//                 of the geometric solver.
+// Fixed (penalized) rows are eliminated: x_d = b_d / A_dd, and they belong to
+// no aggregate.  Every step is deterministic: handshakes read the previous
+// round's state only, the sorts are stable and the sums run in a fixed order.
+#define AMG_MAX_LEVELS 24
+#define AMG_THETA 0.0
+#define AMG_SCALE 1.5
+
+struct AmgLevel {
+    int64_t n = 0, nnz = 0;
+    const int32_t *rp = nullptr, *ci = nullptr;  // level 0: the matrix's pattern
+    const double *val = nullptr;                 // level 0: the matrix's FP64 values
+    const uint8_t *fixed = nullptr;              // coarse levels: all zero
+    float *val32 = nullptr;
+    double *dinv = nullptr, *lam = nullptr;
+    double *x = nullptr, *b = nullptr, *r = nullptr, *d = nullptr;  // level 0: x = z, b = r of the CG
+    int32_t *agg = nullptr;                      // row -> row of the next level (-1: fixed)
+    int32_t *mptr = nullptr, *midx = nullptr;    // rows of this level: their members on the previous one
+};
+
+struct grmp_amg {
+    grmp_ctx *ctx = nullptr;
+    const grmp_matrix *A = nullptr;
+    int nlev = 0;
+    AmgLevel lv[AMG_MAX_LEVELS];
+    double *inv = nullptr;
+    double *x = nullptr, *rhs = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+    double *part = nullptr, *part2 = nullptr;
+    MgScal *sc = nullptr;
+    int nb0 = 0;
+    MgStatus *h = nullptr;  // pinned
+    hipEvent_t ev = nullptr;
+    std::vector<void *> mem;  // device allocations of the hierarchy
+    int64_t nnz_total = 0;    // summed nonzeros of every level
+};
+
+template <class T>
+static int amg_alloc(grmp_amg *m, T **p, size_t count)
+{
+    GRMP_TRY(dalloc(m->ctx, p, count));
+    m->mem.push_back((void *)*p);
+    return GRMP_OK;
+}
+
+// diagonal of each row (0 if it has none)
+__global__ void k_amg_diag(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                           const double *__restrict__ val, int64_t n, double *__restrict__ diag)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = 0.0;
+    for (int32_t k = rp[i]; k < rp[i + 1]; ++k)
+        if (ci[k] == i) d = val[k];
+    diag[i] = d;
+}
+
+// best[i]: the strongest free, unmatched neighbour of the free, unmatched row
+// i, or -1 (fixed == nullptr: no fixed rows)
+__global__ void k_amg_best(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                           const double *__restrict__ val, const double *__restrict__ diag,
+                           const uint8_t *__restrict__ fixed, const int32_t *__restrict__ mate, int64_t n,
+                           double theta, int32_t *__restrict__ best)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t bj = -1;
+    if (mate[i] < 0 && !(fixed && fixed[i]) && diag[i] > 0.0) {
+        double bs = 0.0;
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+            const int32_t j = ci[k];
+            if (j == i || mate[j] >= 0 || (fixed && fixed[j]) || !(diag[j] > 0.0)) continue;
+            const double st = -val[k] / sqrt(diag[i] * diag[j]);
+            if (!(st > theta)) continue;
+            if (bj < 0 || st > bs || (st == bs && j < bj)) {
+                bs = st;
+                bj = j;
+            }
+        }
+    }
+    best[i] = bj;
+}
+
+// mutual choices pair up (each row writes only its own mate)
+__global__ void k_amg_match(const int32_t *__restrict__ best, int32_t *__restrict__ mate, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t j = best[i];
+    if (mate[i] < 0 && j >= 0 && best[j] == (int32_t)i) mate[i] = j;
+}
+
+// a free row left unmatched joins the pair of its strongest matched
+// neighbour (join[i] = that neighbour), else stays single (join[i] = -1)
+__global__ void k_amg_join(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                           const double *__restrict__ val, const double *__restrict__ diag,
+                           const uint8_t *__restrict__ fixed, const int32_t *__restrict__ mate, int64_t n,
+                           int32_t *__restrict__ join)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t bj = -1;
+    if (mate[i] < 0 && !(fixed && fixed[i]) && diag[i] > 0.0) {
+        double bs = 0.0;
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+            const int32_t j = ci[k];
+            if (j == i || mate[j] < 0 || (fixed && fixed[j]) || !(diag[j] > 0.0)) continue;
+            const double st = -val[k] / sqrt(diag[i] * diag[j]);
+            if (!(st > 0.0)) continue;
+            if (bj < 0 || st > bs || (st == bs && j < bj)) {
+                bs = st;
+                bj = j;
+            }
+        }
+    }
+    join[i] = bj;
+}
+
+// lead[i] = 1 for the first row of each pair and for rows left single
+__global__ void k_amg_lead(const int32_t *__restrict__ mate, const int32_t *__restrict__ join,
+                           const uint8_t *__restrict__ fixed, int64_t n, int32_t *__restrict__ lead)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (fixed && fixed[i]) {
+        lead[i] = 0;
+        return;
+    }
+    const int32_t m = mate[i];
+    lead[i] = m >= 0 ? ((int32_t)i <= m ? 1 : 0) : (join[i] < 0 ? 1 : 0);
+}
+
+__global__ void k_amg_assign(const int32_t *__restrict__ mate, const int32_t *__restrict__ join,
+                             const uint8_t *__restrict__ fixed, const int32_t *__restrict__ id, int64_t n,
+                             int32_t *__restrict__ agg)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (fixed && fixed[i]) {
+        agg[i] = -1;
+        return;
+    }
+    int32_t m = mate[i], r = (int32_t)i;
+    if (m < 0 && join[i] >= 0) {  // the pair of the neighbour it joined
+        r = join[i];
+        m = mate[r];
+    }
+    agg[i] = m < 0 ? id[r] : id[m < r ? m : r];
+}
+
+// agg[i] = a2[a1[i]] (composition of two pairwise passes)
+__global__ void k_amg_compose(const int32_t *__restrict__ a1, const int32_t *__restrict__ a2, int64_t n,
+                              int32_t *__restrict__ agg)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    agg[i] = a1[i] < 0 ? -1 : a2[a1[i]];
+}
+
+// Galerkin keys: entry (i, j) -> agg i * nc + agg j; entries of fixed rows or
+// columns get the sentinel nc * nc (sorted last, dropped)
+__global__ void k_amg_keys(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                           const double *__restrict__ val, const int32_t *__restrict__ agg, int64_t n, int64_t nc,
+                           uint64_t *__restrict__ keys, double *__restrict__ vals)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t ai = agg[i];
+    for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+        const int32_t aj = agg[ci[k]];
+        const bool in = ai >= 0 && aj >= 0;
+        keys[k] = in ? (uint64_t)ai * (uint64_t)nc + (uint64_t)aj : (uint64_t)nc * (uint64_t)nc;
+        vals[k] = in ? val[k] : 0.0;
+    }
+}
+
+// the coarse CSR from the sorted unique keys (rows ascending, columns ascending within a row)
+__global__ void k_amg_rowcount(const uint64_t *__restrict__ ukeys, int64_t m, int64_t nc, int32_t *__restrict__ cnt)
+{
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= m) return;
+    atomicAdd(&cnt[ukeys[u] / (uint64_t)nc], 1);
+}
+
+__global__ void k_amg_cols(const uint64_t *__restrict__ ukeys, int64_t m, int64_t nc, int32_t *__restrict__ cols)
+{
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= m) return;
+    cols[u] = (int32_t)(ukeys[u] % (uint64_t)nc);
+}
+
+// members of each aggregate: keys agg (fixed rows -> nc, sorted last), values the row
+__global__ void k_amg_member_keys(const int32_t *__restrict__ agg, int64_t n, int32_t nc, int32_t *__restrict__ key,
+                                  int32_t *__restrict__ idx)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    key[i] = agg[i] < 0 ? nc : agg[i];
+    idx[i] = (int32_t)i;
+}
+
+__global__ void k_amg_member_count(const int32_t *__restrict__ agg, int64_t n, int32_t *__restrict__ cnt)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (agg[i] >= 0) atomicAdd(&cnt[agg[i]], 1);
+}
+
+__global__ void k_amg_dinv(const double *__restrict__ diag, const uint8_t *__restrict__ fixed, int64_t n,
+                           double *__restrict__ dinv)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dinv[i] = (fixed[i] || !(diag[i] > 0.0)) ? 0.0 : 1.0 / diag[i];
+}
+
+// restriction b_c[I] = sum of r over the members of I, in row order
+__global__ void k_amg_restrict(const int32_t *__restrict__ mptr, const int32_t *__restrict__ midx,
+                               const double *__restrict__ r, double *__restrict__ bc, int64_t nc, const MgScal *sc)
+{
+    if (sc->done) return;
+    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= nc) return;
+    double s = 0.0;
+    for (int32_t k = mptr[I]; k < mptr[I + 1]; ++k) s += r[midx[k]];
+    bc[I] = s;
+}
+
+// prolongation x_i += scale x_c[agg i] on the free rows
+__global__ void k_amg_prolong(const int32_t *__restrict__ agg, const double *__restrict__ xc, double scale,
+                              double *__restrict__ x, int64_t n, const MgScal *sc)
+{
+    if (sc->done) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t a = agg[i];
+    if (a >= 0) x[i] += scale * xc[a];
+}
+
+// One pairwise pass over a level (rp, ci, val, fixed): agg[n], *nc.
+static int amg_pairwise(grmp_ctx *ctx, const int32_t *rp, const int32_t *ci, const double *val, const uint8_t *fixed,
+                        int64_t n, int32_t *agg, int64_t *nc)
+{
+    hipStream_t st = ctx->stream;
+    double *diag = nullptr;
+    int32_t *mate = nullptr, *best = nullptr, *lead = nullptr, *id = nullptr;
+    int rc = GRMP_OK;
+    do {
+        if ((rc = dalloc(ctx, &diag, n)) || (rc = dalloc(ctx, &mate, n)) || (rc = dalloc(ctx, &best, n)) ||
+            (rc = dalloc(ctx, &lead, n)) || (rc = dalloc(ctx, &id, n + 1)))
+            break;
+        hipLaunchKernelGGL(k_amg_diag, dim3(nblk(n, 256)), dim3(256), 0, st, rp, ci, val, n, diag);
+        if (hipMemsetAsync(mate, 0xff, sizeof(int32_t) * n, st) != hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "AMG aggregation: memset");
+            break;
+        }
+        for (int round = 0; round < 3; ++round) {
+            hipLaunchKernelGGL(k_amg_best, dim3(nblk(n, 256)), dim3(256), 0, st, rp, ci, val, diag, fixed, mate, n,
+                               AMG_THETA, best);
+            hipLaunchKernelGGL(k_amg_match, dim3(nblk(n, 256)), dim3(256), 0, st, best, mate, n);
+        }
+        hipLaunchKernelGGL(k_amg_join, dim3(nblk(n, 256)), dim3(256), 0, st, rp, ci, val, diag, fixed, mate, n, best);
+        hipLaunchKernelGGL(k_amg_lead, dim3(nblk(n, 256)), dim3(256), 0, st, mate, best, fixed, n, lead);
+        if ((rc = exclusive_scan_i32(ctx, lead, id, n))) break;
+        int32_t cnt = 0;
+        if (hipMemcpyAsync(&cnt, id + n, sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "AMG aggregation: count");
+            break;
+        }
+        hipLaunchKernelGGL(k_amg_assign, dim3(nblk(n, 256)), dim3(256), 0, st, mate, best, fixed, id, n, agg);
+        *nc = cnt;
+        if (hipGetLastError() != hipSuccess) rc = grmp_fail(ctx, GRMP_ERR_HIP, "AMG aggregation: launch");
+    } while (0);
+    dfree(ctx, diag);
+    dfree(ctx, mate);
+    dfree(ctx, best);
+    dfree(ctx, lead);
+    dfree(ctx, id);
+    return rc;
+}
+
+// Galerkin product of a level with the aggregation agg (nc aggregates): the
+// coarse CSR in new device arrays (caller frees, or keeps via amg_alloc-owned copies)
+static int amg_galerkin(grmp_ctx *ctx, const int32_t *rp, const int32_t *ci, const double *val, int64_t n, int64_t nnz,
+                        const int32_t *agg, int64_t nc, int32_t **crp, int32_t **cci, double **cval, int64_t *cnnz)
+{
+    hipStream_t st = ctx->stream;
+    *crp = nullptr;
+    *cci = nullptr;
+    *cval = nullptr;
+    uint64_t *k0 = nullptr, *k1 = nullptr, *uk = nullptr;
+    double *v0 = nullptr, *v1 = nullptr, *us = nullptr;
+    int32_t *cnt = nullptr;
+    int rc = GRMP_OK;
+    do {
+        if ((rc = dalloc(ctx, &k0, nnz)) || (rc = dalloc(ctx, &k1, nnz)) || (rc = dalloc(ctx, &v0, nnz)) ||
+            (rc = dalloc(ctx, &v1, nnz)) || (rc = dalloc(ctx, &uk, nnz)) || (rc = dalloc(ctx, &us, nnz)))
+            break;
+        hipLaunchKernelGGL(k_amg_keys, dim3(nblk(n, 256)), dim3(256), 0, st, rp, ci, val, agg, n, nc, k0, v0);
+        int bits = 1;
+        while (bits < 64 && ((uint64_t)1 << bits) <= (uint64_t)nc * (uint64_t)nc) ++bits;
+        if ((rc = sort_pairs_u64_f64(ctx, k0, k1, v0, v1, nnz, bits))) break;
+        int64_t m = 0;
+        if ((rc = reduce_runs_u64_f64(ctx, k1, v1, nnz, uk, us, &m))) break;
+        // the sentinel run (entries of fixed rows or columns) is the last one
+        uint64_t last = 0;
+        if (m > 0) {
+            if (hipMemcpyAsync(&last, uk + (m - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess) {
+                rc = grmp_fail(ctx, GRMP_ERR_HIP, "AMG Galerkin: readback");
+                break;
+            }
+            if (last == (uint64_t)nc * (uint64_t)nc) --m;
+        }
+        if ((rc = dalloc(ctx, &cnt, nc)) || (rc = dalloc(ctx, crp, nc + 1)) || (rc = dalloc(ctx, cci, m)) ||
+            (rc = dalloc(ctx, cval, m)))
+            break;
+        if (hipMemsetAsync(cnt, 0, sizeof(int32_t) * nc, st) != hipSuccess ||
+            hipMemcpyAsync(*cval, us, sizeof(double) * m, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "AMG Galerkin: copy");
+            break;
+        }
+        hipLaunchKernelGGL(k_amg_rowcount, dim3(nblk(m, 256)), dim3(256), 0, st, uk, m, nc, cnt);
+        hipLaunchKernelGGL(k_amg_cols, dim3(nblk(m, 256)), dim3(256), 0, st, uk, m, nc, *cci);
+        if ((rc = exclusive_scan_i32(ctx, cnt, *crp, nc))) break;
+        *cnnz = m;
+        if (hipGetLastError() != hipSuccess) rc = grmp_fail(ctx, GRMP_ERR_HIP, "AMG Galerkin: launch");
+    } while (0);
+    dfree(ctx, k0);
+    dfree(ctx, k1);
+    dfree(ctx, v0);
+    dfree(ctx, v1);
+    dfree(ctx, uk);
+    dfree(ctx, us);
+    dfree(ctx, cnt);
+    if (rc) {
+        dfree(ctx, *crp);
+        dfree(ctx, *cci);
+        dfree(ctx, *cval);
+    }
+    return rc;
+}
+
+// smoother data of a level: FP32 values, inverse diagonal, Gershgorin bound
+static int amg_level_smoother(grmp_amg *m, AmgLevel &L)
+{
+    grmp_ctx *ctx = m->ctx;
+    hipStream_t st = ctx->stream;
+    double *diag = nullptr;
+    GRMP_TRY(amg_alloc(m, &L.val32, L.nnz));
+    GRMP_TRY(amg_alloc(m, &L.dinv, L.n));
+    GRMP_TRY(amg_alloc(m, &L.lam, 1));
+    GRMP_TRY(dalloc(ctx, &diag, L.n));
+    hipLaunchKernelGGL(k_amg_diag, dim3(nblk(L.n, 256)), dim3(256), 0, st, L.rp, L.ci, L.val, L.n, diag);
+    hipLaunchKernelGGL(k_amg_dinv, dim3(nblk(L.n, 256)), dim3(256), 0, st, diag, L.fixed, L.n, L.dinv);
+    hipLaunchKernelGGL(k_mg_val32, dim3(nblk(L.n, 256)), dim3(256), 0, st, L.rp, L.val, L.fixed, L.val32, L.n);
+    const hipError_t e = hipMemsetAsync(L.lam, 0, 8, st);
+    hipLaunchKernelGGL(k_mg_gersh, dim3(nblk(L.n, 256)), dim3(256), 0, st, L.rp, L.val, L.dinv, L.fixed, L.n,
+                       (unsigned long long *)L.lam);
+    dfree(ctx, diag);
+    GRMP_HIP(ctx, e);
+    GRMP_HIP(ctx, hipGetLastError());
+    return GRMP_OK;
+}
+
+static void amg_free(grmp_amg *m)
+{
+    if (!m) return;
+    for (void *p : m->mem) dfree(m->ctx, p);
+    m->mem.clear();
+    if (m->h) (void)hipHostFree(m->h);
+    if (m->ev) (void)hipEventDestroy(m->ev);
+    delete m;
+}
+
+static int amg_build(grmp_amg *m)
+{
+    grmp_ctx *ctx = m->ctx;
+    hipStream_t st = ctx->stream;
+    const grmp_matrix *A = m->A;
+    AmgLevel &L0 = m->lv[0];
+    L0.n = A->n;
+    L0.nnz = A->nnz;
+    L0.rp = A->rowptr;
+    L0.ci = A->cols;
+    L0.val = A->val64;
+    L0.fixed = A->fixed;
+    m->nlev = 1;
+    m->nnz_total = L0.nnz;
+    GRMP_TRY(amg_level_smoother(m, L0));
+    for (;;) {
+        AmgLevel &F = m->lv[m->nlev - 1];
+        // coarsest once few enough free rows remain (the dense inverse replaces fixed rows by the identity)
+        if (F.n <= MG_DENSE_MAX) break;
+        GRMP_CHECK(ctx, m->nlev < AMG_MAX_LEVELS, GRMP_ERR_UNSUPPORTED,
+                   "AMG: %lld rows left after %d levels", (long long)F.n, m->nlev);
+        // first pairwise pass on the level, second on the intermediate Galerkin product
+        int32_t *a1 = nullptr, *a2 = nullptr, *irp = nullptr, *ici = nullptr;
+        double *ival = nullptr;
+        int64_t n1 = 0, n2 = 0, innz = 0;
+        int rc = GRMP_OK;
+        do {
+            if ((rc = dalloc(ctx, &a1, F.n))) break;
+            if ((rc = amg_pairwise(ctx, F.rp, F.ci, F.val, F.fixed, F.n, a1, &n1))) break;
+            if (n1 == 0) {
+                rc = grmp_fail(ctx, GRMP_ERR_ARG, "AMG: no free rows");
+                break;
+            }
+            if ((rc = amg_galerkin(ctx, F.rp, F.ci, F.val, F.n, F.nnz, a1, n1, &irp, &ici, &ival, &innz))) break;
+            if ((rc = dalloc(ctx, &a2, n1))) break;
+            if ((rc = amg_pairwise(ctx, irp, ici, ival, nullptr, n1, a2, &n2))) break;
+        } while (0);
+        dfree(ctx, irp);
+        dfree(ctx, ici);
+        dfree(ctx, ival);
+        if (rc) {
+            dfree(ctx, a1);
+            dfree(ctx, a2);
+            return rc;
+        }
+        if (n2 > F.n - F.n / 10) {  // coarsening stalled
+            dfree(ctx, a1);
+            dfree(ctx, a2);
+            return grmp_fail(ctx, GRMP_ERR_UNSUPPORTED, "AMG: coarsening stalled at %lld rows", (long long)F.n);
+        }
+        rc = amg_alloc(m, &F.agg, F.n);
+        if (!rc) hipLaunchKernelGGL(k_amg_compose, dim3(nblk(F.n, 256)), dim3(256), 0, st, a1, a2, F.n, F.agg);
+        dfree(ctx, a1);
+        dfree(ctx, a2);
+        GRMP_TRY(rc);
+        // the coarse level
+        AmgLevel &C = m->lv[m->nlev];
+        C.n = n2;
+        int32_t *crp = nullptr, *cci = nullptr;
+        double *cval = nullptr;
+        GRMP_TRY(amg_galerkin(ctx, F.rp, F.ci, F.val, F.n, F.nnz, F.agg, n2, &crp, &cci, &cval, &C.nnz));
+        m->mem.push_back(crp);
+        m->mem.push_back(cci);
+        m->mem.push_back(cval);
+        C.rp = crp;
+        C.ci = cci;
+        C.val = cval;
+        uint8_t *cfix = nullptr;
+        GRMP_TRY(amg_alloc(m, &cfix, C.n));
+        GRMP_HIP(ctx, hipMemsetAsync(cfix, 0, C.n, st));
+        C.fixed = cfix;
+        // members of each coarse row, in fine-row order, for the restriction
+        {
+            int32_t *key = nullptr, *idx = nullptr, *skey = nullptr, *cnt = nullptr;
+            int rc2 = GRMP_OK;
+            do {
+                if ((rc2 = dalloc(ctx, &key, F.n)) || (rc2 = dalloc(ctx, &idx, F.n)) || (rc2 = dalloc(ctx, &skey, F.n)) ||
+                    (rc2 = dalloc(ctx, &cnt, C.n)) || (rc2 = amg_alloc(m, &C.midx, F.n)) ||
+                    (rc2 = amg_alloc(m, &C.mptr, C.n + 1)))
+                    break;
+                hipLaunchKernelGGL(k_amg_member_keys, dim3(nblk(F.n, 256)), dim3(256), 0, st, F.agg, F.n, (int32_t)C.n,
+                                   key, idx);
+                int bits = 1;
+                while (bits < 31 && ((int64_t)1 << bits) <= C.n) ++bits;
+                if ((rc2 = sort_pairs_i32_i32(ctx, key, skey, idx, C.midx, F.n, bits))) break;
+                if (hipMemsetAsync(cnt, 0, sizeof(int32_t) * C.n, st) != hipSuccess) {
+                    rc2 = grmp_fail(ctx, GRMP_ERR_HIP, "AMG members: memset");
+                    break;
+                }
+                hipLaunchKernelGGL(k_amg_member_count, dim3(nblk(F.n, 256)), dim3(256), 0, st, F.agg, F.n, cnt);
+                rc2 = exclusive_scan_i32(ctx, cnt, C.mptr, C.n);
+            } while (0);
+            dfree(ctx, key);
+            dfree(ctx, idx);
+            dfree(ctx, skey);
+            dfree(ctx, cnt);
+            GRMP_TRY(rc2);
+        }
+        ++m->nlev;
+        m->nnz_total += C.nnz;
+        GRMP_TRY(amg_level_smoother(m, C));
+    }
+    // vectors: level 0 aliases the CG's z and r; coarse levels own theirs
+    const int64_t n0 = L0.n;
+    m->nb0 = (int)nblk(n0, KB);
+    GRMP_TRY(amg_alloc(m, &m->x, n0));
+    GRMP_TRY(amg_alloc(m, &m->rhs, n0));
+    GRMP_TRY(amg_alloc(m, &m->r, n0));
+    GRMP_TRY(amg_alloc(m, &m->z, n0));
+    GRMP_TRY(amg_alloc(m, &m->p, n0));
+    GRMP_TRY(amg_alloc(m, &m->q, n0));
+    GRMP_TRY(amg_alloc(m, &m->part, m->nb0));
+    GRMP_TRY(amg_alloc(m, &m->part2, m->nb0));
+    GRMP_TRY(amg_alloc(m, &m->sc, 1));
+    for (int l = 0; l < m->nlev; ++l) {
+        AmgLevel &L = m->lv[l];
+        if (l == 0) {
+            L.x = m->z;
+            L.b = m->r;
+        } else {
+            GRMP_TRY(amg_alloc(m, &L.x, L.n));
+            GRMP_TRY(amg_alloc(m, &L.b, L.n));
+        }
+        GRMP_TRY(amg_alloc(m, &L.r, L.n));
+        GRMP_TRY(amg_alloc(m, &L.d, L.n));
+    }
+    const AmgLevel &C = m->lv[m->nlev - 1];
+    GRMP_TRY(amg_alloc(m, &m->inv, C.n * C.n));
+    hipLaunchKernelGGL(k_mg_dense_inv, dim3(1), dim3(256), 0, st, C.rp, C.ci, C.val, C.fixed, (int)C.n, m->inv);
+    GRMP_HIP(ctx, hipGetLastError());
+    GRMP_HIP(ctx, hipStreamSynchronize(st));
+    return GRMP_OK;
+}
+
+static void amg_rows(MgRows &a, const AmgLevel &L, const grmp_amg *m)
+{
+    memset(&a, 0, sizeof(a));
+    a.rp = L.rp;
+    a.ci = L.ci;
+    a.val = L.val;
+    a.val32 = L.val32;
+    a.fixed = L.fixed;
+    a.n = L.n;
+    a.dinv = L.dinv;
+    a.lam = L.lam;
+    a.sc = m->sc;
+}
+
+// z (= x_0) = V(r (= b_0)); level-0 partials of r.z land in part
+static void amg_vcycle(grmp_amg *m)
+{
+    hipStream_t st = m->ctx->stream;
+    const int L1 = m->nlev - 1;
+    for (int l = 0; l < L1; ++l) {
+        const AmgLevel &L = m->lv[l], &C = m->lv[l + 1];
+        MgRows a;
+        amg_rows(a, L, m);
+        a.b = L.b;
+        a.o1 = L.x;
+        hipLaunchKernelGGL(k_mg_pre, dim3(nblk(L.n, KB)), dim3(KB), 0, st, a);
+        amg_rows(a, L, m);
+        a.v = L.x;
+        a.b = L.b;
+        a.o1 = L.r;
+        hipLaunchKernelGGL(k_mg_rows<OP_RESID>, dim3(nblk(L.n, KB)), dim3(KB), 0, st, a);
+        hipLaunchKernelGGL(k_amg_restrict, dim3(nblk(C.n, 256)), dim3(256), 0, st, C.mptr, C.midx, L.r, C.b, C.n,
+                           m->sc);
+    }
+    const AmgLevel &C = m->lv[L1];
+    hipLaunchKernelGGL(k_mg_dense_apply, dim3(1), dim3(MG_DENSE_MAX), 0, st, m->inv, C.b, (int)C.n, C.x, m->sc);
+    for (int l = L1 - 1; l >= 0; --l) {
+        const AmgLevel &L = m->lv[l], &Cc = m->lv[l + 1];
+        hipLaunchKernelGGL(k_amg_prolong, dim3(nblk(L.n, 256)), dim3(256), 0, st, L.agg, Cc.x, AMG_SCALE, L.x, L.n,
+                           m->sc);
+        MgRows a;
+        amg_rows(a, L, m);
+        a.v = L.x;
+        a.b = L.b;
+        a.o1 = L.d;
+        a.o2 = L.r;
+        hipLaunchKernelGGL(k_mg_rows<OP_CHEB_POST1>, dim3(nblk(L.n, KB)), dim3(KB), 0, st, a);
+        amg_rows(a, L, m);
+        a.v = L.r;
+        a.b = L.b;
+        a.d = L.d;
+        a.o1 = L.x;
+        a.part = l == 0 ? m->part : nullptr;
+        hipLaunchKernelGGL(k_mg_rows<OP_CHEB_POST2>, dim3(nblk(L.n, KB)), dim3(KB), 0, st, a);
+    }
+    if (L1 == 0) {  // one level: z = the dense solve, partials of r.z
+        hipLaunchKernelGGL(k_mg_dot, dim3(m->nb0), dim3(KB), 0, st, m->r, m->z, C.n, m->part, m->sc);
+    }
+}
+
+static void amg_iterate(grmp_amg *m, int iters)
+{
+    hipStream_t st = m->ctx->stream;
+    const AmgLevel &L0 = m->lv[0];
+    const int64_t n = L0.n;
+    for (int k = 0; k < iters; ++k) {
+        MgRows a;
+        amg_rows(a, L0, m);
+        a.v = m->p;
+        a.o1 = m->q;
+        a.part = m->part;
+        hipLaunchKernelGGL(k_mg_rows<OP_PCG_Q>, dim3(nblk(n, KB)), dim3(KB), 0, st, a);
+        hipLaunchKernelGGL(k_mg_red<RED_ALPHA>, dim3(1), dim3(KB), 0, st, m->part, m->nb0, nullptr, m->sc);
+        hipLaunchKernelGGL(k_mg_axpy, dim3(m->nb0), dim3(KB), 0, st, m->x, m->r, m->p, m->q, n, m->part, m->sc);
+        hipLaunchKernelGGL(k_mg_red<RED_CONV>, dim3(1), dim3(KB), 0, st, m->part, m->nb0, nullptr, m->sc);
+        amg_vcycle(m);
+        hipLaunchKernelGGL(k_mg_red<RED_BETA>, dim3(1), dim3(KB), 0, st, m->part, m->nb0, nullptr, m->sc);
+        hipLaunchKernelGGL(k_mg_update_p, dim3(nblk(n, 256)), dim3(256), 0, st, m->p, m->z, n, m->sc);
+    }
+}
+
+// the true residual b - A x (FP64 values): accept, or restart CG from it
+static void amg_verify(grmp_amg *m)
+{
+    hipStream_t st = m->ctx->stream;
+    const AmgLevel &L0 = m->lv[0];
+    const int64_t n = L0.n;
+    hipLaunchKernelGGL(k_mg_verify_begin, dim3(1), dim3(1), 0, st, m->sc);
+    MgRows a;
+    amg_rows(a, L0, m);
+    a.v = m->x;
+    a.b = m->rhs;
+    a.o1 = m->r;
+    hipLaunchKernelGGL(k_mg_rows<OP_TRUE_RESID>, dim3(nblk(n, KB)), dim3(KB), 0, st, a);
+    hipLaunchKernelGGL(k_mg_dot, dim3(m->nb0), dim3(KB), 0, st, m->r, m->r, n, m->part, m->sc);
+    hipLaunchKernelGGL(k_mg_red<RED_VERIFY>, dim3(1), dim3(KB), 0, st, m->part, m->nb0, nullptr, m->sc);
+    amg_vcycle(m);
+    hipLaunchKernelGGL(k_mg_red<RED_RESTART>, dim3(1), dim3(KB), 0, st, m->part, m->nb0, nullptr, m->sc);
+    hipLaunchKernelGGL(k_mg_update_p, dim3(nblk(n, 256)), dim3(256), 0, st, m->p, m->z, n, m->sc);
+}
+
+static int amg_status(grmp_amg *m)
+{
+    hipStream_t st = m->ctx->stream;
+    GRMP_HIP(m->ctx, hipMemcpyAsync(&m->h->relres, &m->sc->relres, 8, hipMemcpyDeviceToHost, st));
+    GRMP_HIP(m->ctx, hipMemcpyAsync(&m->h->it, &m->sc->it, 12, hipMemcpyDeviceToHost, st));
+    GRMP_HIP(m->ctx, hipEventRecord(m->ev, st));
+    GRMP_HIP(m->ctx, hipEventSynchronize(m->ev));
+    return GRMP_OK;
+}
+
+extern "C" int grmp_amg_destroy(grmp_amg *m)
+{
+    if (!m) return GRMP_OK;
+    grmp_ctx *ctx = m->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipStreamSynchronize(ctx->stream);
+    amg_free(m);
+    return GRMP_OK;
+}
+
+extern "C" int grmp_amg_create(grmp_matrix *A, grmp_amg **out)
+{
+    if (!A || !out) return GRMP_ERR_ARG;
+    *out = nullptr;
+    grmp_ctx *ctx = A->ctx;
+    GRMP_CHECK(ctx, A->have_pattern && A->val64 && A->fixed && A->n >= 1, GRMP_ERR_ARG,
+               "grmp_amg_create: matrix not assembled in FP64");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    grmp_amg *m = new grmp_amg();
+    m->ctx = ctx;
+    m->A = A;
+    int rc = GRMP_OK;
+    if (hipHostMalloc((void **)&m->h, sizeof(MgStatus), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&m->ev, hipEventDisableTiming) != hipSuccess) {
+        m->h = nullptr;
+        rc = grmp_fail(ctx, GRMP_ERR_HIP, "grmp_amg_create: status buffer");
+    }
+    if (!rc) rc = amg_build(m);
+    if (rc) {
+        (void)hipStreamSynchronize(ctx->stream);
+        amg_free(m);
+        return rc;
+    }
+    *out = m;
+    return GRMP_OK;
+}
+
+extern "C" int grmp_amg_info(const grmp_amg *m, int *nlev, int64_t *coarse_rows, double *complexity)
+{
+    if (!m) return GRMP_ERR_ARG;
+    if (nlev) *nlev = m->nlev;
+    if (coarse_rows) *coarse_rows = m->lv[m->nlev - 1].n;
+    if (complexity) *complexity = (double)m->nnz_total / (double)std::max<int64_t>(m->lv[0].nnz, 1);
+    return GRMP_OK;
+}
+
+extern "C" int grmp_amg_apply(grmp_amg *m, const grmp_vector *b, grmp_vector *x, double rtol, int64_t maxit,
+                              grmp_solve_info *info)
+{
+    if (!m || !b || !x) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = m->ctx;
+    const int64_t n = m->lv[0].n;
+    GRMP_CHECK(ctx, b->ctx == ctx && x->ctx == ctx && b->n == n && x->n == n, GRMP_ERR_ARG,
+               "grmp_amg_apply: b and x need length %lld", (long long)n);
+    GRMP_CHECK(ctx, rtol > 0.0 && rtol < 1.0 && maxit >= 1 && maxit <= (1 << 30), GRMP_ERR_ARG,
+               "grmp_amg_apply: rtol in (0, 1) and maxit >= 1 needed");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipStream_t st = ctx->stream;
+    const double t0 = wall_seconds();
+    MgStatus *h = m->h;
+    memset(h, 0, sizeof(MgStatus));
+    int rc = GRMP_OK;
+    do {
+        // x = b_d / A_dd on fixed rows, 0 elsewhere; r = b - A x on free rows; z = V(r); p = z
+        MgScal s0;
+        memset(&s0, 0, sizeof(s0));
+        s0.rtol = rtol;
+        s0.maxit = (int)maxit;
+        if (hipMemcpyAsync(m->sc, &s0, sizeof(s0), hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(m->rhs, b->d, sizeof(double) * n, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+            hipMemsetAsync(m->p, 0, sizeof(double) * n, st) != hipSuccess) {
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "grmp_amg_apply: start");
+            break;
+        }
+        if ((rc = init_x_launch(m->A, m->A->val64, m->rhs, m->x, st))) break;
+        MgRows a;
+        amg_rows(a, m->lv[0], m);
+        a.v = m->x;
+        a.b = m->rhs;
+        a.o1 = m->r;
+        hipLaunchKernelGGL(k_mg_rows<OP_TRUE_RESID>, dim3(nblk(n, KB)), dim3(KB), 0, st, a);
+        amg_vcycle(m);
+        hipLaunchKernelGGL(k_mg_dot, dim3(m->nb0), dim3(KB), 0, st, m->r, m->r, n, m->part2, m->sc);
+        hipLaunchKernelGGL(k_mg_red<RED_INIT>, dim3(1), dim3(KB), 0, st, m->part, m->nb0, m->part2, m->sc);
+        hipLaunchKernelGGL(k_mg_update_p, dim3(nblk(n, 256)), dim3(256), 0, st, m->p, m->z, n, m->sc);
+        amg_iterate(m, 4);
+        if ((rc = amg_status(m))) break;
+        while (h->done != 1) {
+            if (h->done == 2) amg_verify(m);
+            else amg_iterate(m, 4);
+            if ((rc = amg_status(m))) break;
+        }
+        if (rc) break;
+        if (hipMemcpyAsync(x->d, m->x, sizeof(double) * n, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            rc = grmp_fail(ctx, GRMP_ERR_HIP, "grmp_amg_apply: solution");
+    } while (0);
+    grmp_solve_info inf;
+    memset(&inf, 0, sizeof(inf));
+    inf.status = h->status;
+    inf.iterations = h->it;
+    inf.relres = h->relres;
+    inf.outer = 1;
+    inf.seconds = wall_seconds() - t0;
+    if (info) *info = inf;
+    if (rc) {
+        (void)hipStreamSynchronize(st);
+        return rc;
+    }
+    if (inf.status == GRMP_ERR_NOCONV)
+        return grmp_fail(ctx, GRMP_ERR_NOCONV, "grmp_amg_apply: no convergence, relres %.3e after %lld iterations",
+                         inf.relres, (long long)inf.iterations);
+    return inf.status;
+}

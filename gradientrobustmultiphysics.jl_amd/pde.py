@@ -672,6 +672,45 @@ class MultigridSolver:
             pass
 
 
+class AlgebraicMultigridSolver:
+    """Algebraic multigrid (grmp_amg_create) for an assembled FP64 system on
+    any mesh, then FP64 CG with its V-cycle per right-hand side
+    (grmp_amg_apply); b is the penalized right-hand side, fixed dofs are
+    eliminated.  A must stay alive and unchanged while the handle lives."""
+
+    def __init__(self, A):
+        self.A, self.ctx = A, A.ctx
+        h = _abi.VP()
+        self.ctx.check(lib().grmp_amg_create(A.h, ctypes.byref(h)))
+        self.h = h
+
+    def solve(self, b, x, rtol, maxit=2000):
+        info = _abi.SolveInfo()
+        self.ctx.check(lib().grmp_amg_apply(self.h, b.vec.h, x.vec.h, float(rtol), int(maxit), ctypes.byref(info)))
+        return info
+
+    def info(self):
+        """(levels, rows of the coarsest level, operator complexity)"""
+        nl, cr, cx = ctypes.c_int(), ctypes.c_int64(), ctypes.c_double()
+        self.ctx.check(lib().grmp_amg_info(self.h, ctypes.byref(nl), ctypes.byref(cr), ctypes.byref(cx)))
+        return nl.value, cr.value, cx.value
+
+    def __del__(self):
+        try:
+            if self.h and _abi.alive():
+                lib().grmp_amg_destroy(self.h)
+            self.h = None
+        except Exception:
+            pass
+
+
+def amg_eligible(pde, Solution):
+    """True when the algebraic multigrid CG applies: a linear stationary
+    problem without Lagrange multipliers (symmetric positive definite on the
+    free dofs)."""
+    return not _saddle_point(pde, Solution) and not is_nonlinear(pde)
+
+
 def lift_dirichlet(A, b, target, out):
     """out = b - A target on the free rows of A (penalty rows as target
     times the penalty; the multigrid solve zeroes them): the right-hand side
@@ -698,16 +737,28 @@ def solve(Solution, pde, config=None, solver="krylov", **kwargs):
     SolveInfo of the device solve.  solver="krylov": the device Krylov
     solvers (CG / MINRES, FP64 or FP32 inner); "mg": the multigrid V-cycle CG
     (FP64) where mg_eligible holds, and for the Stokes system MINRES with the
-    multigrid velocity block (stokes_mg_eligible); "auto": multigrid where it
-    applies."""
+    multigrid velocity block (stokes_mg_eligible); "amg": CG with algebraic
+    multigrid on any mesh (amg_eligible); "auto": the geometric multigrid where
+    it applies, else (FP64) the algebraic one, else Krylov."""
     cfg = config or SolverConfig(**kwargs)
-    if solver not in ("auto", "krylov", "mg"):
-        raise _abi.GrmpError(_abi.ERR_ARG, f"solver {solver!r}: 'auto', 'krylov' or 'mg'")
+    if solver not in ("auto", "krylov", "mg", "amg"):
+        raise _abi.GrmpError(_abi.ERR_ARG, f"solver {solver!r}: 'auto', 'krylov', 'mg' or 'amg'")
     if is_nonlinear(pde):
         if solver == "mg":
             raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "nonlinear problems are solved by the Newton fixpoint loop")
         return solve_fixpoint(Solution, pde, cfg)
-    use_mg = solver != "krylov" and mg_eligible(pde, Solution)
+    use_mg = solver in ("auto", "mg") and mg_eligible(pde, Solution)
+    use_amg = (solver == "amg" or (solver == "auto" and not use_mg and cfg.precision == "fp64")) and \
+        amg_eligible(pde, Solution)
+    if solver == "amg" and not use_amg:
+        raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "algebraic multigrid needs a linear problem without Lagrange "
+                             "multipliers")
+    if use_amg:
+        A, b, fixed = assemble(pde, Solution, _abi.VALUES_FP64, cfg.fixed_penalty)
+        amg = AlgebraicMultigridSolver(A)
+        info = amg.solve(b, Solution, cfg.rtol, min(cfg.maxiterations, 2000))
+        realize_constraints(pde, Solution)
+        return info, A, b
     if solver == "mg" and not use_mg and not stokes_mg_eligible(pde, Solution):
         raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "multigrid needs one H1P1 / H1P2 unknown on a generated unit "
                              "square, Dirichlet data and no global constraints")

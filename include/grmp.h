@@ -411,6 +411,25 @@ int grmp_mg_destroy(grmp_mg *m);
 int grmp_solve_minres_mg(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, const grmp_solve_opts *opts,
                          grmp_mg *velocity, int ncomp, grmp_solve_info *info);
 
+/* Algebraic multigrid as the preconditioner of FP64 CG, for symmetric
+ * positive definite systems on any mesh (uploaded meshes, tetrahedra: the
+ * multigrid above needs generated unit squares).  The hierarchy is built on
+ * the device from the assembled FP64 matrix: two passes of pairwise matching
+ * per level (aggregates of up to four rows), Galerkin coarse operators,
+ * Chebyshev smoothing with FP32-stored values, the dense inverse on the
+ * coarsest level.  Fixed (penalized) dofs are eliminated (x_d = b_d / A_dd),
+ * so b is the penalized right-hand side of grmp_solve.  The handle reads A's
+ * pattern and values: keep A alive and unchanged, create again after it is
+ * reassembled.  No counterpart in the reference, which factorizes
+ * (src/solvers.jl:55, 656-660). */
+typedef struct grmp_amg grmp_amg;
+int grmp_amg_create(grmp_matrix *A, grmp_amg **out);
+int grmp_amg_apply(grmp_amg *m, const grmp_vector *b, grmp_vector *x, double rtol, int64_t maxit,
+                   grmp_solve_info *info);
+/* levels, rows of the coarsest level, operator complexity (summed nonzeros / level-0 nonzeros) */
+int grmp_amg_info(const grmp_amg *m, int *nlev, int64_t *coarse_rows, double *complexity);
+int grmp_amg_destroy(grmp_amg *m);
+
 /* Solver plug-in: solves LinearProblem(A.entries.cscmatrix, b.entries) as the
  * linsolver of solve! (src/solvers.jl:55, 89-101, 656-660).  colptr[n+1],
  * rowval[nnz] 1-based (Int64, as SparseMatrixCSC{Float64,Int64}); fixed[nfixed]
