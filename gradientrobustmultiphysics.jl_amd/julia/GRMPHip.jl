@@ -345,6 +345,31 @@ function LinearSolve.SciMLBase.solve!(cache::LinearSolve.LinearCache, alg::GRMPK
     return LinearSolve.SciMLBase.build_linear_solution(alg, x, nothing, cache)
 end
 
-export GRMPKrylov, HIPOperator, hip_l2error, hip_assemble_batch!
+# ---------------------------------------------------- algebraic multigrid
+"""
+    hip_amg_solve!(A, b, x; rtol = 1e-10, maxit = 2000) -> GrmpSolveInfo
+
+CG preconditioned by the device's algebraic multigrid for an SPD device
+matrix `A` (a `grmp_matrix` handle with FP64 values, penalized rows set by
+`grmp_matrix_set_fixed`) on any mesh; `b` and `x` are `grmp_vector` handles.
+The hierarchy is built once per call here; a caller that solves many
+right-hand sides with one matrix keeps the `grmp_amg` handle of
+`grmp_amg_create` and calls `grmp_amg_apply` for each.
+"""
+function hip_amg_solve!(A::Ptr{Cvoid}, b::Ptr{Cvoid}, x::Ptr{Cvoid}; rtol = 1e-10, maxit = 2000)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(context(), ccall((:grmp_amg_create, libgrmp), Cint, (Ptr{Cvoid}, Ref{Ptr{Cvoid}}), A, h))
+    info = GrmpSolveInfo()
+    try
+        check(context(), ccall((:grmp_amg_apply, libgrmp), Cint,
+                               (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Int64, Ref{GrmpSolveInfo}),
+                               h[], b, x, rtol, maxit, info))
+    finally
+        ccall((:grmp_amg_destroy, libgrmp), Cint, (Ptr{Cvoid},), h[])
+    end
+    return info
+end
+
+export GRMPKrylov, HIPOperator, hip_l2error, hip_assemble_batch!, hip_amg_solve!
 
 end # module
