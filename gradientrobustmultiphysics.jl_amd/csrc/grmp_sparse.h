@@ -62,10 +62,28 @@ __device__ __forceinline__ T csr_rows_lds(const int32_t *__restrict__ rp, const 
         my1 = rp[r + 1];
     }
     T s = (T)0;
+    const int bd = blockDim.x;
     for (int32_t c0 = base; c0 < end; c0 += CAP) {
         const int32_t c1 = c0 + CAP < end ? c0 + CAP : end;
         __syncthreads();
-        for (int32_t k = c0 + t; k < c1; k += blockDim.x) lds[k - c0] = (T)v[k] * (T)x[ci[k]];
+        // four products per step: all column and value loads, then all x
+        // gathers, are issued before the first is used
+        int32_t k = c0 + t;
+        for (; k + 3 * bd < c1; k += 4 * bd) {
+            int32_t c[4];
+            V w[4];
+            X xv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                c[u] = ci[k + u * bd];
+                w[u] = v[k + u * bd];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xv[u] = x[c[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) lds[k + u * bd - c0] = (T)w[u] * (T)xv[u];
+        }
+        for (; k < c1; k += bd) lds[k - c0] = (T)v[k] * (T)x[ci[k]];
         __syncthreads();
         const int32_t a = my0 > c0 ? my0 : c0, b = my1 < c1 ? my1 : c1;
         for (int32_t k = a; k < b; ++k) s += lds[k - c0];
