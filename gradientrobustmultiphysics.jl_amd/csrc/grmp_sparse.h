@@ -41,6 +41,9 @@ __device__ __forceinline__ void block_sum2(double &a, double &b)
 
 // Nonzeros staged per block by csr_rows_lds (32 KB of LDS in FP64).
 #define SPMV_CAP 4096
+#ifndef SPMV_UNROLL
+#define SPMV_UNROLL 8
+#endif
 
 // Rows r0 .. r1-1 (r1 - r0 <= blockDim.x) of y = A x.  The block streams its
 // contiguous nonzero range rowptr[r0] .. rowptr[r1] through LDS as products
@@ -66,24 +69,32 @@ __device__ __forceinline__ T csr_rows_lds(const int32_t *__restrict__ rp, const 
     for (int32_t c0 = base; c0 < end; c0 += CAP) {
         const int32_t c1 = c0 + CAP < end ? c0 + CAP : end;
         __syncthreads();
-        // four products per step: all column and value loads, then all x
-        // gathers, are issued before the first is used
-        int32_t k = c0 + t;
-        for (; k + 3 * bd < c1; k += 4 * bd) {
-            int32_t c[4];
-            V w[4];
-            X xv[4];
+        // SPMV_UNROLL products per step: all column and value loads, then all
+        // x gathers, are issued before the first is used.  Loads past the
+        // chunk's end are masked off (whole waves skip them).
+        constexpr int U = SPMV_UNROLL;
+        for (int32_t k = c0 + t; k < c1; k += U * bd) {
+            int32_t c[U];
+            V w[U];
+            X xv[U];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                c[u] = ci[k + u * bd];
-                w[u] = v[k + u * bd];
+            for (int u = 0; u < U; ++u) {
+                c[u] = 0;
+                w[u] = (V)0;
+                if (k + u * bd < c1) {
+                    c[u] = ci[k + u * bd];
+                    w[u] = v[k + u * bd];
+                }
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) xv[u] = x[c[u]];
+            for (int u = 0; u < U; ++u) {
+                xv[u] = (X)0;
+                if (k + u * bd < c1) xv[u] = x[c[u]];
+            }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) lds[k + u * bd - c0] = (T)w[u] * (T)xv[u];
+            for (int u = 0; u < U; ++u)
+                if (k + u * bd < c1) lds[k + u * bd - c0] = (T)w[u] * (T)xv[u];
         }
-        for (; k < c1; k += bd) lds[k - c0] = (T)v[k] * (T)x[ci[k]];
         __syncthreads();
         const int32_t a = my0 > c0 ? my0 : c0, b = my1 < c1 ? my1 : c1;
         for (int32_t k = a; k < b; ++k) s += lds[k - c0];
