@@ -610,17 +610,21 @@ struct Kind {
     static constexpr int CN = CN_;
     static constexpr bool SPLIT = false;  // no separate gather step
     static constexpr int NV = 1;
+    static constexpr int ROW_NB = 1;
     using Geo = int;
     __device__ static __forceinline__ void row(const DevForm &F, int64_t e, int a, double *v) { ROW(F, e, a, v); }
     __device__ static __forceinline__ void rowg(const DevForm &, const Geo &, int, double *) {}
 };
 // kinds whose rows depend on the cell only through its corners and
-// coefficient: row() gathers them, rowg() computes from a gathered cell
-template <int CN_, class GEO, int NV_, void (*ROWG)(const DevForm &, const GEO &, int, double *)>
+// coefficient: row() gathers them, rowg() computes from a gathered cell.
+// RNB: incidences whose gathers the serial row walk keeps in flight together
+// (3 measured fastest for rows of 6 to 24 cells, 2 for P2's rows of 1 to 6).
+template <int CN_, class GEO, int NV_, void (*ROWG)(const DevForm &, const GEO &, int, double *), int RNB = 3>
 struct KindG {
     static constexpr int CN = CN_;
     static constexpr bool SPLIT = true;
     static constexpr int NV = NV_;
+    static constexpr int ROW_NB = RNB;
     using Geo = GEO;
     __device__ static __forceinline__ void rowg(const DevForm &F, const Geo &g, int a, double *v) { ROWG(F, g, a, v); }
     __device__ static __forceinline__ void row(const DevForm &F, int64_t e, int a, double *v)
@@ -634,7 +638,7 @@ struct KindG {
 };
 using K_LAP_P1T1 = KindG<3, TriGeo, 3, row_laplace_tri_g<1, 1>>;
 using K_LAP_P1T3 = KindG<3, TriGeo, 3, row_laplace_tri_g<1, 3>>;
-using K_LAP_P2T3 = KindG<6, TriGeo, 3, row_laplace_tri_g<2, 3>>;
+using K_LAP_P2T3 = KindG<6, TriGeo, 3, row_laplace_tri_g<2, 3>, 2>;
 using K_LAP_P1TET = KindG<4, TetGeo, 4, row_laplace_tet_g>;
 using K_HOOKE1 = KindG<6, TriGeo, 3, row_hooke_tri_g<1>>;
 using K_HOOKE3 = KindG<6, TriGeo, 3, row_hooke_tri_g<3>>;
@@ -794,6 +798,48 @@ __device__ __forceinline__ void assemble_row(const DevForm &F, int64_t i, double
 {
     const int32_t b0 = F.inc_ptr[i], b1 = F.inc_ptr[i + 1];
     constexpr int W = (K::CN + 3) / 4;
+    if constexpr (K::SPLIT) {
+        // K::ROW_NB incidences at a time: their incidence and slot words,
+        // node ids, then corners and coefficients are each loaded together
+        // before any local row is computed; the row adds them in incidence
+        // order as the loop below does.  Past the row's end a lane loads the
+        // row's last incidence again and adds nothing.
+        constexpr int NB = K::ROW_NB, NV = K::NV;
+        for (int32_t t0 = b0; t0 < b1; t0 += NB) {
+            int32_t pk[NB];
+            uint32_t w[NB][W];
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int32_t t = t0 + k < b1 ? t0 + k : b1 - 1;
+                pk[k] = F.inc[t];
+#pragma unroll
+                for (int q = 0; q < W; ++q) w[k][q] = F.pos[(int64_t)t * W + q];
+            }
+            int64_t e[NB];
+            int32_t nv[NB][NV];
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                e[k] = pk[k] / F.rnloc;
+                cell_nodes<NV>(F, e[k], nv[k]);
+            }
+            typename K::Geo G[NB];
+#pragma unroll
+            for (int k = 0; k < NB; ++k) geo_load(F, e[k], nv[k], G[k]);
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                if (t0 + k < b1) {
+                    double v[K::CN];
+                    K::rowg(F, G[k], pk[k] - (int)e[k] * F.rnloc, v);
+#pragma unroll
+                    for (int b = 0; b < K::CN; ++b) {
+                        const uint32_t p = (w[k][b >> 2] >> (8 * (b & 3))) & 255u;
+                        if (p != 255u) row[p] += v[b];
+                    }
+                }
+            }
+        }
+        return;
+    }
     for (int32_t t = b0; t < b1; ++t) {
         const int32_t pk = F.inc[t];
         const int64_t e = pk / F.rnloc;
@@ -1149,10 +1195,10 @@ static bool launch_assemble1g(const DevForm &F, int rb, int remap, int64_t g0, i
 
 // Launches the value assembly of row block [g0, g1): a per-kind kernel when
 // the block has one form (hf = its host copy), the generic one otherwise.
-// Built-in choice for one form: the serial row walk up to 12 cell incidences
-// per row (triangles), the staged kernel with batched gathers above
-// (tetrahedra; the staged kernel for kinds without a gather step), blocks in
-// the hardware's order.
+// Built-in choice for one form: the serial row walk with batched gathers for
+// kinds with a separate gather step; for the others the serial walk up to 12
+// cell incidences per row, the staged kernel above; blocks in the hardware's
+// order.
 static void launch_assemble(const grmp_matrix *A, const grmp_space *sp, const DevForm *hf, FormPack P, int64_t g0,
                             int64_t g1, int B, size_t shm, double *val64, float *val32, double *dinv64, float *dinv32,
                             hipStream_t st)
@@ -1161,7 +1207,12 @@ static void launch_assemble(const grmp_matrix *A, const grmp_space *sp, const De
     const int mr = A->max_row > 0 ? A->max_row : 1;
     if (P.n == 1) {
         const AsmTune T = asm_tune(A, *hf);
-        const int variant = T.variant ? T.variant : (inc_per_row <= 12.0 ? 1 : 3);
+        bool split = false;
+        {
+            const DevForm F = *hf;
+            GRMP_DISPATCH(F, split = KT::SPLIT);
+        }
+        const int variant = T.variant ? T.variant : ((split || inc_per_row <= 12.0) ? 1 : 3);
         const int remap = T.remap >= 0 ? T.remap : 0;
         if (variant == 1) {
             // rows per block: a multiple of 64 whose rows fit the 48 KB of LDS
