@@ -1808,6 +1808,13 @@ struct grmp_mg {
     MgStatus *h;  // pinned
     const grmp_matrix *A0;
     int64_t n;
+    // a second copy of the hierarchy's values and vectors on a stream of its
+    // own, so that two components' V-cycles run at the same time
+    // (mg_precond_apply_pair); built on first use
+    std::vector<MgLevelIn> in;
+    MgJob *job2 = nullptr;
+    hipStream_t st2 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 extern "C" int grmp_mg_destroy(grmp_mg *m)
@@ -1818,6 +1825,14 @@ extern "C" int grmp_mg_destroy(grmp_mg *m)
         mg_job_destroy(m->job);
         (void)hipStreamSynchronize(m->ctx->stream);
     }
+    if (m->st2) {
+        (void)hipStreamSynchronize(m->st2);
+        if (m->job2) mg_job_destroy(m->job2);
+        (void)hipStreamSynchronize(m->st2);
+        (void)hipStreamDestroy(m->st2);
+    }
+    if (m->fork) (void)hipEventDestroy(m->fork);
+    if (m->join) (void)hipEventDestroy(m->join);
     if (m->h) (void)hipHostFree(m->h);
     delete m;
     return GRMP_OK;
@@ -1842,6 +1857,7 @@ extern "C" int grmp_mg_create(grmp_ctx *ctx, int nlev, const grmp_matrix *const 
     std::lock_guard<std::mutex> g(ctx->mu);
     hipStream_t st = ctx->stream;
     grmp_mg *m = new grmp_mg{ctx, nullptr, nullptr, levels[0], levels[0]->n};
+    m->in = in;
     int rc = GRMP_OK;
     do {
         if ((rc = mg_job_create(ctx, st, in.data(), nlev, false, 2048, 2048, false, &m->job))) break;
@@ -1942,6 +1958,59 @@ int64_t mg_level0_rows(const grmp_mg *m) { return m->n; }
 // hierarchy's own stream): symmetric and positive definite, so it can
 // precondition MINRES as it does CG.  z and v are n-vectors of the level-0
 // numbering; fixed rows of z are 0.
+// the second hierarchy of mg_precond_apply_pair: the same levels assembled
+// from the same coefficient, so its V-cycle gives the first one's bits
+int mg_pair_prepare(grmp_mg *m)
+{
+    if (m->job2) return GRMP_OK;
+    grmp_ctx *ctx = m->ctx;
+    if (!m->st2 && hipStreamCreateWithFlags(&m->st2, hipStreamNonBlocking) != hipSuccess) {
+        m->st2 = nullptr;
+        return grmp_fail(ctx, GRMP_ERR_HIP, "multigrid pair: stream");
+    }
+    if ((!m->fork && hipEventCreateWithFlags(&m->fork, hipEventDisableTiming) != hipSuccess) ||
+        (!m->join && hipEventCreateWithFlags(&m->join, hipEventDisableTiming) != hipSuccess))
+        return grmp_fail(ctx, GRMP_ERR_HIP, "multigrid pair: events");
+    MgJob *j2 = nullptr;
+    GRMP_TRY(mg_job_create(ctx, m->st2, m->in.data(), (int)m->in.size(), false, 2048, 2048, false, &j2));
+    int rc = GRMP_OK;
+    // the first hierarchy's coefficient is complete on the context stream
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(mg_job_kappa0(j2), mg_job_kappa0(m->job), sizeof(double) * m->in[0].sp->mesh->ncells,
+                       hipMemcpyDeviceToDevice, m->st2) != hipSuccess)
+        rc = grmp_fail(ctx, GRMP_ERR_HIP, "multigrid pair: coefficient");
+    if (!rc) rc = mg_job_assemble(j2);
+    if (!rc && hipStreamSynchronize(m->st2) != hipSuccess) rc = grmp_fail(ctx, GRMP_ERR_HIP, "multigrid pair: assembly");
+    if (rc) {
+        (void)hipStreamSynchronize(m->st2);
+        mg_job_destroy(j2);
+        return rc;
+    }
+    m->job2 = j2;
+    return GRMP_OK;
+}
+
+// z0 = V v0 on the hierarchy's stream st and z1 = V v1 on the second
+// hierarchy's stream, at the same time; st waits for both
+int mg_precond_apply_pair(grmp_mg *m, const double *v0, double *z0, const double *v1, double *z1, hipStream_t st)
+{
+    MgJob *j = m->job, *j2 = m->job2;
+    const int64_t n = m->n;
+    GRMP_CHECK(m->ctx, st == j->st && j2, GRMP_ERR_ARG, "multigrid preconditioner pair: not prepared");
+    GRMP_HIP(m->ctx, hipEventRecord(m->fork, st));
+    GRMP_HIP(m->ctx, hipStreamWaitEvent(m->st2, m->fork, 0));
+    hipLaunchKernelGGL(k_mg_pc_in, dim3(nblk(n, 256)), dim3(256), 0, m->st2, v1, m->A0->fixed, j2->r, j2->sc, n);
+    mg_vcycle(j2);
+    GRMP_HIP(m->ctx, hipMemcpyAsync(z1, j2->z, sizeof(double) * n, hipMemcpyDeviceToDevice, m->st2));
+    hipLaunchKernelGGL(k_mg_pc_in, dim3(nblk(n, 256)), dim3(256), 0, st, v0, m->A0->fixed, j->r, j->sc, n);
+    mg_vcycle(j);
+    GRMP_HIP(m->ctx, hipMemcpyAsync(z0, j->z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    GRMP_HIP(m->ctx, hipEventRecord(m->join, m->st2));
+    GRMP_HIP(m->ctx, hipStreamWaitEvent(st, m->join, 0));
+    GRMP_HIP(m->ctx, hipGetLastError());
+    return GRMP_OK;
+}
+
 int mg_precond_apply(grmp_mg *m, const double *v, double *z, hipStream_t st)
 {
     MgJob *j = m->job;

@@ -366,7 +366,10 @@ struct MrPrec {
 // np partials of z.v
 static int mr_precondition(MrHost &H, const MrPrec &pc, const double *v, double *z, int64_t n, int np)
 {
-    for (int k = 0; k < pc.ncomp; ++k) GRMP_TRY(mg_precond_apply(pc.mg, v + k * pc.nv, z + k * pc.nv, H.st));
+    if (pc.ncomp == 2)  // both components' V-cycles at once, on two copies of the hierarchy
+        GRMP_TRY(mg_precond_apply_pair(pc.mg, v, z, v + pc.nv, z + pc.nv, H.st));
+    else
+        for (int k = 0; k < pc.ncomp; ++k) GRMP_TRY(mg_precond_apply(pc.mg, v + k * pc.nv, z + k * pc.nv, H.st));
     hipLaunchKernelGGL(k_mr_dot, dim3(np), dim3(KB), 0, H.st, z, v, n, H.part);
     return GRMP_OK;
 }
@@ -433,6 +436,7 @@ int minres_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solve_
         GRMP_CHECK(ctx, vel_ncomp >= 1 && vel_ncomp <= 3 && pc.nv * vel_ncomp <= n, GRMP_ERR_ARG,
                    "MINRES: %d velocity components of %lld rows exceed the system's %lld", vel_ncomp,
                    (long long)pc.nv, (long long)n);
+        if (vel_ncomp == 2) GRMP_TRY(mg_pair_prepare(vel_mg));
     }
     MrHost H;
     H.ctx = ctx;
