@@ -267,18 +267,45 @@ __device__ __forceinline__ void grad_tet(const double M[3][3], int d, double g[3
     }
 }
 
+// grad_tet for d = 0..3 at once, with the same bits when every entry of M is
+// finite: its sum starts at +0, so the running sum is never -0 and the
+// products M[k][j] * 0 (zeros of either sign) leave it unchanged; what
+// remains is the sum from +0 of the terms with coefficient -1 or 1
+__device__ __forceinline__ void grad_tet_all(const double M[3][3], double g[4][3])
+{
+    bool fin = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) fin &= isfinite(M[k][j]);
+    if (!fin) {
+#pragma unroll
+        for (int d = 0; d < 4; ++d) grad_tet(M, d, g[d]);
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        g[0][k] = ((0.0 - M[k][0]) - M[k][1]) - M[k][2];
+        g[1][k] = 0.0 + M[k][0];
+        g[2][k] = 0.0 + M[k][1];
+        g[3][k] = 0.0 + M[k][2];
+    }
+}
+
 __device__ __forceinline__ void row_laplace_tet_g(const DevForm &F, const TetGeo &G, int a, double *v)
 {
     double M[3][3], vol;
     trafo_x(G.X, M, vol);
     const double kap = G.kap;
     const double itemfactor = vol * F.factor * 1.0;
+    double g4[4][3];
+    grad_tet_all(M, g4);
     double ga[3];
-    grad_tet(M, a, ga);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ga[k] = a == 0 ? g4[0][k] : a == 1 ? g4[1][k] : a == 2 ? g4[2][k] : g4[3][k];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-        double gb[3];
-        grad_tet(M, b, gb);
+        const double *gb = g4[b];
         double t = 0.0;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
