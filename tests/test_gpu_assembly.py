@@ -153,6 +153,21 @@ def test_tet_parity_and_mesh(grmp, O):
                                             coef=O.COEF_CELL, kappa=kap.download(), coords=c, cells=cn,
                                             celldofs=cn)], fixed=bd.astype(np.int32))
     assert _check_csc(A, ref) == 1.0
+    # the unjittered Kuhn mesh: axis-aligned cells, whose maps have exact
+    # zeros, and exactly zero local entries that the pattern leaves out;
+    # values compared bit for bit (signs of zero included)
+    c0, cn0 = g0.download()
+    fes0 = grmp.FESpace(grmp.H1P1(1), g0)
+    A0 = grmp.FEMatrix([fes0])
+    form0, keep0 = op.form(fes0, 0, 0, g0.ctx)
+    A0.add_form(form0, keep0)
+    A0.build_pattern()
+    A0.assemble()
+    ref0 = O.assemble_matrix(fes0.ndofs, [dict(kind=O.BLF_LAPLACE, geom=3, fe=O.FE_P1, quadorder=0, factor=1.0,
+                                              coef=O.COEF_CELL, kappa=kap.download(), coords=c0, cells=cn0,
+                                              celldofs=cn0)])
+    assert _check_csc(A0, ref0) == 1.0
+    assert np.array_equal(A0.csc()[2].view(np.uint64), ref0.nzval.view(np.uint64))
     # LinearForm(Identity, f = 1) on the tetrahedra, bitwise
     b = grmp.FEVector([fes])
     grmp.assemble_linearform(b, 0, grmp.LinearForm(grmp.Identity, grmp.DataFunction([1.0])))
