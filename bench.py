@@ -778,7 +778,7 @@ def main():
             "avg_launch_us": round(avg_launch * 1e6, 2), "launches": int(st["spmv_launches"]),
             "bytes_per_launch": round(st["spmv_bytes"] / max(st["spmv_launches"], 1), 1)}
     if mode == "served":
-        roof["traffic_over_algorithmic"] = 1.20  # FETCH_SIZE + WRITE_SIZE passes, profiles/engine_spmv_hbm_traffic.md
+        roof["traffic_over_algorithmic"] = 1.19  # FETCH_SIZE + WRITE_SIZE passes, profiles/engine_spmv_hbm_traffic.md
     elif batched:
         # FETCH_SIZE + WRITE_SIZE passes on the same kernel (profiles/mgb_resid_hbm_traffic.md): HBM bytes
         # are 1.08 x the algorithmic bytes; the measured ratio scales this run's bytes per launch
