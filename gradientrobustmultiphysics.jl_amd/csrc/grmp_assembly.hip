@@ -645,7 +645,8 @@ struct Kind {
 // kinds whose rows depend on the cell only through its corners and
 // coefficient: row() gathers them, rowg() computes from a gathered cell.
 // RNB: incidences whose gathers the serial row walk keeps in flight together
-// (3 measured fastest for rows of 6 to 24 cells, 2 for P2's rows of 1 to 6).
+// (measured fastest: 4 for tetrahedra's rows of about 24 cells, 3 for
+// triangles' rows of 6, 2 for P2's rows of 1 to 6).
 template <int CN_, class GEO, int NV_, void (*ROWG)(const DevForm &, const GEO &, int, double *), int RNB = 3>
 struct KindG {
     static constexpr int CN = CN_;
@@ -666,7 +667,7 @@ struct KindG {
 using K_LAP_P1T1 = KindG<3, TriGeo, 3, row_laplace_tri_g<1, 1>>;
 using K_LAP_P1T3 = KindG<3, TriGeo, 3, row_laplace_tri_g<1, 3>>;
 using K_LAP_P2T3 = KindG<6, TriGeo, 3, row_laplace_tri_g<2, 3>, 2>;
-using K_LAP_P1TET = KindG<4, TetGeo, 4, row_laplace_tet_g>;
+using K_LAP_P1TET = KindG<4, TetGeo, 4, row_laplace_tet_g, 4>;
 using K_HOOKE1 = KindG<6, TriGeo, 3, row_hooke_tri_g<1>>;
 using K_HOOKE3 = KindG<6, TriGeo, 3, row_hooke_tri_g<3>>;
 using K_MASS_P1T1 = KindG<3, TriGeo, 3, row_mass_tri_g<1, 1>>;
@@ -1174,7 +1175,7 @@ static size_t asmg_lds(int rb, int mr)
 // One form of a known kind, one thread per row walking its cells serially:
 // the better choice when rows have few incidences (triangles)
 template <class K>
-__global__ void k_assemble1s(DevForm F, int remap, int64_t g0, int64_t g1, int64_t rblock,
+__global__ __launch_bounds__(256) void k_assemble1s(DevForm F, int remap, int64_t g0, int64_t g1, int64_t rblock,
                              const int32_t *__restrict__ rowptr, const uint8_t *__restrict__ diagpos,
                              const uint8_t *__restrict__ fixed, double penalty, double *__restrict__ val64,
                              float *__restrict__ val32, double *__restrict__ dinv64, float *__restrict__ dinv32)
