@@ -1068,6 +1068,10 @@ extern "C" int grmp_server_create(grmp_ctx *ctx, const grmp_server_config *cfg, 
         if (cfg->mg_coarse_rows > 0) S->cfg.mg_coarse_rows = cfg->mg_coarse_rows;
         if (cfg->mg_batch >= 0) S->cfg.mg_batch = std::min(cfg->mg_batch, 4096);
     }
+    // lockstep batches: two setup threads by default.  On the 4096-request
+    // stream they served 92.9M DoF-solves/s against 91.8M with one and 85.7M
+    // with four, whose HIP calls and kernels contend with the batch's.
+    if (S->cfg.mg_batch > 0 && (!cfg || cfg->mg_threads == 0)) S->cfg.mg_threads = 2;
     if (S->cfg.mode < 1 || S->cfg.mode > 3) {
         const int mode = S->cfg.mode;
         delete S;

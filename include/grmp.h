@@ -480,8 +480,8 @@ typedef struct grmp_server_config {
     int fp64_first_sweep;     /* > 0: a request whose first FP32 sweep takes more iterations
                                  continues in FP64 (0 = only on stalls) */
     int mg_slots;             /* mode 3: requests solved concurrently, one HIP stream each (0 = 8) */
-    int mg_threads;           /* mode 3: host threads launching for those slots, or with mg_batch the
-                                 threads that set requests up (0 = 4; -1 with mg_batch: the batch
+    int mg_threads;           /* mode 3: host threads launching for those slots (0 = 4), or with
+                                 mg_batch the threads that set requests up (0 = 2; -1: the batch
                                  thread sets them up itself, one HIP-calling thread) */
     int mg_block_rows;        /* mode 3: a request with at most this many rows is solved inside one
                                  workgroup (0 = 2048) */
