@@ -2062,7 +2062,9 @@ extern "C" int grmp_mg_solve(grmp_ctx *ctx, int nlev, const grmp_matrix *const *
 // round's state only, the sorts are stable and the sums run in a fixed order.
 #define AMG_MAX_LEVELS 24
 #define AMG_THETA 0.0
-#define AMG_SCALE 1.5
+// 1.8 measured best over 1.3-1.9: 48 / 66 / 35 iterations on the 2048^2 P1,
+// 2048^2 P2 and 160^3 tet Poisson systems, against 70 / 101 / 36 with 1.5
+#define AMG_SCALE 1.8
 
 struct AmgLevel {
     int64_t n = 0, nnz = 0;
