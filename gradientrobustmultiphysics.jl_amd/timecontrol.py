@@ -27,8 +27,11 @@ Each step's system is solved by the served path's multigrid V-cycle CG
 (FP64, grmp_mg_create / grmp_mg_apply) when the problem is one scalar H1P1
 unknown on a generated unit square with Dirichlet data (inhomogeneous values
 lifted): the coarse levels
-rediscretize A + M/dt on squares of ceil(N/2) down to N <= 4.  Otherwise,
-or with solver="krylov", the device Krylov solver of solve_system runs.
+rediscretize A + M/dt on squares of ceil(N/2) down to N <= 4.  Other
+symmetric positive definite systems (uploaded meshes; no Lagrange
+multipliers) are solved by the algebraic multigrid CG (grmp_amg_create once
+per step size, grmp_amg_apply per step).  Otherwise, or with
+solver="krylov", the device Krylov solver of solve_system runs.
 
 Dirichlet data (homogeneous, interpolated or best-approximated) is fixed in
 time.  Nonlinear operators, dt_action / dt_lump and time-dependent data are
@@ -41,8 +44,9 @@ import numpy as np
 from . import _abi
 from ._abi import lib
 from .fe import FEMatrix, FEVector
-from .pde import (ReactionOperator, SolverConfig, _saddle_point, assemble_linearform, boundary_values, fixed_dofs,
-                  MultigridSolver, lift_dirichlet, mg_eligible, mg_hierarchy, realize_constraints, solve_system)
+from .pde import (AlgebraicMultigridSolver, ReactionOperator, SolverConfig, _saddle_point, amg_eligible,
+                  assemble_linearform, boundary_values, fixed_dofs, MultigridSolver, lift_dirichlet, mg_eligible,
+                  mg_hierarchy, realize_constraints, solve_system)
 
 
 class BackwardEuler:
@@ -118,16 +122,27 @@ class TimeControlSolver:
             for _ in range(2):
                 self.f2.ctx.check(lib().grmp_vector_axpy(1.0, self.f.vec.h, self.f2.vec.h))
             self._tmp = FEVector(FES, name="2 f - A u")
-        # step solver: the multigrid V-cycle CG where the served hierarchy applies, else the Krylov solver
-        if solver not in ("auto", "krylov", "mg"):
-            raise _abi.GrmpError(_abi.ERR_ARG, f"solver {solver!r}: 'auto', 'krylov' or 'mg'")
+        # step solver: the multigrid V-cycle CG where the served hierarchy applies, else the algebraic
+        # multigrid CG for symmetric positive definite systems, else the Krylov solver
+        if solver not in ("auto", "krylov", "mg", "amg"):
+            raise _abi.GrmpError(_abi.ERR_ARG, f"solver {solver!r}: 'auto', 'krylov', 'mg' or 'amg'")
         mg_ok = self._mg_eligible()
+        amg_ok = amg_eligible(pde, InitialValues) and not pde.GlobalConstraints
         if solver == "mg" and not mg_ok:
             raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "multigrid steps need one scalar H1P1 unknown on a generated "
                                  "unit square with Dirichlet data on its whole boundary (inhomogeneous values are "
                                  "lifted) and no global constraints")
-        self.solver = "mg" if (solver == "mg" or (solver == "auto" and mg_ok)) else "krylov"
+        if solver == "amg" and not amg_ok:
+            raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "algebraic multigrid steps need a symmetric positive definite "
+                                 "system: no Lagrange multipliers and no global constraints")
+        if solver == "mg" or (solver == "auto" and mg_ok):
+            self.solver = "mg"
+        elif solver == "amg" or (solver == "auto" and amg_ok):
+            self.solver = "amg"
+        else:
+            self.solver = "krylov"
         self.levels = None
+        self.amg = None
         self.S = None
         self.cstep, self.ctime, self.last_timestep = 0, 0.0, None
         self.assemblies = 0
@@ -195,15 +210,20 @@ class TimeControlSolver:
         self.cstep += 1
         self.ctime += timestep
         if self.S is None or timestep != self.last_timestep:
+            self.amg = None
             self.S = self.levels = None
             self.S = self._system(timestep)
             if self.solver == "mg":  # assembled once per step size, reused by every step
                 self.levels = MultigridSolver(*mg_hierarchy(self.PDE, self.X.FES[0], self.S,
                                                             [ReactionOperator(self._mfactor(timestep))],
                                                             self.SC.fixed_penalty))
+            elif self.solver == "amg":  # built once per step size from S's FP64 values
+                self.amg = AlgebraicMultigridSolver(self.S)
         self.rhs(timestep)
         if self.solver == "mg":
             info = self._mg_solve()
+        elif self.solver == "amg":
+            info = self.amg.solve(self.b, self.X, self.SC.rtol, min(self.SC.maxiterations, 2000))
         else:
             info = solve_system(self.S, self.b, self.X, self.SC, self.method)
         realize_constraints(self.PDE, self.X)

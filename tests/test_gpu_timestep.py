@@ -119,6 +119,8 @@ def test_uploaded_mesh_until_stationarity(grmp, O):
     fes, pde = _heat(grmp, grid, diffusion=1.3)
     X = grmp.FEVector([fes])
     tcs = grmp.TimeControlSolver(pde, X, grmp.BackwardEuler, timedependent_equations=[0], rtol=1e-13)
+    # an uploaded mesh: the default step solver is the algebraic multigrid CG
+    assert tcs.solver == "amg"
     nsteps = tcs.advance_until_stationarity(0.5, stationarity_threshold=1e-9, maxtimesteps=200)
     assert nsteps < 200 and tcs.last_change < 1e-9
     A, M, f, fixed = _oracle(O, coords, cells, diffusion=1.3)
@@ -188,7 +190,7 @@ def test_argument_errors(grmp):
         X.ctx.check(grmp.lib().grmp_timestep_rhs(tcs.S.h, tcs.AM.h, tcs.f.vec.h, other.vec.h, 10.0, other.vec.h))
 
 
-@pytest.mark.parametrize("solver", ["krylov", "mg"])
+@pytest.mark.parametrize("solver", ["krylov", "mg", "amg"])
 def test_crank_nicolson_matches_oracle(grmp, O, solver):
     N = 24
     print(f"[case] crank nicolson steps N={N} {solver}", flush=True)
