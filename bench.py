@@ -604,7 +604,8 @@ def run_workload(grmp, which, args, log):
         out["nonlinear"] = {"status": info.status, "newton_steps": info.iterations,
                             "residuals": [float("%.3e" % r) for r in info.residuals],
                             "bicgstab_iterations": info.linear_iterations,
-                            "bicgstab_restarts": info.linear_restarts, "s": round(t, 3),
+                            "bicgstab_restarts": info.linear_restarts,
+                            "bicgstab_preconditioner": info.preconditioner, "s": round(t, 3),
                             "newton_matrix_ms": round(min(ts) * 1e3, 3),
                             "newton_matrix_elements_per_s": round(grid.ncells / min(ts), 1),
                             "l2_error_vs_xy": err}

@@ -164,6 +164,7 @@ SIGNATURES = {
     "grmp_amg_info": (ctypes.c_int, [VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(I64),
                                      ctypes.POINTER(ctypes.c_double)]),
     "grmp_amg_destroy": (ctypes.c_int, [VP]),
+    "grmp_solve_bicgstab_amg": (ctypes.c_int, [VP, VP, VP, VP, VP, VP]),
     "grmp_matrix_tune_key": (ctypes.c_int, [VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                             ctypes.POINTER(ctypes.c_int)]),
     "grmp_tune_load": (ctypes.c_int, [VP, ctypes.c_char_p]),

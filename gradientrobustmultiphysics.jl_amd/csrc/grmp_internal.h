@@ -282,6 +282,10 @@ int mg_jobs_setup_batched(MgJob *const *jobs, int n, hipStream_t st, double *con
 // one V-cycle of a created hierarchy, z = V v on the hierarchy's stream (the
 // velocity block of the Stokes preconditioner); level-0 rows of the hierarchy
 int mg_precond_apply(grmp_mg *m, const double *v, double *z, hipStream_t st);
+// one V-cycle of an algebraic multigrid hierarchy as a preconditioner
+int amg_precond_apply(grmp_amg *m, const double *v, double *z, hipStream_t st);
+int64_t amg_level0_rows(const grmp_amg *m);
+grmp_ctx *amg_context(const grmp_amg *m);
 // two components at once: a second copy of the hierarchy on its own stream
 int mg_pair_prepare(grmp_mg *m);
 int mg_precond_apply_pair(grmp_mg *m, const double *v0, double *z0, const double *v1, double *z1, hipStream_t st);

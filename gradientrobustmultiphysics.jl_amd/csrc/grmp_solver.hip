@@ -518,7 +518,10 @@ __global__ void k_bic_p(const double *__restrict__ r, const double *__restrict__
         p[i] = r[i] + beta * (p[i] - omega * v[i]);
 }
 
-int bicgstab_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solve_opts &o, grmp_solve_info &info)
+// pc: the right preconditioner (one algebraic multigrid V-cycle) in place of
+// the Jacobi scaling, or nullptr
+int bicgstab_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solve_opts &o, grmp_solve_info &info,
+                   grmp_amg *pc = nullptr)
 {
     grmp_ctx *ctx = A->ctx;
     const int64_t n = A->n;
@@ -584,7 +587,8 @@ int bicgstab_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solv
             continue;
         }
         double rv, vv, ss, dummy, ts, tt;
-        hipLaunchKernelGGL(k_bic_scale, dim3(gride), dim3(PB), 0, ctx->stream, A->dinv64, p, ph, n);
+        if (pc) GRMP_TRY(amg_precond_apply(pc, p, ph, ctx->stream));
+        else hipLaunchKernelGGL(k_bic_scale, dim3(gride), dim3(PB), 0, ctx->stream, A->dinv64, p, ph, n);
         hipLaunchKernelGGL(k_bic_spmv, dim3(grid), dim3(PB), 0, ctx->stream, A->rowptr, A->cols, A->val64, A->fixed, ph,
                            rh, v, n, part);
         GRMP_TRY(plain_sum(ctx, part, fold, B.hp, grid, rv, vv));
@@ -607,7 +611,8 @@ int bicgstab_solve(grmp_matrix *A, const double *bd, double *xd, const grmp_solv
             GRMP_TRY(plain_sum(ctx, part, fold, B.hp, gride, dummy, rr));
             break;
         }
-        hipLaunchKernelGGL(k_bic_scale, dim3(gride), dim3(PB), 0, ctx->stream, A->dinv64, s, sh, n);
+        if (pc) GRMP_TRY(amg_precond_apply(pc, s, sh, ctx->stream));
+        else hipLaunchKernelGGL(k_bic_scale, dim3(gride), dim3(PB), 0, ctx->stream, A->dinv64, s, sh, n);
         hipLaunchKernelGGL(k_bic_spmv, dim3(grid), dim3(PB), 0, ctx->stream, A->rowptr, A->cols, A->val64, A->fixed, sh,
                            s, t, n, part);
         GRMP_TRY(plain_sum(ctx, part, fold, B.hp, grid, ts, tt));
@@ -1432,6 +1437,40 @@ extern "C" int grmp_solve_minres_mg(grmp_matrix *A, const grmp_vector *b, grmp_v
                          inf.relres, (long long)inf.iterations);
     if (inf.status == GRMP_ERR_NONFINITE)
         return grmp_fail(ctx, GRMP_ERR_NONFINITE, "grmp_solve_minres_mg: non-finite values in the Krylov loop");
+    return inf.status;
+}
+
+// BiCGStab right-preconditioned by one algebraic multigrid V-cycle per
+// application (the Newton systems of nonlinear problems)
+extern "C" int grmp_solve_bicgstab_amg(grmp_matrix *A, const grmp_vector *b, grmp_vector *x,
+                                       const grmp_solve_opts *opts, grmp_amg *pc, grmp_solve_info *info)
+{
+    if (!A || !b || !x || !pc) return GRMP_ERR_ARG;
+    grmp_ctx *ctx = A->ctx;
+    grmp_solve_opts o;
+    grmp_solve_opts_default(&o);
+    if (opts) o = *opts;
+    o.method = GRMP_SOLVE_BICGSTAB;
+    grmp_solve_info inf;
+    memset(&inf, 0, sizeof(inf));
+    GRMP_CHECK(ctx, A->have_pattern && A->val64, GRMP_ERR_ARG, "grmp_solve_bicgstab_amg: matrix not assembled in FP64");
+    GRMP_CHECK(ctx, b->n == A->n && x->n == A->n, GRMP_ERR_ARG, "grmp_solve_bicgstab_amg: vector length != %lld",
+               (long long)A->n);
+    GRMP_CHECK(ctx, amg_context(pc) == ctx && amg_level0_rows(pc) == A->n, GRMP_ERR_ARG,
+               "grmp_solve_bicgstab_amg: the hierarchy belongs to another context or has %lld rows, not %lld",
+               (long long)amg_level0_rows(pc), (long long)A->n);
+    GRMP_CHECK(ctx, o.precision == GRMP_PRECISION_FP64, GRMP_ERR_UNSUPPORTED, "grmp_solve_bicgstab_amg: FP64 only");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    const double t0 = wall_seconds();
+    const int rc = bicgstab_solve(A, b->d, x->d, o, inf, pc);
+    inf.seconds = wall_seconds() - t0;
+    if (info) *info = inf;
+    if (rc) return rc;
+    if (inf.status == GRMP_ERR_NOCONV)
+        return grmp_fail(ctx, GRMP_ERR_NOCONV, "grmp_solve_bicgstab_amg: no convergence, relres %.3e after %lld iterations",
+                         inf.relres, (long long)inf.iterations);
+    if (inf.status == GRMP_ERR_NONFINITE)
+        return grmp_fail(ctx, GRMP_ERR_NONFINITE, "grmp_solve_bicgstab_amg: non-finite values in the Krylov loop");
     return inf.status;
 }
 

@@ -455,13 +455,19 @@ class SolverConfig:
 
     def __init__(self, precision="fp64", rtol=1e-10, maxiterations=1000000, inner_rtol=0.1,
                  fixed_penalty=1e60, minres_inner_rtol=1e-4, target_residual=1e-12, nonlinear_maxiterations=10,
-                 damping=0.0):
+                 damping=0.0, newton_preconditioner="auto"):
         self.precision, self.rtol, self.maxiterations = precision, rtol, maxiterations
         self.inner_rtol, self.fixed_penalty = inner_rtol, fixed_penalty
         self.minres_inner_rtol = minres_inner_rtol
         # fixpoint iterations of nonlinear problems (default_solver_kwargs, src/solvers.jl:37-59)
         self.target_residual, self.nonlinear_maxiterations = target_residual, nonlinear_maxiterations
         self.damping = damping
+        # the Newton steps' BiCGStab preconditioner: "amg" (one algebraic multigrid V-cycle, hierarchy built
+        # from each Newton matrix), "jacobi", or "auto" (amg, jacobi where the hierarchy cannot be built)
+        if newton_preconditioner not in ("auto", "amg", "jacobi"):
+            raise _abi.GrmpError(_abi.ERR_ARG, f"newton_preconditioner {newton_preconditioner!r}: "
+                                 "'auto', 'amg' or 'jacobi'")
+        self.newton_preconditioner = newton_preconditioner
 
 
 def fixed_dofs(pde, Solution):
@@ -784,6 +790,23 @@ def solve(Solution, pde, config=None, solver="krylov", **kwargs):
     return info, A, b
 
 
+def _newton_solve_amg(A, b, Solution, cfg, pc):
+    """grmp_solve_bicgstab_amg from the current iterate: BiCGStab with one
+    algebraic multigrid V-cycle of pc as the right preconditioner."""
+    o = _abi.SolveOpts()
+    lib().grmp_solve_opts_default(ctypes.byref(o))
+    o.method = _abi.SOLVE_BICGSTAB
+    o.precision = _abi.PRECISION_FP64
+    o.rtol = cfg.rtol
+    o.maxit = int(cfg.maxiterations)
+    o.initial_guess = 1
+    info = _abi.SolveInfo()
+    rc = lib().grmp_solve_bicgstab_amg(A.h, b.vec.h, Solution.vec.h, ctypes.byref(o), pc.h, ctypes.byref(info))
+    if rc not in (_abi.OK,):
+        A.ctx.check(rc)
+    return info
+
+
 def solve_system(A, b, Solution, cfg, method=_abi.SOLVE_CG, initial_guess=False, velocity_mg=None):
     o = _abi.SolveOpts()
     lib().grmp_solve_opts_default(ctypes.byref(o))
@@ -818,6 +841,7 @@ class FixpointInfo:
     def __init__(self):
         self.status, self.iterations, self.residuals, self.linear_iterations = 0, 0, [], []
         self.linear_restarts = []  # BiCGStab restarts after near-breakdowns, per Newton step
+        self.preconditioner = None  # "amg" or "jacobi": the Newton steps' BiCGStab preconditioner
 
     @property
     def relres(self):
@@ -871,6 +895,8 @@ def solve_fixpoint(Solution, pde, cfg):
         x[fixed] = 0.0 if target is None else target.download()[fixed]
         Solution.set_entries(x)
     info = FixpointInfo()
+    use_amg = cfg.newton_preconditioner in ("auto", "amg")
+    pc = None
     last = Solution.entries() if cfg.damping > 0 else None
     res = FEVector(Solution.FES)
     mask = np.zeros(Solution.n, dtype=bool)
@@ -878,8 +904,24 @@ def solve_fixpoint(Solution, pde, cfg):
     for it in range(1, cfg.nonlinear_maxiterations + 1):
         if fixed.size:  # b[fixed] = penalty * Target[fixed]
             A.ctx.check(lib().grmp_matrix_penalize_rhs(A.h, Solution.vec.h, b.vec.h))
-        # BiCGStab from the current iterate (the Newton step is the full system for the new iterate)
-        li = solve_system(A, b, Solution, cfg, _abi.SOLVE_BICGSTAB, initial_guess=True)
+        # BiCGStab from the current iterate (the Newton step is the full system for the new iterate),
+        # preconditioned by the multigrid hierarchy of the first Newton matrix.  Its V-cycle uses the
+        # values copied when it was built; later Newton matrices differ by a bounded coefficient
+        # (1 + u^2) and a lower-order nonsymmetric term, so it stays a good preconditioner, while
+        # hierarchies rebuilt from them degrade: their coarse Galerkin levels turn convection-dominated.
+        li = None
+        if use_amg and pc is None:
+            try:
+                pc = AlgebraicMultigridSolver(A)
+            except _abi.GrmpError as e:
+                if cfg.newton_preconditioner == "amg" or e.code != _abi.ERR_UNSUPPORTED:
+                    raise
+                use_amg = False
+        if pc is not None:
+            li = _newton_solve_amg(A, b, Solution, cfg, pc)
+        if li is None:
+            li = solve_system(A, b, Solution, cfg, _abi.SOLVE_BICGSTAB, initial_guess=True)
+        info.preconditioner = "amg" if use_amg else "jacobi"
         info.linear_iterations.append(int(li.iterations))
         info.linear_restarts.append(int(li.outer) - 1)
         if cfg.damping > 0:
@@ -899,4 +941,5 @@ def solve_fixpoint(Solution, pde, cfg):
             break
     else:
         info.status = _abi.ERR_NOCONV
+    del pc
     return info, A, b

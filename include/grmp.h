@@ -419,9 +419,10 @@ int grmp_solve_minres_mg(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, c
  * Chebyshev smoothing with FP32-stored values, the dense inverse on the
  * coarsest level.  Fixed (penalized) dofs are eliminated (x_d = b_d / A_dd),
  * so b is the penalized right-hand side of grmp_solve.  The handle reads A's
- * pattern and values: keep A alive and unchanged, create again after it is
- * reassembled.  No counterpart in the reference, which factorizes
- * (src/solvers.jl:55, 656-660). */
+ * pattern and values: keep A alive, and for grmp_amg_apply unchanged (create
+ * again after it is reassembled; grmp_solve_bicgstab_amg below also takes a
+ * hierarchy of an earlier assembly of A's pattern).  No counterpart in the
+ * reference, which factorizes (src/solvers.jl:55, 656-660). */
 typedef struct grmp_amg grmp_amg;
 int grmp_amg_create(grmp_matrix *A, grmp_amg **out);
 int grmp_amg_apply(grmp_amg *m, const grmp_vector *b, grmp_vector *x, double rtol, int64_t maxit,
@@ -429,6 +430,18 @@ int grmp_amg_apply(grmp_amg *m, const grmp_vector *b, grmp_vector *x, double rto
 /* levels, rows of the coarsest level, operator complexity (summed nonzeros / level-0 nonzeros) */
 int grmp_amg_info(const grmp_amg *m, int *nlev, int64_t *coarse_rows, double *complexity);
 int grmp_amg_destroy(grmp_amg *m);
+
+/* The linear solve of each Newton step of solve_fixpoint_full!
+ * (src/solvers.jl:906-911, the cache's solve with UMFPACK) by BiCGStab,
+ * right-preconditioned by one V-cycle of pc per application instead of the
+ * Jacobi scaling of grmp_solve.  pc comes from grmp_amg_create on a matrix
+ * with A's pattern, e.g. the first Newton matrix: its V-cycle uses the
+ * values copied when it was built, so A may since have been reassembled in
+ * place (the later Newton matrices of a diffusion-dominated problem).  FP64;
+ * opts->initial_guess starts from x (the current iterate).  The other
+ * fields and the return codes are grmp_solve's. */
+int grmp_solve_bicgstab_amg(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, const grmp_solve_opts *opts,
+                            grmp_amg *pc, grmp_solve_info *info);
 
 /* Solver plug-in: solves LinearProblem(A.entries.cscmatrix, b.entries) as the
  * linsolver of solve! (src/solvers.jl:55, 89-101, 656-660).  colptr[n+1],

@@ -122,3 +122,25 @@ def test_amg_auto_on_uploaded_mesh(grmp, O):
     assert info.status == 0 and info.iterations < 60
     assert not grmp.amg_eligible(grmp.IncompressibleNavierStokesProblem(2, 1e-2),
                                  grmp.FEVector([grmp.FESpace(grmp.H1BR(2), grid), grmp.FESpace(grmp.L2P0(1), grid)]))
+
+
+@pytest.mark.parametrize("fe", [1, 2])
+def test_amg_equal_strengths(grmp, O, fe):
+    """A constant coefficient on generated squares: every pair of a row has the
+    same strength, so the matching ranks ties by its symmetric pair key (ranking
+    them by index stalled the coarsening of P1 and P2 at 256 x 256)."""
+    print(f"[case] amg with equal strengths, fe {fe}, generated 256x256", flush=True)
+    grid, fes, pde, kap = _poisson(grmp, O, fe, 256, seed=1, sigma=0.0, uploaded=False)
+    sol = grmp.FEVector([fes])
+    A, b, fixed = grmp.assemble(pde, sol)
+    m = grmp.AlgebraicMultigridSolver(A)
+    nl, coarse, cx = m.info()
+    xa, xj = grmp.FEVector([fes]), grmp.FEVector([fes])
+    ia = m.solve(b, xa, 1e-12)
+    ij = grmp.solve_system(A, b, xj, grmp.SolverConfig(rtol=1e-12, maxiterations=20000))
+    u, v = xa.entries(), xj.entries()
+    err = np.abs(u - v).max() / np.abs(v).max()
+    print(f"  {nl} levels, coarsest {coarse} rows, complexity {cx:.3f}; amg {ia.iterations} iterations, "
+          f"Jacobi CG {ij.iterations}; max rel diff {err:.2e}")
+    assert ia.status == 0 and ij.status == 0
+    assert nl >= 4 and cx < 1.6 and ia.iterations < 80 and err < 1e-9

@@ -99,12 +99,15 @@ def test_newton_p2_reproduces_xy(grmp, O):
     assert info.status == 0 and err < 1e-10
 
 
-def test_newton_p1_matches_oracle(grmp, O):
-    print("[case] newton P1 vs oracle", flush=True)
+@pytest.mark.parametrize("pc", ["amg", "jacobi"])
+def test_newton_p1_matches_oracle(grmp, O, pc):
+    print(f"[case] newton P1 vs oracle, {pc}", flush=True)
     grid, fes, coords, cells, pts = _setup(grmp, O, 12, 1, seed=9)
     pde, nlf = _pde(grmp)
     sol = grmp.FEVector([fes])
-    info, A, b = grmp.solve(sol, pde, target_residual=1e-11, nonlinear_maxiterations=12, rtol=1e-13)
+    info, A, b = grmp.solve(sol, pde, target_residual=1e-11, nonlinear_maxiterations=12, rtol=1e-13,
+                            newton_preconditioner=pc)
+    assert info.preconditioner == pc
     cd = fes.celldofs()
     fixed = fes.boundary_dofs()
     xo, res = O.newton_solve(1, coords, cells, cd, fes.ndofs, fixed, pts[:, 0] * pts[:, 1], C, F, quadorder=3,
@@ -161,16 +164,23 @@ def test_bicgstab_initial_guess(grmp, O):
             grmp._abi.ERR_UNSUPPORTED
 
 
-def test_newton_p2_large(grmp):
-    """the fixpoint loop at 148,225 P2 dofs, where BiCGStab's recurrence
-    meets near-breakdowns and restarts: ||u_h - x y||_L2 below 1e-9"""
-    print("[case] newton P2 192x192", flush=True)
+@pytest.mark.parametrize("pc", ["jacobi", "amg"])
+def test_newton_p2_large(grmp, pc):
+    """the fixpoint loop at 148,225 P2 dofs: with Jacobi, BiCGStab's
+    recurrence meets near-breakdowns and restarts; with the algebraic
+    multigrid V-cycle it needs far fewer iterations.  ||u_h - x y||_L2 below
+    1e-9 either way"""
+    print(f"[case] newton P2 192x192, {pc}", flush=True)
     N = 192
     grid = grmp.grid_unitsquare(N)
     fes = grmp.FESpace(grmp.H1P2(1), grid)
     pde, nlf = _pde(grmp)
     sol = grmp.FEVector([fes])
-    info, A, b = grmp.solve(sol, pde, target_residual=1e-10, nonlinear_maxiterations=12, rtol=1e-12)
+    info, A, b = grmp.solve(sol, pde, target_residual=1e-10, nonlinear_maxiterations=12, rtol=1e-12,
+                            newton_preconditioner=pc)
+    assert info.preconditioner == pc
+    if pc == "amg":
+        assert max(info.linear_iterations) < 200, info.linear_iterations
     err = float(grmp.evaluate(grmp.L2ErrorIntegrator(grmp.PolynomialData([0, 0, 0, 0, 1.0]), quadorder=2),
                               sol[0])) ** 0.5
     print(f"  {info.iterations} Newton steps {['%.1e' % r for r in info.residuals]}, BiCGStab "
