@@ -21,7 +21,7 @@ from .pde import (APT_BilinearForm, APT_SymmetricBilinearForm, BestapproxDirichl
                   PoissonProblem, ReactionOperator, ReconstructionIdentity, SolverConfig, SymmetricGradient, assemble,
                   assemble_linearform, MultigridSolver, mg_eligible, mg_hierarchy, mg_solve, realize_constraints,
                   solve, solve_system, stokes_mg_eligible, stokes_velocity_mg, AlgebraicMultigridSolver,
-                  amg_eligible)
+                  AndersonAccelerator, amg_eligible)
 
 from .dataexport import nodevalues, writeCSV  # noqa: F401
 from .timecontrol import BackwardEuler, CrankNicolson, TimeControlSolver  # noqa: F401

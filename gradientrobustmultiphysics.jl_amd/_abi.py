@@ -165,6 +165,11 @@ SIGNATURES = {
                                      ctypes.POINTER(ctypes.c_double)]),
     "grmp_amg_destroy": (ctypes.c_int, [VP]),
     "grmp_solve_bicgstab_amg": (ctypes.c_int, [VP, VP, VP, VP, VP, VP]),
+    "grmp_anderson_create": (ctypes.c_int, [VP, ctypes.c_int64, ctypes.c_int, ctypes.c_double, VP, ctypes.c_int64,
+                                            VP, VP, VP]),
+    "grmp_anderson_update": (ctypes.c_int, [VP, VP, VP, VP]),
+    "grmp_anderson_depth": (ctypes.c_int, [VP]),
+    "grmp_anderson_destroy": (ctypes.c_int, [VP]),
     "grmp_matrix_tune_key": (ctypes.c_int, [VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                             ctypes.POINTER(ctypes.c_int)]),
     "grmp_tune_load": (ctypes.c_int, [VP, ctypes.c_char_p]),

@@ -12,6 +12,7 @@ action's bonus order.
 """
 import copy
 import ctypes
+import warnings
 
 import numpy as np
 
@@ -455,7 +456,8 @@ class SolverConfig:
 
     def __init__(self, precision="fp64", rtol=1e-10, maxiterations=1000000, inner_rtol=0.1,
                  fixed_penalty=1e60, minres_inner_rtol=1e-4, target_residual=1e-12, nonlinear_maxiterations=10,
-                 damping=0.0, newton_preconditioner="auto"):
+                 damping=0.0, newton_preconditioner="auto", anderson_iterations=0, anderson_metric="l2",
+                 anderson_damping=1.0, anderson_unknowns=(0,)):
         self.precision, self.rtol, self.maxiterations = precision, rtol, maxiterations
         self.inner_rtol, self.fixed_penalty = inner_rtol, fixed_penalty
         self.minres_inner_rtol = minres_inner_rtol
@@ -463,11 +465,20 @@ class SolverConfig:
         self.target_residual, self.nonlinear_maxiterations = target_residual, nonlinear_maxiterations
         self.damping = damping
         # the Newton steps' BiCGStab preconditioner: "amg" (one algebraic multigrid V-cycle, hierarchy built
-        # from each Newton matrix), "jacobi", or "auto" (amg, jacobi where the hierarchy cannot be built)
+        # from the first Newton matrix), "jacobi", or "auto" (amg, jacobi where the hierarchy cannot be built)
         if newton_preconditioner not in ("auto", "amg", "jacobi"):
             raise _abi.GrmpError(_abi.ERR_ARG, f"newton_preconditioner {newton_preconditioner!r}: "
                                  "'auto', 'amg' or 'jacobi'")
         self.newton_preconditioner = newton_preconditioner
+        # Anderson acceleration of the fixpoint loop (src/solvers.jl:49-52); unknowns are 0-based here
+        if anderson_metric not in ("l2", "L2", "H1"):
+            raise _abi.GrmpError(_abi.ERR_ARG, f"anderson_metric {anderson_metric!r}: 'l2', 'L2' or 'H1'")
+        if not (0 <= int(anderson_iterations) <= 10) or not (0.0 < anderson_damping <= 1.0):
+            raise _abi.GrmpError(_abi.ERR_ARG, f"anderson_iterations {anderson_iterations} not in 0..10 or "
+                                 f"anderson_damping {anderson_damping} not in (0, 1]")
+        self.anderson_iterations, self.anderson_metric = int(anderson_iterations), anderson_metric
+        self.anderson_damping = float(anderson_damping)
+        self.anderson_unknowns = tuple(int(u) for u in anderson_unknowns)
 
 
 def fixed_dofs(pde, Solution):
@@ -866,15 +877,68 @@ def _nl_system(pde, Solution, A):
     return b
 
 
+class AndersonAccelerator:
+    """AndersonAccelerationManager (src/solvers.jl:688-746) on the device:
+    grmp_anderson_create with the norm matrix of the metric ("L2": mass,
+    "H1": gradient stiffness, assembled on the device for the accelerated
+    unknowns; "l2": none) and the entry ranges of those unknowns.  update()
+    is update_anderson! (:748-808) on the FEVector given at construction."""
+
+    def __init__(self, Solution, depth, metric="l2", damping=1.0, unknowns=(0,)):
+        self.ctx = Solution.vec.ctx
+        unknowns = sorted(set(int(u) for u in unknowns))
+        if not unknowns or unknowns[0] < 0 or unknowns[-1] >= len(Solution.FES):
+            raise _abi.GrmpError(_abi.ERR_ARG, f"anderson_unknowns {unknowns} outside 0..{len(Solution.FES) - 1}")
+        self.metric = None
+        if metric != "l2":
+            B = FEMatrix(Solution.FES)
+            op = ReactionOperator(1.0) if metric == "L2" else LaplaceOperator(1.0)
+            for u in unknowns:
+                form, keep = op.form(Solution.FES[u], u, u, B.ctx, Solution.FES[u])
+                B.add_form(form, keep)
+            B.build_pattern()
+            B.assemble(_abi.VALUES_FP64)
+            self.metric = B
+        r = []
+        for u in unknowns:
+            r += [int(Solution.offsets[u]), int(Solution.offsets[u]) + int(Solution.FES[u].ndofs)]
+        self.ranges = np.array(r, dtype=np.int64)
+        self.depth = int(depth)
+        self.last_alpha = np.zeros(self.depth + 1)
+        self.last_local_residual = 0.0
+        h = _abi.VP()
+        self.ctx.check(lib().grmp_anderson_create(
+            self.ctx.h, int(Solution.n), self.depth, float(damping), self.metric.h if self.metric else None,
+            len(unknowns), self.ranges.ctypes.data, Solution.vec.h, ctypes.byref(h)))
+        self.h = h
+
+    def update(self, Solution):
+        alpha = np.zeros(self.depth + 1)
+        res = ctypes.c_double()
+        self.ctx.check(lib().grmp_anderson_update(self.h, Solution.vec.h, alpha.ctypes.data, ctypes.byref(res)))
+        self.last_alpha, self.last_local_residual = alpha, res.value
+        if res.value > 1e-4:
+            warnings.warn(f"Anderson acceleration local system residual unsatisfactory ({res.value} > 1e-4)")
+        return alpha
+
+    def __del__(self):
+        try:
+            if self.h and _abi.alive():
+                lib().grmp_anderson_destroy(self.h)
+            self.h = None
+        except Exception:
+            pass
+
+
 def solve_fixpoint(Solution, pde, cfg):
     """solve_fixpoint_full! (src/solvers.jl:812-1017) for PDEs with
     NonlinearForms: assemble at the initial iterate, set the boundary data,
     then per iteration penalize the fixed dofs, solve the Newton system on
     the device (BiCGStab: the Newton matrix is nonsymmetric), optionally
-    damp, realize constraints, reassemble at the new iterate and stop when
-    the nonlinear residual ||A x - b|| (fixed rows zeroed) drops below
-    target_residual.  Anderson acceleration is not built.  Returns
-    (FixpointInfo, A, b)."""
+    accelerate (Anderson, grmp_anderson_update) and damp, realize
+    constraints, reassemble at the new iterate and stop when the nonlinear
+    residual ||A x - b|| (fixed rows zeroed) drops below target_residual.
+    Returns (FixpointInfo, A, b)."""
     if cfg.precision != "fp64":
         raise _abi.GrmpError(_abi.ERR_UNSUPPORTED, "nonlinear problems are solved in FP64")
     A = FEMatrix(Solution.FES)
@@ -895,6 +959,10 @@ def solve_fixpoint(Solution, pde, cfg):
         x[fixed] = 0.0 if target is None else target.download()[fixed]
         Solution.set_entries(x)
     info = FixpointInfo()
+    aa = None
+    if cfg.anderson_iterations > 0:  # after the boundary data, as the reference builds it (src/solvers.jl:849-859)
+        aa = AndersonAccelerator(Solution, cfg.anderson_iterations, cfg.anderson_metric, cfg.anderson_damping,
+                                 cfg.anderson_unknowns)
     use_amg = cfg.newton_preconditioner in ("auto", "amg")
     pc = None
     last = Solution.entries() if cfg.damping > 0 else None
@@ -924,6 +992,8 @@ def solve_fixpoint(Solution, pde, cfg):
         info.preconditioner = "amg" if use_amg else "jacobi"
         info.linear_iterations.append(int(li.iterations))
         info.linear_restarts.append(int(li.outer) - 1)
+        if aa is not None:  # POSTPROCESS: ANDERSON ITERATE, before the damping (src/solvers.jl:923-938)
+            aa.update(Solution)
         if cfg.damping > 0:
             x = cfg.damping * last + (1.0 - cfg.damping) * Solution.entries()
             Solution.set_entries(x)
@@ -941,5 +1011,5 @@ def solve_fixpoint(Solution, pde, cfg):
             break
     else:
         info.status = _abi.ERR_NOCONV
-    del pc
+    del pc, aa
     return info, A, b

@@ -443,6 +443,31 @@ int grmp_amg_destroy(grmp_amg *m);
 int grmp_solve_bicgstab_amg(grmp_matrix *A, const grmp_vector *b, grmp_vector *x, const grmp_solve_opts *opts,
                             grmp_amg *pc, grmp_solve_info *info);
 
+/* Anderson acceleration of the fixpoint loop: AndersonAccelerationManager
+ * (src/solvers.jl:688-746) and update_anderson! (:748-808), called after
+ * each linear solve of solve_fixpoint_full! (:923-929).
+ * grmp_anderson_create: depth = anderson_iterations (1..10), damping =
+ * anderson_damping in (0, 1], metric = the FP64-assembled norm matrix of the
+ * accelerated unknowns (mass matrix for "L2", gradient stiffness for "H1";
+ * NULL for "l2", the identity), ranges = nranges pairs [lo, hi) of entries
+ * of the accelerated unknowns (their blocks' offsets: anderson_unknowns),
+ * initial = the iterate the loop starts from (every stored iterate starts as
+ * it).  grmp_anderson_update takes the fixpoint output in target and
+ * overwrites its accelerated entries with the combination of stored
+ * iterates; from the third call on, alpha_out (depth + 1 doubles, may be
+ * NULL) gets the coefficients and *local_residual the residual of the small
+ * system (the reference warns above 1e-4); before that, zeros.
+ * GRMP_ERR_NONFINITE if the small system is singular or not finite.  The
+ * handle keeps 2 (depth + 1) iterates (plus depth + 1 work vectors, twice with
+ * a metric) of length n on the device and reads metric at every update. */
+typedef struct grmp_anderson grmp_anderson;
+int grmp_anderson_create(grmp_ctx *ctx, int64_t n, int depth, double damping, const grmp_matrix *metric,
+                         int64_t nranges, const int64_t *ranges, const grmp_vector *initial, grmp_anderson **out);
+int grmp_anderson_update(grmp_anderson *aa, grmp_vector *target, double *alpha_out, double *local_residual);
+/* updates made so far minus one (AAM.cdepth of the reference: -1 after create) */
+int grmp_anderson_depth(const grmp_anderson *aa);
+int grmp_anderson_destroy(grmp_anderson *aa);
+
 /* Solver plug-in: solves LinearProblem(A.entries.cscmatrix, b.entries) as the
  * linsolver of solve! (src/solvers.jl:55, 89-101, 656-660).  colptr[n+1],
  * rowval[nnz] 1-based (Int64, as SparseMatrixCSC{Float64,Int64}); fixed[nfixed]

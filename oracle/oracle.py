@@ -26,6 +26,8 @@ the host-side bookkeeping the reference does in Julia:
   Newton system of NonlinearForm (src/assemblypatterns/nonlinearform.jl:44-246)
   and the fixpoint loop of solve_fixpoint_full! (src/solvers.jl:812-1017),
   and polynomial LinearForms (src/assemblypatterns/linearform.jl:181-220).
+* ``Anderson`` -- AndersonAccelerationManager / update_anderson!
+  (src/solvers.jl:688-808).
 
 What pins each function independently of the device code is in the tests
 that exercise it (tests/test_oracle_*.py): exact-rational fixtures, closed
@@ -798,13 +800,72 @@ def assemble_lf_poly(fe, coef, quadorder, coords, cells, celldofs, ndofs, factor
     return b
 
 
+class Anderson:
+    """AndersonAccelerationManager + update_anderson! (src/solvers.jl:688-808),
+    restated with the reference's array shifts: LastIteratesTilde and
+    LastIterates of depth + 1 vectors each (all the initial iterate at
+    construction), the (depth + 2) system Matrix (identity on the first
+    depth + 1 diagonal entries) and Rhs = e_{depth+2}, the depth counter
+    starting at -1.  B is the norm matrix (None: the identity of "l2"), mask
+    the accelerated entries.  update(x) changes x in place and returns
+    (alpha[:depth + 1], local residual)."""
+
+    def __init__(self, x0, depth, damping=1.0, B=None, mask=None):
+        self.m, self.damping, self.B = int(depth), float(damping), B
+        self.mask = np.ones(len(x0), dtype=bool) if mask is None else np.asarray(mask, dtype=bool)
+        self.last = [np.array(x0, dtype=np.float64) for _ in range(self.m + 1)]
+        self.tilde = [np.array(x0, dtype=np.float64) for _ in range(self.m + 1)]
+        K = self.m + 2
+        self.M = np.zeros((K, K))
+        self.rhs = np.zeros(K)
+        self.rhs[K - 1] = 1.0
+        for j in range(self.m + 1):
+            self.M[j, j] = 1.0
+        self.cdepth = -1
+
+    def _inner(self, a1, a2, b1, b2):
+        """ldrdmatmul (src/fematrix.jl:524-535): (a1 - a2)' B (b1 - b2)"""
+        da = np.where(self.mask, a1 - a2, 0.0)
+        db = np.where(self.mask, b1 - b2, 0.0)
+        return float(da @ (db if self.B is None else self.B @ db))
+
+    def update(self, x):
+        m = self.m
+        cd = min(m, self.cdepth)
+        for j in range(m):
+            self.tilde[j][:] = self.tilde[j + 1]
+        self.tilde[m][:] = x
+        alpha, res = np.zeros(m + 1), 0.0
+        if cd > 0:
+            K = m + 2
+            for j in range(cd + 1):
+                for k in range(j, cd + 1):
+                    g = self._inner(self.tilde[m - j], self.last[m - j], self.tilde[m - k], self.last[m - k])
+                    self.M[j, k] = self.M[k, j] = g
+                    self.M[j, K - 1] = self.M[K - 1, j] = 1.0
+            a = np.linalg.solve(self.M, self.rhs)
+            res = float(np.sqrt(np.sum((self.M @ a - self.rhs) ** 2)))
+            out = np.zeros(len(x))
+            for i in range(cd + 1):
+                out += self.damping * a[i] * self.tilde[m - i]
+                out += (1.0 - self.damping) * a[i] * self.last[m - i]
+            x[self.mask] = out[self.mask]
+            for j in range(m):
+                self.last[j][:] = self.last[j + 1]
+            alpha = a[:m + 1].copy()
+        self.last[m][:] = x
+        self.cdepth += 1
+        return alpha, res
+
+
 def newton_solve(fe, coords, cells, celldofs, ndofs, fixed, target, c, f_coef, quadorder, lf_quadorder,
-                 maxit=10, target_residual=1e-12, u0=None):
+                 maxit=10, target_residual=1e-12, u0=None, anderson=None):
     """solve_fixpoint_full! (src/solvers.jl:812-1017) for the nonlinear
     Poisson problem: assemble at the initial iterate, set the boundary
     values, then per iteration: penalized direct solve (fixed dofs
-    eliminated), reassemble, residual ||A x - b|| on the free rows.
-    Returns (x, residuals)."""
+    eliminated), Anderson update when anderson = dict(depth, damping, B)
+    (src/solvers.jl:849-859, :923-929), reassemble, residual ||A x - b|| on
+    the free rows.  Returns (x, residuals)."""
     import scipy.sparse.linalg as spla
     u = np.zeros(ndofs) if u0 is None else np.array(u0, dtype=np.float64)
     f = assemble_lf_poly(fe, f_coef, lf_quadorder, coords, cells, celldofs, ndofs)
@@ -812,10 +873,15 @@ def newton_solve(fe, coords, cells, celldofs, ndofs, fixed, target, c, f_coef, q
     b = f + bn
     u[fixed] = target[fixed]
     free = np.setdiff1d(np.arange(ndofs), fixed)
+    aam = None
+    if anderson is not None:
+        aam = Anderson(u, anderson["depth"], anderson.get("damping", 1.0), anderson.get("B"))
     res = []
     for _ in range(maxit):
         rhs = b[free] - A[free][:, fixed] @ u[fixed]
         u[free] = spla.spsolve(A[free][:, free].tocsc(), rhs)
+        if aam is not None:
+            aam.update(u)
         A, bn = newton_assemble(fe, coords, cells, celldofs, ndofs, u, c, quadorder)
         b = f + bn
         r = A @ u - b
