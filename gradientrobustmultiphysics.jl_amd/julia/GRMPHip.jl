@@ -370,6 +370,40 @@ function hip_amg_solve!(A::Ptr{Cvoid}, b::Ptr{Cvoid}, x::Ptr{Cvoid}; rtol = 1e-1
     return info
 end
 
-export GRMPKrylov, HIPOperator, hip_l2error, hip_assemble_batch!, hip_amg_solve!
+"""
+    hip_anderson(x, n, depth; damping = 1.0, metric = C_NULL, ranges = [(0, n)]) -> Ptr{Cvoid}
+    hip_update_anderson!(aa, x) -> Vector{Float64}
+
+The device counterpart of `AndersonAccelerationManager` / `update_anderson!`
+(src/solvers.jl:688-808) for a fixpoint loop whose iterate lives in the
+`grmp_vector` `x` of length `n`: create it after the boundary data are set
+and call `hip_update_anderson!` after each linear solve, before the damping,
+as solve_fixpoint_full! does (src/solvers.jl:923-929).  `metric` is the
+`grmp_matrix` of the "L2" / "H1" norm (C_NULL for "l2"); `ranges` are the
+0-based `[offset, offset + ndofs)` of the blocks in `anderson_unknowns`.
+Returns the coefficients (zeros for the first two calls).  Free the handle
+with `ccall((:grmp_anderson_destroy, libgrmp), Cint, (Ptr{Cvoid},), aa)`.
+"""
+function hip_anderson(x::Ptr{Cvoid}, n::Integer, depth::Integer; damping = 1.0, metric::Ptr{Cvoid} = C_NULL,
+                      ranges = [(0, n)])
+    r = Int64[v for rg in ranges for v in rg]
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(context(), ccall((:grmp_anderson_create, libgrmp), Cint,
+                           (Ptr{Cvoid}, Int64, Cint, Cdouble, Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Cvoid},
+                            Ref{Ptr{Cvoid}}),
+                           context(), n, depth, damping, metric, length(ranges), r, x, h))
+    return h[]
+end
+
+function hip_update_anderson!(aa::Ptr{Cvoid}, x::Ptr{Cvoid}; depth::Integer)
+    alpha = zeros(Float64, depth + 1)
+    res = Ref{Cdouble}(0.0)
+    check(context(), ccall((:grmp_anderson_update, libgrmp), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cdouble},
+                                                                      Ref{Cdouble}), aa, x, alpha, res))
+    res[] > 1e-4 && @warn "Anderson acceleration local system residual unsatisfactory ($(res[]) > 1e-4)"
+    return alpha
+end
+
+export GRMPKrylov, HIPOperator, hip_l2error, hip_assemble_batch!, hip_amg_solve!, hip_anderson, hip_update_anderson!
 
 end # module
