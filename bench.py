@@ -584,8 +584,12 @@ def run_workload(grmp, which, args, log):
         F = [0.0] * 15
         F[11], F[13] = -2.0, -2.0
         pde.add_rhsdata(0, grmp.LinearForm(grmp.Identity, grmp.PolynomialData(F)))
-        sol = grmp.FEVector([fes])
         log(f"[bench] nonlinear: P2 {N}x{N}, {fes.ndofs} dofs")
+        # one untimed loop first: the first launches of each kernel and the first allocations of the
+        # stream-ordered pool are one-time costs of the process (0.104 s cold against 0.077 s warm)
+        grmp.solve(grmp.FEVector([fes]), pde, target_residual=1e-10, nonlinear_maxiterations=12, rtol=1e-12)
+        ctx.sync()
+        sol = grmp.FEVector([fes])
         t0 = time.perf_counter()
         info, A, b = grmp.solve(sol, pde, target_residual=1e-10, nonlinear_maxiterations=12, rtol=1e-12)
         ctx.sync()
