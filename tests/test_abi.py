@@ -48,3 +48,20 @@ def test_quadrature_orders(grmp):
     assert grmp.HookStiffnessOperator2D(1.0, 1.0).quadorder(p1v) == 0
     lf = grmp.LinearForm(grmp.Identity, grmp.DataFunction([1.0]))
     assert lf.lform(p1).quadorder == 1 and lf.lform(p2).quadorder == 2
+
+
+def test_solver_config_validation(grmp):
+    """The fixpoint keywords are checked on the host before any device work:
+    the Newton preconditioner and the Anderson keywords of
+    default_solver_kwargs (src/solvers.jl:37-59)."""
+    import pytest
+    cfg = grmp.SolverConfig(anderson_iterations=3, anderson_metric="H1", anderson_damping=0.5,
+                            anderson_unknowns=[0, 1], newton_preconditioner="jacobi")
+    assert (cfg.anderson_iterations, cfg.anderson_metric, cfg.anderson_damping, cfg.anderson_unknowns) == \
+        (3, "H1", 0.5, (0, 1))
+    assert grmp.SolverConfig().anderson_iterations == 0 and grmp.SolverConfig().anderson_unknowns == (0,)
+    for kw in (dict(anderson_metric="H2"), dict(anderson_iterations=11), dict(anderson_iterations=-1),
+               dict(anderson_damping=0.0), dict(anderson_damping=1.5), dict(newton_preconditioner="ilu")):
+        with pytest.raises(grmp.GrmpError) as ei:
+            grmp.SolverConfig(**kw)
+        assert ei.value.code == grmp._abi.ERR_ARG, kw
